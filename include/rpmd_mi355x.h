@@ -1,0 +1,150 @@
+/*
+ * rpmd_mi355x.h -- C-ABI of the MI355X (gfx950) ring-polymer force-and-propagate library.
+ *
+ * Every entry point takes plain pointers and sizes.  Arrays follow the reference's Fortran
+ * layout: r(3,na,nb), dvdr(3,na,nb), p(3,na,nb) are column-major FP64 (xyz fastest, then
+ * atom, then bead); virial(3,3) is column-major FP64.  Molecules are (O,H,H) triplets.
+ * Atomic units (bohr, Hartree, electron masses, a.u. time) throughout.
+ *
+ * The caller owns every host array.  Inputs are never written (the reference overwrites the
+ * oxygen with the M-site in place and restores it, src/md_forces.f90:181-188,213; this
+ * library never writes r).  Device copies live in the context.  One context per host
+ * thread; calls on a context are serialised and synchronous on return.
+ *
+ * Return value: RPMD_OK (0) or one of the RPMD_ERR_* codes.  rpmd_strerror() gives the
+ * reference's own STOP message for the code.
+ */
+#ifndef RPMD_MI355X_H
+#define RPMD_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: one per reference STOP condition on the path ---------------------- */
+#define RPMD_OK                    0
+#define RPMD_ERR_NA_NOT_MULT3      1  /* 'potenl 1'                 src/md_forces.f90:157-158 */
+#define RPMD_ERR_NEIGH_OVERFLOW    2  /* 'neighbour list too small' src/neigh_list.f90:63    */
+#define RPMD_ERR_NBDF_EVEN         3  /* 'Beadiabatic factor must be ODD' src/evolve_pi.f90:344-347 */
+#define RPMD_ERR_NBDF_TOO_LARGE    4  /* 'must not exceed nb'       src/setup_options.f90:55-58 */
+#define RPMD_ERR_RATTLE            5  /* 'max iterations exceeded'  src/md_rattle.f90:149-151   */
+#define RPMD_ERR_REALFT_N          6  /* 'realft 1' (n > 256)       src/fourier.f90:40          */
+#define RPMD_ERR_REALFT_MODE       7  /* 'realft 3'                 src/fourier.f90:56          */
+#define RPMD_ERR_EWC_CUTOFF        8  /* 'ewc_basic : rcut too large' src/correct_tem.f90:104-109 */
+#define RPMD_ERR_SMEARED           9  /* smeared charges (wm,wh>0) are not on this path       */
+#define RPMD_ERR_KMAX             10  /* Ewald kmax beyond the device k-tile (kmax > 15)      */
+#define RPMD_ERR_ARG              11  /* bad argument (null pointer, nb/na beyond ctx sizes)  */
+#define RPMD_ERR_HIP              12  /* HIP runtime error                                     */
+#define RPMD_ERR_IOPT             13  /* iopt not in {0,1,2,3,4,5}                             */
+#define RPMD_ERR_NOMEM            14  /* device allocation failed                              */
+#define RPMD_ERR_THERM            15  /* thermostat not supported by this entry point          */
+#define RPMD_ERR_NOISE            16  /* host noise buffer too short for the draws needed      */
+
+/* ---- precision modes ------------------------------------------------------------------ */
+#define RPMD_PREC_FP64   0  /* everything FP64 (the "plain FP64 GPU form")                   */
+#define RPMD_PREC_MIXED  1  /* FP32 pair/erfc/eikr arithmetic, FP64 accumulators, FP64 state */
+
+/* ---- intramolecular model switch ------------------------------------------------------ */
+#define RPMD_INTRA_AUTO        0  /* reference rule src/md_forces.f90:254-263               */
+#define RPMD_INTRA_HARMONIC    1  /* intra_harmonic (Toukan-Rahman SPC/F constants)         */
+#define RPMD_INTRA_SPCFW       2  /* intra_spcfw (kb, ktheta harmonic bend)                 */
+#define RPMD_INTRA_MORSE_HARM  3  /* intra_morse_harm (q-TIP4P/F)                           */
+#define RPMD_INTRA_MORSE       4  /* intra_morse                                            */
+
+/* ---- thermostats (3-letter codes of src/main.f90:218-252) ----------------------------- */
+#define RPMD_THERM_NONE  0  /* NVE ('AND' and unknown codes do nothing inside evolve_pi)     */
+#define RPMD_THERM_PRO   1  /* PILE-L   src/module_gle.f90:8                                 */
+#define RPMD_THERM_PRM   2  /* PILE-G   src/module_gle.f90:75                                */
+#define RPMD_THERM_PRG   3  /* Bussi global src/module_thermostat.f90:6                      */
+#define RPMD_THERM_PRL   4  /* local Langevin src/module_thermostat.f90:56                   */
+#define RPMD_THERM_GLE   5  /* GLE src/module_gle.f90:229                                    */
+
+/*
+ * Model parameters: the reference's COMMON blocks and thermostat module state, made explicit
+ * (src/main.f90:53-72; src/module_thermostat.f90:1-4).
+ */
+typedef struct rpmd_params {
+    /* /oo_param/ : LJ or Buckingham O-O term.  rcut already min(rcut, Lmin/2), bohr. */
+    double oo_eps, oo_sig, oo_gam, rcut;
+    /* /ew_param/ : alpha is the M-site fraction (1 = no M-site); wm, wh smeared widths. */
+    double alpha, ecut, wm, wh;
+    /* /ew_setup/ : filled by rpmd_setup_ewald (unit-box units when iamcub). */
+    double wrcut, walpha, rkmax;
+    int kmax;
+    /* /symmetry/ */
+    int iamcub;
+    /* /intra_param/ and /geometry/ (theta in radians, reoh in bohr) */
+    double apot, bpot, alp, alpb;
+    double theta, reoh;
+    /* /correct/ : TEM switch width, bohr (0 = no electrostatic contraction correction) */
+    double sig;
+    /* /beaddiabatic/ and /multiple_ts/ */
+    int nbdf1, nbdf2;
+    int mts;
+    /* intramolecular model switch (RPMD_INTRA_*) */
+    int intra_model;
+    /* thermostat: RPMD_THERM_* and ttau in a.u. time */
+    int therm;
+    double ttau;
+} rpmd_params;
+
+typedef struct rpmd_ctx rpmd_ctx;
+
+/* Host-side Ewald set-up, src/setup_options.f90:104-189: fills wrcut, walpha, rkmax, kmax
+ * and iamcub (iamcub on entry requests the cubic code, as the namelist flag does). */
+int rpmd_setup_ewald(int na, const double boxlxyz[3], rpmd_params* prm);
+
+/* Context: device buffers for up to na atoms x nb_max beads, a stream, cached tables.
+ * precision is RPMD_PREC_FP64 or RPMD_PREC_MIXED.  No reference counterpart: the reference
+ * keeps this state in COMMON blocks and reallocates scratch on every forces call
+ * (src/md_forces.f90:19). */
+int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, rpmd_ctx** out);
+int rpmd_ctx_set_params(rpmd_ctx* ctx, const rpmd_params* prm);
+int rpmd_ctx_set_precision(rpmd_ctx* ctx, int precision);
+int rpmd_ctx_destroy(rpmd_ctx* ctx);
+const char* rpmd_strerror(int code);
+
+/* subroutine forces(r,v,dvdr,nb,na,boxlxyz,z,virial,iopt)      src/md_forces.f90:1-110
+ * iopt 0 all terms, 1 Ewald+LJ, 2 Ewald, 3 LJ, 4 intramolecular, 5 TEM Ewald correction.
+ * v and virial are bead averages; dvdr is per bead (dV_k/dr_k). */
+int rpmd_forces(rpmd_ctx* ctx, const double* r, int nb, int na, const double boxlxyz[3],
+                const double* z, int iopt, double* v, double* dvdr, double* virial);
+
+/* subroutine full_forces(r,na,nb,v,vew,voo,vint,vir,z,boxlxyz,dvdr,dvdr2)
+ *                                                              src/md_forces.f90:305-472
+ * dvdr = low-frequency (Ewald + O-O, RP-contracted when nbdf1/nbdf2 > 0), dvdr2 = intra. */
+int rpmd_full_forces(rpmd_ctx* ctx, const double* r, int na, int nb, double* v, double* vew,
+                     double* voo, double* vint, double* vir, const double* z,
+                     const double boxlxyz[3], double* dvdr, double* dvdr2);
+
+/* subroutine realft(data,m,n,mode)                              src/fourier.f90:16-63
+ * In place: each of the m rows data(k,1:n) is R2HC (mode +1) or HC2R (mode -1)
+ * transformed and scaled by 1/sqrt(n) (FFTW half-complex order). */
+int rpmd_realft(rpmd_ctx* ctx, double* data, int m, int n, int mode);
+
+/* subroutine evolve_pi(p,r,v,vew,vlj,vint,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,
+ *                      irun,nbaro)                               src/evolve_pi.f90:1-322
+ * One RPMD step (type 'RPMD', nbaro = 0).  p, r, dvdr, dvdr2 are updated in place.
+ * noise: Gaussian deviates consumed in the reference's draw order (host RNG replay), or
+ * NULL to draw them on the device from the context's counter-based stream. */
+int rpmd_evolve_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* vew, double* vlj,
+                   double* vint, double* dvdr, double* dvdr2, double dt, const double* mass,
+                   int na, int nb, const double boxlxyz[3], const double* z, double beta,
+                   double* vir, double* vir_lf, const double* noise, long long n_noise);
+
+/* Device-resident stepping (no counterpart in the reference, whose trajectory loop calls
+ * evolve once per step, src/md_dynamics.f90:499-517): upload a state once, run nsteps
+ * evolve_pi steps with all data in HBM, read energies/state back when needed. */
+int rpmd_state_upload(rpmd_ctx* ctx, const double* p, const double* r, const double* mass,
+                      const double* z, int na, int nb, const double boxlxyz[3], double beta,
+                      double dt, unsigned long long seed);
+int rpmd_run_steps(rpmd_ctx* ctx, int nsteps);
+int rpmd_state_download(rpmd_ctx* ctx, double* p, double* r, double* dvdr, double* dvdr2,
+                        double energies[4], double vir[9]);
+/* Capture rpmd_run_steps(ctx, 1) into a HIP graph and replay it for later calls. */
+int rpmd_use_graph(rpmd_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPMD_MI355X_H */

@@ -1,0 +1,296 @@
+"""ctypes front-end to the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module,
+and only as the checker or the timed CPU baseline -- never as part of the product path.
+
+* ``Oracle``  -- this repository's C restatement of the reference path (liboracle.so).
+* ``RefPath`` -- the reference's own Fortran force path, RNG and thermostat helpers compiled in
+  place from the reference sources by ``oracle/Makefile`` (``_ref/libref_path.so``).  The
+  reference COMMON blocks are set through their exported symbols and the Fortran routines are
+  called with by-reference arguments, exactly as the Fortran callers would.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_ORACLE = os.path.join(HERE, "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libref_path.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class RpmdParams(C.Structure):
+    """Mirror of ``rpmd_params`` in include/rpmd_mi355x.h."""
+
+    _fields_ = [
+        ("oo_eps", C.c_double), ("oo_sig", C.c_double), ("oo_gam", C.c_double), ("rcut", C.c_double),
+        ("alpha", C.c_double), ("ecut", C.c_double), ("wm", C.c_double), ("wh", C.c_double),
+        ("wrcut", C.c_double), ("walpha", C.c_double), ("rkmax", C.c_double),
+        ("kmax", C.c_int), ("iamcub", C.c_int),
+        ("apot", C.c_double), ("bpot", C.c_double), ("alp", C.c_double), ("alpb", C.c_double),
+        ("theta", C.c_double), ("reoh", C.c_double),
+        ("sig", C.c_double),
+        ("nbdf1", C.c_int), ("nbdf2", C.c_int), ("mts", C.c_int),
+        ("intra_model", C.c_int), ("therm", C.c_int), ("ttau", C.c_double),
+    ]
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "RpmdParams":
+        p = cls()
+        for k, v in d.items():
+            setattr(p, k, v)
+        return p
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RngState(C.Structure):
+    _fields_ = [("idum", C.c_int), ("idum2", C.c_int), ("iy", C.c_int), ("iv", C.c_int * 32),
+                ("iset", C.c_int), ("gset", C.c_double)]
+
+
+class GleState(C.Structure):
+    _fields_ = [("ns", C.c_int), ("na", C.c_int), ("nb", C.c_int),
+                ("gT", C.c_double * 64), ("gS", C.c_double * 64), ("gp", _dp)]
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """The C restatement (oracle/rpmd_oracle.c)."""
+
+    def __init__(self, path: str = LIB_ORACLE):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.orc_forces.restype = C.c_int
+        L.orc_full_forces.restype = C.c_int
+        L.orc_realft.restype = C.c_int
+        L.orc_evolve_pi.restype = C.c_int
+        L.orc_gaussian.restype = C.c_double
+        L.orc_gaussian.argtypes = [C.c_void_p, C.c_double]
+        L.orc_ran2.restype = C.c_double
+        L.orc_ran2.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_gle_init.restype = C.c_int
+
+    # -- set-up --------------------------------------------------------------------------
+    def setup_ewald(self, na: int, box, prm: RpmdParams) -> RpmdParams:
+        b = _f64(box)
+        self.lib.orc_setup_ewald(C.c_int(na), _d(b), C.byref(prm))
+        return prm
+
+    def setup_box_size(self, nm, rho, wmass, ncell):
+        box = np.zeros(3)
+        nc = (C.c_int * 3)(*ncell)
+        self.lib.orc_setup_box_size(C.c_int(nm), C.c_double(rho), C.c_double(wmass), nc, _d(box))
+        return box
+
+    # -- forces ----------------------------------------------------------------------------
+    def forces(self, prm: RpmdParams, r: np.ndarray, box, z, iopt: int):
+        """forces(r,v,dvdr,nb,na,boxlxyz,z,virial,iopt); r has shape (nb, na, 3)."""
+        r = _f64(r)
+        nb, na = r.shape[0], r.shape[1]
+        dvdr = np.zeros_like(r)
+        vir = np.zeros(9)
+        v = C.c_double()
+        err = self.lib.orc_forces(C.byref(prm), _d(r), C.c_int(nb), C.c_int(na), _d(_f64(box)),
+                                  _d(_f64(z)), C.c_int(iopt), C.byref(v), _d(dvdr), _d(vir))
+        if err:
+            raise RuntimeError(f"orc_forces error {err}")
+        return v.value, dvdr, vir.reshape(3, 3).T.copy()
+
+    def full_forces(self, prm: RpmdParams, r: np.ndarray, box, z):
+        r = _f64(r)
+        nb, na = r.shape[0], r.shape[1]
+        dvdr = np.zeros_like(r)
+        dvdr2 = np.zeros_like(r)
+        vir = np.zeros(9)
+        v, vew, voo, vint = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        err = self.lib.orc_full_forces(C.byref(prm), _d(r), C.c_int(na), C.c_int(nb), C.byref(v),
+                                       C.byref(vew), C.byref(voo), C.byref(vint), _d(vir),
+                                       _d(_f64(z)), _d(_f64(box)), _d(dvdr), _d(dvdr2))
+        if err:
+            raise RuntimeError(f"orc_full_forces error {err}")
+        return dict(v=v.value, vew=vew.value, voo=voo.value, vint=vint.value,
+                    vir=vir.reshape(3, 3).T.copy(), dvdr=dvdr, dvdr2=dvdr2)
+
+    def realft(self, data: np.ndarray, mode: int) -> np.ndarray:
+        """data shape (n, m) = Fortran data(m, n); returns the transformed copy."""
+        d = _f64(data).copy()
+        n, m = d.shape
+        err = self.lib.orc_realft(_d(d), C.c_int(m), C.c_int(n), C.c_int(mode))
+        if err:
+            raise RuntimeError(f"orc_realft error {err}")
+        return d
+
+    def ring_rpmd(self, mass3, nb, dt, beta):
+        mon = np.zeros((nb, 4, 3))
+        m = _f64(mass3)
+        self.lib.orc_ring_rpmd(_d(m), _d(mon), C.c_int(3), C.c_int(nb), C.c_double(dt), C.c_double(beta))
+        return mon
+
+    # -- RNG / thermostats -------------------------------------------------------------------
+    def rng(self, irun: int) -> RngState:
+        s = RngState()
+        self.lib.orc_rng_seed(C.byref(s), C.c_int(irun))
+        return s
+
+    def gaussians(self, s: RngState, n: int) -> np.ndarray:
+        out = np.zeros(n)
+        self.lib.orc_gaussian_fill(C.byref(s), _d(out), C.c_longlong(n))
+        return out
+
+    def therm_pro(self, p, mass, beta, ctau, dt, s: RngState):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_therm_pro(_d(p), _d(_f64(mass)), C.c_double(beta), C.c_double(ctau), C.c_double(dt),
+                               C.c_int(na), C.c_int(nb), C.byref(s))
+        return p
+
+    def matrix_exp(self, M, j=15, k=15):
+        M = _f64(np.asarray(M).T)  # column-major
+        n = M.shape[0]
+        out = np.zeros(n * n)
+        self.lib.orc_matrix_exp(_d(M), C.c_int(n), C.c_int(j), C.c_int(k), _d(out))
+        return out.reshape(n, n).T.copy()
+
+    def cholesky(self, A):
+        A = _f64(np.asarray(A).T)
+        n = A.shape[0]
+        out = np.zeros(n * n)
+        self.lib.orc_cholesky(_d(A), _d(out), C.c_int(n))
+        return out.reshape(n, n).T.copy()
+
+    def gle_init(self, A, ttau, na, nb, dt, s: RngState, beta) -> GleState:
+        g = GleState()
+        A = _f64(np.asarray(A).T)
+        err = self.lib.orc_gle_init(C.byref(g), _d(A), C.c_int(A.shape[0] - 1), C.c_double(ttau),
+                                    C.c_int(na), C.c_int(nb), C.c_double(dt), C.byref(s), C.c_double(beta))
+        if err:
+            raise RuntimeError(f"orc_gle_init error {err}")
+        return g
+
+    def gle_free(self, g: GleState):
+        self.lib.orc_gle_free(C.byref(g))
+
+    # -- the step ------------------------------------------------------------------------------
+    def evolve_pi(self, prm: RpmdParams, p, r, dvdr, dvdr2, dt, mass, box, z, beta,
+                  s: RngState | None = None, g: GleState | None = None):
+        p, r = _f64(p).copy(), _f64(r).copy()
+        dvdr, dvdr2 = _f64(dvdr).copy(), _f64(dvdr2).copy()
+        nb, na = r.shape[0], r.shape[1]
+        vir, vir_lf = np.zeros(9), np.zeros(9)
+        v, vew, vlj, vint = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        if s is None:
+            s = RngState()
+            self.lib.orc_rng_init(C.byref(s))
+        err = self.lib.orc_evolve_pi(C.byref(prm), _d(p), _d(r), C.byref(v), C.byref(vew), C.byref(vlj),
+                                     C.byref(vint), _d(dvdr), _d(dvdr2), C.c_double(dt), _d(_f64(mass)),
+                                     C.c_int(na), C.c_int(nb), _d(_f64(box)), _d(_f64(z)),
+                                     C.c_double(beta), _d(vir), _d(vir_lf), C.byref(s),
+                                     C.byref(g) if g is not None else None)
+        if err:
+            raise RuntimeError(f"orc_evolve_pi error {err}")
+        return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, vew=vew.value, vlj=vlj.value,
+                    vint=vint.value, vir=vir.reshape(3, 3).T.copy(), vir_lf=vir_lf.reshape(3, 3).T.copy())
+
+
+class RefPath:
+    """The reference's own Fortran (force path, RNG, thermostat helpers) built from source."""
+
+    def __init__(self, path: str = LIB_REF):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref` where the reference is mounted")
+        # RTLD_LAZY: realft_/rp_contract_nm_ (FFTW-dependent) stay unresolved and are never called
+        self.lib = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        self.lib.gaussian_.restype = C.c_double
+        self.lib.ran2_.restype = C.c_double
+
+    def _common(self, name, ctype):
+        return ctype.in_dll(self.lib, name)
+
+    def set_params(self, prm: RpmdParams):
+        """Fill the COMMON blocks the force path reads (src/main.f90:53-72)."""
+        L = self.lib
+        oo = (C.c_double * 4).in_dll(L, "oo_param_")
+        oo[:] = [prm.oo_eps, prm.oo_sig, prm.oo_gam, prm.rcut]
+        ew = (C.c_double * 4).in_dll(L, "ew_param_")
+        ew[:] = [prm.alpha, prm.ecut, prm.wm, prm.wh]
+
+        class EwSetup(C.Structure):
+            _fields_ = [("wrcut", C.c_double), ("walpha", C.c_double), ("rkmax", C.c_double), ("kmax", C.c_int)]
+
+        es = EwSetup.in_dll(L, "ew_setup_")
+        es.wrcut, es.walpha, es.rkmax, es.kmax = prm.wrcut, prm.walpha, prm.rkmax, prm.kmax
+        ip = (C.c_double * 4).in_dll(L, "intra_param_")
+        ip[:] = [prm.apot, prm.bpot, prm.alp, prm.alpb]
+        ge = (C.c_double * 2).in_dll(L, "geometry_")
+        ge[:] = [prm.theta, prm.reoh]
+        C.c_int.in_dll(L, "symmetry_").value = 1 if prm.iamcub else 0
+        C.c_double.in_dll(L, "correct_").value = prm.sig
+        (C.c_int * 3).in_dll(L, "rpmddft_")[:] = [0, 0, 0]
+        C.c_int.in_dll(L, "epsr_").value = 0
+        C.c_int.in_dll(L, "e3b_").value = 0
+
+    def forces(self, prm: RpmdParams, r: np.ndarray, box, z, iopt: int):
+        self.set_params(prm)
+        r = _f64(r).copy()
+        nb, na = r.shape[0], r.shape[1]
+        dvdr = np.zeros_like(r)
+        vir = np.zeros(9)
+        v = C.c_double()
+        b = _f64(box).copy()
+        zz = _f64(z).copy()
+        self.lib.forces_(_d(r), C.byref(v), _d(dvdr), C.byref(C.c_int(nb)), C.byref(C.c_int(na)), _d(b),
+                         _d(zz), _d(vir), C.byref(C.c_int(iopt)))
+        return v.value, dvdr, vir.reshape(3, 3).T.copy()
+
+    def seed(self, irun: int):
+        """src/main.f90:298-299: irun = -irun; test = gaussian(irun,1.d0)."""
+        self._idum = C.c_int(-irun)
+        self.gaussian()
+
+    def gaussian(self, sigma: float = 1.0) -> float:
+        return self.lib.gaussian_(C.byref(self._idum), C.byref(C.c_double(sigma)))
+
+    def therm_pro(self, p, mass, beta, ctau, dt):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib._QMglePtherm_pro(_d(p), _d(_f64(mass)), C.byref(C.c_double(beta)), C.byref(C.c_double(ctau)),
+                                  C.byref(C.c_double(dt)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                  C.byref(self._idum))
+        return p
+
+    def matrix_exp(self, M, j=15, k=15):
+        M = np.asfortranarray(M, dtype=np.float64)
+        n = M.shape[0]
+        out = np.zeros((n, n), order="F")
+        self.lib._QMglePmatrix_exp(M.ctypes.data_as(_dp), C.byref(C.c_int(n)), C.byref(C.c_int(j)),
+                                   C.byref(C.c_int(k)), out.ctypes.data_as(_dp))
+        return np.array(out)
+
+    def cholesky(self, A):
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        out = np.zeros((n, n), order="F")
+        self.lib._QMglePcholesky(A.ctypes.data_as(_dp), out.ctypes.data_as(_dp), C.byref(C.c_int(n)))
+        return np.array(out)
+
+    def ring_rpmd(self, mass3, nb, dt, beta):
+        mon = np.zeros((nb, 4, 3))
+        self.lib.ring_rpmd_(_d(_f64(mass3)), _d(mon), C.byref(C.c_int(3)), C.byref(C.c_int(nb)),
+                            C.byref(C.c_double(dt)), C.byref(C.c_double(beta)))
+        return mon
