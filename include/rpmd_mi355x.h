@@ -144,6 +144,13 @@ int rpmd_state_download(rpmd_ctx* ctx, double* p, double* r, double* dvdr, doubl
 /* Capture rpmd_run_steps(ctx, 1) into a HIP graph and replay it for later calls. */
 int rpmd_use_graph(rpmd_ctx* ctx, int enable);
 
+/* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
+ * the context stream around the Ewald kernels of eager launches between begin and end.
+ * ms/counts index: 0 real-space pair kernel, 1 structure-factor GEMM, 2 k-space force kernel,
+ * 3 structure-factor weights/energy kernel. */
+int rpmd_prof_begin(rpmd_ctx* ctx);
+int rpmd_prof_end(rpmd_ctx* ctx, double ms[4], int counts[4]);
+
 #ifdef __cplusplus
 }
 #endif

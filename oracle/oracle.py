@@ -153,6 +153,11 @@ class Oracle:
         self.lib.orc_gaussian_fill(C.byref(s), _d(out), C.c_longlong(n))
         return out
 
+    def sample_rpmd(self, nb, na, mass, beta, s: RngState) -> np.ndarray:
+        p = np.zeros((nb, na, 3))
+        self.lib.orc_sample_rpmd(_d(p), C.c_int(na), C.c_int(nb), _d(_f64(mass)), C.c_double(beta), C.byref(s))
+        return p
+
     def therm_pro(self, p, mass, beta, ctau, dt, s: RngState):
         p = _f64(p).copy()
         nb, na = p.shape[0], p.shape[1]

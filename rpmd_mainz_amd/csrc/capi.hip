@@ -1,0 +1,1152 @@
+// capi.hip -- the C-ABI of include/rpmd_mi355x.h: context, host-side set-up, and the device
+// pipelines of forces / full_forces / realft / evolve_pi (reference file:line cited per routine).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace rpmd;
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return RPMD_ERR_HIP; \
+    } while (0)
+#define RETCHK(x)            \
+    do {                     \
+        int r_ = (x);        \
+        if (r_) return r_;   \
+    } while (0)
+
+namespace {
+
+// bead transform matrices for one n: F = R2HC/sqrt(n), B = HC2R/sqrt(n) (src/fourier.f90:42-59,
+// FFTW's documented half-complex definitions), row-major [k][j]; built in long double
+void realft_matrices(int n, std::vector<double>& F, std::vector<double>& B) {
+    F.assign((size_t)n * n, 0.0);
+    B.assign((size_t)n * n, 0.0);
+    const long double pi = acosl(-1.0L);
+    const long double sc = sqrtl(1.0L / n);
+    for (int j = 0; j < n; j++) {
+        for (int k = 0; k <= n / 2; k++) {
+            long double a = 2.0L * pi * (long double)(((long long)j * k) % n) / n;
+            F[(size_t)k * n + j] = (double)(sc * cosl(a));
+        }
+        for (int k = 1; k <= (n - 1) / 2; k++) {
+            long double a = 2.0L * pi * (long double)(((long long)j * k) % n) / n;
+            F[(size_t)(n - k) * n + j] = (double)(-sc * sinl(a));
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        B[(size_t)j * n + 0] = (double)sc;
+        for (int k = 1; k <= (n - 1) / 2; k++) {
+            long double a = 2.0L * pi * (long double)(((long long)j * k) % n) / n;
+            B[(size_t)j * n + k] = (double)(2.0L * sc * cosl(a));
+            B[(size_t)j * n + (n - k)] = (double)(-2.0L * sc * sinl(a));
+        }
+        if (n % 2 == 0) B[(size_t)j * n + n / 2] = (double)((j % 2 == 0 ? 1.0L : -1.0L) * sc);
+    }
+}
+
+struct NMMat {
+    double *F = nullptr, *B = nullptr;    // plain
+    double *Fs = nullptr, *Bs = nullptr;  // contraction-scaled: B*sqrt(n/nb), F*sqrt(nb/n)
+    int scaled_nb = -1;
+};
+
+}  // namespace
+
+struct rpmd_ctx {
+    rpmd_params prm;
+    int na = 0, nb_max = 0, prec = 0;
+    hipStream_t st = nullptr;
+    // force pipeline
+    double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
+    double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
+    int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
+        *ovf = nullptr;
+    unsigned char* kmask = nullptr;
+    size_t off_real = 0, off_k = 0, off_oo = 0, off_tem = 0, off_intra = 0, nparts = 0;
+    int ncell_cap = 0;
+    // step state
+    double *p = nullptr, *r = nullptr, *dvdr = nullptr, *dvdr2 = nullptr, *dvdrl = nullptr, *dvdre = nullptr,
+           *rst = nullptr, *pnm = nullptr, *rnm = nullptr, *tmp = nullptr, *rb = nullptr, *dvdrb = nullptr,
+           *tmp2 = nullptr;
+    double *monod = nullptr, *pile_c1 = nullptr, *pile_c3 = nullptr, *com_parts = nullptr, *lang_c3m = nullptr;
+    int* maps = nullptr;  // contraction mode maps
+    double* noise = nullptr;
+    size_t noise_cap = 0;
+    uint64_t* step_ctr = nullptr;
+    uint64_t seed = 0;
+    std::map<int, NMMat> nm;
+    // host staging (pinned)
+    double* h_stage = nullptr;
+    size_t h_cap = 0;
+    // cached k set
+    KSet ks{};
+    double ks_key[8] = {0};
+    bool ks_valid = false;
+    double sumq2 = 0.0, mtot = 0.0;
+    std::vector<double> hq;
+    // device-resident state
+    int st_nb = 0;
+    Box st_box{};
+    double st_beta = 0, st_dt = 0;
+    bool st_valid = false;
+    std::vector<double> hmass;
+    // energies of the last step: slots [HF, EW, LJ, TEM, EWC] x 5 terms x 10
+    double* ev_step = nullptr;
+    double vir_hf[9] = {0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int use_graph = 0;
+    // kernel timing with HIP events on the context stream (eager launches only)
+    int prof = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, size_t>> ev_marks;  // (kernel id, index of the start event)
+};
+
+namespace {
+
+// timed launch bracket: kernel ids 0 ewald_real, 1 k_sk, 2 k_kforce, 3 k_sk_final
+struct ProfScope {
+    rpmd_ctx* c;
+    int id;
+    size_t i0;
+    ProfScope(rpmd_ctx* c_, int id_) : c(c_), id(id_), i0(0) {
+        if (!c->prof) return;
+        i0 = c->ev_marks.size() * 2;
+        while (c->ev_pool.size() < i0 + 2) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            c->ev_pool.push_back(e);
+        }
+        hipEventRecord(c->ev_pool[i0], c->st);
+    }
+    ~ProfScope() {
+        if (!c->prof) return;
+        hipEventRecord(c->ev_pool[i0 + 1], c->st);
+        c->ev_marks.push_back({id, i0});
+    }
+};
+
+Box make_box(const double b[3]) {
+    Box x;
+    for (int d = 0; d < 3; d++) {
+        x.L[d] = b[d];
+        x.inv[d] = 1.0 / b[d];
+    }
+    return x;
+}
+Box unit_box() {
+    Box x;
+    for (int d = 0; d < 3; d++) x.L[d] = x.inv[d] = 1.0;
+    return x;
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T)) == hipSuccess ? 0 : RPMD_ERR_NOMEM;
+}
+
+int intra_model(const rpmd_params& P) {
+    if (P.intra_model != RPMD_INTRA_AUTO) return P.intra_model;
+    if (P.alp == 0.0) return RPMD_INTRA_HARMONIC;  // src/md_forces.f90:254-256
+    return P.alpb != 0.0 ? RPMD_INTRA_MORSE : RPMD_INTRA_MORSE_HARM;
+}
+
+// the reference's k set as a mask [kx][ky+15][kz+15] (src/ewald_cubic.f90:292-520 loop exits;
+// src/ewald_noncubic.f90:385-397 strict comparisons)
+int build_kset(rpmd_ctx* c, const double box[3]) {
+    const rpmd_params& P = c->prm;
+    double key[8] = {box[0], box[1], box[2], P.walpha, P.rkmax, (double)P.kmax, (double)P.iamcub, 0};
+    if (c->ks_valid && !memcmp(key, c->ks_key, sizeof key)) return 0;
+    if (P.kmax > RPMD_KMAX_DEV || P.kmax < 0) return RPMD_ERR_KMAX;
+    const int K = P.kmax, T = RPMD_KTILE, H = RPMD_KMAX_DEV;
+    std::vector<unsigned char> m((size_t)(H + 1) * T * T, 0);
+    auto M = [&](int kx, int ky, int kz) -> unsigned char& { return m[((size_t)kx * T + ky + H) * T + kz + H]; };
+    const double pi = acos(-1.0);
+    KSet ks{};
+    ks.kmax = K;
+    const double rkmax2 = P.rkmax * P.rkmax;
+    if (P.iamcub) {
+        const double rlat = 2.0 * pi;
+        for (int kx = 0; kx <= K; kx++) {
+            double rkx = rlat * kx, rky, rkz, rk2 = rkx * rkx;
+            if (rk2 > 0.0) M(kx, 0, 0) = 1;
+            for (int kz = 1; kz <= K; kz++) {
+                rkz = rlat * kz;
+                rk2 = rkx * rkx + rkz * rkz;
+                if (rk2 > rkmax2) break;
+                M(kx, 0, kz) = M(kx, 0, -kz) = 1;
+            }
+            for (int ky = 1; ky <= K; ky++) {
+                rky = rlat * ky;
+                rk2 = rkx * rkx + rky * rky;
+                if (rk2 > rkmax2) break;
+                M(kx, ky, 0) = M(kx, -ky, 0) = 1;
+                for (int kz = 1; kz <= K; kz++) {
+                    rkz = rlat * kz;
+                    rk2 = rkx * rkx + rky * rky + rkz * rkz;
+                    if (rk2 > rkmax2) break;
+                    M(kx, ky, kz) = M(kx, ky, -kz) = M(kx, -ky, kz) = M(kx, -ky, -kz) = 1;
+                }
+            }
+        }
+        ks.lat[0] = ks.lat[1] = ks.lat[2] = rlat;
+        ks.f0 = rlat;
+    } else {
+        const double twopi = 2.0 * pi;
+        const double xl = twopi / box[0], yl = twopi / box[1], zl = twopi / box[2];
+        for (int kx = 0; kx <= K; kx++)
+            for (int ky = -K; ky <= K; ky++)
+                for (int kz = -K; kz <= K; kz++) {
+                    double rkx = xl * kx, rky = yl * ky, rkz = zl * kz;
+                    double rk2 = rkx * rkx + rky * rky + rkz * rkz;
+                    if (rk2 < rkmax2 && rk2 != 0.0) M(kx, ky, kz) = 1;
+                }
+        ks.lat[0] = xl;
+        ks.lat[1] = yl;
+        ks.lat[2] = zl;
+        ks.f0 = twopi / (box[0] * box[1] * box[2]);
+    }
+    ks.b = 0.25 / (P.walpha * P.walpha);
+    ks.cubic = P.iamcub ? 1 : 0;
+    for (int kx = 0; kx <= H; kx++) {
+        int kyb = -1, kzb = 0;
+        for (int ky = -H; ky <= H; ky++)
+            for (int kz = -H; kz <= H; kz++)
+                if (kx <= K && M(kx, ky, kz)) {
+                    kyb = std::max(kyb, std::abs(ky));
+                    kzb = std::max(kzb, std::abs(kz));
+                }
+        ks.ky_max[kx] = kyb;
+        ks.kz_bound[kx] = kzb;
+    }
+    HIPCHK(hipMemcpyAsync(c->kmask, m.data(), m.size(), hipMemcpyHostToDevice, c->st));
+    c->ks = ks;
+    memcpy(c->ks_key, key, sizeof key);
+    c->ks_valid = true;
+    return 0;
+}
+
+CellGrid make_grid(const Box& sb, double rc) {
+    CellGrid g;
+    for (int d = 0; d < 3; d++) {
+        int n = (int)floor(3.0 * sb.L[d] / rc);
+        n = std::max(1, std::min(n, 64));
+        g.nc[d] = n;
+        g.reach[d] = 3;
+        if (n * rc / 3.0 > sb.L[d]) g.reach[d] = n;  // never
+    }
+    g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+    return g;
+}
+
+NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
+    auto it = c->nm.find(n);
+    if (it == c->nm.end()) {
+        std::vector<double> F, B;
+        realft_matrices(n, F, B);
+        NMMat m;
+        if (dalloc(&m.F, F.size()) || dalloc(&m.B, B.size()) || dalloc(&m.Fs, F.size()) || dalloc(&m.Bs, B.size()))
+            return nullptr;
+        hipMemcpy(m.F, F.data(), F.size() * 8, hipMemcpyHostToDevice);
+        hipMemcpy(m.B, B.data(), B.size() * 8, hipMemcpyHostToDevice);
+        it = c->nm.emplace(n, m).first;
+    }
+    NMMat& m = it->second;
+    if (nb_full > 0 && m.scaled_nb != nb_full) {
+        std::vector<double> F, B;
+        realft_matrices(n, F, B);
+        const double sB = sqrt((double)n / (double)nb_full), sF = sqrt((double)nb_full / (double)n);
+        for (auto& x : B) x *= sB;
+        for (auto& x : F) x *= sF;
+        hipMemcpy(m.Fs, F.data(), F.size() * 8, hipMemcpyHostToDevice);
+        hipMemcpy(m.Bs, B.data(), B.size() * 8, hipMemcpyHostToDevice);
+        m.scaled_nb = nb_full;
+    }
+    return &m;
+}
+
+// ---- forces on device positions (SoA, nb beads); grad overwritten; ev_slot[5][10] ---------------
+// ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
+// 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
+int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
+               bool check_lists) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na;
+    hipStream_t s = c->st;
+    if (iopt < 0 || iopt > 5) return RPMD_ERR_IOPT;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    const Box bx = make_box(box);
+    const double boxmin = std::min(box[0], std::min(box[1], box[2]));
+    const double boxmax = std::max(box[0], std::max(box[1], box[2]));
+    const size_t N = (size_t)3 * na * nb;
+    launch_zero(grad, N, s);
+    launch_zero(ev_slot, 50, s);
+    const bool mixed = c->prec == RPMD_PREC_MIXED;
+    const bool need_oo = (iopt == 3 || iopt <= 1);
+    int use_list = 0;
+    if (need_oo && nb > 3) {
+        // centroid O-O list (src/md_forces.f90:34-53)
+        launch_centroid(pos, c->cen, na, nb, s);
+        launch_zero_int(c->ovf, 2, s);
+        launch_nlist(c->cen, na, 3, bx, std::min(P.rcut, 0.5 * boxmin), c->nnb, c->nbl, c->ovf, s);
+        use_list = 1;
+        if (check_lists) {
+            int h[2];
+            HIPCHK(hipMemcpyAsync(h, c->ovf, sizeof h, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (h[0] > 0) return RPMD_ERR_NEIGH_OVERFLOW;
+            if (h[1] == 0) use_list = 0;  // list(1) = 0: lj_driver falls back (src/ljones.f90:14-26)
+        }
+    }
+    if (iopt <= 2) {
+        if (P.wm > 0.0 || P.wh > 0.0) return RPMD_ERR_SMEARED;
+        RETCHK(build_kset(c, box));
+        const int cubic = P.iamcub;
+        const Box sb = cubic ? unit_box() : bx;
+        launch_sites(pos, c->site, na, nb, P.alpha, cubic, bx, s);
+        CellGrid g = make_grid(sb, P.wrcut);
+        if (g.ncell > c->ncell_cap) return RPMD_ERR_ARG;
+        launch_cell_build(c->site, na, nb, g, sb, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
+        double* pr = c->parts + c->off_real;
+        {
+            ProfScope ps(c, 0);
+            if (mixed)
+                launch_ewald_real<float>(c->sorted, c->perm, c->start, c->q, na, nb, g, sb, P.wrcut, P.walpha,
+                                         c->gsite, pr, s);
+            else
+                launch_ewald_real<double>(c->sorted, c->perm, c->start, c->q, na, nb, g, sb, P.wrcut, P.walpha,
+                                          c->gsite, pr, s);
+        }
+        double* pk = c->parts + c->off_k;
+        {
+            ProfScope ps(c, 1);
+            launch_sk(c->site, c->q, na, nb, c->ks, c->skp, s);
+        }
+        {
+            ProfScope ps(c, 3);
+            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, pk, s);
+        }
+        {
+            ProfScope ps(c, 2);
+            launch_kforce(c->site, c->q, na, nb, c->ks, c->gk, c->gsite, s);
+        }
+        const double sc = cubic ? 1.0 / box[0] : 1.0;
+        launch_chain_rule(grad, c->gsite, na, nb, P.alpha, sc * sc, s);
+        // energies: real and k parts into the Ewald slot (two reductions summed on the host)
+        launch_reduce_ev(pr, nb * ewald_real_blocks(na), 10, ev_slot + 0, 10, sc / nb, s);
+        launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);
+    }
+    if (need_oo) {
+        double* po = c->parts + c->off_oo;
+        if (mixed)
+            launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
+        else
+            launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
+        launch_reduce_ev(po, nb * oo_blocks(na), 10, ev_slot + 20, 10, 1.0 / nb, s);
+    }
+    if (iopt == 0 || iopt == 4) {
+        double* pi = c->parts + c->off_intra;
+        launch_intra(pos, na, nb, intra_model(P), P, grad, pi, s);
+        launch_reduce_ev(pi, nb * intra_blocks(na), 10, ev_slot + 30, 10, 1.0 / nb, s);
+    }
+    if (iopt == 5) {
+        // TEM correction (src/md_forces.f90:65-100; src/correct_tem.f90:1-71)
+        if (P.wm > 0.0 || P.wh > 0.0) return RPMD_ERR_SMEARED;
+        const double aeff = fabs(P.alpha - 1.0) > 1e-4 ? P.alpha : 1.0;
+        int tl = 0;
+        if (nb > 1) {
+            double cut = P.sig + 5.0;
+            if (cut > 0.5 * boxmin) cut = 0.5 * boxmin;
+            launch_centroid(pos, c->cen, na, nb, s);
+            launch_zero_int(c->ovf, 2, s);
+            launch_nlist(c->cen, na, 1, bx, cut, c->nnb, c->nbl, c->ovf, s);
+            tl = 1;
+            if (check_lists) {
+                int h[2];
+                HIPCHK(hipMemcpyAsync(h, c->ovf, sizeof h, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if (h[0] > 0) return RPMD_ERR_NEIGH_OVERFLOW;
+                if (h[1] == 0) tl = 0;
+            }
+        }
+        if ((tl || 3.0 * P.sig > boxmax) && P.sig > 0.5 * boxmax) return RPMD_ERR_EWC_CUTOFF;
+        launch_sites(pos, c->site, na, nb, aeff, 0, bx, s);
+        double* pt = c->parts + c->off_tem;
+        launch_tem(c->site, c->q, na, nb, bx, c->nnb, c->nbl, tl, P.sig, c->gsite, pt, s);
+        launch_chain_rule(grad, c->gsite, na, nb, aeff, 1.0, s);
+        launch_reduce_ev(pt, nb * tem_blocks(na), 10, ev_slot + 40, 10, 1.0 / nb, s);
+    }
+    if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
+    return 0;
+}
+
+// combine an ev slot read back to the host: v and vir (column-major) of one forces call
+void combine_ev(const rpmd_ctx* c, const double* h, int iopt, const double box[3], double* v, double* vir) {
+    double vv = 0.0;
+    double w[9] = {0};
+    for (int t = 0; t < 5; t++) {
+        vv += h[t * 10];
+        for (int k = 0; k < 9; k++) w[k] += h[t * 10 + 1 + k];
+    }
+    if (iopt <= 2) {
+        // self term -alpha/sqrt(pi) sum q^2 (src/ewald_cubic.f90:524-530; src/ewald_noncubic.f90:449-454)
+        const double sc = c->prm.iamcub ? 1.0 / box[0] : 1.0;
+        vv -= (c->prm.walpha / sqrt(acos(-1.0))) * c->sumq2 * sc;
+    }
+    *v = vv;
+    if (vir) memcpy(vir, w, sizeof w);
+}
+
+int ensure_stage(rpmd_ctx* c, size_t n) {
+    if (n <= c->h_cap) return 0;
+    if (c->h_stage) hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    if (hipHostMalloc((void**)&c->h_stage, n * sizeof(double)) != hipSuccess) return RPMD_ERR_NOMEM;
+    c->h_cap = n;
+    return 0;
+}
+
+int upload_charges(rpmd_ctx* c, const double* z) {
+    const int na = c->na;
+    bool same = (int)c->hq.size() == na && !memcmp(c->hq.data(), z, sizeof(double) * na);
+    if (!same) {
+        c->hq.assign(z, z + na);
+        c->sumq2 = 0.0;
+        for (int i = 0; i < na; i++) c->sumq2 = c->sumq2 + z[i] * z[i];
+        HIPCHK(hipMemcpy(c->q, z, sizeof(double) * na, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// host AoS -> device SoA (through pinned staging)
+int upload_aos(rpmd_ctx* c, const double* h, double* d_soa, int nb) {
+    const size_t N = (size_t)3 * c->na * nb;
+    RETCHK(ensure_stage(c, N));
+    memcpy(c->h_stage, h, N * sizeof(double));
+    HIPCHK(hipMemcpyAsync(c->aos, c->h_stage, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    // the staging buffer is reused by the next upload: the copy must have read it first
+    HIPCHK(hipStreamSynchronize(c->st));
+    launch_aos_to_soa(c->aos, d_soa, c->na, nb, c->st);
+    return 0;
+}
+int download_aos(rpmd_ctx* c, const double* d_soa, double* h, int nb) {
+    const size_t N = (size_t)3 * c->na * nb;
+    RETCHK(ensure_stage(c, N));
+    launch_soa_to_aos(d_soa, c->aos, c->na, nb, c->st);
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->aos, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    memcpy(h, c->h_stage, N * sizeof(double));
+    return 0;
+}
+
+int read_ev(rpmd_ctx* c, const double* d_slot, int iopt, const double box[3], double* v, double* vir) {
+    double h[50];
+    HIPCHK(hipMemcpyAsync(h, d_slot, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    combine_ev(c, h, iopt, box, v, vir);
+    return 0;
+}
+
+// rp_contract_nm (src/evolve_pi.f90:324-375): rnm (nb modes) -> forces on nbdf contracted beads ->
+// out_nm (nb modes, zero outside the kept modes)
+int rp_contract_dev(rpmd_ctx* c, const double* rnm, int nb, int nbdf, const double box[3], int iopt, double* out_nm,
+                    double* ev_slot, bool check) {
+    if ((nbdf - 1) % 2 != 0) return RPMD_ERR_NBDF_EVEN;
+    if (nbdf > nb) return RPMD_ERR_NBDF_TOO_LARGE;
+    const int na = c->na;
+    NMMat* m = get_nm(c, nbdf, nb);
+    if (!m) return RPMD_ERR_NOMEM;
+    // maps: in (nbdf entries) and out (nb entries)
+    std::vector<int> mi(nbdf), mo(nb, -1);
+    const int h = (nbdf - 1) / 2;
+    mi[0] = 0;
+    mo[0] = 0;
+    for (int k = 1; k <= h; k++) {
+        mi[k] = k;
+        mi[nbdf - k] = nb - k;
+        mo[k] = k;
+        mo[nb - k] = nbdf - k;
+    }
+    int* dmi = c->maps;
+    int* dmo = c->maps + 512;
+    HIPCHK(hipMemcpyAsync(dmi, mi.data(), sizeof(int) * nbdf, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dmo, mo.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
+    launch_mode_copy(c->tmp2, rnm, dmi, nbdf, na, 1.0, c->st);                 // ring_contract :417-425
+    launch_nm(c->tmp2, c->rb, m->Bs, nbdf, (size_t)3 * na, c->st);           // realft(-1) * sqrt(nbdf/nb)
+    RETCHK(forces_dev(c, c->rb, nbdf, box, iopt, c->dvdrb, ev_slot, check));  // :366
+    launch_nm(c->dvdrb, c->tmp2, m->Fs, nbdf, (size_t)3 * na, c->st);        // force_contract :384-386
+    launch_mode_copy(out_nm, c->tmp2, dmo, nb, na, 1.0, c->st);               // :388-401
+    return 0;
+}
+
+// thermostat coefficients for PILE-L (src/module_gle.f90:22-41) at time step dt
+void pile_coeffs(const rpmd_params& P, const double* mass3, double beta, int nb, double dt, double* c1, double* c3) {
+    const double hbar = 1.0, betan = beta / nb, pi = acos(-1.0), twown = 2.0 / (betan * hbar), pibyn = pi / nb;
+    const double radtwo = sqrt(2.0);
+    double ck[3];
+    for (int j = 0; j < 3; j++) ck[j] = sqrt(mass3[j]);
+    for (int k = 1; k <= nb; k++) {
+        double wk = twown * sin((k - 1) * pibyn), gamma;
+        if (k == 1) gamma = 2.0 * pi / P.ttau;
+        else gamma = 2.0 * wk;
+        double cc1 = exp(-gamma * dt);
+        double c2 = sqrt((1.0 - cc1 * cc1) * 0.5 / betan);
+        if (k == 1 || ((nb % 2 == 0) && k == (nb / 2 + 1))) c2 = radtwo * c2;
+        c1[k - 1] = cc1;
+        for (int j = 0; j < 3; j++) c3[(k - 1) * 3 + j] = c2 * ck[j];
+    }
+}
+
+// ring_rpmd (src/md_pathi.f90:190-231): monod(3,4,nb)
+void ring_rpmd(const double* mass3, double* monod, int nb, double dt, double beta) {
+    const double hbar = 1.0, pi = acos(-1.0);
+    memset(monod, 0, sizeof(double) * 12 * nb);
+    auto MON = [&](int j, int a, int k) -> double& { return monod[((size_t)k * 4 + a) * 3 + j]; };
+    for (int j = 0; j < 3; j++) {
+        MON(j, 0, 0) = 1.0;
+        MON(j, 1, 0) = 0.0;
+        MON(j, 2, 0) = dt / mass3[j];
+        MON(j, 3, 0) = 1.0;
+        if (nb > 1) {
+            double betan = beta / (double)nb, twown = 2.0 / (betan * hbar), pibyn = pi / (double)nb;
+            for (int k = 1; k <= nb / 2; k++) {
+                double wk = twown * sin(k * pibyn), wt = wk * dt, wm = wk * mass3[j];
+                double cwt = cos(wt), swt = sin(wt);
+                MON(j, 0, k) = cwt;
+                MON(j, 1, k) = -wm * swt;
+                MON(j, 2, k) = swt / wm;
+                MON(j, 3, k) = cwt;
+            }
+            for (int k = 1; k <= (nb - 1) / 2; k++)
+                for (int a = 0; a < 4; a++) MON(j, a, nb - k) = MON(j, a, k);
+        }
+    }
+}
+
+// noise source for one draw block of 3*na*nb deviates
+struct NoiseCursor {
+    const double* base;  // device pointer of host-supplied deviates, or null (device stream)
+    size_t used;
+    size_t cap;
+    uint64_t stream_add;
+    const double* take(size_t n) {
+        if (!base) return nullptr;
+        const double* p = base + used;
+        used += n;
+        return p;
+    }
+};
+
+// one evolve_pi step (type 'RPMD', nbaro = 0) on the context's device state
+// (src/evolve_pi.f90:1-322).  Energies -> c->ev_step slots; vir_hf averaged on the host.
+int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nb = c->st_nb, mts = P.mts;
+    const size_t N = (size_t)3 * na * nb, ne = (size_t)3 * na;
+    hipStream_t s = c->st;
+    const double dt = c->st_dt, dtsmall = dt / (double)mts, halfdtsmall = 0.5 * dtsmall, halfdt = 0.5 * dt;
+    const double* box = c->st_box.L;
+    NMMat* nmm = get_nm(c, nb, 0);
+    if (!nmm) return RPMD_ERR_NOMEM;
+    const size_t draws = (size_t)3 * na * nb;
+    if (P.therm == RPMD_THERM_PRG || P.therm == RPMD_THERM_GLE || P.therm == RPMD_THERM_PRM) return RPMD_ERR_THERM;
+    auto outer = [&]() {
+        if (P.therm == RPMD_THERM_PRL) {
+            // parinello_therm_loc with dt (src/evolve_pi.f90:75-77, src/module_thermostat.f90:56-82)
+            const double gamma = 1.0 / (2.0 * P.ttau), c1 = exp(-0.5 * gamma * dt);
+            launch_langevin(c->p, c->lang_c3m, c1, na, nb, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++, s);
+        }
+    };
+    outer();
+    launch_kick(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, N, s);
+    launch_nm(c->p, c->pnm, nmm->F, nb, ne, s);
+    launch_nm(c->r, c->rnm, nmm->F, nb, ne, s);
+    const int nblk = pile_blocks(na);
+    const double inv_mtot = 1.0 / c->mtot;
+    auto pile = [&]() {
+        if (P.therm == RPMD_THERM_PRO) {
+            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
+                        c->com_parts, s);
+            launch_com_remove(c->pnm, c->mass, c->com_parts, nblk, inv_mtot, na, s);
+        }
+    };
+    double* ev_hf = c->ev_step;
+    for (int im = 1; im <= mts; im++) {
+        pile();
+        launch_free_ring(c->pnm, c->rnm, c->monod, na, nb, s);
+        pile();
+        launch_nm(c->rnm, c->rst, nmm->B, nb, ne, s);
+        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, ev_hf + (size_t)(im - 1) * 50, check));
+        if (im != mts) {
+            launch_nm(c->dvdr2, c->tmp, nmm->F, nb, ne, s);
+            launch_kick(c->pnm, c->tmp, dtsmall, nullptr, 0.0, N, s);
+        }
+    }
+    double* ev_lf = c->ev_step + (size_t)mts * 50;  // [EW, EWC-contracted, LJ, EWC-bead] x 50
+    const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
+    if (nbdf1 > 0) {
+        RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, box, 2, c->dvdr, ev_lf, check));
+        if (P.sig > 0.0) {
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, box, 5, c->dvdre, ev_lf + 50, check));
+            launch_axpy(c->dvdr, c->dvdre, -1.0, N, s);
+        }
+    }
+    if (nbdf2 > 0) RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, box, 3, c->dvdrl, ev_lf + 100, check));
+    launch_nm(c->pnm, c->p, nmm->B, nb, ne, s);
+    std::swap(c->r, c->rst);
+    launch_kick(c->p, c->dvdr2, halfdtsmall, nullptr, 0.0, N, s);
+    if (nbdf1 > 0 && nbdf2 > 0) {
+        launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
+        launch_nm(c->dvdr, c->tmp, nmm->B, nb, ne, s);
+        std::swap(c->dvdr, c->tmp);
+    } else if (nbdf1 > 0) {
+        launch_nm(c->dvdr, c->tmp, nmm->B, nb, ne, s);
+        std::swap(c->dvdr, c->tmp);
+    } else if (nbdf2 > 0) {
+        launch_nm(c->dvdrl, c->tmp, nmm->B, nb, ne, s);
+        std::swap(c->dvdrl, c->tmp);
+    }
+    if (nbdf1 <= 0) {
+        RETCHK(forces_dev(c, c->r, nb, box, 2, c->dvdr, ev_lf, check));
+    } else if (P.sig > 0.0) {
+        RETCHK(forces_dev(c, c->r, nb, box, 5, c->dvdre, ev_lf + 150, check));
+        launch_axpy(c->dvdr, c->dvdre, 1.0, N, s);
+    }
+    if (nbdf2 <= 0) RETCHK(forces_dev(c, c->r, nb, box, 3, c->dvdrl, ev_lf + 100, check));
+    if (nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
+    launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
+    outer();
+    launch_counter_inc(c->step_ctr, s);
+    if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
+    return 0;
+}
+
+// energies of the last step (src/evolve_pi.f90:166-169, 292-299)
+int step_energies(rpmd_ctx* c, double* v, double* vew, double* vlj, double* vint, double* vir, double* vir_lf) {
+    const rpmd_params& P = c->prm;
+    const int mts = P.mts;
+    std::vector<double> h((size_t)(mts + 4) * 50);
+    HIPCHK(hipMemcpyAsync(h.data(), c->ev_step, h.size() * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const double* box = c->st_box.L;
+    double vhf = 0, whf[9] = {0}, vi_last = 0;
+    for (int im = 0; im < mts; im++) {
+        double vv, w[9];
+        combine_ev(c, &h[(size_t)im * 50], 4, box, &vv, w);
+        for (int k = 0; k < 9; k++) whf[k] += w[k];
+        vi_last = vv;
+    }
+    for (int k = 0; k < 9; k++) whf[k] /= (double)mts;
+    const double* lf = &h[(size_t)mts * 50];
+    double e_ew, w_ew[9], e_c, w_c[9], e_lj, w_lj[9], e_cb, w_cb[9];
+    combine_ev(c, lf, 2, box, &e_ew, w_ew);
+    combine_ev(c, lf + 50, 5, box, &e_c, w_c);
+    combine_ev(c, lf + 100, 3, box, &e_lj, w_lj);
+    combine_ev(c, lf + 150, 5, box, &e_cb, w_cb);
+    double ew = e_ew, wew[9];
+    for (int k = 0; k < 9; k++) wew[k] = w_ew[k];
+    if (P.nbdf1 > 0 && P.sig > 0.0) {
+        ew = ew - e_c + e_cb;
+        for (int k = 0; k < 9; k++) wew[k] = wew[k] - w_c[k] + w_cb[k];
+    }
+    if (vew) *vew = ew;
+    if (vlj) *vlj = e_lj;
+    if (vint) *vint = vi_last;
+    if (v) *v = ew + e_lj + vi_last;
+    for (int k = 0; k < 9; k++) {
+        double lf9 = w_lj[k] + wew[k];
+        if (vir_lf) vir_lf[k] = lf9;
+        if (vir) vir[k] = lf9 + whf[k];
+    }
+    return 0;
+}
+
+int set_state_consts(rpmd_ctx* c, const double* mass, double beta, double dt, int nb) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na;
+    c->hmass.assign(mass, mass + 3);
+    c->mtot = 0.0;
+    for (int j = 0; j < na; j++) c->mtot = c->mtot + mass[j];
+    if (!get_nm(c, nb, 0)) return RPMD_ERR_NOMEM;
+    std::vector<double> monod((size_t)12 * nb);
+    ring_rpmd(mass, monod.data(), nb, dt / (double)P.mts, beta);
+    HIPCHK(hipMemcpyAsync(c->monod, monod.data(), monod.size() * 8, hipMemcpyHostToDevice, c->st));
+    std::vector<double> c1(nb), c3((size_t)3 * nb);
+    pile_coeffs(P, mass, beta, nb, 0.5 * dt / (double)P.mts, c1.data(), c3.data());
+    HIPCHK(hipMemcpyAsync(c->pile_c1, c1.data(), c1.size() * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pile_c3, c3.data(), c3.size() * 8, hipMemcpyHostToDevice, c->st));
+    std::vector<double> c3m(na);
+    const double gamma = 1.0 / (2.0 * P.ttau), betan = beta / (double)nb;
+    const double cc1 = exp(-0.5 * gamma * dt), c2 = sqrt(1.0 - cc1 * cc1);
+    for (int j = 0; j < na; j++) c3m[j] = c2 * sqrt(mass[j] / betan);
+    HIPCHK(hipMemcpyAsync(c->lang_c3m, c3m.data(), c3m.size() * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->mass, mass, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+size_t noise_per_step(const rpmd_params& P, int na, int nb) {
+    const size_t d = (size_t)3 * na * nb;
+    if (P.therm == RPMD_THERM_PRO) return 2 * (size_t)P.mts * d;
+    if (P.therm == RPMD_THERM_PRL) return 2 * d;
+    return 0;
+}
+
+}  // namespace
+
+// =================================================================================================
+// C-ABI
+// =================================================================================================
+extern "C" {
+
+const char* rpmd_strerror(int code) {
+    switch (code) {
+        case RPMD_OK: return "ok";
+        case RPMD_ERR_NA_NOT_MULT3: return "potenl 1";
+        case RPMD_ERR_NEIGH_OVERFLOW: return "neighbour list too small";
+        case RPMD_ERR_NBDF_EVEN: return "* Beadiabatic factor must be ODD";
+        case RPMD_ERR_NBDF_TOO_LARGE: return " * RP contraction beads must not exceed nb";
+        case RPMD_ERR_RATTLE: return "max iterations exceeded in rattle";
+        case RPMD_ERR_REALFT_N: return "realft 1";
+        case RPMD_ERR_REALFT_MODE: return "realft 3";
+        case RPMD_ERR_EWC_CUTOFF: return " ewc_basic : rcut too large ";
+        case RPMD_ERR_SMEARED: return "smeared charges are not supported on this path";
+        case RPMD_ERR_KMAX: return "Ewald kmax exceeds the device k tile (15)";
+        case RPMD_ERR_ARG: return "invalid argument";
+        case RPMD_ERR_HIP: return "HIP runtime error";
+        case RPMD_ERR_IOPT: return "invalid iopt";
+        case RPMD_ERR_NOMEM: return "device allocation failed";
+        case RPMD_ERR_THERM: return "thermostat not supported by this entry point";
+        case RPMD_ERR_NOISE: return "noise buffer too short";
+    }
+    return "unknown error";
+}
+
+int rpmd_setup_ewald(int na, const double boxlxyz[3], rpmd_params* P) {
+    if (!P || !boxlxyz || na <= 0) return RPMD_ERR_ARG;
+    const double pi = acos(-1.0), cvar = 1.2;
+    const double w = cvar * pow((double)na, -1.0 / 6.0);
+    if (boxlxyz[0] == boxlxyz[1] && boxlxyz[0] == boxlxyz[2] && P->iamcub) {
+        P->iamcub = 1;
+        P->wrcut = std::min(0.5, w);
+        if (P->ecut <= 0.0) {
+            P->walpha = pi / P->wrcut;
+            P->kmax = (int)(P->walpha);
+            P->rkmax = 2.0 * pi * P->walpha;
+        } else {
+            double s = sqrt(-log(P->ecut));
+            P->walpha = s / P->wrcut;
+            P->kmax = (int)(s * P->walpha / pi);
+            P->rkmax = 2.0 * s * P->walpha;
+        }
+    } else {
+        P->iamcub = 0;
+        const double rshort = std::min(boxlxyz[0], std::min(boxlxyz[1], boxlxyz[2]));
+        const double rlong = std::max(boxlxyz[0], std::max(boxlxyz[1], boxlxyz[2]));
+        P->wrcut = rshort * std::min(0.5, w);
+        P->walpha = pi / P->wrcut;
+        P->rkmax = 2.0 * pi * P->walpha;
+        P->kmax = (int)(P->walpha * rlong);
+    }
+    return 0;
+}
+
+int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, rpmd_ctx** out) {
+    if (!prm || !out || na <= 0 || nb_max <= 0) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (nb_max > 256) return RPMD_ERR_REALFT_N;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return RPMD_ERR_HIP;
+    rpmd_ctx* c = new rpmd_ctx();
+    c->prm = *prm;
+    c->na = na;
+    c->nb_max = nb_max;
+    c->prec = precision;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RPMD_ERR_HIP;
+    }
+    const size_t N = (size_t)3 * na * nb_max;
+    const int nbl = nb_max;
+    const int nck = kspace_nchunk(na);
+    c->ncell_cap = 64 * 64 * 64;
+    c->off_real = 0;
+    c->off_k = c->off_real + (size_t)nbl * ewald_real_blocks(na) * 10;
+    c->off_oo = c->off_k + (size_t)nbl * (RPMD_KMAX_DEV + 1) * 10;
+    c->off_tem = c->off_oo + (size_t)nbl * oo_blocks(na) * 10;
+    c->off_intra = c->off_tem + (size_t)nbl * tem_blocks(na) * 10;
+    c->nparts = c->off_intra + (size_t)nbl * intra_blocks(na) * 10;
+    int e = 0;
+    e |= dalloc(&c->aos, N);
+    e |= dalloc(&c->pos, N);
+    e |= dalloc(&c->grad, N);
+    e |= dalloc(&c->site, N);
+    e |= dalloc(&c->gsite, N);
+    e |= dalloc(&c->sorted, N);
+    e |= dalloc(&c->q, na);
+    e |= dalloc(&c->mass, na);
+    e |= dalloc(&c->cen, (size_t)3 * na);
+    e |= dalloc(&c->parts, c->nparts);
+    e |= dalloc(&c->ev, 50);
+    e |= dalloc(&c->skp, (size_t)nbl * (RPMD_KMAX_DEV + 1) * nck * RPMD_KTILE * RPMD_KTILE * 2);
+    e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
+    e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
+    e |= dalloc(&c->cell_of, (size_t)na * nbl);
+    e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
+    e |= dalloc(&c->start, (size_t)(c->ncell_cap + 1) * nbl);
+    e |= dalloc(&c->perm, (size_t)na * nbl);
+    e |= dalloc(&c->nnb, na);
+    e |= dalloc(&c->nbl, (size_t)na * RPMD_NCAP);
+    e |= dalloc(&c->ovf, 2);
+    for (double** b : {&c->p, &c->r, &c->dvdr, &c->dvdr2, &c->dvdrl, &c->dvdre, &c->rst, &c->pnm, &c->rnm, &c->tmp,
+                       &c->rb, &c->dvdrb, &c->tmp2})
+        e |= dalloc(b, N);
+    e |= dalloc(&c->monod, (size_t)12 * nbl);
+    e |= dalloc(&c->pile_c1, nbl);
+    e |= dalloc(&c->pile_c3, (size_t)3 * nbl);
+    e |= dalloc(&c->com_parts, (size_t)3 * pile_blocks(na));
+    e |= dalloc(&c->lang_c3m, na);
+    e |= dalloc(&c->maps, 1024);
+    e |= dalloc(&c->step_ctr, 1);
+    e |= dalloc(&c->ev_step, (size_t)(64 + 4) * 50);
+    if (e) {
+        rpmd_ctx_destroy(c);
+        return RPMD_ERR_NOMEM;
+    }
+    hipMemset(c->step_ctr, 0, sizeof(uint64_t));
+    *out = c;
+    return 0;
+}
+
+int rpmd_ctx_set_params(rpmd_ctx* c, const rpmd_params* prm) {
+    if (!c || !prm) return RPMD_ERR_ARG;
+    c->prm = *prm;
+    c->ks_valid = false;
+    c->st_valid = false;
+    return 0;
+}
+
+int rpmd_ctx_set_precision(rpmd_ctx* c, int precision) {
+    if (!c) return RPMD_ERR_ARG;
+    c->prec = precision;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return 0;
+}
+
+int rpmd_ctx_destroy(rpmd_ctx* c) {
+    if (!c) return 0;
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    void* bufs[] = {c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+                    c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
+                    c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
+                    c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
+                    c->step_ctr, c->ev_step};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (auto& kv : c->nm) {
+        hipFree(kv.second.F);
+        hipFree(kv.second.B);
+        hipFree(kv.second.Fs);
+        hipFree(kv.second.Bs);
+    }
+    if (c->h_stage) hipHostFree(c->h_stage);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return 0;
+}
+
+int rpmd_forces(rpmd_ctx* c, const double* r, int nb, int na, const double boxlxyz[3], const double* z, int iopt,
+                double* v, double* dvdr, double* virial) {
+    if (!c || !r || !boxlxyz || !z || !v || !dvdr || !virial) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    if (iopt < 0 || iopt > 5) return RPMD_ERR_IOPT;
+    RETCHK(upload_charges(c, z));
+    RETCHK(upload_aos(c, r, c->pos, nb));
+    RETCHK(forces_dev(c, c->pos, nb, boxlxyz, iopt, c->grad, c->ev, true));
+    RETCHK(download_aos(c, c->grad, dvdr, nb));
+    RETCHK(read_ev(c, c->ev, iopt, boxlxyz, v, virial));
+    return 0;
+}
+
+int rpmd_full_forces(rpmd_ctx* c, const double* r, int na, int nb, double* v, double* vew, double* voo, double* vint,
+                     double* vir, const double* z, const double boxlxyz[3], double* dvdr, double* dvdr2) {
+    if (!c || !r || !z || !boxlxyz || !dvdr || !dvdr2) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    const rpmd_params& P = c->prm;
+    const size_t N = (size_t)3 * na * nb;
+    hipStream_t s = c->st;
+    RETCHK(upload_charges(c, z));
+    RETCHK(upload_aos(c, r, c->pos, nb));
+    double e_int, w_int[9], e_ew = 0, w_ew[9] = {0}, e_oo = 0, w_oo[9] = {0}, tv, tw[9];
+    // high-frequency forces (src/md_forces.f90:350)
+    RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 4, c->dvdr2, c->ev, true));
+    RETCHK(read_ev(c, c->ev, 4, boxlxyz, &e_int, w_int));
+    launch_zero(c->dvdr, N, s);
+    launch_zero(c->dvdrl, N, s);
+    const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
+    if (nb > 1 && (nbdf1 > 0 || nbdf2 > 0)) {
+        NMMat* m = get_nm(c, nb, 0);
+        if (!m) return RPMD_ERR_NOMEM;
+        launch_nm(c->pos, c->rnm, m->F, nb, (size_t)3 * na, s);  // :359
+        if (nbdf1 > 0) {
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 2, c->dvdr, c->ev, true));
+            RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
+            if (P.sig > 0.0) {
+                RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 5, c->dvdre, c->ev, true));
+                RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
+                e_ew -= tv;
+                for (int k = 0; k < 9; k++) w_ew[k] -= tw[k];
+                launch_axpy(c->dvdr, c->dvdre, -1.0, N, s);
+            }
+        }
+        if (nbdf2 > 0) {
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, boxlxyz, 3, c->dvdrl, c->ev, true));
+            RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
+        }
+        if (nbdf1 > 0 && nbdf2 > 0) {
+            launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
+            launch_nm(c->dvdr, c->tmp, m->B, nb, (size_t)3 * na, s);
+            std::swap(c->dvdr, c->tmp);
+        } else if (nbdf1 > 0) {
+            launch_nm(c->dvdr, c->tmp, m->B, nb, (size_t)3 * na, s);
+            std::swap(c->dvdr, c->tmp);
+        } else if (nbdf2 > 0) {
+            launch_nm(c->dvdrl, c->tmp, m->B, nb, (size_t)3 * na, s);
+            std::swap(c->dvdrl, c->tmp);
+        }
+    }
+    if (nb == 1 || nbdf1 == 0) {
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 2, c->dvdr, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
+    } else if (P.sig > 0.0) {
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 5, c->dvdre, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
+        e_ew += tv;
+        for (int k = 0; k < 9; k++) w_ew[k] += tw[k];
+        launch_axpy(c->dvdr, c->dvdre, 1.0, N, s);
+    }
+    if (nb == 1 || nbdf2 == 0) {
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 3, c->dvdrl, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
+    }
+    if (nb == 1 || nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
+    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
+    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    if (vew) *vew = e_ew;
+    if (voo) *voo = e_oo;
+    if (vint) *vint = e_int;
+    if (v) *v = e_ew + e_oo + e_int;
+    if (vir)
+        for (int k = 0; k < 9; k++) vir[k] = w_oo[k] + w_ew[k] + w_int[k];
+    return 0;
+}
+
+int rpmd_realft(rpmd_ctx* c, double* data, int m, int n, int mode) {
+    if (!c || !data || m <= 0 || n <= 0) return RPMD_ERR_ARG;
+    if (n > 256) return RPMD_ERR_REALFT_N;
+    if (mode != 1 && mode != -1) return RPMD_ERR_REALFT_MODE;
+    const size_t N = (size_t)m * n;
+    if (N > (size_t)3 * c->na * c->nb_max) return RPMD_ERR_ARG;
+    NMMat* mm = get_nm(c, n, 0);
+    if (!mm) return RPMD_ERR_NOMEM;
+    RETCHK(ensure_stage(c, N));
+    memcpy(c->h_stage, data, N * 8);
+    HIPCHK(hipMemcpyAsync(c->tmp, c->h_stage, N * 8, hipMemcpyHostToDevice, c->st));
+    launch_nm(c->tmp, c->tmp2, mode == 1 ? mm->F : mm->B, n, (size_t)m, c->st);
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->tmp2, N * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    memcpy(data, c->h_stage, N * 8);
+    return 0;
+}
+
+int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint, double* dvdr,
+                   double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
+                   const double* z, double beta, double* vir, double* vir_lf, const double* noise, long long n_noise) {
+    if (!c || !p || !r || !dvdr || !dvdr2 || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    const size_t need = noise_per_step(c->prm, na, nb);
+    if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
+    RETCHK(upload_charges(c, z));
+    c->st_nb = nb;
+    c->st_box = make_box(boxlxyz);
+    c->st_beta = beta;
+    c->st_dt = dt;
+    RETCHK(set_state_consts(c, mass, beta, dt, nb));
+    RETCHK(upload_aos(c, p, c->p, nb));
+    RETCHK(upload_aos(c, r, c->r, nb));
+    RETCHK(upload_aos(c, dvdr, c->dvdr, nb));
+    RETCHK(upload_aos(c, dvdr2, c->dvdr2, nb));
+    NoiseCursor nz{nullptr, 0, 0, 0};
+    if (noise && need) {
+        if (c->noise_cap < need) {
+            if (c->noise) hipFree(c->noise);
+            c->noise = nullptr;
+            RETCHK(dalloc(&c->noise, need));
+            c->noise_cap = need;
+        }
+        HIPCHK(hipMemcpyAsync(c->noise, noise, need * 8, hipMemcpyHostToDevice, c->st));
+        nz.base = c->noise;
+        nz.cap = need;
+    }
+    RETCHK(step_dev(c, nz, true));
+    RETCHK(download_aos(c, c->p, p, nb));
+    RETCHK(download_aos(c, c->r, r, nb));
+    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
+    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    RETCHK(step_energies(c, v, vew, vlj, vint, vir, vir_lf));
+    c->st_valid = false;
+    return 0;
+}
+
+int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const double* mass, const double* z, int na,
+                      int nb, const double boxlxyz[3], double beta, double dt, unsigned long long seed) {
+    if (!c || !p || !r || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    RETCHK(upload_charges(c, z));
+    c->st_nb = nb;
+    c->st_box = make_box(boxlxyz);
+    c->st_beta = beta;
+    c->st_dt = dt;
+    c->seed = seed;
+    RETCHK(set_state_consts(c, mass, beta, dt, nb));
+    RETCHK(upload_aos(c, p, c->p, nb));
+    RETCHK(upload_aos(c, r, c->r, nb));
+    HIPCHK(hipMemsetAsync(c->step_ctr, 0, sizeof(uint64_t), c->st));
+    // initial forces: full_forces semantics for the uncontracted case (src/md_forces.f90:305-472)
+    const size_t N = (size_t)3 * na * nb;
+    RETCHK(forces_dev(c, c->r, nb, boxlxyz, 4, c->dvdr2, c->ev, true));
+    RETCHK(forces_dev(c, c->r, nb, boxlxyz, 2, c->dvdr, c->ev, true));
+    RETCHK(forces_dev(c, c->r, nb, boxlxyz, 3, c->dvdrl, c->ev, true));
+    launch_axpy(c->dvdr, c->dvdrl, 1.0, N, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->st_valid = true;
+    return 0;
+}
+
+int rpmd_use_graph(rpmd_ctx* c, int enable) {
+    if (!c) return RPMD_ERR_ARG;
+    c->use_graph = enable;
+    if (!enable && c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return 0;
+}
+
+int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
+    if (!c || !c->st_valid || nsteps < 0) return RPMD_ERR_ARG;
+    if (c->prm.nbdf1 > 0 || c->prm.nbdf2 > 0) {
+        // contraction maps are uploaded per call: run eagerly
+        for (int i = 0; i < nsteps; i++) {
+            NoiseCursor nz{nullptr, 0, 0, 0};
+            RETCHK(step_dev(c, nz, false));
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        return 0;
+    }
+    if (!c->use_graph || c->prof) {
+        for (int i = 0; i < nsteps; i++) {
+            NoiseCursor nz{nullptr, 0, 0, 0};
+            RETCHK(step_dev(c, nz, false));
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        return 0;
+    }
+    // graph path: one step per replay; the step swaps r/rst (and dvdr/tmp), so capture two steps
+    // to return the buffers to their starting roles
+    if (!c->gexec) {
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < 2; i++) {
+            NoiseCursor nz{nullptr, 0, 0, 0};
+            int rc = step_dev(c, nz, false);
+            if (rc) {
+                hipGraph_t g;
+                hipStreamEndCapture(c->st, &g);
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+        }
+        HIPCHK(hipStreamEndCapture(c->st, &c->graph));
+        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    }
+    int i = 0;
+    for (; i + 1 < nsteps; i += 2) HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    if (i < nsteps) {
+        NoiseCursor nz{nullptr, 0, 0, 0};
+        // an odd tail runs eagerly and leaves buffer roles swapped; the graph is rebuilt next time
+        RETCHK(step_dev(c, nz, false));
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        hipGraphDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int rpmd_prof_begin(rpmd_ctx* c) {
+    if (!c) return RPMD_ERR_ARG;
+    c->prof = 1;
+    c->ev_marks.clear();
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return 0;
+}
+
+int rpmd_prof_end(rpmd_ctx* c, double ms[4], int counts[4]) {
+    if (!c) return RPMD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < 4; k++) {
+        ms[k] = 0.0;
+        counts[k] = 0;
+    }
+    for (auto& m : c->ev_marks) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        ms[m.first] += t;
+        counts[m.first] += 1;
+    }
+    c->prof = 0;
+    c->ev_marks.clear();
+    return 0;
+}
+
+int rpmd_state_download(rpmd_ctx* c, double* p, double* r, double* dvdr, double* dvdr2, double energies[4],
+                        double vir[9]) {
+    if (!c || !c->st_valid) return RPMD_ERR_ARG;
+    const int nb = c->st_nb;
+    if (p) RETCHK(download_aos(c, c->p, p, nb));
+    if (r) RETCHK(download_aos(c, c->r, r, nb));
+    if (dvdr) RETCHK(download_aos(c, c->dvdr, dvdr, nb));
+    if (dvdr2) RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    if (energies) RETCHK(step_energies(c, &energies[0], &energies[1], &energies[2], &energies[3], vir, nullptr));
+    return 0;
+}
+
+}  // extern "C"

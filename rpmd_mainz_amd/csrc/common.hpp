@@ -1,0 +1,92 @@
+// common.hpp -- shared device/host definitions of the MI355X RPMD path.
+//
+// Device layout (all in HBM, one context per host thread):
+//   positions, gradients, momenta : SoA per bead, X[b][d][a] = base[(b*3 + d)*na + a], FP64
+//   Ewald sites                   : same SoA layout; the oxygen replaced by the M-site and the
+//                                   coordinates scaled to the unit box when the box is cubic
+//   per-block energy/virial parts : double[10] = {v, vir(3,3) column-major}
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+#include "../../include/rpmd_mi355x.h"
+
+#define RPMD_KTILE 32   // k-space tile: ky, kz in [-15, 16] -> kmax <= 15
+#define RPMD_KMAX_DEV 15
+#define RPMD_NCAP 512   // O-O / TEM neighbour capacity per atom (full list)
+
+namespace rpmd {
+
+struct Box {
+    double L[3], inv[3];
+};
+
+struct CellGrid {
+    int nc[3];     // cells per box side
+    int reach[3];  // stencil half-width in cells (2*reach+1 > nc: the whole axis once)
+    int ncell;     // nc[0]*nc[1]*nc[2]
+};
+
+struct KSet {
+    int kmax;
+    int ky_max[RPMD_KMAX_DEV + 1];    // per kx: largest |ky| with a member (-1: none)
+    int kz_bound[RPMD_KMAX_DEV + 1];  // per kx: largest |kz| with a member
+    double lat[3];                    // 2 pi (cubic, unit box) or 2 pi / L_d
+    double f0;                        // 2 pi (cubic) or 2 pi / V
+    double b;                         // 1/(4 alpha^2)
+    int cubic;
+};
+
+// ---- k_forces.hip -----------------------------------------------------------------------------
+void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double* out, int nout, double scale,
+                      hipStream_t s);
+void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
+void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
+void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
+void launch_chain_rule(double* grad, const double* gsite, int na, int nb, double alpha, double scale, hipStream_t s);
+void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
+void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
+void launch_zero(double* y, size_t n, hipStream_t s);
+void launch_zero_int(int* y, size_t n, hipStream_t s);
+void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int* cell_of, int* count,
+                       int* start, int* perm, double* sorted, hipStream_t s);
+template <typename T>
+void launch_ewald_real(const double* sorted, const int* perm, const int* start, const double* q, int na, int nb,
+                       CellGrid g, Box box, double rcut, double alpha, double* gsite, double* parts, hipStream_t s);
+int ewald_real_blocks(int na);
+int kspace_nchunk(int na);
+void launch_sk(const double* site, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s);
+void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
+                     double* parts_k, hipStream_t s);
+void launch_kforce(const double* site, const double* q, int na, int nb, const KSet& ks, const double* gk,
+                   double* gsite, hipStream_t s);
+void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
+                  hipStream_t s);
+template <typename T>
+void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
+               const rpmd_params& p, double* grad, double* parts, hipStream_t s);
+int oo_blocks(int na);
+void launch_tem(const double* site, const double* q, int na, int nb, Box box, const int* nnb, const int* nbl,
+                int use_list, double sig, double* gsite, double* parts, hipStream_t s);
+int tem_blocks(int na);
+void launch_intra(const double* pos, int na, int nb, int model, const rpmd_params& p, double* grad, double* parts,
+                  hipStream_t s);
+int intra_blocks(int na);
+
+// ---- k_step.hip -------------------------------------------------------------------------------
+void launch_nm(const double* in, double* out, const double* M, int n, size_t ne, hipStream_t s);
+void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s);
+void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s);
+int pile_blocks(int na);
+void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, const double* noise, uint64_t seed,
+                 const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);
+void launch_com_remove(double* p, const double* mass, const double* com_parts, int nblk, double inv_mtot, int na,
+                       hipStream_t s);
+void launch_langevin(double* p, const double* c3m, double c1, int na, int nb, const double* noise, uint64_t seed,
+                     const uint64_t* stream, uint64_t stream_add, hipStream_t s);
+void launch_mode_copy(double* dst, const double* src, const int* map, int ndst, int na, double scale, hipStream_t s);
+void launch_scale(double* y, double a, size_t n, hipStream_t s);
+void launch_counter_inc(uint64_t* c, hipStream_t s);
+
+}  // namespace rpmd

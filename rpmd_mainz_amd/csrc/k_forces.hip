@@ -1,0 +1,916 @@
+// k_forces.hip -- force kernels of the RPMD path on gfx950.
+//
+// All bead-batched: one launch covers every bead of a forces() call (grid.y or grid.z = bead).
+// Per-atom gradients are owned by one thread per atom (full neighbour sums, no atomics), so every
+// result is deterministic run to run.  Energies and virials are accumulated in FP64 per thread,
+// reduced per block into parts[block][10] and summed in a fixed order by k_reduce_parts.
+#include "common.hpp"
+
+namespace rpmd {
+
+// A&S 7.1.26 erfc constants (src/ewald_cubic.f90:94-99)
+constexpr double AS_P = 3.0525860, AS_A1 = 0.254829592, AS_A2 = -0.284496736, AS_A3 = 1.421413741,
+                 AS_A4 = -1.453152027, AS_A5 = 1.061405429;
+
+// ------------------------------------------------------------------------------------------------
+// reductions
+// ------------------------------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out) {
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < NV; i++) v[i] += __shfl_down(v[i], off, 64);
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; i++) red[w][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int k = 0; k < nw; k++) s += red[k][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// out[k] = scale * sum_p parts[p*stride + k], k < nout; one block, fixed order per k
+__global__ void k_reduce_parts(const double* __restrict__ parts, int nparts, int stride, double* out, int nout,
+                               double scale) {
+    __shared__ double red[1024];
+    for (int k = 0; k < nout; k++) {
+        double s = 0.0;
+        for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += parts[(size_t)p * stride + k];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = scale * red[0];
+        __syncthreads();
+    }
+}
+
+void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double* out, int nout, double scale,
+                      hipStream_t s) {
+    k_reduce_parts<<<1, 1024, 0, s>>>(parts, nparts, stride_parts, out, nout, scale);
+}
+
+// ------------------------------------------------------------------------------------------------
+// layout: Fortran r(3,na,nb) <-> SoA X[b][d][a]
+// ------------------------------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, int na, int nb) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)na * nb;
+    if (i >= n) return;
+    size_t b = i / na, a = i % na;
+#pragma unroll
+    for (int d = 0; d < 3; d++) soa[(b * 3 + d) * na + a] = aos[i * 3 + d];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, int na, int nb) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)na * nb;
+    if (i >= n) return;
+    size_t b = i / na, a = i % na;
+#pragma unroll
+    for (int d = 0; d < 3; d++) aos[i * 3 + d] = soa[(b * 3 + d) * na + a];
+}
+void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s) {
+    size_t n = (size_t)na * nb;
+    k_aos_to_soa<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(aos, soa, na, nb);
+}
+void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s) {
+    size_t n = (size_t)na * nb;
+    k_soa_to_aos<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(soa, aos, na, nb);
+}
+
+// M-site substitution r_M = alpha r_O + (1-alpha)/2 (r_H1 + r_H2) (src/md_forces.f90:181-188),
+// optionally scaled to the unit box (src/ewald_cubic.f90:21-22).
+__global__ void k_sites(const double* __restrict__ pos, double* __restrict__ site, int na, int nb, double alpha,
+                        double s0, double s1, double s2) {
+    int nm = na / 3;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nm * nb) return;
+    size_t b = i / nm, m = i % nm;
+    double alpha2 = 0.5 * (1.0 - alpha);
+    double sc[3] = {s0, s1, s2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double* p = pos + (b * 3 + d) * na + 3 * m;
+        double* q = site + (b * 3 + d) * na + 3 * m;
+        double o = alpha * p[0] + alpha2 * (p[1] + p[2]);
+        q[0] = sc[d] * o;
+        q[1] = sc[d] * p[1];
+        q[2] = sc[d] * p[2];
+    }
+}
+void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s) {
+    size_t n = (size_t)(na / 3) * nb;
+    double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
+    k_sites<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2);
+}
+
+// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215)
+__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs, int na, int nb, double alpha,
+                        double scale) {
+    int nm = na / 3;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nm * nb) return;
+    size_t b = i / nm, m = i % nm;
+    double alpha2 = 0.5 * (1.0 - alpha);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        size_t o = (b * 3 + d) * na + 3 * m;
+        double g0 = scale * gs[o], g1 = scale * gs[o + 1], g2 = scale * gs[o + 2];
+        grad[o] += alpha * g0;
+        grad[o + 1] += g1 + alpha2 * g0;
+        grad[o + 2] += g2 + alpha2 * g0;
+    }
+}
+void launch_chain_rule(double* grad, const double* gsite, int na, int nb, double alpha, double scale,
+                       hipStream_t s) {
+    size_t n = (size_t)(na / 3) * nb;
+    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, gsite, na, nb, alpha, scale);
+}
+
+// bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
+__global__ void k_centroid(const double* __restrict__ pos, double* __restrict__ cen, int na, int nb) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)3 * na) return;
+    size_t d = i / na, a = i % na;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s = s + pos[((size_t)b * 3 + d) * na + a];
+    cen[d * na + a] = s / (double)nb;
+}
+void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s) {
+    size_t n = (size_t)3 * na;
+    k_centroid<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, cen, na, nb);
+}
+
+__global__ void k_axpy(double* y, const double* x, double a, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s) {
+    k_axpy<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, x, a, n);
+}
+__global__ void k_zero(double* y, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = 0.0;
+}
+void launch_zero(double* y, size_t n, hipStream_t s) {
+    if (n) k_zero<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, n);
+}
+__global__ void k_zero_int(int* y, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = 0;
+}
+void launch_zero_int(int* y, size_t n, hipStream_t s) {
+    if (n) k_zero_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// cell lists for the real-space sum (per bead; counting sort, each cell then ordered by atom index)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
+    double f = x * inv;
+    f = f - floor(f);
+    int c = (int)(f * nc);
+    return c < nc ? (c >= 0 ? c : 0) : nc - 1;
+}
+
+__global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, double i0, double i1,
+                              double i2, int* __restrict__ cell_of, int* __restrict__ count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)na * nb) return;
+    size_t b = i / na, a = i % na;
+    const double* s = site + b * 3 * na;
+    int cx = cell_coord(s[a], i0, g.nc[0]);
+    int cy = cell_coord(s[na + a], i1, g.nc[1]);
+    int cz = cell_coord(s[2 * na + a], i2, g.nc[2]);
+    int cid = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    cell_of[i] = cid;
+    atomicAdd(&count[b * (g.ncell + 1) + cid], 1);
+}
+
+// exclusive scan per bead (one block per bead); resets count to 0 for the scatter pass
+__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell) {
+    int b = blockIdx.x;
+    int* c = count + (size_t)b * (ncell + 1);
+    int* st = start + (size_t)b * (ncell + 1);
+    __shared__ int carry;
+    __shared__ int buf[1024];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < ncell; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < ncell ? c[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < ncell) {
+            st[i] = carry + buf[threadIdx.x] - v;
+            c[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st[ncell] = carry;
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, int ncell, const int* __restrict__ start,
+                               int* __restrict__ fill, int* __restrict__ perm) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)na * nb) return;
+    size_t b = i / na, a = i % na;
+    int cid = cell_of[i];
+    int slot = start[b * (ncell + 1) + cid] + atomicAdd(&fill[b * (ncell + 1) + cid], 1);
+    perm[b * na + slot] = (int)a;
+}
+
+// order each cell by atom index, then gather the sorted positions
+__global__ void k_cell_order(const double* __restrict__ site, int na, int nb, int ncell, const int* __restrict__ start,
+                             int* __restrict__ perm, double* __restrict__ sorted) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ncell * nb) return;
+    size_t b = i / ncell, c = i % ncell;
+    int s0 = start[b * (ncell + 1) + c], s1 = start[b * (ncell + 1) + c + 1];
+    int* p = perm + b * na;
+    for (int x = s0 + 1; x < s1; x++) {
+        int v = p[x], y = x - 1;
+        while (y >= s0 && p[y] > v) {
+            p[y + 1] = p[y];
+            y--;
+        }
+        p[y + 1] = v;
+    }
+    const double* s = site + b * 3 * na;
+    double* o = sorted + b * 3 * na;
+    for (int x = s0; x < s1; x++) {
+        int a = p[x];
+        o[x] = s[a];
+        o[na + x] = s[na + a];
+        o[2 * na + x] = s[2 * na + a];
+    }
+}
+
+void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int* cell_of, int* count,
+                       int* start, int* perm, double* sorted, hipStream_t s) {
+    size_t n = (size_t)na * nb;
+    launch_zero_int(count, (size_t)(g.ncell + 1) * nb, s);
+    k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, unitbox.inv[0], unitbox.inv[1],
+                                                              unitbox.inv[2], cell_of, count);
+    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, g.ncell);
+    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, g.ncell, start, count, perm);
+    size_t nc = (size_t)g.ncell * nb;
+    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, g.ncell, start, perm, sorted);
+}
+
+// ------------------------------------------------------------------------------------------------
+// real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
+// one thread per (bead, sorted atom); sums every neighbour (each pair seen from both ends, so the
+// energy and virial carry a factor 1/2); T is the arithmetic type of the pair body.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& du, T& dur) {
+    T dr = sqrt(dr2);
+    T x = alpha * dr;
+    T e = exp(-x * x);
+    T t = (T)AS_P / ((T)AS_P + x);
+    T erfc_ = e * t * ((T)AS_A1 + t * ((T)AS_A2 + t * ((T)AS_A3 + t * ((T)AS_A4 + t * (T)AS_A5))));
+    if (same) erfc_ = erfc_ - (T)1;
+    du = erfc_ / dr;
+    dur = (sfac * e - du) / dr2;
+}
+template <>
+__device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float sfac, bool same, float& du, float& dur) {
+    float rinv = __frsqrt_rn(dr2);
+    float dr = dr2 * rinv;
+    float x = alpha * dr;
+    float e = __expf(-x * x);
+    float t = __frcp_rn((float)AS_P + x) * (float)AS_P;
+    float erfc_ = e * t * ((float)AS_A1 + t * ((float)AS_A2 + t * ((float)AS_A3 + t * ((float)AS_A4 + t * (float)AS_A5))));
+    if (same) erfc_ = erfc_ - 1.0f;
+    du = erfc_ * rinv;
+    dur = (sfac * e - du) * (rinv * rinv);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_real(const double* __restrict__ sorted, const int* __restrict__ perm,
+                                                    const int* __restrict__ start, const double* __restrict__ q, int na,
+                                                    CellGrid g, double L0, double L1, double L2, double i0, double i1,
+                                                    double i2, double rc2, double alpha, double sfac,
+                                                    double* __restrict__ gsite, double* __restrict__ parts) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double* X = sorted + (size_t)b * 3 * na;
+    const int* P = perm + (size_t)b * na;
+    const int* S = start + (size_t)b * (g.ncell + 1);
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    if (i < na) {
+        const double xi = X[i], yi = X[na + i], zi = X[2 * na + i];
+        const int ai = P[i];
+        const double qi = q[ai];
+        const int mi = ai / 3;
+        const int cx = cell_coord(xi, i0, g.nc[0]), cy = cell_coord(yi, i1, g.nc[1]), cz = cell_coord(zi, i2, g.nc[2]);
+        int lo[3], hi[3];
+        const int cc[3] = {cx, cy, cz};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (2 * g.reach[d] + 1 <= g.nc[d]) {
+                lo[d] = cc[d] - g.reach[d];
+                hi[d] = cc[d] + g.reach[d];
+            } else {
+                lo[d] = 0;
+                hi[d] = g.nc[d] - 1;
+            }
+        }
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const T ta = (T)alpha, tsf = (T)sfac;
+        for (int oz = lo[2]; oz <= hi[2]; oz++) {
+            int z = oz < 0 ? oz + g.nc[2] : (oz >= g.nc[2] ? oz - g.nc[2] : oz);
+            for (int oy = lo[1]; oy <= hi[1]; oy++) {
+                int y = oy < 0 ? oy + g.nc[1] : (oy >= g.nc[1] ? oy - g.nc[1] : oy);
+                const int rowbase = (z * g.nc[1] + y) * g.nc[0];
+                for (int ox = lo[0]; ox <= hi[0]; ox++) {
+                    int x = ox < 0 ? ox + g.nc[0] : (ox >= g.nc[0] ? ox - g.nc[0] : ox);
+                    const int c = rowbase + x;
+                    const int j0 = S[c], j1 = S[c + 1];
+                    for (int j = j0; j < j1; j++) {
+                        if (j == i) continue;
+                        double dx = xi - X[j], dy = yi - X[na + j], dz = zi - X[2 * na + j];
+                        dx -= L0 * rint(dx * i0);
+                        dy -= L1 * rint(dy * i1);
+                        dz -= L2 * rint(dz * i2);
+                        const double dr2 = dx * dx + dy * dy + dz * dz;
+                        if (dr2 < rc2) {
+                            const int aj = P[j];
+                            T du, dur;
+                            erfc_pair<T>((T)dr2, ta, tsf, (aj / 3) == mi, du, dur);
+                            const double qij = qi * q[aj];
+                            const double dv = qij * (double)du, dvr = qij * (double)dur;
+                            const double fx = dvr * dx, fy = dvr * dy, fz = dvr * dz;
+                            gx += fx;
+                            gy += fy;
+                            gz += fz;
+                            acc[0] += 0.5 * dv;
+                            acc[1] += 0.5 * dx * fx; acc[2] += 0.5 * dy * fx; acc[3] += 0.5 * dz * fx;
+                            acc[4] += 0.5 * dx * fy; acc[5] += 0.5 * dy * fy; acc[6] += 0.5 * dz * fy;
+                            acc[7] += 0.5 * dx * fz; acc[8] += 0.5 * dy * fz; acc[9] += 0.5 * dz * fz;
+                        }
+                    }
+                }
+            }
+        }
+        double* G = gsite + (size_t)b * 3 * na;
+        G[ai] = gx;
+        G[na + ai] = gy;
+        G[2 * na + ai] = gz;
+    }
+    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+}
+
+template <typename T>
+void launch_ewald_real(const double* sorted, const int* perm, const int* start, const double* q, int na, int nb,
+                       CellGrid g, Box box, double rcut, double alpha, double* gsite, double* parts, hipStream_t s) {
+    double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
+    dim3 grid((na + 255) / 256, nb);
+    k_ewald_real<T><<<grid, 256, 0, s>>>(sorted, perm, start, q, na, g, box.L[0], box.L[1], box.L[2], box.inv[0],
+                                         box.inv[1], box.inv[2], rcut * rcut, alpha, sfac, gsite, parts);
+}
+template void launch_ewald_real<double>(const double*, const int*, const int*, const double*, int, int, CellGrid, Box,
+                                        double, double, double*, double*, hipStream_t);
+template void launch_ewald_real<float>(const double*, const int*, const int*, const double*, int, int, CellGrid, Box,
+                                       double, double, double*, double*, hipStream_t);
+int ewald_real_blocks(int na) { return (na + 255) / 256; }
+
+// ------------------------------------------------------------------------------------------------
+// reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
+//
+// Structure factor, per bead and kx: S(kx,ky,kz) = sum_j [q_j X_j(kx) Z_j(kz)] Y_j(ky) is a
+// complex GEMM over atoms of a (ky) x (kz) tile; k_sk stages NJ atoms' Y and qXZ rows in LDS and
+// each of 64 lanes accumulates a 4x4 complex register tile.  Atom chunks give partial sums.
+// k_sk_final sums the chunks, applies the reference weights w = (f/k^2) exp(-b k^2) on the
+// reference k set (mask built on the host with the reference's own comparisons) and stores
+// G = 2 w S.  k_kforce evaluates, per atom, U(ky) = sum_kz G w^kz with w = conj(exp(i lat_z z))
+// by Horner's rule (and its kz-weighted derivative), so the force pass needs no tables.
+// ------------------------------------------------------------------------------------------------
+#define KT 32
+#define KH 15
+#define NJ 16
+#define SK_CHUNK 512
+
+struct KDev {
+    int kmax;
+    int cubic;
+    int kyb[KH + 1];   // per kx: max |ky| with a member
+    int kzb[KH + 1];   // per kx: max |kz| with a member
+    double lat0, lat1, lat2;
+    double f0, b;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(64) void k_sk(const double* __restrict__ site, const double* __restrict__ q, int na,
+                                           int nchunk, KDev kd, double2* __restrict__ skp) {
+    const int kx = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
+    const int t = threadIdx.x;
+    __shared__ double2 A[NJ][KT];   // Y_j(ky),     ky = -15..16 -> column ky+15
+    __shared__ double2 B[NJ][KT];   // q X_j(kx) Z_j(kz)
+    const int ty = t >> 3, tz = t & 7;
+    double2 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) acc[i][l] = make_double2(0.0, 0.0);
+    const double* X = site + (size_t)b * 3 * na;
+    const int a0 = ch * SK_CHUNK, a1 = min(na, a0 + SK_CHUNK);
+    const int kmax = kd.kmax;
+    for (int base = a0; base < a1; base += NJ) {
+        if (t < 2 * NJ) {
+            const int j = t >> 1, part = t & 1, a = base + j;
+            double2* row = part ? &B[j][0] : &A[j][0];
+            for (int c = 0; c < KT; c++) row[c] = make_double2(0.0, 0.0);
+            if (a < a1) {
+                if (part == 0) {
+                    double s, c;
+                    sincos(kd.lat1 * X[na + a], &s, &c);
+                    double2 y = make_double2(1.0, 0.0);
+                    row[KH] = y;
+                    for (int k = 1; k <= kmax; k++) {
+                        y = make_double2(c * y.x - s * y.y, s * y.x + c * y.y);
+                        row[KH + k] = y;
+                        row[KH - k] = make_double2(y.x, -y.y);
+                    }
+                } else {
+                    double s, c;
+                    sincos(kd.lat0 * X[a], &s, &c);
+                    double2 xk = make_double2(q[a], 0.0);
+                    for (int k = 1; k <= kx; k++) xk = make_double2(c * xk.x - s * xk.y, s * xk.x + c * xk.y);
+                    sincos(kd.lat2 * X[2 * na + a], &s, &c);
+                    double2 zz = make_double2(1.0, 0.0);
+                    row[KH] = xk;
+                    for (int k = 1; k <= kmax; k++) {
+                        zz = make_double2(c * zz.x - s * zz.y, s * zz.x + c * zz.y);
+                        row[KH + k] = cmul(xk, zz);
+                        row[KH - k] = cmul(xk, make_double2(zz.x, -zz.y));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int nj = min(NJ, a1 - base);
+        for (int j = 0; j < nj; j++) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = A[j][ty * 4 + i];
+#pragma unroll
+            for (int l = 0; l < 4; l++) bv[l] = B[j][tz * 4 + l];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int l = 0; l < 4; l++) {
+                    acc[i][l].x = fma(av[i].x, bv[l].x, fma(-av[i].y, bv[l].y, acc[i][l].x));
+                    acc[i][l].y = fma(av[i].x, bv[l].y, fma(av[i].y, bv[l].x, acc[i][l].y));
+                }
+        }
+        __syncthreads();
+    }
+    double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) o[(ty * 4 + i) * KT + tz * 4 + l] = acc[i][l];
+}
+
+// one block of 1024 threads per (kx, bead): thread = (iy, iz)
+__global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ skp, int nchunk, KDev kd,
+                                                   const unsigned char* __restrict__ mask, double2* __restrict__ gk,
+                                                   double* __restrict__ parts) {
+    const int kx = blockIdx.x, b = blockIdx.y, kmax = kd.kmax;
+    const int iy = threadIdx.x >> 5, iz = threadIdx.x & 31;
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    const double2* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
+    double2 S = make_double2(0.0, 0.0);
+    for (int ch = 0; ch < nchunk; ch++) {
+        double2 v = src[(size_t)ch * (KT * KT) + threadIdx.x];
+        S.x += v.x;
+        S.y += v.y;
+    }
+    double2 G = make_double2(0.0, 0.0);
+    if (mask[(kx * KT + iy) * KT + iz]) {
+        const int ky = iy - KH, kz = iz - KH;
+        const double rkx = kd.lat0 * kx, rky = kd.lat1 * ky, rkz = kd.lat2 * kz;
+        const double rk2 = rkx * rkx + rky * rky + rkz * rkz;
+        const double f = kx >= 1 ? 2.0 * kd.f0 : kd.f0;
+        const double w = (f / rk2) * exp(-kd.b * rk2);
+        const double et = w * (S.x * S.x + S.y * S.y);
+        const double term = 2.0 * (1.0 / rk2 + kd.b);
+        // The cubic reference sums the (+-ky, +-kz) members of one loop iteration into one et and
+        // multiplies by the positive rky, rkz (src/ewald_cubic.f90:362-371,415-430,488-503), so its
+        // off-diagonal virial uses |ky|, |kz|; the non-cubic sum keeps the signs
+        // (src/ewald_noncubic.f90:426-441).
+        const double rk[3] = {rkx, kd.cubic ? fabs(rky) : rky, kd.cubic ? fabs(rkz) : rkz};
+        acc[0] = et;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) acc[1 + c * 3 + r] = et * (term * rk[r] * rk[c] - (r == c ? 1.0 : 0.0));
+        G = make_double2(2.0 * w * S.x, 2.0 * w * S.y);
+    }
+    gk[(((size_t)b * (kmax + 1) + kx) * KT + iy) * KT + iz] = G;
+    block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
+}
+
+// one thread per (bead, atom); adds the reciprocal-space gradient to gsite
+__global__ __launch_bounds__(256) void k_kforce(const double* __restrict__ site, const double* __restrict__ q, int na,
+                                                KDev kd, const double2* __restrict__ gk, double* __restrict__ gsite) {
+    const int b = blockIdx.y;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const int kmax = kd.kmax;
+    const double* X = site + (size_t)b * 3 * na;
+    double sx, cx, sy, cy, sz, cz;
+    sincos(kd.lat0 * X[a], &sx, &cx);
+    sincos(kd.lat1 * X[na + a], &sy, &cy);
+    sincos(kd.lat2 * X[2 * na + a], &sz, &cz);
+    const double2 Y1 = make_double2(cy, sy), Z1 = make_double2(cz, sz);
+    const double2 w = make_double2(cz, -sz);
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    double2 xk = make_double2(q[a], 0.0);
+    const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
+    for (int kx = 0; kx <= kmax; kx++) {
+        if (kx > 0) xk = make_double2(cx * xk.x - sx * xk.y, sx * xk.x + cx * xk.y);
+        const int KY = kd.kyb[kx], KZ = kd.kzb[kx];
+        if (KY < 0) continue;
+        // Z1^KZ and conj(Y1)^KY
+        double2 zK = make_double2(1.0, 0.0), yc = make_double2(1.0, 0.0);
+        for (int k = 0; k < KZ; k++) zK = cmul(zK, Z1);
+        for (int k = 0; k < KY; k++) yc = cmul(yc, make_double2(Y1.x, -Y1.y));
+        double tx = 0.0, tz = 0.0;   // sum over ky of Im(P U), Im(P Uz)
+        double ty = 0.0;             // sum over ky of ky Im(P U)
+        const double2* Gx = G0 + (size_t)kx * KT * KT;
+        for (int ky = -KY; ky <= KY; ky++) {
+            const double2* Grow = Gx + (ky + KH) * KT;
+            double2 h = Grow[KZ + KH], d = make_double2(0.0, 0.0);
+            for (int kz = KZ - 1; kz >= -KZ; kz--) {
+                const double2 gv = Grow[kz + KH];
+                d = make_double2(fma(d.x, w.x, fma(-d.y, w.y, h.x)), fma(d.x, w.y, fma(d.y, w.x, h.y)));
+                h = make_double2(fma(h.x, w.x, fma(-h.y, w.y, gv.x)), fma(h.x, w.y, fma(h.y, w.x, gv.y)));
+            }
+            // U = zK h ; Uz = zK (w d - KZ h)
+            const double2 U = cmul(zK, h);
+            double2 wd = cmul(w, d);
+            const double2 Uz = cmul(zK, make_double2(wd.x - KZ * h.x, wd.y - KZ * h.y));
+            const double2 xy = cmul(xk, yc);            // q X Y
+            const double2 P = make_double2(xy.x, -xy.y);  // conj
+            const double ipu = P.x * U.y + P.y * U.x;
+            const double ipz = P.x * Uz.y + P.y * Uz.x;
+            tx += ipu;
+            ty += ky * ipu;
+            tz += ipz;
+            yc = cmul(yc, Y1);
+        }
+        gx += kd.lat0 * kx * tx;
+        gy += kd.lat1 * ty;
+        gz += kd.lat2 * tz;
+    }
+    double* Gs = gsite + (size_t)b * 3 * na;
+    Gs[a] += gx;
+    Gs[na + a] += gy;
+    Gs[2 * na + a] += gz;
+}
+
+int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
+
+static KDev make_kdev(const KSet& ks) {
+    KDev kd;
+    kd.kmax = ks.kmax;
+    for (int kx = 0; kx <= KH; kx++) {
+        kd.kyb[kx] = ks.ky_max[kx];
+        kd.kzb[kx] = ks.kz_bound[kx];
+    }
+    kd.lat0 = ks.lat[0];
+    kd.lat1 = ks.lat[1];
+    kd.lat2 = ks.lat[2];
+    kd.f0 = ks.f0;
+    kd.b = ks.b;
+    kd.cubic = ks.cubic;
+    return kd;
+}
+
+void launch_sk(const double* site, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s) {
+    const KDev kd = make_kdev(ks);
+    const int nchunk = kspace_nchunk(na);
+    k_sk<<<dim3(ks.kmax + 1, nchunk, nb), 64, 0, s>>>(site, q, na, nchunk, kd, (double2*)sk_part);
+}
+void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
+                     double* parts_k, hipStream_t s) {
+    const KDev kd = make_kdev(ks);
+    k_sk_final<<<dim3(ks.kmax + 1, nb), 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
+                                                      (double2*)gk, parts_k);
+}
+void launch_kforce(const double* site, const double* q, int na, int nb, const KSet& ks, const double* gk,
+                   double* gsite, hipStream_t s) {
+    const KDev kd = make_kdev(ks);
+    k_kforce<<<dim3((na + 255) / 256, nb), 256, 0, s>>>(site, q, na, kd, (const double2*)gk, gsite);
+}
+
+// ------------------------------------------------------------------------------------------------
+// O-O neighbour lists on centroids (src/neigh_list.f90; gating src/ljones.f90:144-163) and the
+// LJ / Buckingham pair sums (src/ljones.f90, src/buckingham.f90); TEM correction
+// (src/correct_tem.f90).  Lists are full (both directions), capacity RPMD_NCAP per atom.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void mimage(double& dx, double& dy, double& dz, const Box& bx) {
+    dx -= bx.L[0] * rint(dx * bx.inv[0]);
+    dy -= bx.L[1] * rint(dy * bx.inv[1]);
+    dz -= bx.L[2] * rint(dz * bx.inv[2]);
+}
+
+// thread per listed atom i (stride njump); cen is SoA [3][na]
+__global__ void k_nlist(const double* __restrict__ cen, int na, int njump, Box bx, double cut2, int* __restrict__ nnb,
+                        int* __restrict__ nbl, int* __restrict__ overflow) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = na / njump;
+    if (ii >= n) return;
+    const int i = ii * njump;
+    const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
+    int cnt = 0;
+    for (int jj = 0; jj < n; jj++) {
+        const int j = jj * njump;
+        if (j == i) continue;
+        double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
+        mimage(dx, dy, dz, bx);
+        if (dx * dx + dy * dy + dz * dz < cut2) {
+            if (cnt < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + cnt] = j;
+            cnt++;
+        }
+    }
+    if (cnt > RPMD_NCAP) atomicMax(overflow, cnt);
+    if (cnt > 0) atomicAdd(overflow + 1, cnt);
+    nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
+}
+void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
+                  hipStream_t s) {
+    int n = na / njump;
+    k_nlist<<<(n + 255) / 256, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
+}
+
+// LJ (gam == 0) or Buckingham exp-6; thread per (bead, O).  use_list = 0: every other O.
+template <typename T>
+__global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int na, Box bx, const int* __restrict__ nnb,
+                                            const int* __restrict__ nbl, int use_list, double rc2, double sigsq,
+                                            double ba, double bb, double bc, int buck, double vs, double ds,
+                                            double* __restrict__ grad, double* __restrict__ parts) {
+    const int b = blockIdx.y;
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = na / 3;
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    if (ii < n) {
+        const int i = 3 * ii;
+        const double* X = pos + (size_t)b * 3 * na;
+        const double xi = X[i], yi = X[na + i], zi = X[2 * na + i];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const int cnt = use_list ? nnb[ii] : n;
+        for (int k = 0; k < cnt; k++) {
+            int j;
+            if (use_list) j = nbl[(size_t)ii * RPMD_NCAP + k];
+            else {
+                if (k == ii) continue;
+                j = 3 * k;
+            }
+            double dx = xi - X[j], dy = yi - X[na + j], dz = zi - X[2 * na + j];
+            mimage(dx, dy, dz, bx);
+            const double dr2 = dx * dx + dy * dy + dz * dz;
+            if (dr2 < rc2) {
+                double v, fr;   // fr: dV/dr / r (gradient on i = fr * (r_i - r_j))
+                if (!buck) {
+                    const T onr2 = (T)1 / (T)dr2;
+                    const T sr2 = (T)sigsq * onr2;
+                    const T sr6 = sr2 * sr2 * sr2;
+                    v = vs * (double)(sr6 * (sr6 - (T)1));
+                    fr = -ds * (double)(sr6 * (sr6 - (T)0.5) * onr2);
+                } else {
+                    const T tr = sqrt((T)dr2);
+                    const T br6 = (T)bb / ((T)dr2 * (T)dr2 * (T)dr2);
+                    const T ex = (T)ba * exp((T)bc * tr);
+                    v = (double)(ex - br6);
+                    fr = (double)(((T)bc * ex + (T)6 * br6 / tr) / tr);
+                }
+                const double fx = fr * dx, fy = fr * dy, fz = fr * dz;
+                gx += fx;
+                gy += fy;
+                gz += fz;
+                acc[0] += 0.5 * v;
+                acc[1] += 0.5 * dx * fx; acc[2] += 0.5 * dy * fx; acc[3] += 0.5 * dz * fx;
+                acc[4] += 0.5 * dx * fy; acc[5] += 0.5 * dy * fy; acc[6] += 0.5 * dz * fy;
+                acc[7] += 0.5 * dx * fz; acc[8] += 0.5 * dy * fz; acc[9] += 0.5 * dz * fz;
+            }
+        }
+        double* G = grad + (size_t)b * 3 * na;
+        G[i] += gx;
+        G[na + i] += gy;
+        G[2 * na + i] += gz;
+    }
+    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+}
+
+template <typename T>
+void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
+               const rpmd_params& p, double* grad, double* parts, hipStream_t s) {
+    const int buck = p.oo_gam != 0.0;
+    double ba = 0, bb = 0, bc = 0;
+    if (buck) {
+        ba = exp(p.oo_gam) * 6.0 * p.oo_eps / (p.oo_gam - 6.0);
+        bb = p.oo_eps * pow(p.oo_sig, 6) / (1.0 - (6.0 / p.oo_gam));
+        bc = -p.oo_gam / p.oo_sig;
+    }
+    const int n = na / 3;
+    const double vs = buck ? 1.0 : 4.0 * p.oo_eps, ds = buck ? 1.0 : 48.0 * p.oo_eps;
+    k_oo<T><<<dim3((n + 255) / 256, nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
+                                                      p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, grad, parts);
+}
+template void launch_oo<double>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
+                                double*, double*, hipStream_t);
+template void launch_oo<float>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
+                               double*, double*, hipStream_t);
+int oo_blocks(int na) { return (na / 3 + 255) / 256; }
+
+// TEM switched short-range Coulomb between different molecules (src/correct_tem.f90:73-177);
+// site holds physical M-site positions; thread per (bead, atom)
+__global__ __launch_bounds__(256) void k_tem(const double* __restrict__ site, const double* __restrict__ q, int na, Box bx,
+                                             const int* __restrict__ nnb, const int* __restrict__ nbl, int use_list,
+                                             double sig, double* __restrict__ gsite, double* __restrict__ parts) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    if (i < na) {
+        const double* X = site + (size_t)b * 3 * na;
+        const double xi = X[i], yi = X[na + i], zi = X[2 * na + i], qi = q[i];
+        const double rc2 = sig * sig, a1 = 2.0 / sig, a2 = 2.0 / (sig * sig * sig), a3 = 1.0 / (sig * sig * sig * sig);
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const int cnt = use_list ? nnb[i] : na;
+        for (int k = 0; k < cnt; k++) {
+            const int j = use_list ? nbl[(size_t)i * RPMD_NCAP + k] : k;
+            if (j / 3 == i / 3) continue;
+            double dx = xi - X[j], dy = yi - X[na + j], dz = zi - X[2 * na + j];
+            mimage(dx, dy, dz, bx);
+            const double drsq = dx * dx + dy * dy + dz * dz;
+            if (drsq < rc2) {
+                const double dr = sqrt(drsq), onr = 1.0 / dr, zij = qi * q[j];
+                const double vc = zij * onr, dvc = -vc * onr, dr3 = drsq * dr, dr4 = dr3 * dr;
+                const double f = 1.0 - a1 * dr + a2 * dr3 - a3 * dr4;
+                const double df = -a1 + 3.0 * a2 * drsq - 4.0 * a3 * dr3;
+                const double dv = vc * f, dvr = (vc * df + dvc * f) * onr;
+                const double fx = dvr * dx, fy = dvr * dy, fz = dvr * dz;
+                gx += fx;
+                gy += fy;
+                gz += fz;
+                acc[0] += 0.5 * dv;
+                acc[1] += 0.5 * dx * fx; acc[2] += 0.5 * dy * fx; acc[3] += 0.5 * dz * fx;
+                acc[4] += 0.5 * dx * fy; acc[5] += 0.5 * dy * fy; acc[6] += 0.5 * dz * fy;
+                acc[7] += 0.5 * dx * fz; acc[8] += 0.5 * dy * fz; acc[9] += 0.5 * dz * fz;
+            }
+        }
+        double* G = gsite + (size_t)b * 3 * na;
+        G[i] = gx;
+        G[na + i] = gy;
+        G[2 * na + i] = gz;
+    }
+    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+}
+void launch_tem(const double* site, const double* q, int na, int nb, Box box, const int* nnb, const int* nbl,
+                int use_list, double sig, double* gsite, double* parts, hipStream_t s) {
+    k_tem<<<dim3((na + 255) / 256, nb), 256, 0, s>>>(site, q, na, box, nnb, nbl, use_list, sig, gsite, parts);
+}
+int tem_blocks(int na) { return (na + 255) / 256; }
+
+// ------------------------------------------------------------------------------------------------
+// intramolecular terms (src/intra_harmonic.f90, intra_spcfw.f90, intra_morse_harm.f90,
+// intra_morse.f90); thread per (bead, molecule)
+// ------------------------------------------------------------------------------------------------
+struct IntraDev {
+    int model;
+    double apot, bpot, alp, alpb, theta, reoh;
+};
+
+__global__ __launch_bounds__(256) void k_intra(const double* __restrict__ pos, int na, IntraDev p,
+                                               double* __restrict__ grad, double* __restrict__ parts) {
+    const int b = blockIdx.y;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    if (m < na / 3) {
+        const double* X = pos + (size_t)b * 3 * na;
+        const int j = 3 * m;
+        double dx[3][3], dr[3], u[3][3], sq[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            dx[0][c] = X[c * na + j + 1] - X[c * na + j];
+            dx[1][c] = X[c * na + j + 2] - X[c * na + j];
+            dx[2][c] = X[c * na + j + 2] - X[c * na + j + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sq[k] = dx[k][0] * dx[k][0] + dx[k][1] * dx[k][1] + dx[k][2] * dx[k][2];
+            dr[k] = sqrt(sq[k]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[k][c] = dx[k][c] / dr[k];
+        }
+        double v = 0.0, dvb[3];
+        if (p.model == RPMD_INTRA_HARMONIC) {
+            const double ap = 0.5993, bp = 0.1466, cp = -0.09435, dp = 0.04984;
+            const double rehh = 2.0 * sin(0.5 * p.theta) * p.reoh;
+            const double d1 = dr[0] - p.reoh, d2 = dr[1] - p.reoh, d3 = dr[2] - rehh;
+            v = 0.5 * ap * (d1 * d1 + d2 * d2) + 0.5 * bp * d3 * d3 + cp * (d1 + d2) * d3 + dp * d1 * d2;
+            dvb[0] = ap * d1 + cp * d3 + dp * d2;
+            dvb[1] = ap * d2 + cp * d3 + dp * d1;
+            dvb[2] = bp * d3 + cp * (d1 + d2);
+        } else {
+            const double uu = sq[0] + sq[1] - sq[2], vv = 2.0 * dr[0] * dr[1], v2 = 1.0 / (vv * vv);
+            const double arg = uu / vv, darg = -1.0 / sqrt(1.0 - arg * arg), ang = acos(arg);
+            const double dth0 = darg * ((2.0 * dr[0] * vv - 2.0 * dr[1] * uu) * v2);
+            const double dth1 = darg * ((2.0 * dr[1] * vv - 2.0 * dr[0] * uu) * v2);
+            const double dth2 = darg * ((-2.0 * dr[2] * vv) * v2);
+            const double dang = ang - p.theta, d1 = dr[0] - p.reoh, d2 = dr[1] - p.reoh;
+            if (p.model == RPMD_INTRA_SPCFW) {
+                v = 0.5 * p.apot * (d1 * d1 + d2 * d2) + 0.5 * p.bpot * dang * dang;
+                dvb[0] = p.apot * d1 + p.bpot * dang * dth0;
+                dvb[1] = p.apot * d2 + p.bpot * dang * dth1;
+                dvb[2] = p.bpot * dang * dth2;
+            } else {
+                const double de = p.apot, deb = p.bpot, al = p.alp, al2 = al * al, al3 = al * al2, al4 = al3 * al;
+                const double f1 = 7.0 / 12.0, f2 = 7.0 / 3.0, ra = d1 * d1, rb = d2 * d2;
+                v = de * (al2 * ra - al3 * d1 * ra + f1 * al4 * ra * ra) + de * (al2 * rb - al3 * d2 * rb + f1 * al4 * rb * rb);
+                const double a1 = de * (2.0 * al2 * d1 - 3.0 * al3 * ra + f2 * al4 * d1 * ra);
+                const double a2 = de * (2.0 * al2 * d2 - 3.0 * al3 * rb + f2 * al4 * d2 * rb);
+                double a3;
+                if (p.model == RPMD_INTRA_MORSE_HARM) {
+                    v += deb * dang * dang;
+                    a3 = 2.0 * deb * dang;
+                } else {
+                    const double ab = p.alpb, ab2 = ab * ab, ab3 = ab2 * ab, ab4 = ab2 * ab2;
+                    v += deb * (ab2 * dang * dang - ab3 * dang * dang * dang + f1 * ab4 * dang * dang * dang * dang);
+                    a3 = deb * (2.0 * ab2 * dang - 3.0 * ab3 * dang * dang + f2 * ab4 * dang * dang * dang);
+                }
+                dvb[0] = a1 + a3 * dth0;
+                dvb[1] = a2 + a3 * dth1;
+                dvb[2] = a3 * dth2;
+            }
+        }
+        double dv[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) dv[k][c] = dvb[k] * u[k][c];
+        double* G = grad + (size_t)b * 3 * na;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            G[c * na + j] += -dv[0][c] - dv[1][c];
+            G[c * na + j + 1] += dv[0][c] - dv[2][c];
+            G[c * na + j + 2] += dv[1][c] + dv[2][c];
+        }
+        acc[0] = v;
+        if (p.model == RPMD_INTRA_SPCFW) {
+            // W = sum r_i . dvdr_i, diagonal only (src/intra_spcfw.f90:215-220)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                acc[1 + c * 4] = X[c * na + j] * (-dv[0][c] - dv[1][c]) + X[c * na + j + 1] * (dv[0][c] - dv[2][c]) +
+                                 X[c * na + j + 2] * (dv[1][c] + dv[2][c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    acc[1 + c * 3 + a] = dx[0][a] * dv[0][c] + dx[1][a] * dv[1][c] + dx[2][a] * dv[2][c];
+        }
+    }
+    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+}
+
+void launch_intra(const double* pos, int na, int nb, int model, const rpmd_params& p, double* grad, double* parts,
+                  hipStream_t s) {
+    IntraDev d{model, p.apot, p.bpot, p.alp, p.alpb, p.theta, p.reoh};
+    k_intra<<<dim3((na / 3 + 255) / 256, nb), 256, 0, s>>>(pos, na, d, grad, parts);
+}
+int intra_blocks(int na) { return (na / 3 + 255) / 256; }
+
+}  // namespace rpmd
