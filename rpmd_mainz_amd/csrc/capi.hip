@@ -68,6 +68,7 @@ struct rpmd_ctx {
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
+    double *ph = nullptr, *kpart = nullptr;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
     unsigned char* kmask = nullptr;
@@ -168,9 +169,9 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
     double key[8] = {box[0], box[1], box[2], P.walpha, P.rkmax, (double)P.kmax, (double)P.iamcub, 0};
     if (c->ks_valid && !memcmp(key, c->ks_key, sizeof key)) return 0;
     if (P.kmax > RPMD_KMAX_DEV || P.kmax < 0) return RPMD_ERR_KMAX;
-    const int K = P.kmax, T = RPMD_KTILE, H = RPMD_KMAX_DEV;
+    const int K = P.kmax, T = RPMD_KTILE, H = RPMD_KMAX_DEV, O = RPMD_KTILE / 2;
     std::vector<unsigned char> m((size_t)(H + 1) * T * T, 0);
-    auto M = [&](int kx, int ky, int kz) -> unsigned char& { return m[((size_t)kx * T + ky + H) * T + kz + H]; };
+    auto M = [&](int kx, int ky, int kz) -> unsigned char& { return m[((size_t)kx * T + ky + O) * T + kz + O]; };
     const double pi = acos(-1.0);
     KSet ks{};
     ks.kmax = K;
@@ -329,9 +330,10 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
                                           c->gsite, pr, s);
         }
         double* pk = c->parts + c->off_k;
+        launch_phase(c->site, na, nb, c->ks, c->ph, s);
         {
             ProfScope ps(c, 1);
-            launch_sk(c->site, c->q, na, nb, c->ks, c->skp, s);
+            launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, s);
         }
         {
             ProfScope ps(c, 3);
@@ -339,10 +341,10 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         {
             ProfScope ps(c, 2);
-            launch_kforce(c->site, c->q, na, nb, c->ks, c->gk, c->gsite, s);
+            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->kpart, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
-        launch_chain_rule(grad, c->gsite, na, nb, P.alpha, sc * sc, s);
+        launch_chain_rule(grad, c->gsite, c->kpart, kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
         launch_reduce_ev(pr, nb * ewald_real_blocks(na), 10, ev_slot + 0, 10, sc / nb, s);
         launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);
@@ -384,7 +386,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_sites(pos, c->site, na, nb, aeff, 0, bx, s);
         double* pt = c->parts + c->off_tem;
         launch_tem(c->site, c->q, na, nb, bx, c->nnb, c->nbl, tl, P.sig, c->gsite, pt, s);
-        launch_chain_rule(grad, c->gsite, na, nb, aeff, 1.0, s);
+        launch_chain_rule(grad, c->gsite, nullptr, 0, na, nb, aeff, 1.0, s);
         launch_reduce_ev(pt, nb * tem_blocks(na), 10, ev_slot + 40, 10, 1.0 / nb, s);
     }
     if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
@@ -801,6 +803,8 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->ev, 50);
     e |= dalloc(&c->skp, (size_t)nbl * (RPMD_KMAX_DEV + 1) * nck * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
+    e |= dalloc(&c->ph, (size_t)6 * na * nbl);
+    e |= dalloc(&c->kpart, (size_t)kspace_ngroups(RPMD_KMAX_DEV) * 3 * na * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
@@ -851,7 +855,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* bufs[] = {c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+    void* bufs[] = {c->ph, c->kpart, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,

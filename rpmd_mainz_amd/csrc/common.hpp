@@ -12,7 +12,7 @@
 
 #include "../../include/rpmd_mi355x.h"
 
-#define RPMD_KTILE 32   // k-space tile: ky, kz in [-15, 16] -> kmax <= 15
+#define RPMD_KTILE 32   // k-space tile: ky, kz in [-16, 15] (index k + 16) -> kmax <= 15
 #define RPMD_KMAX_DEV 15
 #define RPMD_NCAP 512   // O-O / TEM neighbour capacity per atom (full list)
 
@@ -44,7 +44,8 @@ void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double*
 void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
 void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
-void launch_chain_rule(double* grad, const double* gsite, int na, int nb, double alpha, double scale, hipStream_t s);
+void launch_chain_rule(double* grad, const double* gsite, const double* parts, int npart, int na, int nb, double alpha,
+                       double scale, hipStream_t s);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
@@ -56,11 +57,13 @@ void launch_ewald_real(const double* sorted, const int* perm, const int* start, 
                        CellGrid g, Box box, double rcut, double alpha, double* gsite, double* parts, hipStream_t s);
 int ewald_real_blocks(int na);
 int kspace_nchunk(int na);
-void launch_sk(const double* site, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s);
+int kspace_ngroups(int kmax);
+void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s);
+void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s);
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
                      double* parts_k, hipStream_t s);
-void launch_kforce(const double* site, const double* q, int na, int nb, const KSet& ks, const double* gk,
-                   double* gsite, hipStream_t s);
+void launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
+                   hipStream_t s);
 void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
                   hipStream_t s);
 template <typename T>

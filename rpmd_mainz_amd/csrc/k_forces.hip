@@ -111,9 +111,10 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
     k_sites<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2);
 }
 
-// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215)
-__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs, int na, int nb, double alpha,
-                        double scale) {
+// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient
+// gs plus npart partial gradients (parts[p*pstride + ...]) summed in fixed order
+__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs, const double* __restrict__ parts,
+                        int npart, size_t pstride, int na, int nb, double alpha, double scale) {
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nm * nb) return;
@@ -122,16 +123,23 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         size_t o = (b * 3 + d) * na + 3 * m;
-        double g0 = scale * gs[o], g1 = scale * gs[o + 1], g2 = scale * gs[o + 2];
+        double s0 = gs[o], s1 = gs[o + 1], s2 = gs[o + 2];
+        for (int p = 0; p < npart; p++) {
+            s0 += parts[p * pstride + o];
+            s1 += parts[p * pstride + o + 1];
+            s2 += parts[p * pstride + o + 2];
+        }
+        double g0 = scale * s0, g1 = scale * s1, g2 = scale * s2;
         grad[o] += alpha * g0;
         grad[o + 1] += g1 + alpha2 * g0;
         grad[o + 2] += g2 + alpha2 * g0;
     }
 }
-void launch_chain_rule(double* grad, const double* gsite, int na, int nb, double alpha, double scale,
-                       hipStream_t s) {
+void launch_chain_rule(double* grad, const double* gsite, const double* parts, int npart, int na, int nb, double alpha,
+                       double scale, hipStream_t s) {
     size_t n = (size_t)(na / 3) * nb;
-    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, gsite, na, nb, alpha, scale);
+    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, gsite, parts, npart, (size_t)3 * na * nb, na, nb, alpha,
+                                                       scale);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
@@ -396,23 +404,31 @@ int ewald_real_blocks(int na) { return (na + 255) / 256; }
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
 //
-// Structure factor, per bead and kx: S(kx,ky,kz) = sum_j [q_j X_j(kx) Z_j(kz)] Y_j(ky) is a
-// complex GEMM over atoms of a (ky) x (kz) tile; k_sk stages NJ atoms' Y and qXZ rows in LDS and
-// each of 64 lanes accumulates a 4x4 complex register tile.  Atom chunks give partial sums.
-// k_sk_final sums the chunks, applies the reference weights w = (f/k^2) exp(-b k^2) on the
-// reference k set (mask built on the host with the reference's own comparisons) and stores
-// G = 2 w S.  k_kforce evaluates, per atom, U(ky) = sum_kz G w^kz with w = conj(exp(i lat_z z))
-// by Horner's rule (and its kz-weighted derivative), so the force pass needs no tables.
+// k tile: ky, kz in [-16, 15] (index k + 16), kx in [0, kmax], kmax <= 15.
+// k_phase   : per (bead, atom) the base phases e^{i lat_d s_d}.
+// k_sk_mfma : S(kx,ky,kz) = sum_j Y_j(ky) [q_j X_j(kx) Z_j(kz)], a complex GEMM over atoms on FP64
+//             MFMA (v_mfma_f64_16x16x4f64).  A block owns KXB values of kx and one atom chunk; its
+//             four waves own the four 16x16 (ky,kz) quadrants.  NJ atoms at a time are staged in LDS
+//             as Y and Z tables (shared by the KXB values of kx) and X per kx.
+// k_sk_final: sums the chunks, applies the reference weights w = (f/k^2) exp(-b k^2) on the
+//             reference k set (mask built on the host with the reference's own comparisons), adds
+//             energy and virial, stores G = 2 w S.
+// k_kforce  : per atom, U(ky) = sum_kz G w^kz with w = conj(exp(i lat_z z)) by Horner's rule and its
+//             kz-weighted derivative, two ky rows interleaved; kx split over grid.z into partial
+//             gradients summed (in fixed order) by the chain-rule kernel.
 // ------------------------------------------------------------------------------------------------
 #define KT 32
+#define KO 16
 #define KH 15
-#define NJ 16
-#define SK_CHUNK 512
+#define NJ 32
+#define SK_CHUNK 256
+#define KXB 4
+#define KXF 4
 
 struct KDev {
     int kmax;
     int cubic;
-    int kyb[KH + 1];   // per kx: max |ky| with a member
+    int kyb[KH + 1];   // per kx: max |ky| with a member (-1: none)
     int kzb[KH + 1];   // per kx: max |kz| with a member
     double lat0, lat1, lat2;
     double f0, b;
@@ -421,77 +437,118 @@ struct KDev {
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// z^n, n >= 0, by binary powering
+__device__ __forceinline__ double2 cpow(double2 z, int n) {
+    double2 r = make_double2(1.0, 0.0);
+    while (n > 0) {
+        if (n & 1) r = cmul(r, z);
+        z = cmul(z, z);
+        n >>= 1;
+    }
+    return r;
+}
 
-__global__ __launch_bounds__(64) void k_sk(const double* __restrict__ site, const double* __restrict__ q, int na,
-                                           int nchunk, KDev kd, double2* __restrict__ skp) {
-    const int kx = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
-    const int t = threadIdx.x;
-    __shared__ double2 A[NJ][KT];   // Y_j(ky),     ky = -15..16 -> column ky+15
-    __shared__ double2 B[NJ][KT];   // q X_j(kx) Z_j(kz)
-    const int ty = t >> 3, tz = t & 7;
-    double2 acc[4][4];
+__global__ void k_phase(const double* __restrict__ site, int na, int nb, double l0, double l1, double l2,
+                        double2* __restrict__ ph) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)na * nb) return;
+    const size_t b = i / na, a = i % na;
+    const double* X = site + b * 3 * na;
+    const double lat[3] = {l0, l1, l2};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int l = 0; l < 4; l++) acc[i][l] = make_double2(0.0, 0.0);
-    const double* X = site + (size_t)b * 3 * na;
-    const int a0 = ch * SK_CHUNK, a1 = min(na, a0 + SK_CHUNK);
+    for (int d = 0; d < 3; d++) {
+        double sn, cs;
+        sincos(lat[d] * X[d * na + a], &sn, &cs);
+        ph[(b * 3 + d) * na + a] = make_double2(cs, sn);
+    }
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                                 int nchunk, KDev kd, double2* __restrict__ skp) {
+    const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int qa = w >> 1, qc = w & 1;   // ky half, kz half of this wave
     const int kmax = kd.kmax;
+    __shared__ double Yr[NJ][KT], Yi[NJ][KT], Zr[NJ][KT], Zi[NJ][KT];
+    __shared__ double Xr[KXB][NJ], Xi[KXB][NJ];
+    dbl4 accR[KXB], accI[KXB];
+#pragma unroll
+    for (int g = 0; g < KXB; g++) {
+        accR[g] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        accI[g] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    }
+    const double2* PX = ph + ((size_t)b * 3 + 0) * na;
+    const double2* PY = ph + ((size_t)b * 3 + 1) * na;
+    const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
+    const int a0 = ch * SK_CHUNK, a1 = min(na, a0 + SK_CHUNK);
     for (int base = a0; base < a1; base += NJ) {
-        if (t < 2 * NJ) {
-            const int j = t >> 1, part = t & 1, a = base + j;
-            double2* row = part ? &B[j][0] : &A[j][0];
-            for (int c = 0; c < KT; c++) row[c] = make_double2(0.0, 0.0);
-            if (a < a1) {
-                if (part == 0) {
-                    double s, c;
-                    sincos(kd.lat1 * X[na + a], &s, &c);
-                    double2 y = make_double2(1.0, 0.0);
-                    row[KH] = y;
-                    for (int k = 1; k <= kmax; k++) {
-                        y = make_double2(c * y.x - s * y.y, s * y.x + c * y.y);
-                        row[KH + k] = y;
-                        row[KH - k] = make_double2(y.x, -y.y);
-                    }
-                } else {
-                    double s, c;
-                    sincos(kd.lat0 * X[a], &s, &c);
-                    double2 xk = make_double2(q[a], 0.0);
-                    for (int k = 1; k <= kx; k++) xk = make_double2(c * xk.x - s * xk.y, s * xk.x + c * xk.y);
-                    sincos(kd.lat2 * X[2 * na + a], &s, &c);
-                    double2 zz = make_double2(1.0, 0.0);
-                    row[KH] = xk;
-                    for (int k = 1; k <= kmax; k++) {
-                        zz = make_double2(c * zz.x - s * zz.y, s * zz.x + c * zz.y);
-                        row[KH + k] = cmul(xk, zz);
-                        row[KH - k] = cmul(xk, make_double2(zz.x, -zz.y));
-                    }
+        {
+            // Y and Z rows: thread -> (atom j = t/8, entries 4*(t%8) .. +3)
+            const int j = t >> 3, part = t & 7, a = base + j;
+            const int k0 = 4 * part - KO;
+            double2 y1 = make_double2(1.0, 0.0), z1 = make_double2(1.0, 0.0);
+            const bool in = a < a1;
+            if (in) {
+                y1 = PY[a];
+                z1 = PZ[a];
+            }
+            double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
+            double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int k = k0 + m;
+                const bool ok = in && k >= -kmax && k <= kmax;
+                Yr[j][4 * part + m] = ok ? yk.x : 0.0;
+                Yi[j][4 * part + m] = ok ? yk.y : 0.0;
+                Zr[j][4 * part + m] = ok ? zk.x : 0.0;
+                Zi[j][4 * part + m] = ok ? zk.y : 0.0;
+                yk = cmul(yk, y1);
+                zk = cmul(zk, z1);
+            }
+            if (t < KXB * NJ) {
+                const int g = t / NJ, jj = t % NJ, aa = base + jj, kx = kxg * KXB + g;
+                double2 xv = make_double2(0.0, 0.0);
+                if (aa < a1 && kx <= kmax) {
+                    const double2 x1 = PX[aa];
+                    const double2 p = cpow(x1, kx);
+                    xv = make_double2(q[aa] * p.x, q[aa] * p.y);
                 }
+                Xr[g][jj] = xv.x;
+                Xi[g][jj] = xv.y;
             }
         }
         __syncthreads();
-        const int nj = min(NJ, a1 - base);
-        for (int j = 0; j < nj; j++) {
-            double2 av[4], bv[4];
+#pragma unroll 2
+        for (int st = 0; st < NJ / 4; st++) {
+            const int j = 4 * st + lk;
+            const double ar = Yr[j][16 * qa + li], ai = Yi[j][16 * qa + li];
+            const double zr = Zr[j][16 * qc + li], zi = Zi[j][16 * qc + li];
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = A[j][ty * 4 + i];
-#pragma unroll
-            for (int l = 0; l < 4; l++) bv[l] = B[j][tz * 4 + l];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int l = 0; l < 4; l++) {
-                    acc[i][l].x = fma(av[i].x, bv[l].x, fma(-av[i].y, bv[l].y, acc[i][l].x));
-                    acc[i][l].y = fma(av[i].x, bv[l].y, fma(av[i].y, bv[l].x, acc[i][l].y));
-                }
+            for (int g = 0; g < KXB; g++) {
+                const double xr = Xr[g][j], xi = Xi[g][j];
+                const double br = xr * zr - xi * zi, bi = xr * zi + xi * zr;
+                accR[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accR[g], 0, 0, 0);
+                accR[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, accR[g], 0, 0, 0);
+                accI[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, accI[g], 0, 0, 0);
+                accI[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, accI[g], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
-    double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
+    // D layout of v_mfma_f64_16x16x4f64: lane l, item v -> row lk + 4v, column li
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int g = 0; g < KXB; g++) {
+        const int kx = kxg * KXB + g;
+        if (kx > kmax) break;
+        double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
 #pragma unroll
-        for (int l = 0; l < 4; l++) o[(ty * 4 + i) * KT + tz * 4 + l] = acc[i][l];
+        for (int v = 0; v < 4; v++) {
+            const int iy = 16 * qa + lk + 4 * v, iz = 16 * qc + li;
+            o[iy * KT + iz] = make_double2(accR[g][v], accI[g][v]);
+        }
+    }
 }
 
 // one block of 1024 threads per (kx, bead): thread = (iy, iz)
@@ -512,7 +569,7 @@ __global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ s
     }
     double2 G = make_double2(0.0, 0.0);
     if (mask[(kx * KT + iy) * KT + iz]) {
-        const int ky = iy - KH, kz = iz - KH;
+        const int ky = iy - KO, kz = iz - KO;
         const double rkx = kd.lat0 * kx, rky = kd.lat1 * ky, rkz = kd.lat2 * kz;
         const double rk2 = rkx * rkx + rky * rky + rkz * rkz;
         const double f = kx >= 1 ? 2.0 * kd.f0 : kd.f0;
@@ -535,66 +592,90 @@ __global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ s
     block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
 }
 
-// one thread per (bead, atom); adds the reciprocal-space gradient to gsite
-__global__ __launch_bounds__(256) void k_kforce(const double* __restrict__ site, const double* __restrict__ q, int na,
-                                                KDev kd, const double2* __restrict__ gk, double* __restrict__ gsite) {
-    const int b = blockIdx.y;
+// Horner over kz of one ky row: h = sum_m c_m w^m, d = dh/dw, with c_m = G(kz = m - KZ)
+__device__ __forceinline__ void horner2(const double2* __restrict__ ra, const double2* __restrict__ rb, int KZ,
+                                        double2 w, double2& ha, double2& da, double2& hb, double2& db) {
+    ha = ra[KZ + KO];
+    hb = rb[KZ + KO];
+    da = make_double2(0.0, 0.0);
+    db = make_double2(0.0, 0.0);
+    for (int kz = KZ - 1; kz >= -KZ; kz--) {
+        const double2 ga = ra[kz + KO], gb = rb[kz + KO];
+        da = make_double2(fma(da.x, w.x, fma(-da.y, w.y, ha.x)), fma(da.x, w.y, fma(da.y, w.x, ha.y)));
+        db = make_double2(fma(db.x, w.x, fma(-db.y, w.y, hb.x)), fma(db.x, w.y, fma(db.y, w.x, hb.y)));
+        ha = make_double2(fma(ha.x, w.x, fma(-ha.y, w.y, ga.x)), fma(ha.x, w.y, fma(ha.y, w.x, ga.y)));
+        hb = make_double2(fma(hb.x, w.x, fma(-hb.y, w.y, gb.x)), fma(hb.x, w.y, fma(hb.y, w.x, gb.y)));
+    }
+}
+
+// one thread per (bead, atom, kx group); writes the group's partial gradient (unit-box / physical)
+__global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                                KDev kd, const double2* __restrict__ gk, double* __restrict__ kpart) {
+    const int b = blockIdx.y, g = blockIdx.z, nb = gridDim.y;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= na) return;
     const int kmax = kd.kmax;
-    const double* X = site + (size_t)b * 3 * na;
-    double sx, cx, sy, cy, sz, cz;
-    sincos(kd.lat0 * X[a], &sx, &cx);
-    sincos(kd.lat1 * X[na + a], &sy, &cy);
-    sincos(kd.lat2 * X[2 * na + a], &sz, &cz);
-    const double2 Y1 = make_double2(cy, sy), Z1 = make_double2(cz, sz);
-    const double2 w = make_double2(cz, -sz);
+    const double2 X1 = ph[((size_t)b * 3 + 0) * na + a], Y1 = ph[((size_t)b * 3 + 1) * na + a];
+    const double2 Z1 = ph[((size_t)b * 3 + 2) * na + a];
+    const double2 w = make_double2(Z1.x, -Z1.y), Yc = make_double2(Y1.x, -Y1.y);
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    double2 xk = make_double2(q[a], 0.0);
+    const int kx0 = g * KXF, kx1 = min(kmax, kx0 + KXF - 1);
+    double2 xk = cpow(X1, kx0);
+    xk = make_double2(q[a] * xk.x, q[a] * xk.y);
     const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
-    for (int kx = 0; kx <= kmax; kx++) {
-        if (kx > 0) xk = make_double2(cx * xk.x - sx * xk.y, sx * xk.x + cx * xk.y);
+    for (int kx = kx0; kx <= kx1; kx++) {
+        if (kx > kx0) xk = cmul(xk, X1);
         const int KY = kd.kyb[kx], KZ = kd.kzb[kx];
         if (KY < 0) continue;
-        // Z1^KZ and conj(Y1)^KY
-        double2 zK = make_double2(1.0, 0.0), yc = make_double2(1.0, 0.0);
-        for (int k = 0; k < KZ; k++) zK = cmul(zK, Z1);
-        for (int k = 0; k < KY; k++) yc = cmul(yc, make_double2(Y1.x, -Y1.y));
-        double tx = 0.0, tz = 0.0;   // sum over ky of Im(P U), Im(P Uz)
-        double ty = 0.0;             // sum over ky of ky Im(P U)
+        const double2 zK = cpow(Z1, KZ);
+        double2 yc = cpow(Yc, KY);   // Y(-KY)
+        double tx = 0.0, ty = 0.0, tz = 0.0;
         const double2* Gx = G0 + (size_t)kx * KT * KT;
-        for (int ky = -KY; ky <= KY; ky++) {
-            const double2* Grow = Gx + (ky + KH) * KT;
-            double2 h = Grow[KZ + KH], d = make_double2(0.0, 0.0);
-            for (int kz = KZ - 1; kz >= -KZ; kz--) {
-                const double2 gv = Grow[kz + KH];
-                d = make_double2(fma(d.x, w.x, fma(-d.y, w.y, h.x)), fma(d.x, w.y, fma(d.y, w.x, h.y)));
-                h = make_double2(fma(h.x, w.x, fma(-h.y, w.y, gv.x)), fma(h.x, w.y, fma(h.y, w.x, gv.y)));
+        int ky = -KY;
+        for (; ky + 1 <= KY; ky += 2) {
+            double2 ha, da, hb, db;
+            horner2(Gx + (ky + KO) * KT, Gx + (ky + 1 + KO) * KT, KZ, w, ha, da, hb, db);
+            const double2 yb = cmul(yc, Y1);
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const double2 h = r ? hb : ha, d = r ? db : da, yy = r ? yb : yc;
+                const double2 U = cmul(zK, h);
+                const double2 wd = cmul(w, d);
+                const double2 Uz = cmul(zK, make_double2(wd.x - KZ * h.x, wd.y - KZ * h.y));
+                const double2 xy = cmul(xk, yy);
+                const double ipu = xy.x * U.y - xy.y * U.x;   // Im(conj(xy) U)
+                const double ipz = xy.x * Uz.y - xy.y * Uz.x;
+                tx += ipu;
+                ty += (ky + r) * ipu;
+                tz += ipz;
             }
-            // U = zK h ; Uz = zK (w d - KZ h)
-            const double2 U = cmul(zK, h);
-            double2 wd = cmul(w, d);
-            const double2 Uz = cmul(zK, make_double2(wd.x - KZ * h.x, wd.y - KZ * h.y));
-            const double2 xy = cmul(xk, yc);            // q X Y
-            const double2 P = make_double2(xy.x, -xy.y);  // conj
-            const double ipu = P.x * U.y + P.y * U.x;
-            const double ipz = P.x * Uz.y + P.y * Uz.x;
+            yc = cmul(yb, Y1);
+        }
+        if (ky <= KY) {
+            double2 ha, da, hb, db;
+            horner2(Gx + (ky + KO) * KT, Gx + (ky + KO) * KT, KZ, w, ha, da, hb, db);
+            const double2 U = cmul(zK, ha);
+            const double2 wd = cmul(w, da);
+            const double2 Uz = cmul(zK, make_double2(wd.x - KZ * ha.x, wd.y - KZ * ha.y));
+            const double2 xy = cmul(xk, yc);
+            const double ipu = xy.x * U.y - xy.y * U.x;
+            const double ipz = xy.x * Uz.y - xy.y * Uz.x;
             tx += ipu;
             ty += ky * ipu;
             tz += ipz;
-            yc = cmul(yc, Y1);
         }
         gx += kd.lat0 * kx * tx;
         gy += kd.lat1 * ty;
         gz += kd.lat2 * tz;
     }
-    double* Gs = gsite + (size_t)b * 3 * na;
-    Gs[a] += gx;
-    Gs[na + a] += gy;
-    Gs[2 * na + a] += gz;
+    double* o = kpart + ((size_t)g * nb + b) * 3 * na;
+    o[a] = gx;
+    o[na + a] = gy;
+    o[2 * na + a] = gz;
 }
 
 int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
+int kspace_ngroups(int kmax) { return (kmax + KXF) / KXF; }
 
 static KDev make_kdev(const KSet& ks) {
     KDev kd;
@@ -612,10 +693,15 @@ static KDev make_kdev(const KSet& ks) {
     return kd;
 }
 
-void launch_sk(const double* site, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s) {
+void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s) {
+    const size_t n = (size_t)na * nb;
+    k_phase<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, ks.lat[0], ks.lat[1], ks.lat[2], (double2*)ph);
+}
+void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     const int nchunk = kspace_nchunk(na);
-    k_sk<<<dim3(ks.kmax + 1, nchunk, nb), 64, 0, s>>>(site, q, na, nchunk, kd, (double2*)sk_part);
+    k_sk_mfma<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
+                                                                      (double2*)sk_part);
 }
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
                      double* parts_k, hipStream_t s) {
@@ -623,10 +709,11 @@ void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask,
     k_sk_final<<<dim3(ks.kmax + 1, nb), 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
                                                       (double2*)gk, parts_k);
 }
-void launch_kforce(const double* site, const double* q, int na, int nb, const KSet& ks, const double* gk,
-                   double* gsite, hipStream_t s) {
+void launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
+                   hipStream_t s) {
     const KDev kd = make_kdev(ks);
-    k_kforce<<<dim3((na + 255) / 256, nb), 256, 0, s>>>(site, q, na, kd, (const double2*)gk, gsite);
+    k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
+        (const double2*)ph, q, na, kd, (const double2*)gk, kpart);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -640,33 +727,46 @@ __device__ __forceinline__ void mimage(double& dx, double& dy, double& dz, const
     dz -= bx.L[2] * rint(dz * bx.inv[2]);
 }
 
-// thread per listed atom i (stride njump); cen is SoA [3][na]
-__global__ void k_nlist(const double* __restrict__ cen, int na, int njump, Box bx, double cut2, int* __restrict__ nnb,
-                        int* __restrict__ nbl, int* __restrict__ overflow) {
-    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+// one wave per listed atom i (stride njump); cen is SoA [3][na].  Lanes test 64 candidates at a
+// time; ballot + prefix count keep the list in ascending j, so it is deterministic.
+__global__ __launch_bounds__(256) void k_nlist(const double* __restrict__ cen, int na, int njump, Box bx, double cut2,
+                                               int* __restrict__ nnb, int* __restrict__ nbl, int* __restrict__ overflow) {
     const int n = na / njump;
+    const int ii = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
     if (ii >= n) return;
     const int i = ii * njump;
     const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
     int cnt = 0;
-    for (int jj = 0; jj < n; jj++) {
-        const int j = jj * njump;
-        if (j == i) continue;
-        double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
-        mimage(dx, dy, dz, bx);
-        if (dx * dx + dy * dy + dz * dz < cut2) {
-            if (cnt < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + cnt] = j;
-            cnt++;
+    for (int base = 0; base < n; base += 64) {
+        const int jj = base + lane;
+        bool in = false;
+        int j = 0;
+        if (jj < n) {
+            j = jj * njump;
+            if (j != i) {
+                double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
+                mimage(dx, dy, dz, bx);
+                in = dx * dx + dy * dy + dz * dz < cut2;
+            }
         }
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (slot < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + slot] = j;
+        }
+        cnt += __popcll(m);
     }
-    if (cnt > RPMD_NCAP) atomicMax(overflow, cnt);
-    if (cnt > 0) atomicAdd(overflow + 1, cnt);
-    nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
+    if (lane == 0) {
+        if (cnt > RPMD_NCAP) atomicMax(overflow, cnt);
+        if (cnt > 0) atomicAdd(overflow + 1, cnt);
+        nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
+    }
 }
 void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
                   hipStream_t s) {
-    int n = na / njump;
-    k_nlist<<<(n + 255) / 256, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
+    const int n = na / njump;
+    k_nlist<<<(n + 3) / 4, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
 }
 
 // LJ (gam == 0) or Buckingham exp-6; thread per (bead, O).  use_list = 0: every other O.

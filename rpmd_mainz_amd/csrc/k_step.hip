@@ -4,6 +4,8 @@
 // All arrays are SoA per bead: x[(k*3 + d)*na + a].  Element e = d*na + a of bead/mode k sits at
 // x[k*3*na + e], so the bead transform is out[k][e] = sum_j M[k][j] in[j][e]: a dense nb x nb
 // matrix applied to 3*na columns -- HBM bound (2*nb*8 bytes per column per call).
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace rpmd {
@@ -39,13 +41,58 @@ __global__ __launch_bounds__(256) void k_nm_gen(const double* __restrict__ in, d
     }
 }
 
+// FP64 MFMA form (v_mfma_f64_16x16x4f64): D[i][jj] = sum_kk A[i][kk] B[kk][jj] with
+// A = M (16 output modes x 4 input beads), B = in (4 input beads x 16 columns).  Lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; it receives D[(l/16) + 4v][l%16], v = 0..3.  One wave owns a
+// 16-column tile; all NT*16 output rows of the tile accumulate over ceil(n/4) bead steps.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+template <int NT>
+__global__ __launch_bounds__(256) void k_nm_mfma(const double* __restrict__ in, double* __restrict__ out,
+                                                 const double* __restrict__ M, int n, size_t ne) {
+    const int lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const size_t ntile = (ne + 15) / 16;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t nwave = ((size_t)gridDim.x * blockDim.x) >> 6;
+    const int nstep = (n + 3) / 4;
+    for (size_t t = wave; t < ntile; t += nwave) {
+        const size_t e = t * 16 + li;
+        const bool ein = e < ne;
+        dbl4 acc[NT];
+#pragma unroll
+        for (int q = 0; q < NT; q++) acc[q] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        for (int st = 0; st < nstep; st++) {
+            const int j = st * 4 + lk;
+            const double b = (ein && j < n) ? in[(size_t)j * ne + e] : 0.0;
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+                const int k = q * 16 + li;
+                const double a = (k < n && j < n) ? M[(size_t)k * n + j] : 0.0;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+        }
+        if (ein) {
+#pragma unroll
+            for (int q = 0; q < NT; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = q * 16 + lk + 4 * v;
+                    if (k < n) out[(size_t)k * ne + e] = acc[q][v];
+                }
+        }
+    }
+}
+
 // out may not alias in
 void launch_nm(const double* in, double* out, const double* M, int n, size_t ne, hipStream_t s) {
     const unsigned g = (unsigned)((ne + 255) / 256);
+    const size_t ntile = (ne + 15) / 16;
+    const unsigned gm = (unsigned)std::min<size_t>((ntile + 3) / 4, 2048);  // 4 waves per block
     if (n <= 8) k_nm_reg<8><<<g, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 16) k_nm_reg<16><<<g, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 32) k_nm_reg<32><<<g, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 64) k_nm_reg<64><<<g, 256, 0, s>>>(in, out, M, n, ne);
+    else if (n <= 16) k_nm_mfma<1><<<gm, 256, 0, s>>>(in, out, M, n, ne);
+    else if (n <= 32) k_nm_mfma<2><<<gm, 256, 0, s>>>(in, out, M, n, ne);
+    else if (n <= 48) k_nm_mfma<3><<<gm, 256, 0, s>>>(in, out, M, n, ne);
+    else if (n <= 64) k_nm_mfma<4><<<gm, 256, 0, s>>>(in, out, M, n, ne);
     else k_nm_gen<<<g, 256, 0, s>>>(in, out, M, n, ne);
 }
 
