@@ -68,7 +68,8 @@ struct rpmd_ctx {
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
-    double *ph = nullptr, *kpart = nullptr;
+    double *ph = nullptr, *fpart = nullptr, *bb = nullptr;
+    int *ccnt = nullptr, *clist = nullptr;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
     unsigned char* kmask = nullptr;
@@ -237,15 +238,11 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
     return 0;
 }
 
-CellGrid make_grid(const Box& sb, double rc) {
+// Morton cell grid of the cluster sort: about 4 sites per cell, cells near-cubic in space
+CellGrid make_grid(const Box& sb, int na) {
     CellGrid g;
-    for (int d = 0; d < 3; d++) {
-        int n = (int)floor(3.0 * sb.L[d] / rc);
-        n = std::max(1, std::min(n, 64));
-        g.nc[d] = n;
-        g.reach[d] = 3;
-        if (n * rc / 3.0 > sb.L[d]) g.reach[d] = n;  // never
-    }
+    const double h = cbrt(sb.L[0] * sb.L[1] * sb.L[2] * 4.0 / std::max(na, 1));
+    for (int d = 0; d < 3; d++) g.nc[d] = std::max(1, std::min(64, (int)lround(sb.L[d] / h)));
     g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
     return g;
 }
@@ -316,18 +313,18 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         const int cubic = P.iamcub;
         const Box sb = cubic ? unit_box() : bx;
         launch_sites(pos, c->site, na, nb, P.alpha, cubic, bx, s);
-        CellGrid g = make_grid(sb, P.wrcut);
-        if (g.ncell > c->ncell_cap) return RPMD_ERR_ARG;
-        launch_cell_build(c->site, na, nb, g, sb, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
+        CellGrid g = make_grid(sb, na);
+        launch_cell_build(c->site, na, nb, g, sb, 1, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
+        launch_clusters(c->sorted, na, nb, sb, P.wrcut, c->bb, c->ccnt, c->clist, s);
         double* pr = c->parts + c->off_real;
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(c->sorted, c->perm, c->start, c->q, na, nb, g, sb, P.wrcut, P.walpha,
-                                         c->gsite, pr, s);
+                launch_ewald_real<float>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
+                                         c->fpart, pr, s);
             else
-                launch_ewald_real<double>(c->sorted, c->perm, c->start, c->q, na, nb, g, sb, P.wrcut, P.walpha,
-                                          c->gsite, pr, s);
+                launch_ewald_real<double>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
+                                          c->fpart, pr, s);
         }
         double* pk = c->parts + c->off_k;
         launch_phase(c->site, na, nb, c->ks, c->ph, s);
@@ -341,12 +338,13 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         {
             ProfScope ps(c, 2);
-            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->kpart, s);
+            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->fpart + (size_t)ewald_rsplit() * 3 * na * nb, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
-        launch_chain_rule(grad, c->gsite, c->kpart, kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
+        // real-space split slots then k-space kx-group slots, summed in that order
+        launch_chain_rule(grad, c->fpart, ewald_rsplit() + kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
-        launch_reduce_ev(pr, nb * ewald_real_blocks(na), 10, ev_slot + 0, 10, sc / nb, s);
+        launch_reduce_ev(pr, nb * ewald_real_parts(na), 10, ev_slot + 0, 10, sc / nb, s);
         launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);
     }
     if (need_oo) {
@@ -386,7 +384,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_sites(pos, c->site, na, nb, aeff, 0, bx, s);
         double* pt = c->parts + c->off_tem;
         launch_tem(c->site, c->q, na, nb, bx, c->nnb, c->nbl, tl, P.sig, c->gsite, pt, s);
-        launch_chain_rule(grad, c->gsite, nullptr, 0, na, nb, aeff, 1.0, s);
+        launch_chain_rule(grad, c->gsite, 1, na, nb, aeff, 1.0, s);
         launch_reduce_ev(pt, nb * tem_blocks(na), 10, ev_slot + 40, 10, 1.0 / nb, s);
     }
     if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
@@ -784,7 +782,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     const int nck = kspace_nchunk(na);
     c->ncell_cap = 64 * 64 * 64;
     c->off_real = 0;
-    c->off_k = c->off_real + (size_t)nbl * ewald_real_blocks(na) * 10;
+    c->off_k = c->off_real + (size_t)nbl * ewald_real_parts(na) * 10;
     c->off_oo = c->off_k + (size_t)nbl * (RPMD_KMAX_DEV + 1) * 10;
     c->off_tem = c->off_oo + (size_t)nbl * oo_blocks(na) * 10;
     c->off_intra = c->off_tem + (size_t)nbl * tem_blocks(na) * 10;
@@ -804,7 +802,11 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->skp, (size_t)nbl * (RPMD_KMAX_DEV + 1) * nck * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->ph, (size_t)6 * na * nbl);
-    e |= dalloc(&c->kpart, (size_t)kspace_ngroups(RPMD_KMAX_DEV) * 3 * na * nbl);
+    e |= dalloc(&c->fpart, (size_t)(ewald_rsplit() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
+    const int nclu = ewald_nclu(na);
+    e |= dalloc(&c->bb, (size_t)6 * nclu * nbl);
+    e |= dalloc(&c->ccnt, (size_t)nclu * nbl);
+    e |= dalloc(&c->clist, (size_t)nclu * nclu * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
@@ -855,7 +857,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* bufs[] = {c->ph, c->kpart, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+    void* bufs[] = {c->ph, c->fpart, c->bb, c->ccnt, c->clist, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,

@@ -23,9 +23,8 @@ struct Box {
 };
 
 struct CellGrid {
-    int nc[3];     // cells per box side
-    int reach[3];  // stencil half-width in cells (2*reach+1 > nc: the whole axis once)
-    int ncell;     // nc[0]*nc[1]*nc[2]
+    int nc[3];  // cells per box side (at most 1024, the Morton code's 10 bits per axis)
+    int ncell;  // nc[0]*nc[1]*nc[2]
 };
 
 struct KSet {
@@ -44,18 +43,22 @@ void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double*
 void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
 void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
-void launch_chain_rule(double* grad, const double* gsite, const double* parts, int npart, int na, int nb, double alpha,
-                       double scale, hipStream_t s);
+void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
+                       hipStream_t s);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
 void launch_zero_int(int* y, size_t n, hipStream_t s);
-void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int* cell_of, int* count,
-                       int* start, int* perm, double* sorted, hipStream_t s);
+void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
+                       int* count, int* start, int* perm, double* sorted, hipStream_t s);
+int ewald_nclu(int na);
+int ewald_rsplit();
+int ewald_real_parts(int na);
+void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
+                     hipStream_t s);
 template <typename T>
-void launch_ewald_real(const double* sorted, const int* perm, const int* start, const double* q, int na, int nb,
-                       CellGrid g, Box box, double rcut, double alpha, double* gsite, double* parts, hipStream_t s);
-int ewald_real_blocks(int na);
+void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
+                       double alpha, const int* ccnt, const int* clist, double* rpart, double* parts, hipStream_t s);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
 void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s);

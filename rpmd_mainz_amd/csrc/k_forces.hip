@@ -111,10 +111,10 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
     k_sites<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2);
 }
 
-// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient
-// gs plus npart partial gradients (parts[p*pstride + ...]) summed in fixed order
-__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs, const double* __restrict__ parts,
-                        int npart, size_t pstride, int na, int nb, double alpha, double scale) {
+// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient,
+// the sum of npart partial site gradients (parts[p*pstride + ...]) taken in fixed order
+__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart, size_t pstride, int na,
+                        int nb, double alpha, double scale) {
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nm * nb) return;
@@ -123,8 +123,8 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         size_t o = (b * 3 + d) * na + 3 * m;
-        double s0 = gs[o], s1 = gs[o + 1], s2 = gs[o + 2];
-        for (int p = 0; p < npart; p++) {
+        double s0 = parts[o], s1 = parts[o + 1], s2 = parts[o + 2];
+        for (int p = 1; p < npart; p++) {
             s0 += parts[p * pstride + o];
             s1 += parts[p * pstride + o + 1];
             s2 += parts[p * pstride + o + 2];
@@ -135,11 +135,10 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ gs
         grad[o + 2] += g2 + alpha2 * g0;
     }
 }
-void launch_chain_rule(double* grad, const double* gsite, const double* parts, int npart, int na, int nb, double alpha,
-                       double scale, hipStream_t s) {
+void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
+                       hipStream_t s) {
     size_t n = (size_t)(na / 3) * nb;
-    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, gsite, parts, npart, (size_t)3 * na * nb, na, nb, alpha,
-                                                       scale);
+    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
@@ -179,8 +178,19 @@ void launch_zero_int(int* y, size_t n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// cell lists for the real-space sum (per bead; counting sort, each cell then ordered by atom index)
+// cell sort of the Ewald sites (per bead; counting sort, each cell then ordered by atom index).  With
+// morton = 1 the cells are numbered in Morton (Z-curve) order over a power-of-two grid, so that
+// consecutive sorted sites stay close in space.
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t part1by2(uint32_t x) {
+    x &= 0x3ff;
+    x = (x | (x << 16)) & 0x030000FF;
+    x = (x | (x << 8)) & 0x0300F00F;
+    x = (x | (x << 4)) & 0x030C30C3;
+    x = (x | (x << 2)) & 0x09249249;
+    return x;
+}
+
 __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
     double f = x * inv;
     f = f - floor(f);
@@ -188,8 +198,8 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
     return c < nc ? (c >= 0 ? c : 0) : nc - 1;
 }
 
-__global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, double i0, double i1,
-                              double i2, int* __restrict__ cell_of, int* __restrict__ count) {
+__global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, int morton, int nbins,
+                              double i0, double i1, double i2, int* __restrict__ cell_of, int* __restrict__ count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)na * nb) return;
     size_t b = i / na, a = i % na;
@@ -197,9 +207,9 @@ __global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, C
     int cx = cell_coord(s[a], i0, g.nc[0]);
     int cy = cell_coord(s[na + a], i1, g.nc[1]);
     int cz = cell_coord(s[2 * na + a], i2, g.nc[2]);
-    int cid = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    int cid = morton ? (int)(part1by2(cx) | (part1by2(cy) << 1) | (part1by2(cz) << 2)) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
     cell_of[i] = cid;
-    atomicAdd(&count[b * (g.ncell + 1) + cid], 1);
+    atomicAdd(&count[b * (nbins + 1) + cid], 1);
 }
 
 // exclusive scan per bead (one block per bead); resets count to 0 for the scatter pass
@@ -269,22 +279,35 @@ __global__ void k_cell_order(const double* __restrict__ site, int na, int nb, in
     }
 }
 
-void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int* cell_of, int* count,
-                       int* start, int* perm, double* sorted, hipStream_t s) {
+void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
+                       int* count, int* start, int* perm, double* sorted, hipStream_t s) {
     size_t n = (size_t)na * nb;
-    launch_zero_int(count, (size_t)(g.ncell + 1) * nb, s);
-    k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, unitbox.inv[0], unitbox.inv[1],
-                                                              unitbox.inv[2], cell_of, count);
-    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, g.ncell);
-    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, g.ncell, start, count, perm);
-    size_t nc = (size_t)g.ncell * nb;
-    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, g.ncell, start, perm, sorted);
+    int nbins = g.ncell;
+    if (morton) {
+        int p = 1;
+        while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
+        nbins = p * p * p;
+    }
+    launch_zero_int(count, (size_t)(nbins + 1) * nb, s);
+    k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, morton, nbins, unitbox.inv[0],
+                                                              unitbox.inv[1], unitbox.inv[2], cell_of, count);
+    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, nbins);
+    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, nbins, start, count, perm);
+    size_t nc = (size_t)nbins * nb;
+    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted);
 }
 
 // ------------------------------------------------------------------------------------------------
 // real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
-// one thread per (bead, sorted atom); sums every neighbour (each pair seen from both ends, so the
-// energy and virial carry a factor 1/2); T is the arithmetic type of the pair body.
+//
+// Cluster-pair scheme.  The sites of a bead are sorted by cell in Morton order (launch_cell_build
+// with morton = 1), so each run of 64 sorted sites -- a cluster -- is spatially compact.  A cluster
+// pair (I, J) is kept when the minimum-image distance between the clusters' bounding boxes is below
+// the cut-off.  One wave owns cluster I (lane = site i) and every RSPLIT-th entry of I's J list: per
+// J it stages J's sites in LDS, tests the 64 distances of each lane in FP32 with a relative margin
+// (one bit per j), then evaluates only the flagged pairs, re-testing the cut-off exactly in FP64.
+// Every pair is seen from both ends, so energy and virial carry a factor 1/2; each (split, bead)
+// writes its own gradient slot, summed in fixed order by k_chain.  T is the pair-body type.
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& du, T& dur) {
@@ -310,96 +333,207 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
     dur = (sfac * e - du) * (rinv * rinv);
 }
 
+#define CLU 64
+#define RSPLIT 4
+
+__device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
+
+// bounding box of each cluster in wrapped fractional coordinates: bb[(b*nclu + c)*6] = {centre, half}
+__global__ __launch_bounds__(64) void k_clu_bbox(const double* __restrict__ sorted, int na, int nclu, double i0,
+                                                 double i1, double i2, double* __restrict__ bb) {
+    const int c = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+    const int a = c * CLU + l;
+    const double* X = sorted + (size_t)b * 3 * na;
+    const double inv[3] = {i0, i1, i2};
+    double lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double f = a < na ? wrap01(X[d * na + a] * inv[d]) : 0.0;
+        lo[d] = a < na ? f : 2.0;
+        hi[d] = a < na ? f : -1.0;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+        }
+    if (l == 0) {
+        double* o = bb + ((size_t)b * nclu + c) * 6;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            o[d] = 0.5 * (lo[d] + hi[d]);
+            o[3 + d] = 0.5 * (hi[d] - lo[d]);
+        }
+    }
+}
+
+// cluster-pair list of (I, bead), one wave each: J ascending, kept when the bounding boxes come
+// closer than rc (minimum image of the centres; L = box side, 1 for the unit box)
+__global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ bb, int nclu, double L0, double L1,
+                                                  double L2, double rc2, int* __restrict__ cnt,
+                                                  int* __restrict__ list) {
+    const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+    const double* bi = bb + ((size_t)b * nclu + I) * 6;
+    const double L[3] = {L0, L1, L2};
+    int* out = list + ((size_t)b * nclu + I) * nclu;
+    int n = 0;
+    for (int J0 = 0; J0 < nclu; J0 += 64) {
+        const int J = J0 + l;
+        bool keep = false;
+        if (J < nclu) {
+            const double* bj = bb + ((size_t)b * nclu + J) * 6;
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double dc = bi[d] - bj[d];
+                dc -= rint(dc);
+                const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
+                r2 += gap * gap;
+            }
+            keep = r2 < rc2;
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
+        n += __popcll(m);
+    }
+    if (l == 0) cnt[b * nclu + I] = n;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_real(const double* __restrict__ sorted, const int* __restrict__ perm,
-                                                    const int* __restrict__ start, const double* __restrict__ q, int na,
-                                                    CellGrid g, double L0, double L1, double L2, double i0, double i1,
-                                                    double i2, double rc2, double alpha, double sfac,
-                                                    double* __restrict__ gsite, double* __restrict__ parts) {
-    const int b = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sorted, const int* __restrict__ perm,
+                                                  const double* __restrict__ q, int na, int nclu,
+                                                  const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                  double L0, double L1, double L2, double i0, double i1, double i2,
+                                                  double rc2, float rc2f, double alpha, double sfac,
+                                                  double* __restrict__ rpart, double* __restrict__ parts) {
+    const int I = blockIdx.x, sp = blockIdx.y, b = blockIdx.z, nb = gridDim.z;
+    const int l = threadIdx.x;
+    const int i = I * CLU + l;
+    const bool iv = i < na;
     const double* X = sorted + (size_t)b * 3 * na;
     const int* P = perm + (size_t)b * na;
-    const int* S = start + (size_t)b * (g.ncell + 1);
+    __shared__ double sx[CLU], sy[CLU], sz[CLU], sq[CLU];
+    __shared__ float fx[CLU], fy[CLU], fz[CLU];
+    __shared__ int sm[CLU];
+    double xi = 0, yi = 0, zi = 0, qi = 0;
+    int ai = 0, mi = -1;
+    float fxi = 0, fyi = 0, fzi = 0;
+    if (iv) {
+        xi = X[i];
+        yi = X[na + i];
+        zi = X[2 * na + i];
+        ai = P[i];
+        qi = q[ai];
+        mi = ai / 3;
+        fxi = (float)wrap01(xi * i0);
+        fyi = (float)wrap01(yi * i1);
+        fzi = (float)wrap01(zi * i2);
+    }
+    const float Lf0 = (float)L0, Lf1 = (float)L1, Lf2 = (float)L2;
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    if (i < na) {
-        const double xi = X[i], yi = X[na + i], zi = X[2 * na + i];
-        const int ai = P[i];
-        const double qi = q[ai];
-        const int mi = ai / 3;
-        const int cx = cell_coord(xi, i0, g.nc[0]), cy = cell_coord(yi, i1, g.nc[1]), cz = cell_coord(zi, i2, g.nc[2]);
-        int lo[3], hi[3];
-        const int cc[3] = {cx, cy, cz};
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            if (2 * g.reach[d] + 1 <= g.nc[d]) {
-                lo[d] = cc[d] - g.reach[d];
-                hi[d] = cc[d] + g.reach[d];
-            } else {
-                lo[d] = 0;
-                hi[d] = g.nc[d] - 1;
-            }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const T ta = (T)alpha, tsf = (T)sfac;
+    const int nJ = ccnt[b * nclu + I];
+    const int* lst = clist + ((size_t)b * nclu + I) * nclu;
+    for (int n = sp; n < nJ; n += RSPLIT) {
+        const int J = lst[n];
+        const int j = J * CLU + l;
+        __syncthreads();
+        if (j < na) {
+            const double xj = X[j], yj = X[na + j], zj = X[2 * na + j];
+            sx[l] = xj;
+            sy[l] = yj;
+            sz[l] = zj;
+            fx[l] = (float)wrap01(xj * i0);
+            fy[l] = (float)wrap01(yj * i1);
+            fz[l] = (float)wrap01(zj * i2);
+            const int aj = P[j];
+            sm[l] = aj / 3;
+            sq[l] = q[aj];
         }
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-        const T ta = (T)alpha, tsf = (T)sfac;
-        for (int oz = lo[2]; oz <= hi[2]; oz++) {
-            int z = oz < 0 ? oz + g.nc[2] : (oz >= g.nc[2] ? oz - g.nc[2] : oz);
-            for (int oy = lo[1]; oy <= hi[1]; oy++) {
-                int y = oy < 0 ? oy + g.nc[1] : (oy >= g.nc[1] ? oy - g.nc[1] : oy);
-                const int rowbase = (z * g.nc[1] + y) * g.nc[0];
-                for (int ox = lo[0]; ox <= hi[0]; ox++) {
-                    int x = ox < 0 ? ox + g.nc[0] : (ox >= g.nc[0] ? ox - g.nc[0] : ox);
-                    const int c = rowbase + x;
-                    const int j0 = S[c], j1 = S[c + 1];
-                    for (int j = j0; j < j1; j++) {
-                        if (j == i) continue;
-                        double dx = xi - X[j], dy = yi - X[na + j], dz = zi - X[2 * na + j];
-                        dx -= L0 * rint(dx * i0);
-                        dy -= L1 * rint(dy * i1);
-                        dz -= L2 * rint(dz * i2);
-                        const double dr2 = dx * dx + dy * dy + dz * dz;
-                        if (dr2 < rc2) {
-                            const int aj = P[j];
-                            T du, dur;
-                            erfc_pair<T>((T)dr2, ta, tsf, (aj / 3) == mi, du, dur);
-                            const double qij = qi * q[aj];
-                            const double dv = qij * (double)du, dvr = qij * (double)dur;
-                            const double fx = dvr * dx, fy = dvr * dy, fz = dvr * dz;
-                            gx += fx;
-                            gy += fy;
-                            gz += fz;
-                            acc[0] += 0.5 * dv;
-                            acc[1] += 0.5 * dx * fx; acc[2] += 0.5 * dy * fx; acc[3] += 0.5 * dz * fx;
-                            acc[4] += 0.5 * dx * fy; acc[5] += 0.5 * dy * fy; acc[6] += 0.5 * dz * fy;
-                            acc[7] += 0.5 * dx * fz; acc[8] += 0.5 * dy * fz; acc[9] += 0.5 * dz * fz;
-                        }
-                    }
-                }
+        __syncthreads();
+        uint64_t mask = 0;
+        const int jmax = min(CLU, na - J * CLU);
+        if (iv) {
+            for (int jj = 0; jj < jmax; jj++) {
+                float dx = fxi - fx[jj], dy = fyi - fy[jj], dz = fzi - fz[jj];
+                dx = (dx - rintf(dx)) * Lf0;
+                dy = (dy - rintf(dy)) * Lf1;
+                dz = (dz - rintf(dz)) * Lf2;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < rc2f) mask |= (1ull << jj);
             }
+            if (J == I) mask &= ~(1ull << l);
         }
-        double* G = gsite + (size_t)b * 3 * na;
+        while (mask) {
+            const int jj = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+            dx -= L0 * rint(dx * i0);
+            dy -= L1 * rint(dy * i1);
+            dz -= L2 * rint(dz * i2);
+            const double dr2 = dx * dx + dy * dy + dz * dz;
+            if (!(dr2 < rc2)) continue;
+            T du, dur;
+            erfc_pair<T>((T)dr2, ta, tsf, sm[jj] == mi, du, dur);
+            const double qij = qi * sq[jj];
+            const double dv = qij * (double)du, dvr = qij * (double)dur;
+            const double fxx = dvr * dx, fyy = dvr * dy, fzz = dvr * dz;
+            gx += fxx;
+            gy += fyy;
+            gz += fzz;
+            acc[0] += 0.5 * dv;
+            acc[1] += 0.5 * dx * fxx; acc[2] += 0.5 * dy * fxx; acc[3] += 0.5 * dz * fxx;
+            acc[4] += 0.5 * dx * fyy; acc[5] += 0.5 * dy * fyy; acc[6] += 0.5 * dz * fyy;
+            acc[7] += 0.5 * dx * fzz; acc[8] += 0.5 * dy * fzz; acc[9] += 0.5 * dz * fzz;
+        }
+    }
+    if (iv) {
+        double* G = rpart + ((size_t)sp * nb + b) * 3 * na;
         G[ai] = gx;
         G[na + ai] = gy;
         G[2 * na + ai] = gz;
     }
-    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+    block_reduce_store<10>(acc, parts + (((size_t)b * RSPLIT + sp) * nclu + I) * 10);
+}
+
+int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
+int ewald_rsplit() { return RSPLIT; }
+int ewald_real_parts(int na) { return RSPLIT * ewald_nclu(na); }
+
+// Pair-test margins.  FP32 wrapped fractional coordinates carry an absolute error of about
+// 2^-24 per component, so |dr| errs by less than 1e-6 of the box side.
+// The margin below grows as rc^2 (1e-4 + 1e-5 L_max/rc) so short cut-offs stay covered too.
+static double pair_margin(const Box& sb, double rc) {
+    const double lmax = fmax(sb.L[0], fmax(sb.L[1], sb.L[2]));
+    return 1e-4 + 1e-5 * lmax / rc;
+}
+
+void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
+                     hipStream_t s) {
+    const int nclu = ewald_nclu(na);
+    k_clu_bbox<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.inv[0], sb.inv[1], sb.inv[2], bb);
+    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2], rc * rc * (1.0 + pair_margin(sb, rc)),
+                                             ccnt, clist);
 }
 
 template <typename T>
-void launch_ewald_real(const double* sorted, const int* perm, const int* start, const double* q, int na, int nb,
-                       CellGrid g, Box box, double rcut, double alpha, double* gsite, double* parts, hipStream_t s) {
-    double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
-    dim3 grid((na + 255) / 256, nb);
-    k_ewald_real<T><<<grid, 256, 0, s>>>(sorted, perm, start, q, na, g, box.L[0], box.L[1], box.L[2], box.inv[0],
-                                         box.inv[1], box.inv[2], rcut * rcut, alpha, sfac, gsite, parts);
+void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
+                       double alpha, const int* ccnt, const int* clist, double* rpart, double* parts, hipStream_t s) {
+    const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
+    const int nclu = ewald_nclu(na);
+    const float rc2f = (float)(rcut * rcut * (1.0 + pair_margin(sb, rcut)));
+    k_ewald_clu<T><<<dim3(nclu, RSPLIT, nb), 64, 0, s>>>(sorted, perm, q, na, nclu, ccnt, clist, sb.L[0], sb.L[1],
+                                                        sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, rc2f,
+                                                        alpha, sfac, rpart, parts);
 }
-template void launch_ewald_real<double>(const double*, const int*, const int*, const double*, int, int, CellGrid, Box,
-                                        double, double, double*, double*, hipStream_t);
-template void launch_ewald_real<float>(const double*, const int*, const int*, const double*, int, int, CellGrid, Box,
-                                       double, double, double*, double*, hipStream_t);
-int ewald_real_blocks(int na) { return (na + 255) / 256; }
+template void launch_ewald_real<double>(const double*, const int*, const double*, int, int, Box, double, double,
+                                        const int*, const int*, double*, double*, hipStream_t);
+template void launch_ewald_real<float>(const double*, const int*, const double*, int, int, Box, double, double,
+                                       const int*, const int*, double*, double*, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
