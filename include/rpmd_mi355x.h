@@ -39,6 +39,7 @@ extern "C" {
 #define RPMD_ERR_NOMEM            14  /* device allocation failed                              */
 #define RPMD_ERR_THERM            15  /* thermostat not supported by this entry point          */
 #define RPMD_ERR_NOISE            16  /* host noise buffer too short for the draws needed      */
+#define RPMD_ERR_GLE              17  /* therm GLE without rpmd_gle_init for this (na, nb)     */
 
 /* ---- precision modes ------------------------------------------------------------------ */
 #define RPMD_PREC_FP64   0  /* everything FP64 (the "plain FP64 GPU form")                   */
@@ -131,6 +132,16 @@ int rpmd_evolve_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* vew, 
                    double* vint, double* dvdr, double* dvdr2, double dt, const double* mass,
                    int na, int nb, const double boxlxyz[3], const double* z, double beta,
                    double* vir, double* vir_lf, const double* noise, long long n_noise);
+
+/* subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)              src/module_gle.f90:146-227
+ * Sets up the GLE thermostat (therm = RPMD_THERM_GLE) of this context for (na, nb): A is the
+ * (ns+1) x (ns+1) drift matrix in the GLE-A file's row order (A[i*(ns+1)+j] = A(i+1,j+1)), in
+ * units of 2 pi / ttau (ttau from the context parameters); C = (nb/beta) I.  dt is the time step
+ * the reference passes here, half the evolve_pi step (src/main.f90:609).  gp: the initial
+ * auxiliary momenta in the reference layout gp(3,na,nb,ns+1), or NULL to draw chol(C) xi on the
+ * device from seed.  ns <= 7. */
+int rpmd_gle_init(rpmd_ctx* ctx, int ns, const double* A, int na, int nb, double dt, double beta,
+                  const double* gp, unsigned long long seed);
 
 /* Device-resident stepping (no counterpart in the reference, whose trajectory loop calls
  * evolve once per step, src/md_dynamics.f90:499-517): upload a state once, run nsteps

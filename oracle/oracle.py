@@ -165,6 +165,38 @@ class Oracle:
                                C.c_int(na), C.c_int(nb), C.byref(s))
         return p
 
+    def therm_pro_global(self, p, mass, beta, ctau, dt, s: RngState):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_therm_pro_global(_d(p), _d(_f64(mass)), C.c_double(beta), C.c_double(ctau), C.c_double(dt),
+                                      C.c_int(na), C.c_int(nb), C.byref(s))
+        return p
+
+    def parinello_therm(self, p, mass, tau, dt, s: RngState, beta, nctot=0):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_parinello_therm(_d(p), _d(_f64(mass)), C.c_double(tau), C.c_int(na), C.c_int(nb),
+                                     C.c_double(dt), C.byref(s), C.c_double(beta), C.c_int(nctot))
+        return p
+
+    def parinello_therm_loc(self, p, mass, tau, dt, s: RngState, beta):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_parinello_therm_loc(_d(p), _d(_f64(mass)), C.c_double(tau), C.c_int(na), C.c_int(nb),
+                                         C.c_double(dt), C.byref(s), C.c_double(beta))
+        return p
+
+    def therm_gle(self, g: GleState, p, mass, s: RngState):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_therm_gle(C.byref(g), _d(p), _d(_f64(mass)), C.c_int(na), C.c_int(nb), C.byref(s))
+        return p
+
+    def gle_channels(self, g: GleState) -> np.ndarray:
+        """the auxiliary momenta gp(3,na,nb,ns+1) as a (ns+1, nb, na, 3) array"""
+        n = (g.ns + 1) * g.nb * g.na * 3
+        return np.ctypeslib.as_array(g.gp, shape=(n,)).reshape(g.ns + 1, g.nb, g.na, 3).copy()
+
     def matrix_exp(self, M, j=15, k=15):
         M = _f64(np.asarray(M).T)  # column-major
         n = M.shape[0]
@@ -264,9 +296,13 @@ class RefPath:
         return v.value, dvdr, vir.reshape(3, 3).T.copy()
 
     def seed(self, irun: int):
-        """src/main.f90:298-299: irun = -irun; test = gaussian(irun,1.d0)."""
+        """src/main.f90:298-299: irun = -irun; test = gaussian(irun,1.d0).  gaussian keeps a
+        cached second deviate across calls (src/random.f90:9); when one is pending the first call
+        returns it without touching ran2 (idum stays negative), so draw again to re-seed."""
         self._idum = C.c_int(-irun)
         self.gaussian()
+        if self._idum.value < 0:
+            self.gaussian()
 
     def gaussian(self, sigma: float = 1.0) -> float:
         return self.lib.gaussian_(C.byref(self._idum), C.byref(C.c_double(sigma)))
@@ -276,6 +312,57 @@ class RefPath:
         nb, na = p.shape[0], p.shape[1]
         self.lib._QMglePtherm_pro(_d(p), _d(_f64(mass)), C.byref(C.c_double(beta)), C.byref(C.c_double(ctau)),
                                   C.byref(C.c_double(dt)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                  C.byref(self._idum))
+        return p
+
+    def parinello_therm(self, p, mass, tau, dt, beta, nctot=0):
+        """thermostat::parinello_therm (src/module_thermostat.f90:6-54); nctot via /constraint/"""
+        (C.c_int * 2).in_dll(self.lib, "constraint_")[0] = nctot
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib._QMthermostatPparinello_therm(_d(p), _d(_f64(mass)), C.byref(C.c_double(tau)), C.byref(C.c_int(na)),
+                                               C.byref(C.c_int(nb)), C.byref(C.c_double(dt)), C.byref(self._idum),
+                                               C.byref(C.c_double(beta)))
+        return p
+
+    def parinello_therm_loc(self, p, mass, tau, dt, beta):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib._QMthermostatPparinello_therm_loc(_d(p), _d(_f64(mass)), C.byref(C.c_double(tau)),
+                                                   C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                                   C.byref(C.c_double(dt)), C.byref(self._idum),
+                                                   C.byref(C.c_double(beta)))
+        return p
+
+    def therm_pro_global(self, p, mass, beta, ctau, dt):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib._QMglePtherm_pro_global(_d(p), _d(_f64(mass)), C.byref(C.c_double(beta)), C.byref(C.c_double(ctau)),
+                                         C.byref(C.c_double(dt)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                         C.byref(self._idum))
+        return p
+
+    def gle_init(self, A, ttau, na, nb, dt, beta, workdir):
+        """gle::therm_gle_init (src/module_gle.f90:146-227).  It reads GLE-A from the working
+        directory and allocates module arrays, so it can run once per process."""
+        A = np.asarray(A, dtype=np.float64)
+        with open(os.path.join(workdir, "GLE-A"), "w") as f:
+            f.write(f"{A.shape[0] - 1}\n")
+            for row in A:
+                f.write(" ".join(repr(float(x)) for x in row) + "\n")
+        cwd = os.getcwd()
+        os.chdir(workdir)
+        try:
+            self.lib._QMglePtherm_gle_init(C.byref(C.c_double(ttau)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                           C.byref(C.c_double(dt)), C.byref(self._idum), C.byref(C.c_double(beta)))
+        finally:
+            os.chdir(cwd)
+
+    def therm_gle(self, p, mass):
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        dheat = C.c_double()
+        self.lib._QMglePtherm_gle(_d(p), C.byref(dheat), _d(_f64(mass)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
                                   C.byref(self._idum))
         return p
 

@@ -66,7 +66,7 @@ def lib() -> C.CDLL:
         for name in ("rpmd_setup_ewald", "rpmd_ctx_create", "rpmd_ctx_set_params", "rpmd_ctx_set_precision",
                      "rpmd_ctx_destroy", "rpmd_forces", "rpmd_full_forces", "rpmd_realft", "rpmd_evolve_pi",
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
-                     "rpmd_prof_begin", "rpmd_prof_end"):
+                     "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -182,6 +182,17 @@ class RPMDContext:
                                     _p(vir), _p(vir_lf), npt, nn))
         return dict(v=v.value, vew=vew.value, vlj=vlj.value, vint=vint.value, vir=vir.reshape(3, 3).T.copy(),
                     vir_lf=vir_lf.reshape(3, 3).T.copy())
+
+    # -- subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)  src/module_gle.f90:146 ---------------
+    def gle_init(self, A, na: int, nb: int, dt: float, beta: float, gp=None, seed: int = 0):
+        """A: (ns+1, ns+1) drift matrix as the GLE-A file lists it (rows); dt: the half step the
+        reference passes (src/main.f90:609); gp: initial auxiliary momenta, shape (ns+1, nb, na, 3)
+        (= Fortran gp(3,na,nb,ns+1)), or None to draw them on the device."""
+        A = _arr(A)
+        ns = A.shape[0] - 1
+        gpp = _p(_arr(gp, (ns + 1, nb, na, 3))) if gp is not None else None
+        _check(lib().rpmd_gle_init(self._h, C.c_int(ns), _p(A), C.c_int(na), C.c_int(nb), C.c_double(dt),
+                                   C.c_double(beta), gpp, C.c_ulonglong(seed)))
 
     # -- device-resident stepping (no reference counterpart) ---------------------------------------
     def state_upload(self, p, r, mass, z, na, nb, boxlxyz, beta, dt, seed=12345):

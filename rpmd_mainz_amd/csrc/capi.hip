@@ -85,6 +85,11 @@ struct rpmd_ctx {
     size_t noise_cap = 0;
     uint64_t* step_ctr = nullptr;
     uint64_t seed = 0;
+    // thermostat state: Bussi partial sums and scale; GLE auxiliary momenta gp (n1 channels of
+    // 3 na nb, channel 0 = mass-scaled p), propagators T and S, partial sums
+    double *bussi_parts = nullptr, *bussi_alpha = nullptr;
+    double *gle_gp = nullptr, *gle_TS = nullptr, *gle_parts = nullptr, *gle_pcom = nullptr;
+    int gle_n1 = 0, gle_nb = 0;
     std::map<int, NMMat> nm;
     // host staging (pinned)
     double* h_stage = nullptr;
@@ -534,6 +539,67 @@ void ring_rpmd(const double* mass3, double* monod, int nb, double dt, double bet
     }
 }
 
+// GLE propagators (src/module_gle.f90:156-195).  A is (n x n) in the GLE-A file's row order, in
+// units of 2 pi / ttau, where the reference forms 2 pi in single precision (2.*3.1415927).
+// T = exp(-dt A) by matrix_exp(.., 15, 15) (:323-354: Taylor series of 15 terms by Horner's rule
+// on A / 2^15, then 15 squarings); S = cholesky(C - T C T^T) (:357-378) with C = (nb/beta) I.
+// T and S are returned column-major.
+void gle_propagators(const double* A, int n, double ttau, int nb, double dt, double beta, double* T, double* S) {
+    auto at = [n](double* m, int i, int j) -> double& { return m[(size_t)j * n + i]; };
+    std::vector<double> M((size_t)n * n), E((size_t)n * n), W((size_t)n * n), C((size_t)n * n, 0.0);
+    const float twopi_f = 2.0f * 3.1415927f;
+    const double fac = (double)twopi_f / ttau;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) at(M.data(), i, j) = -dt * (A[(size_t)i * n + j] * fac);
+    const int jt = 15, ks = 15;
+    std::vector<double> tc(jt + 1);
+    tc[0] = 1.0;
+    for (int i = 1; i <= jt; i++) tc[i] = tc[i - 1] / (double)i;
+    const double scl = (double)(1.0f / powf(2.0f, (float)ks));
+    for (double& x : M) x = x * scl;
+    auto matmul = [&](const std::vector<double>& X, const std::vector<double>& Y, std::vector<double>& Z) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double acc = 0.0;
+                for (int c = 0; c < n; c++) acc += X[(size_t)c * n + i] * Y[(size_t)j * n + c];
+                Z[(size_t)j * n + i] = acc;
+            }
+    };
+    std::fill(E.begin(), E.end(), 0.0);
+    for (int i = 0; i < n; i++) at(E.data(), i, i) = tc[jt];
+    for (int q = jt; q >= 1; q--) {
+        matmul(M, E, W);
+        E.swap(W);
+        for (int i = 0; i < n; i++) at(E.data(), i, i) += tc[q - 1];
+    }
+    for (int q = 0; q < ks; q++) {
+        matmul(E, E, W);
+        E.swap(W);
+    }
+    for (int i = 0; i < n; i++) at(C.data(), i, i) = (double)nb / beta;
+    // D = C - T C T^T
+    std::vector<double> TC((size_t)n * n), TCT((size_t)n * n), Et((size_t)n * n), D((size_t)n * n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Et[(size_t)j * n + i] = E[(size_t)i * n + j];
+    matmul(C, Et, TC);
+    matmul(E, TC, TCT);
+    for (size_t i = 0; i < D.size(); i++) D[i] = C[i] - TCT[i];
+    std::vector<double> L((size_t)n * n, 0.0);
+    at(L.data(), 0, 0) = sqrt(at(D.data(), 0, 0));
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < i; j++) {
+            double x = at(D.data(), i, j);
+            for (int k = 0; k < j; k++) x = x - at(L.data(), i, k) * at(L.data(), j, k);
+            at(L.data(), i, j) = x / at(L.data(), j, j);
+        }
+        double x = at(D.data(), i, i);
+        for (int k = 0; k < i; k++) x = x - at(L.data(), i, k) * at(L.data(), i, k);
+        at(L.data(), i, i) = sqrt(x);
+    }
+    memcpy(T, E.data(), sizeof(double) * n * n);
+    memcpy(S, L.data(), sizeof(double) * n * n);
+}
+
 // noise source for one draw block of 3*na*nb deviates
 struct NoiseCursor {
     const double* base;  // device pointer of host-supplied deviates, or null (device stream)
@@ -560,12 +626,21 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     NMMat* nmm = get_nm(c, nb, 0);
     if (!nmm) return RPMD_ERR_NOMEM;
     const size_t draws = (size_t)3 * na * nb;
-    if (P.therm == RPMD_THERM_PRG || P.therm == RPMD_THERM_GLE || P.therm == RPMD_THERM_PRM) return RPMD_ERR_THERM;
+    if (P.therm == RPMD_THERM_GLE && (!c->gle_gp || c->gle_nb != nb)) return RPMD_ERR_GLE;
+    const double beta = c->st_beta, betan = beta / (double)nb;
     auto outer = [&]() {
         if (P.therm == RPMD_THERM_PRL) {
             // parinello_therm_loc with dt (src/evolve_pi.f90:75-77, src/module_thermostat.f90:56-82)
             const double gamma = 1.0 / (2.0 * P.ttau), c1 = exp(-0.5 * gamma * dt);
             launch_langevin(c->p, c->lang_c3m, c1, na, nb, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++, s);
+        } else if (P.therm == RPMD_THERM_PRG) {
+            // parinello_therm(p, mass, ttau, na, nb, 0.5 dt, irun, beta) on the beads, nctot = 0
+            const long long ndof = 3LL * na * nb - 3;
+            launch_bussi(c->p, c->mass, na, nb, ndof, betan, exp(-(0.5 * dt) / P.ttau), nz.take((size_t)ndof + 1),
+                         c->seed, c->step_ctr, nz.stream_add++, c->bussi_parts, c->bussi_alpha, nullptr, s);
+        } else if (P.therm == RPMD_THERM_GLE) {
+            launch_gle(c->p, c->gle_gp, c->gle_TS, c->mass, na, nb, c->gle_n1, nz.take(draws * c->gle_n1), c->seed,
+                       c->step_ctr, nz.stream_add++, c->gle_parts, c->gle_pcom, 1.0 / sqrt(nb * c->mtot), s);
         }
     };
     outer();
@@ -576,8 +651,17 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     const double inv_mtot = 1.0 / c->mtot;
     auto pile = [&]() {
         if (P.therm == RPMD_THERM_PRO) {
-            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
-                        c->com_parts, s);
+            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr,
+                        nz.stream_add++, c->com_parts, s);
+            launch_com_remove(c->pnm, c->mass, c->com_parts, nblk, inv_mtot, na, s);
+        } else if (P.therm == RPMD_THERM_PRM) {
+            // therm_pro_global (src/module_gle.f90:75-143): PILE on modes 2..nb, Bussi on the
+            // centroid (parinello_therm with nb = 1 and beta_n), then the centroid COM removal
+            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 1, nz.take((size_t)3 * na * (nb - 1)), c->seed,
+                        c->step_ctr, nz.stream_add++, nullptr, s);
+            const long long ndof = 3LL * na - 3;
+            launch_bussi(c->pnm, c->mass, na, 1, ndof, betan, exp(-halfdtsmall / P.ttau), nz.take((size_t)ndof + 1),
+                         c->seed, c->step_ctr, nz.stream_add++, c->bussi_parts, c->bussi_alpha, c->com_parts, s);
             launch_com_remove(c->pnm, c->mass, c->com_parts, nblk, inv_mtot, na, s);
         }
     };
@@ -628,6 +712,7 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
     outer();
     launch_counter_inc(c->step_ctr, s);
+    if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
     if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
     return 0;
 }
@@ -696,10 +781,13 @@ int set_state_consts(rpmd_ctx* c, const double* mass, double beta, double dt, in
     return 0;
 }
 
-size_t noise_per_step(const rpmd_params& P, int na, int nb) {
+size_t noise_per_step(const rpmd_params& P, int na, int nb, int ngle) {
     const size_t d = (size_t)3 * na * nb;
     if (P.therm == RPMD_THERM_PRO) return 2 * (size_t)P.mts * d;
     if (P.therm == RPMD_THERM_PRL) return 2 * d;
+    if (P.therm == RPMD_THERM_PRG) return 2 * (d - 2);
+    if (P.therm == RPMD_THERM_PRM) return 2 * (size_t)P.mts * (d - 2);
+    if (P.therm == RPMD_THERM_GLE) return 2 * d * (size_t)std::max(ngle, 1);
     return 0;
 }
 
@@ -729,6 +817,7 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_NOMEM: return "device allocation failed";
         case RPMD_ERR_THERM: return "thermostat not supported by this entry point";
         case RPMD_ERR_NOISE: return "noise buffer too short";
+        case RPMD_ERR_GLE: return "GLE not initialised for this (na, nb): call rpmd_gle_init";
     }
     return "unknown error";
 }
@@ -825,6 +914,10 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->lang_c3m, na);
     e |= dalloc(&c->maps, 1024);
     e |= dalloc(&c->step_ctr, 1);
+    e |= dalloc(&c->bussi_parts, (size_t)bussi_parts_size());
+    e |= dalloc(&c->bussi_alpha, 1);
+    e |= dalloc(&c->gle_TS, 128);
+    e |= dalloc(&c->gle_pcom, 24);
     e |= dalloc(&c->ev_step, (size_t)(64 + 4) * 50);
     if (e) {
         rpmd_ctx_destroy(c);
@@ -857,7 +950,8 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* bufs[] = {c->ph, c->fpart, c->bb, c->ccnt, c->clist, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+    void* bufs[] = {c->ph, c->fpart, c->bb, c->ccnt, c->clist, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
+                    c->gle_parts, c->gle_pcom, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
@@ -989,7 +1083,7 @@ int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, do
     if (!c || !p || !r || !dvdr || !dvdr2 || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
-    const size_t need = noise_per_step(c->prm, na, nb);
+    const size_t need = noise_per_step(c->prm, na, nb, c->gle_n1);
     if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
     RETCHK(upload_charges(c, z));
     c->st_nb = nb;
@@ -1050,6 +1144,48 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
         c->gexec = nullptr;
     }
     c->st_valid = true;
+    return 0;
+}
+
+int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double dt, double beta, const double* gp,
+                  unsigned long long seed) {
+    if (!c || !A || ns < 0 || ns > 7 || nb < 1 || nb > c->nb_max || beta <= 0.0) return RPMD_ERR_ARG;
+    if (na != c->na) return RPMD_ERR_ARG;
+    const int n1 = ns + 1;
+    const size_t N = (size_t)3 * na * nb;
+    double TS[128] = {0};
+    gle_propagators(A, n1, c->prm.ttau, nb, dt, beta, TS, TS + 64);
+    if (c->gle_gp && c->gle_nb * (size_t)c->gle_n1 < (size_t)nb * n1) {
+        hipFree(c->gle_gp);
+        hipFree(c->gle_parts);
+        c->gle_gp = c->gle_parts = nullptr;
+    }
+    if (!c->gle_gp) {
+        RETCHK(dalloc(&c->gle_gp, N * n1) ? RPMD_ERR_NOMEM : 0);
+        RETCHK(dalloc(&c->gle_parts, (size_t)gle_parts_size(na, nb)) ? RPMD_ERR_NOMEM : 0);
+    }
+    HIPCHK(hipMemcpyAsync(c->gle_TS, TS, sizeof TS, hipMemcpyHostToDevice, c->st));
+    if (gp) {
+        // reference layout gp(3,na,nb,ns+1) -> channel-major SoA per bead
+        std::vector<double> h(N * n1);
+        for (int i = 0; i < n1; i++)
+            for (int b = 0; b < nb; b++)
+                for (int a = 0; a < na; a++)
+                    for (int d = 0; d < 3; d++)
+                        h[(size_t)i * N + ((size_t)b * 3 + d) * na + a] = gp[(size_t)i * N + ((size_t)b * na + a) * 3 + d];
+        HIPCHK(hipMemcpyAsync(c->gle_gp, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    } else {
+        // C = (nb/beta) I, so chol(C) xi = sqrt(nb/beta) xi; a stream id apart from the step streams
+        launch_gle_draw(c->gle_gp, sqrt((double)nb / beta), na, nb, n1, seed, ~0ull, c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    c->gle_n1 = n1;
+    c->gle_nb = nb;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
     return 0;
 }
 

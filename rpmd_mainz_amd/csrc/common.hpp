@@ -15,6 +15,7 @@
 #define RPMD_KTILE 32   // k-space tile: ky, kz in [-16, 15] (index k + 16) -> kmax <= 15
 #define RPMD_KMAX_DEV 15
 #define RPMD_NCAP 512   // O-O / TEM neighbour capacity per atom (full list)
+#define RPMD_STREAMS_PER_STEP 64  // counter-based noise streams reserved per propagation step
 
 namespace rpmd {
 
@@ -85,8 +86,17 @@ void launch_nm(const double* in, double* out, const double* M, int n, size_t ne,
 void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s);
 void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s);
 int pile_blocks(int na);
-void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, const double* noise, uint64_t seed,
-                 const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);
+void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
+                 uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);
+int bussi_parts_size();
+void launch_bussi(double* p, const double* mass, int na, int nbq, long long ndof, double betan, double c,
+                  const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* parts,
+                  double* alpha, double* com_parts, hipStream_t s);
+int gle_parts_size(int na, int nb);
+void launch_gle(double* p, double* gp, const double* TS, const double* mass, int na, int nb, int n1,
+                const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* parts,
+                double* pcom, double mm, hipStream_t s);
+void launch_gle_draw(double* gp, double sc, int na, int nb, int n1, uint64_t seed, uint64_t stream, hipStream_t s);
 void launch_com_remove(double* p, const double* mass, const double* com_parts, int nblk, double inv_mtot, int na,
                        hipStream_t s);
 void launch_langevin(double* p, const double* c3m, double c1, int na, int nb, const double* noise, uint64_t seed,

@@ -162,17 +162,17 @@ __device__ __forceinline__ double normal_at(uint64_t seed, uint64_t stream, uint
     return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
 }
 
-// ---- PILE-L kick in the NM representation (src/module_gle.f90:8-73) ------------------------------
-// p(i,j,k) = c1(k) p + c2(k) sqrt(m(ic)) xi ; xi from the host buffer in the reference draw order
-// (mode, atom, xyz) or from the counter-based stream.  Block partial sums of the centroid momentum
-// go to com_parts[blk][3].
+// ---- PILE kick in the NM representation (src/module_gle.f90:8-73; modes k0.. for PILE-G :97-115) ----
+// p(i,j,k) = c1(k) p + c2(k) sqrt(m(ic)) xi for modes k >= k0; xi from the host buffer in the
+// reference draw order (mode, atom, xyz) or from the counter-based stream.  With com_parts, block
+// partial sums of the centroid momentum go to com_parts[blk][3].
 __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const double* __restrict__ c1,
-                                              const double* __restrict__ c3, int na, int nb,
+                                              const double* __restrict__ c3, int na, int k0,
                                               const double* __restrict__ noise, uint64_t seed,
                                               const uint64_t* __restrict__ streamp, uint64_t stream_add,
                                               double* __restrict__ com_parts) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+    const int k = blockIdx.y + k0;
     const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
     double s[3] = {0.0, 0.0, 0.0};
     if (a < na) {
@@ -181,14 +181,14 @@ __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const doub
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const size_t o = ((size_t)k * 3 + d) * na + a;
-            const size_t ni = ((size_t)k * na + a) * 3 + d;
+            const size_t ni = ((size_t)(k - k0) * na + a) * 3 + d;
             const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
             const double v = __dadd_rn(__dmul_rn(cc1, p[o]), __dmul_rn(cc3, xi));
             p[o] = v;
             s[d] = v;
         }
     }
-    if (k == 0) {
+    if (k == 0 && com_parts) {
         for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
             for (int d = 0; d < 3; d++) s[d] += __shfl_down(s[d], off, 64);
@@ -205,9 +205,11 @@ __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const doub
     }
 }
 int pile_blocks(int na) { return (na + 255) / 256; }
-void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, const double* noise, uint64_t seed,
-                 const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s) {
-    k_pile<<<dim3(pile_blocks(na), nb), 256, 0, s>>>(p, c1, c3, na, nb, noise, seed, stream, stream_add, com_parts);
+void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
+                 uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s) {
+    if (nb <= k0) return;
+    k_pile<<<dim3(pile_blocks(na), nb - k0), 256, 0, s>>>(p, c1, c3, na, k0, noise, seed, stream, stream_add,
+                                                         com_parts);
 }
 
 // p(:,j,1) -= pcom * m(j), pcom = sum_j p(:,j,1) / sum m  (src/module_gle.f90:55-70)
@@ -251,6 +253,240 @@ void launch_langevin(double* p, const double* c3m, double c1, int na, int nb, co
     k_langevin<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, c3m, c1, na, nb, noise, seed, stream, stream_add);
 }
 
+// ---- Bussi global rescale (src/module_thermostat.f90:6-54) ---------------------------------------
+// One draw block per call: [0] rt (drawn, unused by the reference), [1, ndof) the chi-square
+// deviates, [ndof] ranno.  kt sums p^2/(2m) over the first nbq beads (modes) of p.  Three launches:
+// fixed-grid partial sums, one block that forms alpha in device memory, and the rescale.
+#define BUSSI_BLOCKS 256
+__global__ __launch_bounds__(256) void k_bussi_parts(const double* __restrict__ p, const double* __restrict__ mass,
+                                                     int na, int nbq, long long ndof,
+                                                     const double* __restrict__ noise, uint64_t seed,
+                                                     const uint64_t* __restrict__ streamp, uint64_t stream_add,
+                                                     double* __restrict__ parts) {
+    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const size_t n = (size_t)3 * na * nbq;
+    const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, nth = (size_t)gridDim.x * 256;
+    double v[2] = {0.0, 0.0};
+    for (size_t e = tid; e < n; e += nth) {
+        const double x = p[e];
+        v[0] += x * x / (2.0 * mass[e % na]);
+    }
+    for (long long i = 1 + (long long)tid; i < ndof; i += (long long)nth) {
+        const double g = noise ? noise[i] : normal_at(seed, stream, (uint64_t)i);
+        v[1] += g * g;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 2; k++) v[k] += __shfl_down(v[k], off, 64);
+    __shared__ double red[4][2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[w][0] = v[0];
+        red[w][1] = v[1];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+        parts[blockIdx.x * 2 + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                                              red[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_bussi_alpha(const double* __restrict__ parts, int nblk, long long ndof,
+                                                     double betan, double c, const double* __restrict__ noise,
+                                                     uint64_t seed, const uint64_t* __restrict__ streamp,
+                                                     uint64_t stream_add, double* __restrict__ alpha) {
+    __shared__ double red[256][2];
+    double a0 = 0.0, a1 = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += 256) {
+        a0 += parts[q * 2];
+        a1 += parts[q * 2 + 1];
+    }
+    red[threadIdx.x][0] = a0;
+    red[threadIdx.x][1] = a1;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[threadIdx.x][0] += red[threadIdx.x + h][0];
+            red[threadIdx.x][1] += red[threadIdx.x + h][1];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+        const double kt = red[0][0], snfm1 = red[0][1];
+        const double ranno = noise ? noise[ndof] : normal_at(seed, stream, (uint64_t)ndof);
+        const double ransq = ranno * ranno;
+        const double ktav = (double)ndof / (2.0 * betan);
+        const double fac = (1.0 - c) * ktav / ((double)ndof * kt);
+        const double t1 = fac * (snfm1 + ransq);
+        const double t2 = 2 * ranno * sqrt(c * fac);
+        double al = sqrt(c + t1 + t2);
+        const double plusmin = ranno + sqrt(c / fac);
+        al = plusmin >= 0.0 ? fabs(al) : -fabs(al);
+        alpha[0] = al;
+    }
+}
+
+// p *= alpha on the first nbq beads; with com_parts, partial centroid momenta as k_pile writes them
+__global__ __launch_bounds__(256) void k_bussi_scale(double* __restrict__ p, const double* __restrict__ alpha, int na,
+                                                     double* __restrict__ com_parts) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const double al = alpha[0];
+    double s[3] = {0.0, 0.0, 0.0};
+    if (a < na) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t o = ((size_t)k * 3 + d) * na + a;
+            const double v = al * p[o];
+            p[o] = v;
+            s[d] = v;
+        }
+    }
+    if (k == 0 && com_parts) {
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) s[d] += __shfl_down(s[d], off, 64);
+        __shared__ double red[4][3];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0)
+            for (int d = 0; d < 3; d++) red[w][d] = s[d];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double t = 0.0;
+            for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q][threadIdx.x];
+            com_parts[blockIdx.x * 3 + threadIdx.x] = t;
+        }
+    }
+}
+
+int bussi_parts_size() { return 2 * BUSSI_BLOCKS; }
+void launch_bussi(double* p, const double* mass, int na, int nbq, long long ndof, double betan, double c,
+                  const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* parts,
+                  double* alpha, double* com_parts, hipStream_t s) {
+    k_bussi_parts<<<BUSSI_BLOCKS, 256, 0, s>>>(p, mass, na, nbq, ndof, noise, seed, stream, stream_add, parts);
+    k_bussi_alpha<<<1, 256, 0, s>>>(parts, BUSSI_BLOCKS, ndof, betan, c, noise, seed, stream, stream_add, alpha);
+    k_bussi_scale<<<dim3(pile_blocks(na), nbq), 256, 0, s>>>(p, alpha, na, com_parts);
+}
+
+// ---- GLE kick (src/module_gle.f90:229-283) -------------------------------------------------------
+// Per element (bead b, xyz d, atom a) of the (3 na nb) x (ns+1) matrix gp: channel 0 <- p/sqrt(m);
+// gp <- gp T^T + xi S^T (T = TS[0..], S = TS[64..], column-major N1 x N1); xi in the reference draw
+// order (channel, bead, atom, xyz).  Block partial sums of sqrt(m) gp per channel and xyz feed the
+// centre-of-mass removal of pshift_gle (:285-320).
+template <int N1>
+__global__ __launch_bounds__(256) void k_gle(const double* __restrict__ p, double* __restrict__ gp,
+                                             const double* __restrict__ TS, const double* __restrict__ mass, int na,
+                                             int nb, const double* __restrict__ noise, uint64_t seed,
+                                             const uint64_t* __restrict__ streamp, uint64_t stream_add,
+                                             double* __restrict__ parts) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const size_t N = (size_t)3 * na * nb;
+    double pc[N1 * 3];
+#pragma unroll
+    for (int i = 0; i < N1 * 3; i++) pc[i] = 0.0;
+    if (a < na) {
+        const double m = mass[a];
+        const double mfac = 1.0 / sqrt(m), sm = sqrt(m);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t e = ((size_t)b * 3 + d) * na + a;
+            double g[N1], x[N1], o[N1];
+            g[0] = p[e] * mfac;
+#pragma unroll
+            for (int i = 1; i < N1; i++) g[i] = gp[i * N + e];
+#pragma unroll
+            for (int i = 0; i < N1; i++) {
+                const size_t ni = (size_t)i * N + ((size_t)b * na + a) * 3 + d;
+                x[i] = noise ? noise[ni] : normal_at(seed, stream, ni);
+            }
+#pragma unroll
+            for (int j = 0; j < N1; j++) {
+                double t = 0.0;
+#pragma unroll
+                for (int c = 0; c < N1; c++) t += g[c] * TS[c * N1 + j];
+                double u = 0.0;
+#pragma unroll
+                for (int c = 0; c < N1; c++) u += x[c] * TS[64 + c * N1 + j];
+                o[j] = t + u;
+            }
+#pragma unroll
+            for (int i = 0; i < N1; i++) {
+                gp[i * N + e] = o[i];
+                pc[i * 3 + d] = o[i] * sm;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < N1 * 3; i++) pc[i] += __shfl_down(pc[i], off, 64);
+    __shared__ double red[4][N1 * 3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < N1 * 3; i++) red[w][i] = pc[i];
+    __syncthreads();
+    if ((int)threadIdx.x < N1 * 3) {
+        double t = 0.0;
+        for (int q = 0; q < 4; q++) t += red[q][threadIdx.x];
+        parts[((size_t)b * gridDim.x + blockIdx.x) * (N1 * 3) + threadIdx.x] = t;
+    }
+}
+
+// pcom[i][d] = (sum of the partials) * mm * mm, mm = 1/sqrt(nb sum m)  (fixed order)
+__global__ void k_gle_com(const double* __restrict__ parts, int nparts, int nv, double mm, double* __restrict__ pcom) {
+    const int t = threadIdx.x;
+    if (t >= nv) return;
+    double s = 0.0;
+    for (int q = 0; q < nparts; q++) s += parts[(size_t)q * nv + t];
+    pcom[t] = s * mm * mm;
+}
+
+// gp -= pcom sqrt(m) on every channel, then p = gp(channel 0) sqrt(m)
+__global__ __launch_bounds__(256) void k_gle_shift(double* __restrict__ p, double* __restrict__ gp,
+                                                   const double* __restrict__ pcom, const double* __restrict__ mass,
+                                                   int na, int nb, int n1) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (a >= na) return;
+    const size_t N = (size_t)3 * na * nb;
+    const double sm = sqrt(mass[a]);
+    for (int d = 0; d < 3; d++) {
+        const size_t e = ((size_t)b * 3 + d) * na + a;
+        for (int i = 0; i < n1; i++) gp[i * N + e] = gp[i * N + e] - pcom[i * 3 + d] * sm;
+        p[e] = gp[e] * sm;
+    }
+}
+
+int gle_parts_size(int na, int nb) { return pile_blocks(na) * nb * 24; }
+void launch_gle(double* p, double* gp, const double* TS, const double* mass, int na, int nb, int n1,
+                const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* parts,
+                double* pcom, double mm, hipStream_t s) {
+    const dim3 grid(pile_blocks(na), nb);
+    switch (n1) {
+#define GLE_CASE(K) \
+    case K: k_gle<K><<<grid, 256, 0, s>>>(p, gp, TS, mass, na, nb, noise, seed, stream, stream_add, parts); break;
+        GLE_CASE(1) GLE_CASE(2) GLE_CASE(3) GLE_CASE(4) GLE_CASE(5) GLE_CASE(6) GLE_CASE(7) GLE_CASE(8)
+#undef GLE_CASE
+        default: return;
+    }
+    k_gle_com<<<1, 64, 0, s>>>(parts, (int)(grid.x * grid.y), n1 * 3, mm, pcom);
+    k_gle_shift<<<grid, 256, 0, s>>>(p, gp, pcom, mass, na, nb, n1);
+}
+
+// initial auxiliary momenta gp = chol(C) xi with C = (nb/beta) I (src/module_gle.f90:196-213)
+__global__ void k_gle_draw(double* __restrict__ gp, double sc, int na, int nb, int n1, uint64_t seed, uint64_t stream) {
+    const size_t N = (size_t)3 * na * nb;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * n1) return;
+    gp[t] = sc * normal_at(seed, stream, t);
+}
+void launch_gle_draw(double* gp, double sc, int na, int nb, int n1, uint64_t seed, uint64_t stream, hipStream_t s) {
+    const size_t n = (size_t)3 * na * nb * n1;
+    k_gle_draw<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(gp, sc, na, nb, n1, seed, stream);
+}
+
 // ---- contraction copies (src/evolve_pi.f90:377-431): modes 0..h and their mirrors ---------------
 // dst[k] = scale * src[map[k]] for k < ndst (map[k] < 0: zero)
 __global__ void k_mode_copy(double* __restrict__ dst, const double* __restrict__ src, const int* __restrict__ map,
@@ -276,7 +512,7 @@ void launch_scale(double* y, double a, size_t n, hipStream_t s) {
 }
 
 // per-step noise stream counter kept on the device so that a captured step replays with fresh noise
-__global__ void k_counter_inc(uint64_t* c) { *c += 16; }
+__global__ void k_counter_inc(uint64_t* c) { *c += RPMD_STREAMS_PER_STEP; }
 void launch_counter_inc(uint64_t* c, hipStream_t s) { k_counter_inc<<<1, 1, 0, s>>>(c); }
 
 }  // namespace rpmd

@@ -183,6 +183,79 @@ def test_evolve_pi_pile_replay(capi, oracle):
     ctx.close()
 
 
+def _noise_per_step(therm, na, nb, mts, n1=1):
+    """Gaussian deviates one evolve_pi step draws (src/evolve_pi.f90:75-77, 100-108, 304-310)."""
+    d = 3 * na * nb
+    return {"PRO": 2 * mts * d, "PRL": 2 * d, "PRG": 2 * (d - 2), "PRM": 2 * mts * (d - 2),
+            "GLE": 2 * n1 * d}[therm]
+
+
+@pytest.mark.parametrize("therm", ["PRG", "PRM", "PRL", "GLE"])
+def test_evolve_pi_thermostat_replay(capi, oracle, therm):
+    """Bussi global (PRG), PILE-G (PRM), local Langevin (PRL) and GLE steps on the device with the
+    reference's ran2/gaussian stream replayed, against the oracle's evolve_pi."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8, ecut=1e-6, mts=2, therm=therm, ttaufs=100.0)
+    na, nb = r.shape[1], r.shape[0]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    s0 = oracle.rng(4711)
+    p = oracle.sample_rpmd(nb, na, mass, beta, s0)
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    s_or, s_gp = oracle.rng(99), oracle.rng(99)
+    g, n1 = None, 1
+    if therm == "GLE":
+        A = np.loadtxt(os.path.join(GOLD, "gle_A_ns4.txt"))
+        n1 = A.shape[0]
+        g = oracle.gle_init(A, prm["ttau"], na, nb, 0.5 * dt, s_or, beta)
+        oracle.gaussians(s_gp, n1 * 3 * na * nb)          # the draws of therm_gle_init
+        ctx.gle_init(A, na, nb, 0.5 * dt, beta, gp=oracle.gle_channels(g))
+    po, ro, do1, do2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    pg, rg, dg1, dg2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    for k in range(3):
+        print(f"case {therm} replay step {k}", flush=True)
+        noise = oracle.gaussians(s_gp, _noise_per_step(therm, na, nb, prm["mts"], n1))
+        o = oracle.evolve_pi(P, po, ro, do1, do2, dt, mass, box, z, beta, s=s_or, g=g)
+        po, ro, do1, do2 = o["p"], o["r"], o["dvdr"], o["dvdr2"]
+        out = ctx.evolve_pi(pg, rg, dg1, dg2, dt, mass, na, nb, box, z, beta, noise=noise)
+        assert abs(out["v"] - o["v"]) < 1e-10 * abs(o["v"])
+    assert _rel(rg, ro) < 1e-12
+    assert _rel(pg, po) < 1e-10
+    if g is not None:
+        oracle.gle_free(g)
+    ctx.close()
+
+
+@pytest.mark.parametrize("therm", ["PRO", "PRM", "PRG", "GLE"])
+def test_device_noise_thermostats_sample_temperature(capi, therm):
+    """Device noise stream: from a cool start (0.3 of the thermal momenta) the bead kinetic energy of a ring polymer of
+    molecules relaxes towards the canonical value 3 na nb / (2 beta_n) within a few tau; a check
+    that every thermostat couples with the right sign and variance, run through run_steps
+    (hipGraph)."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, therm=therm, ttaufs=10.0,
+                                           intra_model=cf.INTRA_SPCFW)
+    na, nb = r.shape[1], r.shape[0]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    p0 = 0.3 * np.random.default_rng(1).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    ctx.state_upload(p0, r, mass, z, na, nb, box, beta, dt, seed=2024)
+    if therm == "GLE":
+        A = np.loadtxt(os.path.join(GOLD, "gle_A_ns4.txt"))
+        ctx.gle_init(A, na, nb, 0.5 * dt, beta, seed=5)
+    ctx.use_graph(True)
+    kin = []
+    ctx.run_steps(400)
+    for _ in range(20):
+        ctx.run_steps(20)
+        st = ctx.state_download()
+        kin.append(float(np.sum(st["p"] ** 2 / (2.0 * mass[None, :, None]))))
+    target = 3 * na * nb / (2.0 * beta / nb)
+    ratio = np.mean(kin) / target
+    print(f"{therm}: <K>/K_target = {ratio:.3f}")
+    assert 0.8 < ratio < 1.2
+    ctx.close()
+
+
 def test_device_resident_steps_match_oracle(capi, oracle):
     """NVE steps run with all state in HBM agree with the oracle's evolve_pi."""
     r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8, ecut=1e-6, intra_model=cf.INTRA_SPCFW)

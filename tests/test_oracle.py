@@ -143,6 +143,83 @@ def test_pile_matches_reference(oracle, refpath):
     assert np.array_equal(a, b)
 
 
+def test_bussi_and_langevin_match_reference(oracle, refpath):
+    """parinello_therm (global, PRG), parinello_therm_loc (PRL) and therm_pro_global (PRM)
+    against the reference's own Fortran, same ran2 stream: bitwise."""
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    p0 = np.random.default_rng(5).normal(size=r.shape) * 30.0
+    beta, ttau, dt = 1.0 / (cf.KBOLTZ * 298.0), 100 * cf.TOFS, 0.125 * cf.TOFS
+    for irun, nctot in ((123, 0), (321, 7)):
+        s = oracle.rng(irun)
+        refpath.seed(irun)
+        a = oracle.parinello_therm(p0, mass, ttau, dt, s, beta, nctot)
+        b = refpath.parinello_therm(p0, mass, ttau, dt, beta, nctot)
+        assert np.array_equal(a, b)
+        # a second call continues both streams in step
+        a = oracle.parinello_therm(a, mass, ttau, dt, s, beta, nctot)
+        b = refpath.parinello_therm(b, mass, ttau, dt, beta, nctot)
+        assert np.array_equal(a, b)
+    refpath.parinello_therm(p0, mass, ttau, dt, beta, 0)
+    s = oracle.rng(77)
+    refpath.seed(77)
+    assert np.array_equal(oracle.parinello_therm_loc(p0, mass, ttau, dt, s, beta),
+                          refpath.parinello_therm_loc(p0, mass, ttau, dt, beta))
+    s = oracle.rng(78)
+    refpath.seed(78)
+    assert np.array_equal(oracle.therm_pro_global(p0, mass, beta, ttau, dt, s),
+                          refpath.therm_pro_global(p0, mass, beta, ttau, dt))
+
+
+def test_bussi_invariants(oracle):
+    """Closed forms of the Bussi rescale: p' = alpha p with one scalar alpha; with tau -> 0 the
+    new kinetic energy is K' = (snfm1 + R^2) / (2 beta_n) exactly (c = 0), a chi-square draw that
+    no longer depends on K; with tau -> inf (c = 1) p is unchanged."""
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    p0 = np.random.default_rng(6).normal(size=r.shape) * 30.0
+    beta = 1.0 / (cf.KBOLTZ * 298.0)
+    s = oracle.rng(5)
+    a = oracle.parinello_therm(p0, mass, 1e-300, 1.0, s, beta)
+    ratio = a / p0
+    assert np.allclose(ratio, ratio.flat[0], rtol=1e-13, atol=0)
+    s1 = oracle.rng(5)
+    g = oracle.gaussians(s1, 3 * na * nb - 3 + 1)
+    ndof = 3 * na * nb - 3
+    k_new = (np.sum(g[1:ndof] ** 2) + g[ndof] ** 2) / (2.0 * beta / nb)
+    k_out = float(np.sum(a ** 2 / (2.0 * mass[None, :, None])))
+    assert abs(k_out - k_new) < 1e-12 * k_new
+    s = oracle.rng(5)
+    assert np.allclose(oracle.parinello_therm(p0, mass, 1e300, 1.0, s, beta), p0, rtol=1e-14, atol=0)
+
+
+def test_therm_gle_matches_reference(oracle, refpath, tmp_path):
+    """therm_gle_init + two therm_gle kicks (src/module_gle.f90:146-283) against the reference's
+    Fortran on the same ran2 stream.  The reference's DGEMM accumulates in another order than the
+    restatement's explicit sums, so agreement is to rounding, not bitwise."""
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    A = np.loadtxt(os.path.join(GOLD, "gle_A_ns4.txt"))
+    beta, ttau, dt = 1.0 / (cf.KBOLTZ * 298.0), 100 * cf.TOFS, 0.25 * cf.TOFS
+    p0 = np.random.default_rng(8).normal(size=r.shape) * 30.0
+    s = oracle.rng(31)
+    refpath.seed(31)
+    g = oracle.gle_init(A, ttau, na, nb, 0.5 * dt, s, beta)
+    refpath.gle_init(A, ttau, na, nb, 0.5 * dt, beta, str(tmp_path))
+    a, b = p0, p0
+    for _ in range(2):
+        a = oracle.therm_gle(g, a, mass, s)
+        b = refpath.therm_gle(b, mass)
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    # the streams stay in step: the next deviate agrees
+    assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
+    # invariant: the centre-of-mass momentum of every channel is removed (pshift_gle)
+    ch = oracle.gle_channels(g)
+    sm = np.sqrt(mass)[None, None, :, None]
+    assert np.abs((ch * sm).sum(axis=(1, 2))).max() < 1e-10 * np.abs(ch * sm).sum()
+    oracle.gle_free(g)
+
+
 def test_gle_helpers_match_reference(oracle, refpath):
     import scipy.linalg
     A = np.loadtxt(os.path.join(GOLD, "gle_A_ns4.txt"))
