@@ -87,6 +87,8 @@ typedef struct rpmd_params {
     /* thermostat: RPMD_THERM_* and ttau in a.u. time */
     int therm;
     double ttau;
+    /* /structure/ iamrigid: rigid molecules held by RATTLE (rpmd_evolve_rig_pi) */
+    int rigid;
 } rpmd_params;
 
 typedef struct rpmd_ctx rpmd_ctx;
@@ -133,6 +135,18 @@ int rpmd_evolve_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* vew, 
                    int na, int nb, const double boxlxyz[3], const double* z, double beta,
                    double* vir, double* vir_lf, const double* noise, long long n_noise);
 
+/* subroutine evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,
+ *                          irun,nbaro)                          src/evolve_rigid.f90:100-234
+ * One rigid-molecule RPMD step (params.rigid = 1, nbaro = 0): nb bead-spring sub-steps with
+ * RATTLE (src/md_rattle.f90, FP64, tolerance 1e-6, at most 1000 iterations), forces(iopt = 1),
+ * and RATTLE stage 2 with dt.  vir is in/out: the reference adds the incoming value.  Thermostats
+ * PRG and PRL as the reference's; other codes leave the step thermostat-free, as there.  A
+ * RATTLE that does not converge returns RPMD_ERR_RATTLE. */
+int rpmd_evolve_rig_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                       double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                       const double boxlxyz[3], const double* z, double beta, double* vir,
+                       double* vir_lf, const double* noise, long long n_noise);
+
 /* subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)              src/module_gle.f90:146-227
  * Sets up the GLE thermostat (therm = RPMD_THERM_GLE) of this context for (na, nb): A is the
  * (ns+1) x (ns+1) drift matrix in the GLE-A file's row order (A[i*(ns+1)+j] = A(i+1,j+1)), in
@@ -145,7 +159,8 @@ int rpmd_gle_init(rpmd_ctx* ctx, int ns, const double* A, int na, int nb, double
 
 /* Device-resident stepping (no counterpart in the reference, whose trajectory loop calls
  * evolve once per step, src/md_dynamics.f90:499-517): upload a state once, run nsteps
- * evolve_pi steps with all data in HBM, read energies/state back when needed. */
+ * evolve_pi steps (evolve_rig_pi steps when params.rigid) with all data in HBM, read
+ * energies/state back when needed.  energies = {v, vew, vlj, vint}. */
 int rpmd_state_upload(rpmd_ctx* ctx, const double* p, const double* r, const double* mass,
                       const double* z, int na, int nb, const double boxlxyz[3], double beta,
                       double dt, unsigned long long seed);

@@ -37,6 +37,7 @@ class RpmdParams(C.Structure):
         ("sig", C.c_double),
         ("nbdf1", C.c_int), ("nbdf2", C.c_int), ("mts", C.c_int),
         ("intra_model", C.c_int), ("therm", C.c_int), ("ttau", C.c_double),
+        ("rigid", C.c_int),
     ]
 
     @classmethod
@@ -85,6 +86,9 @@ class Oracle:
         L.orc_ran2.restype = C.c_double
         L.orc_ran2.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_gle_init.restype = C.c_int
+        L.orc_rattle_s1.restype = C.c_int
+        L.orc_rattle_s2.restype = C.c_int
+        L.orc_evolve_rig_pi.restype = C.c_int
 
     # -- set-up --------------------------------------------------------------------------
     def setup_ewald(self, na: int, box, prm: RpmdParams) -> RpmdParams:
@@ -196,6 +200,49 @@ class Oracle:
         """the auxiliary momenta gp(3,na,nb,ns+1) as a (ns+1, nb, na, 3) array"""
         n = (g.ns + 1) * g.nb * g.na * 3
         return np.ctypeslib.as_array(g.gp, shape=(n,)).reshape(g.ns + 1, g.nb, g.na, 3).copy()
+
+    # -- rigid molecules --------------------------------------------------------------------
+    def rattle_s1(self, r, rold, p, mass, dt, dvdr, prm: RpmdParams):
+        """one bead: r, rold, p, dvdr of shape (na, 3); returns (r, p, dvdr, vir, err)"""
+        r, p, dvdr = _f64(r).copy(), _f64(p).copy(), _f64(dvdr).copy()
+        vir = np.zeros(9)
+        err = self.lib.orc_rattle_s1(_d(r), _d(_f64(rold)), _d(p), C.c_int(r.shape[0] // 3), _d(_f64(mass)),
+                                     C.c_double(dt), _d(dvdr), _d(vir), C.c_double(prm.reoh), C.c_double(prm.theta))
+        return r, p, dvdr, vir.reshape(3, 3).T.copy(), err
+
+    def rattle_s2(self, r, p, mass, dt, dvdr, prm: RpmdParams):
+        r, p, dvdr = _f64(r), _f64(p).copy(), _f64(dvdr).copy()
+        vir = np.zeros(9)
+        err = self.lib.orc_rattle_s2(_d(r), _d(p), C.c_int(r.shape[0] // 3), _d(_f64(mass)), C.c_double(dt),
+                                     _d(dvdr), _d(vir), C.c_double(prm.reoh), C.c_double(prm.theta))
+        return p, dvdr, vir.reshape(3, 3).T.copy(), err
+
+    def bead_forces(self, r, beta, mass):
+        r = _f64(r)
+        nb, na = r.shape[0], r.shape[1]
+        out = np.zeros_like(r)
+        self.lib.orc_bead_forces(_d(r), C.c_int(na), C.c_int(nb), _d(out), C.c_double(beta), _d(_f64(mass)))
+        return out
+
+    def evolve_rig_pi(self, prm: RpmdParams, p, r, dvdr, dvdr2, dt, mass, box, z, beta, vir=None,
+                      s: RngState | None = None):
+        p, r = _f64(p).copy(), _f64(r).copy()
+        dvdr, dvdr2 = _f64(dvdr).copy(), _f64(dvdr2).copy()
+        nb, na = r.shape[0], r.shape[1]
+        vir = np.zeros(9) if vir is None else _f64(np.asarray(vir).T).reshape(9).copy()
+        vir_lf = np.zeros(9)
+        v, vlf, vhf = C.c_double(), C.c_double(), C.c_double()
+        if s is None:
+            s = RngState()
+            self.lib.orc_rng_init(C.byref(s))
+        err = self.lib.orc_evolve_rig_pi(C.byref(prm), _d(p), _d(r), C.byref(v), C.byref(vlf), C.byref(vhf),
+                                         _d(dvdr), _d(dvdr2), C.c_double(dt), _d(_f64(mass)), C.c_int(na),
+                                         C.c_int(nb), _d(_f64(box)), _d(_f64(z)), C.c_double(beta), _d(vir),
+                                         _d(vir_lf), C.byref(s))
+        if err:
+            raise RuntimeError(f"orc_evolve_rig_pi error {err}")
+        return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, v_lf=vlf.value, v_hf=vhf.value,
+                    vir=vir.reshape(3, 3).T.copy(), vir_lf=vir_lf.reshape(3, 3).T.copy())
 
     def matrix_exp(self, M, j=15, k=15):
         M = _f64(np.asarray(M).T)  # column-major
@@ -365,6 +412,36 @@ class RefPath:
         self.lib._QMglePtherm_gle(_d(p), C.byref(dheat), _d(_f64(mass)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
                                   C.byref(self._idum))
         return p
+
+    # -- rigid molecules (src/md_rattle.f90, src/evolve_rigid.f90) ---------------------------
+    def rattle_setup(self, nm: int, prm: RpmdParams):
+        """rattle_setup (src/md_rattle.f90:1-31) reads /geometry/, so the parameters go first"""
+        self.set_params(prm)
+        self.lib.rattle_setup_(C.byref(C.c_int(nm)))
+
+    def rattle_s1(self, r, rold, p, mass, dt, dvdr):
+        r, p, dvdr = _f64(r).copy(), _f64(p).copy(), _f64(dvdr).copy()
+        na = r.shape[0]
+        vir = np.zeros(9)
+        self.lib.rattle_s1_(_d(r), _d(_f64(rold).copy()), _d(p), C.byref(C.c_int(na // 3)), _d(_f64(mass)),
+                            C.byref(C.c_double(dt)), _d(dvdr), _d(vir), C.byref(C.c_int(na)))
+        return r, p, dvdr, vir.reshape(3, 3).T.copy()
+
+    def rattle_s2(self, r, p, mass, dt, dvdr):
+        r, p, dvdr = _f64(r).copy(), _f64(p).copy(), _f64(dvdr).copy()
+        na = r.shape[0]
+        vir = np.zeros(9)
+        self.lib.rattle_s2_(_d(r), _d(p), C.byref(C.c_int(na // 3)), _d(_f64(mass)), C.byref(C.c_double(dt)),
+                            _d(dvdr), _d(vir), C.byref(C.c_int(na)))
+        return p, dvdr, vir.reshape(3, 3).T.copy()
+
+    def bead_forces(self, r, beta, mass):
+        r = _f64(r).copy()
+        nb, na = r.shape[0], r.shape[1]
+        out = np.zeros_like(r)
+        self.lib.bead_forces_(_d(r), C.byref(C.c_int(na)), C.byref(C.c_int(nb)), _d(out), C.byref(C.c_double(beta)),
+                              _d(_f64(mass)))
+        return out
 
     def matrix_exp(self, M, j=15, k=15):
         M = np.asfortranarray(M, dtype=np.float64)

@@ -1935,3 +1935,228 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
     free(dvdre); free(rst); free(dvdrl); free(monod);
     return err;
 }
+
+/* ======================================================================================
+ * Rigid molecules: RATTLE (src/md_rattle.f90) and evolve_rig_pi (src/evolve_rigid.f90)
+ * ==================================================================================== */
+
+/* rattle_setup src/md_rattle.f90:1-31: per molecule the bonds (O,H1), (O,H2), (H1,H2) with
+ * squared lengths reoh^2, reoh^2, rehh^2, rehh = 2 sin(theta/2) reoh */
+static void rattle_bonds(double reoh, double theta, double rcb[3], int bi[3], int bj[3]) {
+    double rehh = 2.0 * sin(0.5 * theta) * reoh;
+    rcb[0] = reoh * reoh; bi[0] = 0; bj[0] = 1;
+    rcb[1] = reoh * reoh; bi[1] = 0; bj[1] = 2;
+    rcb[2] = rehh * rehh; bi[2] = 1; bj[2] = 2;
+}
+
+/* rattle_s1 src/md_rattle.f90:33-158 on one bead r(3,na); vir(3,3) column-major is zeroed first.
+ * The reference accumulates vir(1,3) as vir(1,2) - gdx dz2 (:124) and checks only the iteration
+ * count of the last molecule (:144-146); both are kept. */
+int orc_rattle_s1(double* r, const double* rold, double* p, int nm, const double* mass, double dt,
+                  double* dvdr, double* vir, double reoh, double theta) {
+    const double tol = 1e-6;
+    const int itmax = 1000;
+    double rcb[3];
+    int bi[3], bj[3];
+    rattle_bonds(reoh, theta, rcb, bi, bj);
+    for (int m = 0; m < 9; m++) vir[m] = 0.0;
+#define V(i, j) vir[(j - 1) * 3 + (i - 1)]
+    int it = 0;
+    for (int ik = 0; ik < nm; ik++) {
+        int done = 0;
+        it = 0;
+        while (!done && it < itmax) {
+            done = 1;
+            for (int b = 0; b < 3; b++) {
+                int ii = 3 * ik + bi[b], jj = 3 * ik + bj[b];
+                double dx1 = RX(r, 0, jj) - RX(r, 0, ii);
+                double dy1 = RX(r, 1, jj) - RX(r, 1, ii);
+                double dz1 = RX(r, 2, jj) - RX(r, 2, ii);
+                double d2 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+                double diff = rcb[b] - d2;
+                if (fabs(diff) > rcb[b] * tol) {
+                    double rm1 = 1.0 / mass[jj], rm2 = 1.0 / mass[ii];
+                    double dx2 = RX(rold, 0, jj) - RX(rold, 0, ii);
+                    double dy2 = RX(rold, 1, jj) - RX(rold, 1, ii);
+                    double dz2 = RX(rold, 2, jj) - RX(rold, 2, ii);
+                    double dot = dx1 * dx2 + dy1 * dy2 + dz1 * dz2;
+                    double gamma = diff / (2.0 * (rm1 + rm2) * dot);
+                    double gdx = gamma * dx2, gdy = gamma * dy2, gdz = gamma * dz2;
+                    RX(r, 0, jj) = RX(r, 0, jj) + rm1 * gdx;
+                    RX(r, 1, jj) = RX(r, 1, jj) + rm1 * gdy;
+                    RX(r, 2, jj) = RX(r, 2, jj) + rm1 * gdz;
+                    RX(r, 0, ii) = RX(r, 0, ii) - rm2 * gdx;
+                    RX(r, 1, ii) = RX(r, 1, ii) - rm2 * gdy;
+                    RX(r, 2, ii) = RX(r, 2, ii) - rm2 * gdz;
+                    gdx = gdx / dt; gdy = gdy / dt; gdz = gdz / dt;
+                    RX(p, 0, jj) = RX(p, 0, jj) + gdx;
+                    RX(p, 1, jj) = RX(p, 1, jj) + gdy;
+                    RX(p, 2, jj) = RX(p, 2, jj) + gdz;
+                    RX(p, 0, ii) = RX(p, 0, ii) - gdx;
+                    RX(p, 1, ii) = RX(p, 1, ii) - gdy;
+                    RX(p, 2, ii) = RX(p, 2, ii) - gdz;
+                    gdx = 2.0 * gdx; gdy = 2.0 * gdy; gdz = 2.0 * gdz;
+                    V(1, 1) = V(1, 1) - gdx * dx2;
+                    V(1, 2) = V(1, 2) - gdx * dy2;
+                    V(1, 3) = V(1, 2) - gdx * dz2;
+                    V(2, 3) = V(2, 3) - gdy * dz2;
+                    V(2, 2) = V(2, 2) - gdy * dy2;
+                    V(3, 3) = V(3, 3) - gdz * dz2;
+                    RX(dvdr, 0, jj) = RX(dvdr, 0, jj) + gdx;
+                    RX(dvdr, 1, jj) = RX(dvdr, 1, jj) + gdy;
+                    RX(dvdr, 2, jj) = RX(dvdr, 2, jj) + gdz;
+                    RX(dvdr, 0, ii) = RX(dvdr, 0, ii) - gdx;
+                    RX(dvdr, 1, ii) = RX(dvdr, 1, ii) - gdy;
+                    RX(dvdr, 2, ii) = RX(dvdr, 2, ii) - gdz;
+                    done = 0;
+                }
+            }
+            it = it + 1;
+        }
+    }
+    int err = (nm > 0 && it >= itmax) ? RPMD_ERR_RATTLE : 0;
+    V(2, 1) = V(1, 2);
+    V(3, 1) = V(1, 3);
+    V(3, 2) = V(2, 3);
+    return err;
+}
+
+/* rattle_s2 src/md_rattle.f90:160-271 (velocity stage, tolerance on gamma, it <= itmax) */
+int orc_rattle_s2(const double* r, double* p, int nm, const double* mass, double dt, double* dvdr,
+                  double* vir, double reoh, double theta) {
+    const double tol = 1e-6;
+    const int itmax = 1000;
+    const double halfdt = 0.5 * dt;
+    double rcb[3];
+    int bi[3], bj[3];
+    rattle_bonds(reoh, theta, rcb, bi, bj);
+    for (int m = 0; m < 9; m++) vir[m] = 0.0;
+    int it = 0;
+    for (int ik = 0; ik < nm; ik++) {
+        int done = 0;
+        it = 0;
+        while (!done && it <= itmax) {
+            done = 1;
+            for (int b = 0; b < 3; b++) {
+                int ii = 3 * ik + bi[b], jj = 3 * ik + bj[b];
+                double dx1 = RX(r, 0, jj) - RX(r, 0, ii);
+                double dy1 = RX(r, 1, jj) - RX(r, 1, ii);
+                double dz1 = RX(r, 2, jj) - RX(r, 2, ii);
+                double rm1 = 1.0 / mass[jj], rm2 = 1.0 / mass[ii];
+                double dvx1 = RX(p, 0, jj) * rm1 - RX(p, 0, ii) * rm2;
+                double dvy1 = RX(p, 1, jj) * rm1 - RX(p, 1, ii) * rm2;
+                double dvz1 = RX(p, 2, jj) * rm1 - RX(p, 2, ii) * rm2;
+                double dot = dx1 * dvx1 + dy1 * dvy1 + dz1 * dvz1;
+                double gamma = -dot / ((rm1 + rm2) * rcb[b]);
+                if (fabs(gamma) > tol) {
+                    double gdx = gamma * dx1, gdy = gamma * dy1, gdz = gamma * dz1;
+                    RX(p, 0, jj) = RX(p, 0, jj) + gdx;
+                    RX(p, 1, jj) = RX(p, 1, jj) + gdy;
+                    RX(p, 2, jj) = RX(p, 2, jj) + gdz;
+                    RX(p, 0, ii) = RX(p, 0, ii) - gdx;
+                    RX(p, 1, ii) = RX(p, 1, ii) - gdy;
+                    RX(p, 2, ii) = RX(p, 2, ii) - gdz;
+                    gdx = gdx / halfdt; gdy = gdy / halfdt; gdz = gdz / halfdt;
+                    V(1, 1) = V(1, 1) - gdx * dx1;
+                    V(1, 2) = V(1, 2) - gdx * dy1;
+                    V(1, 3) = V(1, 2) - gdx * dz1;
+                    V(2, 3) = V(2, 3) - gdy * dz1;
+                    V(2, 2) = V(2, 2) - gdy * dy1;
+                    V(3, 3) = V(3, 3) - gdz * dz1;
+                    RX(dvdr, 0, jj) = RX(dvdr, 0, jj) + gdx;
+                    RX(dvdr, 1, jj) = RX(dvdr, 1, jj) + gdy;
+                    RX(dvdr, 2, jj) = RX(dvdr, 2, jj) + gdz;
+                    RX(dvdr, 0, ii) = RX(dvdr, 0, ii) - gdx;
+                    RX(dvdr, 1, ii) = RX(dvdr, 1, ii) - gdy;
+                    RX(dvdr, 2, ii) = RX(dvdr, 2, ii) - gdz;
+                    done = 0;
+                }
+            }
+            it = it + 1;
+        }
+    }
+    int err = (nm > 0 && it >= itmax) ? RPMD_ERR_RATTLE : 0;
+    V(2, 1) = V(1, 2);
+    V(3, 1) = V(1, 3);
+    V(3, 2) = V(2, 3);
+#undef V
+    return err;
+}
+
+/* bead_forces src/evolve_rigid.f90:236-278: free ring-polymer spring gradient in bead space,
+ * m omega_n^2 (2 x_k - x_{k-1} - x_{k+1}), omega_n = nb / beta (hbar = 1); zero for nb = 1 */
+void orc_bead_forces(const double* r, int na, int nb, double* dvdrb, double beta, const double* mass) {
+    size_t A = (size_t)3 * na;
+    double omegan = (double)nb / (beta * 1.0);
+    omegan = omegan * omegan;
+    for (size_t i = 0; i < A * nb; i++) dvdrb[i] = 0.0;
+    if (nb <= 1) return;
+    for (int i = 0; i < na; i++)
+        for (int k = 0; k < nb; k++) {
+            int k2 = (k == 0) ? nb - 1 : k - 1, k3 = (k == nb - 1) ? 0 : k + 1;
+            for (int c = 0; c < 3; c++) {
+                double dx1 = r[k * A + 3 * i + c], dx2 = r[k2 * A + 3 * i + c], dx3 = r[k3 * A + 3 * i + c];
+                dvdrb[k * A + 3 * i + c] = mass[i] * omegan * (2.0 * dx1 - dx2 - dx3);
+            }
+        }
+}
+
+/* evolve_rig_pi src/evolve_rigid.f90:100-234 (nbaro = 0).  The reference adds dvdr_corr, v_corr
+ * and vir_corr, which it never sets there (the correction call is commented out, :178-180); they
+ * are taken as zero, as evolve_rig_cl sets them (:19-21).  vir is in/out: the reference adds the
+ * incoming value (:208).  nctot = 3 nm constraints enter the Bussi degrees of freedom. */
+int orc_evolve_rig_pi(const rpmd_params* P, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                      double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                      const double box[3], const double* z, double beta, double* vir, double* vir_lf,
+                      orc_rng* s) {
+    size_t A = (size_t)3 * na, N = A * nb;
+    int nm = na / 3, nstep = nb, err = 0;
+    int nctot = 3 * nm;
+    double halfdt = 0.5 * dt, smalldt = dt / (double)nstep, halfsmalldt = 0.5 * smalldt;
+    double* rold = (double*)malloc(sizeof(double) * A);
+    double vir_hf[9] = {0}, vir_tmp[9];
+    if (P->therm == RPMD_THERM_PRG) orc_parinello_therm(p, mass, P->ttau, na, nb, halfdt, s, beta, nctot);
+    else if (P->therm == RPMD_THERM_PRL) orc_parinello_therm_loc(p, mass, P->ttau, na, nb, halfdt, s, beta);
+    for (size_t i = 0; i < N; i++) p[i] = p[i] - halfdt * dvdr[i];
+    orc_bead_forces(r, na, nb, dvdr2, beta, mass);
+    for (int is = 0; is < nstep && !err; is++) {
+        for (int k = 0; k < nb && !err; k++) {
+            double *rk = r + k * A, *pk = p + k * A, *dk = dvdr2 + k * A;
+            for (int j = 0; j < na; j++)
+                for (int i = 0; i < 3; i++) {
+                    RX(rold, i, j) = RX(rk, i, j);
+                    RX(pk, i, j) = RX(pk, i, j) - halfsmalldt * RX(dk, i, j);
+                    RX(rk, i, j) = RX(rk, i, j) + RX(pk, i, j) * smalldt / mass[j];
+                }
+            err = orc_rattle_s1(rk, rold, pk, nm, mass, smalldt, dk, vir_hf, P->reoh, P->theta);
+        }
+        if (err) break;
+        orc_bead_forces(r, na, nb, dvdr2, beta, mass);
+        for (size_t i = 0; i < N; i++) p[i] = p[i] - halfsmalldt * dvdr2[i];
+        for (int k = 0; k < nb && !err; k++)
+            err = orc_rattle_s2(r + k * A, p + k * A, nm, mass, smalldt, dvdr2 + k * A, vir_hf, P->reoh,
+                                P->theta);
+    }
+    if (!err) err = orc_forces(P, r, nb, na, box, z, 1, v, dvdr, vir_lf);
+    if (!err) {
+        for (size_t i = 0; i < N; i++) p[i] = p[i] - halfdt * dvdr[i];
+        for (int m = 0; m < 9; m++) vir_hf[m] = 0.0;
+        for (int k = 0; k < nb && !err; k++) {
+            err = orc_rattle_s2(r + k * A, p + k * A, nm, mass, dt, dvdr + k * A, vir_tmp, P->reoh, P->theta);
+            for (int m = 0; m < 9; m++) vir_hf[m] = vir_hf[m] + vir_tmp[m];
+        }
+    }
+    if (!err) {
+        if (P->therm == RPMD_THERM_PRG) orc_parinello_therm(p, mass, P->ttau, na, nb, halfdt, s, beta, nctot);
+        else if (P->therm == RPMD_THERM_PRL) orc_parinello_therm_loc(p, mass, P->ttau, na, nb, halfdt, s, beta);
+        for (int m = 0; m < 9; m++) {
+            vir_hf[m] = vir_hf[m] / (double)nb;
+            vir[m] = vir_lf[m] + vir_hf[m] + vir[m];
+        }
+        for (size_t i = 0; i < N; i++) dvdr2[i] = 0.0;
+        *v_lf = *v;
+        *v_hf = 0.0;
+    }
+    free(rold);
+    return err;
+}

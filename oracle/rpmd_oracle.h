@@ -74,6 +74,16 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
                   const double* mass, int na, int nb, const double box[3], const double* z,
                   double beta, double* vir, double* vir_lf, orc_rng* s, orc_gle* g);
 
+int orc_rattle_s1(double* r, const double* rold, double* p, int nm, const double* mass, double dt,
+                  double* dvdr, double* vir, double reoh, double theta);
+int orc_rattle_s2(const double* r, double* p, int nm, const double* mass, double dt, double* dvdr,
+                  double* vir, double reoh, double theta);
+void orc_bead_forces(const double* r, int na, int nb, double* dvdrb, double beta, const double* mass);
+int orc_evolve_rig_pi(const rpmd_params* P, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                      double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                      const double box[3], const double* z, double beta, double* vir, double* vir_lf,
+                      orc_rng* s);
+
 #ifdef __cplusplus
 }
 #endif

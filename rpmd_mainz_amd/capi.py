@@ -37,6 +37,7 @@ class RpmdParams(C.Structure):
         ("sig", C.c_double),
         ("nbdf1", C.c_int), ("nbdf2", C.c_int), ("mts", C.c_int),
         ("intra_model", C.c_int), ("therm", C.c_int), ("ttau", C.c_double),
+        ("rigid", C.c_int),
     ]
 
     @classmethod
@@ -66,7 +67,7 @@ def lib() -> C.CDLL:
         for name in ("rpmd_setup_ewald", "rpmd_ctx_create", "rpmd_ctx_set_params", "rpmd_ctx_set_precision",
                      "rpmd_ctx_destroy", "rpmd_forces", "rpmd_full_forces", "rpmd_realft", "rpmd_evolve_pi",
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
-                     "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init"):
+                     "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -182,6 +183,30 @@ class RPMDContext:
                                     _p(vir), _p(vir_lf), npt, nn))
         return dict(v=v.value, vew=vew.value, vlj=vlj.value, vint=vint.value, vir=vir.reshape(3, 3).T.copy(),
                     vir_lf=vir_lf.reshape(3, 3).T.copy())
+
+    # -- subroutine evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,
+    #                             vir_lf,irun,nbaro)  src/evolve_rigid.f90:100 ---------------------
+    def evolve_rig_pi(self, p, r, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir=None, noise=None):
+        """One rigid step; p, r, dvdr, dvdr2 updated in place; vir (3x3) in/out as in the reference."""
+        for a in (p, r, dvdr, dvdr2):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.shape == (nb, na, 3)):
+                raise TypeError("evolve_rig_pi state arrays must be C-contiguous float64 (nb, na, 3)")
+        w = np.zeros(9) if vir is None else np.ascontiguousarray(np.asarray(vir, dtype=np.float64).T).reshape(9)
+        w = w.copy()
+        wl = np.zeros(9)
+        v, vlf, vhf = C.c_double(), C.c_double(), C.c_double()
+        if noise is not None:
+            noise = _arr(noise)
+            npt, nn = _p(noise), C.c_longlong(noise.size)
+        else:
+            npt, nn = None, C.c_longlong(0)
+        _check(lib().rpmd_evolve_rig_pi(self._h, _p(p), _p(r), C.byref(v), C.byref(vlf), C.byref(vhf), _p(dvdr),
+                                        _p(dvdr2), C.c_double(dt), _p(_arr(mass, (na,))), C.c_int(na), C.c_int(nb),
+                                        _p(_arr(boxlxyz, (3,))), _p(_arr(z, (na,))), C.c_double(beta), _p(w),
+                                        _p(wl), npt, nn))
+        return dict(v=v.value, v_lf=vlf.value, v_hf=vhf.value, vir=w.reshape(3, 3).T.copy(),
+                    vir_lf=wl.reshape(3, 3).T.copy())
 
     # -- subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)  src/module_gle.f90:146 ---------------
     def gle_init(self, A, na: int, nb: int, dt: float, beta: float, gp=None, seed: int = 0):

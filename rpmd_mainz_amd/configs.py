@@ -98,7 +98,7 @@ def masses_charges(model: dict, na: int):
 
 def make_params(model: dict, box, rcut_A: float = 9.0, ecut: float = 0.0, iamcub: bool = True,
                 intra_model: int = INTRA_AUTO, sig_A: float = 0.0, nbdf1: int = 0, nbdf2: int = 0,
-                mts: int = 1, therm: str = "NONE", ttaufs: float = 0.0) -> dict:
+                mts: int = 1, therm: str = "NONE", ttaufs: float = 0.0, rigid: bool = False) -> dict:
     """The COMMON-block values of src/main.f90 for a model, before setup_ewald."""
     theta = model["thetad"] * (math.pi / 180.0)
     rcut = rcut_A / TOA
@@ -109,7 +109,7 @@ def make_params(model: dict, box, rcut_A: float = 9.0, ecut: float = 0.0, iamcub
                 apot=model["apot"], bpot=model["bpot"], alp=model["alp"], alpb=model["alpb"],
                 theta=theta, reoh=model["reoh"], sig=sig_A / TOA if sig_A > 0 else 0.0,
                 nbdf1=nbdf1, nbdf2=nbdf2, mts=mts, intra_model=intra_model,
-                therm=THERM[therm], ttau=TOFS * ttaufs)
+                therm=THERM[therm], ttau=TOFS * ttaufs, rigid=1 if rigid else 0)
 
 
 def setup_ewald(na: int, box, prm: dict) -> dict:
@@ -140,9 +140,13 @@ def setup_ewald(na: int, box, prm: dict) -> dict:
     return prm
 
 
-def thermal_beads(r0: np.ndarray, nb: int, amp: float, seed: int) -> np.ndarray:
-    """(nb, na, 3) bead positions: r0 plus a seeded Gaussian displacement per bead (test inputs)."""
+def thermal_beads(r0: np.ndarray, nb: int, amp: float, seed: int, rigid: bool = False) -> np.ndarray:
+    """(nb, na, 3) bead positions: r0 plus a seeded Gaussian displacement per bead (test inputs).
+    rigid: one displacement per molecule and bead, so every bead keeps the molecular geometry."""
     rng = np.random.default_rng(seed)
+    if rigid:
+        d = amp * rng.normal(size=(nb, r0.shape[0] // 3, 1, 3))
+        return r0[None, :, :] + np.repeat(d, 3, axis=2).reshape((nb,) + r0.shape)
     return r0[None, :, :] + amp * rng.normal(size=(nb,) + r0.shape)
 
 
@@ -152,12 +156,15 @@ def water_system(model_name: str, ncell, nb: int, rho: float = 0.997, seed: int 
     model = MODELS[model_name]
     nm = int(np.prod(ncell))
     box = box_size(nm, rho, model["wmass"], ncell, aspect)
+    if np.allclose(box, box[0], rtol=1e-12, atol=0.0):
+        box = np.full(3, box[0])          # e.g. an 8x8x16 lattice in a cubic box (aspect 2:2:1)
     theta = model["thetad"] * math.pi / 180.0
     r0 = water_lattice(ncell, box, model["reoh"], theta, seed)
-    r = thermal_beads(r0, nb, amp, seed + 1000) if amp > 0 else np.repeat(r0[None], nb, 0)
+    r = (thermal_beads(r0, nb, amp, seed + 1000, bool(kw.get("rigid", False))) if amp > 0
+         else np.repeat(r0[None], nb, 0))
     na = r0.shape[0]
     mass, z = masses_charges(model, na)
-    cub = aspect[0] == aspect[1] == aspect[2]
+    cub = bool(box[0] == box[1] == box[2])
     prm = make_params(model, box, iamcub=cub, **kw)
     prm = setup_ewald(na, box, prm)
     return r, box, z, mass, prm

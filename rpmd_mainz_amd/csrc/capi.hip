@@ -90,6 +90,9 @@ struct rpmd_ctx {
     double *bussi_parts = nullptr, *bussi_alpha = nullptr;
     double *gle_gp = nullptr, *gle_TS = nullptr, *gle_parts = nullptr, *gle_pcom = nullptr;
     int gle_n1 = 0, gle_nb = 0;
+    // rigid molecules: RATTLE failure flag, virial partials
+    int* rat_err = nullptr;
+    double* rig_vparts = nullptr;
     std::map<int, NMMat> nm;
     // host staging (pinned)
     double* h_stage = nullptr;
@@ -717,6 +720,71 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     return 0;
 }
 
+// one rigid RPMD step, evolve_rig_pi (src/evolve_rigid.f90:100-234, nbaro = 0), on the context's
+// device state.  The reference's dvdr_corr, v_corr and vir_corr are never set there; they are zero
+// here.  ev_step[0..49]: the forces(iopt = 1) slot; ev_step[50..58]: vir_hf (column-major).
+int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nb = c->st_nb, nm = na / 3;
+    const size_t N = (size_t)3 * na * nb;
+    hipStream_t s = c->st;
+    const double dt = c->st_dt, halfdt = 0.5 * dt, smalldt = dt / (double)nb, halfsmalldt = 0.5 * smalldt;
+    const double beta = c->st_beta, betan = beta / (double)nb;
+    const double* box = c->st_box.L;
+    double omegan = (double)nb / (beta * 1.0);
+    omegan = omegan * omegan;
+    // rattle_setup (src/md_rattle.f90:14-28)
+    const double rehh = 2.0 * sin(0.5 * P.theta) * P.reoh;
+    const double rcb[3] = {P.reoh * P.reoh, P.reoh * P.reoh, rehh * rehh};
+    auto thermo = [&]() {
+        if (P.therm == RPMD_THERM_PRG) {
+            // parinello_therm with nctot = 3 nm constraints (src/md_rattle.f90:29)
+            const long long ndof = 3LL * na * nb - 3LL * nm - 3;
+            launch_bussi(c->p, c->mass, na, nb, ndof, betan, exp(-halfdt / P.ttau), nz.take((size_t)ndof + 1),
+                         c->seed, c->step_ctr, nz.stream_add++, c->bussi_parts, c->bussi_alpha, nullptr, s);
+        } else if (P.therm == RPMD_THERM_PRL) {
+            const double gamma = 1.0 / (2.0 * P.ttau), c1 = exp(-0.5 * gamma * halfdt);
+            launch_langevin(c->p, c->lang_c3m, c1, na, nb, nz.take(N), c->seed, c->step_ctr, nz.stream_add++, s);
+        }
+    };
+    thermo();
+    launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
+    launch_bead_forces(c->r, c->dvdr2, c->mass, na, nb, omegan, s);
+    for (int is = 0; is < nb; is++)
+        launch_rig_substep(c->p, c->r, c->dvdr2, c->mass, na, nb, halfsmalldt, smalldt, omegan, rcb, c->rat_err, s);
+    RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step, check));
+    launch_rig_final(c->p, c->r, c->dvdr, c->mass, na, nb, halfdt, dt, rcb, c->rat_err, c->rig_vparts,
+                     c->ev_step + 50, s);
+    thermo();
+    launch_zero(c->dvdr2, N, s);
+    launch_counter_inc(c->step_ctr, s);
+    if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
+    if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
+    return 0;
+}
+
+// energies of the last rigid step: v = v_lf = forces(iopt = 1) energy, v_hf = 0,
+// vir_lf, and vir_lf + vir_hf (src/evolve_rigid.f90:203-229)
+int step_rig_energies(rpmd_ctx* c, double* v, double* vir_lfhf, double* vir_lf) {
+    double h[60];
+    HIPCHK(hipMemcpyAsync(h, c->ev_step, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double w[9];
+    combine_ev(c, h, 1, c->st_box.L, v, w);
+    for (int k = 0; k < 9; k++) {
+        if (vir_lf) vir_lf[k] = w[k];
+        if (vir_lfhf) vir_lfhf[k] = w[k] + h[50 + k];
+    }
+    return 0;
+}
+
+int rattle_status(rpmd_ctx* c) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->rat_err, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return h ? RPMD_ERR_RATTLE : 0;
+}
+
 // energies of the last step (src/evolve_pi.f90:166-169, 292-299)
 int step_energies(rpmd_ctx* c, double* v, double* vew, double* vlj, double* vint, double* vir, double* vir_lf) {
     const rpmd_params& P = c->prm;
@@ -772,8 +840,9 @@ int set_state_consts(rpmd_ctx* c, const double* mass, double beta, double dt, in
     HIPCHK(hipMemcpyAsync(c->pile_c1, c1.data(), c1.size() * 8, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->pile_c3, c3.data(), c3.size() * 8, hipMemcpyHostToDevice, c->st));
     std::vector<double> c3m(na);
-    const double gamma = 1.0 / (2.0 * P.ttau), betan = beta / (double)nb;
-    const double cc1 = exp(-0.5 * gamma * dt), c2 = sqrt(1.0 - cc1 * cc1);
+    // local Langevin: evolve_pi passes dt (src/evolve_pi.f90:75-77), evolve_rig_pi dt/2 (:126-130)
+    const double gamma = 1.0 / (2.0 * P.ttau), betan = beta / (double)nb, dtl = P.rigid ? 0.5 * dt : dt;
+    const double cc1 = exp(-0.5 * gamma * dtl), c2 = sqrt(1.0 - cc1 * cc1);
     for (int j = 0; j < na; j++) c3m[j] = c2 * sqrt(mass[j] / betan);
     HIPCHK(hipMemcpyAsync(c->lang_c3m, c3m.data(), c3m.size() * 8, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->mass, mass, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
@@ -783,6 +852,11 @@ int set_state_consts(rpmd_ctx* c, const double* mass, double beta, double dt, in
 
 size_t noise_per_step(const rpmd_params& P, int na, int nb, int ngle) {
     const size_t d = (size_t)3 * na * nb;
+    if (P.rigid) {
+        if (P.therm == RPMD_THERM_PRG) return 2 * (d - (size_t)na - 2);  // ndof + 1 = d - 3 nm - 2
+        if (P.therm == RPMD_THERM_PRL) return 2 * d;
+        return 0;
+    }
     if (P.therm == RPMD_THERM_PRO) return 2 * (size_t)P.mts * d;
     if (P.therm == RPMD_THERM_PRL) return 2 * d;
     if (P.therm == RPMD_THERM_PRG) return 2 * (d - 2);
@@ -918,6 +992,8 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->bussi_alpha, 1);
     e |= dalloc(&c->gle_TS, 128);
     e |= dalloc(&c->gle_pcom, 24);
+    e |= dalloc(&c->rat_err, 1);
+    e |= dalloc(&c->rig_vparts, (size_t)7 * rigid_blocks(na) * nbl);
     e |= dalloc(&c->ev_step, (size_t)(64 + 4) * 50);
     if (e) {
         rpmd_ctx_destroy(c);
@@ -951,7 +1027,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     void* bufs[] = {c->ph, c->fpart, c->bb, c->ccnt, c->clist, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
-                    c->gle_parts, c->gle_pcom, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+                    c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
@@ -1117,6 +1193,57 @@ int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, do
     return 0;
 }
 
+int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_lf, double* v_hf, double* dvdr,
+                       double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
+                       const double* z, double beta, double* vir, double* vir_lf, const double* noise,
+                       long long n_noise) {
+    if (!c || !p || !r || !dvdr || !dvdr2 || !mass || !z || !boxlxyz || !vir) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    if (!c->prm.rigid) return RPMD_ERR_ARG;
+    const size_t need = noise_per_step(c->prm, na, nb, 0);
+    if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
+    RETCHK(upload_charges(c, z));
+    c->st_nb = nb;
+    c->st_box = make_box(boxlxyz);
+    c->st_beta = beta;
+    c->st_dt = dt;
+    RETCHK(set_state_consts(c, mass, beta, dt, nb));
+    RETCHK(upload_aos(c, p, c->p, nb));
+    RETCHK(upload_aos(c, r, c->r, nb));
+    RETCHK(upload_aos(c, dvdr, c->dvdr, nb));
+    launch_zero_int(c->rat_err, 1, c->st);
+    NoiseCursor nz{nullptr, 0, 0, 0};
+    if (noise && need) {
+        if (c->noise_cap < need) {
+            if (c->noise) hipFree(c->noise);
+            c->noise = nullptr;
+            RETCHK(dalloc(&c->noise, need));
+            c->noise_cap = need;
+        }
+        HIPCHK(hipMemcpyAsync(c->noise, noise, need * 8, hipMemcpyHostToDevice, c->st));
+        nz.base = c->noise;
+        nz.cap = need;
+    }
+    RETCHK(step_rig_dev(c, nz, true));
+    RETCHK(rattle_status(c));
+    RETCHK(download_aos(c, c->p, p, nb));
+    RETCHK(download_aos(c, c->r, r, nb));
+    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
+    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    double vv, w[9], wl[9];
+    RETCHK(step_rig_energies(c, &vv, w, wl));
+    if (v) *v = vv;
+    if (v_lf) *v_lf = vv;
+    if (v_hf) *v_hf = 0.0;
+    for (int k = 0; k < 9; k++) {
+        vir[k] = w[k] + vir[k];
+        if (vir_lf) vir_lf[k] = wl[k];
+    }
+    c->st_valid = false;
+    return 0;
+}
+
 int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const double* mass, const double* z, int na,
                       int nb, const double boxlxyz[3], double beta, double dt, unsigned long long seed) {
     if (!c || !p || !r || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
@@ -1132,6 +1259,7 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
     RETCHK(upload_aos(c, p, c->p, nb));
     RETCHK(upload_aos(c, r, c->r, nb));
     HIPCHK(hipMemsetAsync(c->step_ctr, 0, sizeof(uint64_t), c->st));
+    launch_zero_int(c->rat_err, 1, c->st);
     // initial forces: full_forces semantics for the uncontracted case (src/md_forces.f90:305-472)
     const size_t N = (size_t)3 * na * nb;
     RETCHK(forces_dev(c, c->r, nb, boxlxyz, 4, c->dvdr2, c->ev, true));
@@ -1199,24 +1327,28 @@ int rpmd_use_graph(rpmd_ctx* c, int enable) {
     return 0;
 }
 
+namespace {
+int step_any(rpmd_ctx* c, NoiseCursor& nz) { return c->prm.rigid ? step_rig_dev(c, nz, false) : step_dev(c, nz, false); }
+}  // namespace
+
 int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
     if (!c || !c->st_valid || nsteps < 0) return RPMD_ERR_ARG;
     if (c->prm.nbdf1 > 0 || c->prm.nbdf2 > 0) {
         // contraction maps are uploaded per call: run eagerly
         for (int i = 0; i < nsteps; i++) {
             NoiseCursor nz{nullptr, 0, 0, 0};
-            RETCHK(step_dev(c, nz, false));
+            RETCHK(step_any(c, nz));
         }
         HIPCHK(hipStreamSynchronize(c->st));
-        return 0;
+        return c->prm.rigid ? rattle_status(c) : 0;
     }
     if (!c->use_graph || c->prof) {
         for (int i = 0; i < nsteps; i++) {
             NoiseCursor nz{nullptr, 0, 0, 0};
-            RETCHK(step_dev(c, nz, false));
+            RETCHK(step_any(c, nz));
         }
         HIPCHK(hipStreamSynchronize(c->st));
-        return 0;
+        return c->prm.rigid ? rattle_status(c) : 0;
     }
     // graph path: one step per replay; the step swaps r/rst (and dvdr/tmp), so capture two steps
     // to return the buffers to their starting roles
@@ -1224,7 +1356,7 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < 2; i++) {
             NoiseCursor nz{nullptr, 0, 0, 0};
-            int rc = step_dev(c, nz, false);
+            int rc = step_any(c, nz);
             if (rc) {
                 hipGraph_t g;
                 hipStreamEndCapture(c->st, &g);
@@ -1240,14 +1372,14 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
     if (i < nsteps) {
         NoiseCursor nz{nullptr, 0, 0, 0};
         // an odd tail runs eagerly and leaves buffer roles swapped; the graph is rebuilt next time
-        RETCHK(step_dev(c, nz, false));
+        RETCHK(step_any(c, nz));
         hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
         hipGraphDestroy(c->graph);
         c->graph = nullptr;
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    return 0;
+    return c->prm.rigid ? rattle_status(c) : 0;
 }
 
 int rpmd_prof_begin(rpmd_ctx* c) {
@@ -1287,7 +1419,18 @@ int rpmd_state_download(rpmd_ctx* c, double* p, double* r, double* dvdr, double*
     if (r) RETCHK(download_aos(c, c->r, r, nb));
     if (dvdr) RETCHK(download_aos(c, c->dvdr, dvdr, nb));
     if (dvdr2) RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
-    if (energies) RETCHK(step_energies(c, &energies[0], &energies[1], &energies[2], &energies[3], vir, nullptr));
+    if (energies && c->prm.rigid) {
+        // v = vew + vlj of the forces(iopt = 1) call, vint = 0
+        double h[60];
+        HIPCHK(hipMemcpyAsync(h, c->ev_step, sizeof h, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        RETCHK(step_rig_energies(c, &energies[0], vir, nullptr));
+        energies[2] = h[20];
+        energies[1] = energies[0] - energies[2];
+        energies[3] = 0.0;
+    } else if (energies) {
+        RETCHK(step_energies(c, &energies[0], &energies[1], &energies[2], &energies[3], vir, nullptr));
+    }
     return 0;
 }
 

@@ -105,4 +105,13 @@ void launch_mode_copy(double* dst, const double* src, const int* map, int ndst, 
 void launch_scale(double* y, double a, size_t n, hipStream_t s);
 void launch_counter_inc(uint64_t* c, hipStream_t s);
 
+// ---- k_rigid.hip ------------------------------------------------------------------------------
+int rigid_blocks(int na);
+void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,
+                        hipStream_t s);
+void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
+                        double omegan2, const double rcb[3], int* err, hipStream_t s);
+void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
+                      double dt, const double rcb[3], int* err, double* vparts, double* vir_out, hipStream_t s);
+
 }  // namespace rpmd

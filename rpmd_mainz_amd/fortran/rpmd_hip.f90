@@ -24,6 +24,7 @@ module rpmd_hip
      integer(c_int) :: nbdf1, nbdf2, mts
      integer(c_int) :: intra_model, therm
      real(c_double) :: ttau
+     integer(c_int) :: rigid
   end type rpmd_params_t
 
   interface
@@ -63,6 +64,40 @@ module rpmd_hip
        real(c_double), intent(inout) :: data(*)
        integer(c_int), value :: m, n, mode
      end function c_realft
+     ! the whole step, src/evolve_pi.f90:1 (noise: c_null_ptr draws on the device)
+     integer(c_int) function c_evolve_pi(ctx, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, &
+          beta, vir, vir_lf, noise, n_noise) bind(C, name='rpmd_evolve_pi')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double) :: p(*), r(*), v, vew, vlj, vint, dvdr(*), dvdr2(*), vir(9), vir_lf(9)
+       real(c_double), value :: dt, beta
+       real(c_double), intent(in) :: mass(*), box(3), z(*)
+       integer(c_int), value :: na, nb
+       type(c_ptr), value :: noise
+       integer(c_long_long), value :: n_noise
+     end function c_evolve_pi
+     ! the rigid step, src/evolve_rigid.f90:100 (vir in/out as there)
+     integer(c_int) function c_evolve_rig_pi(ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box, z, &
+          beta, vir, vir_lf, noise, n_noise) bind(C, name='rpmd_evolve_rig_pi')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double) :: p(*), r(*), v, v_lf, v_hf, dvdr(*), dvdr2(*), vir(9), vir_lf(9)
+       real(c_double), value :: dt, beta
+       real(c_double), intent(in) :: mass(*), box(3), z(*)
+       integer(c_int), value :: na, nb
+       type(c_ptr), value :: noise
+       integer(c_long_long), value :: n_noise
+     end function c_evolve_rig_pi
+     ! therm_gle_init, src/module_gle.f90:146 (A in the GLE-A row order; gp: c_null_ptr draws)
+     integer(c_int) function c_gle_init(ctx, ns, a, na, nb, dt, beta, gp, seed) bind(C, name='rpmd_gle_init')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ns, na, nb
+       real(c_double), intent(in) :: a(*)
+       real(c_double), value :: dt, beta
+       type(c_ptr), value :: gp
+       integer(c_long_long), value :: seed
+     end function c_gle_init
      type(c_ptr) function c_strerror(code) bind(C, name='rpmd_strerror')
        import :: c_int, c_ptr
        integer(c_int), value :: code
@@ -82,7 +117,7 @@ contains
     real(8) oo_eps, oo_sig, oo_gam, rcut, alpha, ecut, wm, wh, wrcut, walpha, rkmax
     real(8) apot, bpot, alp, alpb, theta, reoh, sig
     integer kmax, nbdf1, nbdf2, mts
-    logical iamcub
+    logical iamcub, iamrigid
     common /oo_param/ oo_eps, oo_sig, oo_gam, rcut
     common /ew_param/ alpha, ecut, wm, wh
     common /ew_setup/ wrcut, walpha, rkmax, kmax
@@ -92,6 +127,7 @@ contains
     common /correct/ sig
     common /beaddiabatic/ nbdf1, nbdf2
     common /multiple_ts/ mts
+    common /structure/ iamrigid
     prm%oo_eps = oo_eps; prm%oo_sig = oo_sig; prm%oo_gam = oo_gam; prm%rcut = rcut
     prm%alpha = alpha; prm%ecut = ecut; prm%wm = wm; prm%wh = wh
     prm%wrcut = wrcut; prm%walpha = walpha; prm%rkmax = rkmax; prm%kmax = kmax
@@ -102,6 +138,7 @@ contains
     prm%intra_model = 0          ! RPMD_INTRA_AUTO: the reference's own selection rule
     prm%therm = 0
     prm%ttau = 0.d0
+    prm%rigid = merge(1, 0, iamrigid)
   end subroutine rpmd_fill_params
 
   subroutine rpmd_check(code)

@@ -278,6 +278,101 @@ def test_device_resident_steps_match_oracle(capi, oracle):
     ctx.close()
 
 
+def _rigid_case(therm="NONE", ncell=(3, 3, 3), nb=4, seed=1, aspect=(1, 1, 1)):
+    r, box, z, mass, prm = cf.water_system("tip4p2005", ncell, nb, ecut=1e-6, rigid=True, therm=therm,
+                                           ttaufs=50.0, seed=seed, amp=0.1, aspect=aspect)
+    return r, box, z, mass, prm
+
+
+def _rigid_start(oracle, P, r, mass, beta, dt):
+    nb, na = r.shape[:2]
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(11))
+    for k in range(nb):
+        p[k] = oracle.rattle_s2(r[k], p[k], mass, dt, np.zeros((na, 3)), P)[0]
+    return p
+
+
+def _bond_residual(r, prm):
+    rehh = 2.0 * math.sin(0.5 * prm["theta"]) * prm["reoh"]
+    m = r.reshape(r.shape[0], -1, 3, 3)
+    out = 0.0
+    for (i, j, d0) in ((0, 1, prm["reoh"]), (0, 2, prm["reoh"]), (1, 2, rehh)):
+        v = m[:, :, j] - m[:, :, i]
+        out = max(out, float(np.abs((v * v).sum(-1) - d0 * d0).max()) / (d0 * d0))
+    return out
+
+
+@pytest.mark.parametrize("therm", ["NONE", "PRG", "PRL"])
+def test_evolve_rig_pi_matches_oracle(capi, oracle, therm):
+    """Rigid TIP4P/2005 RPMD steps (RATTLE on the device, FP64) with the reference's ran2 stream
+    replayed, against the oracle's evolve_rig_pi."""
+    r, box, z, mass, prm = _rigid_case(therm)
+    nb, na = r.shape[:2]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    p = _rigid_start(oracle, P, r, mass, beta, dt)
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    s_or, s_gp = oracle.rng(57), oracle.rng(57)
+    nm = na // 3
+    per_step = {"NONE": 0, "PRG": 2 * (3 * na * nb - 3 * nm - 2), "PRL": 2 * 3 * na * nb}[therm]
+    o = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], vir=np.zeros((3, 3)))
+    pg, rg, dg1, dg2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    vg = np.zeros((3, 3))
+    for k in range(3):
+        print(f"case rigid {therm} step {k}", flush=True)
+        noise = oracle.gaussians(s_gp, per_step) if per_step else None
+        o = oracle.evolve_rig_pi(P, o["p"], o["r"], o["dvdr"], o["dvdr2"], dt, mass, box, z, beta, o["vir"], s_or)
+        g = ctx.evolve_rig_pi(pg, rg, dg1, dg2, dt, mass, na, nb, box, z, beta, vir=vg, noise=noise)
+        vg = g["vir"]
+        assert abs(g["v"] - o["v"]) < 1e-10 * _escale(prm, box, z, o["v"])
+        assert np.abs(vg - o["vir"]).max() < 1e-9 * np.abs(o["vir"]).max()
+    assert _rel(rg, o["r"]) < 1e-12
+    assert _rel(pg, o["p"]) < 1e-9
+    assert _rel(dg1, o["dvdr"]) < 1e-9
+    assert _bond_residual(rg, prm) <= 1.0001e-6
+    ctx.close()
+
+
+def test_rigid_device_resident_config5_shape(capi, oracle):
+    """The rigid TIP4P/2005 workload shape (1024 molecules, 32 beads, M-site Ewald, mixed-precision
+    forces, RATTLE in FP64) stepped in HBM with a hipGraph: every constraint holds to the RATTLE
+    tolerance and the bead-averaged energy stays finite; a 125-molecule FP64 run agrees with the
+    oracle after 4 steps."""
+    r, box, z, mass, prm = _rigid_case("PRG", ncell=(8, 8, 16), nb=32, aspect=(2, 2, 1))
+    assert prm["iamcub"] == 1 and abs(box[0] * cf.TOA - 31.318) < 0.01
+    nb, na = r.shape[:2]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    P = RpmdParams.from_dict(prm)
+    p = _rigid_start(oracle, P, r, mass, beta, dt)
+    ctx = capi.RPMDContext(prm, na, nb, "mixed")
+    ctx.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=9)
+    ctx.use_graph(True)
+    ctx.run_steps(10)
+    st = ctx.state_download()
+    assert np.isfinite(st["v"]) and np.isfinite(st["p"]).all()
+    assert _bond_residual(st["r"], prm) <= 1.0001e-6
+    ctx.close()
+    # FP64, small: device-resident steps against the oracle
+    r, box, z, mass, prm = _rigid_case("NONE", ncell=(5, 5, 5), nb=4)
+    nb, na = r.shape[:2]
+    P = RpmdParams.from_dict(prm)
+    p = _rigid_start(oracle, P, r, mass, beta, dt)
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    ctx.state_upload(p, r, mass, z, na, nb, box, beta, dt)
+    ctx.use_graph(True)
+    ctx.run_steps(4)
+    g = ctx.state_download()
+    o = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], vir=np.zeros((3, 3)))
+    for _ in range(4):
+        o = oracle.evolve_rig_pi(P, o["p"], o["r"], o["dvdr"], o["dvdr2"], dt, mass, box, z, beta, o["vir"])
+    assert _rel(g["r"], o["r"]) < 1e-12
+    assert _rel(g["p"], o["p"]) < 1e-9
+    assert abs(g["v"] - o["v"]) < 1e-10 * _escale(prm, box, z, o["v"])
+    ctx.close()
+
+
 def test_error_paths(capi):
     r, box, z, mass, prm = cf.water_system("qspcfw", (2, 2, 2), 2)
     ctx = capi.RPMDContext(prm, 24, 2, "fp64")

@@ -297,3 +297,146 @@ def test_ewald_madelung_nacl(oracle):
     madelung = -2.0 * e / len(q)
     assert abs(madelung - 1.747564594633182) < 2e-6
     assert np.abs(dvdr[0].sum(axis=0)).max() < 1e-10   # translation invariance: sum of gradients is 0
+
+
+# ---- rigid molecules: RATTLE and the rigid RPMD step ----------------------------------------------
+def _rigid(ncell=(3, 3, 3), nb=4, therm="NONE", seed=1):
+    r, box, z, mass, prm = cf.water_system("tip4p2005", ncell, nb, ecut=1e-6, rigid=True, therm=therm,
+                                           ttaufs=50.0, seed=seed, amp=0.1)
+    return r, box, z, mass, RpmdParams.from_dict(prm)
+
+
+def _bond_residuals(r, prm):
+    """|d^2 - d0^2| / d0^2 of the three constraints of every molecule of one bead"""
+    rehh = 2.0 * math.sin(0.5 * prm.theta) * prm.reoh
+    m = r.reshape(-1, 3, 3)
+    d = [(m[:, 1] - m[:, 0], prm.reoh), (m[:, 2] - m[:, 0], prm.reoh), (m[:, 2] - m[:, 1], rehh)]
+    return max(float(np.abs((v * v).sum(-1) - d0 * d0).max()) / (d0 * d0) for v, d0 in d)
+
+
+def test_rattle_matches_reference(oracle, refpath):
+    """rattle_s1 / rattle_s2 (src/md_rattle.f90) against the reference's Fortran, bitwise, and the
+    properties they must have: stage 1 restores every bond length to the 1e-6 relative tolerance;
+    stage 2 removes the relative velocity along every bond (|gamma| <= 1e-6)."""
+    r, box, z, mass, P = _rigid()
+    na = r.shape[1]
+    refpath.rattle_setup(na // 3, P)
+    rng = np.random.default_rng(2)
+    dt = 4.0 * cf.TOFS
+    p = rng.normal(size=(na, 3)) * np.sqrt(mass)[:, None] * 0.05
+    rold = r[0]
+    rnew = rold + p * dt / mass[:, None]
+    dv = rng.normal(size=(na, 3))
+    a = oracle.rattle_s1(rnew, rold, p, mass, dt, dv, P)
+    b = refpath.rattle_s1(rnew, rold, p, mass, dt, dv)
+    assert a[4] == 0
+    for x, y in zip(a[:4], b):
+        assert np.array_equal(x, y)
+    assert _bond_residuals(a[0], P) <= 1e-6 * 1.0001
+    assert _bond_residuals(rnew, P) > 1e-4            # the input did violate the constraints
+    p2 = p + rng.normal(size=p.shape) * np.sqrt(mass)[:, None] * 0.05
+    c = oracle.rattle_s2(a[0], p2, mass, dt, dv, P)
+    d = refpath.rattle_s2(a[0], p2, mass, dt, dv)
+    assert c[3] == 0
+    for x, y in zip(c[:3], d):
+        assert np.array_equal(x, y)
+    m = a[0].reshape(-1, 3, 3)
+    v = (c[0] / mass[:, None]).reshape(-1, 3, 3)
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        along = ((m[:, j] - m[:, i]) * (v[:, j] - v[:, i])).sum(-1)
+        assert np.abs(along).max() < 1e-5 * np.abs(v).max() * P.reoh
+
+
+def test_bead_forces_closed_form(oracle):
+    """bead_forces is the gradient of the ring-polymer spring energy
+    U = sum_k m omega_n^2 |x_k - x_{k+1}|^2 / 2, omega_n = nb / beta (central differences)."""
+    rng = np.random.default_rng(3)
+    nb, na, beta = 5, 6, 1000.0
+    r = rng.normal(size=(nb, na, 3))
+    mass = rng.uniform(1.0, 3.0, size=na)
+    g = oracle.bead_forces(r, beta, mass)
+    wn2 = (nb / beta) ** 2
+
+    def U(x):
+        return 0.5 * wn2 * float((mass[None, :, None] * (x - np.roll(x, -1, axis=0)) ** 2).sum())
+
+    h = 1e-6
+    for (k, a, c) in ((0, 0, 0), (2, 3, 1), (4, 5, 2)):
+        e = np.zeros_like(r)
+        e[k, a, c] = h
+        fd = (U(r + e) - U(r - e)) / (2 * h)
+        assert abs(fd - g[k, a, c]) < 1e-7 * max(1.0, abs(fd))
+    assert np.array_equal(oracle.bead_forces(r[:1], beta, mass), np.zeros((1, na, 3)))
+
+
+def _ref_rig_step(ref, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta, therm, vir):
+    """evolve_rig_pi (src/evolve_rigid.f90:100-234, nbaro = 0) composed from the reference's own
+    compiled routines (parinello_therm[_loc], bead_forces, rattle_s1/s2, forces) in its order.
+    The reference routine itself cannot be run as the pin: it adds dvdr_corr, v_corr and vir_corr,
+    which it never sets (:112-113, :182, :208-215), and built here it returns NaN.  They are zero
+    here, as in evolve_rig_cl (:19-21)."""
+    nb, na = r.shape[:2]
+    nm = na // 3
+    p, r, dvdr = p.copy(), r.copy(), dvdr.copy()
+    halfdt, smalldt = 0.5 * dt, dt / nb
+    halfsmalldt = 0.5 * smalldt
+
+    def thermo(p):
+        if therm == "PRG":
+            return ref.parinello_therm(p, mass, P.ttau, halfdt, beta, 3 * nm)
+        if therm == "PRL":
+            return ref.parinello_therm_loc(p, mass, P.ttau, halfdt, beta)
+        return p
+
+    p = thermo(p)
+    p = p - halfdt * dvdr
+    d2 = ref.bead_forces(r, beta, mass)
+    for _ in range(nb):
+        for k in range(nb):
+            rold = r[k].copy()
+            p[k] = p[k] - halfsmalldt * d2[k]
+            r[k] = r[k] + p[k] * smalldt / mass[:, None]
+            r[k], p[k], d2[k], _ = ref.rattle_s1(r[k], rold, p[k], mass, smalldt, d2[k])
+        d2 = ref.bead_forces(r, beta, mass)
+        p = p - halfsmalldt * d2
+        for k in range(nb):
+            p[k], d2[k], _ = ref.rattle_s2(r[k], p[k], mass, smalldt, d2[k])
+    v, dvdr, vir_lf = ref.forces(P, r, box, z, 1)
+    p = p - halfdt * dvdr
+    vir_hf = np.zeros((3, 3))
+    for k in range(nb):
+        p[k], dvdr[k], vt = ref.rattle_s2(r[k], p[k], mass, dt, dvdr[k])
+        vir_hf = vir_hf + vt
+    p = thermo(p)
+    vir = vir_lf + vir_hf / nb + vir
+    return dict(p=p, r=r, dvdr=dvdr, dvdr2=np.zeros_like(r), v=v, vir=vir, vir_lf=vir_lf)
+
+
+@pytest.mark.parametrize("therm", ["NONE", "PRG", "PRL"])
+def test_evolve_rig_pi_matches_reference(oracle, refpath, therm):
+    """Three rigid RPMD steps of the oracle against the same steps composed from the reference's
+    own routines on the same ran2 stream; the molecules stay rigid on every bead."""
+    r, box, z, mass, P = _rigid(nb=4, therm=therm)
+    nb, na = r.shape[:2]
+    refpath.rattle_setup(na // 3, P)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    s0 = oracle.rng(11)
+    p = oracle.sample_rpmd(nb, na, mass, beta, s0)
+    # momenta consistent with the constraints (as the reference's md_eq start does after stage 2)
+    for k in range(nb):
+        p[k] = oracle.rattle_s2(r[k], p[k], mass, dt, np.zeros((na, 3)), P)[0]
+    ff = oracle.full_forces(P, r, box, z)
+    s = oracle.rng(57)
+    refpath.seed(57)
+    a = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], vir=np.zeros((3, 3)))
+    b = dict(a)
+    for step in range(3):
+        a = oracle.evolve_rig_pi(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, box, z, beta, a["vir"], s)
+        b = _ref_rig_step(refpath, P, b["p"], b["r"], b["dvdr"], b["dvdr2"], dt, mass, box, z, beta, therm,
+                          b["vir"])
+        for key in ("p", "r", "dvdr"):
+            assert np.array_equal(a[key], b[key]), (step, key)
+        assert a["v"] == b["v"]
+        assert np.array_equal(a["vir"], b["vir"])
+    for k in range(nb):
+        assert _bond_residuals(a["r"][k], P) <= 1.0001e-6
