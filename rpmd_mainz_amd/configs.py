@@ -98,12 +98,15 @@ def masses_charges(model: dict, na: int):
 
 def make_params(model: dict, box, rcut_A: float = 9.0, ecut: float = 0.0, iamcub: bool = True,
                 intra_model: int = INTRA_AUTO, sig_A: float = 0.0, nbdf1: int = 0, nbdf2: int = 0,
-                mts: int = 1, therm: str = "NONE", ttaufs: float = 0.0, rigid: bool = False) -> dict:
+                mts: int = 1, therm: str = "NONE", ttaufs: float = 0.0, rigid: bool = False,
+                oo_gam: float | None = None) -> dict:
     """The COMMON-block values of src/main.f90 for a model, before setup_ewald."""
     theta = model["thetad"] * (math.pi / 180.0)
     rcut = rcut_A / TOA
     rcut = min(rcut, 0.5 * float(min(box)))      # src/main.f90:615-616
-    return dict(oo_eps=model["oo_eps"], oo_sig=model["oo_sig"], oo_gam=model["oo_gam"], rcut=rcut,
+    # oo_gam != 0 selects the Buckingham exp-6 O-O term (src/md_forces.f90:226)
+    gam = model["oo_gam"] if oo_gam is None else oo_gam
+    return dict(oo_eps=model["oo_eps"], oo_sig=model["oo_sig"], oo_gam=gam, rcut=rcut,
                 alpha=model["alpha"], ecut=ecut, wm=model["wm"], wh=model["wh"],
                 wrcut=0.0, walpha=0.0, rkmax=0.0, kmax=0, iamcub=1 if iamcub else 0,
                 apot=model["apot"], bpot=model["bpot"], alp=model["alp"], alpb=model["alpb"],

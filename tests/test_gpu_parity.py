@@ -31,6 +31,8 @@ SYSTEMS = {
     "spcfw_ecut": ("qspcfw", (6, 6, 6), 4, (1, 1, 1), dict(ecut=1e-6)),
     "spcfw_nc1": ("qspcfw", (6, 6, 6), 1, (1, 1, 1.2), {}),
     "spcfw_model": ("qspcfw", (5, 5, 5), 2, (1, 1, 1), dict(intra_model=cf.INTRA_SPCFW)),
+    "buck_list": ("qspcfw", (4, 4, 4), 4, (1, 1, 1), dict(oo_gam=13.5)),
+    "buck_nc": ("qspcfw", (5, 5, 5), 1, (1, 1, 1.2), dict(oo_gam=13.5)),
 }
 
 
@@ -275,6 +277,59 @@ def test_device_resident_steps_match_oracle(capi, oracle):
     assert _rel(g["r"], ro) < 1e-12
     assert _rel(g["p"], po) < 1e-10
     assert abs(g["v"] - o["v"]) < 1e-10 * abs(o["v"])
+    ctx.close()
+
+
+def test_contracted_mts_noncubic_matches_oracle(capi, oracle):
+    """Config-4 form at small size: q-SPC/Fw in a 1:1:1.2 box (non-cubic Ewald), 64 beads, mts 4,
+    Ewald on 7 contracted modes and LJ on 15 (odd, as src/evolve_pi.f90:344-347 requires): two
+    evolve_pi steps on the device against the oracle."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 64, aspect=(1, 1, 1.2), nbdf1=7, nbdf2=15, mts=4,
+                                           intra_model=cf.INTRA_SPCFW)
+    assert prm["iamcub"] == 0
+    na, nb = r.shape[1], r.shape[0]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 1.0 * cf.TOFS
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(3))
+    ff = oracle.full_forces(P, r, box, z)
+    g = capi.RPMDContext(prm, na, nb, "fp64").full_forces(r, na, nb, z, box)
+    assert _rel(g["dvdr"], ff["dvdr"]) < 1e-11 and _rel(g["dvdr2"], ff["dvdr2"]) < 1e-11
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    po, ro, d1, d2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    pg, rg, g1, g2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    for k in range(2):
+        print(f"case contracted mts step {k}", flush=True)
+        o = oracle.evolve_pi(P, po, ro, d1, d2, dt, mass, box, z, beta)
+        po, ro, d1, d2 = o["p"], o["r"], o["dvdr"], o["dvdr2"]
+        out = ctx.evolve_pi(pg, rg, g1, g2, dt, mass, na, nb, box, z, beta)
+        assert abs(out["v"] - o["v"]) < 1e-10 * _escale(prm, box, z, o["v"])
+    assert _rel(rg, ro) < 1e-12
+    assert _rel(pg, po) < 1e-10
+    ctx.close()
+
+
+def test_config4_shape_device_resident(capi):
+    """Config-4 shape: 4096 q-SPC/Fw (16^3) in a 1:1:1.2 box, 64 beads, mts 4, Ewald on 7 modes
+    and LJ on 15, mixed precision, stepped in HBM; the contracted forces of the mixed path agree
+    with the FP64 path to the 1e-5 RMS bar."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (16, 16, 16), 64, aspect=(1, 1, 1.2), nbdf1=7, nbdf2=15,
+                                           mts=4, intra_model=cf.INTRA_SPCFW, amp=0.02)
+    na, nb = r.shape[1], r.shape[0]
+    assert prm["kmax"] <= 15 and prm["iamcub"] == 0
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 1.0 * cf.TOFS
+    ctx = capi.RPMDContext(prm, na, nb, "mixed")
+    f_mx = ctx.full_forces(r, na, nb, z, box)
+    ctx.set_precision("fp64")
+    f_64 = ctx.full_forces(r, na, nb, z, box)
+    err = math.sqrt(((f_mx["dvdr"] - f_64["dvdr"]) ** 2).sum() / (f_64["dvdr"] ** 2).sum())
+    print(f"config4 contracted LF force rms rel error mixed vs fp64: {err:.3e}")
+    assert err < 1e-5
+    ctx.set_precision("mixed")
+    p = np.random.default_rng(4).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    ctx.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=3)
+    ctx.run_steps(2)
+    st = ctx.state_download()
+    assert np.isfinite(st["v"]) and np.isfinite(st["p"]).all()
     ctx.close()
 
 

@@ -28,6 +28,8 @@ SYSTEMS = [
     ("qspcfw", (6, 6, 6), 4, (1, 1, 1), dict(ecut=1e-6)),              # ecut>0, 6^3 Ewald cells
     ("qspcfw", (6, 6, 6), 1, (1, 1, 1.2), {}),                         # non-cubic rwald_cell
     ("qspcfw", (5, 5, 5), 2, (1, 1, 1), dict(intra_model=cf.INTRA_SPCFW)),
+    ("qspcfw", (4, 4, 4), 4, (1, 1, 1), dict(oo_gam=13.5)),              # Buckingham exp-6, list
+    ("qspcfw", (5, 5, 5), 1, (1, 1, 1.2), dict(oo_gam=13.5)),            # Buckingham, cell/basic
 ]
 
 
