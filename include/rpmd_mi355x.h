@@ -170,6 +170,15 @@ int rpmd_state_download(rpmd_ctx* ctx, double* p, double* r, double* dvdr, doubl
 /* Capture rpmd_run_steps(ctx, 1) into a HIP graph and replay it for later calls. */
 int rpmd_use_graph(rpmd_ctx* ctx, int enable);
 
+/* Replica batches (no counterpart in the reference, which runs trajectories one after another):
+ * contexts with uploaded states (each with its own na, nb, box, Ewald set-up, temperature,
+ * thermostat; no RP contraction) advanced together, nsteps steps each, by one hipGraph whose
+ * branches are the replicas.  Re-create the batch after a context's state is uploaded again. */
+typedef struct rpmd_batch rpmd_batch;
+int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out);
+int rpmd_batch_run_steps(rpmd_batch* batch, int nsteps);
+int rpmd_batch_destroy(rpmd_batch* batch);
+
 /* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
  * the context stream around the Ewald kernels of eager launches between begin and end.
  * ms/counts index: 0 real-space pair kernel, 1 structure-factor GEMM, 2 k-space force kernel,

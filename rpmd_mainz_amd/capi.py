@@ -67,7 +67,8 @@ def lib() -> C.CDLL:
         for name in ("rpmd_setup_ewald", "rpmd_ctx_create", "rpmd_ctx_set_params", "rpmd_ctx_set_precision",
                      "rpmd_ctx_destroy", "rpmd_forces", "rpmd_full_forces", "rpmd_realft", "rpmd_evolve_pi",
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
-                     "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi"):
+                     "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
+                     "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -252,3 +253,28 @@ class RPMDContext:
         _check(lib().rpmd_state_download(self._h, _p(p), _p(r), _p(dvdr), _p(dvdr2), _p(e), _p(vir)))
         return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=e[0], vew=e[1], vlj=e[2], vint=e[3],
                     vir=vir.reshape(3, 3).T.copy())
+
+
+class ReplicaBatch:
+    """Independent replicas (each an RPMDContext with an uploaded state) stepped by one hipGraph."""
+
+    def __init__(self, contexts):
+        self._ctx = list(contexts)
+        arr = (C.c_void_p * len(self._ctx))(*[c._h.value for c in self._ctx])
+        h = C.c_void_p()
+        _check(lib().rpmd_batch_create(arr, C.c_int(len(self._ctx)), C.byref(h)))
+        self._h = h
+
+    def run_steps(self, nsteps: int):
+        _check(lib().rpmd_batch_run_steps(self._h, C.c_int(nsteps)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rpmd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -246,6 +246,13 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
     return 0;
 }
 
+// Morton (power-of-two) bin count of a cell grid
+int morton_bins(const CellGrid& g) {
+    int p = 1;
+    while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
+    return p * p * p;
+}
+
 // Morton cell grid of the cluster sort: about 4 sites per cell, cells near-cubic in space
 CellGrid make_grid(const Box& sb, int na) {
     CellGrid g;
@@ -322,6 +329,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         const Box sb = cubic ? unit_box() : bx;
         launch_sites(pos, c->site, na, nb, P.alpha, cubic, bx, s);
         CellGrid g = make_grid(sb, na);
+        if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
         launch_cell_build(c->site, na, nb, g, sb, 1, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
         launch_clusters(c->sorted, na, nb, sb, P.wrcut, c->bb, c->ccnt, c->clist, s);
         double* pr = c->parts + c->off_real;
@@ -943,7 +951,12 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     const size_t N = (size_t)3 * na * nb_max;
     const int nbl = nb_max;
     const int nck = kspace_nchunk(na);
-    c->ncell_cap = 64 * 64 * 64;
+    {
+        // room for the cluster-sort grid of any box with aspect ratios up to 8 (at most 64^3)
+        int p = 1;
+        while (p < std::min(64, (int)ceil(8.0 * cbrt(std::max(na, 1) / 4.0)) + 1)) p <<= 1;
+        c->ncell_cap = p * p * p;
+    }
     c->off_real = 0;
     c->off_k = c->off_real + (size_t)nbl * ewald_real_parts(na) * 10;
     c->off_oo = c->off_k + (size_t)nbl * (RPMD_KMAX_DEV + 1) * 10;
@@ -1380,6 +1393,103 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
     }
     HIPCHK(hipStreamSynchronize(c->st));
     return c->prm.rigid ? rattle_status(c) : 0;
+}
+
+// ---- replica batches ---------------------------------------------------------------------------
+// Independent trajectories (each its own context: box, Ewald set-up, na, nb, temperature) stepped
+// together: one hipGraph per batch holds two steps of every replica, each replica on its own
+// stream branch (fork from the batch stream, join back), so one graph launch advances all of them
+// and the replicas' kernels overlap on the GPU.
+struct rpmd_batch {
+    std::vector<rpmd_ctx*> ctx;
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out) {
+    if (!ctxs || n < 1 || !out) return RPMD_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !ctxs[i]->st_valid) return RPMD_ERR_ARG;
+        if (ctxs[i]->prm.nbdf1 > 0 || ctxs[i]->prm.nbdf2 > 0) return RPMD_ERR_ARG;  // contraction runs eagerly
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return RPMD_ERR_ARG;
+    }
+    rpmd_batch* b = new rpmd_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return RPMD_ERR_HIP;
+    }
+    b->ev.resize((size_t)2 * n + 1, nullptr);
+    for (auto& e : b->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            rpmd_batch_destroy(b);
+            return RPMD_ERR_HIP;
+        }
+    *out = b;
+    return 0;
+}
+
+int rpmd_batch_destroy(rpmd_batch* b) {
+    if (!b) return 0;
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->ev)
+        if (e) hipEventDestroy(e);
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+    return 0;
+}
+
+int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return RPMD_ERR_ARG;
+    const int n = (int)b->ctx.size();
+    for (rpmd_ctx* c : b->ctx)
+        if (!c->st_valid) return RPMD_ERR_ARG;
+    auto fork_steps = [&](int k) -> int {
+        HIPCHK(hipEventRecord(b->ev[2 * n], b->st));
+        for (int i = 0; i < n; i++) {
+            rpmd_ctx* c = b->ctx[i];
+            HIPCHK(hipStreamWaitEvent(c->st, b->ev[2 * n], 0));
+            for (int q = 0; q < k; q++) {
+                NoiseCursor nz{nullptr, 0, 0, 0};
+                RETCHK(step_any(c, nz));
+            }
+            HIPCHK(hipEventRecord(b->ev[2 * i], c->st));
+            HIPCHK(hipStreamWaitEvent(b->st, b->ev[2 * i], 0));
+        }
+        return 0;
+    };
+    if (!b->gexec && nsteps >= 2) {
+        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int rc = fork_steps(2);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->st, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : RPMD_ERR_HIP;
+        }
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    int i = 0;
+    for (; i + 1 < nsteps; i += 2) HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    if (i < nsteps) {
+        // an odd tail runs eagerly and leaves buffer roles swapped; the graph is rebuilt next time
+        RETCHK(fork_steps(1));
+        if (b->gexec) hipGraphExecDestroy(b->gexec);
+        if (b->graph) hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(b->st));
+    for (rpmd_ctx* c : b->ctx) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (c->prm.rigid) RETCHK(rattle_status(c));
+    }
+    return 0;
 }
 
 int rpmd_prof_begin(rpmd_ctx* c) {

@@ -333,6 +333,74 @@ def test_config4_shape_device_resident(capi):
     ctx.close()
 
 
+def _replica_specs(n, seed, sizes=(128, 256, 512, 1024), beads=(8, 16, 32, 64)):
+    """Config-3 style draws: molecule counts, bead counts, densities 0.95-1.05 g/cm^3 and
+    temperatures 250-350 K from a seed; every replica in its own cubic box."""
+    rng = np.random.default_rng(seed)
+    lattice = {8: ((2, 2, 2), (1, 1, 1)), 27: ((3, 3, 3), (1, 1, 1)), 64: ((4, 4, 4), (1, 1, 1)),
+               128: ((4, 4, 8), (2, 2, 1)), 256: ((4, 8, 8), (2, 1, 1)), 512: ((8, 8, 8), (1, 1, 1)),
+               1024: ((8, 8, 16), (2, 2, 1))}
+    out = []
+    for i in range(n):
+        nm = int(rng.choice(sizes))
+        out.append(dict(ncell=lattice[nm][0], aspect=lattice[nm][1], nb=int(rng.choice(beads)),
+                        rho=float(rng.uniform(0.95, 1.05)), temp=float(rng.uniform(250.0, 350.0)), seed=100 + i))
+    return out
+
+
+def _replica_ctx(capi, spec, prec, therm="PRO"):
+    r, box, z, mass, prm = cf.water_system("qspcfw", spec["ncell"], spec["nb"], rho=spec["rho"], aspect=spec["aspect"],
+                                           seed=spec["seed"], ecut=1e-6, intra_model=cf.INTRA_SPCFW, therm=therm,
+                                           ttaufs=100.0)
+    assert prm["iamcub"] == 1
+    na, nb = r.shape[1], r.shape[0]
+    beta, dt = 1.0 / (cf.KBOLTZ * spec["temp"]), 0.25 * cf.TOFS
+    p = np.random.default_rng(spec["seed"]).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    ctx = capi.RPMDContext(prm, na, nb, prec)
+    ctx.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=spec["seed"])
+    return ctx
+
+
+def test_replica_batch_equals_independent_runs(capi):
+    """Uneven replicas (different na, nb, density, temperature) stepped by one batch graph give
+    bit-identical states to the same replicas stepped one by one."""
+    specs = _replica_specs(5, 7, sizes=(8, 27, 64), beads=(4, 8, 16))
+    solo = []
+    for sp in specs:
+        c = _replica_ctx(capi, sp, "fp64")
+        c.use_graph(True)
+        c.run_steps(6)
+        solo.append(c.state_download())
+        c.close()
+    ctxs = [_replica_ctx(capi, sp, "fp64") for sp in specs]
+    batch = capi.ReplicaBatch(ctxs)
+    batch.run_steps(3)
+    batch.run_steps(3)
+    for c, ref in zip(ctxs, solo):
+        st = c.state_download()
+        assert np.array_equal(st["r"], ref["r"]) and np.array_equal(st["p"], ref["p"])
+        assert st["v"] == ref["v"]
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_replica_batch_config3_shape(capi):
+    """Config 3: 48 replicas drawn from a seed (128-1024 molecules, 8-64 beads, own cubic boxes,
+    kmax 9-13, 250-350 K), mixed precision, stepped together; every state stays finite."""
+    specs = _replica_specs(48, 2024)
+    ctxs = [_replica_ctx(capi, sp, "mixed") for sp in specs]
+    batch = capi.ReplicaBatch(ctxs)
+    batch.run_steps(4)
+    for c in ctxs:
+        st = c.state_download()
+        assert np.isfinite(st["v"]) and np.isfinite(st["p"]).all()
+        assert 9 <= c.prm["kmax"] <= 13
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
 def _rigid_case(therm="NONE", ncell=(3, 3, 3), nb=4, seed=1, aspect=(1, 1, 1)):
     r, box, z, mass, prm = cf.water_system("tip4p2005", ncell, nb, ecut=1e-6, rigid=True, therm=therm,
                                            ttaufs=50.0, seed=seed, amp=0.1, aspect=aspect)
