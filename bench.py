@@ -91,6 +91,83 @@ def run_gpu(prec: str, steps: int, warmup: int, r, p, box, z, mass, prm, beta, n
     return ms, prof, st
 
 
+def h_rp(st, mass, beta, nb, na) -> float:
+    """H_RP per atom as src/md_eq.f90:247-278 prints it: (nb v + K + V_springs) / na."""
+    p, r = st["p"], st["r"]
+    tk = float(np.sum(p ** 2 / (2.0 * mass[None, :, None])))
+    wn = nb / beta
+    d = r - np.roll(r, 1, axis=0)
+    vrp = float(np.sum(0.5 * mass[None, :, None] * wn ** 2 * d ** 2))
+    return (nb * st["v"] + tk + vrp) / na
+
+
+def equilibrate(nsteps: int, r, p, box, z, mass, prm, beta, na):
+    """PILE-L steps (mixed precision) from the lattice start, so that an NVE window does not begin
+    with the violent relaxation of the randomly oriented lattice."""
+    ctx = capi.RPMDContext(prm, na, NB, "mixed")
+    ctx.state_upload(p, r, mass, z, na, NB, box, beta, DT_FS * cf.TOFS, seed=5)
+    ctx.use_graph(True)
+    ctx.run_steps(nsteps)
+    st = ctx.state_download()
+    ctx.close()
+    return st["r"], st["p"]
+
+
+def nve_drift(prec: str, nsteps: int, every: int, r, p, box, z, mass, prm, beta, na):
+    """Slope of H_RP per atom (hartree / ps) over an NVE window run in HBM with hipGraphs."""
+    q = dict(prm)
+    q["therm"] = 0
+    ctx = capi.RPMDContext(q, na, NB, prec)
+    dt = DT_FS * cf.TOFS
+    ctx.state_upload(p, r, mass, z, na, NB, box, beta, dt, seed=1)
+    ctx.use_graph(True)
+    ts, hs = [], []
+    for i in range(nsteps // every):
+        ctx.run_steps(every)
+        st = ctx.state_download()
+        ts.append((i + 1) * every * DT_FS * 1e-3)
+        hs.append(h_rp(st, mass, beta, NB, na))
+    ctx.close()
+    ts, hs = np.array(ts), np.array(hs)
+    slope = float(np.polyfit(ts, hs, 1)[0]) if len(ts) > 1 else float("nan")
+    return dict(slope_hartree_per_atom_ps=slope, h0=float(hs[0]), spread=float(hs.max() - hs.min()))
+
+
+def replicas(nrep: int, steps: int, seed: int = 2024):
+    """Config 3: nrep q-SPC/Fw replicas, sizes and conditions drawn from seed, stepped together by
+    one replica-batch graph in mixed precision.  Returns bead-force-evals/s over all replicas."""
+    rng = np.random.default_rng(seed)
+    lattice = {128: ((4, 4, 8), (2, 2, 1)), 256: ((4, 8, 8), (2, 1, 1)), 512: ((8, 8, 8), (1, 1, 1)),
+               1024: ((8, 8, 16), (2, 2, 1))}
+    ctxs, sum_nb = [], 0
+    for i in range(nrep):
+        nm = int(rng.choice([128, 256, 512, 1024]))
+        nb = int(rng.choice([8, 16, 32, 64]))
+        rho, temp = float(rng.uniform(0.95, 1.05)), float(rng.uniform(250.0, 350.0))
+        ncell, aspect = lattice[nm]
+        r, box, z, mass, prm = cf.water_system("qspcfw", ncell, nb, rho=rho, aspect=aspect, seed=100 + i, ecut=1e-6,
+                                               intra_model=cf.INTRA_SPCFW, therm="PRO", ttaufs=100.0)
+        na = r.shape[1]
+        beta = 1.0 / (cf.KBOLTZ * temp)
+        p = np.random.default_rng(100 + i).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+        c = capi.RPMDContext(prm, na, nb, "mixed")
+        c.state_upload(p, r, mass, z, na, nb, box, beta, DT_FS * cf.TOFS, seed=100 + i)
+        ctxs.append(c)
+        sum_nb += nb
+    batch = capi.ReplicaBatch(ctxs)
+    batch.run_steps(2)
+    t0 = time.perf_counter()
+    batch.run_steps(steps)
+    dt = time.perf_counter() - t0
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return dict(replicas=nrep, beads_total=sum_nb, steps=steps, ms_per_step=dt * 1e3 / steps,
+                bead_force_evals_s=steps * sum_nb / dt, dtype="f32-pair/f64-acc",
+                workload="config 3: q-SPC/Fw replicas of 128-1024 molecules, 8-64 beads, rho 0.95-1.05, "
+                         "250-350 K, own cubic boxes (kmax 9-13), PILE-L, one batch hipGraph per 2 steps")
+
+
 def cpu_baseline(r, box, z, prm, nbs: int):
     """The reference's own forces(iopt=0) (oracle/_ref, Fortran from source), 1 thread."""
     try:
@@ -123,6 +200,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-beads", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nve-steps", type=int, default=2000, help="NVE window per precision for H_RP drift (0: skip)")
+    ap.add_argument("--replicas", type=int, default=48, help="config-3 replica count (0: skip)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -158,10 +239,33 @@ def main():
     dom = max(("ewald_real", "k_sk", "k_kforce"), key=lambda k: prof[k][0])
     t_ms = prof[dom][0] / max(prof[dom][1], 1)
     achieved = flops[dom] / (t_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_1728x32_mixed.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get("kernels", {}).get(dom)
+        if t:
+            traffic = t["bytes_per_launch"]
     roof = dict(bound="mfma", kernel=dom, achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic,
                 note="FP64 vector-FMA bound; MI355X FP64 peak (vector = matrix = 78.6 TFLOP/s)",
                 kernel_ms={k: v[0] / max(v[1], 1) for k, v in prof.items()})
+    drift = None
+    if a.nve_steps > 0:
+        drift = {}
+        print(f"[bench] PILE-L equilibration: {a.nve_steps} steps", file=sys.stderr, flush=True)
+        re, pe = equilibrate(a.nve_steps, r, p, box, z, mass, prm, beta, na)
+        for prec in precs:
+            print(f"[bench] NVE drift window {prec}: {a.nve_steps} steps", file=sys.stderr, flush=True)
+            drift[prec] = nve_drift(prec, a.nve_steps, 100, re, pe, box, z, mass, prm, beta, na)
+        if "fp64" in drift and "mixed" in drift:
+            drift["ratio_mixed_fp64"] = (abs(drift["mixed"]["slope_hartree_per_atom_ps"]) /
+                                         max(abs(drift["fp64"]["slope_hartree_per_atom_ps"]), 1e-300))
+        drift["window_ps"] = a.nve_steps * DT_FS * 1e-3
+        drift["start"] = f"after {a.nve_steps} PILE-L steps from the lattice"
+    rep = None
+    if a.replicas > 0:
+        print(f"[bench] config-3 replicas: {a.replicas}", file=sys.stderr, flush=True)
+        rep = replicas(a.replicas, 10)
     cpu = None if a.no_cpu else cpu_baseline(r, box, z, prm, a.cpu_beads)
     out = {
         "metric": "bead-force-evals/s (1728 q-SPC/Fw H2O x 32 beads, RPMD step)",
@@ -184,10 +288,34 @@ def main():
                       "ns_per_day": 1e3 / res["fp64"][0] * DT_FS * 0.0864} if "fp64" in res else None),
         "mixed_vs_fp64": (res["fp64"][0] / res["mixed"][0]) if ("fp64" in res and "mixed" in res) else None,
         "mixed_force_error": err,
+        "nve_drift": drift,
+        "replicas": rep,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if a.copy_probe_mb > 0:
+        copy_probe(a.copy_probe_mb)
+
+
+def copy_probe(mib: int):
+    """A plain device-to-device copy of known size (mib MiB read and written, 4 times) for scaling
+    the HBM byte counters of a profiler run: hipMemcpy between two device buffers, which the HIP
+    runtime performs with its copy kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(4):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), C.c_int(3)):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy probe: 4 copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

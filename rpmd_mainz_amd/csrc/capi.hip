@@ -68,8 +68,8 @@ struct rpmd_ctx {
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
-    double *ph = nullptr, *fpart = nullptr, *bb = nullptr;
-    int *ccnt = nullptr, *clist = nullptr;
+    double *ph = nullptr, *fpart = nullptr, *bb = nullptr, *jpart = nullptr;
+    int *ccnt = nullptr, *clist = nullptr, *crev = nullptr;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
     unsigned char* kmask = nullptr;
@@ -331,16 +331,16 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         CellGrid g = make_grid(sb, na);
         if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
         launch_cell_build(c->site, na, nb, g, sb, 1, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
-        launch_clusters(c->sorted, na, nb, sb, P.wrcut, c->bb, c->ccnt, c->clist, s);
+        launch_clusters(c->sorted, na, nb, sb, P.wrcut, c->bb, c->ccnt, c->clist, c->crev, s);
         double* pr = c->parts + c->off_real;
         {
             ProfScope ps(c, 0);
             if (mixed)
                 launch_ewald_real<float>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
-                                         c->fpart, pr, s);
+                                         c->crev, c->jpart, c->fpart, pr, s);
             else
                 launch_ewald_real<double>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
-                                          c->fpart, pr, s);
+                                          c->crev, c->jpart, c->fpart, pr, s);
         }
         double* pk = c->parts + c->off_k;
         launch_phase(c->site, na, nb, c->ks, c->ph, s);
@@ -354,11 +354,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         {
             ProfScope ps(c, 2);
-            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->fpart + (size_t)ewald_rsplit() * 3 * na * nb, s);
+            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
-        // real-space split slots then k-space kx-group slots, summed in that order
-        launch_chain_rule(grad, c->fpart, ewald_rsplit() + kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
+        // real-space slots (splits, then the j-side gather) then k-space kx-group slots, in that order
+        launch_chain_rule(grad, c->fpart, ewald_real_slots() + kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
         launch_reduce_ev(pr, nb * ewald_real_parts(na), 10, ev_slot + 0, 10, sc / nb, s);
         launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);
@@ -978,11 +978,13 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->skp, (size_t)nbl * (RPMD_KMAX_DEV + 1) * nck * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->ph, (size_t)6 * na * nbl);
-    e |= dalloc(&c->fpart, (size_t)(ewald_rsplit() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
+    e |= dalloc(&c->fpart, (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
     const int nclu = ewald_nclu(na);
     e |= dalloc(&c->bb, (size_t)6 * nclu * nbl);
     e |= dalloc(&c->ccnt, (size_t)nclu * nbl);
-    e |= dalloc(&c->clist, (size_t)nclu * nclu * nbl);
+    e |= dalloc(&c->clist, ewald_list_size(na) * nbl);
+    e |= dalloc(&c->crev, (size_t)nclu * nclu * nbl);
+    e |= dalloc(&c->jpart, ewald_jpart_size(na) * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
@@ -1039,7 +1041,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* bufs[] = {c->ph, c->fpart, c->bb, c->ccnt, c->clist, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
+    void* bufs[] = {c->ph, c->fpart, c->bb, c->jpart, c->ccnt, c->clist, c->crev, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,

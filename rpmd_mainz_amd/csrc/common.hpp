@@ -54,12 +54,16 @@ void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitb
                        int* count, int* start, int* perm, double* sorted, hipStream_t s);
 int ewald_nclu(int na);
 int ewald_rsplit();
+int ewald_real_slots();
 int ewald_real_parts(int na);
+size_t ewald_list_size(int na);
+size_t ewald_jpart_size(int na);
 void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
-                     hipStream_t s);
+                     int* rev, hipStream_t s);
 template <typename T>
 void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
-                       double alpha, const int* ccnt, const int* clist, double* rpart, double* parts, hipStream_t s);
+                       double alpha, const int* ccnt, const int* clist, const int* rev, double* jpart, double* rpart,
+                       double* parts, hipStream_t s);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
 void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s);

@@ -34,27 +34,25 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out)
     }
 }
 
-// out[k] = scale * sum_p parts[p*stride + k], k < nout; one block, fixed order per k
-__global__ void k_reduce_parts(const double* __restrict__ parts, int nparts, int stride, double* out, int nout,
-                               double scale) {
-    __shared__ double red[1024];
-    for (int k = 0; k < nout; k++) {
-        double s = 0.0;
-        for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += parts[(size_t)p * stride + k];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = scale * red[0];
+// out[k] = scale * sum_p parts[p*stride + k], k < nout; one block per k, fixed order
+__global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__ parts, int nparts, int stride,
+                                                      double* out, double scale) {
+    __shared__ double red[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += 256) s += parts[(size_t)p * stride + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[k] = scale * red[0];
 }
 
 void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double* out, int nout, double scale,
                       hipStream_t s) {
-    k_reduce_parts<<<1, 1024, 0, s>>>(parts, nparts, stride_parts, out, nout, scale);
+    k_reduce_parts<<<nout, 256, 0, s>>>(parts, nparts, stride_parts, out, scale);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -300,14 +298,19 @@ void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitb
 // ------------------------------------------------------------------------------------------------
 // real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
 //
-// Cluster-pair scheme.  The sites of a bead are sorted by cell in Morton order (launch_cell_build
-// with morton = 1), so each run of 64 sorted sites -- a cluster -- is spatially compact.  A cluster
-// pair (I, J) is kept when the minimum-image distance between the clusters' bounding boxes is below
-// the cut-off.  One wave owns cluster I (lane = site i) and every RSPLIT-th entry of I's J list: per
-// J it stages J's sites in LDS, tests the 64 distances of each lane in FP32 with a relative margin
-// (one bit per j), then evaluates only the flagged pairs, re-testing the cut-off exactly in FP64.
-// Every pair is seen from both ends, so energy and virial carry a factor 1/2; each (split, bead)
-// writes its own gradient slot, summed in fixed order by k_chain.  T is the pair-body type.
+// Cluster-pair scheme with Newton's third law.  The sites of a bead are sorted by cell in Morton
+// order (launch_cell_build with morton = 1), so each run of 64 sorted sites -- a cluster -- is
+// spatially compact.  Every unordered cluster pair {I, J} whose bounding boxes come closer than the
+// cut-off is assigned to exactly one of its clusters by a parity rule that splits each cluster's
+// neighbours in half (J in list(I) when J = I, or J > I with I + J odd, or J < I with I + J even).
+// One wave owns cluster I (lane = site i) and every RSPLIT-th entry of I's list; per entry J it
+// stages J's sites in LDS and walks the 64 j in lock step: a lane takes the pair (i, j) when the
+// FP32 distance test passes (diagonal tile: only j > i), the FP64 mode re-tests it exactly, the
+// i-side force stays in the lane and the j-side force is summed across the wave (a fixed
+// butterfly) into the lane that owns site j of J.  After the tile those j-side sums go to the
+// tile's slot, which k_ewald_jgather adds up per site in fixed order (I ascending).  Energy and
+// virial count each pair once.  T is the pair-body type; in FP32 the per-tile sums are flushed into
+// FP64 accumulators after every tile.
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& du, T& dur) {
@@ -368,55 +371,78 @@ __global__ __launch_bounds__(64) void k_clu_bbox(const double* __restrict__ sort
     }
 }
 
-// cluster-pair list of (I, bead), one wave each: J ascending, kept when the bounding boxes come
-// closer than rc (minimum image of the centres; L = box side, 1 for the unit box)
+__host__ __device__ __forceinline__ int clu_cap(int nclu) { return nclu / 2 + 2; }
+
+// half cluster-pair lists, one wave per (I, bead): J ascending, kept when the bounding boxes come
+// closer than rc (minimum image of the centres; L = box side, 1 for the unit box) and the parity
+// rule gives the pair to I.  rev[(b*nclu + J)*nclu + I] = position in list(I) + 1 (0: absent).
 __global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ bb, int nclu, double L0, double L1,
                                                   double L2, double rc2, int* __restrict__ cnt,
-                                                  int* __restrict__ list) {
+                                                  int* __restrict__ list, int* __restrict__ rev) {
     const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
     const double* bi = bb + ((size_t)b * nclu + I) * 6;
     const double L[3] = {L0, L1, L2};
-    int* out = list + ((size_t)b * nclu + I) * nclu;
+    const int cap = clu_cap(nclu);
+    int* out = list + ((size_t)b * nclu + I) * cap;
     int n = 0;
     for (int J0 = 0; J0 < nclu; J0 += 64) {
         const int J = J0 + l;
         bool keep = false;
         if (J < nclu) {
-            const double* bj = bb + ((size_t)b * nclu + J) * 6;
-            double r2 = 0.0;
+            const bool mine = J == I || (J > I && ((I + J) & 1)) || (J < I && !((I + J) & 1));
+            if (mine) {
+                const double* bj = bb + ((size_t)b * nclu + J) * 6;
+                double r2 = 0.0;
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double dc = bi[d] - bj[d];
-                dc -= rint(dc);
-                const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
-                r2 += gap * gap;
+                for (int d = 0; d < 3; d++) {
+                    double dc = bi[d] - bj[d];
+                    dc -= rint(dc);
+                    const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
+                    r2 += gap * gap;
+                }
+                keep = r2 < rc2;
             }
-            keep = r2 < rc2;
         }
         const uint64_t m = __ballot(keep);
-        if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
+        if (keep) {
+            const int pos = n + __popcll(m & ((1ull << l) - 1));
+            out[pos] = J;
+            rev[((size_t)b * nclu + J) * nclu + I] = pos + 1;
+        }
         n += __popcll(m);
     }
     if (l == 0) cnt[b * nclu + I] = n;
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sorted, const int* __restrict__ perm,
-                                                  const double* __restrict__ q, int na, int nclu,
-                                                  const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                  double L0, double L1, double L2, double i0, double i1, double i2,
-                                                  double rc2, float rc2f, double alpha, double sfac,
-                                                  double* __restrict__ rpart, double* __restrict__ parts) {
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_ewald_half(const double* __restrict__ sorted, const int* __restrict__ perm,
+                                                   const double* __restrict__ q, int na, int nclu,
+                                                   const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                   double L0, double L1, double L2, double i0, double i1, double i2,
+                                                   double rc2, float rc2f, double alpha, double sfac,
+                                                   double* __restrict__ rpart, double* __restrict__ jpart,
+                                                   double* __restrict__ parts) {
+    constexpr bool F64 = sizeof(T) == 8;
     const int I = blockIdx.x, sp = blockIdx.y, b = blockIdx.z, nb = gridDim.z;
     const int l = threadIdx.x;
     const int i = I * CLU + l;
     const bool iv = i < na;
+    const int cap = clu_cap(nclu);
     const double* X = sorted + (size_t)b * 3 * na;
     const int* P = perm + (size_t)b * na;
-    __shared__ double sx[CLU], sy[CLU], sz[CLU], sq[CLU];
+    __shared__ double sx[CLU], sy[CLU], sz[CLU];
     __shared__ float fx[CLU], fy[CLU], fz[CLU];
+    __shared__ T sq[CLU];
     __shared__ int sm[CLU];
-    double xi = 0, yi = 0, zi = 0, qi = 0;
+    double xi = 0, yi = 0, zi = 0;
+    T qi = 0;
     int ai = 0, mi = -1;
     float fxi = 0, fyi = 0, fzi = 0;
     if (iv) {
@@ -424,7 +450,7 @@ __global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sor
         yi = X[na + i];
         zi = X[2 * na + i];
         ai = P[i];
-        qi = q[ai];
+        qi = (T)q[ai];
         mi = ai / 3;
         fxi = (float)wrap01(xi * i0);
         fyi = (float)wrap01(yi * i1);
@@ -437,7 +463,7 @@ __global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sor
     double gx = 0.0, gy = 0.0, gz = 0.0;
     const T ta = (T)alpha, tsf = (T)sfac;
     const int nJ = ccnt[b * nclu + I];
-    const int* lst = clist + ((size_t)b * nclu + I) * nclu;
+    const int* lst = clist + ((size_t)b * nclu + I) * cap;
     for (int n = sp; n < nJ; n += RSPLIT) {
         const int J = lst[n];
         const int j = J * CLU + l;
@@ -452,43 +478,80 @@ __global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sor
             fz[l] = (float)wrap01(zj * i2);
             const int aj = P[j];
             sm[l] = aj / 3;
-            sq[l] = q[aj];
+            sq[l] = (T)q[aj];
         }
         __syncthreads();
-        uint64_t mask = 0;
         const int jmax = min(CLU, na - J * CLU);
-        if (iv) {
-            for (int jj = 0; jj < jmax; jj++) {
-                float dx = fxi - fx[jj], dy = fyi - fy[jj], dz = fzi - fz[jj];
-                dx = (dx - rintf(dx)) * Lf0;
-                dy = (dy - rintf(dy)) * Lf1;
-                dz = (dz - rintf(dz)) * Lf2;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < rc2f) mask |= (1ull << jj);
+        const bool diag = J == I;
+        T ti[3] = {0, 0, 0}, tj[3] = {0, 0, 0}, ta10[10];
+#pragma unroll
+        for (int k = 0; k < 10; k++) ta10[k] = 0;
+        for (int jj = 0; jj < jmax; jj++) {
+            float dxf = fxi - fx[jj], dyf = fyi - fy[jj], dzf = fzi - fz[jj];
+            dxf = (dxf - rintf(dxf)) * Lf0;
+            dyf = (dyf - rintf(dyf)) * Lf1;
+            dzf = (dzf - rintf(dzf)) * Lf2;
+            const float r2f = dxf * dxf + dyf * dyf + dzf * dzf;
+            bool pass = iv && r2f < rc2f && (!diag || jj > l);
+            T dx = 0, dy = 0, dz = 0, dr2 = 0;
+            if (pass) {
+                if (F64) {
+                    double ex = xi - sx[jj], ey = yi - sy[jj], ez = zi - sz[jj];
+                    ex -= L0 * rint(ex * i0);
+                    ey -= L1 * rint(ey * i1);
+                    ez -= L2 * rint(ez * i2);
+                    dx = (T)ex;
+                    dy = (T)ey;
+                    dz = (T)ez;
+                    dr2 = (T)(ex * ex + ey * ey + ez * ez);
+                    pass = (double)dr2 < rc2;
+                } else {
+                    dx = (T)dxf;
+                    dy = (T)dyf;
+                    dz = (T)dzf;
+                    dr2 = (T)r2f;
+                }
             }
-            if (J == I) mask &= ~(1ull << l);
+            T fxx = 0, fyy = 0, fzz = 0;
+            if (pass) {
+                T du, dur;
+                erfc_pair<T>(dr2, ta, tsf, sm[jj] == mi, du, dur);
+                const T qij = qi * sq[jj];
+                const T dv = qij * du, dvr = qij * dur;
+                fxx = dvr * dx;
+                fyy = dvr * dy;
+                fzz = dvr * dz;
+                ti[0] += fxx;
+                ti[1] += fyy;
+                ti[2] += fzz;
+                ta10[0] += dv;
+                ta10[1] += dx * fxx; ta10[2] += dy * fxx; ta10[3] += dz * fxx;
+                ta10[4] += dx * fyy; ta10[5] += dy * fyy; ta10[6] += dz * fyy;
+                ta10[7] += dx * fzz; ta10[8] += dy * fzz; ta10[9] += dz * fzz;
+            }
+            if (__any(pass)) {
+                const T sxj = wave_sum<T>(fxx), syj = wave_sum<T>(fyy), szj = wave_sum<T>(fzz);
+                if (l == jj) {
+                    tj[0] -= sxj;
+                    tj[1] -= syj;
+                    tj[2] -= szj;
+                }
+            }
         }
-        while (mask) {
-            const int jj = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
-            dx -= L0 * rint(dx * i0);
-            dy -= L1 * rint(dy * i1);
-            dz -= L2 * rint(dz * i2);
-            const double dr2 = dx * dx + dy * dy + dz * dz;
-            if (!(dr2 < rc2)) continue;
-            T du, dur;
-            erfc_pair<T>((T)dr2, ta, tsf, sm[jj] == mi, du, dur);
-            const double qij = qi * sq[jj];
-            const double dv = qij * (double)du, dvr = qij * (double)dur;
-            const double fxx = dvr * dx, fyy = dvr * dy, fzz = dvr * dz;
-            gx += fxx;
-            gy += fyy;
-            gz += fzz;
-            acc[0] += 0.5 * dv;
-            acc[1] += 0.5 * dx * fxx; acc[2] += 0.5 * dy * fxx; acc[3] += 0.5 * dz * fxx;
-            acc[4] += 0.5 * dx * fyy; acc[5] += 0.5 * dy * fyy; acc[6] += 0.5 * dz * fyy;
-            acc[7] += 0.5 * dx * fzz; acc[8] += 0.5 * dy * fzz; acc[9] += 0.5 * dz * fzz;
+        gx += (double)ti[0];
+        gy += (double)ti[1];
+        gz += (double)ti[2];
+#pragma unroll
+        for (int k = 0; k < 10; k++) acc[k] += (double)ta10[k];
+        if (diag) {
+            gx += (double)tj[0];
+            gy += (double)tj[1];
+            gz += (double)tj[2];
+        } else {
+            double* o = jpart + (((size_t)b * nclu + I) * cap + n) * (3 * CLU);
+            o[l] = (double)tj[0];
+            o[CLU + l] = (double)tj[1];
+            o[2 * CLU + l] = (double)tj[2];
         }
     }
     if (iv) {
@@ -500,9 +563,38 @@ __global__ __launch_bounds__(64) void k_ewald_clu(const double* __restrict__ sor
     block_reduce_store<10>(acc, parts + (((size_t)b * RSPLIT + sp) * nclu + I) * 10);
 }
 
+// j-side sums of every tile that touched site j, I ascending, into gradient slot RSPLIT
+__global__ __launch_bounds__(64) void k_ewald_jgather(const int* __restrict__ perm, int na, int nclu,
+                                                      const int* __restrict__ rev, const double* __restrict__ jpart,
+                                                      double* __restrict__ rpart) {
+    const int J = blockIdx.x, b = blockIdx.y, nb = gridDim.y, l = threadIdx.x;
+    const int j = J * CLU + l;
+    const int cap = clu_cap(nclu);
+    const int* rv = rev + ((size_t)b * nclu + J) * nclu;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int I = 0; I < nclu; I++) {
+        const int n = rv[I] - 1;
+        if (n < 0 || I == J) continue;
+        const double* o = jpart + (((size_t)b * nclu + I) * cap + n) * (3 * CLU);
+        s0 += o[l];
+        s1 += o[CLU + l];
+        s2 += o[2 * CLU + l];
+    }
+    if (j < na) {
+        const int aj = perm[(size_t)b * na + j];
+        double* G = rpart + ((size_t)RSPLIT * nb + b) * 3 * na;
+        G[aj] = s0;
+        G[na + aj] = s1;
+        G[2 * na + aj] = s2;
+    }
+}
+
 int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
 int ewald_rsplit() { return RSPLIT; }
+int ewald_real_slots() { return RSPLIT + 1; }
 int ewald_real_parts(int na) { return RSPLIT * ewald_nclu(na); }
+size_t ewald_list_size(int na) { return (size_t)ewald_nclu(na) * clu_cap(ewald_nclu(na)); }
+size_t ewald_jpart_size(int na) { return ewald_list_size(na) * 3 * CLU; }
 
 // Pair-test margins.  FP32 wrapped fractional coordinates carry an absolute error of about
 // 2^-24 per component, so |dr| errs by less than 1e-6 of the box side.
@@ -513,27 +605,32 @@ static double pair_margin(const Box& sb, double rc) {
 }
 
 void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
-                     hipStream_t s) {
+                     int* rev, hipStream_t s) {
     const int nclu = ewald_nclu(na);
+    launch_zero_int(rev, (size_t)nclu * nclu * nb, s);
     k_clu_bbox<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.inv[0], sb.inv[1], sb.inv[2], bb);
-    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2], rc * rc * (1.0 + pair_margin(sb, rc)),
-                                             ccnt, clist);
+    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2],
+                                             rc * rc * (1.0 + pair_margin(sb, rc)), ccnt, clist, rev);
 }
 
+// rpart: RSPLIT + 1 gradient slots [slot][bead][3][na] by original atom index; jpart: tile slots
 template <typename T>
 void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
-                       double alpha, const int* ccnt, const int* clist, double* rpart, double* parts, hipStream_t s) {
+                       double alpha, const int* ccnt, const int* clist, const int* rev, double* jpart, double* rpart,
+                       double* parts, hipStream_t s) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
     const int nclu = ewald_nclu(na);
-    const float rc2f = (float)(rcut * rcut * (1.0 + pair_margin(sb, rcut)));
-    k_ewald_clu<T><<<dim3(nclu, RSPLIT, nb), 64, 0, s>>>(sorted, perm, q, na, nclu, ccnt, clist, sb.L[0], sb.L[1],
-                                                        sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, rc2f,
-                                                        alpha, sfac, rpart, parts);
+    // FP64: a margin, then the exact test; FP32: the cut-off itself
+    const float rc2f = sizeof(T) == 8 ? (float)(rcut * rcut * (1.0 + pair_margin(sb, rcut))) : (float)(rcut * rcut);
+    k_ewald_half<T><<<dim3(nclu, RSPLIT, nb), 64, 0, s>>>(sorted, perm, q, na, nclu, ccnt, clist, sb.L[0], sb.L[1],
+                                                         sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, rc2f,
+                                                         alpha, sfac, rpart, jpart, parts);
+    k_ewald_jgather<<<dim3(nclu, nb), 64, 0, s>>>(perm, na, nclu, rev, jpart, rpart);
 }
 template void launch_ewald_real<double>(const double*, const int*, const double*, int, int, Box, double, double,
-                                        const int*, const int*, double*, double*, hipStream_t);
+                                        const int*, const int*, const int*, double*, double*, double*, hipStream_t);
 template void launch_ewald_real<float>(const double*, const int*, const double*, int, int, Box, double, double,
-                                       const int*, const int*, double*, double*, hipStream_t);
+                                       const int*, const int*, const int*, double*, double*, double*, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)

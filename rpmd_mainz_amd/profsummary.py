@@ -1,20 +1,67 @@
-"""Summarise a rocprofv3 kernel-trace database (rocpd SQLite) as a markdown table:
-python -m rpmd_mainz_amd.profsummary <results.db> "<command that produced it>" > summary.md"""
-import sqlite3
+"""Summaries of rocprofv3 output for profiles/ (text only).
+
+  python -m rpmd_mainz_amd.profsummary stats <kernel_stats.csv> "<command>"          > kernels.md
+  python -m rpmd_mainz_amd.profsummary pmc <fetch.csv> <write.csv> <copy_mib> "<commands>" > traffic.json
+
+stats: the per-kernel table of a --kernel-trace --stats run.
+pmc:   FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of two --pmc runs, averaged per kernel and scaled
+       by what the same counters read for the plain device copy of copy_mib MiB that the run also
+       made (bench.py --copy-probe-mb), so that the copy reads as exactly its own bytes.
+"""
+import csv
+import json
 import sys
 
+KERNEL_KEYS = {"ewald_real": "k_ewald_clu", "k_sk": "k_sk_mfma", "k_kforce": "k_kforce", "k_sk_final": "k_sk_final"}
 
-def main(db_path: str, command: str, top: int = 30):
-    db = sqlite3.connect(db_path)
-    rows = db.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "")
+
+
+def stats(path: str, command: str, top: int = 30):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     print("# rocprofv3 kernel statistics\n")
     print(f"Command: `{command}`\n")
     print("| kernel | calls | total (us) | average (us) | % |")
     print("|---|---|---|---|---|")
-    for name, calls, tot, avg, pct in rows[:top]:
-        short = name.split("(")[0]
-        print(f"| `{short}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.2f} |")
+    for r in rows[:top]:
+        print(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+              f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+
+
+def _per_kernel(path: str, counter: str):
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        k = _short(r["Kernel_Name"])
+        acc.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def pmc(fetch_csv: str, write_csv: str, copy_mib: float, command: str):
+    f = _per_kernel(fetch_csv, "FETCH_SIZE")
+    w = _per_kernel(write_csv, "WRITE_SIZE")
+    copy_kib = copy_mib * 1024.0
+    # the copy kernel: the one whose fetch is closest to the known size
+    ck = min((k for k in f if k in w), key=lambda k: abs(f[k] - copy_kib))
+    sf, sw = copy_kib / f[ck], copy_kib / w[ck]
+    out = dict(command=command, copy_kernel=ck, copy_mib=copy_mib, fetch_scale=sf, write_scale=sw, kernels={})
+    for key, sub in KERNEL_KEYS.items():
+        ks = [k for k in f if sub in k and k in w]
+        if not ks:
+            continue
+        k = ks[0]
+        fb, wb = f[k] * sf * 1024.0, w[k] * sw * 1024.0
+        out["kernels"][key] = dict(kernel=k, fetch_kib_raw=f[k], write_kib_raw=w[k], fetch_bytes=fb,
+                                   write_bytes=wb, bytes_per_launch=fb + wb)
+    print(json.dumps(out, indent=1))
 
 
 if __name__ == "__main__":
-    main(sys.argv[1], sys.argv[2])
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2], sys.argv[3])
+    elif sys.argv[1] == "pmc":
+        pmc(sys.argv[2], sys.argv[3], float(sys.argv[4]), sys.argv[5])
