@@ -68,8 +68,16 @@ struct rpmd_ctx {
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
-    double *ph = nullptr, *fpart = nullptr, *bb = nullptr, *jpart = nullptr;
-    int *ccnt = nullptr, *clist = nullptr, *crev = nullptr;
+    double *ph = nullptr, *fpart = nullptr, *bb = nullptr;
+    // real-space Verlet-list caches (two bead sets: e.g. all beads and RP-contracted ones)
+    struct VCache {
+        VList v{};
+        bool alloc = false;
+        int nb = -1;
+        double key[5] = {0, 0, 0, 0, 0};  // box, rc, skin
+        long long last_use = 0;
+    } vc[2];
+    long long vc_clock = 0;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
     unsigned char* kmask = nullptr;
@@ -291,6 +299,76 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // ---- forces on device positions (SoA, nb beads); grad overwritten; ev_slot[5][10] ---------------
 // ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
+void free_vlist(VList& v) {
+    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dparts, v.srt, v.srf, v.sq, v.iperm, v.nex, v.ccnt, v.clist};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    v = VList{};
+}
+
+void free_vcache(rpmd_ctx* c) {
+    for (auto& x : c->vc) {
+        if (x.alloc) free_vlist(x.v);
+        x.alloc = false;
+        x.nb = -1;
+    }
+}
+
+// the Verlet-list cache for this bead count (least recently used slot otherwise); allocated, or
+// grown, outside of stream capture only; *force = 1 when the cached lists belong to another set-up
+int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, const Box& sb, VList** out,
+              int* force) {
+    const int na = c->na, nbl = c->nb_max;
+    int slot = -1;
+    for (int i = 0; i < 2; i++)
+        if (c->vc[i].alloc && c->vc[i].nb == nb) slot = i;
+    if (slot < 0) slot = c->vc[0].last_use <= c->vc[1].last_use ? 0 : 1;
+    auto& x = c->vc[slot];
+    x.last_use = ++c->vc_clock;
+    const double key[5] = {box[0], box[1], box[2], rc, skin};
+    *force = (x.nb != nb || memcmp(key, x.key, sizeof key) != 0) ? 1 : 0;
+    const int cap = vlist_cap(na, sb, rc + skin);
+    if (!x.alloc || x.v.cap < cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(c->st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return RPMD_ERR_ARG;
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (x.alloc) free_vlist(x.v);
+        x.alloc = false;
+        VList v{};
+        const int nclu = ewald_nclu(na);
+        const size_t ns = (size_t)na * nbl;
+        int e = 0;
+        e |= dalloc(&v.perm, ns);
+        e |= dalloc(&v.cnt, ns);
+        e |= dalloc(&v.nl, ns * cap);
+        e |= dalloc(&v.rb, 3 * ns);
+        e |= dalloc(&v.gate, 2);
+        e |= dalloc(&v.dparts, (ns + 255) / 256);
+        e |= dalloc(&v.srt, ns);
+        e |= dalloc(&v.srf, ns);
+        e |= dalloc(&v.sq, ns);
+        e |= dalloc(&v.iperm, ns);
+        e |= dalloc(&v.nex, ns);
+        e |= dalloc(&v.ccnt, (size_t)nclu * nbl);
+        e |= dalloc(&v.clist, (size_t)nclu * nclu * nbl);
+        if (e) {
+            free_vlist(v);
+            return RPMD_ERR_NOMEM;
+        }
+        v.cap = cap;
+        // reference positions NaN (all bits set): the first call always builds
+        HIPCHK(hipMemset(v.rb, 0xff, sizeof(double) * 3 * ns));
+        HIPCHK(hipMemset(v.gate, 0, sizeof(int) * 2));
+        x.v = v;
+        x.alloc = true;
+        *force = 1;
+    }
+    x.nb = nb;
+    memcpy(x.key, key, sizeof key);
+    *out = &x.v;
+    return 0;
+}
+
 int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
                bool check_lists) {
     const rpmd_params& P = c->prm;
@@ -330,17 +408,26 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_sites(pos, c->site, na, nb, P.alpha, cubic, bx, s);
         CellGrid g = make_grid(sb, na);
         if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
-        launch_cell_build(c->site, na, nb, g, sb, 1, c->cell_of, c->cnt, c->start, c->perm, c->sorted, s);
-        launch_clusters(c->sorted, na, nb, sb, P.wrcut, c->bb, c->ccnt, c->clist, c->crev, s);
+        // list skin of 1 A (src/main.f90 uses none: the reference rebuilds its cells every call)
+        const double skin = (cubic ? 1.0 / box[0] : 1.0) * (1.0 / 0.5291772108);
+        VList* vl = nullptr;
+        int force = 0;
+        RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
+        launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, c->sorted,
+                     c->bb, s);
+        if (check_lists) {
+            int h[2];
+            HIPCHK(hipMemcpyAsync(h, vl->gate, sizeof h, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (h[1]) return RPMD_ERR_NEIGH_OVERFLOW;
+        }
         double* pr = c->parts + c->off_real;
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
-                                         c->crev, c->jpart, c->fpart, pr, s);
+                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, s);
             else
-                launch_ewald_real<double>(c->sorted, c->perm, c->q, na, nb, sb, P.wrcut, P.walpha, c->ccnt, c->clist,
-                                          c->crev, c->jpart, c->fpart, pr, s);
+                launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, s);
         }
         double* pk = c->parts + c->off_k;
         launch_phase(c->site, na, nb, c->ks, c->ph, s);
@@ -786,6 +873,18 @@ int step_rig_energies(rpmd_ctx* c, double* v, double* vir_lfhf, double* vir_lf) 
     return 0;
 }
 
+// a Verlet list that overflowed its capacity in any step since the lists were allocated
+int vlist_status(rpmd_ctx* c) {
+    for (auto& x : c->vc) {
+        if (!x.alloc) continue;
+        int h[2];
+        HIPCHK(hipMemcpyAsync(h, x.v.gate, sizeof h, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (h[1]) return RPMD_ERR_NEIGH_OVERFLOW;
+    }
+    return 0;
+}
+
 int rattle_status(rpmd_ctx* c) {
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, c->rat_err, sizeof h, hipMemcpyDeviceToHost, c->st));
@@ -981,10 +1080,6 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->fpart, (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
     const int nclu = ewald_nclu(na);
     e |= dalloc(&c->bb, (size_t)6 * nclu * nbl);
-    e |= dalloc(&c->ccnt, (size_t)nclu * nbl);
-    e |= dalloc(&c->clist, ewald_list_size(na) * nbl);
-    e |= dalloc(&c->crev, (size_t)nclu * nclu * nbl);
-    e |= dalloc(&c->jpart, ewald_jpart_size(na) * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
@@ -1041,7 +1136,8 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* bufs[] = {c->ph, c->fpart, c->bb, c->jpart, c->ccnt, c->clist, c->crev, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
+    free_vcache(c);
+    void* bufs[] = {c->ph, c->fpart, c->bb, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
@@ -1343,6 +1439,10 @@ int rpmd_use_graph(rpmd_ctx* c, int enable) {
 }
 
 namespace {
+int run_status(rpmd_ctx* c) {
+    RETCHK(vlist_status(c));
+    return c->prm.rigid ? rattle_status(c) : 0;
+}
 int step_any(rpmd_ctx* c, NoiseCursor& nz) { return c->prm.rigid ? step_rig_dev(c, nz, false) : step_dev(c, nz, false); }
 }  // namespace
 
@@ -1355,7 +1455,7 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
             RETCHK(step_any(c, nz));
         }
         HIPCHK(hipStreamSynchronize(c->st));
-        return c->prm.rigid ? rattle_status(c) : 0;
+        return run_status(c);
     }
     if (!c->use_graph || c->prof) {
         for (int i = 0; i < nsteps; i++) {
@@ -1363,7 +1463,7 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
             RETCHK(step_any(c, nz));
         }
         HIPCHK(hipStreamSynchronize(c->st));
-        return c->prm.rigid ? rattle_status(c) : 0;
+        return run_status(c);
     }
     // graph path: one step per replay; the step swaps r/rst (and dvdr/tmp), so capture two steps
     // to return the buffers to their starting roles
@@ -1394,7 +1494,7 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
         c->graph = nullptr;
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    return c->prm.rigid ? rattle_status(c) : 0;
+    return run_status(c);
 }
 
 // ---- replica batches ---------------------------------------------------------------------------
@@ -1489,7 +1589,7 @@ int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
     HIPCHK(hipStreamSynchronize(b->st));
     for (rpmd_ctx* c : b->ctx) {
         HIPCHK(hipStreamSynchronize(c->st));
-        if (c->prm.rigid) RETCHK(rattle_status(c));
+        RETCHK(run_status(c));
     }
     return 0;
 }

@@ -51,19 +51,35 @@ void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
 void launch_zero_int(int* y, size_t n, hipStream_t s);
 void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, hipStream_t s);
+                       int* count, int* start, int* perm, double* sorted, const int* gate, hipStream_t s);
+
+// Verlet lists of the real-space Ewald sum (one cache per bead set); arrays sized for nb_max beads
+struct VList {
+    int* perm;       // [nb][na] sorted index -> site index (of the last build)
+    int* cnt;        // [nb][na] list length of sorted site k
+    int* nl;         // [nb][cap][na] sorted neighbour indices, column-major
+    double* rb;      // [nb][3][na] site coordinates at the last build
+    int* gate;       // [0] rebuild this call, [1] list overflow (sticky)
+    double* dparts;  // per-block largest displacement
+    int* iperm;      // [nb][na] site index -> sorted index (of the last build)
+    int* nex;        // [nb][na] leading list entries that are the site's own molecule
+    double4* srt;    // [nb][na] current coordinates and charge in sorted order
+    float4* srf;     // [nb][na] wrapped fractional FP32 coordinates and charge, sorted order
+    double* sq;      // [nb][na] charges, sorted order
+    int* ccnt;       // [nb][nclu] cluster-pair list lengths
+    int* clist;      // [nb][nclu][nclu] cluster-pair lists
+    int cap;
+};
 int ewald_nclu(int na);
-int ewald_rsplit();
 int ewald_real_slots();
 int ewald_real_parts(int na);
-size_t ewald_list_size(int na);
-size_t ewald_jpart_size(int na);
-void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
-                     int* rev, hipStream_t s);
+int vlist_cap(int na, Box sb, double rl);
+void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
+                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, double* bb,
+                  hipStream_t s);
 template <typename T>
-void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
-                       double alpha, const int* ccnt, const int* clist, const int* rev, double* jpart, double* rpart,
-                       double* parts, hipStream_t s);
+void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
+                       hipStream_t s);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
 void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s);

@@ -6,6 +6,8 @@
 // reduced per block into parts[block][10] and summed in a fixed order by k_reduce_parts.
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace rpmd {
 
 // A&S 7.1.26 erfc constants (src/ewald_cubic.f90:94-99)
@@ -197,7 +199,9 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
 }
 
 __global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, int morton, int nbins,
-                              double i0, double i1, double i2, int* __restrict__ cell_of, int* __restrict__ count) {
+                              double i0, double i1, double i2, int* __restrict__ cell_of, int* __restrict__ count,
+                              const int* __restrict__ gate) {
+    if (gate && !gate[0]) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)na * nb) return;
     size_t b = i / na, a = i % na;
@@ -211,7 +215,9 @@ __global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, C
 }
 
 // exclusive scan per bead (one block per bead); resets count to 0 for the scatter pass
-__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell) {
+__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell,
+                            const int* __restrict__ gate) {
+    if (gate && !gate[0]) return;
     int b = blockIdx.x;
     int* c = count + (size_t)b * (ncell + 1);
     int* st = start + (size_t)b * (ncell + 1);
@@ -242,7 +248,8 @@ __global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, in
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, int ncell, const int* __restrict__ start,
-                               int* __restrict__ fill, int* __restrict__ perm) {
+                               int* __restrict__ fill, int* __restrict__ perm, const int* __restrict__ gate) {
+    if (gate && !gate[0]) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)na * nb) return;
     size_t b = i / na, a = i % na;
@@ -253,7 +260,8 @@ __global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, 
 
 // order each cell by atom index, then gather the sorted positions
 __global__ void k_cell_order(const double* __restrict__ site, int na, int nb, int ncell, const int* __restrict__ start,
-                             int* __restrict__ perm, double* __restrict__ sorted) {
+                             int* __restrict__ perm, double* __restrict__ sorted, const int* __restrict__ gate) {
+    if (gate && !gate[0]) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)ncell * nb) return;
     size_t b = i / ncell, c = i % ncell;
@@ -277,8 +285,15 @@ __global__ void k_cell_order(const double* __restrict__ site, int na, int nb, in
     }
 }
 
+__global__ void k_fill_int_gated(int* __restrict__ y, size_t n, int v, const int* __restrict__ gate) {
+    if (gate && !gate[0]) return;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+// counting sort of the sites into cells; with gate, every kernel returns at once unless gate[0]
 void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, hipStream_t s) {
+                       int* count, int* start, int* perm, double* sorted, const int* gate, hipStream_t s) {
     size_t n = (size_t)na * nb;
     int nbins = g.ncell;
     if (morton) {
@@ -286,31 +301,32 @@ void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitb
         while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
         nbins = p * p * p;
     }
-    launch_zero_int(count, (size_t)(nbins + 1) * nb, s);
+    const size_t nz = (size_t)(nbins + 1) * nb;
+    k_fill_int_gated<<<(unsigned)((nz + 255) / 256), 256, 0, s>>>(count, nz, 0, gate);
     k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, morton, nbins, unitbox.inv[0],
-                                                              unitbox.inv[1], unitbox.inv[2], cell_of, count);
-    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, nbins);
-    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, nbins, start, count, perm);
+                                                              unitbox.inv[1], unitbox.inv[2], cell_of, count, gate);
+    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, nbins, gate);
+    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, nbins, start, count, perm, gate);
     size_t nc = (size_t)nbins * nb;
-    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted);
+    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted, gate);
 }
 
 // ------------------------------------------------------------------------------------------------
 // real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
 //
-// Cluster-pair scheme with Newton's third law.  The sites of a bead are sorted by cell in Morton
-// order (launch_cell_build with morton = 1), so each run of 64 sorted sites -- a cluster -- is
-// spatially compact.  Every unordered cluster pair {I, J} whose bounding boxes come closer than the
-// cut-off is assigned to exactly one of its clusters by a parity rule that splits each cluster's
-// neighbours in half (J in list(I) when J = I, or J > I with I + J odd, or J < I with I + J even).
-// One wave owns cluster I (lane = site i) and every RSPLIT-th entry of I's list; per entry J it
-// stages J's sites in LDS and walks the 64 j in lock step: a lane takes the pair (i, j) when the
-// FP32 distance test passes (diagonal tile: only j > i), the FP64 mode re-tests it exactly, the
-// i-side force stays in the lane and the j-side force is summed across the wave (a fixed
-// butterfly) into the lane that owns site j of J.  After the tile those j-side sums go to the
-// tile's slot, which k_ewald_jgather adds up per site in fixed order (I ascending).  Energy and
-// virial count each pair once.  T is the pair-body type; in FP32 the per-tile sums are flushed into
-// FP64 accumulators after every tile.
+// Per-site Verlet lists with a skin.  A list holds, for every site of every bead, the sites within
+// rc + skin at the time it was built; it stays exact -- it contains every pair now within rc --
+// while no site has moved more than skin/2 since (a pair distance changes by at most twice the
+// largest displacement).  Each call measures that displacement on the device (k_vl_disp,
+// k_vl_decide) and sets gate[0]; the build kernels return at once unless it is set, so a captured
+// hipGraph rebuilds exactly when needed.  The build sorts the sites by cell in Morton order
+// (launch_cell_build), groups runs of 64 sorted sites into clusters with bounding boxes, lists the
+// cluster pairs within reach, and fills each site's list (sorted indices, cluster order) with an
+// exact FP64 distance test.  The pair kernel gives one lane to a sorted site, which walks its own
+// list: every pair is evaluated from both ends (energy and virial with a factor 1/2) and each
+// site's gradient is summed by its own lane in list order, so results are deterministic.  T is the
+// pair-body type; in FP32 the distances come from wrapped fractional FP32 coordinates and the
+// sums are flushed into FP64 every 32 pairs.
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& du, T& dur) {
@@ -337,13 +353,15 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
 }
 
 #define CLU 64
-#define RSPLIT 4
+#define VLS 4   // lanes per site in the list walk
 
 __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 
 // bounding box of each cluster in wrapped fractional coordinates: bb[(b*nclu + c)*6] = {centre, half}
-__global__ __launch_bounds__(64) void k_clu_bbox(const double* __restrict__ sorted, int na, int nclu, double i0,
-                                                 double i1, double i2, double* __restrict__ bb) {
+__global__ __launch_bounds__(64) void k_clu_bbox_gated(const double* __restrict__ sorted, int na, int nclu,
+                                                       double i0, double i1, double i2, double* __restrict__ bb,
+                                                       const int* __restrict__ gate) {
+    if (!gate[0]) return;
     const int c = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
     const int a = c * CLU + l;
     const double* X = sorted + (size_t)b * 3 * na;
@@ -371,266 +389,342 @@ __global__ __launch_bounds__(64) void k_clu_bbox(const double* __restrict__ sort
     }
 }
 
-__host__ __device__ __forceinline__ int clu_cap(int nclu) { return nclu / 2 + 2; }
-
-// half cluster-pair lists, one wave per (I, bead): J ascending, kept when the bounding boxes come
-// closer than rc (minimum image of the centres; L = box side, 1 for the unit box) and the parity
-// rule gives the pair to I.  rev[(b*nclu + J)*nclu + I] = position in list(I) + 1 (0: absent).
+// full cluster-pair lists, one wave per (I, bead): J ascending with bounding boxes closer than rc
 __global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ bb, int nclu, double L0, double L1,
                                                   double L2, double rc2, int* __restrict__ cnt,
-                                                  int* __restrict__ list, int* __restrict__ rev) {
+                                                  int* __restrict__ list, const int* __restrict__ gate) {
+    if (!gate[0]) return;
     const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
     const double* bi = bb + ((size_t)b * nclu + I) * 6;
     const double L[3] = {L0, L1, L2};
-    const int cap = clu_cap(nclu);
-    int* out = list + ((size_t)b * nclu + I) * cap;
+    int* out = list + ((size_t)b * nclu + I) * nclu;
     int n = 0;
     for (int J0 = 0; J0 < nclu; J0 += 64) {
         const int J = J0 + l;
         bool keep = false;
         if (J < nclu) {
-            const bool mine = J == I || (J > I && ((I + J) & 1)) || (J < I && !((I + J) & 1));
-            if (mine) {
-                const double* bj = bb + ((size_t)b * nclu + J) * 6;
-                double r2 = 0.0;
+            const double* bj = bb + ((size_t)b * nclu + J) * 6;
+            double r2 = 0.0;
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    double dc = bi[d] - bj[d];
-                    dc -= rint(dc);
-                    const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
-                    r2 += gap * gap;
-                }
-                keep = r2 < rc2;
+            for (int d = 0; d < 3; d++) {
+                double dc = bi[d] - bj[d];
+                dc -= rint(dc);
+                const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
+                r2 += gap * gap;
             }
+            keep = r2 < rc2;
         }
         const uint64_t m = __ballot(keep);
-        if (keep) {
-            const int pos = n + __popcll(m & ((1ull << l) - 1));
-            out[pos] = J;
-            rev[((size_t)b * nclu + J) * nclu + I] = pos + 1;
-        }
+        if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
         n += __popcll(m);
     }
     if (l == 0) cnt[b * nclu + I] = n;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// largest squared displacement of any site since the last build, per block
+__global__ __launch_bounds__(256) void k_vl_disp(const double* __restrict__ site, const double* __restrict__ rb,
+                                                 size_t n3, size_t stride, double* __restrict__ dparts) {
+    // site and rb are [nb][3][na]; element e = (b, d, a) -> squared displacement summed over d by
+    // visiting the three components of one (b, a) in one thread
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (t < n3) {
+        const size_t b = t / stride, a = t % stride;
+        const size_t o = b * 3 * stride + a;
+        const double dx = site[o] - rb[o], dy = site[o + stride] - rb[o + stride],
+                     dz = site[o + 2 * stride] - rb[o + 2 * stride];
+        m = dx * dx + dy * dy + dz * dz;
+        if (!(m == m)) m = 1e300;  // never built (NaN reference positions)
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) dparts[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_ewald_half(const double* __restrict__ sorted, const int* __restrict__ perm,
-                                                   const double* __restrict__ q, int na, int nclu,
-                                                   const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                   double L0, double L1, double L2, double i0, double i1, double i2,
-                                                   double rc2, float rc2f, double alpha, double sfac,
-                                                   double* __restrict__ rpart, double* __restrict__ jpart,
-                                                   double* __restrict__ parts) {
-    constexpr bool F64 = sizeof(T) == 8;
-    const int I = blockIdx.x, sp = blockIdx.y, b = blockIdx.z, nb = gridDim.z;
-    const int l = threadIdx.x;
-    const int i = I * CLU + l;
-    const bool iv = i < na;
-    const int cap = clu_cap(nclu);
-    const double* X = sorted + (size_t)b * 3 * na;
-    const int* P = perm + (size_t)b * na;
-    __shared__ double sx[CLU], sy[CLU], sz[CLU];
-    __shared__ float fx[CLU], fy[CLU], fz[CLU];
-    __shared__ T sq[CLU];
-    __shared__ int sm[CLU];
-    double xi = 0, yi = 0, zi = 0;
-    T qi = 0;
-    int ai = 0, mi = -1;
-    float fxi = 0, fyi = 0, fzi = 0;
-    if (iv) {
-        xi = X[i];
-        yi = X[na + i];
-        zi = X[2 * na + i];
-        ai = P[i];
-        qi = (T)q[ai];
-        mi = ai / 3;
-        fxi = (float)wrap01(xi * i0);
-        fyi = (float)wrap01(yi * i1);
-        fzi = (float)wrap01(zi * i2);
+// gate[0] = rebuild (largest displacement above skin/2, or forced); gate[1] (overflow) is sticky
+__global__ __launch_bounds__(256) void k_vl_decide(const double* __restrict__ dparts, int nblk, double lim2,
+                                                   int force, int* __restrict__ gate) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += 256) m = fmax(m, dparts[q]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
     }
-    const float Lf0 = (float)L0, Lf1 = (float)L1, Lf2 = (float)L2;
-    double acc[10];
-#pragma unroll
-    for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    const T ta = (T)alpha, tsf = (T)sfac;
+    if (threadIdx.x == 0) gate[0] = (force || red[0] > lim2) ? 1 : 0;
+}
+
+// list entry n of sorted site k: the VLS lanes of a site read consecutive words
+__device__ __forceinline__ size_t vl_slot(int n, int k, int na) {
+    return ((size_t)(n / VLS) * na + k) * VLS + (n % VLS);
+}
+
+// inverse of the sort permutation of the build: iperm[b][site] = sorted index
+__global__ void k_vl_iperm(const int* __restrict__ perm, int na, int nb, int* __restrict__ iperm,
+                           const int* __restrict__ gate) {
+    if (!gate[0]) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)na * nb) return;
+    const size_t b = t / na;
+    iperm[b * na + perm[t]] = (int)(t % na);
+}
+
+// per sorted site: its list of sorted neighbours within rc + skin (exact FP64 test); the sites of
+// its own molecule first (nex[] of them; the real-space term treats those pairs as excluded,
+// src/ewald_cubic.f90:134), then the others in cluster order.  Also the reference positions and
+// the charges of the build.
+__global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sorted, const int* __restrict__ perm,
+                                                 const int* __restrict__ iperm, const double* __restrict__ site,
+                                                 const double* __restrict__ q, int na, int nclu,
+                                                 const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                 double L0, double L1, double L2, double i0, double i1, double i2,
+                                                 double rl2, int cap, int* __restrict__ cnt, int* __restrict__ nex,
+                                                 int* __restrict__ nl, double* __restrict__ rb,
+                                                 double* __restrict__ sq, int* __restrict__ gate) {
+    if (!gate[0]) return;
+    const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+    const int k = I * CLU + l;
+    const bool kv = k < na;
+    const double* X = sorted + (size_t)b * 3 * na;
+    __shared__ double sx[CLU], sy[CLU], sz[CLU];
+    __shared__ int sa[CLU];
+    double xi = 0, yi = 0, zi = 0;
+    int a = -3, n = 0;
+    bool ovf = false;
+    int* out = nl + (size_t)b * cap * na;
+    auto dist2 = [&](double xj, double yj, double zj) {
+        double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        dx -= L0 * rint(dx * i0);
+        dy -= L1 * rint(dy * i1);
+        dz -= L2 * rint(dz * i2);
+        return dx * dx + dy * dy + dz * dz;
+    };
+    if (kv) {
+        xi = X[k];
+        yi = X[na + k];
+        zi = X[2 * na + k];
+        a = perm[(size_t)b * na + k];
+        const double* S = site + (size_t)b * 3 * na;
+        double* R = rb + (size_t)b * 3 * na;
+        R[a] = S[a];
+        R[na + a] = S[na + a];
+        R[2 * na + a] = S[2 * na + a];
+        sq[(size_t)b * na + k] = q[a];
+        const int m0 = 3 * (a / 3);
+        for (int p = m0; p < m0 + 3; p++) {
+            if (p == a || p >= na) continue;
+            const int jk = iperm[(size_t)b * na + p];
+            if (dist2(X[jk], X[na + jk], X[2 * na + jk]) < rl2) {
+                if (n < cap) out[vl_slot(n, k, na)] = jk;
+                else ovf = true;
+                n++;
+            }
+        }
+        nex[(size_t)b * na + k] = min(n, cap);
+    }
     const int nJ = ccnt[b * nclu + I];
-    const int* lst = clist + ((size_t)b * nclu + I) * cap;
-    for (int n = sp; n < nJ; n += RSPLIT) {
-        const int J = lst[n];
+    const int* lst = clist + ((size_t)b * nclu + I) * nclu;
+    const int mi = a / 3;
+    for (int e = 0; e < nJ; e++) {
+        const int J = lst[e];
         const int j = J * CLU + l;
         __syncthreads();
         if (j < na) {
-            const double xj = X[j], yj = X[na + j], zj = X[2 * na + j];
-            sx[l] = xj;
-            sy[l] = yj;
-            sz[l] = zj;
-            fx[l] = (float)wrap01(xj * i0);
-            fy[l] = (float)wrap01(yj * i1);
-            fz[l] = (float)wrap01(zj * i2);
-            const int aj = P[j];
-            sm[l] = aj / 3;
-            sq[l] = (T)q[aj];
+            sx[l] = X[j];
+            sy[l] = X[na + j];
+            sz[l] = X[2 * na + j];
+            sa[l] = perm[(size_t)b * na + j];
         }
         __syncthreads();
         const int jmax = min(CLU, na - J * CLU);
-        const bool diag = J == I;
-        T ti[3] = {0, 0, 0}, tj[3] = {0, 0, 0}, ta10[10];
-#pragma unroll
-        for (int k = 0; k < 10; k++) ta10[k] = 0;
-        for (int jj = 0; jj < jmax; jj++) {
-            float dxf = fxi - fx[jj], dyf = fyi - fy[jj], dzf = fzi - fz[jj];
-            dxf = (dxf - rintf(dxf)) * Lf0;
-            dyf = (dyf - rintf(dyf)) * Lf1;
-            dzf = (dzf - rintf(dzf)) * Lf2;
-            const float r2f = dxf * dxf + dyf * dyf + dzf * dzf;
-            bool pass = iv && r2f < rc2f && (!diag || jj > l);
-            T dx = 0, dy = 0, dz = 0, dr2 = 0;
-            if (pass) {
-                if (F64) {
-                    double ex = xi - sx[jj], ey = yi - sy[jj], ez = zi - sz[jj];
-                    ex -= L0 * rint(ex * i0);
-                    ey -= L1 * rint(ey * i1);
-                    ez -= L2 * rint(ez * i2);
-                    dx = (T)ex;
-                    dy = (T)ey;
-                    dz = (T)ez;
-                    dr2 = (T)(ex * ex + ey * ey + ez * ez);
-                    pass = (double)dr2 < rc2;
-                } else {
-                    dx = (T)dxf;
-                    dy = (T)dyf;
-                    dz = (T)dzf;
-                    dr2 = (T)r2f;
-                }
-            }
-            T fxx = 0, fyy = 0, fzz = 0;
-            if (pass) {
-                T du, dur;
-                erfc_pair<T>(dr2, ta, tsf, sm[jj] == mi, du, dur);
-                const T qij = qi * sq[jj];
-                const T dv = qij * du, dvr = qij * dur;
-                fxx = dvr * dx;
-                fyy = dvr * dy;
-                fzz = dvr * dz;
-                ti[0] += fxx;
-                ti[1] += fyy;
-                ti[2] += fzz;
-                ta10[0] += dv;
-                ta10[1] += dx * fxx; ta10[2] += dy * fxx; ta10[3] += dz * fxx;
-                ta10[4] += dx * fyy; ta10[5] += dy * fyy; ta10[6] += dz * fyy;
-                ta10[7] += dx * fzz; ta10[8] += dy * fzz; ta10[9] += dz * fzz;
-            }
-            if (__any(pass)) {
-                const T sxj = wave_sum<T>(fxx), syj = wave_sum<T>(fyy), szj = wave_sum<T>(fzz);
-                if (l == jj) {
-                    tj[0] -= sxj;
-                    tj[1] -= syj;
-                    tj[2] -= szj;
+        if (kv) {
+            for (int jj = 0; jj < jmax; jj++) {
+                const int jk = J * CLU + jj;
+                if (jk == k || sa[jj] / 3 == mi) continue;
+                if (dist2(sx[jj], sy[jj], sz[jj]) < rl2) {
+                    if (n < cap) out[vl_slot(n, k, na)] = jk;
+                    else ovf = true;
+                    n++;
                 }
             }
         }
-        gx += (double)ti[0];
-        gy += (double)ti[1];
-        gz += (double)ti[2];
-#pragma unroll
-        for (int k = 0; k < 10; k++) acc[k] += (double)ta10[k];
-        if (diag) {
-            gx += (double)tj[0];
-            gy += (double)tj[1];
-            gz += (double)tj[2];
-        } else {
-            double* o = jpart + (((size_t)b * nclu + I) * cap + n) * (3 * CLU);
-            o[l] = (double)tj[0];
-            o[CLU + l] = (double)tj[1];
-            o[2 * CLU + l] = (double)tj[2];
-        }
     }
-    if (iv) {
-        double* G = rpart + ((size_t)sp * nb + b) * 3 * na;
-        G[ai] = gx;
-        G[na + ai] = gy;
-        G[2 * na + ai] = gz;
-    }
-    block_reduce_store<10>(acc, parts + (((size_t)b * RSPLIT + sp) * nclu + I) * 10);
+    if (kv) cnt[(size_t)b * na + k] = min(n, cap);
+    if (ovf) gate[1] = 1;
 }
 
-// j-side sums of every tile that touched site j, I ascending, into gradient slot RSPLIT
-__global__ __launch_bounds__(64) void k_ewald_jgather(const int* __restrict__ perm, int na, int nclu,
-                                                      const int* __restrict__ rev, const double* __restrict__ jpart,
-                                                      double* __restrict__ rpart) {
-    const int J = blockIdx.x, b = blockIdx.y, nb = gridDim.y, l = threadIdx.x;
-    const int j = J * CLU + l;
-    const int cap = clu_cap(nclu);
-    const int* rv = rev + ((size_t)b * nclu + J) * nclu;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int I = 0; I < nclu; I++) {
-        const int n = rv[I] - 1;
-        if (n < 0 || I == J) continue;
-        const double* o = jpart + (((size_t)b * nclu + I) * cap + n) * (3 * CLU);
-        s0 += o[l];
-        s1 += o[CLU + l];
-        s2 += o[2 * CLU + l];
+// current coordinates in sorted order: (x, y, z, q) in FP64 and wrapped fractional FP32
+__global__ void k_vl_gather(const double* __restrict__ site, const int* __restrict__ perm,
+                            const double* __restrict__ sq, int na, int nb, double i0, double i1, double i2,
+                            double4* __restrict__ srt, float4* __restrict__ srf) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)na * nb) return;
+    const size_t b = t / na;
+    const int a = perm[t];
+    const double* S = site + b * 3 * na;
+    const double x = S[a], y = S[na + a], z = S[2 * na + a], qq = sq[t];
+    srt[t] = make_double4(x, y, z, qq);
+    srf[t] = make_float4((float)wrap01(x * i0), (float)wrap01(y * i1), (float)wrap01(z * i2), (float)qq);
+}
+
+// one pair of the list walk: gradient and the 7 independent sums {v, xx, xy, xz, yy, yz, zz}
+// (the pair force is central, so the virial dx_a f_b is symmetric)
+template <typename T>
+__device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool same, T ta, T tsf, T (&tg)[3],
+                                        T (&tv)[7]) {
+    T du, dur;
+    erfc_pair<T>(dr2, ta, tsf, same, du, dur);
+    const T dv = qij * du, dvr = qij * dur;
+    const T fxx = dvr * dx, fyy = dvr * dy, fzz = dvr * dz;
+    tg[0] += fxx;
+    tg[1] += fyy;
+    tg[2] += fzz;
+    tv[0] += dv;
+    tv[1] += dx * fxx;
+    tv[2] += dx * fyy;
+    tv[3] += dx * fzz;
+    tv[4] += dy * fyy;
+    tv[5] += dy * fzz;
+    tv[6] += dz * fzz;
+}
+
+// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... with the next neighbour's
+// coordinates loaded while the current pair is computed; the site's gradient is summed over its
+// lanes by a fixed butterfly
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
+                                                  const int* __restrict__ perm, const int* __restrict__ cnt,
+                                                  const int* __restrict__ nex, const int* __restrict__ nl, int na,
+                                                  int cap, double L0, double L1, double L2, double i0, double i1,
+                                                  double i2, double rc2, double alpha, double sfac,
+                                                  double* __restrict__ grad, double* __restrict__ parts) {
+    constexpr bool F64 = sizeof(T) == 8;
+    typedef typename std::conditional<F64, double4, float4>::type P4;
+    const int b = blockIdx.y;
+    const int k = (blockIdx.x * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
+    double acc[7];
+#pragma unroll
+    for (int c = 0; c < 7; c++) acc[c] = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (k < na) {
+        const size_t bo = (size_t)b * na;
+        const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
+        const int* L = nl + (size_t)b * cap * na + (size_t)k * VLS + sl;
+        const size_t lstride = (size_t)na * VLS;
+        const int nn = cnt[bo + k], ne = nex[bo + k];
+        const P4 xi = X[k];
+        const T qi = (T)xi.w;
+        const T Lt0 = (T)L0, Lt1 = (T)L1, Lt2 = (T)L2, it0 = (T)i0, it1 = (T)i1, it2 = (T)i2;
+        const T trc2 = (T)rc2;
+        const T ta = (T)alpha, tsf = (T)sfac;
+        T tg[3] = {0, 0, 0}, tv[7];
+#pragma unroll
+        for (int c = 0; c < 7; c++) tv[c] = 0;
+        int n = sl, r = 0;
+        P4 xj = X[n < nn ? L[0] : k];
+        for (; n < nn; n += VLS, r++) {
+            const P4 cur = xj;
+            xj = X[n + VLS < nn ? L[(size_t)(r + 1) * lstride] : k];
+            T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
+            if (F64) {
+                // physical minimum image (the unit box when cubic)
+                ex -= Lt0 * rint(ex * it0);
+                ey -= Lt1 * rint(ey * it1);
+                ez -= Lt2 * rint(ez * it2);
+            } else {
+                // wrapped fractional coordinates
+                ex = (ex - rint(ex)) * Lt0;
+                ey = (ey - rint(ey)) * Lt1;
+                ez = (ez - rint(ez)) * Lt2;
+            }
+            const T r2 = ex * ex + ey * ey + ez * ez;
+            if (r2 < trc2) vl_pair<T>(ex, ey, ez, r2, qi * (T)cur.w, n < ne, ta, tsf, tg, tv);
+            if (!F64 && (r & 15) == 15) {
+                gx += (double)tg[0];
+                gy += (double)tg[1];
+                gz += (double)tg[2];
+                tg[0] = tg[1] = tg[2] = 0;
+#pragma unroll
+                for (int c = 0; c < 7; c++) {
+                    acc[c] += (double)tv[c];
+                    tv[c] = 0;
+                }
+            }
+        }
+        gx += (double)tg[0];
+        gy += (double)tg[1];
+        gz += (double)tg[2];
+#pragma unroll
+        for (int c = 0; c < 7; c++) acc[c] = 0.5 * (acc[c] + (double)tv[c]);
     }
-    if (j < na) {
-        const int aj = perm[(size_t)b * na + j];
-        double* G = rpart + ((size_t)RSPLIT * nb + b) * 3 * na;
-        G[aj] = s0;
-        G[na + aj] = s1;
-        G[2 * na + aj] = s2;
+#pragma unroll
+    for (int off = 1; off < VLS; off <<= 1) {
+        gx += __shfl_xor(gx, off, 64);
+        gy += __shfl_xor(gy, off, 64);
+        gz += __shfl_xor(gz, off, 64);
     }
+    if (k < na && sl == 0) {
+        const int a = perm[(size_t)b * na + k];
+        double* G = grad + (size_t)b * 3 * na;
+        G[a] = gx;
+        G[na + a] = gy;
+        G[2 * na + a] = gz;
+    }
+    // {v, vir(3,3) column-major} from the symmetric sums
+    double w[10] = {acc[0], acc[1], acc[2], acc[3], acc[2], acc[4], acc[5], acc[3], acc[5], acc[6]};
+    block_reduce_store<10>(w, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
 }
 
 int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
-int ewald_rsplit() { return RSPLIT; }
-int ewald_real_slots() { return RSPLIT + 1; }
-int ewald_real_parts(int na) { return RSPLIT * ewald_nclu(na); }
-size_t ewald_list_size(int na) { return (size_t)ewald_nclu(na) * clu_cap(ewald_nclu(na)); }
-size_t ewald_jpart_size(int na) { return ewald_list_size(na) * 3 * CLU; }
+int ewald_real_slots() { return 1; }
+int ewald_real_parts(int na) { return (na * VLS + 255) / 256; }
 
-// Pair-test margins.  FP32 wrapped fractional coordinates carry an absolute error of about
-// 2^-24 per component, so |dr| errs by less than 1e-6 of the box side.
-// The margin below grows as rc^2 (1e-4 + 1e-5 L_max/rc) so short cut-offs stay covered too.
-static double pair_margin(const Box& sb, double rc) {
-    const double lmax = fmax(sb.L[0], fmax(sb.L[1], sb.L[2]));
-    return 1e-4 + 1e-5 * lmax / rc;
+// expected neighbours within rl of a site, with margin, as the list capacity
+int vlist_cap(int na, Box sb, double rl) {
+    const double vol = sb.L[0] * sb.L[1] * sb.L[2];
+    const double frac = 4.0 / 3.0 * acos(-1.0) * rl * rl * rl / vol;
+    double c = 1.5 * frac * na + 64.0;
+    if (c > na - 1) c = na - 1;
+    int cap = (int)ceil(c);
+    if (cap < 1) cap = 1;
+    return (cap + VLS - 1) / VLS * VLS;
 }
 
-void launch_clusters(const double* sorted, int na, int nb, Box sb, double rc, double* bb, int* ccnt, int* clist,
-                     int* rev, hipStream_t s) {
+// The list cut-off rl = rc + skin; the build rebuilds when a site has moved by more than skin/2.
+void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
+                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, double* bb,
+                  hipStream_t s) {
+    const size_t nsite = (size_t)na * nb;
+    const unsigned nblk = (unsigned)((nsite + 255) / 256);
+    k_vl_disp<<<nblk, 256, 0, s>>>(site, v.rb, nsite, (size_t)na, v.dparts);
+    k_vl_decide<<<1, 256, 0, s>>>(v.dparts, (int)nblk, 0.25 * skin * skin, force, v.gate);
+    launch_cell_build(site, na, nb, g, sb, 1, cell_of, count, start, v.perm, sorted, v.gate, s);
     const int nclu = ewald_nclu(na);
-    launch_zero_int(rev, (size_t)nclu * nclu * nb, s);
-    k_clu_bbox<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.inv[0], sb.inv[1], sb.inv[2], bb);
-    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2],
-                                             rc * rc * (1.0 + pair_margin(sb, rc)), ccnt, clist, rev);
+    const double rl = rc + skin;
+    k_clu_bbox_gated<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.inv[0], sb.inv[1], sb.inv[2], bb, v.gate);
+    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2], rl * rl * (1.0 + 1e-10), v.ccnt,
+                                             v.clist, v.gate);
+    k_vl_iperm<<<nblk, 256, 0, s>>>(v.perm, na, nb, v.iperm, v.gate);
+    k_vl_build<<<dim3(nclu, nb), 64, 0, s>>>(sorted, v.perm, v.iperm, site, q, na, nclu, v.ccnt, v.clist, sb.L[0],
+                                            sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rl * rl, v.cap, v.cnt,
+                                            v.nex, v.nl, v.rb, v.sq, v.gate);
+    k_vl_gather<<<nblk, 256, 0, s>>>(site, v.perm, v.sq, na, nb, sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf);
 }
 
-// rpart: RSPLIT + 1 gradient slots [slot][bead][3][na] by original atom index; jpart: tile slots
 template <typename T>
-void launch_ewald_real(const double* sorted, const int* perm, const double* q, int na, int nb, Box sb, double rcut,
-                       double alpha, const int* ccnt, const int* clist, const int* rev, double* jpart, double* rpart,
-                       double* parts, hipStream_t s) {
+void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
+                       hipStream_t s) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
-    const int nclu = ewald_nclu(na);
-    // FP64: a margin, then the exact test; FP32: the cut-off itself
-    const float rc2f = sizeof(T) == 8 ? (float)(rcut * rcut * (1.0 + pair_margin(sb, rcut))) : (float)(rcut * rcut);
-    k_ewald_half<T><<<dim3(nclu, RSPLIT, nb), 64, 0, s>>>(sorted, perm, q, na, nclu, ccnt, clist, sb.L[0], sb.L[1],
-                                                         sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, rc2f,
-                                                         alpha, sfac, rpart, jpart, parts);
-    k_ewald_jgather<<<dim3(nclu, nb), 64, 0, s>>>(perm, na, nclu, rev, jpart, rpart);
+    k_ewald_vl<T><<<dim3((na * VLS + 255) / 256, nb), 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, v.cap,
+                                                            sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
+                                                            sb.inv[2], rcut * rcut, alpha, sfac, grad, parts);
 }
-template void launch_ewald_real<double>(const double*, const int*, const double*, int, int, Box, double, double,
-                                        const int*, const int*, const int*, double*, double*, double*, hipStream_t);
-template void launch_ewald_real<float>(const double*, const int*, const double*, int, int, Box, double, double,
-                                       const int*, const int*, const int*, double*, double*, double*, hipStream_t);
+template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
+template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
