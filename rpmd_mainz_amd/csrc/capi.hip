@@ -433,19 +433,21 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_phase(c->site, na, nb, c->ks, c->ph, s);
         {
             ProfScope ps(c, 1);
-            launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, s);
+            launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, mixed, s);
         }
         {
             ProfScope ps(c, 3);
             launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, pk, s);
         }
+        int nkslots = 0;
         {
             ProfScope ps(c, 2);
-            launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, s);
+            nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk,
+                                    c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
-        // real-space slots (splits, then the j-side gather) then k-space kx-group slots, in that order
-        launch_chain_rule(grad, c->fpart, ewald_real_slots() + kspace_ngroups(P.kmax), na, nb, P.alpha, sc * sc, s);
+        // real-space slot then the k-space slots, summed in that order
+        launch_chain_rule(grad, c->fpart, ewald_real_slots() + nkslots, na, nb, P.alpha, sc * sc, s);
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
         launch_reduce_ev(pr, nb * ewald_real_parts(na), 10, ev_slot + 0, 10, sc / nb, s);
         launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);

@@ -789,20 +789,42 @@ __global__ void k_phase(const double* __restrict__ site, int na, int nb, double 
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA in the precision of T; D layouts (lane l, item v; lk = l >> 4, column l & 15):
+// f64 row lk + 4 v, f32 row 4 lk + v.  A and B: lane l gives A[l % 16][l / 16], B[l / 16][l % 16].
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef dbl4 acc;
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <> struct Mfma<float> {
+    typedef flt4 acc;
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+
+// T: MFMA precision (FP64, or FP32 in the mixed mode; tables formed in FP64 and rounded once)
+template <typename T>
 __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph, const double* __restrict__ q, int na,
                                                  int nchunk, KDev kd, double2* __restrict__ skp) {
+    typedef typename Mfma<T>::acc acc_t;
     const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int qa = w >> 1, qc = w & 1;   // ky half, kz half of this wave
     const int kmax = kd.kmax;
-    __shared__ double Yr[NJ][KT], Yi[NJ][KT], Zr[NJ][KT], Zi[NJ][KT];
-    __shared__ double Xr[KXB][NJ], Xi[KXB][NJ];
-    dbl4 accR[KXB], accI[KXB];
+    __shared__ T Yr[NJ][KT], Yi[NJ][KT], Zr[NJ][KT], Zi[NJ][KT];
+    __shared__ T Xr[KXB][NJ], Xi[KXB][NJ];
+    acc_t accR[KXB], accI[KXB];
 #pragma unroll
     for (int g = 0; g < KXB; g++) {
-        accR[g] = (dbl4){0.0, 0.0, 0.0, 0.0};
-        accI[g] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        accR[g] = (acc_t){0, 0, 0, 0};
+        accI[g] = (acc_t){0, 0, 0, 0};
     }
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
@@ -825,10 +847,10 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
             for (int m = 0; m < 4; m++) {
                 const int k = k0 + m;
                 const bool ok = in && k >= -kmax && k <= kmax;
-                Yr[j][4 * part + m] = ok ? yk.x : 0.0;
-                Yi[j][4 * part + m] = ok ? yk.y : 0.0;
-                Zr[j][4 * part + m] = ok ? zk.x : 0.0;
-                Zi[j][4 * part + m] = ok ? zk.y : 0.0;
+                Yr[j][4 * part + m] = ok ? (T)yk.x : (T)0;
+                Yi[j][4 * part + m] = ok ? (T)yk.y : (T)0;
+                Zr[j][4 * part + m] = ok ? (T)zk.x : (T)0;
+                Zi[j][4 * part + m] = ok ? (T)zk.y : (T)0;
                 yk = cmul(yk, y1);
                 zk = cmul(zk, z1);
             }
@@ -840,29 +862,28 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
                     const double2 p = cpow(x1, kx);
                     xv = make_double2(q[aa] * p.x, q[aa] * p.y);
                 }
-                Xr[g][jj] = xv.x;
-                Xi[g][jj] = xv.y;
+                Xr[g][jj] = (T)xv.x;
+                Xi[g][jj] = (T)xv.y;
             }
         }
         __syncthreads();
 #pragma unroll 2
         for (int st = 0; st < NJ / 4; st++) {
             const int j = 4 * st + lk;
-            const double ar = Yr[j][16 * qa + li], ai = Yi[j][16 * qa + li];
-            const double zr = Zr[j][16 * qc + li], zi = Zi[j][16 * qc + li];
+            const T ar = Yr[j][16 * qa + li], ai = Yi[j][16 * qa + li];
+            const T zr = Zr[j][16 * qc + li], zi = Zi[j][16 * qc + li];
 #pragma unroll
             for (int g = 0; g < KXB; g++) {
-                const double xr = Xr[g][j], xi = Xi[g][j];
-                const double br = xr * zr - xi * zi, bi = xr * zi + xi * zr;
-                accR[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accR[g], 0, 0, 0);
-                accR[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, accR[g], 0, 0, 0);
-                accI[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, accI[g], 0, 0, 0);
-                accI[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, accI[g], 0, 0, 0);
+                const T xr = Xr[g][j], xi = Xi[g][j];
+                const T br = xr * zr - xi * zi, bi = xr * zi + xi * zr;
+                accR[g] = Mfma<T>::op(ar, br, accR[g]);
+                accR[g] = Mfma<T>::op(-ai, bi, accR[g]);
+                accI[g] = Mfma<T>::op(ar, bi, accI[g]);
+                accI[g] = Mfma<T>::op(ai, br, accI[g]);
             }
         }
         __syncthreads();
     }
-    // D layout of v_mfma_f64_16x16x4f64: lane l, item v -> row lk + 4v, column li
 #pragma unroll
     for (int g = 0; g < KXB; g++) {
         const int kx = kxg * KXB + g;
@@ -870,8 +891,8 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
         double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int iy = 16 * qa + lk + 4 * v, iz = 16 * qc + li;
-            o[iy * KT + iz] = make_double2(accR[g][v], accI[g][v]);
+            const int iy = 16 * qa + Mfma<T>::row(lk, v), iz = 16 * qc + li;
+            o[iy * KT + iz] = make_double2((double)accR[g][v], (double)accI[g][v]);
         }
     }
 }
@@ -999,6 +1020,142 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
     o[2 * na + a] = gz;
 }
 
+// ---- mixed mode: the force pass as a GEMM on FP32 MFMA --------------------------------------------
+// For each kx, U = [G; kz G] conj(Z): A = the (ky, kz) tile of G = 2 w S (rows 32..63 weighted by
+// kz), B = conj(Z_j(kz)) for 64 atoms, so U(ky, j) = sum_kz G(ky, kz) e^{-i kz z_j} and its kz-weighted
+// twin; then per atom the sums over ky of Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the
+// twin) give the three gradient components, exactly the quantities the Horner pass forms.
+// One block per (64 atoms, bead) and all kx; wave w owns rows 16 w .. 16 w + 15 of U.
+#define KFA 64      // atoms per block
+#define KPAD 33     // padded LDS row (no bank conflicts on the 16-row MFMA operand reads)
+__global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict__ ph, const double* __restrict__ q,
+                                                       int na, KDev kd, const double2* __restrict__ gk,
+                                                       double* __restrict__ kpart) {
+    typedef Mfma<float> M;
+    const int a0 = blockIdx.x * KFA, b = blockIdx.y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int kmax = kd.kmax;
+    __shared__ float Gr[KT][KPAD], Gi[KT][KPAD];    // G(kx) tile [iy][iz]
+    __shared__ float Zr[KFA][KPAD], Zi[KFA][KPAD];  // conj(Z_j(kz)) [atom][iz]
+    __shared__ float Yr[KFA][KPAD], Yi[KFA][KPAD];  // Y_j(ky) [atom][iy]
+    __shared__ float Xr[KFA], Xi[KFA];              // q X_j^kx
+    __shared__ float red[4][KFA][2];                // per wave: (plain, ky-weighted) row sums
+    const double2* PX = ph + ((size_t)b * 3 + 0) * na;
+    const double2* PY = ph + ((size_t)b * 3 + 1) * na;
+    const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
+    // Y and conj(Z) tables: thread -> (atom t / 4, entries 8 (t % 4) .. + 7)
+    {
+        const int j = t >> 2, part = t & 3, a = a0 + j;
+        const int k0 = 8 * part - KO;
+        double2 y1 = make_double2(1.0, 0.0), z1 = make_double2(1.0, 0.0);
+        const bool in = a < na;
+        if (in) {
+            y1 = PY[a];
+            z1 = PZ[a];
+        }
+        double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
+        double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int k = k0 + m;
+            const bool ok = in && k >= -kmax && k <= kmax;
+            Yr[j][8 * part + m] = ok ? (float)yk.x : 0.0f;
+            Yi[j][8 * part + m] = ok ? (float)yk.y : 0.0f;
+            Zr[j][8 * part + m] = ok ? (float)zk.x : 0.0f;
+            Zi[j][8 * part + m] = ok ? -(float)zk.y : 0.0f;
+            yk = cmul(yk, y1);
+            zk = cmul(zk, z1);
+        }
+    }
+    double2 xk = make_double2(0.0, 0.0), x1 = make_double2(1.0, 0.0);
+    if (t < KFA && a0 + t < na) {
+        x1 = PX[a0 + t];
+        xk = make_double2(q[a0 + t], 0.0);
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
+    const int mb = w & 1, wz = w >> 1;  // row block of G, kz-weighted twin
+    for (int kx = 0; kx <= kmax; kx++) {
+        __syncthreads();
+        {
+            const double2* Gx = G0 + (size_t)kx * KT * KT;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int idx = t + 256 * e, iy = idx >> 5, iz = idx & 31;
+                const double2 g = Gx[idx];
+                Gr[iy][iz] = (float)g.x;
+                Gi[iy][iz] = (float)g.y;
+            }
+            if (t < KFA) {
+                if (kx > 0) xk = cmul(xk, x1);
+                Xr[t] = (float)xk.x;
+                Xi[t] = (float)xk.y;
+            }
+        }
+        __syncthreads();
+        if (kd.kyb[kx] < 0) continue;  // uniform: no member at this kx
+        typename M::acc ur[4], ui[4];
+#pragma unroll
+        for (int nbk = 0; nbk < 4; nbk++) {
+            ur[nbk] = (typename M::acc){0, 0, 0, 0};
+            ui[nbk] = (typename M::acc){0, 0, 0, 0};
+        }
+#pragma unroll 2
+        for (int st = 0; st < KT / 4; st++) {
+            const int iz = 4 * st + lk;
+            const float wk = wz ? (float)(iz - KO) : 1.0f;
+            const float ar = wk * Gr[16 * mb + li][iz], ai = wk * Gi[16 * mb + li][iz];
+#pragma unroll
+            for (int nbk = 0; nbk < 4; nbk++) {
+                const float br = Zr[16 * nbk + li][iz], bi = Zi[16 * nbk + li][iz];
+                ur[nbk] = M::op(ar, br, ur[nbk]);
+                ur[nbk] = M::op(-ai, bi, ur[nbk]);
+                ui[nbk] = M::op(ar, bi, ui[nbk]);
+                ui[nbk] = M::op(ai, br, ui[nbk]);
+            }
+        }
+        // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 nbk + li
+#pragma unroll
+        for (int nbk = 0; nbk < 4; nbk++) {
+            const int j = 16 * nbk + li;
+            const float xr = Xr[j], xi = Xi[j];
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int iy = 16 * mb + M::row(lk, v);
+                const float yr = Yr[j][iy], yi = Yi[j][iy];
+                const float pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;   // q X^kx Y^ky
+                const float ipu = pr * ui[nbk][v] - pi * ur[nbk][v];          // Im(conj(p) U)
+                s0 += ipu;
+                s1 += (float)(iy - KO) * ipu;
+            }
+            s0 += __shfl_xor(s0, 16, 64);
+            s1 += __shfl_xor(s1, 16, 64);
+            s0 += __shfl_xor(s0, 32, 64);
+            s1 += __shfl_xor(s1, 32, 64);
+            if (lk == 0) {
+                red[w][j][0] = s0;
+                red[w][j][1] = s1;
+            }
+        }
+        __syncthreads();
+        if (t < KFA) {
+            const double tx = (double)red[0][t][0] + (double)red[1][t][0];
+            const double ty = (double)red[0][t][1] + (double)red[1][t][1];
+            const double tz = (double)red[2][t][0] + (double)red[3][t][0];
+            gx += kd.lat0 * kx * tx;
+            gy += kd.lat1 * ty;
+            gz += kd.lat2 * tz;
+        }
+    }
+    if (t < KFA && a0 + t < na) {
+        double* o = kpart + (size_t)b * 3 * na;
+        o[a0 + t] = gx;
+        o[na + a0 + t] = gy;
+        o[2 * na + a0 + t] = gz;
+    }
+}
+
 int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
 int kspace_ngroups(int kmax) { return (kmax + KXF) / KXF; }
 
@@ -1022,11 +1179,15 @@ void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph
     const size_t n = (size_t)na * nb;
     k_phase<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, ks.lat[0], ks.lat[1], ks.lat[2], (double2*)ph);
 }
-void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, hipStream_t s) {
+void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
+               hipStream_t s) {
     const KDev kd = make_kdev(ks);
     const int nchunk = kspace_nchunk(na);
-    k_sk_mfma<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
-                                                                      (double2*)sk_part);
+    const dim3 grid((ks.kmax + KXB) / KXB, nchunk, nb);
+    if (fp32)
+        k_sk_mfma<float><<<grid, 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd, (double2*)sk_part);
+    else
+        k_sk_mfma<double><<<grid, 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd, (double2*)sk_part);
 }
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
                      double* parts_k, hipStream_t s) {
@@ -1034,11 +1195,18 @@ void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask,
     k_sk_final<<<dim3(ks.kmax + 1, nb), 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
                                                       (double2*)gk, parts_k);
 }
-void launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
-                   hipStream_t s) {
+// the force pass; returns the number of partial-gradient slots written (kx groups, or 1)
+int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
+                  int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
+    if (fp32) {
+        k_kforce_mfma32<<<dim3((na + KFA - 1) / KFA, nb), 256, 0, s>>>((const double2*)ph, q, na, kd,
+                                                                      (const double2*)gk, kpart);
+        return 1;
+    }
     k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
         (const double2*)ph, q, na, kd, (const double2*)gk, kpart);
+    return kspace_ngroups(ks.kmax);
 }
 
 // ------------------------------------------------------------------------------------------------
