@@ -33,6 +33,7 @@ NCELL = (12, 12, 12)
 DT_FS = 0.25
 TEMP = 298.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -235,19 +236,32 @@ def main():
     sites = st["r"][0].copy()
     rc_bohr = prm["wrcut"] * box[0]
     p_ew = count_pairs(sites, box, rc_bohr)
+    # per launch (all NB beads): 67 flop per in-cut-off real-space site pair; 8 flop per (k, site) for
+    # the structure factor and for the force projection (kx >= 0 set of the reference)
     flops = {"ewald_real": 67.0 * p_ew * NB, "k_sk": 8.0 * nk * na * NB, "k_kforce": 8.0 * nk * na * NB}
     dom = max(("ewald_real", "k_sk", "k_kforce"), key=lambda k: prof[k][0])
     t_ms = prof[dom][0] / max(prof[dom][1], 1)
     achieved = flops[dom] / (t_ms * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if head == "fp64" else PEAK_FP32_TFLOPS
+    unit_note = {
+        ("fp64", "ewald_real"): "FP64 vector ALU",
+        ("fp64", "k_sk"): "FP64 MFMA (v_mfma_f64_16x16x4f64)",
+        ("fp64", "k_kforce"): "FP64 vector FMA (Horner)",
+        ("mixed", "ewald_real"): "FP32 vector ALU (packed FP32 not used)",
+        ("mixed", "k_sk"): "FP32 MFMA (v_mfma_f32_16x16x4f32)",
+        ("mixed", "k_kforce"): "FP32 MFMA GEMM (v_mfma_f32_16x16x4f32)",
+    }[(head, dom)]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_1728x32_mixed.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_1728x32_{head}.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath)).get("kernels", {}).get(dom)
         if t:
             traffic = t["bytes_per_launch"]
-    roof = dict(bound="mfma", kernel=dom, achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic,
-                note="FP64 vector-FMA bound; MI355X FP64 peak (vector = matrix = 78.6 TFLOP/s)",
+    roof = dict(bound="mfma", kernel=dom, achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak,
+                traffic=traffic, flops_per_launch=flops[dom], ms_per_launch=t_ms,
+                note=f"{unit_note}; algorithmic flops (not the tile-padded count the kernel issues) over the "
+                     f"mean launch time from HIP events on the library stream; peak = MI355X "
+                     f"{'FP64 78.6' if head == 'fp64' else 'FP32 157.3'} TFLOP/s",
                 kernel_ms={k: v[0] / max(v[1], 1) for k, v in prof.items()})
     drift = None
     if a.nve_steps > 0:
@@ -285,7 +299,9 @@ def main():
                    "graph": not a.no_graph},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "fp64_gpu": ({"bead_force_evals_s": 1e3 / res["fp64"][0] * NB, "ms_per_step": res["fp64"][0],
-                      "ns_per_day": 1e3 / res["fp64"][0] * DT_FS * 0.0864} if "fp64" in res else None),
+                      "ns_per_day": 1e3 / res["fp64"][0] * DT_FS * 0.0864,
+                      "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in res["fp64"][1].items()}}
+                     if "fp64" in res else None),
         "mixed_vs_fp64": (res["fp64"][0] / res["mixed"][0]) if ("fp64" in res and "mixed" in res) else None,
         "mixed_force_error": err,
         "nve_drift": drift,

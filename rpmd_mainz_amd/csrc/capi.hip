@@ -246,6 +246,12 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
                 }
         ks.ky_max[kx] = kyb;
         ks.kz_bound[kx] = kzb;
+        for (int iy = 0; iy < T; iy++) {
+            int r = -1;
+            for (int iz = 0; iz < T; iz++)
+                if (kx <= K && m[((size_t)kx * T + iy) * T + iz]) r = std::max(r, std::abs(iz - O));
+            ks.kz_row[kx][iy] = (signed char)r;
+        }
     }
     HIPCHK(hipMemcpyAsync(c->kmask, m.data(), m.size(), hipMemcpyHostToDevice, c->st));
     c->ks = ks;

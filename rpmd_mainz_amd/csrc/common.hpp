@@ -32,6 +32,7 @@ struct KSet {
     int kmax;
     int ky_max[RPMD_KMAX_DEV + 1];    // per kx: largest |ky| with a member (-1: none)
     int kz_bound[RPMD_KMAX_DEV + 1];  // per kx: largest |kz| with a member
+    signed char kz_row[RPMD_KMAX_DEV + 1][RPMD_KTILE];  // per (kx, ky + 16): largest |kz| with a member, -1: none
     double lat[3];                    // 2 pi (cubic, unit box) or 2 pi / L_d
     double f0;                        // 2 pi (cubic) or 2 pi / V
     double b;                         // 1/(4 alpha^2)

@@ -755,6 +755,7 @@ struct KDev {
     int cubic;
     int kyb[KH + 1];   // per kx: max |ky| with a member (-1: none)
     int kzb[KH + 1];   // per kx: max |kz| with a member
+    signed char kzr[KH + 1][KT];  // per (kx, ky + KO): max |kz| with a member (-1: none)
     double lat0, lat1, lat2;
     double f0, b;
 };
@@ -971,33 +972,39 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
     const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
     for (int kx = kx0; kx <= kx1; kx++) {
         if (kx > kx0) xk = cmul(xk, X1);
-        const int KY = kd.kyb[kx], KZ = kd.kzb[kx];
+        const int KY = kd.kyb[kx];
         if (KY < 0) continue;
-        const double2 zK = cpow(Z1, KZ);
         double2 yc = cpow(Yc, KY);   // Y(-KY)
         double tx = 0.0, ty = 0.0, tz = 0.0;
         const double2* Gx = G0 + (size_t)kx * KT * KT;
         int ky = -KY;
         for (; ky + 1 <= KY; ky += 2) {
-            double2 ha, da, hb, db;
-            horner2(Gx + (ky + KO) * KT, Gx + (ky + 1 + KO) * KT, KZ, w, ha, da, hb, db);
+            // the pair's kz range: the larger of the two rows' bounds (G is zero beyond)
+            const int KZ = max((int)kd.kzr[kx][ky + KO], (int)kd.kzr[kx][ky + 1 + KO]);
             const double2 yb = cmul(yc, Y1);
+            if (KZ >= 0) {
+                const double2 zK = cpow(Z1, KZ);
+                double2 ha, da, hb, db;
+                horner2(Gx + (ky + KO) * KT, Gx + (ky + 1 + KO) * KT, KZ, w, ha, da, hb, db);
 #pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const double2 h = r ? hb : ha, d = r ? db : da, yy = r ? yb : yc;
-                const double2 U = cmul(zK, h);
-                const double2 wd = cmul(w, d);
-                const double2 Uz = cmul(zK, make_double2(wd.x - KZ * h.x, wd.y - KZ * h.y));
-                const double2 xy = cmul(xk, yy);
-                const double ipu = xy.x * U.y - xy.y * U.x;   // Im(conj(xy) U)
-                const double ipz = xy.x * Uz.y - xy.y * Uz.x;
-                tx += ipu;
-                ty += (ky + r) * ipu;
-                tz += ipz;
+                for (int r = 0; r < 2; r++) {
+                    const double2 h = r ? hb : ha, d = r ? db : da, yy = r ? yb : yc;
+                    const double2 U = cmul(zK, h);
+                    const double2 wd = cmul(w, d);
+                    const double2 Uz = cmul(zK, make_double2(wd.x - KZ * h.x, wd.y - KZ * h.y));
+                    const double2 xy = cmul(xk, yy);
+                    const double ipu = xy.x * U.y - xy.y * U.x;   // Im(conj(xy) U)
+                    const double ipz = xy.x * Uz.y - xy.y * Uz.x;
+                    tx += ipu;
+                    ty += (ky + r) * ipu;
+                    tz += ipz;
+                }
             }
             yc = cmul(yb, Y1);
         }
-        if (ky <= KY) {
+        const int KZ = ky <= KY ? (int)kd.kzr[kx][ky + KO] : -1;
+        if (KZ >= 0) {
+            const double2 zK = cpow(Z1, KZ);
             double2 ha, da, hb, db;
             horner2(Gx + (ky + KO) * KT, Gx + (ky + KO) * KT, KZ, w, ha, da, hb, db);
             const double2 U = cmul(zK, ha);
@@ -1100,8 +1107,9 @@ __global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict
             ur[nbk] = (typename M::acc){0, 0, 0, 0};
             ui[nbk] = (typename M::acc){0, 0, 0, 0};
         }
+        const int st0 = (KO - kd.kzb[kx]) >> 2, st1 = (KO + kd.kzb[kx]) >> 2;  // G is zero beyond
 #pragma unroll 2
-        for (int st = 0; st < KT / 4; st++) {
+        for (int st = st0; st <= st1; st++) {
             const int iz = 4 * st + lk;
             const float wk = wz ? (float)(iz - KO) : 1.0f;
             const float ar = wk * Gr[16 * mb + li][iz], ai = wk * Gi[16 * mb + li][iz];
@@ -1165,6 +1173,7 @@ static KDev make_kdev(const KSet& ks) {
     for (int kx = 0; kx <= KH; kx++) {
         kd.kyb[kx] = ks.ky_max[kx];
         kd.kzb[kx] = ks.kz_bound[kx];
+        for (int iy = 0; iy < KT; iy++) kd.kzr[kx][iy] = ks.kz_row[kx][iy];
     }
     kd.lat0 = ks.lat[0];
     kd.lat1 = ks.lat[1];

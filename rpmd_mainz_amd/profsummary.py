@@ -12,7 +12,7 @@ import csv
 import json
 import sys
 
-KERNEL_KEYS = {"ewald_real": "k_ewald_clu", "k_sk": "k_sk_mfma", "k_kforce": "k_kforce", "k_sk_final": "k_sk_final"}
+KERNEL_KEYS = {"ewald_real": "k_ewald_vl", "k_sk": "k_sk_mfma", "k_kforce": "k_kforce", "k_sk_final": "k_sk_final"}
 
 
 def _short(name: str) -> str:
@@ -41,14 +41,29 @@ def _per_kernel(path: str, counter: str):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+def _copy_dispatch(path: str, counter: str, copy_kib: float) -> float:
+    """the counter of the runtime's copy kernel dispatch that moved the calibration buffer: the
+    copyBuffer dispatch with the largest grid (the probe's copies are the largest in the run)"""
+    best, grid = None, -1
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter or "copyBuffer" not in r["Kernel_Name"]:
+            continue
+        g = int(r["Grid_Size"])
+        if g > grid:
+            best, grid = float(r["Counter_Value"]), g
+    if best is None:
+        raise SystemExit("no copy kernel in " + path)
+    return best
+
+
 def pmc(fetch_csv: str, write_csv: str, copy_mib: float, command: str):
     f = _per_kernel(fetch_csv, "FETCH_SIZE")
     w = _per_kernel(write_csv, "WRITE_SIZE")
     copy_kib = copy_mib * 1024.0
-    # the copy kernel: the one whose fetch is closest to the known size
-    ck = min((k for k in f if k in w), key=lambda k: abs(f[k] - copy_kib))
-    sf, sw = copy_kib / f[ck], copy_kib / w[ck]
-    out = dict(command=command, copy_kernel=ck, copy_mib=copy_mib, fetch_scale=sf, write_scale=sw, kernels={})
+    cf, cw = _copy_dispatch(fetch_csv, "FETCH_SIZE", copy_kib), _copy_dispatch(write_csv, "WRITE_SIZE", copy_kib)
+    sf, sw = copy_kib / cf, copy_kib / cw
+    out = dict(command=command, copy_kernel="__amd_rocclr_copyBuffer", copy_mib=copy_mib, copy_fetch_kib=cf,
+               copy_write_kib=cw, fetch_scale=sf, write_scale=sw, kernels={})
     for key, sub in KERNEL_KEYS.items():
         ks = [k for k in f if sub in k and k in w]
         if not ks:

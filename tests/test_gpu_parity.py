@@ -258,6 +258,33 @@ def test_device_noise_thermostats_sample_temperature(capi, therm):
     ctx.close()
 
 
+def test_config1_pile_step_by_step(capi, oracle):
+    """Config 1: 216 q-SPC/Fw (6^3), 32 beads, PILE-L 298 K (tau 100 fs), dt 0.25 fs, ecut 1e-6,
+    FP64; 20 steps with the reference's ran2 stream replayed, energies and forces compared with
+    the oracle after every step."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (6, 6, 6), 32, ecut=1e-6, therm="PRO", ttaufs=100.0,
+                                           intra_model=cf.INTRA_SPCFW, amp=0.0)
+    assert prm["kmax"] == 10 and abs(box[0] * cf.TOA - 18.643) < 0.01
+    na, nb = r.shape[1], r.shape[0]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(1))
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    s_or, s_gp = oracle.rng(2), oracle.rng(2)
+    po, ro, d1, d2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    pg, rg, g1, g2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    for k in range(20):
+        noise = oracle.gaussians(s_gp, 2 * 3 * na * nb)
+        o = oracle.evolve_pi(P, po, ro, d1, d2, dt, mass, box, z, beta, s=s_or)
+        po, ro, d1, d2 = o["p"], o["r"], o["dvdr"], o["dvdr2"]
+        g = ctx.evolve_pi(pg, rg, g1, g2, dt, mass, na, nb, box, z, beta, noise=noise)
+        assert abs(g["v"] - o["v"]) < 1e-10 * _escale(prm, box, z, o["v"]), k
+        assert _rel(g1, d1) < 1e-9 and _rel(g2, d2) < 1e-9, k
+    assert _rel(rg, ro) < 1e-11
+    ctx.close()
+
+
 def test_device_resident_steps_match_oracle(capi, oracle):
     """NVE steps run with all state in HBM agree with the oracle's evolve_pi."""
     r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8, ecut=1e-6, intra_model=cf.INTRA_SPCFW)
