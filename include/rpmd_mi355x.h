@@ -147,6 +147,13 @@ int rpmd_evolve_rig_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* v
                        const double boxlxyz[3], const double* z, double beta, double* vir,
                        double* vir_lf, const double* noise, long long n_noise);
 
+/* subroutine sample_rpmd(p,na,nb,mass,beta,irun)                 src/module_thermostat.f90:199-226
+ * Maxwell-Boltzmann bead momenta at beta/nb, then pshift (:229-264) removes the total momentum.
+ * noise: 3*na*nb standard normal deviates in the reference's draw order (bead, atom, xyz), or NULL
+ * to draw them on the device from seed.  p is written in the reference layout p(3,na,nb). */
+int rpmd_sample_rpmd(rpmd_ctx* ctx, double* p, int na, int nb, const double* mass, double beta,
+                     const double* noise, long long n_noise, unsigned long long seed);
+
 /* subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)              src/module_gle.f90:146-227
  * Sets up the GLE thermostat (therm = RPMD_THERM_GLE) of this context for (na, nb): A is the
  * (ns+1) x (ns+1) drift matrix in the GLE-A file's row order (A[i*(ns+1)+j] = A(i+1,j+1)), in

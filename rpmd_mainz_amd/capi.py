@@ -68,7 +68,7 @@ def lib() -> C.CDLL:
                      "rpmd_ctx_destroy", "rpmd_forces", "rpmd_full_forces", "rpmd_realft", "rpmd_evolve_pi",
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
-                     "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy"):
+                     "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -208,6 +208,19 @@ class RPMDContext:
                                         _p(wl), npt, nn))
         return dict(v=v.value, v_lf=vlf.value, v_hf=vhf.value, vir=w.reshape(3, 3).T.copy(),
                     vir_lf=wl.reshape(3, 3).T.copy())
+
+    # -- subroutine sample_rpmd(p,na,nb,mass,beta,irun)  src/module_thermostat.f90:199 ----------
+    def sample_rpmd(self, nb: int, na: int, mass, beta: float, noise=None, seed: int = 0) -> np.ndarray:
+        """Bead momenta (nb, na, 3); noise: standard normals in the reference draw order, or None."""
+        p = np.zeros((nb, na, 3))
+        if noise is not None:
+            noise = _arr(noise)
+            npt, nn = _p(noise), C.c_longlong(noise.size)
+        else:
+            npt, nn = None, C.c_longlong(0)
+        _check(lib().rpmd_sample_rpmd(self._h, _p(p), C.c_int(na), C.c_int(nb), _p(_arr(mass, (na,))),
+                                      C.c_double(beta), npt, nn, C.c_ulonglong(seed)))
+        return p
 
     # -- subroutine therm_gle_init(ttau,na,nb,dt,irun,beta)  src/module_gle.f90:146 ---------------
     def gle_init(self, A, na: int, nb: int, dt: float, beta: float, gp=None, seed: int = 0):

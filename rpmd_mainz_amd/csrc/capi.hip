@@ -1394,6 +1394,34 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
     return 0;
 }
 
+int rpmd_sample_rpmd(rpmd_ctx* c, double* p, int na, int nb, const double* mass, double beta, const double* noise,
+                     long long n_noise, unsigned long long seed) {
+    if (!c || !p || !mass || beta <= 0.0) return RPMD_ERR_ARG;
+    if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    const size_t N = (size_t)3 * na * nb;
+    if (noise && (size_t)n_noise < N) return RPMD_ERR_NOISE;
+    const double* dn = nullptr;
+    if (noise) {
+        if (c->noise_cap < N) {
+            if (c->noise) hipFree(c->noise);
+            c->noise = nullptr;
+            RETCHK(dalloc(&c->noise, N) ? RPMD_ERR_NOMEM : 0);
+            c->noise_cap = N;
+        }
+        HIPCHK(hipMemcpyAsync(c->noise, noise, N * 8, hipMemcpyHostToDevice, c->st));
+        dn = c->noise;
+    }
+    double totmass = 0.0;
+    for (int j = 0; j < na; j++) totmass = totmass + mass[j];
+    totmass = (double)nb * totmass;
+    HIPCHK(hipMemcpyAsync(c->mass, mass, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
+    // scratch: p in tmp2; the per-block partial sums (3 per block) and the shift velocity in tmp
+    const size_t nparts = (size_t)3 * pile_blocks(na) * nb;
+    launch_sample_rpmd(c->tmp2, c->mass, na, nb, beta, totmass, dn, seed, ~1ull, c->tmp, c->tmp + nparts, c->st);
+    RETCHK(download_aos(c, c->tmp2, p, nb));
+    return 0;
+}
+
 int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double dt, double beta, const double* gp,
                   unsigned long long seed) {
     if (!c || !A || ns < 0 || ns > 7 || nb < 1 || nb > c->nb_max || beta <= 0.0) return RPMD_ERR_ARG;

@@ -126,6 +126,9 @@ void launch_langevin(double* p, const double* c3m, double c1, int na, int nb, co
 void launch_mode_copy(double* dst, const double* src, const int* map, int ndst, int na, double scale, hipStream_t s);
 void launch_scale(double* y, double a, size_t n, hipStream_t s);
 void launch_counter_inc(uint64_t* c, hipStream_t s);
+void launch_sample_rpmd(double* p, const double* mass, int na, int nb, double beta, double totmass,
+                        const double* noise, uint64_t seed, uint64_t stream, double* parts, double* v,
+                        hipStream_t s);
 
 // ---- k_rigid.hip ------------------------------------------------------------------------------
 int rigid_blocks(int na);

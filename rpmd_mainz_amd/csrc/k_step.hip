@@ -487,6 +487,66 @@ void launch_gle_draw(double* gp, double sc, int na, int nb, int n1, uint64_t see
     k_gle_draw<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(gp, sc, na, nb, n1, seed, stream);
 }
 
+// ---- sample_rpmd (src/module_thermostat.f90:199-226) and pshift (:229-264) ----------------------
+// p(l,j,k) = sqrt(m_j / beta_n) xi in the draw order (bead, atom, xyz); block partial sums of p per
+// xyz for the centre-of-mass shift
+__global__ __launch_bounds__(256) void k_sample(double* __restrict__ p, const double* __restrict__ mass, int na,
+                                                double betan, const double* __restrict__ noise, uint64_t seed,
+                                                uint64_t stream, double* __restrict__ parts) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (a < na) {
+        const double sigma = sqrt(mass[a] / betan);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t ni = ((size_t)k * na + a) * 3 + d;
+            const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
+            const double v = sigma * xi;
+            p[((size_t)k * 3 + d) * na + a] = v;
+            s[d] = v;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) s[d] += __shfl_down(s[d], off, 64);
+    __shared__ double red[4][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; d++) red[w][d] = s[d];
+    __syncthreads();
+    if (threadIdx.x < 3)
+        parts[((size_t)k * gridDim.x + blockIdx.x) * 3 + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// v = (sum of the partials) / (nb sum m), in fixed order
+__global__ void k_pshift_v(const double* __restrict__ parts, int nparts, double totmass, double* __restrict__ v) {
+    if (threadIdx.x >= 3) return;
+    double s = 0.0;
+    for (int q = 0; q < nparts; q++) s += parts[(size_t)q * 3 + threadIdx.x];
+    v[threadIdx.x] = s / totmass;
+}
+
+__global__ void k_pshift_apply(double* __restrict__ p, const double* __restrict__ mass, const double* __restrict__ v,
+                               int na, int nb) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)na * nb) return;
+    const size_t k = t / na, a = t % na;
+#pragma unroll
+    for (int d = 0; d < 3; d++) p[(k * 3 + d) * na + a] = p[(k * 3 + d) * na + a] - mass[a] * v[d];
+}
+
+void launch_sample_rpmd(double* p, const double* mass, int na, int nb, double beta, double totmass,
+                        const double* noise, uint64_t seed, uint64_t stream, double* parts, double* v,
+                        hipStream_t s) {
+    const dim3 grid(pile_blocks(na), nb);
+    k_sample<<<grid, 256, 0, s>>>(p, mass, na, beta / (double)nb, noise, seed, stream, parts);
+    k_pshift_v<<<1, 64, 0, s>>>(parts, (int)(grid.x * grid.y), totmass, v);
+    const size_t n = (size_t)na * nb;
+    k_pshift_apply<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, mass, v, na, nb);
+}
+
 // ---- contraction copies (src/evolve_pi.f90:377-431): modes 0..h and their mirrors ---------------
 // dst[k] = scale * src[map[k]] for k < ndst (map[k] < 0: zero)
 __global__ void k_mode_copy(double* __restrict__ dst, const double* __restrict__ src, const int* __restrict__ map,

@@ -285,6 +285,24 @@ def test_config1_pile_step_by_step(capi, oracle):
     ctx.close()
 
 
+def test_sample_rpmd_matches_oracle(capi, oracle):
+    """sample_rpmd + pshift on the device: the reference's stream replayed gives the oracle's
+    momenta; the device stream gives zero total momentum and the Maxwell-Boltzmann variance."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8)
+    na, nb = r.shape[1], r.shape[0]
+    beta = 1.0 / (cf.KBOLTZ * 298.0)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    noise = oracle.gaussians(oracle.rng(21), 3 * na * nb)
+    pg = ctx.sample_rpmd(nb, na, mass, beta, noise=noise)
+    po = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(21))
+    assert _rel(pg, po) < 1e-14
+    pd = ctx.sample_rpmd(nb, na, mass, beta, seed=3)
+    assert np.abs(pd.sum(axis=(0, 1))).max() < 1e-9 * np.abs(pd).sum()
+    var = float(np.mean(pd ** 2 / (mass[None, :, None] / (beta / nb))))
+    assert abs(var - 1.0) < 0.05
+    ctx.close()
+
+
 def test_device_resident_steps_match_oracle(capi, oracle):
     """NVE steps run with all state in HBM agree with the oracle's evolve_pi."""
     r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8, ecut=1e-6, intra_model=cf.INTRA_SPCFW)
