@@ -939,19 +939,26 @@ __global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ s
     block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
 }
 
-// Horner over kz of one ky row: h = sum_m c_m w^m, d = dh/dw, with c_m = G(kz = m - KZ)
+// Horner over kz of two ky rows: h = sum_m c_m w^m, d = dh/dw, with c_m = G(kz = m - KZ).  After the
+// leading coefficient there are 2 KZ steps, taken two at a time with the four coefficients loaded
+// first, so the loads overlap the FMA chains.
+__device__ __forceinline__ void horner_step(double2& h, double2& d, double2 w, double2 g) {
+    d = make_double2(fma(d.x, w.x, fma(-d.y, w.y, h.x)), fma(d.x, w.y, fma(d.y, w.x, h.y)));
+    h = make_double2(fma(h.x, w.x, fma(-h.y, w.y, g.x)), fma(h.x, w.y, fma(h.y, w.x, g.y)));
+}
 __device__ __forceinline__ void horner2(const double2* __restrict__ ra, const double2* __restrict__ rb, int KZ,
                                         double2 w, double2& ha, double2& da, double2& hb, double2& db) {
     ha = ra[KZ + KO];
     hb = rb[KZ + KO];
     da = make_double2(0.0, 0.0);
     db = make_double2(0.0, 0.0);
-    for (int kz = KZ - 1; kz >= -KZ; kz--) {
-        const double2 ga = ra[kz + KO], gb = rb[kz + KO];
-        da = make_double2(fma(da.x, w.x, fma(-da.y, w.y, ha.x)), fma(da.x, w.y, fma(da.y, w.x, ha.y)));
-        db = make_double2(fma(db.x, w.x, fma(-db.y, w.y, hb.x)), fma(db.x, w.y, fma(db.y, w.x, hb.y)));
-        ha = make_double2(fma(ha.x, w.x, fma(-ha.y, w.y, ga.x)), fma(ha.x, w.y, fma(ha.y, w.x, ga.y)));
-        hb = make_double2(fma(hb.x, w.x, fma(-hb.y, w.y, gb.x)), fma(hb.x, w.y, fma(hb.y, w.x, gb.y)));
+    for (int kz = KZ - 1; kz >= -KZ; kz -= 2) {
+        const double2 ga0 = ra[kz + KO], gb0 = rb[kz + KO];
+        const double2 ga1 = ra[kz - 1 + KO], gb1 = rb[kz - 1 + KO];
+        horner_step(ha, da, w, ga0);
+        horner_step(hb, db, w, gb0);
+        horner_step(ha, da, w, ga1);
+        horner_step(hb, db, w, gb1);
     }
 }
 
