@@ -328,16 +328,20 @@ void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitb
 // pair-body type; in FP32 the distances come from wrapped fractional FP32 coordinates and the
 // sums are flushed into FP64 every 32 pairs.
 // ------------------------------------------------------------------------------------------------
+// FP64: du = erfc_AS(alpha r)/r (erfc - 1 within a molecule), dur = (sfac e^{-x^2} - du)/r^2, the
+// reference's expressions (src/ewald_cubic.f90:126-140) with the two divisions by r and r^2 taken as
+// products with 1/r (rounding-level differences only)
 template <typename T>
 __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& du, T& dur) {
-    T dr = sqrt(dr2);
+    const T rinv = (T)1 / sqrt(dr2);
+    const T dr = dr2 * rinv;
     T x = alpha * dr;
     T e = exp(-x * x);
     T t = (T)AS_P / ((T)AS_P + x);
     T erfc_ = e * t * ((T)AS_A1 + t * ((T)AS_A2 + t * ((T)AS_A3 + t * ((T)AS_A4 + t * (T)AS_A5))));
     if (same) erfc_ = erfc_ - (T)1;
-    du = erfc_ / dr;
-    dur = (sfac * e - du) / dr2;
+    du = erfc_ * rinv;
+    dur = (sfac * e - du) * (rinv * rinv);
 }
 template <>
 __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float sfac, bool same, float& du, float& dur) {
