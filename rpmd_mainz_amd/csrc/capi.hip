@@ -98,6 +98,7 @@ struct rpmd_ctx {
     double *bussi_parts = nullptr, *bussi_alpha = nullptr;
     double *gle_gp = nullptr, *gle_TS = nullptr, *gle_parts = nullptr, *gle_pcom = nullptr;
     int gle_n1 = 0, gle_nb = 0;
+    bool lf_merged = false;  // the last step's Ewald and O-O forces came from one forces(iopt = 1) call
     // rigid molecules: RATTLE failure flag, virial partials
     int* rat_err = nullptr;
     double* rig_vparts = nullptr;
@@ -387,10 +388,14 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     const double boxmin = std::min(box[0], std::min(box[1], box[2]));
     const double boxmax = std::max(box[0], std::max(box[1], box[2]));
     const size_t N = (size_t)3 * na * nb;
-    launch_zero(grad, N, s);
-    launch_zero(ev_slot, 50, s);
     const bool mixed = c->prec == RPMD_PREC_MIXED;
     const bool need_oo = (iopt == 3 || iopt <= 1);
+    // the first kernel writing grad stores, the others add; the O-O term alone writes only the
+    // oxygens, so its call starts from zero.  ev_slot components not computed here are left as
+    // they are: combine_ev reads only the computed ones.
+    if (iopt == 3) launch_zero(grad, N, s);
+    int gacc = iopt == 3 ? 1 : 0;
+    ReduceJobs jobs;
     int use_list = 0;
     if (need_oo && nb > 3) {
         // centroid O-O list (src/md_forces.f90:34-53)
@@ -453,10 +458,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
         // real-space slot then the k-space slots, summed in that order
-        launch_chain_rule(grad, c->fpart, ewald_real_slots() + nkslots, na, nb, P.alpha, sc * sc, s);
+        launch_chain_rule(grad, c->fpart, ewald_real_slots() + nkslots, na, nb, P.alpha, sc * sc, gacc, s);
+        gacc = 1;
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
-        launch_reduce_ev(pr, nb * ewald_real_parts(na), 10, ev_slot + 0, 10, sc / nb, s);
-        launch_reduce_ev(pk, nb * (P.kmax + 1), 10, ev_slot + 10, 10, sc / nb, s);
+        jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
+        jobs.add(pk, nb * (P.kmax + 1), 10, sc / nb);
     }
     if (need_oo) {
         double* po = c->parts + c->off_oo;
@@ -464,12 +470,13 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
         else
             launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
-        launch_reduce_ev(po, nb * oo_blocks(na), 10, ev_slot + 20, 10, 1.0 / nb, s);
+        jobs.add(po, nb * oo_blocks(na), 20, 1.0 / nb);
     }
     if (iopt == 0 || iopt == 4) {
         double* pi = c->parts + c->off_intra;
-        launch_intra(pos, na, nb, intra_model(P), P, grad, pi, s);
-        launch_reduce_ev(pi, nb * intra_blocks(na), 10, ev_slot + 30, 10, 1.0 / nb, s);
+        launch_intra(pos, na, nb, intra_model(P), P, grad, pi, gacc, s);
+        gacc = 1;
+        jobs.add(pi, nb * intra_blocks(na), 30, 1.0 / nb);
     }
     if (iopt == 5) {
         // TEM correction (src/md_forces.f90:65-100; src/correct_tem.f90:1-71)
@@ -495,18 +502,32 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_sites(pos, c->site, na, nb, aeff, 0, bx, s);
         double* pt = c->parts + c->off_tem;
         launch_tem(c->site, c->q, na, nb, bx, c->nnb, c->nbl, tl, P.sig, c->gsite, pt, s);
-        launch_chain_rule(grad, c->gsite, 1, na, nb, aeff, 1.0, s);
-        launch_reduce_ev(pt, nb * tem_blocks(na), 10, ev_slot + 40, 10, 1.0 / nb, s);
+        launch_chain_rule(grad, c->gsite, 1, na, nb, aeff, 1.0, gacc, s);
+        gacc = 1;
+        jobs.add(pt, nb * tem_blocks(na), 40, 1.0 / nb);
     }
+    launch_reduce_ev(jobs, ev_slot, s);
     if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
     return 0;
 }
 
 // combine an ev slot read back to the host: v and vir (column-major) of one forces call
+// components of an ev slot a forces call with this iopt writes: 0 real, 1 k space, 2 O-O, 3 intra, 4 TEM
+bool ev_component(int iopt, int t) {
+    switch (t) {
+        case 0:
+        case 1: return iopt <= 2;
+        case 2: return iopt == 3 || iopt <= 1;
+        case 3: return iopt == 0 || iopt == 4;
+        default: return iopt == 5;
+    }
+}
+
 void combine_ev(const rpmd_ctx* c, const double* h, int iopt, const double box[3], double* v, double* vir) {
     double vv = 0.0;
     double w[9] = {0};
     for (int t = 0; t < 5; t++) {
+        if (!ev_component(iopt, t)) continue;
         vv += h[t * 10];
         for (int k = 0; k < 9; k++) w[k] += h[t * 10 + 1 + k];
     }
@@ -807,14 +828,21 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
         launch_nm(c->dvdrl, c->tmp, nmm->B, nb, ne, s);
         std::swap(c->dvdrl, c->tmp);
     }
-    if (nbdf1 <= 0) {
-        RETCHK(forces_dev(c, c->r, nb, box, 2, c->dvdr, ev_lf, check));
-    } else if (P.sig > 0.0) {
-        RETCHK(forces_dev(c, c->r, nb, box, 5, c->dvdre, ev_lf + 150, check));
-        launch_axpy(c->dvdr, c->dvdre, 1.0, N, s);
+    c->lf_merged = nbdf1 <= 0 && nbdf2 <= 0;
+    if (c->lf_merged) {
+        // Ewald then O-O into one gradient, as forces(iopt = 1) sums them (src/md_forces.f90:192-236);
+        // both energies stay separate in the slot
+        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, ev_lf, check));
+    } else {
+        if (nbdf1 <= 0) {
+            RETCHK(forces_dev(c, c->r, nb, box, 2, c->dvdr, ev_lf, check));
+        } else if (P.sig > 0.0) {
+            RETCHK(forces_dev(c, c->r, nb, box, 5, c->dvdre, ev_lf + 150, check));
+            launch_axpy(c->dvdr, c->dvdre, 1.0, N, s);
+        }
+        if (nbdf2 <= 0) RETCHK(forces_dev(c, c->r, nb, box, 3, c->dvdrl, ev_lf + 100, check));
+        if (nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
     }
-    if (nbdf2 <= 0) RETCHK(forces_dev(c, c->r, nb, box, 3, c->dvdrl, ev_lf + 100, check));
-    if (nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
     launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
     outer();
     launch_counter_inc(c->step_ctr, s);
@@ -920,7 +948,7 @@ int step_energies(rpmd_ctx* c, double* v, double* vew, double* vlj, double* vint
     double e_ew, w_ew[9], e_c, w_c[9], e_lj, w_lj[9], e_cb, w_cb[9];
     combine_ev(c, lf, 2, box, &e_ew, w_ew);
     combine_ev(c, lf + 50, 5, box, &e_c, w_c);
-    combine_ev(c, lf + 100, 3, box, &e_lj, w_lj);
+    combine_ev(c, c->lf_merged ? lf : lf + 100, 3, box, &e_lj, w_lj);
     combine_ev(c, lf + 150, 5, box, &e_cb, w_cb);
     double ew = e_ew, wew[9];
     for (int k = 0; k < 9; k++) wew[k] = w_ew[k];

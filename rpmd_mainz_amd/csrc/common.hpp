@@ -40,13 +40,28 @@ struct KSet {
 };
 
 // ---- k_forces.hip -----------------------------------------------------------------------------
-void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double* out, int nout, double scale,
-                      hipStream_t s);
+// energy/virial part ranges reduced by one launch: job j sums parts[j][p*10 + k] into out[out[j] + k]
+#define RED_JOBS 5
+struct ReduceJobs {
+    const double* parts[RED_JOBS];
+    int nparts[RED_JOBS];
+    int out[RED_JOBS];
+    double scale[RED_JOBS];
+    int n = 0;
+    void add(const double* p, int np, int o, double sc) {
+        parts[n] = p;
+        nparts[n] = np;
+        out[n] = o;
+        scale[n] = sc;
+        n++;
+    }
+};
+void launch_reduce_ev(const ReduceJobs& jobs, double* out, hipStream_t s);
 void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
 void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       hipStream_t s);
+                       int acc, hipStream_t s);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
@@ -100,7 +115,7 @@ void launch_tem(const double* site, const double* q, int na, int nb, Box box, co
                 int use_list, double sig, double* gsite, double* parts, hipStream_t s);
 int tem_blocks(int na);
 void launch_intra(const double* pos, int na, int nb, int model, const rpmd_params& p, double* grad, double* parts,
-                  hipStream_t s);
+                  int acc, hipStream_t s);
 int intra_blocks(int na);
 
 // ---- k_step.hip -------------------------------------------------------------------------------

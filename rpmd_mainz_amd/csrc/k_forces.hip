@@ -36,25 +36,26 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out)
     }
 }
 
-// out[k] = scale * sum_p parts[p*stride + k], k < nout; one block per k, fixed order
-__global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__ parts, int nparts, int stride,
-                                                      double* out, double scale) {
+// out[k] = scale * sum_p parts[p*10 + k], k < 10, for up to RED_JOBS part ranges in one launch; one
+// block per (k, job), fixed order
+__global__ __launch_bounds__(256) void k_reduce_parts(ReduceJobs jobs, double* out) {
     __shared__ double red[256];
-    const int k = blockIdx.x;
+    const int k = blockIdx.x, jb = blockIdx.y;
+    const double* parts = jobs.parts[jb];
+    const int nparts = jobs.nparts[jb];
     double s = 0.0;
-    for (int p = threadIdx.x; p < nparts; p += 256) s += parts[(size_t)p * stride + k];
+    for (int p = threadIdx.x; p < nparts; p += 256) s += parts[(size_t)p * 10 + k];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[k] = scale * red[0];
+    if (threadIdx.x == 0) out[jobs.out[jb] + k] = jobs.scale[jb] * red[0];
 }
 
-void launch_reduce_ev(const double* parts, int nparts, int stride_parts, double* out, int nout, double scale,
-                      hipStream_t s) {
-    k_reduce_parts<<<nout, 256, 0, s>>>(parts, nparts, stride_parts, out, scale);
+void launch_reduce_ev(const ReduceJobs& jobs, double* out, hipStream_t s) {
+    if (jobs.n > 0) k_reduce_parts<<<dim3(10, jobs.n), 256, 0, s>>>(jobs, out);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -111,10 +112,10 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
     k_sites<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2);
 }
 
-// grad += scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient,
+// grad (+)= scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient,
 // the sum of npart partial site gradients (parts[p*pstride + ...]) taken in fixed order
 __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart, size_t pstride, int na,
-                        int nb, double alpha, double scale) {
+                        int nb, double alpha, double scale, int acc) {
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nm * nb) return;
@@ -130,15 +131,18 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ pa
             s2 += parts[p * pstride + o + 2];
         }
         double g0 = scale * s0, g1 = scale * s1, g2 = scale * s2;
-        grad[o] += alpha * g0;
-        grad[o + 1] += g1 + alpha2 * g0;
-        grad[o + 2] += g2 + alpha2 * g0;
+        const double b0 = acc ? grad[o] : 0.0, b1 = acc ? grad[o + 1] : 0.0, b2 = acc ? grad[o + 2] : 0.0;
+        grad[o] = b0 + alpha * g0;
+        grad[o + 1] = b1 + (g1 + alpha2 * g0);
+        grad[o + 2] = b2 + (g2 + alpha2 * g0);
     }
 }
+// acc = 0: the first writer of grad in a forces call (stores instead of adding)
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       hipStream_t s) {
+                       int acc, hipStream_t s) {
     size_t n = (size_t)(na / 3) * nb;
-    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale);
+    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale,
+                                                       acc);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
@@ -1282,25 +1286,29 @@ void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int
     k_nlist<<<(n + 3) / 4, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
 }
 
-// LJ (gam == 0) or Buckingham exp-6; thread per (bead, O).  use_list = 0: every other O.
+// LJ (gam == 0) or Buckingham exp-6 (src/ljones.f90, src/buckingham.f90); OOL lanes per (bead, O),
+// lane s taking neighbours s, s + OOL, ...; the O gradient is summed over its lanes by a fixed
+// butterfly.  use_list = 0: every other O (the reference's basic / cell branches, same pair set).
+#define OOL 8
 template <typename T>
 __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int na, Box bx, const int* __restrict__ nnb,
                                             const int* __restrict__ nbl, int use_list, double rc2, double sigsq,
                                             double ba, double bb, double bc, int buck, double vs, double ds,
                                             double* __restrict__ grad, double* __restrict__ parts) {
     const int b = blockIdx.y;
-    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ii = t / OOL, sl = t % OOL;
     const int n = na / 3;
-    double acc[10];
+    double acc[7];
 #pragma unroll
-    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    for (int k = 0; k < 7; k++) acc[k] = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
     if (ii < n) {
         const int i = 3 * ii;
         const double* X = pos + (size_t)b * 3 * na;
         const double xi = X[i], yi = X[na + i], zi = X[2 * na + i];
-        double gx = 0.0, gy = 0.0, gz = 0.0;
         const int cnt = use_list ? nnb[ii] : n;
-        for (int k = 0; k < cnt; k++) {
+        for (int k = sl; k < cnt; k += OOL) {
             int j;
             if (use_list) j = nbl[(size_t)ii * RPMD_NCAP + k];
             else {
@@ -1329,18 +1337,34 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
                 gx += fx;
                 gy += fy;
                 gz += fz;
-                acc[0] += 0.5 * v;
-                acc[1] += 0.5 * dx * fx; acc[2] += 0.5 * dy * fx; acc[3] += 0.5 * dz * fx;
-                acc[4] += 0.5 * dx * fy; acc[5] += 0.5 * dy * fy; acc[6] += 0.5 * dz * fy;
-                acc[7] += 0.5 * dx * fz; acc[8] += 0.5 * dy * fz; acc[9] += 0.5 * dz * fz;
+                acc[0] += v;
+                acc[1] += dx * fx;
+                acc[2] += dx * fy;
+                acc[3] += dx * fz;
+                acc[4] += dy * fy;
+                acc[5] += dy * fz;
+                acc[6] += dz * fz;
             }
         }
+    }
+#pragma unroll
+    for (int off = 1; off < OOL; off <<= 1) {
+        gx += __shfl_xor(gx, off, 64);
+        gy += __shfl_xor(gy, off, 64);
+        gz += __shfl_xor(gz, off, 64);
+    }
+    if (ii < n && sl == 0) {
+        const int i = 3 * ii;
         double* G = grad + (size_t)b * 3 * na;
         G[i] += gx;
         G[na + i] += gy;
         G[2 * na + i] += gz;
     }
-    block_reduce_store<10>(acc, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+    // every pair is seen from both ends: energy and virial carry 1/2; the virial of a central
+    // force is symmetric
+    double w[10] = {0.5 * acc[0], 0.5 * acc[1], 0.5 * acc[2], 0.5 * acc[3], 0.5 * acc[2],
+                    0.5 * acc[4], 0.5 * acc[5], 0.5 * acc[3], 0.5 * acc[5], 0.5 * acc[6]};
+    block_reduce_store<10>(w, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
 }
 
 template <typename T>
@@ -1355,14 +1379,14 @@ void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const
     }
     const int n = na / 3;
     const double vs = buck ? 1.0 : 4.0 * p.oo_eps, ds = buck ? 1.0 : 48.0 * p.oo_eps;
-    k_oo<T><<<dim3((n + 255) / 256, nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
-                                                      p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, grad, parts);
+    k_oo<T><<<dim3(oo_blocks(na), nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
+                                                    p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, grad, parts);
 }
 template void launch_oo<double>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
                                 double*, double*, hipStream_t);
 template void launch_oo<float>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
                                double*, double*, hipStream_t);
-int oo_blocks(int na) { return (na / 3 + 255) / 256; }
+int oo_blocks(int na) { return (na / 3 * OOL + 255) / 256; }
 
 // TEM switched short-range Coulomb between different molecules (src/correct_tem.f90:73-177);
 // site holds physical M-site positions; thread per (bead, atom)
@@ -1425,7 +1449,7 @@ struct IntraDev {
 };
 
 __global__ __launch_bounds__(256) void k_intra(const double* __restrict__ pos, int na, IntraDev p,
-                                               double* __restrict__ grad, double* __restrict__ parts) {
+                                               double* __restrict__ grad, double* __restrict__ parts, int accg) {
     const int b = blockIdx.y;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[10];
@@ -1497,9 +1521,10 @@ __global__ __launch_bounds__(256) void k_intra(const double* __restrict__ pos, i
         double* G = grad + (size_t)b * 3 * na;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            G[c * na + j] += -dv[0][c] - dv[1][c];
-            G[c * na + j + 1] += dv[0][c] - dv[2][c];
-            G[c * na + j + 2] += dv[1][c] + dv[2][c];
+            const size_t o = (size_t)c * na + j;
+            G[o] = (accg ? G[o] : 0.0) + (-dv[0][c] - dv[1][c]);
+            G[o + 1] = (accg ? G[o + 1] : 0.0) + (dv[0][c] - dv[2][c]);
+            G[o + 2] = (accg ? G[o + 2] : 0.0) + (dv[1][c] + dv[2][c]);
         }
         acc[0] = v;
         if (p.model == RPMD_INTRA_SPCFW) {
@@ -1520,9 +1545,9 @@ __global__ __launch_bounds__(256) void k_intra(const double* __restrict__ pos, i
 }
 
 void launch_intra(const double* pos, int na, int nb, int model, const rpmd_params& p, double* grad, double* parts,
-                  hipStream_t s) {
+                  int acc, hipStream_t s) {
     IntraDev d{model, p.apot, p.bpot, p.alp, p.alpb, p.theta, p.reoh};
-    k_intra<<<dim3((na / 3 + 255) / 256, nb), 256, 0, s>>>(pos, na, d, grad, parts);
+    k_intra<<<dim3((na / 3 + 255) / 256, nb), 256, 0, s>>>(pos, na, d, grad, parts, acc);
 }
 int intra_blocks(int na) { return (na / 3 + 255) / 256; }
 
