@@ -84,7 +84,7 @@ def run_gpu(prec: str, steps: int, warmup: int, r, p, box, z, mass, prm, beta, n
     ms = (t1 - t0) * 1e3 / steps
     # kernel timing over a separate eager window (HIP events on the library stream)
     ctx.prof_begin()
-    nprof = max(2, min(steps, 5))
+    nprof = max(2, min(steps, 50))
     ctx.run_steps(nprof)
     prof = ctx.prof_end()
     st = ctx.state_download()
@@ -195,7 +195,7 @@ def cpu_baseline(r, box, z, prm, nbs: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--prec", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-graph", action="store_true")

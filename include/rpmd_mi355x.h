@@ -189,9 +189,11 @@ int rpmd_batch_destroy(rpmd_batch* batch);
 /* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
  * the context stream around the Ewald kernels of eager launches between begin and end.
  * ms/counts index: 0 real-space pair kernel, 1 structure-factor GEMM, 2 k-space force kernel,
- * 3 structure-factor weights/energy kernel. */
+ * 3 structure-factor weights/energy kernel, 4 real-space list upkeep (displacement check and, when
+ * due, the rebuild). */
+#define RPMD_PROF_N 5
 int rpmd_prof_begin(rpmd_ctx* ctx);
-int rpmd_prof_end(rpmd_ctx* ctx, double ms[4], int counts[4]);
+int rpmd_prof_end(rpmd_ctx* ctx, double ms[RPMD_PROF_N], int counts[RPMD_PROF_N]);
 
 #ifdef __cplusplus
 }

@@ -252,10 +252,10 @@ class RPMDContext:
         _check(lib().rpmd_prof_begin(self._h))
 
     def prof_end(self) -> dict:
-        ms = (C.c_double * 4)()
-        cnt = (C.c_int * 4)()
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int * 5)()
         _check(lib().rpmd_prof_end(self._h, ms, cnt))
-        names = ("ewald_real", "k_sk", "k_kforce", "k_sk_final")
+        names = ("ewald_real", "k_sk", "k_kforce", "k_sk_final", "vlist")
         return {n: (ms[i], cnt[i]) for i, n in enumerate(names)}
 
     def state_download(self):

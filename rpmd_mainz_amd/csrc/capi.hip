@@ -68,7 +68,7 @@ struct rpmd_ctx {
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
-    double *ph = nullptr, *fpart = nullptr, *bb = nullptr;
+    double *ph = nullptr, *fpart = nullptr;
     // real-space Verlet-list caches (two bead sets: e.g. all beads and RP-contracted ones)
     struct VCache {
         VList v{};
@@ -307,7 +307,7 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
 void free_vlist(VList& v) {
-    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dparts, v.srt, v.srf, v.sq, v.iperm, v.nex, v.ccnt, v.clist};
+    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dmax, v.srt, v.srf, v.sq, v.iperm, v.nex, v.ccnt, v.clist};
     for (void* b : bufs)
         if (b) hipFree(b);
     v = VList{};
@@ -350,7 +350,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.nl, ns * cap);
         e |= dalloc(&v.rb, 3 * ns);
         e |= dalloc(&v.gate, 2);
-        e |= dalloc(&v.dparts, (ns + 255) / 256);
+        e |= dalloc(&v.dmax, 1);
         e |= dalloc(&v.srt, ns);
         e |= dalloc(&v.srf, ns);
         e |= dalloc(&v.sq, ns);
@@ -366,6 +366,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         // reference positions NaN (all bits set): the first call always builds
         HIPCHK(hipMemset(v.rb, 0xff, sizeof(double) * 3 * ns));
         HIPCHK(hipMemset(v.gate, 0, sizeof(int) * 2));
+        HIPCHK(hipMemset(v.dmax, 0, sizeof(unsigned long long)));
         x.v = v;
         x.alloc = true;
         *force = 1;
@@ -424,8 +425,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         VList* vl = nullptr;
         int force = 0;
         RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
-        launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, c->sorted,
-                     c->bb, s);
+        {
+            ProfScope ps(c, 4);
+            launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start,
+                         c->sorted, s);
+        }
         if (check_lists) {
             int h[2];
             HIPCHK(hipMemcpyAsync(h, vl->gate, sizeof h, hipMemcpyDeviceToHost, s));
@@ -1114,11 +1118,10 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->ph, (size_t)6 * na * nbl);
     e |= dalloc(&c->fpart, (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
-    const int nclu = ewald_nclu(na);
-    e |= dalloc(&c->bb, (size_t)6 * nclu * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
+    if (!e && hipMemset(c->cnt, 0, sizeof(int) * (c->ncell_cap + 1) * nbl) != hipSuccess) e = RPMD_ERR_HIP;
     e |= dalloc(&c->start, (size_t)(c->ncell_cap + 1) * nbl);
     e |= dalloc(&c->perm, (size_t)na * nbl);
     e |= dalloc(&c->nnb, na);
@@ -1173,7 +1176,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     free_vcache(c);
-    void* bufs[] = {c->ph, c->fpart, c->bb, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
+    void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
@@ -1669,10 +1672,10 @@ int rpmd_prof_begin(rpmd_ctx* c) {
     return 0;
 }
 
-int rpmd_prof_end(rpmd_ctx* c, double ms[4], int counts[4]) {
+int rpmd_prof_end(rpmd_ctx* c, double ms[RPMD_PROF_N], int counts[RPMD_PROF_N]) {
     if (!c) return RPMD_ERR_ARG;
     HIPCHK(hipStreamSynchronize(c->st));
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < RPMD_PROF_N; k++) {
         ms[k] = 0.0;
         counts[k] = 0;
     }

@@ -66,8 +66,16 @@ void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
 void launch_zero_int(int* y, size_t n, hipStream_t s);
+// the rebuild predicate of a Verlet list, evaluated on the device by every build kernel: forced, or
+// the largest squared site displacement since the last build (bits in *dmax) above lim2
+struct VGate {
+    const unsigned long long* dmax;
+    double lim2;
+    int force;
+    __device__ __forceinline__ bool on() const { return force || __longlong_as_double((long long)*dmax) > lim2; }
+};
 void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, const int* gate, hipStream_t s);
+                       int* count, int* start, int* perm, double* sorted, int* iperm, VGate gate, hipStream_t s);
 
 // Verlet lists of the real-space Ewald sum (one cache per bead set); arrays sized for nb_max beads
 struct VList {
@@ -75,8 +83,8 @@ struct VList {
     int* cnt;        // [nb][na] list length of sorted site k
     int* nl;         // [nb][cap][na] sorted neighbour indices, column-major
     double* rb;      // [nb][3][na] site coordinates at the last build
-    int* gate;       // [0] rebuild this call, [1] list overflow (sticky)
-    double* dparts;  // per-block largest displacement
+    int* gate;       // [1] list overflow (sticky)
+    unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
     int* iperm;      // [nb][na] site index -> sorted index (of the last build)
     int* nex;        // [nb][na] leading list entries that are the site's own molecule
     double4* srt;    // [nb][na] current coordinates and charge in sorted order
@@ -91,8 +99,7 @@ int ewald_real_slots();
 int ewald_real_parts(int na);
 int vlist_cap(int na, Box sb, double rl);
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
-                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, double* bb,
-                  hipStream_t s);
+                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s);
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s);

@@ -204,8 +204,8 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
 
 __global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, int morton, int nbins,
                               double i0, double i1, double i2, int* __restrict__ cell_of, int* __restrict__ count,
-                              const int* __restrict__ gate) {
-    if (gate && !gate[0]) return;
+                              VGate gate) {
+    if (!gate.on()) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)na * nb) return;
     size_t b = i / na, a = i % na;
@@ -219,9 +219,8 @@ __global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, C
 }
 
 // exclusive scan per bead (one block per bead); resets count to 0 for the scatter pass
-__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell,
-                            const int* __restrict__ gate) {
-    if (gate && !gate[0]) return;
+__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell, VGate gate) {
+    if (!gate.on()) return;
     int b = blockIdx.x;
     int* c = count + (size_t)b * (ncell + 1);
     int* st = start + (size_t)b * (ncell + 1);
@@ -252,8 +251,8 @@ __global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, in
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, int ncell, const int* __restrict__ start,
-                               int* __restrict__ fill, int* __restrict__ perm, const int* __restrict__ gate) {
-    if (gate && !gate[0]) return;
+                               int* __restrict__ fill, int* __restrict__ perm, VGate gate) {
+    if (!gate.on()) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)na * nb) return;
     size_t b = i / na, a = i % na;
@@ -264,11 +263,13 @@ __global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, 
 
 // order each cell by atom index, then gather the sorted positions
 __global__ void k_cell_order(const double* __restrict__ site, int na, int nb, int ncell, const int* __restrict__ start,
-                             int* __restrict__ perm, double* __restrict__ sorted, const int* __restrict__ gate) {
-    if (gate && !gate[0]) return;
+                             int* __restrict__ perm, double* __restrict__ sorted, int* __restrict__ iperm,
+                             int* __restrict__ count, VGate gate) {
+    if (!gate.on()) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)ncell * nb) return;
     size_t b = i / ncell, c = i % ncell;
+    count[b * (ncell + 1) + c] = 0;  // all zero again for the next build
     int s0 = start[b * (ncell + 1) + c], s1 = start[b * (ncell + 1) + c + 1];
     int* p = perm + b * na;
     for (int x = s0 + 1; x < s1; x++) {
@@ -286,18 +287,14 @@ __global__ void k_cell_order(const double* __restrict__ site, int na, int nb, in
         o[x] = s[a];
         o[na + x] = s[na + a];
         o[2 * na + x] = s[2 * na + a];
+        iperm[b * na + a] = x;
     }
 }
 
-__global__ void k_fill_int_gated(int* __restrict__ y, size_t n, int v, const int* __restrict__ gate) {
-    if (gate && !gate[0]) return;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = v;
-}
-
-// counting sort of the sites into cells; with gate, every kernel returns at once unless gate[0]
+// counting sort of the sites into cells (count must be all zero on entry; k_cell_order leaves it so);
+// every kernel returns at once unless the gate is on
 void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, const int* gate, hipStream_t s) {
+                       int* count, int* start, int* perm, double* sorted, int* iperm, VGate gate, hipStream_t s) {
     size_t n = (size_t)na * nb;
     int nbins = g.ncell;
     if (morton) {
@@ -305,14 +302,13 @@ void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitb
         while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
         nbins = p * p * p;
     }
-    const size_t nz = (size_t)(nbins + 1) * nb;
-    k_fill_int_gated<<<(unsigned)((nz + 255) / 256), 256, 0, s>>>(count, nz, 0, gate);
     k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, morton, nbins, unitbox.inv[0],
                                                               unitbox.inv[1], unitbox.inv[2], cell_of, count, gate);
     k_cell_scan<<<nb, 1024, 0, s>>>(count, start, nbins, gate);
     k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, nbins, start, count, perm, gate);
     size_t nc = (size_t)nbins * nb;
-    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted, gate);
+    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted, iperm, count,
+                                                              gate);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -365,14 +361,11 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
 
 __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 
-// bounding box of each cluster in wrapped fractional coordinates: bb[(b*nclu + c)*6] = {centre, half}
-__global__ __launch_bounds__(64) void k_clu_bbox_gated(const double* __restrict__ sorted, int na, int nclu,
-                                                       double i0, double i1, double i2, double* __restrict__ bb,
-                                                       const int* __restrict__ gate) {
-    if (!gate[0]) return;
-    const int c = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+// bounding box, in wrapped fractional coordinates, of cluster c of bead b: {centre, half-width}
+// per axis; every lane of the wave returns it
+__device__ __forceinline__ void clu_bbox(const double* __restrict__ X, int na, int c, int l, double i0, double i1,
+                                         double i2, double (&ctr)[3], double (&half)[3]) {
     const int a = c * CLU + l;
-    const double* X = sorted + (size_t)b * 3 * na;
     const double inv[3] = {i0, i1, i2};
     double lo[3], hi[3];
 #pragma unroll
@@ -387,37 +380,51 @@ __global__ __launch_bounds__(64) void k_clu_bbox_gated(const double* __restrict_
             lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
         }
-    if (l == 0) {
-        double* o = bb + ((size_t)b * nclu + c) * 6;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            o[d] = 0.5 * (lo[d] + hi[d]);
-            o[3 + d] = 0.5 * (hi[d] - lo[d]);
-        }
+    for (int d = 0; d < 3; d++) {
+        ctr[d] = 0.5 * (lo[d] + hi[d]);
+        half[d] = 0.5 * (hi[d] - lo[d]);
     }
 }
 
 // full cluster-pair lists, one wave per (I, bead): J ascending with bounding boxes closer than rc
-__global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ bb, int nclu, double L0, double L1,
-                                                  double L2, double rc2, int* __restrict__ cnt,
-                                                  int* __restrict__ list, const int* __restrict__ gate) {
-    if (!gate[0]) return;
+// (minimum image of the centres; L = box side, 1 for the unit box).  The boxes of all clusters are
+// formed by each wave (lanes 0-63 own cluster J0 + lane afterwards).
+__global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ sorted, int na, int nclu, double L0,
+                                                  double L1, double L2, double i0, double i1, double i2, double rc2,
+                                                  int* __restrict__ cnt, int* __restrict__ list, VGate gate) {
+    if (!gate.on()) return;
     const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
-    const double* bi = bb + ((size_t)b * nclu + I) * 6;
+    const double* X = sorted + (size_t)b * 3 * na;
     const double L[3] = {L0, L1, L2};
+    double ci[3], hi_[3];
+    clu_bbox(X, na, I, l, i0, i1, i2, ci, hi_);
     int* out = list + ((size_t)b * nclu + I) * nclu;
     int n = 0;
     for (int J0 = 0; J0 < nclu; J0 += 64) {
+        // the box of cluster J0 + l, formed by the wave one cluster at a time
+        double cj[3] = {0, 0, 0}, hj[3] = {0, 0, 0};
+        const int jn = min(64, nclu - J0);
+        for (int q = 0; q < jn; q++) {
+            double c3[3], h3[3];
+            clu_bbox(X, na, J0 + q, l, i0, i1, i2, c3, h3);
+            if (l == q) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    cj[d] = c3[d];
+                    hj[d] = h3[d];
+                }
+            }
+        }
         const int J = J0 + l;
         bool keep = false;
         if (J < nclu) {
-            const double* bj = bb + ((size_t)b * nclu + J) * 6;
             double r2 = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                double dc = bi[d] - bj[d];
+                double dc = ci[d] - cj[d];
                 dc -= rint(dc);
-                const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
+                const double gap = fmax(0.0, fabs(dc) - hi_[d] - hj[d]) * L[d];
                 r2 += gap * gap;
             }
             keep = r2 < rc2;
@@ -429,11 +436,10 @@ __global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ bb,
     if (l == 0) cnt[b * nclu + I] = n;
 }
 
-// largest squared displacement of any site since the last build, per block
+// largest squared displacement of any site since the last build: one atomicMax per block on the
+// bit pattern (non-negative doubles order as their bits); k_vl_gather clears it after the build
 __global__ __launch_bounds__(256) void k_vl_disp(const double* __restrict__ site, const double* __restrict__ rb,
-                                                 size_t n3, size_t stride, double* __restrict__ dparts) {
-    // site and rb are [nb][3][na]; element e = (b, d, a) -> squared displacement summed over d by
-    // visiting the three components of one (b, a) in one thread
+                                                 size_t n3, size_t stride, unsigned long long* __restrict__ dmax) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (t < n3) {
@@ -442,28 +448,14 @@ __global__ __launch_bounds__(256) void k_vl_disp(const double* __restrict__ site
         const double dx = site[o] - rb[o], dy = site[o + stride] - rb[o + stride],
                      dz = site[o + 2 * stride] - rb[o + 2 * stride];
         m = dx * dx + dy * dy + dz * dz;
-        if (!(m == m)) m = 1e300;  // never built (NaN reference positions)
+        if (!(m <= 1e300)) m = 1e300;  // never built (NaN reference positions)
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
     __shared__ double red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) dparts[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-}
-
-// gate[0] = rebuild (largest displacement above skin/2, or forced); gate[1] (overflow) is sticky
-__global__ __launch_bounds__(256) void k_vl_decide(const double* __restrict__ dparts, int nblk, double lim2,
-                                                   int force, int* __restrict__ gate) {
-    __shared__ double red[256];
-    double m = 0.0;
-    for (int q = threadIdx.x; q < nblk; q += 256) m = fmax(m, dparts[q]);
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gate[0] = (force || red[0] > lim2) ? 1 : 0;
+    if (threadIdx.x == 0)
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
 // list entry n of sorted site k: the VLS lanes of a site read consecutive words
@@ -471,50 +463,42 @@ __device__ __forceinline__ size_t vl_slot(int n, int k, int na) {
     return ((size_t)(n / VLS) * na + k) * VLS + (n % VLS);
 }
 
-// inverse of the sort permutation of the build: iperm[b][site] = sorted index
-__global__ void k_vl_iperm(const int* __restrict__ perm, int na, int nb, int* __restrict__ iperm,
-                           const int* __restrict__ gate) {
-    if (!gate[0]) return;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)na * nb) return;
-    const size_t b = t / na;
-    iperm[b * na + perm[t]] = (int)(t % na);
-}
-
-// per sorted site: its list of sorted neighbours within rc + skin (exact FP64 test); the sites of
-// its own molecule first (nex[] of them; the real-space term treats those pairs as excluded,
-// src/ewald_cubic.f90:134), then the others in cluster order.  Also the reference positions and
-// the charges of the build.
+// per sorted site: its list of sorted neighbours within rc + skin; the sites of its own molecule
+// first (nex[] of them; the real-space term treats those pairs as excluded, src/ewald_cubic.f90:134),
+// then the others in cluster order.  The test is FP32 on wrapped fractional coordinates against
+// rl^2 enlarged by a margin that covers the FP32 error, so the list is a superset of the pairs within
+// rl (a few pairs just beyond do no harm: the pair kernel tests the cut-off exactly).  Also the
+// reference positions and the charges of the build.
 __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sorted, const int* __restrict__ perm,
                                                  const int* __restrict__ iperm, const double* __restrict__ site,
                                                  const double* __restrict__ q, int na, int nclu,
                                                  const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                 double L0, double L1, double L2, double i0, double i1, double i2,
-                                                 double rl2, int cap, int* __restrict__ cnt, int* __restrict__ nex,
+                                                 float L0, float L1, float L2, double i0, double i1, double i2,
+                                                 float rl2f, int cap, int* __restrict__ cnt, int* __restrict__ nex,
                                                  int* __restrict__ nl, double* __restrict__ rb,
-                                                 double* __restrict__ sq, int* __restrict__ gate) {
-    if (!gate[0]) return;
+                                                 double* __restrict__ sq, int* __restrict__ ovfl, VGate gate) {
+    if (!gate.on()) return;
     const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
     const int k = I * CLU + l;
     const bool kv = k < na;
     const double* X = sorted + (size_t)b * 3 * na;
-    __shared__ double sx[CLU], sy[CLU], sz[CLU];
-    __shared__ int sa[CLU];
-    double xi = 0, yi = 0, zi = 0;
+    __shared__ float fx[CLU], fy[CLU], fz[CLU];
+    __shared__ int sm[CLU];
+    float xi = 0.f, yi = 0.f, zi = 0.f;
     int a = -3, n = 0;
     bool ovf = false;
     int* out = nl + (size_t)b * cap * na;
-    auto dist2 = [&](double xj, double yj, double zj) {
-        double dx = xi - xj, dy = yi - yj, dz = zi - zj;
-        dx -= L0 * rint(dx * i0);
-        dy -= L1 * rint(dy * i1);
-        dz -= L2 * rint(dz * i2);
-        return dx * dx + dy * dy + dz * dz;
+    auto near = [&](float xj, float yj, float zj) {
+        float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        dx = (dx - rintf(dx)) * L0;
+        dy = (dy - rintf(dy)) * L1;
+        dz = (dz - rintf(dz)) * L2;
+        return dx * dx + dy * dy + dz * dz < rl2f;
     };
     if (kv) {
-        xi = X[k];
-        yi = X[na + k];
-        zi = X[2 * na + k];
+        xi = (float)wrap01(X[k] * i0);
+        yi = (float)wrap01(X[na + k] * i1);
+        zi = (float)wrap01(X[2 * na + k] * i2);
         a = perm[(size_t)b * na + k];
         const double* S = site + (size_t)b * 3 * na;
         double* R = rb + (size_t)b * 3 * na;
@@ -526,7 +510,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
         for (int p = m0; p < m0 + 3; p++) {
             if (p == a || p >= na) continue;
             const int jk = iperm[(size_t)b * na + p];
-            if (dist2(X[jk], X[na + jk], X[2 * na + jk]) < rl2) {
+            if (near((float)wrap01(X[jk] * i0), (float)wrap01(X[na + jk] * i1), (float)wrap01(X[2 * na + jk] * i2))) {
                 if (n < cap) out[vl_slot(n, k, na)] = jk;
                 else ovf = true;
                 n++;
@@ -542,18 +526,18 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
         const int j = J * CLU + l;
         __syncthreads();
         if (j < na) {
-            sx[l] = X[j];
-            sy[l] = X[na + j];
-            sz[l] = X[2 * na + j];
-            sa[l] = perm[(size_t)b * na + j];
+            fx[l] = (float)wrap01(X[j] * i0);
+            fy[l] = (float)wrap01(X[na + j] * i1);
+            fz[l] = (float)wrap01(X[2 * na + j] * i2);
+            sm[l] = perm[(size_t)b * na + j] / 3;
         }
         __syncthreads();
         const int jmax = min(CLU, na - J * CLU);
         if (kv) {
             for (int jj = 0; jj < jmax; jj++) {
                 const int jk = J * CLU + jj;
-                if (jk == k || sa[jj] / 3 == mi) continue;
-                if (dist2(sx[jj], sy[jj], sz[jj]) < rl2) {
+                if (jk == k || sm[jj] == mi) continue;
+                if (near(fx[jj], fy[jj], fz[jj])) {
                     if (n < cap) out[vl_slot(n, k, na)] = jk;
                     else ovf = true;
                     n++;
@@ -562,14 +546,15 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
         }
     }
     if (kv) cnt[(size_t)b * na + k] = min(n, cap);
-    if (ovf) gate[1] = 1;
+    if (ovf) ovfl[0] = 1;
 }
 
 // current coordinates in sorted order: (x, y, z, q) in FP64 and wrapped fractional FP32
 __global__ void k_vl_gather(const double* __restrict__ site, const int* __restrict__ perm,
                             const double* __restrict__ sq, int na, int nb, double i0, double i1, double i2,
-                            double4* __restrict__ srt, float4* __restrict__ srf) {
+                            double4* __restrict__ srt, float4* __restrict__ srf, unsigned long long* __restrict__ dmax) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) dmax[0] = 0ull;  // every gated kernel of this call has run
     if (t >= (size_t)na * nb) return;
     const size_t b = t / na;
     const int a = perm[t];
@@ -691,6 +676,14 @@ int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
 int ewald_real_slots() { return 1; }
 int ewald_real_parts(int na) { return (na * VLS + 255) / 256; }
 
+// FP32 pair-test margin (relative, on r^2): wrapped fractional FP32 coordinates carry an absolute
+// error of about 2^-24 per component, so |dr| errs by less than 1e-6 of the box side; the margin
+// 1e-4 + 1e-5 L_max / r covers that for any cut-off r
+static double pair_margin(const Box& sb, double r) {
+    const double lmax = fmax(sb.L[0], fmax(sb.L[1], sb.L[2]));
+    return 1e-4 + 1e-5 * lmax / r;
+}
+
 // expected neighbours within rl of a site, with margin, as the list capacity
 int vlist_cap(int na, Box sb, double rl) {
     const double vol = sb.L[0] * sb.L[1] * sb.L[2];
@@ -704,23 +697,22 @@ int vlist_cap(int na, Box sb, double rl) {
 
 // The list cut-off rl = rc + skin; the build rebuilds when a site has moved by more than skin/2.
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
-                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, double* bb,
-                  hipStream_t s) {
+                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s) {
     const size_t nsite = (size_t)na * nb;
     const unsigned nblk = (unsigned)((nsite + 255) / 256);
-    k_vl_disp<<<nblk, 256, 0, s>>>(site, v.rb, nsite, (size_t)na, v.dparts);
-    k_vl_decide<<<1, 256, 0, s>>>(v.dparts, (int)nblk, 0.25 * skin * skin, force, v.gate);
-    launch_cell_build(site, na, nb, g, sb, 1, cell_of, count, start, v.perm, sorted, v.gate, s);
+    k_vl_disp<<<nblk, 256, 0, s>>>(site, v.rb, nsite, (size_t)na, v.dmax);
+    const VGate gate{v.dmax, 0.25 * skin * skin, force};
+    launch_cell_build(site, na, nb, g, sb, 1, cell_of, count, start, v.perm, sorted, v.iperm, gate, s);
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
-    k_clu_bbox_gated<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.inv[0], sb.inv[1], sb.inv[2], bb, v.gate);
-    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(bb, nclu, sb.L[0], sb.L[1], sb.L[2], rl * rl * (1.0 + 1e-10), v.ccnt,
-                                             v.clist, v.gate);
-    k_vl_iperm<<<nblk, 256, 0, s>>>(v.perm, na, nb, v.iperm, v.gate);
-    k_vl_build<<<dim3(nclu, nb), 64, 0, s>>>(sorted, v.perm, v.iperm, site, q, na, nclu, v.ccnt, v.clist, sb.L[0],
-                                            sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rl * rl, v.cap, v.cnt,
-                                            v.nex, v.nl, v.rb, v.sq, v.gate);
-    k_vl_gather<<<nblk, 256, 0, s>>>(site, v.perm, v.sq, na, nb, sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf);
+    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
+                                             sb.inv[2], rl * rl * (1.0 + 1e-10), v.ccnt, v.clist, gate);
+    const float rl2f = (float)(rl * rl * (1.0 + pair_margin(sb, rl)));
+    k_vl_build<<<dim3(nclu, nb), 64, 0, s>>>(sorted, v.perm, v.iperm, site, q, na, nclu, v.ccnt, v.clist,
+                                            (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0], sb.inv[1],
+                                            sb.inv[2], rl2f, v.cap, v.cnt, v.nex, v.nl, v.rb, v.sq, v.gate + 1, gate);
+    k_vl_gather<<<nblk, 256, 0, s>>>(site, v.perm, v.sq, na, nb, sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf,
+                                     v.dmax);
 }
 
 template <typename T>
