@@ -307,7 +307,7 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
 void free_vlist(VList& v) {
-    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dmax, v.srt, v.srf, v.sq, v.iperm, v.nex, v.ccnt, v.clist};
+    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dmax, v.srt, v.srf, v.iperm, v.nex, v.ccnt, v.clist, v.bb};
     for (void* b : bufs)
         if (b) hipFree(b);
     v = VList{};
@@ -353,10 +353,10 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.dmax, 1);
         e |= dalloc(&v.srt, ns);
         e |= dalloc(&v.srf, ns);
-        e |= dalloc(&v.sq, ns);
         e |= dalloc(&v.iperm, ns);
         e |= dalloc(&v.nex, ns);
         e |= dalloc(&v.ccnt, (size_t)nclu * nbl);
+        e |= dalloc(&v.bb, (size_t)6 * nclu * nbl);
         e |= dalloc(&v.clist, (size_t)nclu * nclu * nbl);
         if (e) {
             free_vlist(v);
@@ -417,7 +417,6 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         RETCHK(build_kset(c, box));
         const int cubic = P.iamcub;
         const Box sb = cubic ? unit_box() : bx;
-        launch_sites(pos, c->site, na, nb, P.alpha, cubic, bx, s);
         CellGrid g = make_grid(sb, na);
         if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
         // list skin of 1 A (src/main.f90 uses none: the reference rebuilds its cells every call)
@@ -425,6 +424,8 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         VList* vl = nullptr;
         int force = 0;
         RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
+        // sites, phases, list displacement and (for a built list) the sorted coordinates
+        launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, c->q, *vl, !force, sb, s);
         {
             ProfScope ps(c, 4);
             launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start,
@@ -445,7 +446,6 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, s);
         }
         double* pk = c->parts + c->off_k;
-        launch_phase(c->site, na, nb, c->ks, c->ph, s);
         {
             ProfScope ps(c, 1);
             launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, mixed, s);

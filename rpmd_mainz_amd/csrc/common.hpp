@@ -74,14 +74,12 @@ struct VGate {
     int force;
     __device__ __forceinline__ bool on() const { return force || __longlong_as_double((long long)*dmax) > lim2; }
 };
-void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, int* iperm, VGate gate, hipStream_t s);
-
 // Verlet lists of the real-space Ewald sum (one cache per bead set); arrays sized for nb_max beads
+#define VLS 4   // lanes per site in the Verlet-list walk (list rows of VLS entries)
 struct VList {
     int* perm;       // [nb][na] sorted index -> site index (of the last build)
     int* cnt;        // [nb][na] list length of sorted site k
-    int* nl;         // [nb][cap][na] sorted neighbour indices, column-major
+    int* nl;         // [nb][cap / VLS][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
     double* rb;      // [nb][3][na] site coordinates at the last build
     int* gate;       // [1] list overflow (sticky)
     unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
@@ -89,15 +87,18 @@ struct VList {
     int* nex;        // [nb][na] leading list entries that are the site's own molecule
     double4* srt;    // [nb][na] current coordinates and charge in sorted order
     float4* srf;     // [nb][na] wrapped fractional FP32 coordinates and charge, sorted order
-    double* sq;      // [nb][na] charges, sorted order
     int* ccnt;       // [nb][nclu] cluster-pair list lengths
     int* clist;      // [nb][nclu][nclu] cluster-pair lists
+    double* bb;      // [nb][nclu][6] cluster boxes (fractional centre, half-width)
     int cap;
 };
 int ewald_nclu(int na);
 int ewald_real_slots();
 int ewald_real_parts(int na);
 int vlist_cap(int na, Box sb, double rl);
+void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
+                     const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
+                     hipStream_t s);
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s);
 template <typename T>
@@ -105,7 +106,6 @@ void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, doub
                        hipStream_t s);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
-void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s);
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
                hipStream_t s);
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,

@@ -182,9 +182,8 @@ void launch_zero_int(int* y, size_t n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// cell sort of the Ewald sites (per bead; counting sort, each cell then ordered by atom index).  With
-// morton = 1 the cells are numbered in Morton (Z-curve) order over a power-of-two grid, so that
-// consecutive sorted sites stay close in space.
+// cell coordinates of the Ewald sites; the cells are numbered in Morton (Z-curve) order over a
+// power-of-two grid, so that sites sorted by cell stay close in space
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t part1by2(uint32_t x) {
     x &= 0x3ff;
@@ -202,131 +201,21 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
     return c < nc ? (c >= 0 ? c : 0) : nc - 1;
 }
 
-__global__ void k_cell_assign(const double* __restrict__ site, int na, int nb, CellGrid g, int morton, int nbins,
-                              double i0, double i1, double i2, int* __restrict__ cell_of, int* __restrict__ count,
-                              VGate gate) {
-    if (!gate.on()) return;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)na * nb) return;
-    size_t b = i / na, a = i % na;
-    const double* s = site + b * 3 * na;
-    int cx = cell_coord(s[a], i0, g.nc[0]);
-    int cy = cell_coord(s[na + a], i1, g.nc[1]);
-    int cz = cell_coord(s[2 * na + a], i2, g.nc[2]);
-    int cid = morton ? (int)(part1by2(cx) | (part1by2(cy) << 1) | (part1by2(cz) << 2)) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
-    cell_of[i] = cid;
-    atomicAdd(&count[b * (nbins + 1) + cid], 1);
-}
-
-// exclusive scan per bead (one block per bead); resets count to 0 for the scatter pass
-__global__ void k_cell_scan(int* __restrict__ count, int* __restrict__ start, int ncell, VGate gate) {
-    if (!gate.on()) return;
-    int b = blockIdx.x;
-    int* c = count + (size_t)b * (ncell + 1);
-    int* st = start + (size_t)b * (ncell + 1);
-    __shared__ int carry;
-    __shared__ int buf[1024];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < ncell; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < ncell ? c[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < ncell) {
-            st[i] = carry + buf[threadIdx.x] - v;
-            c[i] = 0;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) st[ncell] = carry;
-}
-
-__global__ void k_cell_scatter(const int* __restrict__ cell_of, int na, int nb, int ncell, const int* __restrict__ start,
-                               int* __restrict__ fill, int* __restrict__ perm, VGate gate) {
-    if (!gate.on()) return;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)na * nb) return;
-    size_t b = i / na, a = i % na;
-    int cid = cell_of[i];
-    int slot = start[b * (ncell + 1) + cid] + atomicAdd(&fill[b * (ncell + 1) + cid], 1);
-    perm[b * na + slot] = (int)a;
-}
-
-// order each cell by atom index, then gather the sorted positions
-__global__ void k_cell_order(const double* __restrict__ site, int na, int nb, int ncell, const int* __restrict__ start,
-                             int* __restrict__ perm, double* __restrict__ sorted, int* __restrict__ iperm,
-                             int* __restrict__ count, VGate gate) {
-    if (!gate.on()) return;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)ncell * nb) return;
-    size_t b = i / ncell, c = i % ncell;
-    count[b * (ncell + 1) + c] = 0;  // all zero again for the next build
-    int s0 = start[b * (ncell + 1) + c], s1 = start[b * (ncell + 1) + c + 1];
-    int* p = perm + b * na;
-    for (int x = s0 + 1; x < s1; x++) {
-        int v = p[x], y = x - 1;
-        while (y >= s0 && p[y] > v) {
-            p[y + 1] = p[y];
-            y--;
-        }
-        p[y + 1] = v;
-    }
-    const double* s = site + b * 3 * na;
-    double* o = sorted + b * 3 * na;
-    for (int x = s0; x < s1; x++) {
-        int a = p[x];
-        o[x] = s[a];
-        o[na + x] = s[na + a];
-        o[2 * na + x] = s[2 * na + a];
-        iperm[b * na + a] = x;
-    }
-}
-
-// counting sort of the sites into cells (count must be all zero on entry; k_cell_order leaves it so);
-// every kernel returns at once unless the gate is on
-void launch_cell_build(const double* site, int na, int nb, CellGrid g, Box unitbox, int morton, int* cell_of,
-                       int* count, int* start, int* perm, double* sorted, int* iperm, VGate gate, hipStream_t s) {
-    size_t n = (size_t)na * nb;
-    int nbins = g.ncell;
-    if (morton) {
-        int p = 1;
-        while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
-        nbins = p * p * p;
-    }
-    k_cell_assign<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, g, morton, nbins, unitbox.inv[0],
-                                                              unitbox.inv[1], unitbox.inv[2], cell_of, count, gate);
-    k_cell_scan<<<nb, 1024, 0, s>>>(count, start, nbins, gate);
-    k_cell_scatter<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(cell_of, na, nb, nbins, start, count, perm, gate);
-    size_t nc = (size_t)nbins * nb;
-    k_cell_order<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(site, na, nb, nbins, start, perm, sorted, iperm, count,
-                                                              gate);
-}
-
 // ------------------------------------------------------------------------------------------------
 // real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
 //
 // Per-site Verlet lists with a skin.  A list holds, for every site of every bead, the sites within
 // rc + skin at the time it was built; it stays exact -- it contains every pair now within rc --
 // while no site has moved more than skin/2 since (a pair distance changes by at most twice the
-// largest displacement).  Each call measures that displacement on the device (k_vl_disp,
-// k_vl_decide) and sets gate[0]; the build kernels return at once unless it is set, so a captured
-// hipGraph rebuilds exactly when needed.  The build sorts the sites by cell in Morton order
-// (launch_cell_build), groups runs of 64 sorted sites into clusters with bounding boxes, lists the
-// cluster pairs within reach, and fills each site's list (sorted indices, cluster order) with an
-// exact FP64 distance test.  The pair kernel gives one lane to a sorted site, which walks its own
-// list: every pair is evaluated from both ends (energy and virial with a factor 1/2) and each
-// site's gradient is summed by its own lane in list order, so results are deterministic.  T is the
-// pair-body type; in FP32 the distances come from wrapped fractional FP32 coordinates and the
-// sums are flushed into FP64 every 32 pairs.
+// largest displacement).  Each call measures that displacement on the device (k_sites_vl, an
+// atomic max); the build kernels return at once unless it exceeds skin/2 (VGate), so a captured
+// hipGraph rebuilds exactly when needed.  The build (k_vl_sort) sorts the sites by cell in Morton
+// order, groups runs of 64 sorted sites into clusters with bounding boxes and lists the cluster pairs
+// within reach; k_vl_build then fills each site's list (sorted indices, cluster order).  The pair
+// kernel gives VLS lanes to a sorted site, which walk its list: every pair is evaluated from both
+// ends (energy and virial with a factor 1/2) and each site's gradient is summed in a fixed order, so
+// results are deterministic.  T is the pair-body type; in FP32 the distances come from wrapped
+// fractional FP32 coordinates and the sums are flushed into FP64 every 16 pairs of a lane.
 // ------------------------------------------------------------------------------------------------
 // FP64: du = erfc_AS(alpha r)/r (erfc - 1 within a molecule), dur = (sfac e^{-x^2} - du)/r^2, the
 // reference's expressions (src/ewald_cubic.f90:126-140) with the two divisions by r and r^2 taken as
@@ -357,98 +246,56 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
 }
 
 #define CLU 64
-#define VLS 4   // lanes per site in the list walk
 
 __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 
-// bounding box, in wrapped fractional coordinates, of cluster c of bead b: {centre, half-width}
-// per axis; every lane of the wave returns it
-__device__ __forceinline__ void clu_bbox(const double* __restrict__ X, int na, int c, int l, double i0, double i1,
-                                         double i2, double (&ctr)[3], double (&half)[3]) {
-    const int a = c * CLU + l;
-    const double inv[3] = {i0, i1, i2};
-    double lo[3], hi[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double f = a < na ? wrap01(X[d * na + a] * inv[d]) : 0.0;
-        lo[d] = a < na ? f : 2.0;
-        hi[d] = a < na ? f : -1.0;
-    }
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
-        }
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        ctr[d] = 0.5 * (lo[d] + hi[d]);
-        half[d] = 0.5 * (hi[d] - lo[d]);
-    }
-}
-
-// full cluster-pair lists, one wave per (I, bead): J ascending with bounding boxes closer than rc
-// (minimum image of the centres; L = box side, 1 for the unit box).  The boxes of all clusters are
-// formed by each wave (lanes 0-63 own cluster J0 + lane afterwards).
-__global__ __launch_bounds__(64) void k_clu_pairs(const double* __restrict__ sorted, int na, int nclu, double L0,
-                                                  double L1, double L2, double i0, double i1, double i2, double rc2,
-                                                  int* __restrict__ cnt, int* __restrict__ list, VGate gate) {
-    if (!gate.on()) return;
-    const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
-    const double* X = sorted + (size_t)b * 3 * na;
-    const double L[3] = {L0, L1, L2};
-    double ci[3], hi_[3];
-    clu_bbox(X, na, I, l, i0, i1, i2, ci, hi_);
-    int* out = list + ((size_t)b * nclu + I) * nclu;
-    int n = 0;
-    for (int J0 = 0; J0 < nclu; J0 += 64) {
-        // the box of cluster J0 + l, formed by the wave one cluster at a time
-        double cj[3] = {0, 0, 0}, hj[3] = {0, 0, 0};
-        const int jn = min(64, nclu - J0);
-        for (int q = 0; q < jn; q++) {
-            double c3[3], h3[3];
-            clu_bbox(X, na, J0 + q, l, i0, i1, i2, c3, h3);
-            if (l == q) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    cj[d] = c3[d];
-                    hj[d] = h3[d];
-                }
-            }
-        }
-        const int J = J0 + l;
-        bool keep = false;
-        if (J < nclu) {
-            double r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double dc = ci[d] - cj[d];
-                dc -= rint(dc);
-                const double gap = fmax(0.0, fabs(dc) - hi_[d] - hj[d]) * L[d];
-                r2 += gap * gap;
-            }
-            keep = r2 < rc2;
-        }
-        const uint64_t m = __ballot(keep);
-        if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
-        n += __popcll(m);
-    }
-    if (l == 0) cnt[b * nclu + I] = n;
-}
-
-// largest squared displacement of any site since the last build: one atomicMax per block on the
-// bit pattern (non-negative doubles order as their bits); k_vl_gather clears it after the build
-__global__ __launch_bounds__(256) void k_vl_disp(const double* __restrict__ site, const double* __restrict__ rb,
-                                                 size_t n3, size_t stride, unsigned long long* __restrict__ dmax) {
+// Per site of every bead: the Ewald site (the M-site combination of the oxygen and its hydrogens at
+// a%3 == 0, as k_sites), scaled by sc; its phase factors e^{i lat_d x_d}; its squared displacement
+// since the list build, max-reduced into dmax as bit patterns (non-negative doubles order as their
+// bits; NaN reference positions of a list never built count as huge); and, when gather is set, the
+// site written at its sorted index iperm[] as (x, y, z, q) in FP64 and as wrapped fractional FP32
+// coordinates (inverse box side f).
+__global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
+                                                  int nb, double alpha, double s0, double s1, double s2, double l0,
+                                                  double l1, double l2, double2* __restrict__ ph,
+                                                  const double* __restrict__ q, const double* __restrict__ rb,
+                                                  const int* __restrict__ iperm, int gather, double f0, double f1,
+                                                  double f2, double4* __restrict__ srt, float4* __restrict__ srf,
+                                                  unsigned long long* __restrict__ dmax) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
-    if (t < n3) {
-        const size_t b = t / stride, a = t % stride;
-        const size_t o = b * 3 * stride + a;
-        const double dx = site[o] - rb[o], dy = site[o + stride] - rb[o + stride],
-                     dz = site[o + 2 * stride] - rb[o + 2 * stride];
+    if (t < (size_t)na * nb) {
+        const size_t b = t / na;
+        const int a = (int)(t % na), m0 = a - a % 3;
+        const double alpha2 = 0.5 * (1.0 - alpha);
+        const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
+        double x[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double* p = pos + (b * 3 + d) * na;
+            const size_t o = (b * 3 + d) * na + a;
+            if (a == m0) {
+                const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
+                x[d] = sc[d] * v;
+            } else {
+                x[d] = sc[d] * p[a];
+            }
+            site[o] = x[d];
+            double sn, cs;
+            sincos(lat[d] * x[d], &sn, &cs);
+            ph[o] = make_double2(cs, sn);
+        }
+        const size_t o = b * 3 * na + a;
+        const double dx = x[0] - rb[o], dy = x[1] - rb[o + na], dz = x[2] - rb[o + 2 * na];
         m = dx * dx + dy * dy + dz * dz;
-        if (!(m <= 1e300)) m = 1e300;  // never built (NaN reference positions)
+        if (!(m <= 1e300)) m = 1e300;
+        if (gather) {
+            const size_t k = b * na + iperm[t];
+            const double qq = q[a];
+            srt[k] = make_double4(x[0], x[1], x[2], qq);
+            srf[k] = make_float4((float)wrap01(x[0] * f0), (float)wrap01(x[1] * f1), (float)wrap01(x[2] * f2),
+                                 (float)qq);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
     __shared__ double red[4];
@@ -458,17 +305,159 @@ __global__ __launch_bounds__(256) void k_vl_disp(const double* __restrict__ site
         atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
-// list entry n of sorted site k: the VLS lanes of a site read consecutive words
-__device__ __forceinline__ size_t vl_slot(int n, int k, int na) {
-    return ((size_t)(n / VLS) * na + k) * VLS + (n % VLS);
+// One block per bead, when the gate is on: counting sort of the sites into Morton cells, each cell
+// then ordered by site index, giving perm / iperm and the sorted positions; the bounding box of each
+// run of CLU sorted sites (a cluster) in wrapped fractional coordinates, bb[(b*nclu + c)*6] =
+// {centre, half-width}; and per cluster I the clusters J whose boxes come closer than rl (minimum
+// image of the centres; L = box side, 1 for the unit box), ascending.  count must be all zero on
+// entry and is left so.
+__global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ site, int na, CellGrid g, int nbins,
+                                                  double i0, double i1, double i2, double L0, double L1, double L2,
+                                                  double rl2, int nclu, int* __restrict__ cell_of,
+                                                  int* __restrict__ count, int* __restrict__ start,
+                                                  int* __restrict__ perm, int* __restrict__ iperm,
+                                                  double* __restrict__ sorted, double* __restrict__ bb,
+                                                  int* __restrict__ ccnt, int* __restrict__ clist, VGate gate) {
+    if (!gate.on()) return;
+    const int b = blockIdx.x, t = threadIdx.x;
+    constexpr int NT = 1024;
+    const double* S = site + (size_t)b * 3 * na;
+    int* cof = cell_of + (size_t)b * na;
+    int* cn = count + (size_t)b * (nbins + 1);
+    int* st = start + (size_t)b * (nbins + 1);
+    int* P = perm + (size_t)b * na;
+    int* IP = iperm + (size_t)b * na;
+    double* O = sorted + (size_t)b * 3 * na;
+    for (int a = t; a < na; a += NT) {
+        const int cx = cell_coord(S[a], i0, g.nc[0]);
+        const int cy = cell_coord(S[na + a], i1, g.nc[1]);
+        const int cz = cell_coord(S[2 * na + a], i2, g.nc[2]);
+        const int cid = (int)(part1by2(cx) | (part1by2(cy) << 1) | (part1by2(cz) << 2));
+        cof[a] = cid;
+        atomicAdd(&cn[cid], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts into start; the counts restart from zero as fill counters
+    __shared__ int buf[NT];
+    __shared__ int carry;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nbins; base += NT) {
+        const int i = base + t;
+        const int v = i < nbins ? cn[i] : 0;
+        buf[t] = v;
+        __syncthreads();
+        for (int off = 1; off < NT; off <<= 1) {
+            const int u = t >= off ? buf[t - off] : 0;
+            __syncthreads();
+            buf[t] += u;
+            __syncthreads();
+        }
+        if (i < nbins) {
+            st[i] = carry + buf[t] - v;
+            cn[i] = 0;
+        }
+        __syncthreads();
+        if (t == NT - 1) carry += buf[NT - 1];
+        __syncthreads();
+    }
+    if (t == 0) st[nbins] = carry;
+    __syncthreads();
+    for (int a = t; a < na; a += NT) {
+        const int cid = cof[a];
+        P[st[cid] + atomicAdd(&cn[cid], 1)] = a;
+    }
+    __syncthreads();
+    // order each cell by site index, then gather the sorted positions
+    for (int c = t; c < nbins; c += NT) {
+        cn[c] = 0;
+        const int s0 = st[c], s1 = st[c + 1];
+        for (int x = s0 + 1; x < s1; x++) {
+            const int v = P[x];
+            int y = x - 1;
+            while (y >= s0 && P[y] > v) {
+                P[y + 1] = P[y];
+                y--;
+            }
+            P[y + 1] = v;
+        }
+        for (int x = s0; x < s1; x++) {
+            const int a = P[x];
+            O[x] = S[a];
+            O[na + x] = S[na + a];
+            O[2 * na + x] = S[2 * na + a];
+            IP[a] = x;
+        }
+    }
+    __syncthreads();
+    // cluster boxes, wave w taking clusters w, w + NT/64, ...
+    const int w = t >> 6, l = t & 63;
+    constexpr int NW = NT / 64;
+    double* B = bb + (size_t)b * nclu * 6;
+    const double inv[3] = {i0, i1, i2};
+    for (int c = w; c < nclu; c += NW) {
+        const int a = c * CLU + l;
+        double lo[3], hi[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double f = a < na ? wrap01(O[d * na + a] * inv[d]) : 0.0;
+            lo[d] = a < na ? f : 2.0;
+            hi[d] = a < na ? f : -1.0;
+        }
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+            }
+        if (l == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                B[c * 6 + d] = 0.5 * (lo[d] + hi[d]);
+                B[c * 6 + 3 + d] = 0.5 * (hi[d] - lo[d]);
+            }
+        }
+    }
+    __syncthreads();
+    const double L[3] = {L0, L1, L2};
+    for (int I = w; I < nclu; I += NW) {
+        const double* bi = B + I * 6;
+        int* out = clist + ((size_t)b * nclu + I) * nclu;
+        int n = 0;
+        for (int J0 = 0; J0 < nclu; J0 += 64) {
+            const int J = J0 + l;
+            bool keep = false;
+            if (J < nclu) {
+                const double* bj = B + J * 6;
+                double r2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double dc = bi[d] - bj[d];
+                    dc -= rint(dc);
+                    const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
+                    r2 += gap * gap;
+                }
+                keep = r2 < rl2;
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
+            n += __popcll(m);
+        }
+        if (l == 0) ccnt[b * nclu + I] = n;
+    }
 }
 
-// per sorted site: its list of sorted neighbours within rc + skin; the sites of its own molecule
-// first (nex[] of them; the real-space term treats those pairs as excluded, src/ewald_cubic.f90:134),
-// then the others in cluster order.  The test is FP32 on wrapped fractional coordinates against
-// rl^2 enlarged by a margin that covers the FP32 error, so the list is a superset of the pairs within
-// rl (a few pairs just beyond do no harm: the pair kernel tests the cut-off exactly).  Also the
-// reference positions and the charges of the build.
+// Per sorted site its list of sorted neighbours within rc + skin: first the sites of its own
+// molecule (nex[] of them; the real-space term treats those pairs as excluded,
+// src/ewald_cubic.f90:134), then the others in cluster order.  Entry n of site k is
+// nl[((n/VLS)*na + k)*VLS + n%VLS]: the VLS lanes that walk a site's list read one 16-byte row, and
+// consecutive entries are usually neighbouring sites.  The test is FP32 on wrapped fractional
+// coordinates against rl^2 enlarged by a margin that covers the FP32 error, so the list is a superset
+// of the pairs within rl (a few pairs just beyond do no harm: the pair kernel tests the cut-off
+// exactly).  One wave per (cluster, bead): the candidates of each listed cluster are staged in LDS
+// (the next cluster's loaded meanwhile), each lane forms the 64-bit mask of its neighbours among them
+// and appends the set bits, storing whole rows.  The wave also records the reference positions and
+// writes the current coordinates in the new order.
 __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sorted, const int* __restrict__ perm,
                                                  const int* __restrict__ iperm, const double* __restrict__ site,
                                                  const double* __restrict__ q, int na, int nclu,
@@ -476,18 +465,32 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
                                                  float L0, float L1, float L2, double i0, double i1, double i2,
                                                  float rl2f, int cap, int* __restrict__ cnt, int* __restrict__ nex,
                                                  int* __restrict__ nl, double* __restrict__ rb,
-                                                 double* __restrict__ sq, int* __restrict__ ovfl, VGate gate) {
+                                                 double4* __restrict__ srt, float4* __restrict__ srf,
+                                                 int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
     const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
     const int k = I * CLU + l;
     const bool kv = k < na;
     const double* X = sorted + (size_t)b * 3 * na;
-    __shared__ float fx[CLU], fy[CLU], fz[CLU];
-    __shared__ int sm[CLU];
+    const int* Pb = perm + (size_t)b * na;
+    __shared__ float4 cj[CLU];  // candidate: wrapped fractional FP32 position, molecule (int bits)
     float xi = 0.f, yi = 0.f, zi = 0.f;
     int a = -3, n = 0;
     bool ovf = false;
-    int* out = nl + (size_t)b * cap * na;
+    int4 row = make_int4(0, 0, 0, 0);
+    int4* out = (int4*)(nl + (size_t)b * cap * na) + k;
+    auto push = [&](int j) {
+        const int f = n & (VLS - 1);
+        row.x = f == 0 ? j : row.x;
+        row.y = f == 1 ? j : row.y;
+        row.z = f == 2 ? j : row.z;
+        row.w = f == 3 ? j : row.w;
+        if (f == VLS - 1) {
+            if (n < cap) out[(size_t)(n / VLS) * na] = row;
+            else ovf = true;
+        }
+        n++;
+    };
     auto near = [&](float xj, float yj, float zj) {
         float dx = xi - xj, dy = yi - yj, dz = zi - zj;
         dx = (dx - rintf(dx)) * L0;
@@ -496,72 +499,66 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
         return dx * dx + dy * dy + dz * dz < rl2f;
     };
     if (kv) {
-        xi = (float)wrap01(X[k] * i0);
-        yi = (float)wrap01(X[na + k] * i1);
-        zi = (float)wrap01(X[2 * na + k] * i2);
-        a = perm[(size_t)b * na + k];
-        const double* S = site + (size_t)b * 3 * na;
+        const double x = X[k], y = X[na + k], z = X[2 * na + k];
+        xi = (float)wrap01(x * i0);
+        yi = (float)wrap01(y * i1);
+        zi = (float)wrap01(z * i2);
+        a = Pb[k];
         double* R = rb + (size_t)b * 3 * na;
-        R[a] = S[a];
-        R[na + a] = S[na + a];
-        R[2 * na + a] = S[2 * na + a];
-        sq[(size_t)b * na + k] = q[a];
+        R[a] = x;
+        R[na + a] = y;
+        R[2 * na + a] = z;
+        const double qq = q[a];
+        srt[(size_t)b * na + k] = make_double4(x, y, z, qq);
+        srf[(size_t)b * na + k] = make_float4(xi, yi, zi, (float)qq);
         const int m0 = 3 * (a / 3);
         for (int p = m0; p < m0 + 3; p++) {
             if (p == a || p >= na) continue;
             const int jk = iperm[(size_t)b * na + p];
-            if (near((float)wrap01(X[jk] * i0), (float)wrap01(X[na + jk] * i1), (float)wrap01(X[2 * na + jk] * i2))) {
-                if (n < cap) out[vl_slot(n, k, na)] = jk;
-                else ovf = true;
-                n++;
-            }
+            if (near((float)wrap01(X[jk] * i0), (float)wrap01(X[na + jk] * i1), (float)wrap01(X[2 * na + jk] * i2)))
+                push(jk);
         }
         nex[(size_t)b * na + k] = min(n, cap);
     }
     const int nJ = ccnt[b * nclu + I];
     const int* lst = clist + ((size_t)b * nclu + I) * nclu;
     const int mi = a / 3;
+    auto stage = [&](int J) {
+        const int j = J * CLU + l;
+        if (j < na)
+            return make_float4((float)wrap01(X[j] * i0), (float)wrap01(X[na + j] * i1),
+                               (float)wrap01(X[2 * na + j] * i2), __int_as_float(Pb[j] / 3));
+        return make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    };
+    float4 nxt = nJ > 0 ? stage(lst[0]) : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int e = 0; e < nJ; e++) {
         const int J = lst[e];
-        const int j = J * CLU + l;
         __syncthreads();
-        if (j < na) {
-            fx[l] = (float)wrap01(X[j] * i0);
-            fy[l] = (float)wrap01(X[na + j] * i1);
-            fz[l] = (float)wrap01(X[2 * na + j] * i2);
-            sm[l] = perm[(size_t)b * na + j] / 3;
-        }
+        cj[l] = nxt;
         __syncthreads();
-        const int jmax = min(CLU, na - J * CLU);
+        if (e + 1 < nJ) nxt = stage(lst[e + 1]);
         if (kv) {
-            for (int jj = 0; jj < jmax; jj++) {
-                const int jk = J * CLU + jj;
-                if (jk == k || sm[jj] == mi) continue;
-                if (near(fx[jj], fy[jj], fz[jj])) {
-                    if (n < cap) out[vl_slot(n, k, na)] = jk;
-                    else ovf = true;
-                    n++;
-                }
+            uint64_t mask = 0;
+#pragma unroll
+            for (int jj = 0; jj < CLU; jj++) {
+                const float4 c = cj[jj];
+                const bool take = near(c.x, c.y, c.z) && __float_as_int(c.w) != mi;
+                mask |= (uint64_t)take << jj;
+            }
+            const int jmax = na - J * CLU;  // candidates that exist
+            if (jmax < CLU) mask &= (1ull << jmax) - 1ull;
+            while (mask) {
+                const int jj = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                push(J * CLU + jj);
             }
         }
     }
-    if (kv) cnt[(size_t)b * na + k] = min(n, cap);
+    if (kv) {
+        if ((n & (VLS - 1)) && n < cap) out[(size_t)(n / VLS) * na] = row;
+        cnt[(size_t)b * na + k] = min(n, cap);
+    }
     if (ovf) ovfl[0] = 1;
-}
-
-// current coordinates in sorted order: (x, y, z, q) in FP64 and wrapped fractional FP32
-__global__ void k_vl_gather(const double* __restrict__ site, const int* __restrict__ perm,
-                            const double* __restrict__ sq, int na, int nb, double i0, double i1, double i2,
-                            double4* __restrict__ srt, float4* __restrict__ srf, unsigned long long* __restrict__ dmax) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) dmax[0] = 0ull;  // every gated kernel of this call has run
-    if (t >= (size_t)na * nb) return;
-    const size_t b = t / na;
-    const int a = perm[t];
-    const double* S = site + b * 3 * na;
-    const double x = S[a], y = S[na + a], z = S[2 * na + a], qq = sq[t];
-    srt[t] = make_double4(x, y, z, qq);
-    srf[t] = make_float4((float)wrap01(x * i0), (float)wrap01(y * i1), (float)wrap01(z * i2), (float)qq);
 }
 
 // one pair of the list walk: gradient and the 7 independent sums {v, xx, xy, xz, yy, yz, zz}
@@ -585,20 +582,22 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
     tv[6] += dz * fzz;
 }
 
-// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... with the next neighbour's
-// coordinates loaded while the current pair is computed; the site's gradient is summed over its
-// lanes by a fixed butterfly
+// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list with the next
+// neighbour's coordinates loaded while the current pair is computed; the site's gradient is summed
+// over its lanes by a fixed butterfly.  Clears dmax for the next call (every gated kernel has run).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
                                                   const int* __restrict__ perm, const int* __restrict__ cnt,
                                                   const int* __restrict__ nex, const int* __restrict__ nl, int na,
                                                   int cap, double L0, double L1, double L2, double i0, double i1,
                                                   double i2, double rc2, double alpha, double sfac,
-                                                  double* __restrict__ grad, double* __restrict__ parts) {
+                                                  double* __restrict__ grad, double* __restrict__ parts,
+                                                  unsigned long long* __restrict__ dmax) {
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, double4, float4>::type P4;
     const int b = blockIdx.y;
     const int k = (blockIdx.x * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
+    if (blockIdx.x == 0 && b == 0 && threadIdx.x == 0) dmax[0] = 0ull;
     double acc[7];
 #pragma unroll
     for (int c = 0; c < 7; c++) acc[c] = 0.0;
@@ -608,7 +607,8 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
         const int* L = nl + (size_t)b * cap * na + (size_t)k * VLS + sl;
         const size_t lstride = (size_t)na * VLS;
-        const int nn = cnt[bo + k], ne = nex[bo + k];
+        const int ntot = cnt[bo + k], ne = nex[bo + k];
+        const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
         const P4 xi = X[k];
         const T qi = (T)xi.w;
         const T Lt0 = (T)L0, Lt1 = (T)L1, Lt2 = (T)L2, it0 = (T)i0, it1 = (T)i1, it2 = (T)i2;
@@ -617,11 +617,11 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         T tg[3] = {0, 0, 0}, tv[7];
 #pragma unroll
         for (int c = 0; c < 7; c++) tv[c] = 0;
-        int n = sl, r = 0;
-        P4 xj = X[n < nn ? L[0] : k];
-        for (; n < nn; n += VLS, r++) {
+        int r = 0;
+        P4 xj = X[r < nn ? L[0] : k];
+        for (; r < nn; r++) {
             const P4 cur = xj;
-            xj = X[n + VLS < nn ? L[(size_t)(r + 1) * lstride] : k];
+            xj = X[r + 1 < nn ? L[(size_t)(r + 1) * lstride] : k];
             T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
             if (F64) {
                 // physical minimum image (the unit box when cubic)
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 ez = (ez - rint(ez)) * Lt2;
             }
             const T r2 = ex * ex + ey * ey + ez * ez;
-            if (r2 < trc2) vl_pair<T>(ex, ey, ez, r2, qi * (T)cur.w, n < ne, ta, tsf, tg, tv);
+            if (r2 < trc2) vl_pair<T>(ex, ey, ez, r2, qi * (T)cur.w, r * VLS + sl < ne, ta, tsf, tg, tv);
             if (!F64 && (r & 15) == 15) {
                 gx += (double)tg[0];
                 gy += (double)tg[1];
@@ -695,24 +695,34 @@ int vlist_cap(int na, Box sb, double rl) {
     return (cap + VLS - 1) / VLS * VLS;
 }
 
-// The list cut-off rl = rc + skin; the build rebuilds when a site has moved by more than skin/2.
+void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
+                     const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
+                     hipStream_t s) {
+    const size_t n = (size_t)na * nb;
+    const double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
+    k_sites_vl<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
+                                                           lat[2], (double2*)ph, q, v.rb, v.iperm, gather, sb.inv[0],
+                                                           sb.inv[1], sb.inv[2], v.srt, v.srf, v.dmax);
+}
+
+// The list cut-off rl = rc + skin; the build runs when a site has moved by more than skin/2 since the
+// last one (or force is set).  count: [nb][nbins + 1] ints, all zero.
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s) {
-    const size_t nsite = (size_t)na * nb;
-    const unsigned nblk = (unsigned)((nsite + 255) / 256);
-    k_vl_disp<<<nblk, 256, 0, s>>>(site, v.rb, nsite, (size_t)na, v.dmax);
     const VGate gate{v.dmax, 0.25 * skin * skin, force};
-    launch_cell_build(site, na, nb, g, sb, 1, cell_of, count, start, v.perm, sorted, v.iperm, gate, s);
+    int p = 1;
+    while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
+    const int nbins = p * p * p;
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
-    k_clu_pairs<<<dim3(nclu, nb), 64, 0, s>>>(sorted, na, nclu, sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
-                                             sb.inv[2], rl * rl * (1.0 + 1e-10), v.ccnt, v.clist, gate);
+    k_vl_sort<<<nb, 1024, 0, s>>>(site, na, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2],
+                                  rl * rl * (1.0 + 1e-10), nclu, cell_of, count, start, v.perm, v.iperm, sorted, v.bb,
+                                  v.ccnt, v.clist, gate);
     const float rl2f = (float)(rl * rl * (1.0 + pair_margin(sb, rl)));
     k_vl_build<<<dim3(nclu, nb), 64, 0, s>>>(sorted, v.perm, v.iperm, site, q, na, nclu, v.ccnt, v.clist,
                                             (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0], sb.inv[1],
-                                            sb.inv[2], rl2f, v.cap, v.cnt, v.nex, v.nl, v.rb, v.sq, v.gate + 1, gate);
-    k_vl_gather<<<nblk, 256, 0, s>>>(site, v.perm, v.sq, na, nb, sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf,
-                                     v.dmax);
+                                            sb.inv[2], rl2f, v.cap, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf,
+                                            v.gate + 1, gate);
 }
 
 template <typename T>
@@ -721,7 +731,8 @@ void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, doub
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
     k_ewald_vl<T><<<dim3((na * VLS + 255) / 256, nb), 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, v.cap,
                                                             sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
-                                                            sb.inv[2], rcut * rcut, alpha, sfac, grad, parts);
+                                                            sb.inv[2], rcut * rcut, alpha, sfac, grad, parts,
+                                                            v.dmax);
 }
 template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
 template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
@@ -730,7 +741,7 @@ template void launch_ewald_real<float>(const VList&, int, int, Box, double, doub
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
 //
 // k tile: ky, kz in [-16, 15] (index k + 16), kx in [0, kmax], kmax <= 15.
-// k_phase   : per (bead, atom) the base phases e^{i lat_d s_d}.
+// k_sites_vl: per (bead, atom) the base phases e^{i lat_d s_d}.
 // k_sk_mfma : S(kx,ky,kz) = sum_j Y_j(ky) [q_j X_j(kx) Z_j(kz)], a complex GEMM over atoms on FP64
 //             MFMA (v_mfma_f64_16x16x4f64).  A block owns KXB values of kx and one atom chunk; its
 //             four waves own the four 16x16 (ky,kz) quadrants.  NJ atoms at a time are staged in LDS
@@ -772,21 +783,6 @@ __device__ __forceinline__ double2 cpow(double2 z, int n) {
         n >>= 1;
     }
     return r;
-}
-
-__global__ void k_phase(const double* __restrict__ site, int na, int nb, double l0, double l1, double l2,
-                        double2* __restrict__ ph) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)na * nb) return;
-    const size_t b = i / na, a = i % na;
-    const double* X = site + b * 3 * na;
-    const double lat[3] = {l0, l1, l2};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double sn, cs;
-        sincos(lat[d] * X[d * na + a], &sn, &cs);
-        ph[(b * 3 + d) * na + a] = make_double2(cs, sn);
-    }
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -1191,10 +1187,6 @@ static KDev make_kdev(const KSet& ks) {
     return kd;
 }
 
-void launch_phase(const double* site, int na, int nb, const KSet& ks, double* ph, hipStream_t s) {
-    const size_t n = (size_t)na * nb;
-    k_phase<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(site, na, nb, ks.lat[0], ks.lat[1], ks.lat[2], (double2*)ph);
-}
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
                hipStream_t s) {
     const KDev kd = make_kdev(ks);
