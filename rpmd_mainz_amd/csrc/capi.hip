@@ -452,7 +452,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         {
             ProfScope ps(c, 3);
-            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, pk, s);
+            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, pk, mixed, s);
         }
         int nkslots = 0;
         {

@@ -752,6 +752,8 @@ template void launch_ewald_real<float>(const VList&, int, int, Box, double, doub
 // k_kforce  : per atom, U(ky) = sum_kz G w^kz with w = conj(exp(i lat_z z)) by Horner's rule and its
 //             kz-weighted derivative, two ky rows interleaved; kx split over grid.z into partial
 //             gradients summed (in fixed order) by the chain-rule kernel.
+// Mixed mode: k_sk_h16 and k_kforce_h16 take the same two contractions as GEMMs on FP16 MFMA with
+// split operands (below); k_sk_final is shared.
 // ------------------------------------------------------------------------------------------------
 #define KT 32
 #define KO 16
@@ -788,8 +790,9 @@ __device__ __forceinline__ double2 cpow(double2 z, int n) {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
-// 16x16x4 MFMA in the precision of T; D layouts (lane l, item v; lk = l >> 4, column l & 15):
-// f64 row lk + 4 v, f32 row 4 lk + v.  A and B: lane l gives A[l % 16][l / 16], B[l / 16][l % 16].
+// 16x16x4 FP64 MFMA; D layout (lane l, item v; lk = l >> 4, column l & 15): row lk + 4 v.  A and B:
+// lane l gives A[l % 16][l / 16], B[l / 16][l % 16].  (The FP32-accumulating 16x16 MFMAs below
+// have row 4 lk + v.)
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef dbl4 acc;
@@ -798,15 +801,8 @@ template <> struct Mfma<double> {
     }
     static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
-template <> struct Mfma<float> {
-    typedef flt4 acc;
-    static __device__ __forceinline__ acc op(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
-};
 
-// T: MFMA precision (FP64, or FP32 in the mixed mode; tables formed in FP64 and rounded once)
+// T: MFMA precision (FP64; the mixed mode uses k_sk_h16)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph, const double* __restrict__ q, int na,
                                                  int nchunk, KDev kd, double2* __restrict__ skp) {
@@ -894,8 +890,9 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
     }
 }
 
-// one block of 1024 threads per (kx, bead): thread = (iy, iz)
-__global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ skp, int nchunk, KDev kd,
+// one block of 1024 threads per (kx, bead): thread = (iy, iz); P: the partial sums' type
+template <typename P>
+__global__ __launch_bounds__(1024) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
                                                    const unsigned char* __restrict__ mask, double2* __restrict__ gk,
                                                    double* __restrict__ parts) {
     const int kx = blockIdx.x, b = blockIdx.y, kmax = kd.kmax;
@@ -903,12 +900,12 @@ __global__ __launch_bounds__(1024) void k_sk_final(const double2* __restrict__ s
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    const double2* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
+    const P* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
     double2 S = make_double2(0.0, 0.0);
     for (int ch = 0; ch < nchunk; ch++) {
-        double2 v = src[(size_t)ch * (KT * KT) + threadIdx.x];
-        S.x += v.x;
-        S.y += v.y;
+        const P v = src[(size_t)ch * (KT * KT) + threadIdx.x];
+        S.x += (double)v.x;
+        S.y += (double)v.y;
     }
     double2 G = make_double2(0.0, 0.0);
     if (mask[(kx * KT + iy) * KT + iz]) {
@@ -1030,34 +1027,167 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
     o[2 * na + a] = gz;
 }
 
-// ---- mixed mode: the force pass as a GEMM on FP32 MFMA --------------------------------------------
-// For each kx, U = [G; kz G] conj(Z): A = the (ky, kz) tile of G = 2 w S (rows 32..63 weighted by
-// kz), B = conj(Z_j(kz)) for 64 atoms, so U(ky, j) = sum_kz G(ky, kz) e^{-i kz z_j} and its kz-weighted
-// twin; then per atom the sums over ky of Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the
-// twin) give the three gradient components, exactly the quantities the Horner pass forms.
-// One block per (64 atoms, bead) and all kx; wave w owns rows 16 w .. 16 w + 15 of U.
-#define KFA 64      // atoms per block
-#define KPAD 33     // padded LDS row (no bank conflicts on the 16-row MFMA operand reads)
-__global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict__ ph, const double* __restrict__ q,
-                                                       int na, KDev kd, const double2* __restrict__ gk,
-                                                       double* __restrict__ kpart) {
-    typedef Mfma<float> M;
-    const int a0 = blockIdx.x * KFA, b = blockIdx.y;
+// ---- mixed mode: the k-space GEMMs on FP16 MFMA with split operands -----------------------------
+// Each FP32 operand v is carried as two FP16 numbers, hi = fp16(v) and lo = fp16(v - hi), so that
+// hi + lo holds about 21 significant bits, and a product is taken as hi*hi + (hi*lo + lo*hi) with
+// the small terms in an accumulator of their own (the split-FP16 product of H. Ootomo and R. Yokota,
+// "Recovering single precision accuracy from Tensor Cores while surpassing the FP32 theoretical
+// peak performance", IJHPCA 36 (2022)).  The tables are scaled by powers of two that keep the lo
+// parts normal FP16 numbers; hi is rounded toward zero (v_cvt_pkrtz), so v - hi is exact in FP32.
+// On v_mfma_f32_16x16x32_f16 the three products cost about a fifth of one FP32 MFMA product.
+//
+// A complex contraction sum_j a_j b_j is one real GEMM over an interleaved K: slot 2j holds the real
+// and slot 2j + 1 the imaginary part, so one dword of the operand tables is the half2 (re, im) of one
+// entry.  With B = (b_r, b_i) per entry, Re = A_r.B with A_r = (a_r, -a_i) and Im = A_i.B with
+// A_i = (a_i, a_r), and A_i follows from A_r by swapping the halves of each dword and negating the
+// new low half.  The f16 MFMA pairs element e of lane group g of A with element e of lane group g of
+// B, so any assignment of K slots to lane elements that A and B share gives the same sum.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
+#define HS 36          // dwords per LDS operand row: 32 entries + 4 (16 rows x 16 B reads hit distinct banks)
+#define SKH_CHUNK 512  // atoms per block of the structure-factor pass
+#define KFH 64         // atoms per block of the force pass
+#define KPAD 33        // padded FP32 LDS row
+
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+    const hp2 h = __builtin_amdgcn_cvt_pkrtz(a, b);
+    const hp2 r = __builtin_amdgcn_cvt_pkrtz(a - (float)h.x, b - (float)h.y);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, r);
+}
+// (re, -im) -> (im, re) in every dword
+__device__ __forceinline__ uint4 conj_swap(uint4 v) {
+    return make_uint4(__builtin_amdgcn_alignbit(v.x, v.x, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.y, v.y, 16) ^ 0x8000u,
+                      __builtin_amdgcn_alignbit(v.z, v.z, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.w, v.w, 16) ^ 0x8000u);
+}
+__device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+// C_main += a_hi b_hi; C_corr += a_hi b_lo + a_lo b_hi
+__device__ __forceinline__ void mfma_split(uint4 ah, uint4 al, uint4 bh, uint4 bl, flt4& cm, flt4& cc) {
+    cm = mfma_h16(ah, bh, cm);
+    cc = mfma_h16(ah, bl, cc);
+    cc = mfma_h16(al, bh, cc);
+}
+
+// S(kx, ky, kz) = sum_j Y_j(ky) W_j(kz), W_j = q_j X_j^kx Z_j^kz.  A block owns KXB values of kx and
+// SKH_CHUNK atoms; its four waves own the four 16x16 (ky, kz) quadrants.  Per 32 atoms the tables
+// A = (Y_r, -Y_i) [ky][atom] and B = W [kz][atom], both x 2^10, are formed in FP64 by powers of the
+// base phases and split into LDS.  Partial sums per chunk in FP32.
+__global__ __launch_bounds__(256) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                                int nchunk, KDev kd, float2* __restrict__ skp) {
+    const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int qa = w >> 1, qc = w & 1;
     const int kmax = kd.kmax;
-    __shared__ float Gr[KT][KPAD], Gi[KT][KPAD];    // G(kx) tile [iy][iz]
-    __shared__ float Zr[KFA][KPAD], Zi[KFA][KPAD];  // conj(Z_j(kz)) [atom][iz]
-    __shared__ float Yr[KFA][KPAD], Yi[KFA][KPAD];  // Y_j(ky) [atom][iy]
-    __shared__ float Xr[KFA], Xi[KFA];              // q X_j^kx
-    __shared__ float red[4][KFA][2];                // per wave: (plain, ky-weighted) row sums
+    __shared__ uint32_t Ah[KT][HS], Al[KT][HS];
+    __shared__ uint32_t Bh[KXB][KT][HS], Bl[KXB][KT][HS];
+    flt4 srm[KXB], src[KXB], sim[KXB], sic[KXB];
+#pragma unroll
+    for (int g = 0; g < KXB; g++) srm[g] = src[g] = sim[g] = sic[g] = (flt4){0, 0, 0, 0};
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
     const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
-    // Y and conj(Z) tables: thread -> (atom t / 4, entries 8 (t % 4) .. + 7)
+    const int a0 = ch * SKH_CHUNK, a1 = min(na, a0 + SKH_CHUNK);
+    const int j = t >> 3, part = t & 7, k0 = 4 * part - KO;  // table entries of this thread
+    const int kx0 = kxg * KXB;
+    constexpr double SC = 1024.0;
+    for (int base = a0; base < a1; base += 32) {
+        {
+            const int a = base + j;
+            const bool in = a < a1;
+            double2 y1 = make_double2(1.0, 0.0), z1 = y1, x1 = y1;
+            double qq = 0.0;
+            if (in) {
+                y1 = PY[a];
+                z1 = PZ[a];
+                x1 = PX[a];
+                qq = q[a];
+            }
+            double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
+            double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
+            float zr[4], zi[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int k = k0 + m;
+                const bool ok = in && k >= -kmax && k <= kmax;
+                split_pair(ok ? (float)(SC * yk.x) : 0.0f, ok ? -(float)(SC * yk.y) : 0.0f, Ah[4 * part + m][j],
+                           Al[4 * part + m][j]);
+                zr[m] = ok ? (float)zk.x : 0.0f;
+                zi[m] = ok ? (float)zk.y : 0.0f;
+                yk = cmul(yk, y1);
+                zk = cmul(zk, z1);
+            }
+            double2 xk = cpow(x1, kx0);
+            xk = make_double2(SC * qq * xk.x, SC * qq * xk.y);
+#pragma unroll
+            for (int g = 0; g < KXB; g++) {
+                const bool okx = kx0 + g <= kmax;
+                const float xr = okx ? (float)xk.x : 0.0f, xi = okx ? (float)xk.y : 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    split_pair(xr * zr[m] - xi * zi[m], xr * zi[m] + xi * zr[m], Bh[g][4 * part + m][j],
+                               Bl[g][4 * part + m][j]);
+                xk = cmul(xk, x1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int c = 16 * ks + 4 * lk;
+            const uint4 ah = *(const uint4*)&Ah[16 * qa + li][c], al = *(const uint4*)&Al[16 * qa + li][c];
+            const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
+#pragma unroll
+            for (int g = 0; g < KXB; g++) {
+                const uint4 bh = *(const uint4*)&Bh[g][16 * qc + li][c], bl = *(const uint4*)&Bl[g][16 * qc + li][c];
+                mfma_split(ah, al, bh, bl, srm[g], src[g]);
+                mfma_split(a2h, a2l, bh, bl, sim[g], sic[g]);
+            }
+        }
+        __syncthreads();
+    }
+    const float us = (float)(1.0 / (SC * SC));
+#pragma unroll
+    for (int g = 0; g < KXB; g++) {
+        const int kx = kx0 + g;
+        if (kx > kmax) break;
+        float2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int iy = 16 * qa + 4 * lk + v, iz = 16 * qc + li;
+            o[iy * KT + iz] = make_float2((srm[g][v] + src[g][v]) * us, (sim[g][v] + sic[g][v]) * us);
+        }
+    }
+}
+
+// The force pass as a GEMM: for each kx, U = [G; kz G] conj(Z), A = the (ky, kz) tile of G = 2 w S
+// (rows 32..63 weighted by kz) and B = conj(Z_j(kz)) for 64 atoms, so U(ky, j) = sum_kz G(ky, kz)
+// e^{-i kz z_j} and its kz-weighted twin; per atom the sums over ky of Im(conj(q X^kx Y^ky) U) (and
+// ky-weighted, and with the twin) give the three gradient components, the quantities of the Horner
+// pass.  B is formed once (x 2^10); the G tile of each kx is scaled by a power of two that brings its
+// largest entry to [2^10, 2^11).  One block per (64 atoms, bead) and all kx; wave w owns rows
+// 16 w .. 16 w + 15 of [G; kz G].
+__global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
+                                                    int na, KDev kd, const double2* __restrict__ gk,
+                                                    double* __restrict__ kpart) {
+    const int a0 = blockIdx.x * KFH, b = blockIdx.y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int kmax = kd.kmax;
+    __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];       // conj(Z_j(kz)) x 2^10 [atom][kz]
+    __shared__ uint32_t Gh[2 * KT][HS], Gl[2 * KT][HS];  // (G_r, -G_i) x s [row][kz]
+    __shared__ float Yr[KFH][KPAD], Yi[KFH][KPAD];      // Y_j(ky) [atom][ky]
+    __shared__ float Xr[KFH], Xi[KFH];                  // q X_j^kx
+    __shared__ float red[4][KFH][2];                    // per wave: (plain, ky-weighted) row sums
+    __shared__ float gmx[4];
+    constexpr double SC = 1024.0;
+    const double2* PX = ph + ((size_t)b * 3 + 0) * na;
+    const double2* PY = ph + ((size_t)b * 3 + 1) * na;
+    const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
     {
+        // thread -> (atom t / 4, entries 8 (t % 4) .. + 7)
         const int j = t >> 2, part = t & 3, a = a0 + j;
         const int k0 = 8 * part - KO;
-        double2 y1 = make_double2(1.0, 0.0), z1 = make_double2(1.0, 0.0);
+        double2 y1 = make_double2(1.0, 0.0), z1 = y1;
         const bool in = a < na;
         if (in) {
             y1 = PY[a];
@@ -1071,14 +1201,14 @@ __global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict
             const bool ok = in && k >= -kmax && k <= kmax;
             Yr[j][8 * part + m] = ok ? (float)yk.x : 0.0f;
             Yi[j][8 * part + m] = ok ? (float)yk.y : 0.0f;
-            Zr[j][8 * part + m] = ok ? (float)zk.x : 0.0f;
-            Zi[j][8 * part + m] = ok ? -(float)zk.y : 0.0f;
+            split_pair(ok ? (float)(SC * zk.x) : 0.0f, ok ? -(float)(SC * zk.y) : 0.0f, Zh[j][8 * part + m],
+                       Zl[j][8 * part + m]);
             yk = cmul(yk, y1);
             zk = cmul(zk, z1);
         }
     }
     double2 xk = make_double2(0.0, 0.0), x1 = make_double2(1.0, 0.0);
-    if (t < KFA && a0 + t < na) {
+    if (t < KFH && a0 + t < na) {
         x1 = PX[a0 + t];
         xk = make_double2(q[a0 + t], 0.0);
     }
@@ -1086,57 +1216,65 @@ __global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict
     const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
     const int mb = w & 1, wz = w >> 1;  // row block of G, kz-weighted twin
     for (int kx = 0; kx <= kmax; kx++) {
-        __syncthreads();
-        {
-            const double2* Gx = G0 + (size_t)kx * KT * KT;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int idx = t + 256 * e, iy = idx >> 5, iz = idx & 31;
-                const double2 g = Gx[idx];
-                Gr[iy][iz] = (float)g.x;
-                Gi[iy][iz] = (float)g.y;
-            }
-            if (t < KFA) {
-                if (kx > 0) xk = cmul(xk, x1);
-                Xr[t] = (float)xk.x;
-                Xi[t] = (float)xk.y;
-            }
-        }
-        __syncthreads();
+        if (t < KFH && kx > 0) xk = cmul(xk, x1);
         if (kd.kyb[kx] < 0) continue;  // uniform: no member at this kx
-        typename M::acc ur[4], ui[4];
+        const double2* Gx = G0 + (size_t)kx * KT * KT;
+        double2 gv[4];
+        double m = 0.0;
 #pragma unroll
-        for (int nbk = 0; nbk < 4; nbk++) {
-            ur[nbk] = (typename M::acc){0, 0, 0, 0};
-            ui[nbk] = (typename M::acc){0, 0, 0, 0};
+        for (int e = 0; e < 4; e++) {
+            gv[e] = Gx[t + 256 * e];
+            m = fmax(m, fmax(fabs(gv[e].x), fabs(gv[e].y)));
         }
-        const int st0 = (KO - kd.kzb[kx]) >> 2, st1 = (KO + kd.kzb[kx]) >> 2;  // G is zero beyond
-#pragma unroll 2
-        for (int st = st0; st <= st1; st++) {
-            const int iz = 4 * st + lk;
-            const float wk = wz ? (float)(iz - KO) : 1.0f;
-            const float ar = wk * Gr[16 * mb + li][iz], ai = wk * Gi[16 * mb + li][iz];
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        __syncthreads();  // the previous kx is done with the G tile, X and red
+        if (l == 0) gmx[w] = (float)m;
+        if (t < KFH) {
+            Xr[t] = (float)xk.x;
+            Xi[t] = (float)xk.y;
+        }
+        __syncthreads();
+        const float gm = fmaxf(fmaxf(gmx[0], gmx[1]), fmaxf(gmx[2], gmx[3]));
+        int ex = 0;
+        frexpf(gm, &ex);
+        const float s = gm > 0.0f ? ldexpf(1.0f, 11 - ex) : 1.0f;  // largest |G| s in [2^10, 2^11)
 #pragma unroll
-            for (int nbk = 0; nbk < 4; nbk++) {
-                const float br = Zr[16 * nbk + li][iz], bi = Zi[16 * nbk + li][iz];
-                ur[nbk] = M::op(ar, br, ur[nbk]);
-                ur[nbk] = M::op(-ai, bi, ur[nbk]);
-                ui[nbk] = M::op(ar, bi, ui[nbk]);
-                ui[nbk] = M::op(ai, br, ui[nbk]);
+        for (int e = 0; e < 4; e++) {
+            const int idx = t + 256 * e, iy = idx >> 5, iz = idx & 31;
+            const float gr = s * (float)gv[e].x, gi = s * (float)gv[e].y, kz = (float)(iz - KO);
+            split_pair(gr, -gi, Gh[iy][iz], Gl[iy][iz]);
+            split_pair(kz * gr, -(kz * gi), Gh[KT + iy][iz], Gl[KT + iy][iz]);
+        }
+        __syncthreads();
+        flt4 urm[4], urc[4], uim[4], uic[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) urm[n] = urc[n] = uim[n] = uic[n] = (flt4){0, 0, 0, 0};
+        const int r0 = KT * wz + 16 * mb;
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int c = 16 * ks + 4 * lk;
+            const uint4 ah = *(const uint4*)&Gh[r0 + li][c], al = *(const uint4*)&Gl[r0 + li][c];
+            const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const uint4 bh = *(const uint4*)&Zh[16 * n + li][c], bl = *(const uint4*)&Zl[16 * n + li][c];
+                mfma_split(ah, al, bh, bl, urm[n], urc[n]);
+                mfma_split(a2h, a2l, bh, bl, uim[n], uic[n]);
             }
         }
-        // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 nbk + li
+        // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
 #pragma unroll
-        for (int nbk = 0; nbk < 4; nbk++) {
-            const int j = 16 * nbk + li;
+        for (int n = 0; n < 4; n++) {
+            const int j = 16 * n + li;
             const float xr = Xr[j], xi = Xi[j];
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int iy = 16 * mb + M::row(lk, v);
+                const int iy = 16 * mb + 4 * lk + v;
+                const float ur = urm[n][v] + urc[n][v], ui = uim[n][v] + uic[n][v];
                 const float yr = Yr[j][iy], yi = Yi[j][iy];
-                const float pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;   // q X^kx Y^ky
-                const float ipu = pr * ui[nbk][v] - pi * ur[nbk][v];          // Im(conj(p) U)
+                const float pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;  // q X^kx Y^ky
+                const float ipu = pr * ui - pi * ur;                         // Im(conj(p) U)
                 s0 += ipu;
                 s1 += (float)(iy - KO) * ipu;
             }
@@ -1150,16 +1288,17 @@ __global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict
             }
         }
         __syncthreads();
-        if (t < KFA) {
-            const double tx = (double)red[0][t][0] + (double)red[1][t][0];
-            const double ty = (double)red[0][t][1] + (double)red[1][t][1];
-            const double tz = (double)red[2][t][0] + (double)red[3][t][0];
+        if (t < KFH) {
+            const double us = 1.0 / ((double)s * SC);
+            const double tx = ((double)red[0][t][0] + (double)red[1][t][0]) * us;
+            const double ty = ((double)red[0][t][1] + (double)red[1][t][1]) * us;
+            const double tz = ((double)red[2][t][0] + (double)red[3][t][0]) * us;
             gx += kd.lat0 * kx * tx;
             gy += kd.lat1 * ty;
             gz += kd.lat2 * tz;
         }
     }
-    if (t < KFA && a0 + t < na) {
+    if (t < KFH && a0 + t < na) {
         double* o = kpart + (size_t)b * 3 * na;
         o[a0 + t] = gx;
         o[na + a0 + t] = gy;
@@ -1168,6 +1307,7 @@ __global__ __launch_bounds__(256) void k_kforce_mfma32(const double2* __restrict
 }
 
 int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
+static int kspace_nchunk_h16(int na) { return (na + SKH_CHUNK - 1) / SKH_CHUNK; }
 int kspace_ngroups(int kmax) { return (kmax + KXF) / KXF; }
 
 static KDev make_kdev(const KSet& ks) {
@@ -1190,26 +1330,34 @@ static KDev make_kdev(const KSet& ks) {
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
                hipStream_t s) {
     const KDev kd = make_kdev(ks);
+    if (fp32) {
+        const int nchunk = kspace_nchunk_h16(na);
+        k_sk_h16<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
+                                                                          (float2*)sk_part);
+        return;
+    }
     const int nchunk = kspace_nchunk(na);
-    const dim3 grid((ks.kmax + KXB) / KXB, nchunk, nb);
-    if (fp32)
-        k_sk_mfma<float><<<grid, 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd, (double2*)sk_part);
-    else
-        k_sk_mfma<double><<<grid, 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd, (double2*)sk_part);
+    k_sk_mfma<double><<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
+                                                                              (double2*)sk_part);
 }
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
-                     double* parts_k, hipStream_t s) {
+                     double* parts_k, int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
-    k_sk_final<<<dim3(ks.kmax + 1, nb), 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
-                                                      (double2*)gk, parts_k);
+    const dim3 grid(ks.kmax + 1, nb);
+    if (fp32)
+        k_sk_final<float2><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na), kd, dmask,
+                                                 (double2*)gk, parts_k);
+    else
+        k_sk_final<double2><<<grid, 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
+                                                  (double2*)gk, parts_k);
 }
 // the force pass; returns the number of partial-gradient slots written (kx groups, or 1)
 int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
                   int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     if (fp32) {
-        k_kforce_mfma32<<<dim3((na + KFA - 1) / KFA, nb), 256, 0, s>>>((const double2*)ph, q, na, kd,
-                                                                      (const double2*)gk, kpart);
+        k_kforce_h16<<<dim3((na + KFH - 1) / KFH, nb), 256, 0, s>>>((const double2*)ph, q, na, kd,
+                                                                   (const double2*)gk, kpart);
         return 1;
     }
     k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
