@@ -67,7 +67,7 @@ struct rpmd_ctx {
     hipStream_t st = nullptr;
     // force pipeline
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
-    double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr;
+    double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr, *gks = nullptr;
     double *ph = nullptr, *fpart = nullptr;
     // real-space Verlet-list caches (two bead sets: e.g. all beads and RP-contracted ones)
     struct VCache {
@@ -452,12 +452,12 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         }
         {
             ProfScope ps(c, 3);
-            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, pk, mixed, s);
+            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, c->gks, pk, mixed, s);
         }
         int nkslots = 0;
         {
             ProfScope ps(c, 2);
-            nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk,
+            nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->gks,
                                     c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
@@ -1116,6 +1116,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->ev, 50);
     e |= dalloc(&c->skp, (size_t)nbl * (RPMD_KMAX_DEV + 1) * nck * RPMD_KTILE * RPMD_KTILE * 2);
     e |= dalloc(&c->gk, (size_t)nbl * (RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE * 2);
+    e |= dalloc(&c->gks, (size_t)nbl * (RPMD_KMAX_DEV + 1));
     e |= dalloc(&c->ph, (size_t)6 * na * nbl);
     e |= dalloc(&c->fpart, (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
@@ -1178,7 +1179,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     free_vcache(c);
     void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
-                    c->skp, c->gk, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
+                    c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
                     c->step_ctr, c->ev_step};

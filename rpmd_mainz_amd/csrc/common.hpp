@@ -109,9 +109,9 @@ int kspace_ngroups(int kmax);
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
                hipStream_t s);
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
-                     double* parts_k, int fp32, hipStream_t s);
-int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
-                  int fp32, hipStream_t s);
+                     double* gks, double* parts_k, int fp32, hipStream_t s);
+int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk,
+                  const double* gks, double* kpart, int fp32, hipStream_t s);
 void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
                   hipStream_t s);
 template <typename T>

@@ -890,11 +890,36 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
     }
 }
 
-// one block of 1024 threads per (kx, bead): thread = (iy, iz); P: the partial sums' type
-template <typename P>
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
+#define HS 36          // dwords per LDS operand row: 32 entries + 4 (16 rows x 16 B reads hit distinct banks)
+#define SKH_CHUNK 512  // atoms per block of the structure-factor pass
+#define KFH 64         // atoms per block of the force pass
+#define KPAD 33        // padded FP32 LDS row
+
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+    const hp2 h = __builtin_amdgcn_cvt_pkrtz(a, b);
+    const hp2 r = __builtin_amdgcn_cvt_pkrtz(a - (float)h.x, b - (float)h.y);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, r);
+}
+// (re, -im) -> (im, re) in every dword
+__device__ __forceinline__ uint4 conj_swap(uint4 v) {
+    return make_uint4(__builtin_amdgcn_alignbit(v.x, v.x, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.y, v.y, 16) ^ 0x8000u,
+                      __builtin_amdgcn_alignbit(v.z, v.z, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.w, v.w, 16) ^ 0x8000u);
+}
+__device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+
+// one block of 1024 threads per (kx, bead): thread = (iy, iz); P: the partial sums' type.  SPLIT
+// (mixed mode): instead of G, the A operand of k_kforce_h16 -- (G_r, -G_i) s as split half2 rows,
+// rows 32..63 weighted by kz, hi then lo ([2][64][32] dwords per (bead, kx)) -- and 1 / (s 2^10) in
+// gks, where the power of two s brings the largest |component| of the tile to [2^10, 2^11)
+template <typename P, bool SPLIT>
 __global__ __launch_bounds__(1024) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
                                                    const unsigned char* __restrict__ mask, double2* __restrict__ gk,
-                                                   double* __restrict__ parts) {
+                                                   double* __restrict__ gks, double* __restrict__ parts) {
     const int kx = blockIdx.x, b = blockIdx.y, kmax = kd.kmax;
     const int iy = threadIdx.x >> 5, iz = threadIdx.x & 31;
     double acc[10];
@@ -928,7 +953,25 @@ __global__ __launch_bounds__(1024) void k_sk_final(const P* __restrict__ skp, in
             for (int r = 0; r < 3; r++) acc[1 + c * 3 + r] = et * (term * rk[r] * rk[c] - (r == c ? 1.0 : 0.0));
         G = make_double2(2.0 * w * S.x, 2.0 * w * S.y);
     }
-    gk[(((size_t)b * (kmax + 1) + kx) * KT + iy) * KT + iz] = G;
+    if (!SPLIT) {
+        gk[(((size_t)b * (kmax + 1) + kx) * KT + iy) * KT + iz] = G;
+    } else {
+        __shared__ double wmax[16];
+        double m = fmax(fabs(G.x), fabs(G.y));
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = wmax[0];
+        for (int i = 1; i < 16; i++) m = fmax(m, wmax[i]);
+        int ex = 0;
+        frexp(m, &ex);
+        const double sc = m > 0.0 ? ldexp(1.0, 11 - ex) : 1.0;
+        const float gr = (float)(sc * G.x), gi = (float)(sc * G.y), kz = (float)(iz - KO);
+        uint32_t* o = (uint32_t*)gk + ((size_t)b * (kmax + 1) + kx) * (4 * KT * KT);
+        split_pair(gr, -gi, o[iy * KT + iz], o[2 * KT * KT + iy * KT + iz]);
+        split_pair(kz * gr, -(kz * gi), o[(KT + iy) * KT + iz], o[2 * KT * KT + (KT + iy) * KT + iz]);
+        if (threadIdx.x == 0) gks[(size_t)b * (kmax + 1) + kx] = 1.0 / (sc * 1024.0);
+    }
     block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
 }
 
@@ -1029,12 +1072,14 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
 
 // ---- mixed mode: the k-space GEMMs on FP16 MFMA with split operands -----------------------------
 // Each FP32 operand v is carried as two FP16 numbers, hi = fp16(v) and lo = fp16(v - hi), so that
-// hi + lo holds about 21 significant bits, and a product is taken as hi*hi + (hi*lo + lo*hi) with
-// the small terms in an accumulator of their own (the split-FP16 product of H. Ootomo and R. Yokota,
-// "Recovering single precision accuracy from Tensor Cores while surpassing the FP32 theoretical
-// peak performance", IJHPCA 36 (2022)).  The tables are scaled by powers of two that keep the lo
-// parts normal FP16 numbers; hi is rounded toward zero (v_cvt_pkrtz), so v - hi is exact in FP32.
-// On v_mfma_f32_16x16x32_f16 the three products cost about a fifth of one FP32 MFMA product.
+// hi + lo holds about 21 significant bits, and a product is taken as hi*hi + hi*lo + lo*hi (the
+// split-FP16 product of H. Ootomo and R. Yokota, "Recovering single precision accuracy from Tensor
+// Cores while surpassing the FP32 theoretical peak performance", IJHPCA 36 (2022)).  The MFMA
+// accumulates in FP32 with rounding to nearest, so the three products share one accumulator (the
+// paper keeps the small ones apart because its accumulator truncates); the force error measured
+// against the FP64 path is the same either way.  The tables are scaled by powers of two that keep
+// the lo parts normal FP16 numbers; hi is rounded toward zero (v_cvt_pkrtz), so v - hi is exact in
+// FP32.  On v_mfma_f32_16x16x32_f16 the three products cost about a fifth of one FP32 MFMA product.
 //
 // A complex contraction sum_j a_j b_j is one real GEMM over an interleaved K: slot 2j holds the real
 // and slot 2j + 1 the imaginary part, so one dword of the operand tables is the half2 (re, im) of one
@@ -1042,93 +1087,108 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
 // A_i = (a_i, a_r), and A_i follows from A_r by swapping the halves of each dword and negating the
 // new low half.  The f16 MFMA pairs element e of lane group g of A with element e of lane group g of
 // B, so any assignment of K slots to lane elements that A and B share gives the same sum.
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
-#define HS 36          // dwords per LDS operand row: 32 entries + 4 (16 rows x 16 B reads hit distinct banks)
-#define SKH_CHUNK 512  // atoms per block of the structure-factor pass
-#define KFH 64         // atoms per block of the force pass
-#define KPAD 33        // padded FP32 LDS row
-
-__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
-    const hp2 h = __builtin_amdgcn_cvt_pkrtz(a, b);
-    const hp2 r = __builtin_amdgcn_cvt_pkrtz(a - (float)h.x, b - (float)h.y);
-    hi = __builtin_bit_cast(uint32_t, h);
-    lo = __builtin_bit_cast(uint32_t, r);
-}
-// (re, -im) -> (im, re) in every dword
-__device__ __forceinline__ uint4 conj_swap(uint4 v) {
-    return make_uint4(__builtin_amdgcn_alignbit(v.x, v.x, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.y, v.y, 16) ^ 0x8000u,
-                      __builtin_amdgcn_alignbit(v.z, v.z, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.w, v.w, 16) ^ 0x8000u);
-}
-__device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
-}
-// C_main += a_hi b_hi; C_corr += a_hi b_lo + a_lo b_hi
-__device__ __forceinline__ void mfma_split(uint4 ah, uint4 al, uint4 bh, uint4 bl, flt4& cm, flt4& cc) {
-    cm = mfma_h16(ah, bh, cm);
-    cc = mfma_h16(ah, bl, cc);
-    cc = mfma_h16(al, bh, cc);
+// z^e for |e| <= 16 with e a multiple of m (m = 4 or 8: the bits below m are zero), branch-free
+__device__ __forceinline__ double2 cpow_sel(double2 z, int e, int m) {
+    const double2 one = make_double2(1.0, 0.0);
+    const int n = abs(e);
+    if (e < 0) z = make_double2(z.x, -z.y);
+    double2 p = z;
+    for (int s = 1; s < m; s <<= 1) p = cmul(p, p);  // z^m
+    double2 r = one;
+    for (int s = m; s <= 16; s <<= 1) {
+        r = cmul(r, (n & s) ? p : one);
+        p = cmul(p, p);
+    }
+    return r;
 }
 
-// S(kx, ky, kz) = sum_j Y_j(ky) W_j(kz), W_j = q_j X_j^kx Z_j^kz.  A block owns KXB values of kx and
-// SKH_CHUNK atoms; its four waves own the four 16x16 (ky, kz) quadrants.  Per 32 atoms the tables
-// A = (Y_r, -Y_i) [ky][atom] and B = W [kz][atom], both x 2^10, are formed in FP64 by powers of the
-// base phases and split into LDS.  Partial sums per chunk in FP32.
-__global__ __launch_bounds__(256) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+// S(kx, ky, kz) = sum_j Y_j(ky) W_j(kz), W_j = q_j X_j^kx Z_j^kz.  A block of 512 threads owns KXB
+// values of kx and SKH_CHUNK atoms; wave w owns the (ky, kz) quadrant w & 3 for kx values 2 (w >> 2)
+// and 2 (w >> 2) + 1 of the block.  Per 32 atoms: (1) the A table (Y_r, -Y_i) x 2^10 [ky][atom] as
+// split half2, Z^kz [atom][kz] and q X^kx x 2^10 [kx][atom] in FP32, all from FP64 powers of the
+// base phases; (2) the B tables W [kz][atom] per kx, split; (3) the MFMAs.  Partial sums per chunk
+// in FP32.
+__global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
                                                 int nchunk, KDev kd, float2* __restrict__ skp) {
     const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-    const int qa = w >> 1, qc = w & 1;
+    const int qa = (w >> 1) & 1, qc = w & 1, gp = w >> 2;  // ky half, kz half, kx pair
     const int kmax = kd.kmax;
     __shared__ uint32_t Ah[KT][HS], Al[KT][HS];
     __shared__ uint32_t Bh[KXB][KT][HS], Bl[KXB][KT][HS];
-    flt4 srm[KXB], src[KXB], sim[KXB], sic[KXB];
+    __shared__ float2 Zs[32][KT + 1];
+    __shared__ float2 Xs[KXB][32];
+    flt4 sr[2], si[2];
 #pragma unroll
-    for (int g = 0; g < KXB; g++) srm[g] = src[g] = sim[g] = sic[g] = (flt4){0, 0, 0, 0};
+    for (int g = 0; g < 2; g++) sr[g] = si[g] = (flt4){0, 0, 0, 0};
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
     const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
     const int a0 = ch * SKH_CHUNK, a1 = min(na, a0 + SKH_CHUNK);
-    const int j = t >> 3, part = t & 7, k0 = 4 * part - KO;  // table entries of this thread
     const int kx0 = kxg * KXB;
     constexpr double SC = 1024.0;
     for (int base = a0; base < a1; base += 32) {
-        {
-            const int a = base + j;
+        if (t < 256) {
+            // Y: thread -> (atom t / 8, ky = 4 (t % 8) - 16 .. + 3)
+            const int j = t >> 3, k0 = 4 * (t & 7) - KO, a = base + j;
             const bool in = a < a1;
-            double2 y1 = make_double2(1.0, 0.0), z1 = y1, x1 = y1;
-            double qq = 0.0;
-            if (in) {
-                y1 = PY[a];
-                z1 = PZ[a];
-                x1 = PX[a];
-                qq = q[a];
-            }
-            double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
-            double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
-            float zr[4], zi[4];
+            const double2 y1 = in ? PY[a] : make_double2(1.0, 0.0);
+            double2 yk = cpow_sel(y1, k0, 4);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const int k = k0 + m;
                 const bool ok = in && k >= -kmax && k <= kmax;
-                split_pair(ok ? (float)(SC * yk.x) : 0.0f, ok ? -(float)(SC * yk.y) : 0.0f, Ah[4 * part + m][j],
-                           Al[4 * part + m][j]);
-                zr[m] = ok ? (float)zk.x : 0.0f;
-                zi[m] = ok ? (float)zk.y : 0.0f;
+                split_pair(ok ? (float)(SC * yk.x) : 0.0f, ok ? -(float)(SC * yk.y) : 0.0f, Ah[k + KO][j],
+                           Al[k + KO][j]);
                 yk = cmul(yk, y1);
+            }
+        } else if (t < 384) {
+            // Z: thread -> (atom (t - 256) / 4, kz = 8 (t % 4) - 16 .. + 7)
+            const int j = (t - 256) >> 2, k0 = 8 * (t & 3) - KO, a = base + j;
+            const bool in = a < a1;
+            const double2 z1 = in ? PZ[a] : make_double2(1.0, 0.0);
+            double2 zk = cpow_sel(z1, k0, 8);
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = k0 + m;
+                const bool ok = in && k >= -kmax && k <= kmax;
+                Zs[j][k + KO] = ok ? make_float2((float)zk.x, (float)zk.y) : make_float2(0.0f, 0.0f);
                 zk = cmul(zk, z1);
             }
-            double2 xk = cpow(x1, kx0);
-            xk = make_double2(SC * qq * xk.x, SC * qq * xk.y);
+        } else {
+            // q X^kx x 2^10: thread -> (atom (t - 384) / 4, kx0 + t % 4)
+            const int j = (t - 384) >> 2, g = t & 3, kx = kx0 + g, a = base + j;
+            double2 v = make_double2(0.0, 0.0);
+            if (a < a1 && kx <= kmax) {
+                const double2 x1 = PX[a];
+                double2 p = x1, r = make_double2(1.0, 0.0);
 #pragma unroll
-            for (int g = 0; g < KXB; g++) {
-                const bool okx = kx0 + g <= kmax;
-                const float xr = okx ? (float)xk.x : 0.0f, xi = okx ? (float)xk.y : 0.0f;
+                for (int s = 1; s <= 8; s <<= 1) {
+                    if (kx & s) r = cmul(r, p);
+                    p = cmul(p, p);
+                }
+                const double qs = SC * q[a];
+                v = make_double2(qs * r.x, qs * r.y);
+            }
+            Xs[g][j] = make_float2((float)v.x, (float)v.y);
+        }
+        __syncthreads();
+        {
+            // W: thread -> (kz = t % 32, atoms 4 ((t / 32) % 8) .. + 3, kx pair t / 256)
+            const int kz = t & 31, j0 = 4 * ((t >> 5) & 7), h = t >> 8;
+            float2 z[4];
 #pragma unroll
-                for (int m = 0; m < 4; m++)
-                    split_pair(xr * zr[m] - xi * zi[m], xr * zi[m] + xi * zr[m], Bh[g][4 * part + m][j],
-                               Bl[g][4 * part + m][j]);
-                xk = cmul(xk, x1);
+            for (int i = 0; i < 4; i++) z[i] = Zs[j0 + i][kz];
+#pragma unroll
+            for (int g = 2 * h; g < 2 * h + 2; g++) {
+                uint32_t hi[4], lo[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float2 x = Xs[g][j0 + i];
+                    split_pair(x.x * z[i].x - x.y * z[i].y, x.x * z[i].y + x.y * z[i].x, hi[i], lo[i]);
+                }
+                *(uint4*)&Bh[g][kz][j0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+                *(uint4*)&Bl[g][kz][j0] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
             }
         }
         __syncthreads();
@@ -1138,47 +1198,51 @@ __global__ __launch_bounds__(256) void k_sk_h16(const double2* __restrict__ ph, 
             const uint4 ah = *(const uint4*)&Ah[16 * qa + li][c], al = *(const uint4*)&Al[16 * qa + li][c];
             const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
 #pragma unroll
-            for (int g = 0; g < KXB; g++) {
-                const uint4 bh = *(const uint4*)&Bh[g][16 * qc + li][c], bl = *(const uint4*)&Bl[g][16 * qc + li][c];
-                mfma_split(ah, al, bh, bl, srm[g], src[g]);
-                mfma_split(a2h, a2l, bh, bl, sim[g], sic[g]);
+            for (int g = 0; g < 2; g++) {
+                const int gg = 2 * gp + g;
+                const uint4 bh = *(const uint4*)&Bh[gg][16 * qc + li][c], bl = *(const uint4*)&Bl[gg][16 * qc + li][c];
+                sr[g] = mfma_h16(ah, bh, sr[g]);
+                sr[g] = mfma_h16(ah, bl, sr[g]);
+                sr[g] = mfma_h16(al, bh, sr[g]);
+                si[g] = mfma_h16(a2h, bh, si[g]);
+                si[g] = mfma_h16(a2h, bl, si[g]);
+                si[g] = mfma_h16(a2l, bh, si[g]);
             }
         }
         __syncthreads();
     }
     const float us = (float)(1.0 / (SC * SC));
 #pragma unroll
-    for (int g = 0; g < KXB; g++) {
-        const int kx = kx0 + g;
+    for (int g = 0; g < 2; g++) {
+        const int kx = kx0 + 2 * gp + g;
         if (kx > kmax) break;
         float2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int iy = 16 * qa + 4 * lk + v, iz = 16 * qc + li;
-            o[iy * KT + iz] = make_float2((srm[g][v] + src[g][v]) * us, (sim[g][v] + sic[g][v]) * us);
+            o[iy * KT + iz] = make_float2(sr[g][v] * us, si[g][v] * us);
         }
     }
 }
 
 // The force pass as a GEMM: for each kx, U = [G; kz G] conj(Z), A = the (ky, kz) tile of G = 2 w S
-// (rows 32..63 weighted by kz) and B = conj(Z_j(kz)) for 64 atoms, so U(ky, j) = sum_kz G(ky, kz)
-// e^{-i kz z_j} and its kz-weighted twin; per atom the sums over ky of Im(conj(q X^kx Y^ky) U) (and
-// ky-weighted, and with the twin) give the three gradient components, the quantities of the Horner
-// pass.  B is formed once (x 2^10); the G tile of each kx is scaled by a power of two that brings its
-// largest entry to [2^10, 2^11).  One block per (64 atoms, bead) and all kx; wave w owns rows
+// (rows 32..63 weighted by kz; split and scaled by k_sk_final) and B = conj(Z_j(kz)) for 64 atoms, so
+// U(ky, j) = sum_kz G(ky, kz) e^{-i kz z_j} and its kz-weighted twin; per atom the sums over ky of
+// Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the twin) give the three gradient components,
+// the quantities of the Horner pass.  B is formed once (x 2^10); the A tile of kx + 1 is loaded into
+// registers while kx is computed.  One block per (64 atoms, bead) and all kx; wave w owns rows
 // 16 w .. 16 w + 15 of [G; kz G].
 __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
-                                                    int na, KDev kd, const double2* __restrict__ gk,
-                                                    double* __restrict__ kpart) {
+                                                    int na, KDev kd, const uint4* __restrict__ ga,
+                                                    const double* __restrict__ gks, double* __restrict__ kpart) {
     const int a0 = blockIdx.x * KFH, b = blockIdx.y;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int kmax = kd.kmax;
-    __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];       // conj(Z_j(kz)) x 2^10 [atom][kz]
-    __shared__ uint32_t Gh[2 * KT][HS], Gl[2 * KT][HS];  // (G_r, -G_i) x s [row][kz]
-    __shared__ float Yr[KFH][KPAD], Yi[KFH][KPAD];      // Y_j(ky) [atom][ky]
-    __shared__ float Xr[KFH], Xi[KFH];                  // q X_j^kx
-    __shared__ float red[4][KFH][2];                    // per wave: (plain, ky-weighted) row sums
-    __shared__ float gmx[4];
+    __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];          // conj(Z_j(kz)) x 2^10 [atom][kz]
+    __shared__ uint32_t Gt[2][2 * KT][HS];                 // A tile [hi/lo][row][kz]
+    __shared__ float Yr[KT][KFH + 4], Yi[KT][KFH + 4];     // Y_j(ky) [ky][atom]
+    __shared__ float Xr[KFH], Xi[KFH];                     // q X_j^kx
+    __shared__ float red[4][KFH][2];                       // per wave: (plain, ky-weighted) row sums
     constexpr double SC = 1024.0;
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
@@ -1199,8 +1263,8 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
         for (int m = 0; m < 8; m++) {
             const int k = k0 + m;
             const bool ok = in && k >= -kmax && k <= kmax;
-            Yr[j][8 * part + m] = ok ? (float)yk.x : 0.0f;
-            Yi[j][8 * part + m] = ok ? (float)yk.y : 0.0f;
+            Yr[8 * part + m][j] = ok ? (float)yk.x : 0.0f;
+            Yi[8 * part + m][j] = ok ? (float)yk.y : 0.0f;
             split_pair(ok ? (float)(SC * zk.x) : 0.0f, ok ? -(float)(SC * zk.y) : 0.0f, Zh[j][8 * part + m],
                        Zl[j][8 * part + m]);
             yk = cmul(yk, y1);
@@ -1213,53 +1277,55 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
         xk = make_double2(q[a0 + t], 0.0);
     }
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
+    // A tile of one kx: 1024 uint4, thread t takes t + 256 e -> (hi/lo, row, 4 dwords)
+    const uint4* GA = ga + (size_t)b * (kmax + 1) * (4 * KT * KT / 4);
+    // (four named registers, not an array: a private array may be moved to LDS by the compiler)
+    uint4 n0 = GA[t], n1 = GA[t + 256], n2 = GA[t + 512], n3 = GA[t + 768];
     const int mb = w & 1, wz = w >> 1;  // row block of G, kz-weighted twin
     for (int kx = 0; kx <= kmax; kx++) {
         if (t < KFH && kx > 0) xk = cmul(xk, x1);
-        if (kd.kyb[kx] < 0) continue;  // uniform: no member at this kx
-        const double2* Gx = G0 + (size_t)kx * KT * KT;
-        double2 gv[4];
-        double m = 0.0;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            gv[e] = Gx[t + 256 * e];
-            m = fmax(m, fmax(fabs(gv[e].x), fabs(gv[e].y)));
+        __syncthreads();  // the previous kx is done with the A tile, X and red
+        {
+            const int row = (t >> 3) & 63, c4 = 4 * (t & 7), h = t >> 9;  // t + 256 e: row + 32 e
+            *(uint4*)&Gt[h][row][c4] = n0;
+            *(uint4*)&Gt[h][row + 32][c4] = n1;
+            *(uint4*)&Gt[h + 1][row][c4] = n2;
+            *(uint4*)&Gt[h + 1][row + 32][c4] = n3;
         }
-        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-        __syncthreads();  // the previous kx is done with the G tile, X and red
-        if (l == 0) gmx[w] = (float)m;
         if (t < KFH) {
             Xr[t] = (float)xk.x;
             Xi[t] = (float)xk.y;
         }
         __syncthreads();
-        const float gm = fmaxf(fmaxf(gmx[0], gmx[1]), fmaxf(gmx[2], gmx[3]));
-        int ex = 0;
-        frexpf(gm, &ex);
-        const float s = gm > 0.0f ? ldexpf(1.0f, 11 - ex) : 1.0f;  // largest |G| s in [2^10, 2^11)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = t + 256 * e, iy = idx >> 5, iz = idx & 31;
-            const float gr = s * (float)gv[e].x, gi = s * (float)gv[e].y, kz = (float)(iz - KO);
-            split_pair(gr, -gi, Gh[iy][iz], Gl[iy][iz]);
-            split_pair(kz * gr, -(kz * gi), Gh[KT + iy][iz], Gl[KT + iy][iz]);
+        if (kx < kmax) {
+            const uint4* g1 = GA + (size_t)(kx + 1) * (KT * KT);
+            n0 = g1[t];
+            n1 = g1[t + 256];
+            n2 = g1[t + 512];
+            n3 = g1[t + 768];
         }
-        __syncthreads();
-        flt4 urm[4], urc[4], uim[4], uic[4];
-#pragma unroll
-        for (int n = 0; n < 4; n++) urm[n] = urc[n] = uim[n] = uic[n] = (flt4){0, 0, 0, 0};
+        if (kd.kyb[kx] < 0) continue;  // uniform: no member at this kx
+        flt4 ur[4], ui[4];
         const int r0 = KT * wz + 16 * mb;
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const int c = 16 * ks + 4 * lk;
-            const uint4 ah = *(const uint4*)&Gh[r0 + li][c], al = *(const uint4*)&Gl[r0 + li][c];
+            const uint4 ah = *(const uint4*)&Gt[0][r0 + li][c], al = *(const uint4*)&Gt[1][r0 + li][c];
             const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
 #pragma unroll
             for (int n = 0; n < 4; n++) {
                 const uint4 bh = *(const uint4*)&Zh[16 * n + li][c], bl = *(const uint4*)&Zl[16 * n + li][c];
-                mfma_split(ah, al, bh, bl, urm[n], urc[n]);
-                mfma_split(a2h, a2l, bh, bl, uim[n], uic[n]);
+                if (ks == 0) {
+                    ur[n] = mfma_h16(ah, bh, (flt4){0, 0, 0, 0});
+                    ui[n] = mfma_h16(a2h, bh, (flt4){0, 0, 0, 0});
+                } else {
+                    ur[n] = mfma_h16(ah, bh, ur[n]);
+                    ui[n] = mfma_h16(a2h, bh, ui[n]);
+                }
+                ur[n] = mfma_h16(ah, bl, ur[n]);
+                ur[n] = mfma_h16(al, bh, ur[n]);
+                ui[n] = mfma_h16(a2h, bl, ui[n]);
+                ui[n] = mfma_h16(a2l, bh, ui[n]);
             }
         }
         // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
@@ -1271,10 +1337,9 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int iy = 16 * mb + 4 * lk + v;
-                const float ur = urm[n][v] + urc[n][v], ui = uim[n][v] + uic[n][v];
-                const float yr = Yr[j][iy], yi = Yi[j][iy];
+                const float yr = Yr[iy][j], yi = Yi[iy][j];
                 const float pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;  // q X^kx Y^ky
-                const float ipu = pr * ui - pi * ur;                         // Im(conj(p) U)
+                const float ipu = pr * ui[n][v] - pi * ur[n][v];              // Im(conj(p) U)
                 s0 += ipu;
                 s1 += (float)(iy - KO) * ipu;
             }
@@ -1289,7 +1354,7 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
         }
         __syncthreads();
         if (t < KFH) {
-            const double us = 1.0 / ((double)s * SC);
+            const double us = gks[(size_t)b * (kmax + 1) + kx];
             const double tx = ((double)red[0][t][0] + (double)red[1][t][0]) * us;
             const double ty = ((double)red[0][t][1] + (double)red[1][t][1]) * us;
             const double tz = ((double)red[2][t][0] + (double)red[3][t][0]) * us;
@@ -1332,7 +1397,7 @@ void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks
     const KDev kd = make_kdev(ks);
     if (fp32) {
         const int nchunk = kspace_nchunk_h16(na);
-        k_sk_h16<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
+        k_sk_h16<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 512, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
                                                                           (float2*)sk_part);
         return;
     }
@@ -1341,23 +1406,23 @@ void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks
                                                                               (double2*)sk_part);
 }
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
-                     double* parts_k, int fp32, hipStream_t s) {
+                     double* gks, double* parts_k, int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     const dim3 grid(ks.kmax + 1, nb);
     if (fp32)
-        k_sk_final<float2><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na), kd, dmask,
-                                                 (double2*)gk, parts_k);
+        k_sk_final<float2, true><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na), kd, dmask,
+                                                       (double2*)gk, gks, parts_k);
     else
-        k_sk_final<double2><<<grid, 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
-                                                  (double2*)gk, parts_k);
+        k_sk_final<double2, false><<<grid, 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
+                                                         (double2*)gk, gks, parts_k);
 }
 // the force pass; returns the number of partial-gradient slots written (kx groups, or 1)
-int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk, double* kpart,
-                  int fp32, hipStream_t s) {
+int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk,
+                  const double* gks, double* kpart, int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     if (fp32) {
-        k_kforce_h16<<<dim3((na + KFH - 1) / KFH, nb), 256, 0, s>>>((const double2*)ph, q, na, kd,
-                                                                   (const double2*)gk, kpart);
+        k_kforce_h16<<<dim3((na + KFH - 1) / KFH, nb), 256, 0, s>>>((const double2*)ph, q, na, kd, (const uint4*)gk,
+                                                                   gks, kpart);
         return 1;
     }
     k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
