@@ -347,7 +347,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         int e = 0;
         e |= dalloc(&v.perm, ns);
         e |= dalloc(&v.cnt, ns);
-        e |= dalloc(&v.nl, ns * cap);
+        e |= dalloc(&v.nl, ns * (cap + VLS));  // + the padding row
         e |= dalloc(&v.rb, 3 * ns);
         e |= dalloc(&v.gate, 2);
         e |= dalloc(&v.dmax, 1);

@@ -79,7 +79,7 @@ struct VGate {
 struct VList {
     int* perm;       // [nb][na] sorted index -> site index (of the last build)
     int* cnt;        // [nb][na] list length of sorted site k
-    int* nl;         // [nb][cap / VLS][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
+    int* nl;         // [nb][cap / VLS + 1][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
     double* rb;      // [nb][3][na] site coordinates at the last build
     int* gate;       // [1] list overflow (sticky)
     unsigned long long* dmax;  // largest squared displacement since the build (bits), this call

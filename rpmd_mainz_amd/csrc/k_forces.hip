@@ -238,7 +238,7 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
     float dr = dr2 * rinv;
     float x = alpha * dr;
     float e = __expf(-x * x);
-    float t = __frcp_rn((float)AS_P + x) * (float)AS_P;
+    float t = __builtin_amdgcn_rcpf((float)AS_P + x) * (float)AS_P;  // v_rcp_f32 (1 ulp)
     float erfc_ = e * t * ((float)AS_A1 + t * ((float)AS_A2 + t * ((float)AS_A3 + t * ((float)AS_A4 + t * (float)AS_A5))));
     if (same) erfc_ = erfc_ - 1.0f;
     du = erfc_ * rinv;
@@ -451,7 +451,8 @@ __global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ sit
 // molecule (nex[] of them; the real-space term treats those pairs as excluded,
 // src/ewald_cubic.f90:134), then the others in cluster order.  Entry n of site k is
 // nl[((n/VLS)*na + k)*VLS + n%VLS]: the VLS lanes that walk a site's list read one 16-byte row, and
-// consecutive entries are usually neighbouring sites.  The test is FP32 on wrapped fractional
+// consecutive entries are usually neighbouring sites.  Row cap/VLS holds k itself four times (the
+// pair kernel reads it past a lane's end).  The test is FP32 on wrapped fractional
 // coordinates against rl^2 enlarged by a margin that covers the FP32 error, so the list is a superset
 // of the pairs within rl (a few pairs just beyond do no harm: the pair kernel tests the cut-off
 // exactly).  One wave per (cluster, bead): the candidates of each listed cluster are staged in LDS
@@ -478,7 +479,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
     int a = -3, n = 0;
     bool ovf = false;
     int4 row = make_int4(0, 0, 0, 0);
-    int4* out = (int4*)(nl + (size_t)b * cap * na) + k;
+    int4* out = (int4*)(nl + (size_t)b * (cap + VLS) * na) + k;  // per bead cap / VLS + 1 rows
     auto push = [&](int j) {
         const int f = n & (VLS - 1);
         row.x = f == 0 ? j : row.x;
@@ -556,6 +557,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sort
     }
     if (kv) {
         if ((n & (VLS - 1)) && n < cap) out[(size_t)(n / VLS) * na] = row;
+        out[(size_t)(cap / VLS) * na] = make_int4(k, k, k, k);  // padding row read past the end
         cnt[(size_t)b * na + k] = min(n, cap);
     }
     if (ovf) ovfl[0] = 1;
@@ -582,9 +584,12 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
     tv[6] += dz * fzz;
 }
 
-// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list with the next
-// neighbour's coordinates loaded while the current pair is computed; the site's gradient is summed
-// over its lanes by a fixed butterfly.  Clears dmax for the next call (every gated kernel has run).
+// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list.  The walk
+// takes the lane's entries in groups of four with two register sets: the coordinates of one group
+// and the list entries of the group after it are in flight while a group is computed (no copy of a
+// loaded register, which would wait for its load).  Entries past the lane's end read the site
+// itself and are masked.  The site's gradient is summed over its lanes by a fixed butterfly.
+// Clears dmax for the next call (every gated kernel has run).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
                                                   const int* __restrict__ perm, const int* __restrict__ cnt,
@@ -605,8 +610,8 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     if (k < na) {
         const size_t bo = (size_t)b * na;
         const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
-        const int* L = nl + (size_t)b * cap * na + (size_t)k * VLS + sl;
-        const size_t lstride = (size_t)na * VLS;
+        const int* L = nl + (size_t)b * (cap + VLS) * na + (size_t)k * VLS + sl;
+        const size_t ls = (size_t)na * VLS;
         const int ntot = cnt[bo + k], ne = nex[bo + k];
         const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
         const P4 xi = X[k];
@@ -617,11 +622,10 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         T tg[3] = {0, 0, 0}, tv[7];
 #pragma unroll
         for (int c = 0; c < 7; c++) tv[c] = 0;
-        int r = 0;
-        P4 xj = X[r < nn ? L[0] : k];
-        for (; r < nn; r++) {
-            const P4 cur = xj;
-            xj = X[r + 1 < nn ? L[(size_t)(r + 1) * lstride] : k];
+        // past the lane's end: row cap / VLS, which holds the site itself (no branch around the load)
+        const int pad = cap / VLS;
+        auto ent = [&](int r) { return L[(size_t)(r < nn ? r : pad) * ls]; };
+        auto pair = [&](const P4& cur, int r) {
             T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
             if (F64) {
                 // physical minimum image (the unit box when cubic)
@@ -635,24 +639,51 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 ez = (ez - rint(ez)) * Lt2;
             }
             const T r2 = ex * ex + ey * ey + ez * ez;
-            if (r2 < trc2) vl_pair<T>(ex, ey, ez, r2, qi * (T)cur.w, r * VLS + sl < ne, ta, tsf, tg, tv);
-            if (!F64 && (r & 15) == 15) {
-                gx += (double)tg[0];
-                gy += (double)tg[1];
-                gz += (double)tg[2];
-                tg[0] = tg[1] = tg[2] = 0;
+            const bool in = r < nn && r2 < trc2;
+            vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, r * VLS + sl < ne, ta, tsf, tg, tv);
+        };
+        auto flush = [&]() {
+            gx += (double)tg[0];
+            gy += (double)tg[1];
+            gz += (double)tg[2];
+            tg[0] = tg[1] = tg[2] = 0;
 #pragma unroll
-                for (int c = 0; c < 7; c++) {
-                    acc[c] += (double)tv[c];
-                    tv[c] = 0;
-                }
+            for (int c = 0; c < 7; c++) {
+                acc[c] += (double)tv[c];
+                tv[c] = 0;
             }
+        };
+        int i0_ = ent(0), i1_ = ent(1), i2_ = ent(2), i3_ = ent(3);
+        P4 a0 = X[i0_], a1 = X[i1_], a2 = X[i2_], a3 = X[i3_];
+        int j0 = ent(4), j1 = ent(5), j2 = ent(6), j3 = ent(7);
+        for (int r = 0; r < nn; r += 8) {
+            const P4 b0 = X[j0], b1 = X[j1], b2 = X[j2], b3 = X[j3];
+            i0_ = ent(r + 8);
+            i1_ = ent(r + 9);
+            i2_ = ent(r + 10);
+            i3_ = ent(r + 11);
+            pair(a0, r);
+            pair(a1, r + 1);
+            pair(a2, r + 2);
+            pair(a3, r + 3);
+            if (r + 4 >= nn) break;
+            a0 = X[i0_];
+            a1 = X[i1_];
+            a2 = X[i2_];
+            a3 = X[i3_];
+            j0 = ent(r + 12);
+            j1 = ent(r + 13);
+            j2 = ent(r + 14);
+            j3 = ent(r + 15);
+            pair(b0, r + 4);
+            pair(b1, r + 5);
+            pair(b2, r + 6);
+            pair(b3, r + 7);
+            if (!F64 && (r & 15) == 8) flush();
         }
-        gx += (double)tg[0];
-        gy += (double)tg[1];
-        gz += (double)tg[2];
+        flush();
 #pragma unroll
-        for (int c = 0; c < 7; c++) acc[c] = 0.5 * (acc[c] + (double)tv[c]);
+        for (int c = 0; c < 7; c++) acc[c] *= 0.5;
     }
 #pragma unroll
     for (int off = 1; off < VLS; off <<= 1) {
