@@ -1260,20 +1260,21 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // (rows 32..63 weighted by kz; split and scaled by k_sk_final) and B = conj(Z_j(kz)) for 64 atoms, so
 // U(ky, j) = sum_kz G(ky, kz) e^{-i kz z_j} and its kz-weighted twin; per atom the sums over ky of
 // Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the twin) give the three gradient components,
-// the quantities of the Horner pass.  B is formed once (x 2^10); the A tile of kx + 1 is loaded into
-// registers while kx is computed.  One block per (64 atoms, bead) and all kx; wave w owns rows
-// 16 w .. 16 w + 15 of [G; kz G].
+// the quantities of the Horner pass.  One block per (64 atoms, bead); wave w owns rows
+// 16 w .. 16 w + 15 of [G; kz G], which it alone reads: its A operand comes from global memory
+// straight into registers, two kx ahead in two register sets, so the kx loop has no barrier.  B
+// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS.  Each lane sums its part of the
+// three components over kx in FP64; lanes and waves are combined once at the end, in fixed order.
 __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
                                                     int na, KDev kd, const uint4* __restrict__ ga,
                                                     const double* __restrict__ gks, double* __restrict__ kpart) {
     const int a0 = blockIdx.x * KFH, b = blockIdx.y;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int kmax = kd.kmax;
-    __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];          // conj(Z_j(kz)) x 2^10 [atom][kz]
-    __shared__ uint32_t Gt[2][2 * KT][HS];                 // A tile [hi/lo][row][kz]
-    __shared__ float Yr[KT][KFH + 4], Yi[KT][KFH + 4];     // Y_j(ky) [ky][atom]
-    __shared__ float Xr[KFH], Xi[KFH];                     // q X_j^kx
-    __shared__ float red[4][KFH][2];                       // per wave: (plain, ky-weighted) row sums
+    __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];   // conj(Z_j(kz)) x 2^10 [atom][kz]
+    __shared__ float2 Ys[KT][KFH + 2];              // Y_j(ky) [ky][atom]
+    __shared__ float2 Xs[KH + 1][KFH];              // q X_j^kx [kx][atom]
+    __shared__ double red[4][KFH][2];               // per wave: its two sums per atom
     constexpr double SC = 1024.0;
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
@@ -1282,66 +1283,43 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
         // thread -> (atom t / 4, entries 8 (t % 4) .. + 7)
         const int j = t >> 2, part = t & 3, a = a0 + j;
         const int k0 = 8 * part - KO;
-        double2 y1 = make_double2(1.0, 0.0), z1 = y1;
         const bool in = a < na;
-        if (in) {
-            y1 = PY[a];
-            z1 = PZ[a];
-        }
-        double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
-        double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
+        const double2 y1 = in ? PY[a] : make_double2(1.0, 0.0), z1 = in ? PZ[a] : make_double2(1.0, 0.0);
+        double2 yk = cpow_sel(y1, k0, 8), zk = cpow_sel(z1, k0, 8);
 #pragma unroll
         for (int m = 0; m < 8; m++) {
             const int k = k0 + m;
             const bool ok = in && k >= -kmax && k <= kmax;
-            Yr[8 * part + m][j] = ok ? (float)yk.x : 0.0f;
-            Yi[8 * part + m][j] = ok ? (float)yk.y : 0.0f;
-            split_pair(ok ? (float)(SC * zk.x) : 0.0f, ok ? -(float)(SC * zk.y) : 0.0f, Zh[j][8 * part + m],
-                       Zl[j][8 * part + m]);
+            Ys[k + KO][j] = ok ? make_float2((float)yk.x, (float)yk.y) : make_float2(0.0f, 0.0f);
+            split_pair(ok ? (float)(SC * zk.x) : 0.0f, ok ? -(float)(SC * zk.y) : 0.0f, Zh[j][k + KO], Zl[j][k + KO]);
             yk = cmul(yk, y1);
             zk = cmul(zk, z1);
         }
+        // q X^kx: thread -> (atom t / 4, kx = 4 (t % 4) .. + 3)
+        const int kx0 = 4 * part;
+        const double2 x1 = in ? PX[a] : make_double2(0.0, 0.0);
+        double2 xk = cpow_sel(x1, kx0, 4);
+        const double qq = in ? q[a] : 0.0;
+        xk = make_double2(qq * xk.x, qq * xk.y);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            Xs[kx0 + m][j] = make_float2((float)xk.x, (float)xk.y);
+            xk = cmul(xk, x1);
+        }
     }
-    double2 xk = make_double2(0.0, 0.0), x1 = make_double2(1.0, 0.0);
-    if (t < KFH && a0 + t < na) {
-        x1 = PX[a0 + t];
-        xk = make_double2(q[a0 + t], 0.0);
-    }
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    // A tile of one kx: 1024 uint4, thread t takes t + 256 e -> (hi/lo, row, 4 dwords)
-    const uint4* GA = ga + (size_t)b * (kmax + 1) * (4 * KT * KT / 4);
-    // (four named registers, not an array: a private array may be moved to LDS by the compiler)
-    uint4 n0 = GA[t], n1 = GA[t + 256], n2 = GA[t + 512], n3 = GA[t + 768];
+    __syncthreads();
     const int mb = w & 1, wz = w >> 1;  // row block of G, kz-weighted twin
-    for (int kx = 0; kx <= kmax; kx++) {
-        if (t < KFH && kx > 0) xk = cmul(xk, x1);
-        __syncthreads();  // the previous kx is done with the A tile, X and red
-        {
-            const int row = (t >> 3) & 63, c4 = 4 * (t & 7), h = t >> 9;  // t + 256 e: row + 32 e
-            *(uint4*)&Gt[h][row][c4] = n0;
-            *(uint4*)&Gt[h][row + 32][c4] = n1;
-            *(uint4*)&Gt[h + 1][row][c4] = n2;
-            *(uint4*)&Gt[h + 1][row + 32][c4] = n3;
-        }
-        if (t < KFH) {
-            Xr[t] = (float)xk.x;
-            Xi[t] = (float)xk.y;
-        }
-        __syncthreads();
-        if (kx < kmax) {
-            const uint4* g1 = GA + (size_t)(kx + 1) * (KT * KT);
-            n0 = g1[t];
-            n1 = g1[t + 256];
-            n2 = g1[t + 512];
-            n3 = g1[t + 768];
-        }
-        if (kd.kyb[kx] < 0) continue;  // uniform: no member at this kx
+    const int r0 = KT * wz + 16 * mb;
+    // A operand of lane (li, lk) for kx: rows r0 + li, dwords 16 ks + 4 lk .. + 3, hi and lo
+    const uint4* GA = ga + (size_t)b * (kmax + 1) * (KT * KT) + (r0 + li) * 8 + lk;
+    double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
+    auto body = [&](int kx, uint4 ah0, uint4 al0, uint4 ah1, uint4 al1) {
+        if (kd.kyb[kx] < 0) return;  // uniform: no member at this kx
         flt4 ur[4], ui[4];
-        const int r0 = KT * wz + 16 * mb;
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const int c = 16 * ks + 4 * lk;
-            const uint4 ah = *(const uint4*)&Gt[0][r0 + li][c], al = *(const uint4*)&Gt[1][r0 + li][c];
+            const uint4 ah = ks ? ah1 : ah0, al = ks ? al1 : al0;
             const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
 #pragma unroll
             for (int n = 0; n < 4; n++) {
@@ -1359,46 +1337,69 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
                 ui[n] = mfma_h16(a2l, bh, ui[n]);
             }
         }
+        const double us = gks[(size_t)b * (kmax + 1) + kx];
+        const double wa = wz ? us : us * kx;  // x: kx-weighted plain sum; z: the twin's plain sum
         // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
 #pragma unroll
         for (int n = 0; n < 4; n++) {
             const int j = 16 * n + li;
-            const float xr = Xr[j], xi = Xi[j];
+            const float2 x = Xs[kx][j];
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int iy = 16 * mb + 4 * lk + v;
-                const float yr = Yr[iy][j], yi = Yi[iy][j];
-                const float pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;  // q X^kx Y^ky
-                const float ipu = pr * ui[n][v] - pi * ur[n][v];              // Im(conj(p) U)
+                const float2 y = Ys[iy][j];
+                const float pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;  // q X^kx Y^ky
+                const float ipu = pr * ui[n][v] - pi * ur[n][v];                      // Im(conj(p) U)
                 s0 += ipu;
                 s1 += (float)(iy - KO) * ipu;
             }
-            s0 += __shfl_xor(s0, 16, 64);
-            s1 += __shfl_xor(s1, 16, 64);
-            s0 += __shfl_xor(s0, 32, 64);
-            s1 += __shfl_xor(s1, 32, 64);
-            if (lk == 0) {
-                red[w][j][0] = s0;
-                red[w][j][1] = s1;
-            }
+            sa[n] += wa * (double)s0;
+            sb[n] += us * (double)s1;
         }
-        __syncthreads();
-        if (t < KFH) {
-            const double us = gks[(size_t)b * (kmax + 1) + kx];
-            const double tx = ((double)red[0][t][0] + (double)red[1][t][0]) * us;
-            const double ty = ((double)red[0][t][1] + (double)red[1][t][1]) * us;
-            const double tz = ((double)red[2][t][0] + (double)red[3][t][0]) * us;
-            gx += kd.lat0 * kx * tx;
-            gy += kd.lat1 * ty;
-            gz += kd.lat2 * tz;
+    };
+    uint4 ah0 = GA[0], ah1 = GA[4], al0 = GA[512], al1 = GA[516];
+    uint4 bh0, bh1, bl0, bl1;
+    for (int kx = 0; kx <= kmax; kx += 2) {
+        if (kx + 1 <= kmax) {
+            const uint4* g = GA + (size_t)(kx + 1) * (KT * KT);
+            bh0 = g[0];
+            bh1 = g[4];
+            bl0 = g[512];
+            bl1 = g[516];
+        }
+        body(kx, ah0, al0, ah1, al1);
+        if (kx + 1 > kmax) break;
+        if (kx + 2 <= kmax) {
+            const uint4* g = GA + (size_t)(kx + 2) * (KT * KT);
+            ah0 = g[0];
+            ah1 = g[4];
+            al0 = g[512];
+            al1 = g[516];
+        }
+        body(kx + 1, bh0, bl0, bh1, bl1);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        sa[n] += __shfl_xor(sa[n], 16, 64);
+        sb[n] += __shfl_xor(sb[n], 16, 64);
+        sa[n] += __shfl_xor(sa[n], 32, 64);
+        sb[n] += __shfl_xor(sb[n], 32, 64);
+        if (lk == 0) {
+            red[w][16 * n + li][0] = sa[n];
+            red[w][16 * n + li][1] = sb[n];
         }
     }
+    __syncthreads();
     if (t < KFH && a0 + t < na) {
+        // waves 0, 1: (kx-weighted, ky-weighted) sums of the two ky halves; waves 2, 3: the twin
+        const double tx = red[0][t][0] + red[1][t][0];
+        const double ty = red[0][t][1] + red[1][t][1];
+        const double tz = red[2][t][0] + red[3][t][0];
         double* o = kpart + (size_t)b * 3 * na;
-        o[a0 + t] = gx;
-        o[na + a0 + t] = gy;
-        o[2 * na + a0 + t] = gz;
+        o[a0 + t] = kd.lat0 * tx;
+        o[na + a0 + t] = kd.lat1 * ty;
+        o[2 * na + a0 + t] = kd.lat2 * tz;
     }
 }
 
