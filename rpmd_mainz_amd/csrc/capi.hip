@@ -66,7 +66,7 @@ struct rpmd_ctx {
     int na = 0, nb_max = 0, prec = 0;
     hipStream_t st = nullptr;
     // force pipeline
-    double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr, *sorted = nullptr;
+    double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr, *gks = nullptr;
     double *ph = nullptr, *fpart = nullptr;
     // real-space Verlet-list caches (two bead sets: e.g. all beads and RP-contracted ones)
@@ -307,7 +307,8 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
 void free_vlist(VList& v) {
-    void* bufs[] = {v.perm, v.cnt, v.nl, v.rb, v.gate, v.dmax, v.srt, v.srf, v.iperm, v.nex, v.ccnt, v.clist, v.bb};
+    void* bufs[] = {v.perm, v.iperm, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf, v.cu, v.cs, v.bb, v.ccnt, v.clist,
+                    v.gate, v.dmax};
     for (void* b : bufs)
         if (b) hipFree(b);
     v = VList{};
@@ -334,7 +335,8 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
     x.last_use = ++c->vc_clock;
     const double key[5] = {box[0], box[1], box[2], rc, skin};
     *force = (x.nb != nb || memcmp(key, x.key, sizeof key) != 0) ? 1 : 0;
-    const int cap = vlist_cap(na, sb, rc + skin);
+    // capacity for the pairs within rc + skin plus twice a bead spread of up to two skins
+    const int cap = vlist_cap(na, sb, rc + 5.0 * skin);
     if (!x.alloc || x.v.cap < cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(c->st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return RPMD_ERR_ARG;
@@ -345,19 +347,21 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         const int nclu = ewald_nclu(na);
         const size_t ns = (size_t)na * nbl;
         int e = 0;
-        e |= dalloc(&v.perm, ns);
-        e |= dalloc(&v.cnt, ns);
-        e |= dalloc(&v.nl, ns * (cap + VLS));  // + the padding row
+        e |= dalloc(&v.perm, na);
+        e |= dalloc(&v.iperm, na);
+        e |= dalloc(&v.cnt, na);
+        e |= dalloc(&v.nex, na);
+        e |= dalloc(&v.nl, (size_t)na * (cap + VLS));  // + the padding row
         e |= dalloc(&v.rb, 3 * ns);
-        e |= dalloc(&v.gate, 2);
-        e |= dalloc(&v.dmax, 1);
         e |= dalloc(&v.srt, ns);
         e |= dalloc(&v.srf, ns);
-        e |= dalloc(&v.iperm, ns);
-        e |= dalloc(&v.nex, ns);
-        e |= dalloc(&v.ccnt, (size_t)nclu * nbl);
-        e |= dalloc(&v.bb, (size_t)6 * nclu * nbl);
-        e |= dalloc(&v.clist, (size_t)nclu * nclu * nbl);
+        e |= dalloc(&v.cu, (size_t)4 * na);
+        e |= dalloc(&v.cs, na);
+        e |= dalloc(&v.bb, (size_t)8 * nclu);
+        e |= dalloc(&v.ccnt, nclu);
+        e |= dalloc(&v.clist, (size_t)nclu * nclu);
+        e |= dalloc(&v.gate, 2);
+        e |= dalloc(&v.dmax, 1);
         if (e) {
             free_vlist(v);
             return RPMD_ERR_NOMEM;
@@ -428,8 +432,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, c->q, *vl, !force, sb, s);
         {
             ProfScope ps(c, 4);
-            launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start,
-                         c->sorted, s);
+            launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, s);
         }
         if (check_lists) {
             int h[2];
@@ -1108,7 +1111,6 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->grad, N);
     e |= dalloc(&c->site, N);
     e |= dalloc(&c->gsite, N);
-    e |= dalloc(&c->sorted, N);
     e |= dalloc(&c->q, na);
     e |= dalloc(&c->mass, na);
     e |= dalloc(&c->cen, (size_t)3 * na);
@@ -1178,7 +1180,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (c->graph) hipGraphDestroy(c->graph);
     free_vcache(c);
     void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
-                    c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->sorted, c->q, c->mass, c->cen, c->parts, c->ev,
+                    c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,

@@ -74,33 +74,37 @@ struct VGate {
     int force;
     __device__ __forceinline__ bool on() const { return force || __longlong_as_double((long long)*dmax) > lim2; }
 };
-// Verlet lists of the real-space Ewald sum (one cache per bead set); arrays sized for nb_max beads
+// Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
+// each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads
 #define VLS 4   // lanes per site in the Verlet-list walk (list rows of VLS entries)
 struct VList {
-    int* perm;       // [nb][na] sorted index -> site index (of the last build)
-    int* cnt;        // [nb][na] list length of sorted site k
-    int* nl;         // [nb][cap / VLS + 1][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
+    int* perm;       // [na] sorted index -> site index (of the last build)
+    int* iperm;      // [na] site index -> sorted index
+    int* cnt;        // [na] list length of sorted site k
+    int* nex;        // [na] leading list entries that are the site's own molecule
+    int* nl;         // [cap / VLS + 1][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
     double* rb;      // [nb][3][na] site coordinates at the last build
-    int* gate;       // [1] list overflow (sticky)
-    unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
-    int* iperm;      // [nb][na] site index -> sorted index (of the last build)
-    int* nex;        // [nb][na] leading list entries that are the site's own molecule
     double4* srt;    // [nb][na] current coordinates and charge in sorted order
     float4* srf;     // [nb][na] wrapped fractional FP32 coordinates and charge, sorted order
-    int* ccnt;       // [nb][nclu] cluster-pair list lengths
-    int* clist;      // [nb][nclu][nclu] cluster-pair lists
-    double* bb;      // [nb][nclu][6] cluster boxes (fractional centre, half-width)
+    double* cu;      // [4][na] centroids and spreads of the build, site order
+    float4* cs;      // [na] wrapped fractional centroid and spread, sorted order
+    double* bb;      // [nclu][8] cluster boxes (centre, half-width, largest spread)
+    int* ccnt;       // [nclu] cluster-pair list lengths
+    int* clist;      // [nclu][nclu] cluster-pair lists
+    int* gate;       // [2]: [1] list overflow (sticky)
+    unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
     int cap;
 };
+// site blocks of 256 / VLS sites per bead, padded to a multiple of 8 (the XCD mapping of k_ewald_vl)
+__host__ __device__ inline int ewald_real_parts(int na) { return ((na * VLS + 255) / 256 + 7) / 8 * 8; }
 int ewald_nclu(int na);
 int ewald_real_slots();
-int ewald_real_parts(int na);
 int vlist_cap(int na, Box sb, double rl);
 void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
                      const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
                      hipStream_t s);
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
-                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s);
+                  double skin, int force, int* cell_of, int* count, int* start, hipStream_t s);
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s);

@@ -253,8 +253,8 @@ __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 // a%3 == 0, as k_sites), scaled by sc; its phase factors e^{i lat_d x_d}; its squared displacement
 // since the list build, max-reduced into dmax as bit patterns (non-negative doubles order as their
 // bits; NaN reference positions of a list never built count as huge); and, when gather is set, the
-// site written at its sorted index iperm[] as (x, y, z, q) in FP64 and as wrapped fractional FP32
-// coordinates (inverse box side f).
+// site written at its sorted index iperm[] (the same for every bead) as (x, y, z, q) in FP64 and as
+// wrapped fractional FP32 coordinates (inverse box side f).
 __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
                                                   int nb, double alpha, double s0, double s1, double s2, double l0,
                                                   double l1, double l2, double2* __restrict__ ph,
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         m = dx * dx + dy * dy + dz * dz;
         if (!(m <= 1e300)) m = 1e300;
         if (gather) {
-            const size_t k = b * na + iperm[t];
+            const size_t k = b * na + iperm[a];
             const double qq = q[a];
             srt[k] = make_double4(x[0], x[1], x[2], qq);
             srf[k] = make_float4((float)wrap01(x[0] * f0), (float)wrap01(x[1] * f1), (float)wrap01(x[2] * f2),
@@ -305,35 +305,54 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
-// One block per bead, when the gate is on: counting sort of the sites into Morton cells, each cell
-// then ordered by site index, giving perm / iperm and the sorted positions; the bounding box of each
-// run of CLU sorted sites (a cluster) in wrapped fractional coordinates, bb[(b*nclu + c)*6] =
-// {centre, half-width}; and per cluster I the clusters J whose boxes come closer than rl (minimum
-// image of the centres; L = box side, 1 for the unit box), ascending.  count must be all zero on
-// entry and is left so.
-__global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ site, int na, CellGrid g, int nbins,
-                                                  double i0, double i1, double i2, double L0, double L1, double L2,
-                                                  double rl2, int nclu, int* __restrict__ cell_of,
+// One block, when the gate is on.  Per site a: the centroid c_a of its beads and its spread
+// s_a = max_b |x_ab - c_a|; a counting sort of the centroids into Morton cells, each cell then
+// ordered by site index (perm / iperm, shared by all beads); the sorted centroids as wrapped
+// fractional coordinates with the spreads (cs); the bounding box of each run of CLU sorted centroids
+// (a cluster) with its largest spread, bb[c*8] = {centre, half-width, spread}; and per cluster I the
+// clusters J whose boxes come closer than rl + spread_I + spread_J (minimum image of the centres;
+// L = box side, 1 for the unit box), ascending.  count must be all zero on entry and is left so.
+__global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ site, int na, int nb, CellGrid g,
+                                                  int nbins, double i0, double i1, double i2, double L0, double L1,
+                                                  double L2, double rl, int nclu, int* __restrict__ cell_of,
                                                   int* __restrict__ count, int* __restrict__ start,
                                                   int* __restrict__ perm, int* __restrict__ iperm,
-                                                  double* __restrict__ sorted, double* __restrict__ bb,
-                                                  int* __restrict__ ccnt, int* __restrict__ clist, VGate gate) {
+                                                  double* __restrict__ cu, float4* __restrict__ cs,
+                                                  double* __restrict__ bb, int* __restrict__ ccnt,
+                                                  int* __restrict__ clist, VGate gate) {
     if (!gate.on()) return;
-    const int b = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
     constexpr int NT = 1024;
-    const double* S = site + (size_t)b * 3 * na;
-    int* cof = cell_of + (size_t)b * na;
-    int* cn = count + (size_t)b * (nbins + 1);
-    int* st = start + (size_t)b * (nbins + 1);
-    int* P = perm + (size_t)b * na;
-    int* IP = iperm + (size_t)b * na;
-    double* O = sorted + (size_t)b * 3 * na;
+    int* cn = count;
+    int* st = start;
+    const double inv[3] = {i0, i1, i2};
+    // centroids and spreads (cu: [4][na] = c_x, c_y, c_z, s)
+    const double rnb = 1.0 / nb;
     for (int a = t; a < na; a += NT) {
-        const int cx = cell_coord(S[a], i0, g.nc[0]);
-        const int cy = cell_coord(S[na + a], i1, g.nc[1]);
-        const int cz = cell_coord(S[2 * na + a], i2, g.nc[2]);
+        double c[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < nb; b++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) c[d] += site[((size_t)b * 3 + d) * na + a];
+#pragma unroll
+        for (int d = 0; d < 3; d++) c[d] *= rnb;
+        double s2 = 0.0;
+        for (int b = 0; b < nb; b++) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double e = site[((size_t)b * 3 + d) * na + a] - c[d];
+                r2 += e * e;
+            }
+            s2 = fmax(s2, r2);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) cu[d * na + a] = c[d];
+        cu[3 * na + a] = sqrt(s2);
+        const int cx = cell_coord(c[0], i0, g.nc[0]);
+        const int cy = cell_coord(c[1], i1, g.nc[1]);
+        const int cz = cell_coord(c[2], i2, g.nc[2]);
         const int cid = (int)(part1by2(cx) | (part1by2(cy) << 1) | (part1by2(cz) << 2));
-        cof[a] = cid;
+        cell_of[a] = cid;
         atomicAdd(&cn[cid], 1);
     }
     __syncthreads();
@@ -364,71 +383,74 @@ __global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ sit
     if (t == 0) st[nbins] = carry;
     __syncthreads();
     for (int a = t; a < na; a += NT) {
-        const int cid = cof[a];
-        P[st[cid] + atomicAdd(&cn[cid], 1)] = a;
+        const int cid = cell_of[a];
+        perm[st[cid] + atomicAdd(&cn[cid], 1)] = a;
     }
     __syncthreads();
-    // order each cell by site index, then gather the sorted positions
+    // order each cell by site index; sorted wrapped centroids and spreads
     for (int c = t; c < nbins; c += NT) {
         cn[c] = 0;
         const int s0 = st[c], s1 = st[c + 1];
         for (int x = s0 + 1; x < s1; x++) {
-            const int v = P[x];
+            const int v = perm[x];
             int y = x - 1;
-            while (y >= s0 && P[y] > v) {
-                P[y + 1] = P[y];
+            while (y >= s0 && perm[y] > v) {
+                perm[y + 1] = perm[y];
                 y--;
             }
-            P[y + 1] = v;
+            perm[y + 1] = v;
         }
         for (int x = s0; x < s1; x++) {
-            const int a = P[x];
-            O[x] = S[a];
-            O[na + x] = S[na + a];
-            O[2 * na + x] = S[2 * na + a];
-            IP[a] = x;
+            const int a = perm[x];
+            cs[x] = make_float4((float)wrap01(cu[a] * i0), (float)wrap01(cu[na + a] * i1),
+                                (float)wrap01(cu[2 * na + a] * i2), (float)cu[3 * na + a]);
+            iperm[a] = x;
         }
     }
     __syncthreads();
-    // cluster boxes, wave w taking clusters w, w + NT/64, ...
+    // cluster boxes (FP64 from the FP32 wrapped centroids), wave w taking clusters w, w + NT/64, ...
     const int w = t >> 6, l = t & 63;
     constexpr int NW = NT / 64;
-    double* B = bb + (size_t)b * nclu * 6;
-    const double inv[3] = {i0, i1, i2};
     for (int c = w; c < nclu; c += NW) {
         const int a = c * CLU + l;
-        double lo[3], hi[3];
+        double lo[3], hi[3], sp = 0.0;
+        const float4 v = a < na ? cs[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const double f[3] = {v.x, v.y, v.z};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double f = a < na ? wrap01(O[d * na + a] * inv[d]) : 0.0;
-            lo[d] = a < na ? f : 2.0;
-            hi[d] = a < na ? f : -1.0;
+            lo[d] = a < na ? f[d] : 2.0;
+            hi[d] = a < na ? f[d] : -1.0;
         }
-        for (int off = 32; off > 0; off >>= 1)
+        sp = a < na ? (double)v.w : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
                 hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
             }
+            sp = fmax(sp, __shfl_xor(sp, off, 64));
+        }
         if (l == 0) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                B[c * 6 + d] = 0.5 * (lo[d] + hi[d]);
-                B[c * 6 + 3 + d] = 0.5 * (hi[d] - lo[d]);
+                bb[c * 8 + d] = 0.5 * (lo[d] + hi[d]);
+                bb[c * 8 + 3 + d] = 0.5 * (hi[d] - lo[d]);
             }
+            bb[c * 8 + 6] = sp;
         }
     }
     __syncthreads();
     const double L[3] = {L0, L1, L2};
+    const double lmax = fmax(L0, fmax(L1, L2));
     for (int I = w; I < nclu; I += NW) {
-        const double* bi = B + I * 6;
-        int* out = clist + ((size_t)b * nclu + I) * nclu;
+        const double* bi = bb + I * 8;
+        int* out = clist + (size_t)I * nclu;
         int n = 0;
         for (int J0 = 0; J0 < nclu; J0 += 64) {
             const int J = J0 + l;
             bool keep = false;
             if (J < nclu) {
-                const double* bj = B + J * 6;
+                const double* bj = bb + J * 8;
                 double r2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
@@ -437,128 +459,150 @@ __global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ sit
                     const double gap = fmax(0.0, fabs(dc) - bi[3 + d] - bj[3 + d]) * L[d];
                     r2 += gap * gap;
                 }
-                keep = r2 < rl2;
+                // reach with a margin for the FP32 centroids (1e-5 of the largest box side)
+                const double reach = rl + bi[6] + bj[6] + 1e-5 * lmax;
+                keep = r2 < reach * reach;
             }
             const uint64_t m = __ballot(keep);
             if (keep) out[n + __popcll(m & ((1ull << l) - 1))] = J;
             n += __popcll(m);
         }
-        if (l == 0) ccnt[b * nclu + I] = n;
+        if (l == 0) ccnt[I] = n;
     }
 }
 
-// Per sorted site its list of sorted neighbours within rc + skin: first the sites of its own
-// molecule (nex[] of them; the real-space term treats those pairs as excluded,
-// src/ewald_cubic.f90:134), then the others in cluster order.  Entry n of site k is
-// nl[((n/VLS)*na + k)*VLS + n%VLS]: the VLS lanes that walk a site's list read one 16-byte row, and
-// consecutive entries are usually neighbouring sites.  Row cap/VLS holds k itself four times (the
-// pair kernel reads it past a lane's end).  The test is FP32 on wrapped fractional
-// coordinates against rl^2 enlarged by a margin that covers the FP32 error, so the list is a superset
-// of the pairs within rl (a few pairs just beyond do no harm: the pair kernel tests the cut-off
-// exactly).  One wave per (cluster, bead): the candidates of each listed cluster are staged in LDS
-// (the next cluster's loaded meanwhile), each lane forms the 64-bit mask of its neighbours among them
-// and appends the set bits, storing whole rows.  The wave also records the reference positions and
-// writes the current coordinates in the new order.
-__global__ __launch_bounds__(64) void k_vl_build(const double* __restrict__ sorted, const int* __restrict__ perm,
-                                                 const int* __restrict__ iperm, const double* __restrict__ site,
-                                                 const double* __restrict__ q, int na, int nclu,
-                                                 const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                 float L0, float L1, float L2, double i0, double i1, double i2,
-                                                 float rl2f, int cap, int* __restrict__ cnt, int* __restrict__ nex,
-                                                 int* __restrict__ nl, double* __restrict__ rb,
-                                                 double4* __restrict__ srt, float4* __restrict__ srf,
-                                                 int* __restrict__ ovfl, VGate gate) {
+// Per sorted site k its list of sorted neighbours j with |c_k - c_j| < rl + s_k + s_j (centroids c,
+// spreads s): every bead pair then lies within rl of each other only if listed, because
+// |x_kb - x_jb| >= |c_k - c_j| - s_k - s_j.  First the other sites of its own molecule (nex[] of them;
+// the real-space term treats those pairs as excluded, src/ewald_cubic.f90:134), then the others in
+// ascending sorted order.  Entry n of site k is nl[((n/VLS)*na + k)*VLS + n%VLS]: the VLS lanes that
+// walk a site's list read one 16-byte row, and consecutive entries are usually neighbouring sites.
+// Row cap/VLS holds k itself four times (the pair kernel reads it past a lane's end).  The test is
+// FP32 on wrapped fractional centroids against the reach enlarged by a margin that covers the FP32
+// error, so the list is a superset (a few pairs beyond do no harm: the pair kernel tests the
+// cut-off exactly).  One block of 4 waves per cluster I, wave w taking its sites 16 w .. 16 w + 15 with
+// 4 lanes per site, lane s testing candidates s, s + 4, ... of each listed cluster (staged in LDS);
+// the four lanes merge their masks into the site's 64-bit mask and each stores its own hits at their
+// ascending positions.  The block also records every bead's reference positions of its sites and
+// writes their current coordinates in the new order, as k_sites_vl does between builds.
+__global__ __launch_bounds__(256) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
+                                                  const int* __restrict__ iperm, const double* __restrict__ site,
+                                                  const double* __restrict__ q, int na, int nb, int nclu,
+                                                  const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                  float L0, float L1, float L2, double f0, double f1, double f2,
+                                                  float rl, float mrg, int cap, int* __restrict__ cnt,
+                                                  int* __restrict__ nex, int* __restrict__ nl,
+                                                  double* __restrict__ rb, double4* __restrict__ srt,
+                                                  float4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
-    const int I = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
-    const int k = I * CLU + l;
+    const int I = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63, sl = l & 3;
+    const int k = I * CLU + 16 * w + (l >> 2);
     const bool kv = k < na;
-    const double* X = sorted + (size_t)b * 3 * na;
-    const int* Pb = perm + (size_t)b * na;
-    __shared__ float4 cj[CLU];  // candidate: wrapped fractional FP32 position, molecule (int bits)
-    float xi = 0.f, yi = 0.f, zi = 0.f;
+    // reference positions and sorted current coordinates of the cluster's sites, every bead
+    for (int e = t; e < CLU * nb; e += 256) {
+        const int kk = I * CLU + (e & (CLU - 1)), b = e / CLU;
+        if (kk < na) {
+            const int a = perm[kk];
+            const size_t o = (size_t)b * 3 * na + a;
+            const double x = site[o], y = site[o + na], z = site[o + 2 * na], qq = q[a];
+            rb[o] = x;
+            rb[o + na] = y;
+            rb[o + 2 * na] = z;
+            srt[(size_t)b * na + kk] = make_double4(x, y, z, qq);
+            srf[(size_t)b * na + kk] =
+                make_float4((float)wrap01(x * f0), (float)wrap01(y * f1), (float)wrap01(z * f2), (float)qq);
+        }
+    }
+    __shared__ float4 cj[CLU];  // candidate centroid (wrapped fractional) and spread
+    __shared__ int mj[CLU];     // candidate molecule
+    float4 ci = make_float4(0.f, 0.f, 0.f, 0.f);
     int a = -3, n = 0;
     bool ovf = false;
-    int4 row = make_int4(0, 0, 0, 0);
-    int4* out = (int4*)(nl + (size_t)b * (cap + VLS) * na) + k;  // per bead cap / VLS + 1 rows
-    auto push = [&](int j) {
-        const int f = n & (VLS - 1);
-        row.x = f == 0 ? j : row.x;
-        row.y = f == 1 ? j : row.y;
-        row.z = f == 2 ? j : row.z;
-        row.w = f == 3 ? j : row.w;
-        if (f == VLS - 1) {
-            if (n < cap) out[(size_t)(n / VLS) * na] = row;
-            else ovf = true;
-        }
-        n++;
+    auto put = [&](int p, int j) {
+        if (p < cap) nl[((size_t)(p / VLS) * na + k) * VLS + (p & (VLS - 1))] = j;
+        else ovf = true;
     };
-    auto near = [&](float xj, float yj, float zj) {
-        float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    auto reach2 = [&](float sj) {
+        const float r = rl + ci.w + sj;
+        return r * r * mrg;
+    };
+    auto d2 = [&](float4 c) {
+        float dx = ci.x - c.x, dy = ci.y - c.y, dz = ci.z - c.z;
         dx = (dx - rintf(dx)) * L0;
         dy = (dy - rintf(dy)) * L1;
         dz = (dz - rintf(dz)) * L2;
-        return dx * dx + dy * dy + dz * dz < rl2f;
+        return dx * dx + dy * dy + dz * dz;
     };
     if (kv) {
-        const double x = X[k], y = X[na + k], z = X[2 * na + k];
-        xi = (float)wrap01(x * i0);
-        yi = (float)wrap01(y * i1);
-        zi = (float)wrap01(z * i2);
-        a = Pb[k];
-        double* R = rb + (size_t)b * 3 * na;
-        R[a] = x;
-        R[na + a] = y;
-        R[2 * na + a] = z;
-        const double qq = q[a];
-        srt[(size_t)b * na + k] = make_double4(x, y, z, qq);
-        srf[(size_t)b * na + k] = make_float4(xi, yi, zi, (float)qq);
-        const int m0 = 3 * (a / 3);
-        for (int p = m0; p < m0 + 3; p++) {
-            if (p == a || p >= na) continue;
-            const int jk = iperm[(size_t)b * na + p];
-            if (near((float)wrap01(X[jk] * i0), (float)wrap01(X[na + jk] * i1), (float)wrap01(X[2 * na + jk] * i2)))
-                push(jk);
+        ci = cs[k];
+        a = perm[k];
+        if (sl == 0) {
+            const int m0 = 3 * (a / 3);
+            for (int p = m0; p < m0 + 3; p++) {
+                if (p == a || p >= na) continue;
+                const int jk = iperm[p];
+                const float4 c = cs[jk];
+                if (d2(c) < reach2(c.w)) put(n++, jk);
+            }
+            nex[k] = min(n, cap);
         }
-        nex[(size_t)b * na + k] = min(n, cap);
     }
-    const int nJ = ccnt[b * nclu + I];
-    const int* lst = clist + ((size_t)b * nclu + I) * nclu;
+    n = __shfl(n, l & ~3, 64);  // the site's count so far, from its lane 0
+    const int nJ = ccnt[I];
+    const int* lst = clist + (size_t)I * nclu;
     const int mi = a / 3;
-    auto stage = [&](int J) {
-        const int j = J * CLU + l;
-        if (j < na)
-            return make_float4((float)wrap01(X[j] * i0), (float)wrap01(X[na + j] * i1),
-                               (float)wrap01(X[2 * na + j] * i2), __int_as_float(Pb[j] / 3));
-        return make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    auto stage = [&](int J, float4& c, int& m) {
+        const int j = J * CLU + (t & (CLU - 1));
+        c = j < na ? cs[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        m = j < na ? perm[j] / 3 : -1;
     };
-    float4 nxt = nJ > 0 ? stage(lst[0]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 nc;
+    int nm_ = -1;
+    if (nJ > 0 && t < CLU) stage(lst[0], nc, nm_);
     for (int e = 0; e < nJ; e++) {
         const int J = lst[e];
         __syncthreads();
-        cj[l] = nxt;
+        if (t < CLU) {
+            cj[t] = nc;
+            mj[t] = nm_;
+        }
         __syncthreads();
-        if (e + 1 < nJ) nxt = stage(lst[e + 1]);
+        if (e + 1 < nJ && t < CLU) stage(lst[e + 1], nc, nm_);
+        uint32_t m16 = 0;
         if (kv) {
-            uint64_t mask = 0;
 #pragma unroll
-            for (int jj = 0; jj < CLU; jj++) {
+            for (int u = 0; u < CLU / 4; u++) {
+                const int jj = 4 * u + sl;
                 const float4 c = cj[jj];
-                const bool take = near(c.x, c.y, c.z) && __float_as_int(c.w) != mi;
-                mask |= (uint64_t)take << jj;
-            }
-            const int jmax = na - J * CLU;  // candidates that exist
-            if (jmax < CLU) mask &= (1ull << jmax) - 1ull;
-            while (mask) {
-                const int jj = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                push(J * CLU + jj);
+                const bool take = mj[jj] >= 0 && mj[jj] != mi && d2(c) < reach2(c.w);
+                m16 |= (uint32_t)take << u;
             }
         }
+        // bit u of lane s -> bit 4 u + s of the site's mask
+        uint64_t m = m16;
+        m = (m | (m << 24)) & 0x000000FF000000FFull;
+        m = (m | (m << 12)) & 0x000F000F000F000Full;
+        m = (m | (m << 6)) & 0x0303030303030303ull;
+        m = (m | (m << 3)) & 0x1111111111111111ull;
+        m <<= sl;
+        m |= __shfl_xor(m, 1, 64);
+        m |= __shfl_xor(m, 2, 64);
+        if (kv) {
+            uint64_t mine = m & (0x1111111111111111ull << sl);
+            while (mine) {
+                const int jj = __builtin_ctzll(mine);
+                mine &= mine - 1;
+                put(n + __popcll(m & ((1ull << jj) - 1ull)), J * CLU + jj);
+            }
+        }
+        n += __popcll(m);
     }
-    if (kv) {
-        if ((n & (VLS - 1)) && n < cap) out[(size_t)(n / VLS) * na] = row;
-        out[(size_t)(cap / VLS) * na] = make_int4(k, k, k, k);  // padding row read past the end
-        cnt[(size_t)b * na + k] = min(n, cap);
+    if (kv && sl == 0) {
+        nl[((size_t)(cap / VLS) * na + k) * VLS + 0] = k;
+        nl[((size_t)(cap / VLS) * na + k) * VLS + 1] = k;
+        nl[((size_t)(cap / VLS) * na + k) * VLS + 2] = k;
+        nl[((size_t)(cap / VLS) * na + k) * VLS + 3] = k;
+        cnt[k] = min(n, cap);
     }
     if (ovf) ovfl[0] = 1;
 }
@@ -584,25 +628,28 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
     tv[6] += dz * fzz;
 }
 
-// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list.  The walk
-// takes the lane's entries in groups of four with two register sets: the coordinates of one group
-// and the list entries of the group after it are in flight while a group is computed (no copy of a
-// loaded register, which would wait for its load).  Entries past the lane's end read the site
-// itself and are masked.  The site's gradient is summed over its lanes by a fixed butterfly.
-// Clears dmax for the next call (every gated kernel has run).
+// VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list (one list for
+// all beads).  The walk takes the lane's entries in groups of four with two register sets: the
+// coordinates of one group and the list entries of the group after it are in flight while a group is
+// computed (no copy of a loaded register, which would wait for its load).  Entries past the lane's
+// end read the site itself and are masked.  The site's gradient is summed over its lanes by a fixed
+// butterfly.  Block id i runs site block (i % 8) + 8 ((i / 8) / nb) of bead (i / 8) % nb, so the nb
+// blocks that read one stretch of the list run on one XCD (blocks go to the 8 XCDs round robin) and
+// share its L2.  Clears dmax for the next call (every gated kernel has run).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
                                                   const int* __restrict__ perm, const int* __restrict__ cnt,
                                                   const int* __restrict__ nex, const int* __restrict__ nl, int na,
-                                                  int cap, double L0, double L1, double L2, double i0, double i1,
-                                                  double i2, double rc2, double alpha, double sfac,
+                                                  int nb, int cap, double L0, double L1, double L2, double i0,
+                                                  double i1, double i2, double rc2, double alpha, double sfac,
                                                   double* __restrict__ grad, double* __restrict__ parts,
                                                   unsigned long long* __restrict__ dmax) {
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, double4, float4>::type P4;
-    const int b = blockIdx.y;
-    const int k = (blockIdx.x * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
-    if (blockIdx.x == 0 && b == 0 && threadIdx.x == 0) dmax[0] = 0ull;
+    const int id = blockIdx.x, m8 = id >> 3;
+    const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
+    const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
+    if (id == 0 && threadIdx.x == 0) dmax[0] = 0ull;
     double acc[7];
 #pragma unroll
     for (int c = 0; c < 7; c++) acc[c] = 0.0;
@@ -610,9 +657,9 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     if (k < na) {
         const size_t bo = (size_t)b * na;
         const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
-        const int* L = nl + (size_t)b * (cap + VLS) * na + (size_t)k * VLS + sl;
+        const int* L = nl + (size_t)k * VLS + sl;
         const size_t ls = (size_t)na * VLS;
-        const int ntot = cnt[bo + k], ne = nex[bo + k];
+        const int ntot = cnt[k], ne = nex[k];
         const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
         const P4 xi = X[k];
         const T qi = (T)xi.w;
@@ -692,7 +739,7 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         gz += __shfl_xor(gz, off, 64);
     }
     if (k < na && sl == 0) {
-        const int a = perm[(size_t)b * na + k];
+        const int a = perm[k];
         double* G = grad + (size_t)b * 3 * na;
         G[a] = gx;
         G[na + a] = gy;
@@ -700,12 +747,11 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     }
     // {v, vir(3,3) column-major} from the symmetric sums
     double w[10] = {acc[0], acc[1], acc[2], acc[3], acc[2], acc[4], acc[5], acc[3], acc[5], acc[6]};
-    block_reduce_store<10>(w, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+    block_reduce_store<10>(w, parts + ((size_t)b * ewald_real_parts(na) + sb) * 10);
 }
 
 int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
 int ewald_real_slots() { return 1; }
-int ewald_real_parts(int na) { return (na * VLS + 255) / 256; }
 
 // FP32 pair-test margin (relative, on r^2): wrapped fractional FP32 coordinates carry an absolute
 // error of about 2^-24 per component, so |dr| errs by less than 1e-6 of the box side; the margin
@@ -736,31 +782,31 @@ void launch_sites_vl(const double* pos, double* site, int na, int nb, double alp
                                                            sb.inv[1], sb.inv[2], v.srt, v.srf, v.dmax);
 }
 
-// The list cut-off rl = rc + skin; the build runs when a site has moved by more than skin/2 since the
-// last one (or force is set).  count: [nb][nbins + 1] ints, all zero.
+// The build runs when a site of any bead has moved by more than skin/2 since the last one (or force
+// is set); it lists, for all beads at once, the pairs within rl = rc + skin.  count: [nbins + 1] ints,
+// all zero.
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
-                  double skin, int force, int* cell_of, int* count, int* start, double* sorted, hipStream_t s) {
+                  double skin, int force, int* cell_of, int* count, int* start, hipStream_t s) {
     const VGate gate{v.dmax, 0.25 * skin * skin, force};
     int p = 1;
     while (p < g.nc[0] || p < g.nc[1] || p < g.nc[2]) p <<= 1;
     const int nbins = p * p * p;
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
-    k_vl_sort<<<nb, 1024, 0, s>>>(site, na, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2],
-                                  rl * rl * (1.0 + 1e-10), nclu, cell_of, count, start, v.perm, v.iperm, sorted, v.bb,
-                                  v.ccnt, v.clist, gate);
-    const float rl2f = (float)(rl * rl * (1.0 + pair_margin(sb, rl)));
-    k_vl_build<<<dim3(nclu, nb), 64, 0, s>>>(sorted, v.perm, v.iperm, site, q, na, nclu, v.ccnt, v.clist,
-                                            (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0], sb.inv[1],
-                                            sb.inv[2], rl2f, v.cap, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf,
-                                            v.gate + 1, gate);
+    k_vl_sort<<<1, 1024, 0, s>>>(site, na, nb, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2],
+                                 rl, nclu, cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist,
+                                 gate);
+    const float mrg = (float)(1.0 + pair_margin(sb, rl));
+    k_vl_build<<<nclu, 256, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist, (float)sb.L[0],
+                                    (float)sb.L[1], (float)sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], (float)rl, mrg,
+                                    v.cap, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf, v.gate + 1, gate);
 }
 
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
-    k_ewald_vl<T><<<dim3((na * VLS + 255) / 256, nb), 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, v.cap,
+    k_ewald_vl<T><<<ewald_real_parts(na) * nb, 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap,
                                                             sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
                                                             sb.inv[2], rcut * rcut, alpha, sfac, grad, parts,
                                                             v.dmax);
