@@ -254,103 +254,125 @@ __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 // since the list build, max-reduced into dmax as bit patterns (non-negative doubles order as their
 // bits; NaN reference positions of a list never built count as huge); and, when gather is set, the
 // site written at its sorted index iperm[] (the same for every bead) as (x, y, z, q) in FP64 and as
-// wrapped fractional FP32 coordinates (inverse box side f).
+// wrapped fractional FP32 coordinates (inverse box side f).  Per site also its bead centroid c and
+// spread s = max_b |x_b - c|, cu = [c_x, c_y, c_z, s][na], for the list build.  A block holds 32 sites
+// x 8 bead groups (thread: site t % 32, beads t / 32, t / 32 + 8, ...); the centroid sums of the
+// groups are added in fixed order.
 __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
                                                   int nb, double alpha, double s0, double s1, double s2, double l0,
                                                   double l1, double l2, double2* __restrict__ ph,
                                                   const double* __restrict__ q, const double* __restrict__ rb,
                                                   const int* __restrict__ iperm, int gather, double f0, double f1,
                                                   double f2, double4* __restrict__ srt, float4* __restrict__ srf,
-                                                  unsigned long long* __restrict__ dmax) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0.0;
-    if (t < (size_t)na * nb) {
-        const size_t b = t / na;
-        const int a = (int)(t % na), m0 = a - a % 3;
+                                                  double* __restrict__ cu, unsigned long long* __restrict__ dmax) {
+    const int il = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int a = blockIdx.x * 32 + il;
+    const bool av = a < na;
+    __shared__ double part[8][32][3];
+    __shared__ double cen[32][3];
+    __shared__ double red[8][32];
+    double m = 0.0, c[3] = {0.0, 0.0, 0.0};
+    if (av) {
+        const int m0 = a - a % 3;
         const double alpha2 = 0.5 * (1.0 - alpha);
         const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
-        double x[3];
+        const double qq = q[a];
+        const int ka = gather ? iperm[a] : 0;
+        for (int b = g; b < nb; b += 8) {
+            double x[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double* p = pos + (b * 3 + d) * na;
-            const size_t o = (b * 3 + d) * na + a;
-            if (a == m0) {
-                const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
-                x[d] = sc[d] * v;
-            } else {
-                x[d] = sc[d] * p[a];
+            for (int d = 0; d < 3; d++) {
+                const double* p = pos + ((size_t)b * 3 + d) * na;
+                const size_t o = ((size_t)b * 3 + d) * na + a;
+                if (a == m0) {
+                    const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
+                    x[d] = sc[d] * v;
+                } else {
+                    x[d] = sc[d] * p[a];
+                }
+                site[o] = x[d];
+                double sn, cs;
+                sincos(lat[d] * x[d], &sn, &cs);
+                ph[o] = make_double2(cs, sn);
+                c[d] += x[d];
             }
-            site[o] = x[d];
-            double sn, cs;
-            sincos(lat[d] * x[d], &sn, &cs);
-            ph[o] = make_double2(cs, sn);
-        }
-        const size_t o = b * 3 * na + a;
-        const double dx = x[0] - rb[o], dy = x[1] - rb[o + na], dz = x[2] - rb[o + 2 * na];
-        m = dx * dx + dy * dy + dz * dz;
-        if (!(m <= 1e300)) m = 1e300;
-        if (gather) {
-            const size_t k = b * na + iperm[a];
-            const double qq = q[a];
-            srt[k] = make_double4(x[0], x[1], x[2], qq);
-            srf[k] = make_float4((float)wrap01(x[0] * f0), (float)wrap01(x[1] * f1), (float)wrap01(x[2] * f2),
-                                 (float)qq);
+            const size_t o = (size_t)b * 3 * na + a;
+            const double dx = x[0] - rb[o], dy = x[1] - rb[o + na], dz = x[2] - rb[o + 2 * na];
+            double dd = dx * dx + dy * dy + dz * dz;
+            if (!(dd <= 1e300)) dd = 1e300;
+            m = fmax(m, dd);
+            if (gather) {
+                const size_t k = (size_t)b * na + ka;
+                srt[k] = make_double4(x[0], x[1], x[2], qq);
+                srf[k] = make_float4((float)wrap01(x[0] * f0), (float)wrap01(x[1] * f1), (float)wrap01(x[2] * f2),
+                                     (float)qq);
+            }
         }
     }
+#pragma unroll
+    for (int d = 0; d < 3; d++) part[g][il][d] = c[d];
+    __syncthreads();
+    if (g == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double v = part[0][il][d];
+            for (int h = 1; h < 8; h++) v += part[h][il][d];
+            cen[il][d] = v / nb;
+        }
+    }
+    __syncthreads();
+    double s2m = 0.0;
+    if (av)
+        for (int b = g; b < nb; b += 8) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double e = site[((size_t)b * 3 + d) * na + a] - cen[il][d];
+                r2 += e * e;
+            }
+            s2m = fmax(s2m, r2);
+        }
+    red[g][il] = s2m;
+    __syncthreads();
+    if (g == 0 && av) {
+        double v = red[0][il];
+        for (int h = 1; h < 8; h++) v = fmax(v, red[h][il]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) cu[(size_t)d * na + a] = cen[il][d];
+        cu[(size_t)3 * na + a] = sqrt(v);
+    }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0][0], red[0][1]),
+                                                                       fmax(red[0][2], red[0][3]))));
 }
 
-// One block, when the gate is on.  Per site a: the centroid c_a of its beads and its spread
-// s_a = max_b |x_ab - c_a|; a counting sort of the centroids into Morton cells, each cell then
-// ordered by site index (perm / iperm, shared by all beads); the sorted centroids as wrapped
-// fractional coordinates with the spreads (cs); the bounding box of each run of CLU sorted centroids
-// (a cluster) with its largest spread, bb[c*8] = {centre, half-width, spread}; and per cluster I the
-// clusters J whose boxes come closer than rl + spread_I + spread_J (minimum image of the centres;
-// L = box side, 1 for the unit box), ascending.  count must be all zero on entry and is left so.
-__global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ site, int na, int nb, CellGrid g,
-                                                  int nbins, double i0, double i1, double i2, double L0, double L1,
-                                                  double L2, double rl, int nclu, int* __restrict__ cell_of,
-                                                  int* __restrict__ count, int* __restrict__ start,
-                                                  int* __restrict__ perm, int* __restrict__ iperm,
-                                                  double* __restrict__ cu, float4* __restrict__ cs,
-                                                  double* __restrict__ bb, int* __restrict__ ccnt,
-                                                  int* __restrict__ clist, VGate gate) {
+// One block, when the gate is on: a counting sort of the site centroids (cu, from k_sites_vl) into
+// Morton cells, each cell then ordered by site index (perm / iperm, shared by all beads); the sorted
+// centroids as wrapped fractional coordinates with the spreads (cs); the bounding box of each run of
+// CLU sorted centroids (a cluster) with its largest spread, bb[c*8] = {centre, half-width, spread};
+// and per cluster I the clusters J whose boxes come closer than rl + spread_I + spread_J (minimum
+// image of the centres; L = box side, 1 for the unit box), ascending.  count must be all zero on entry
+// and is left so.
+__global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins, double i0, double i1, double i2,
+                                                  double L0, double L1, double L2, double rl, int nclu,
+                                                  int* __restrict__ cell_of, int* __restrict__ count,
+                                                  int* __restrict__ start, int* __restrict__ perm,
+                                                  int* __restrict__ iperm, const double* __restrict__ cu,
+                                                  float4* __restrict__ cs, double* __restrict__ bb,
+                                                  int* __restrict__ ccnt, int* __restrict__ clist, VGate gate) {
     if (!gate.on()) return;
     const int t = threadIdx.x;
     constexpr int NT = 1024;
     int* cn = count;
     int* st = start;
-    const double inv[3] = {i0, i1, i2};
-    // centroids and spreads (cu: [4][na] = c_x, c_y, c_z, s)
-    const double rnb = 1.0 / nb;
     for (int a = t; a < na; a += NT) {
-        double c[3] = {0.0, 0.0, 0.0};
-        for (int b = 0; b < nb; b++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) c[d] += site[((size_t)b * 3 + d) * na + a];
-#pragma unroll
-        for (int d = 0; d < 3; d++) c[d] *= rnb;
-        double s2 = 0.0;
-        for (int b = 0; b < nb; b++) {
-            double r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double e = site[((size_t)b * 3 + d) * na + a] - c[d];
-                r2 += e * e;
-            }
-            s2 = fmax(s2, r2);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d++) cu[d * na + a] = c[d];
-        cu[3 * na + a] = sqrt(s2);
-        const int cx = cell_coord(c[0], i0, g.nc[0]);
-        const int cy = cell_coord(c[1], i1, g.nc[1]);
-        const int cz = cell_coord(c[2], i2, g.nc[2]);
+        const int cx = cell_coord(cu[a], i0, g.nc[0]);
+        const int cy = cell_coord(cu[na + a], i1, g.nc[1]);
+        const int cz = cell_coord(cu[2 * na + a], i2, g.nc[2]);
         const int cid = (int)(part1by2(cx) | (part1by2(cy) << 1) | (part1by2(cz) << 2));
         cell_of[a] = cid;
         atomicAdd(&cn[cid], 1);
@@ -480,27 +502,28 @@ __global__ __launch_bounds__(1024) void k_vl_sort(const double* __restrict__ sit
 // Row cap/VLS holds k itself four times (the pair kernel reads it past a lane's end).  The test is
 // FP32 on wrapped fractional centroids against the reach enlarged by a margin that covers the FP32
 // error, so the list is a superset (a few pairs beyond do no harm: the pair kernel tests the
-// cut-off exactly).  One block of 4 waves per cluster I, wave w taking its sites 16 w .. 16 w + 15 with
-// 4 lanes per site, lane s testing candidates s, s + 4, ... of each listed cluster (staged in LDS);
-// the four lanes merge their masks into the site's 64-bit mask and each stores its own hits at their
-// ascending positions.  The block also records every bead's reference positions of its sites and
-// writes their current coordinates in the new order, as k_sites_vl does between builds.
-__global__ __launch_bounds__(256) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
-                                                  const int* __restrict__ iperm, const double* __restrict__ site,
-                                                  const double* __restrict__ q, int na, int nb, int nclu,
-                                                  const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                  float L0, float L1, float L2, double f0, double f1, double f2,
-                                                  float rl, float mrg, int cap, int* __restrict__ cnt,
-                                                  int* __restrict__ nex, int* __restrict__ nl,
-                                                  double* __restrict__ rb, double4* __restrict__ srt,
-                                                  float4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
+// cut-off exactly).  One wave per BLS sites of cluster I, BPS = 64 / BLS lanes per site: lane s of a
+// site tests candidates s, s + BPS, ... of each listed cluster (staged in LDS), the site's lanes merge
+// their hits into its 64-bit mask, and each stores its own hits at their ascending positions.  The
+// wave also records every bead's reference positions of its sites and writes their current
+// coordinates in the new order, as k_sites_vl does between builds.
+constexpr int BLS = 4, BPS = CLU / BLS;
+__global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
+                                                 const int* __restrict__ iperm, const double* __restrict__ site,
+                                                 const double* __restrict__ q, int na, int nb, int nclu,
+                                                 const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                 float L0, float L1, float L2, double f0, double f1, double f2,
+                                                 float rl, float mrg, int cap, int* __restrict__ cnt,
+                                                 int* __restrict__ nex, int* __restrict__ nl,
+                                                 double* __restrict__ rb, double4* __restrict__ srt,
+                                                 float4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
-    const int I = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63, sl = l & 3;
-    const int k = I * CLU + 16 * w + (l >> 2);
+    const int t = threadIdx.x, sl = t % BPS;
+    const int k0 = blockIdx.x * BLS, k = k0 + t / BPS, I = k0 / CLU;
     const bool kv = k < na;
-    // reference positions and sorted current coordinates of the cluster's sites, every bead
-    for (int e = t; e < CLU * nb; e += 256) {
-        const int kk = I * CLU + (e & (CLU - 1)), b = e / CLU;
+    // reference positions and sorted current coordinates of the wave's sites, every bead
+    for (int e = t; e < BLS * nb; e += 64) {
+        const int kk = k0 + e % BLS, b = e / BLS;
         if (kk < na) {
             const int a = perm[kk];
             const size_t o = (size_t)b * 3 * na + a;
@@ -547,55 +570,55 @@ __global__ __launch_bounds__(256) void k_vl_build(const float4* __restrict__ cs,
             nex[k] = min(n, cap);
         }
     }
-    n = __shfl(n, l & ~3, 64);  // the site's count so far, from its lane 0
+    n = __shfl(n, t - sl, 64);  // the site's count so far, from its lane 0
     const int nJ = ccnt[I];
     const int* lst = clist + (size_t)I * nclu;
     const int mi = a / 3;
     auto stage = [&](int J, float4& c, int& m) {
-        const int j = J * CLU + (t & (CLU - 1));
+        const int j = J * CLU + t;
         c = j < na ? cs[j] : make_float4(0.f, 0.f, 0.f, 0.f);
         m = j < na ? perm[j] / 3 : -1;
     };
-    float4 nc;
-    int nm_ = -1;
-    if (nJ > 0 && t < CLU) stage(lst[0], nc, nm_);
-    for (int e = 0; e < nJ; e++) {
-        const int J = lst[e];
-        __syncthreads();
-        if (t < CLU) {
+    // bits BPS u of a 64-bit word
+    uint64_t every = 0;
+#pragma unroll
+    for (int u = 0; u < CLU / BPS; u++) every |= 1ull << (BPS * u);
+    // the listed clusters 64 at a time, one per lane, so staging the next cluster waits on one load
+    for (int e0 = 0; e0 < nJ; e0 += 64) {
+        const int ne = min(64, nJ - e0);
+        const int jl = t < ne ? lst[e0 + t] : 0;
+        float4 nc;
+        int nm_;
+        stage(__shfl(jl, 0, 64), nc, nm_);
+        for (int e = 0; e < ne; e++) {
+            const int J = __shfl(jl, e, 64);
+            __syncthreads();
             cj[t] = nc;
             mj[t] = nm_;
-        }
-        __syncthreads();
-        if (e + 1 < nJ && t < CLU) stage(lst[e + 1], nc, nm_);
-        uint32_t m16 = 0;
-        if (kv) {
+            __syncthreads();
+            if (e + 1 < ne) stage(__shfl(jl, e + 1, 64), nc, nm_);
+            uint64_t m = 0;
+            if (kv) {
 #pragma unroll
-            for (int u = 0; u < CLU / 4; u++) {
-                const int jj = 4 * u + sl;
-                const float4 c = cj[jj];
-                const bool take = mj[jj] >= 0 && mj[jj] != mi && d2(c) < reach2(c.w);
-                m16 |= (uint32_t)take << u;
+                for (int u = 0; u < CLU / BPS; u++) {
+                    const int jj = BPS * u + sl;
+                    const float4 c = cj[jj];
+                    const bool take = mj[jj] >= 0 && mj[jj] != mi && d2(c) < reach2(c.w);
+                    m |= (uint64_t)take << jj;
+                }
             }
-        }
-        // bit u of lane s -> bit 4 u + s of the site's mask
-        uint64_t m = m16;
-        m = (m | (m << 24)) & 0x000000FF000000FFull;
-        m = (m | (m << 12)) & 0x000F000F000F000Full;
-        m = (m | (m << 6)) & 0x0303030303030303ull;
-        m = (m | (m << 3)) & 0x1111111111111111ull;
-        m <<= sl;
-        m |= __shfl_xor(m, 1, 64);
-        m |= __shfl_xor(m, 2, 64);
-        if (kv) {
-            uint64_t mine = m & (0x1111111111111111ull << sl);
-            while (mine) {
-                const int jj = __builtin_ctzll(mine);
-                mine &= mine - 1;
-                put(n + __popcll(m & ((1ull << jj) - 1ull)), J * CLU + jj);
+#pragma unroll
+            for (int off = 1; off < BPS; off <<= 1) m |= __shfl_xor(m, off, 64);
+            if (kv) {
+                uint64_t mine = m & (every << sl);
+                while (mine) {
+                    const int jj = __builtin_ctzll(mine);
+                    mine &= mine - 1;
+                    put(n + __popcll(m & ((1ull << jj) - 1ull)), J * CLU + jj);
+                }
             }
+            n += __popcll(m);
         }
-        n += __popcll(m);
     }
     if (kv && sl == 0) {
         nl[((size_t)(cap / VLS) * na + k) * VLS + 0] = k;
@@ -775,11 +798,10 @@ int vlist_cap(int na, Box sb, double rl) {
 void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
                      const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
                      hipStream_t s) {
-    const size_t n = (size_t)na * nb;
     const double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
-    k_sites_vl<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
-                                                           lat[2], (double2*)ph, q, v.rb, v.iperm, gather, sb.inv[0],
-                                                           sb.inv[1], sb.inv[2], v.srt, v.srf, v.dmax);
+    k_sites_vl<<<(unsigned)((na + 31) / 32), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
+                                                          lat[2], (double2*)ph, q, v.rb, v.iperm, gather, sb.inv[0],
+                                                          sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu, v.dmax);
 }
 
 // The build runs when a site of any bead has moved by more than skin/2 since the last one (or force
@@ -793,13 +815,13 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
     const int nbins = p * p * p;
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
-    k_vl_sort<<<1, 1024, 0, s>>>(site, na, nb, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2],
-                                 rl, nclu, cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist,
-                                 gate);
+    k_vl_sort<<<1, 1024, 0, s>>>(na, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2], rl, nclu,
+                                 cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist, gate);
     const float mrg = (float)(1.0 + pair_margin(sb, rl));
-    k_vl_build<<<nclu, 256, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist, (float)sb.L[0],
-                                    (float)sb.L[1], (float)sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], (float)rl, mrg,
-                                    v.cap, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf, v.gate + 1, gate);
+    k_vl_build<<<(na + BLS - 1) / BLS, 64, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist,
+                                                  (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0],
+                                                  sb.inv[1], sb.inv[2], (float)rl, mrg, v.cap, v.cnt, v.nex, v.nl,
+                                                  v.rb, v.srt, v.srf, v.gate + 1, gate);
 }
 
 template <typename T>
