@@ -80,6 +80,12 @@ struct rpmd_ctx {
     long long vc_clock = 0;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
+    // Verlet O-O centroid list of the device-resident calls
+    int *oo_nnb = nullptr, *oo_nbl = nullptr, *oo_ovf = nullptr;
+    double* oo_cb = nullptr;
+    unsigned long long* oo_dmax = nullptr;
+    double oo_key[5] = {0, 0, 0, 0, 0};
+    bool oo_built = false;
     unsigned char* kmask = nullptr;
     size_t off_real = 0, off_k = 0, off_oo = 0, off_tem = 0, off_intra = 0, nparts = 0;
     int ncell_cap = 0;
@@ -404,16 +410,29 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     int use_list = 0;
     if (need_oo && nb > 3) {
         // centroid O-O list (src/md_forces.f90:34-53)
-        launch_centroid(pos, c->cen, na, nb, s);
-        launch_zero_int(c->ovf, 2, s);
-        launch_nlist(c->cen, na, 3, bx, std::min(P.rcut, 0.5 * boxmin), c->nnb, c->nbl, c->ovf, s);
+        const double cut = std::min(P.rcut, 0.5 * boxmin);
         use_list = 1;
         if (check_lists) {
+            // the list of this call, and lj_driver's fallback when it is empty (src/ljones.f90:14-26)
+            launch_centroid(pos, c->cen, na, nb, s);
+            launch_zero_int(c->ovf, 2, s);
+            launch_nlist(c->cen, na, 3, bx, cut, c->nnb, c->nbl, c->ovf, s);
             int h[2];
             HIPCHK(hipMemcpyAsync(h, c->ovf, sizeof h, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (h[0] > 0) return RPMD_ERR_NEIGH_OVERFLOW;
-            if (h[1] == 0) use_list = 0;  // list(1) = 0: lj_driver falls back (src/ljones.f90:14-26)
+            if (h[1] == 0) use_list = 0;
+        } else {
+            // device-resident calls: a Verlet list of the centroids within cut + 1 A, rebuilt when an
+            // oxygen centroid has moved by about 0.5 A, from which the pairs within cut are taken
+            const double skin = 1.0 / 0.5291772108;
+            const double key[5] = {box[0], box[1], box[2], cut, skin};
+            const int force = (!c->oo_built || memcmp(key, c->oo_key, sizeof key) != 0) ? 1 : 0;
+            memcpy(c->oo_key, key, sizeof key);
+            c->oo_built = true;
+            launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, s);
+            launch_nlist_vl(c->cen, na, bx, cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
+                            c->nnb, c->nbl, s);
         }
     }
     if (iopt <= 2) {
@@ -918,6 +937,12 @@ int step_rig_energies(rpmd_ctx* c, double* v, double* vir_lfhf, double* vir_lf) 
 
 // a Verlet list that overflowed its capacity in any step since the lists were allocated
 int vlist_status(rpmd_ctx* c) {
+    if (c->oo_built) {
+        int h = 0;
+        HIPCHK(hipMemcpyAsync(&h, c->oo_ovf, sizeof h, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (h) return RPMD_ERR_NEIGH_OVERFLOW;
+    }
     for (auto& x : c->vc) {
         if (!x.alloc) continue;
         int h[2];
@@ -1130,6 +1155,14 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->nnb, na);
     e |= dalloc(&c->nbl, (size_t)na * RPMD_NCAP);
     e |= dalloc(&c->ovf, 2);
+    e |= dalloc(&c->oo_nnb, na / 3 + 1);
+    e |= dalloc(&c->oo_nbl, (size_t)(na / 3 + 1) * RPMD_NCAP);
+    e |= dalloc(&c->oo_ovf, 1);
+    e |= dalloc(&c->oo_cb, (size_t)3 * (na / 3 + 1));
+    e |= dalloc(&c->oo_dmax, 1);
+    if (!e && hipMemset(c->oo_ovf, 0, sizeof(int)) != hipSuccess) e = RPMD_ERR_HIP;
+    if (!e && hipMemset(c->oo_dmax, 0, sizeof(unsigned long long)) != hipSuccess) e = RPMD_ERR_HIP;
+    if (!e && hipMemset(c->oo_cb, 0xff, sizeof(double) * 3 * (na / 3 + 1)) != hipSuccess) e = RPMD_ERR_HIP;
     for (double** b : {&c->p, &c->r, &c->dvdr, &c->dvdr2, &c->dvdrl, &c->dvdre, &c->rst, &c->pnm, &c->rnm, &c->tmp,
                        &c->rb, &c->dvdrb, &c->tmp2})
         e |= dalloc(b, N);
@@ -1181,7 +1214,8 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     free_vcache(c);
     void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->q, c->mass, c->cen, c->parts, c->ev,
-                    c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf, c->p,
+                    c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf,
+                    c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
                     c->step_ctr, c->ev_step};

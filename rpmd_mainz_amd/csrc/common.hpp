@@ -63,6 +63,8 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
                        int acc, hipStream_t s);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
+void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const double* cb, unsigned long long* dmax,
+                        hipStream_t s);
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
 void launch_zero_int(int* y, size_t n, hipStream_t s);
@@ -118,6 +120,8 @@ int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet&
                   const double* gks, double* kpart, int fp32, hipStream_t s);
 void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
                   hipStream_t s);
+void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,
+                     double* cb, unsigned long long* dmax, int* overflow, int* nnb, int* nbl, hipStream_t s);
 template <typename T>
 void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
                const rpmd_params& p, double* grad, double* parts, hipStream_t s);

@@ -158,6 +158,38 @@ void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t
     size_t n = (size_t)3 * na;
     k_centroid<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, cen, na, nb);
 }
+// centroids as k_centroid (same sums), and for the oxygens (atoms 0, 3, 6, ...) the largest squared
+// displacement of one centroid coordinate from cb[] (its value at the O-O list build), max-reduced
+// into dmax as bit patterns (NaN reference positions count as huge); |dc|^2 <= 3 max_d dc_d^2
+__global__ __launch_bounds__(256) void k_centroid_vl(const double* __restrict__ pos, double* __restrict__ cen, int na,
+                                                     int nb, const double* __restrict__ cb,
+                                                     unsigned long long* __restrict__ dmax) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i < (size_t)3 * na) {
+        const size_t d = i / na, a = i % na;
+        double s = 0.0;
+        for (int b = 0; b < nb; b++) s = s + pos[((size_t)b * 3 + d) * na + a];
+        const double c = s / (double)nb;
+        cen[d * na + a] = c;
+        if (a % 3 == 0) {
+            const double e = c - cb[d * (na / 3) + a / 3];
+            m = e * e;
+            if (!(m <= 1e300)) m = 1e300;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+}
+void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const double* cb, unsigned long long* dmax,
+                        hipStream_t s) {
+    const size_t n = (size_t)3 * na;
+    k_centroid_vl<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, cen, na, nb, cb, dmax);
+}
 
 __global__ void k_axpy(double* y, const double* x, double a, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1581,6 +1613,92 @@ void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int
                   hipStream_t s) {
     const int n = na / njump;
     k_nlist<<<(n + 3) / 4, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
+}
+
+// Verlet O-O centroid list, when the gate is on: per oxygen the oxygens whose centroids lie within
+// rl = cut + skin (ascending, as k_nlist), and cb = the oxygen centroids of this build.  Valid while
+// no oxygen centroid has moved more than skin/2; k_nlist_pick then takes from it the pairs within cut
+// at the call, which is the list of k_nlist.  overflow[0] is sticky.
+__global__ __launch_bounds__(256) void k_nlist_vl(const double* __restrict__ cen, int na, Box bx, double rl2,
+                                                  int* __restrict__ nnb, int* __restrict__ nbl,
+                                                  double* __restrict__ cb, int* __restrict__ overflow, VGate gate) {
+    if (!gate.on()) return;
+    const int n = na / 3;
+    const int ii = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (ii >= n) return;
+    const int i = 3 * ii;
+    const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
+    if (lane == 0) {
+        cb[ii] = xi;
+        cb[n + ii] = yi;
+        cb[2 * n + ii] = zi;
+    }
+    int cnt = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int jj = base + lane;
+        bool in = false;
+        int j = 0;
+        if (jj < n) {
+            j = 3 * jj;
+            if (j != i) {
+                double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
+                mimage(dx, dy, dz, bx);
+                in = dx * dx + dy * dy + dz * dz < rl2;
+            }
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (slot < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + slot] = j;
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        if (cnt > RPMD_NCAP) overflow[0] = 1;
+        nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
+    }
+}
+// one wave per oxygen: the entries of its Verlet list whose centroids lie within cut, in their
+// (ascending) order and tested as k_nlist tests them, so the result is the list of k_nlist; clears
+// dmax for the next call
+__global__ __launch_bounds__(256) void k_nlist_pick(const double* __restrict__ cen, int na, Box bx, double cut2,
+                                                    const int* __restrict__ vnnb, const int* __restrict__ vnbl,
+                                                    int* __restrict__ nnb, int* __restrict__ nbl,
+                                                    unsigned long long* __restrict__ dmax) {
+    const int n = na / 3;
+    const int ii = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) dmax[0] = 0ull;
+    if (ii >= n) return;
+    const int i = 3 * ii;
+    const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
+    const int cv = vnnb[ii];
+    int cnt = 0;
+    for (int base = 0; base < cv; base += 64) {
+        const int e = base + lane;
+        bool in = false;
+        int j = 0;
+        if (e < cv) {
+            j = vnbl[(size_t)ii * RPMD_NCAP + e];
+            double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
+            mimage(dx, dy, dz, bx);
+            in = dx * dx + dy * dy + dz * dz < cut2;
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) nbl[(size_t)ii * RPMD_NCAP + cnt + __popcll(m & ((1ull << lane) - 1ull))] = j;
+        cnt += __popcll(m);
+    }
+    if (lane == 0) nnb[ii] = cnt;
+}
+// gate: an oxygen centroid coordinate moved by more than skin / (2 sqrt 3)
+void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,
+                     double* cb, unsigned long long* dmax, int* overflow, int* nnb, int* nbl, hipStream_t s) {
+    const VGate gate{dmax, 0.25 * skin * skin / 3.0, force};
+    const double rl = cut + skin;
+    const unsigned nblk = (unsigned)((na / 3 + 3) / 4);
+    k_nlist_vl<<<nblk, 256, 0, s>>>(cen, na, box, rl * rl, vnnb, vnbl, cb, overflow, gate);
+    k_nlist_pick<<<nblk, 256, 0, s>>>(cen, na, box, cut * cut, vnnb, vnbl, nnb, nbl, dmax);
 }
 
 // LJ (gam == 0) or Buckingham exp-6 (src/ljones.f90, src/buckingham.f90); OOL lanes per (bead, O),
