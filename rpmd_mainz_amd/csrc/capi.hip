@@ -796,12 +796,33 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
                        c->step_ctr, nz.stream_add++, c->gle_parts, c->gle_pcom, 1.0 / sqrt(nb * c->mtot), s);
         }
     };
+    const int nblk = pile_blocks(na);
+    const double inv_mtot = 1.0 / c->mtot;
+    if (P.therm == RPMD_THERM_PRO && mts == 1 && P.nbdf1 <= 0 && P.nbdf2 <= 0) {
+        // the same sequence with fewer launches: the opening kick is applied as p is loaded by the NM
+        // transform (p itself is recomputed from pnm below), each COM removal as pnm is loaded by the
+        // next kernel, p = B pnm is taken before the intramolecular forces (which do not read it), the
+        // two closing kicks are one launch, and the noise counter advances in it
+        launch_nm_kick2(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->pnm, c->r, c->rnm, nmm->F, nb, ne, s);
+        launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
+                    c->com_parts, s);
+        launch_free_ring(c->pnm, c->rnm, c->monod, na, nb, s, c->com_parts, nblk, inv_mtot, c->mass);
+        launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
+                    c->com_parts, s);
+        launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
+        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check));
+        std::swap(c->r, c->rst);
+        c->lf_merged = true;
+        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check));
+        launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
+        if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
+        if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
+        return 0;
+    }
     outer();
     launch_kick(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, N, s);
     launch_nm(c->p, c->pnm, nmm->F, nb, ne, s);
     launch_nm(c->r, c->rnm, nmm->F, nb, ne, s);
-    const int nblk = pile_blocks(na);
-    const double inv_mtot = 1.0 / c->mtot;
     auto pile = [&]() {
         if (P.therm == RPMD_THERM_PRO) {
             launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr,

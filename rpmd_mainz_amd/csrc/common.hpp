@@ -135,8 +135,14 @@ int intra_blocks(int na);
 
 // ---- k_step.hip -------------------------------------------------------------------------------
 void launch_nm(const double* in, double* out, const double* M, int n, size_t ne, hipStream_t s);
-void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s);
-void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s);
+void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s,
+                 uint64_t* ctr = nullptr);
+void launch_nm_kick2(const double* p, const double* g1, double a1, const double* g2, double a2, double* pnm,
+                     const double* r, double* rnm, const double* F, int n, size_t ne, hipStream_t s);
+void launch_nm_com2(const double* rnm, double* rst, const double* pnm, double* p, const double* com, int nblk,
+                    double inv_mtot, const double* mass, int na, const double* B, int n, size_t ne, hipStream_t s);
+void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s,
+                      const double* com = nullptr, int nblk = 0, double inv_mtot = 0.0, const double* mass = nullptr);
 int pile_blocks(int na);
 void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
                  uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);

@@ -11,51 +11,87 @@
 namespace rpmd {
 
 // ---- NM transform (src/fourier.f90:16-63 as a matrix) ------------------------------------------
+// One launch transforms up to two arrays (NMSrc a, b) with the same matrix; b.in == nullptr: one.  An
+// input element may be formed on load: in - a1 g1 - a2 g2 (each term rounded, as k_kick), and for
+// mode 0 minus m_a pcom_d, pcom = (sum of com[nblk][3]) / M (as k_com_remove).
+struct NMSrc {
+    const double* in = nullptr;
+    double* out = nullptr;
+    const double* g1 = nullptr;
+    double a1 = 0.0;
+    const double* g2 = nullptr;
+    double a2 = 0.0;
+    const double* com = nullptr;
+    int nblk = 0;
+    double inv_mtot = 0.0;
+    const double* mass = nullptr;
+    int na = 0;
+};
+__device__ __forceinline__ double nm_load(const NMSrc& s, int j, size_t e, size_t ne) {
+    const size_t o = (size_t)j * ne + e;
+    double v = s.in[o];
+    if (s.g1) v = __dadd_rn(v, -__dmul_rn(s.a1, s.g1[o]));
+    if (s.g2) v = __dadd_rn(v, -__dmul_rn(s.a2, s.g2[o]));
+    if (s.com && j == 0) {
+        const int d = (int)(e / s.na), a = (int)(e % s.na);
+        double pc = 0.0;
+        for (int q = 0; q < s.nblk; q++) pc += s.com[q * 3 + d];
+        const double pcd = pc * s.inv_mtot;
+        v -= pcd * s.mass[a];
+    }
+    return v;
+}
+
 template <int NBM>
-__global__ __launch_bounds__(256) void k_nm_reg(const double* __restrict__ in, double* __restrict__ out,
-                                                const double* __restrict__ M, int n, size_t ne) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+__global__ __launch_bounds__(256) void k_nm_reg(NMSrc sa, NMSrc sb, const double* __restrict__ M, int n, size_t ne) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const NMSrc& src = e < ne ? sa : sb;
+    if (e >= ne) e -= ne;
+    if (e >= ne || !src.in) return;
     double x[NBM];
 #pragma unroll
-    for (int j = 0; j < NBM; j++) x[j] = j < n ? in[(size_t)j * ne + e] : 0.0;
+    for (int j = 0; j < NBM; j++) x[j] = j < n ? nm_load(src, j, e, ne) : 0.0;
     for (int k = 0; k < n; k++) {
         const double* row = M + (size_t)k * n;
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < NBM; j++)
             if (j < n) s = fma(row[j], x[j], s);
-        out[(size_t)k * ne + e] = s;
+        src.out[(size_t)k * ne + e] = s;
     }
 }
 
-__global__ __launch_bounds__(256) void k_nm_gen(const double* __restrict__ in, double* __restrict__ out,
-                                                const double* __restrict__ M, int n, size_t ne) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+__global__ __launch_bounds__(256) void k_nm_gen(NMSrc sa, NMSrc sb, const double* __restrict__ M, int n, size_t ne) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const NMSrc& src = e < ne ? sa : sb;
+    if (e >= ne) e -= ne;
+    if (e >= ne || !src.in) return;
     for (int k = 0; k < n; k++) {
         const double* row = M + (size_t)k * n;
         double s = 0.0;
-        for (int j = 0; j < n; j++) s = fma(row[j], in[(size_t)j * ne + e], s);
-        out[(size_t)k * ne + e] = s;
+        for (int j = 0; j < n; j++) s = fma(row[j], nm_load(src, j, e, ne), s);
+        src.out[(size_t)k * ne + e] = s;
     }
 }
 
 // FP64 MFMA form (v_mfma_f64_16x16x4f64): D[i][jj] = sum_kk A[i][kk] B[kk][jj] with
 // A = M (16 output modes x 4 input beads), B = in (4 input beads x 16 columns).  Lane l supplies
 // A[l%16][l/16] and B[l/16][l%16]; it receives D[(l/16) + 4v][l%16], v = 0..3.  One wave owns a
-// 16-column tile; all NT*16 output rows of the tile accumulate over ceil(n/4) bead steps.
+// 16-column tile of one source; all NT*16 output rows of the tile accumulate over ceil(n/4) bead
+// steps.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 template <int NT>
-__global__ __launch_bounds__(256) void k_nm_mfma(const double* __restrict__ in, double* __restrict__ out,
-                                                 const double* __restrict__ M, int n, size_t ne) {
+__global__ __launch_bounds__(256) void k_nm_mfma(NMSrc sa, NMSrc sb, const double* __restrict__ M, int n, size_t ne) {
     const int lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const size_t ntile = (ne + 15) / 16;
+    const size_t ntile = (ne + 15) / 16, ntot = sb.in ? 2 * ntile : ntile;
     const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const size_t nwave = ((size_t)gridDim.x * blockDim.x) >> 6;
     const int nstep = (n + 3) / 4;
-    for (size_t t = wave; t < ntile; t += nwave) {
+    for (size_t tt = wave; tt < ntot; tt += nwave) {
+        const bool second = tt >= ntile;
+        const NMSrc& src = second ? sb : sa;
+        const size_t t = second ? tt - ntile : tt;
         const size_t e = t * 16 + li;
         const bool ein = e < ne;
         dbl4 acc[NT];
@@ -63,7 +99,7 @@ __global__ __launch_bounds__(256) void k_nm_mfma(const double* __restrict__ in, 
         for (int q = 0; q < NT; q++) acc[q] = (dbl4){0.0, 0.0, 0.0, 0.0};
         for (int st = 0; st < nstep; st++) {
             const int j = st * 4 + lk;
-            const double b = (ein && j < n) ? in[(size_t)j * ne + e] : 0.0;
+            const double b = (ein && j < n) ? nm_load(src, j, e, ne) : 0.0;
 #pragma unroll
             for (int q = 0; q < NT; q++) {
                 const int k = q * 16 + li;
@@ -77,44 +113,85 @@ __global__ __launch_bounds__(256) void k_nm_mfma(const double* __restrict__ in, 
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int k = q * 16 + lk + 4 * v;
-                    if (k < n) out[(size_t)k * ne + e] = acc[q][v];
+                    if (k < n) src.out[(size_t)k * ne + e] = acc[q][v];
                 }
         }
     }
 }
 
-// out may not alias in
+// outputs may not alias inputs
+static void launch_nm_src(const NMSrc& sa, const NMSrc& sb, const double* M, int n, size_t ne, hipStream_t s) {
+    const size_t cols = sb.in ? 2 * ne : ne;
+    const unsigned g = (unsigned)((cols + 255) / 256);
+    const size_t ntile = (ne + 15) / 16 * (sb.in ? 2 : 1);
+    const unsigned gm = (unsigned)std::min<size_t>((ntile + 3) / 4, 4096);  // 4 waves per block
+    if (n <= 8) k_nm_reg<8><<<g, 256, 0, s>>>(sa, sb, M, n, ne);
+    else if (n <= 16) k_nm_mfma<1><<<gm, 256, 0, s>>>(sa, sb, M, n, ne);
+    else if (n <= 32) k_nm_mfma<2><<<gm, 256, 0, s>>>(sa, sb, M, n, ne);
+    else if (n <= 48) k_nm_mfma<3><<<gm, 256, 0, s>>>(sa, sb, M, n, ne);
+    else if (n <= 64) k_nm_mfma<4><<<gm, 256, 0, s>>>(sa, sb, M, n, ne);
+    else k_nm_gen<<<g, 256, 0, s>>>(sa, sb, M, n, ne);
+}
 void launch_nm(const double* in, double* out, const double* M, int n, size_t ne, hipStream_t s) {
-    const unsigned g = (unsigned)((ne + 255) / 256);
-    const size_t ntile = (ne + 15) / 16;
-    const unsigned gm = (unsigned)std::min<size_t>((ntile + 3) / 4, 2048);  // 4 waves per block
-    if (n <= 8) k_nm_reg<8><<<g, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 16) k_nm_mfma<1><<<gm, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 32) k_nm_mfma<2><<<gm, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 48) k_nm_mfma<3><<<gm, 256, 0, s>>>(in, out, M, n, ne);
-    else if (n <= 64) k_nm_mfma<4><<<gm, 256, 0, s>>>(in, out, M, n, ne);
-    else k_nm_gen<<<g, 256, 0, s>>>(in, out, M, n, ne);
+    NMSrc a;
+    a.in = in;
+    a.out = out;
+    launch_nm_src(a, NMSrc{}, M, n, ne, s);
+}
+// pnm = F (p - a1 g1 - a2 g2) and rnm = F r in one launch (the kicked p itself is not stored)
+void launch_nm_kick2(const double* p, const double* g1, double a1, const double* g2, double a2, double* pnm,
+                     const double* r, double* rnm, const double* F, int n, size_t ne, hipStream_t s) {
+    NMSrc a, b;
+    a.in = p;
+    a.out = pnm;
+    a.g1 = g1;
+    a.a1 = a1;
+    a.g2 = g2;
+    a.a2 = a2;
+    b.in = r;
+    b.out = rnm;
+    launch_nm_src(a, b, F, n, ne, s);
+}
+// rst = B rnm and p = B (pnm with the centroid COM removed, as k_com_remove) in one launch
+void launch_nm_com2(const double* rnm, double* rst, const double* pnm, double* p, const double* com, int nblk,
+                    double inv_mtot, const double* mass, int na, const double* B, int n, size_t ne, hipStream_t s) {
+    NMSrc a, b;
+    a.in = rnm;
+    a.out = rst;
+    b.in = pnm;
+    b.out = p;
+    b.com = com;
+    b.nblk = nblk;
+    b.inv_mtot = inv_mtot;
+    b.mass = mass;
+    b.na = na;
+    launch_nm_src(a, b, B, n, ne, s);
 }
 
 // ---- momentum kicks: p -= a1 g1 + a2 g2 (src/evolve_pi.f90:82-83, 158, 228, 279) ---------------
 // Each term is applied as its own rounded update, as the reference does.
+// ctr (optional): advanced by the noise streams of one step (nothing after this kick draws in the step)
 __global__ void k_kick(double* __restrict__ p, const double* __restrict__ g1, double a1, const double* __restrict__ g2,
-                       double a2, size_t n) {
+                       double a2, size_t n, uint64_t* __restrict__ ctr) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctr && i == 0) *ctr += RPMD_STREAMS_PER_STEP;
     if (i >= n) return;
     double v = p[i];
     if (g1) v = __dadd_rn(v, -__dmul_rn(a1, g1[i]));
     if (g2) v = __dadd_rn(v, -__dmul_rn(a2, g2[i]));
     p[i] = v;
 }
-void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s) {
-    k_kick<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, g1, a1, g2, a2, n);
+void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s,
+                 uint64_t* ctr) {
+    k_kick<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, g1, a1, g2, a2, n, ctr);
 }
 
 // ---- free ring polymer: monodromy per mode (src/evolve_pi.f90:109-125), no contraction -----------
 // monod(3,4,nb) column-major as ring_rpmd fills it (src/md_pathi.f90:190-231)
+// com (optional): the centroid COM momentum is removed from mode 0 first (as k_com_remove)
 __global__ void k_free_ring(double* __restrict__ p, double* __restrict__ r, const double* __restrict__ monod, int na,
-                            int nb) {
+                            int nb, const double* __restrict__ com, int nblk, double inv_mtot,
+                            const double* __restrict__ mass) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t ne = (size_t)3 * na;
     if (i >= ne * nb) return;
@@ -122,15 +199,24 @@ __global__ void k_free_ring(double* __restrict__ p, double* __restrict__ r, cons
     const int a = (int)((i % ne) % na);
     const int ic = a % 3;
     const double* mo = monod + (size_t)k * 12;
-    const double pv = p[i], rv = r[i];
+    double pv = p[i];
+    const double rv = r[i];
+    if (com && k == 0) {
+        const int d = (int)(i / na);
+        double pc = 0.0;
+        for (int q = 0; q < nblk; q++) pc += com[q * 3 + d];
+        const double pcd = pc * inv_mtot;
+        pv -= pcd * mass[a];
+    }
     const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
     const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
     p[i] = pp;
     r[i] = rr;
 }
-void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s) {
+void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s, const double* com,
+                      int nblk, double inv_mtot, const double* mass) {
     const size_t n = (size_t)3 * na * nb;
-    k_free_ring<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, r, monod, na, nb);
+    k_free_ring<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, r, monod, na, nb, com, nblk, inv_mtot, mass);
 }
 
 // ---- counter-based Gaussian noise: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller ------------
