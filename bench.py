@@ -248,8 +248,8 @@ def main():
         ("fp64", "k_sk"): "FP64 MFMA (v_mfma_f64_16x16x4f64)",
         ("fp64", "k_kforce"): "FP64 vector FMA (Horner)",
         ("mixed", "ewald_real"): "FP32 vector ALU (packed FP32 not used)",
-        ("mixed", "k_sk"): "FP32 MFMA (v_mfma_f32_16x16x4f32)",
-        ("mixed", "k_kforce"): "FP32 MFMA GEMM (v_mfma_f32_16x16x4f32)",
+        ("mixed", "k_sk"): "FP16 MFMA with split FP32 operands (v_mfma_f32_16x16x32_f16), FP32-equivalent flops",
+        ("mixed", "k_kforce"): "FP16 MFMA with split FP32 operands (v_mfma_f32_16x16x32_f16), FP32-equivalent flops",
     }[(head, dom)]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_1728x32_{head}.json")

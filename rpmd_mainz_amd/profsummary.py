@@ -12,7 +12,9 @@ import csv
 import json
 import sys
 
-KERNEL_KEYS = {"ewald_real": "k_ewald_vl", "k_sk": "k_sk_mfma", "k_kforce": "k_kforce", "k_sk_final": "k_sk_final"}
+# bench.py's kernel names -> the kernels that implement them (mixed, then FP64)
+KERNEL_KEYS = {"ewald_real": ["k_ewald_vl"], "k_sk": ["k_sk_h16", "k_sk_mfma"],
+               "k_kforce": ["k_kforce_h16", "k_kforce"], "k_sk_final": ["k_sk_final"]}
 
 
 def _short(name: str) -> str:
@@ -64,8 +66,8 @@ def pmc(fetch_csv: str, write_csv: str, copy_mib: float, command: str):
     sf, sw = copy_kib / cf, copy_kib / cw
     out = dict(command=command, copy_kernel="__amd_rocclr_copyBuffer", copy_mib=copy_mib, copy_fetch_kib=cf,
                copy_write_kib=cw, fetch_scale=sf, write_scale=sw, kernels={})
-    for key, sub in KERNEL_KEYS.items():
-        ks = [k for k in f if sub in k and k in w]
+    for key, subs in KERNEL_KEYS.items():
+        ks = [k for sub in subs for k in f if sub in k and k in w]
         if not ks:
             continue
         k = ks[0]
