@@ -387,8 +387,9 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
     return 0;
 }
 
+// defer: append the energy reductions (outputs relative to c->ev_step) instead of launching them
 int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
-               bool check_lists) {
+               bool check_lists, ReduceJobs* defer = nullptr) {
     const rpmd_params& P = c->prm;
     const int na = c->na;
     hipStream_t s = c->st;
@@ -408,6 +409,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     int gacc = iopt == 3 ? 1 : 0;
     ReduceJobs jobs;
     int use_list = 0;
+    unsigned long long* oo_dmax = nullptr;
     if (need_oo && nb > 3) {
         // centroid O-O list (src/md_forces.f90:34-53)
         const double cut = std::min(P.rcut, 0.5 * boxmin);
@@ -433,8 +435,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, s);
             launch_nlist_vl(c->cen, na, bx, cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
                             c->nnb, c->nbl, s);
+            oo_dmax = c->oo_dmax;  // cleared by the O-O kernel
         }
     }
+    int chain_npart = 0, chain_acc = 0;
+    double chain_scale = 1.0;
     if (iopt <= 2) {
         if (P.wm > 0.0 || P.wh > 0.0) return RPMD_ERR_SMEARED;
         RETCHK(build_kset(c, box));
@@ -483,8 +488,12 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
                                     c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
         }
         const double sc = cubic ? 1.0 / box[0] : 1.0;
-        // real-space slot then the k-space slots, summed in that order
-        launch_chain_rule(grad, c->fpart, ewald_real_slots() + nkslots, na, nb, P.alpha, sc * sc, gacc, s);
+        // real-space slot then the k-space slots, summed in that order; with the O-O term in this call,
+        // its kernel applies the chain rule too
+        chain_npart = ewald_real_slots() + nkslots;
+        chain_scale = sc * sc;
+        chain_acc = gacc;
+        if (!need_oo) launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
         gacc = 1;
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
         jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
@@ -492,10 +501,13 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     }
     if (need_oo) {
         double* po = c->parts + c->off_oo;
+        const double* chp = chain_npart ? c->fpart : nullptr;
         if (mixed)
-            launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
+            launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s, chp, chain_npart,
+                             P.alpha, chain_scale, chain_acc, oo_dmax);
         else
-            launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s);
+            launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s, chp, chain_npart,
+                              P.alpha, chain_scale, chain_acc, oo_dmax);
         jobs.add(po, nb * oo_blocks(na), 20, 1.0 / nb);
     }
     if (iopt == 0 || iopt == 4) {
@@ -532,7 +544,15 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         gacc = 1;
         jobs.add(pt, nb * tem_blocks(na), 40, 1.0 / nb);
     }
-    launch_reduce_ev(jobs, ev_slot, s);
+    if (defer) {
+        const int base = (int)(ev_slot - c->ev_step);
+        for (int j = 0; j < jobs.n; j++) {
+            if (defer->n >= RED_JOBS) return RPMD_ERR_ARG;
+            defer->add(jobs.parts[j], jobs.nparts[j], base + jobs.out[j], jobs.scale[j]);
+        }
+    } else {
+        launch_reduce_ev(jobs, ev_slot, s);
+    }
     if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
     return 0;
 }
@@ -810,11 +830,13 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
         launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
                     c->com_parts, s);
         launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
-        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check));
+        ReduceJobs red;
+        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red));
         std::swap(c->r, c->rst);
         c->lf_merged = true;
-        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check));
+        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check, &red));
         launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
+        launch_reduce_ev(red, c->ev_step, s);
         if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
         if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
         return 0;

@@ -122,9 +122,12 @@ void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int
                   hipStream_t s);
 void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,
                      double* cb, unsigned long long* dmax, int* overflow, int* nnb, int* nbl, hipStream_t s);
+// chain_parts: also apply the Ewald chain rule (launch_chain_rule) in the same launch
 template <typename T>
 void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
-               const rpmd_params& p, double* grad, double* parts, hipStream_t s);
+               const rpmd_params& p, double* grad, double* parts, hipStream_t s, const double* chain_parts = nullptr,
+               int chain_npart = 0, double chain_alpha = 1.0, double chain_scale = 1.0, int chain_acc = 0,
+               unsigned long long* dmax = nullptr);
 int oo_blocks(int na);
 void launch_tem(const double* site, const double* q, int na, int nb, Box box, const int* nnb, const int* nbl,
                 int use_list, double sig, double* gsite, double* parts, hipStream_t s);

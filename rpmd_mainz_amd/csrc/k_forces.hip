@@ -1615,105 +1615,114 @@ void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int
     k_nlist<<<(n + 3) / 4, 256, 0, s>>>(cen, na, njump, box, cut * cut, nnb, nbl, overflow);
 }
 
-// Verlet O-O centroid list, when the gate is on: per oxygen the oxygens whose centroids lie within
-// rl = cut + skin (ascending, as k_nlist), and cb = the oxygen centroids of this build.  Valid while
-// no oxygen centroid has moved more than skin/2; k_nlist_pick then takes from it the pairs within cut
-// at the call, which is the list of k_nlist.  overflow[0] is sticky.
+// O-O centroid lists of the device-resident calls, one wave per oxygen.  When the gate is on, the
+// Verlet list of the oxygens whose centroids lie within rl = cut + skin (ascending, as k_nlist) and
+// cb = the oxygen centroids of this build; the list stays valid while no oxygen centroid has moved
+// more than skin/2.  Every call: the pairs within cut, taken from the Verlet list (or directly,
+// when building) in its order and with the test of k_nlist, so the result is the list of k_nlist.
+// vnnb/vnbl: the Verlet list; nnb/nbl: the list of the call; overflow[0] is sticky.  dmax is
+// cleared by k_oo, after every wave has read the gate.
 __global__ __launch_bounds__(256) void k_nlist_vl(const double* __restrict__ cen, int na, Box bx, double rl2,
+                                                  double cut2, int* __restrict__ vnnb, int* __restrict__ vnbl,
                                                   int* __restrict__ nnb, int* __restrict__ nbl,
                                                   double* __restrict__ cb, int* __restrict__ overflow, VGate gate) {
-    if (!gate.on()) return;
     const int n = na / 3;
     const int ii = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (ii >= n) return;
     const int i = 3 * ii;
     const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
-    if (lane == 0) {
-        cb[ii] = xi;
-        cb[n + ii] = yi;
-        cb[2 * n + ii] = zi;
-    }
     int cnt = 0;
-    for (int base = 0; base < n; base += 64) {
-        const int jj = base + lane;
-        bool in = false;
-        int j = 0;
-        if (jj < n) {
-            j = 3 * jj;
-            if (j != i) {
+    if (gate.on()) {
+        if (lane == 0) {
+            cb[ii] = xi;
+            cb[n + ii] = yi;
+            cb[2 * n + ii] = zi;
+        }
+        int cv = 0;
+        for (int base = 0; base < n; base += 64) {
+            const int jj = base + lane;
+            bool inv = false, inc = false;
+            int j = 0;
+            if (jj < n) {
+                j = 3 * jj;
+                if (j != i) {
+                    double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
+                    mimage(dx, dy, dz, bx);
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    inv = r2 < rl2;
+                    inc = r2 < cut2;
+                }
+            }
+            const unsigned long long mv = __ballot(inv), mc = __ballot(inc);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (inv) {
+                const int slot = cv + __popcll(mv & below);
+                if (slot < RPMD_NCAP) vnbl[(size_t)ii * RPMD_NCAP + slot] = j;
+            }
+            if (inc) {
+                const int slot = cnt + __popcll(mc & below);
+                if (slot < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + slot] = j;
+            }
+            cv += __popcll(mv);
+            cnt += __popcll(mc);
+        }
+        if (lane == 0) {
+            if (cv > RPMD_NCAP) overflow[0] = 1;
+            vnnb[ii] = cv < RPMD_NCAP ? cv : RPMD_NCAP;
+        }
+    } else {
+        const int cv = vnnb[ii];
+        for (int base = 0; base < cv; base += 64) {
+            const int e = base + lane;
+            bool in = false;
+            int j = 0;
+            if (e < cv) {
+                j = vnbl[(size_t)ii * RPMD_NCAP + e];
                 double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
                 mimage(dx, dy, dz, bx);
-                in = dx * dx + dy * dy + dz * dz < rl2;
+                in = dx * dx + dy * dy + dz * dz < cut2;
             }
+            const unsigned long long m = __ballot(in);
+            if (in) nbl[(size_t)ii * RPMD_NCAP + cnt + __popcll(m & ((1ull << lane) - 1ull))] = j;
+            cnt += __popcll(m);
         }
-        const unsigned long long m = __ballot(in);
-        if (in) {
-            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (slot < RPMD_NCAP) nbl[(size_t)ii * RPMD_NCAP + slot] = j;
-        }
-        cnt += __popcll(m);
     }
-    if (lane == 0) {
-        if (cnt > RPMD_NCAP) overflow[0] = 1;
-        nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
-    }
-}
-// one wave per oxygen: the entries of its Verlet list whose centroids lie within cut, in their
-// (ascending) order and tested as k_nlist tests them, so the result is the list of k_nlist; clears
-// dmax for the next call
-__global__ __launch_bounds__(256) void k_nlist_pick(const double* __restrict__ cen, int na, Box bx, double cut2,
-                                                    const int* __restrict__ vnnb, const int* __restrict__ vnbl,
-                                                    int* __restrict__ nnb, int* __restrict__ nbl,
-                                                    unsigned long long* __restrict__ dmax) {
-    const int n = na / 3;
-    const int ii = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (blockIdx.x == 0 && threadIdx.x == 0) dmax[0] = 0ull;
-    if (ii >= n) return;
-    const int i = 3 * ii;
-    const double xi = cen[i], yi = cen[na + i], zi = cen[2 * na + i];
-    const int cv = vnnb[ii];
-    int cnt = 0;
-    for (int base = 0; base < cv; base += 64) {
-        const int e = base + lane;
-        bool in = false;
-        int j = 0;
-        if (e < cv) {
-            j = vnbl[(size_t)ii * RPMD_NCAP + e];
-            double dx = cen[j] - xi, dy = cen[na + j] - yi, dz = cen[2 * na + j] - zi;
-            mimage(dx, dy, dz, bx);
-            in = dx * dx + dy * dy + dz * dz < cut2;
-        }
-        const unsigned long long m = __ballot(in);
-        if (in) nbl[(size_t)ii * RPMD_NCAP + cnt + __popcll(m & ((1ull << lane) - 1ull))] = j;
-        cnt += __popcll(m);
-    }
-    if (lane == 0) nnb[ii] = cnt;
+    if (lane == 0) nnb[ii] = cnt < RPMD_NCAP ? cnt : RPMD_NCAP;
 }
 // gate: an oxygen centroid coordinate moved by more than skin / (2 sqrt 3)
 void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,
                      double* cb, unsigned long long* dmax, int* overflow, int* nnb, int* nbl, hipStream_t s) {
     const VGate gate{dmax, 0.25 * skin * skin / 3.0, force};
     const double rl = cut + skin;
-    const unsigned nblk = (unsigned)((na / 3 + 3) / 4);
-    k_nlist_vl<<<nblk, 256, 0, s>>>(cen, na, box, rl * rl, vnnb, vnbl, cb, overflow, gate);
-    k_nlist_pick<<<nblk, 256, 0, s>>>(cen, na, box, cut * cut, vnnb, vnbl, nnb, nbl, dmax);
+    k_nlist_vl<<<(unsigned)((na / 3 + 3) / 4), 256, 0, s>>>(cen, na, box, rl * rl, cut * cut, vnnb, vnbl, nnb, nbl, cb,
+                                                           overflow, gate);
 }
 
 // LJ (gam == 0) or Buckingham exp-6 (src/ljones.f90, src/buckingham.f90); OOL lanes per (bead, O),
 // lane s taking neighbours s, s + OOL, ...; the O gradient is summed over its lanes by a fixed
 // butterfly.  use_list = 0: every other O (the reference's basic / cell branches, same pair set).
 #define OOL 8
+// ch.parts (optional): the Ewald chain rule (k_chain) of the molecule is applied here as well, its
+// three lanes writing grad, with the O-O gradient added to the oxygen; dmax (optional) is cleared.
+struct ChainArgs {
+    const double* parts = nullptr;
+    int npart = 0;
+    size_t pstride = 0;
+    double alpha = 1.0, scale = 1.0;
+    int acc = 0;
+};
 template <typename T>
 __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int na, Box bx, const int* __restrict__ nnb,
                                             const int* __restrict__ nbl, int use_list, double rc2, double sigsq,
                                             double ba, double bb, double bc, int buck, double vs, double ds,
+                                            ChainArgs ch, unsigned long long* __restrict__ dmax,
                                             double* __restrict__ grad, double* __restrict__ parts) {
     const int b = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int ii = t / OOL, sl = t % OOL;
     const int n = na / 3;
+    if (dmax && t == 0 && b == 0) dmax[0] = 0ull;
     double acc[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) acc[k] = 0.0;
@@ -1768,12 +1777,30 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
         gy += __shfl_xor(gy, off, 64);
         gz += __shfl_xor(gz, off, 64);
     }
-    if (ii < n && sl == 0) {
+    if (ii < n && !ch.parts && sl == 0) {
         const int i = 3 * ii;
         double* G = grad + (size_t)b * 3 * na;
         G[i] += gx;
         G[na + i] += gy;
         G[2 * na + i] += gz;
+    }
+    if (ii < n && ch.parts && sl < 3) {
+        // k_chain for dimension sl of the molecule, then the oxygen's O-O gradient
+        const int d = sl;
+        const size_t o = ((size_t)b * 3 + d) * na + 3 * ii;
+        double s0 = ch.parts[o], s1 = ch.parts[o + 1], s2 = ch.parts[o + 2];
+        for (int p = 1; p < ch.npart; p++) {
+            s0 += ch.parts[p * ch.pstride + o];
+            s1 += ch.parts[p * ch.pstride + o + 1];
+            s2 += ch.parts[p * ch.pstride + o + 2];
+        }
+        const double alpha2 = 0.5 * (1.0 - ch.alpha);
+        const double g0 = ch.scale * s0, g1 = ch.scale * s1, g2 = ch.scale * s2;
+        const double b0 = ch.acc ? grad[o] : 0.0, b1 = ch.acc ? grad[o + 1] : 0.0, b2 = ch.acc ? grad[o + 2] : 0.0;
+        const double lj = d == 0 ? gx : (d == 1 ? gy : gz);
+        grad[o] = (b0 + ch.alpha * g0) + lj;
+        grad[o + 1] = b1 + (g1 + alpha2 * g0);
+        grad[o + 2] = b2 + (g2 + alpha2 * g0);
     }
     // every pair is seen from both ends: energy and virial carry 1/2; the virial of a central
     // force is symmetric
@@ -1784,7 +1811,15 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
 
 template <typename T>
 void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
-               const rpmd_params& p, double* grad, double* parts, hipStream_t s) {
+               const rpmd_params& p, double* grad, double* parts, hipStream_t s, const double* chain_parts,
+               int chain_npart, double chain_alpha, double chain_scale, int chain_acc, unsigned long long* dmax) {
+    ChainArgs ch;
+    ch.parts = chain_parts;
+    ch.npart = chain_npart;
+    ch.pstride = (size_t)3 * na * nb;
+    ch.alpha = chain_alpha;
+    ch.scale = chain_scale;
+    ch.acc = chain_acc;
     const int buck = p.oo_gam != 0.0;
     double ba = 0, bb = 0, bc = 0;
     if (buck) {
@@ -1795,12 +1830,15 @@ void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const
     const int n = na / 3;
     const double vs = buck ? 1.0 : 4.0 * p.oo_eps, ds = buck ? 1.0 : 48.0 * p.oo_eps;
     k_oo<T><<<dim3(oo_blocks(na), nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
-                                                    p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, grad, parts);
+                                                    p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, ch, dmax, grad,
+                                                    parts);
 }
 template void launch_oo<double>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
-                                double*, double*, hipStream_t);
+                                double*, double*, hipStream_t, const double*, int, double, double, int,
+                                unsigned long long*);
 template void launch_oo<float>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
-                               double*, double*, hipStream_t);
+                               double*, double*, hipStream_t, const double*, int, double, double, int,
+                               unsigned long long*);
 int oo_blocks(int na) { return (na / 3 * OOL + 255) / 256; }
 
 // TEM switched short-range Coulomb between different molecules (src/correct_tem.f90:73-177);
