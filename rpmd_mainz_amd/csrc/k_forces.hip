@@ -151,6 +151,7 @@ __global__ void k_centroid(const double* __restrict__ pos, double* __restrict__ 
     if (i >= (size_t)3 * na) return;
     size_t d = i / na, a = i % na;
     double s = 0.0;
+#pragma unroll 8
     for (int b = 0; b < nb; b++) s = s + pos[((size_t)b * 3 + d) * na + a];
     cen[d * na + a] = s / (double)nb;
 }
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(256) void k_centroid_vl(const double* __restrict__ 
     if (i < (size_t)3 * na) {
         const size_t d = i / na, a = i % na;
         double s = 0.0;
+#pragma unroll 8
         for (int b = 0; b < nb; b++) s = s + pos[((size_t)b * 3 + d) * na + a];
         const double c = s / (double)nb;
         cen[d * na + a] = c;
@@ -264,13 +266,17 @@ __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& 
     du = erfc_ * rinv;
     dur = (sfac * e - du) * (rinv * rinv);
 }
+// FP32: alpha arrives multiplied by sqrt(log2 e), so e^{-(alpha r)^2} is one v_exp_f32 (2^y) of
+// -x^2, and t = p / (p + alpha r) = p' / (p' + x) with p' = p sqrt(log2 e)
+constexpr double SQRT_LOG2E = 1.2011224087864498;
 template <>
 __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float sfac, bool same, float& du, float& dur) {
+    constexpr float P2 = (float)(AS_P * SQRT_LOG2E);
     float rinv = __frsqrt_rn(dr2);
     float dr = dr2 * rinv;
     float x = alpha * dr;
-    float e = __expf(-x * x);
-    float t = __builtin_amdgcn_rcpf((float)AS_P + x) * (float)AS_P;  // v_rcp_f32 (1 ulp)
+    float e = __builtin_amdgcn_exp2f(-x * x);
+    float t = __builtin_amdgcn_rcpf(P2 + x) * P2;  // v_rcp_f32 (1 ulp)
     float erfc_ = e * t * ((float)AS_A1 + t * ((float)AS_A2 + t * ((float)AS_A3 + t * ((float)AS_A4 + t * (float)AS_A5))));
     if (same) erfc_ = erfc_ - 1.0f;
     du = erfc_ * rinv;
@@ -691,7 +697,7 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
 // butterfly.  Block id i runs site block (i % 8) + 8 ((i / 8) / nb) of bead (i / 8) % nb, so the nb
 // blocks that read one stretch of the list run on one XCD (blocks go to the 8 XCDs round robin) and
 // share its L2.  Clears dmax for the next call (every gated kernel has run).
-template <typename T>
+template <typename T, bool UNIT>
 __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
                                                   const int* __restrict__ perm, const int* __restrict__ cnt,
                                                   const int* __restrict__ nex, const int* __restrict__ nl, int na,
@@ -729,8 +735,13 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         auto ent = [&](int r) { return L[(size_t)(r < nn ? r : pad) * ls]; };
         auto pair = [&](const P4& cur, int r) {
             T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
-            if (F64) {
-                // physical minimum image (the unit box when cubic)
+            if (UNIT) {
+                // the unit box (cubic): both forms reduce to this exactly
+                ex -= rint(ex);
+                ey -= rint(ey);
+                ez -= rint(ez);
+            } else if (F64) {
+                // physical minimum image
                 ex -= Lt0 * rint(ex * it0);
                 ey -= Lt1 * rint(ey * it1);
                 ez -= Lt2 * rint(ez * it2);
@@ -860,10 +871,12 @@ template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
-    k_ewald_vl<T><<<ewald_real_parts(na) * nb, 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap,
-                                                            sb.L[0], sb.L[1], sb.L[2], sb.inv[0], sb.inv[1],
-                                                            sb.inv[2], rcut * rcut, alpha, sfac, grad, parts,
-                                                            v.dmax);
+    const double ak = sizeof(T) == 4 ? alpha * SQRT_LOG2E : alpha;  // see erfc_pair<float>
+    const bool unit = sb.L[0] == 1.0 && sb.L[1] == 1.0 && sb.L[2] == 1.0;
+    auto k = unit ? k_ewald_vl<T, true> : k_ewald_vl<T, false>;
+    k<<<ewald_real_parts(na) * nb, 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap, sb.L[0],
+                                                sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, ak,
+                                                sfac, grad, parts, v.dmax);
 }
 template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
 template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
