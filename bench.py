@@ -72,8 +72,9 @@ def kset_size(prm) -> int:
     return n
 
 
-def run_gpu(prec: str, steps: int, warmup: int, r, p, box, z, mass, prm, beta, na, use_graph: bool):
+def run_gpu(prec: str, steps: int, warmup: int, r, p, box, z, mass, prm, beta, na, use_graph: bool, streams: int):
     ctx = capi.RPMDContext(prm, na, NB, prec)
+    ctx.set_streams(streams)
     dt = DT_FS * cf.TOFS
     ctx.state_upload(p, r, mass, z, na, NB, box, beta, dt, seed=77)
     ctx.use_graph(use_graph)
@@ -82,7 +83,8 @@ def run_gpu(prec: str, steps: int, warmup: int, r, p, box, z, mass, prm, beta, n
     ctx.run_steps(steps)
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
-    # kernel timing over a separate eager window (HIP events on the library stream)
+    # kernel timing over a separate eager window (HIP events on the library stream; the library runs
+    # every kernel on one stream while it times them)
     ctx.prof_begin()
     nprof = max(2, min(steps, 50))
     ctx.run_steps(nprof)
@@ -199,6 +201,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--prec", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="2: real-space Ewald beside the k-space kernels on a second stream; 1: one stream")
     ap.add_argument("--cpu-beads", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nve-steps", type=int, default=2000, help="NVE window per precision for H_RP drift (0: skip)")
@@ -215,7 +219,7 @@ def main():
     precs = ["fp64", "mixed"] if a.prec == "both" else [a.prec]
     for prec in precs:
         print(f"[bench] {prec}: warmup {a.warmup}, timing {a.steps} steps", file=sys.stderr, flush=True)
-        res[prec] = run_gpu(prec, a.steps, a.warmup, r, p, box, z, mass, prm, beta, na, not a.no_graph)
+        res[prec] = run_gpu(prec, a.steps, a.warmup, r, p, box, z, mass, prm, beta, na, not a.no_graph, a.streams)
     # force error of the mixed path against FP64 on the same configuration
     err = None
     if "mixed" in res and "fp64" in res:
@@ -296,7 +300,7 @@ def main():
         "data": "synthetic (seeded 12^3 lattice, random orientations, Maxwell-Boltzmann momenta)",
         "config": {"workload": "1728 q-SPC/Fw x 32 beads, cubic Ewald ecut 1e-6, LJ 9 A, PILE-L 298 K, "
                                "dt 0.25 fs, mts 1", "molecules": 1728, "beads": NB, "precision": head,
-                   "graph": not a.no_graph},
+                   "graph": not a.no_graph, "streams": a.streams},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "fp64_gpu": ({"bead_force_evals_s": 1e3 / res["fp64"][0] * NB, "ms_per_step": res["fp64"][0],
                       "ns_per_day": 1e3 / res["fp64"][0] * DT_FS * 0.0864,

@@ -68,7 +68,8 @@ def lib() -> C.CDLL:
                      "rpmd_ctx_destroy", "rpmd_forces", "rpmd_full_forces", "rpmd_realft", "rpmd_evolve_pi",
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
-                     "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd"):
+                     "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
+                     "rpmd_ctx_set_streams"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -243,6 +244,10 @@ class RPMDContext:
 
     def use_graph(self, enable: bool = True):
         _check(lib().rpmd_use_graph(self._h, C.c_int(1 if enable else 0)))
+
+    def set_streams(self, streams: int):
+        """2: the real-space Ewald branch on a second stream (default); 1: one stream"""
+        _check(lib().rpmd_ctx_set_streams(self._h, C.c_int(streams)))
 
     def run_steps(self, nsteps: int):
         _check(lib().rpmd_run_steps(self._h, C.c_int(nsteps)))

@@ -130,6 +130,11 @@ struct rpmd_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int use_graph = 0;
+    // second stream for the real-space Ewald branch of device-resident force calls (streams = 2):
+    // forked after the sites, joined before the chain rule; default priority, as the first
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int streams = 2;
     // kernel timing with HIP events on the context stream (eager launches only)
     int prof = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -440,6 +445,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     }
     int chain_npart = 0, chain_acc = 0;
     double chain_scale = 1.0;
+    bool fork = false;
     if (iopt <= 2) {
         if (P.wm > 0.0 || P.wh > 0.0) return RPMD_ERR_SMEARED;
         RETCHK(build_kset(c, box));
@@ -454,9 +460,18 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
         // sites, phases, list displacement and (for a built list) the sorted coordinates
         launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, c->q, *vl, !force, sb, s);
+        // the real-space branch (list upkeep and pairs) runs on the second stream, beside the k-space
+        // kernels on the first, unless the lists are checked on the host or kernels are being timed
+        fork = c->streams > 1 && !check_lists && !c->prof;
+        hipStream_t sr = s;
+        if (fork) {
+            HIPCHK(hipEventRecord(c->ev_fork, s));
+            HIPCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+            sr = c->st2;
+        }
         {
             ProfScope ps(c, 4);
-            launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, s);
+            launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, sr);
         }
         if (check_lists) {
             int h[2];
@@ -468,10 +483,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, s);
+                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
             else
-                launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, s);
+                launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
         }
+        if (fork) HIPCHK(hipEventRecord(c->ev_join, c->st2));
         double* pk = c->parts + c->off_k;
         {
             ProfScope ps(c, 1);
@@ -487,6 +503,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->gks,
                                     c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
         }
+        if (fork) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
         const double sc = cubic ? 1.0 / box[0] : 1.0;
         // real-space slot then the k-space slots, summed in that order; with the O-O term in this call,
         // its kernel applies the chain rule too
@@ -1154,7 +1171,17 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     c->na = na;
     c->nb_max = nb_max;
     c->prec = precision;
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->st2) hipStreamDestroy(c->st2);
+    if (c->st) hipStreamDestroy(c->st);
+        if (c->st2) hipStreamDestroy(c->st2);
+        if (c->ev_fork) hipEventDestroy(c->ev_fork);
+        if (c->ev_join) hipEventDestroy(c->ev_join);
         delete c;
         return RPMD_ERR_HIP;
     }
@@ -1568,6 +1595,16 @@ int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double d
     }
     c->gle_n1 = n1;
     c->gle_nb = nb;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return 0;
+}
+
+int rpmd_ctx_set_streams(rpmd_ctx* c, int streams) {
+    if (!c || streams < 1 || streams > 2) return RPMD_ERR_ARG;
+    c->streams = streams;
     if (c->gexec) {
         hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;

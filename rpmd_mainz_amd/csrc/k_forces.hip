@@ -1745,42 +1745,56 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
         const double* X = pos + (size_t)b * 3 * na;
         const double xi = X[i], yi = X[na + i], zi = X[2 * na + i];
         const int cnt = use_list ? nnb[ii] : n;
-        for (int k = sl; k < cnt; k += OOL) {
-            int j;
-            if (use_list) j = nbl[(size_t)ii * RPMD_NCAP + k];
-            else {
-                if (k == ii) continue;
-                j = 3 * k;
+        // the lane's entries k = sl, sl + OOL, ... in groups of four, their loads issued together
+        for (int k0 = sl; k0 < cnt; k0 += 4 * OOL) {
+            int jv[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u * OOL;
+                ok[u] = k < cnt && (use_list || k != ii);
+                jv[u] = !ok[u] ? i : (use_list ? nbl[(size_t)ii * RPMD_NCAP + k] : 3 * k);
             }
-            double dx = xi - X[j], dy = yi - X[na + j], dz = zi - X[2 * na + j];
-            mimage(dx, dy, dz, bx);
-            const double dr2 = dx * dx + dy * dy + dz * dz;
-            if (dr2 < rc2) {
-                double v, fr;   // fr: dV/dr / r (gradient on i = fr * (r_i - r_j))
-                if (!buck) {
-                    const T onr2 = (T)1 / (T)dr2;
-                    const T sr2 = (T)sigsq * onr2;
-                    const T sr6 = sr2 * sr2 * sr2;
-                    v = vs * (double)(sr6 * (sr6 - (T)1));
-                    fr = -ds * (double)(sr6 * (sr6 - (T)0.5) * onr2);
-                } else {
-                    const T tr = sqrt((T)dr2);
-                    const T br6 = (T)bb / ((T)dr2 * (T)dr2 * (T)dr2);
-                    const T ex = (T)ba * exp((T)bc * tr);
-                    v = (double)(ex - br6);
-                    fr = (double)(((T)bc * ex + (T)6 * br6 / tr) / tr);
+            double xj[4], yj[4], zj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                xj[u] = X[jv[u]];
+                yj[u] = X[na + jv[u]];
+                zj[u] = X[2 * na + jv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (!ok[u]) continue;
+                double dx = xi - xj[u], dy = yi - yj[u], dz = zi - zj[u];
+                mimage(dx, dy, dz, bx);
+                const double dr2 = dx * dx + dy * dy + dz * dz;
+                if (dr2 < rc2) {
+                    double v, fr;   // fr: dV/dr / r (gradient on i = fr * (r_i - r_j))
+                    if (!buck) {
+                        const T onr2 = (T)1 / (T)dr2;
+                        const T sr2 = (T)sigsq * onr2;
+                        const T sr6 = sr2 * sr2 * sr2;
+                        v = vs * (double)(sr6 * (sr6 - (T)1));
+                        fr = -ds * (double)(sr6 * (sr6 - (T)0.5) * onr2);
+                    } else {
+                        const T tr = sqrt((T)dr2);
+                        const T br6 = (T)bb / ((T)dr2 * (T)dr2 * (T)dr2);
+                        const T ex = (T)ba * exp((T)bc * tr);
+                        v = (double)(ex - br6);
+                        fr = (double)(((T)bc * ex + (T)6 * br6 / tr) / tr);
+                    }
+                    const double fx = fr * dx, fy = fr * dy, fz = fr * dz;
+                    gx += fx;
+                    gy += fy;
+                    gz += fz;
+                    acc[0] += v;
+                    acc[1] += dx * fx;
+                    acc[2] += dx * fy;
+                    acc[3] += dx * fz;
+                    acc[4] += dy * fy;
+                    acc[5] += dy * fz;
+                    acc[6] += dz * fz;
                 }
-                const double fx = fr * dx, fy = fr * dy, fz = fr * dz;
-                gx += fx;
-                gy += fy;
-                gz += fz;
-                acc[0] += v;
-                acc[1] += dx * fx;
-                acc[2] += dx * fy;
-                acc[3] += dx * fz;
-                acc[4] += dy * fy;
-                acc[5] += dy * fz;
-                acc[6] += dz * fz;
             }
         }
     }

@@ -97,14 +97,23 @@ __global__ __launch_bounds__(256) void k_nm_mfma(NMSrc sa, NMSrc sb, const doubl
         dbl4 acc[NT];
 #pragma unroll
         for (int q = 0; q < NT; q++) acc[q] = (dbl4){0.0, 0.0, 0.0, 0.0};
-        for (int st = 0; st < nstep; st++) {
+        // the tile's inputs (NT*4 bead steps cover n <= 16 NT) are loaded before the MFMA chain,
+        // so the loads are in flight together
+        double bv[NT * 4];
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
             const int j = st * 4 + lk;
-            const double b = (ein && j < n) ? nm_load(src, j, e, ne) : 0.0;
+            bv[st] = (ein && j < n) ? nm_load(src, j, e, ne) : 0.0;
+        }
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
+            if (st >= nstep) break;
+            const int j = st * 4 + lk;
 #pragma unroll
             for (int q = 0; q < NT; q++) {
                 const int k = q * 16 + li;
                 const double a = (k < n && j < n) ? M[(size_t)k * n + j] : 0.0;
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[st], acc[q], 0, 0, 0);
             }
         }
         if (ein) {
