@@ -1037,20 +1037,24 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
 #define HS 36          // dwords per LDS operand row: 32 entries + 4 (16 rows x 16 B reads hit distinct banks)
-#define SKH_CHUNK 512  // atoms per block of the structure-factor pass
 #define KFH 64         // atoms per block of the force pass
 #define KPAD 33        // padded FP32 LDS row
 
+// (a, b) -> hi = fp16 pair rounded toward zero, lo = fp16 pair of (a, b) - hi (rounded to nearest).
+// The mixed-precision FMA takes the difference straight from the packed halves of hi and rounds it
+// to fp16 into one half of lo: three instructions per pair.
 __device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
-    const hp2 h = __builtin_amdgcn_cvt_pkrtz(a, b);
-    const hp2 r = __builtin_amdgcn_cvt_pkrtz(a - (float)h.x, b - (float)h.y);
-    hi = __builtin_bit_cast(uint32_t, h);
-    lo = __builtin_bit_cast(uint32_t, r);
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+    uint32_t r;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(r) : "v"(h), "v"(b));
+    hi = h;
+    lo = r;
 }
-// (re, -im) -> (im, re) in every dword
+// (re, -im) -> (im, re) in a dword
+__device__ __forceinline__ uint32_t conj_swap1(uint32_t v) { return __builtin_amdgcn_alignbit(v, v, 16) ^ 0x8000u; }
 __device__ __forceinline__ uint4 conj_swap(uint4 v) {
-    return make_uint4(__builtin_amdgcn_alignbit(v.x, v.x, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.y, v.y, 16) ^ 0x8000u,
-                      __builtin_amdgcn_alignbit(v.z, v.z, 16) ^ 0x8000u, __builtin_amdgcn_alignbit(v.w, v.w, 16) ^ 0x8000u);
+    return make_uint4(conj_swap1(v.x), conj_swap1(v.y), conj_swap1(v.z), conj_swap1(v.w));
 }
 __device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
@@ -1246,126 +1250,145 @@ __device__ __forceinline__ double2 cpow_sel(double2 z, int e, int m) {
     return r;
 }
 
-// S(kx, ky, kz) = sum_j Y_j(ky) W_j(kz), W_j = q_j X_j^kx Z_j^kz.  A block of 512 threads owns KXB
-// values of kx and SKH_CHUNK atoms; wave w owns the (ky, kz) quadrant w & 3 for kx values 2 (w >> 2)
-// and 2 (w >> 2) + 1 of the block.  Per 32 atoms: (1) the A table (Y_r, -Y_i) x 2^10 [ky][atom] as
-// split half2, Z^kz [atom][kz] and q X^kx x 2^10 [kx][atom] in FP32, all from FP64 powers of the
-// base phases; (2) the B tables W [kz][atom] per kx, split; (3) the MFMAs.  Partial sums per chunk
-// in FP32.
+// S(kx, ky, kz) = sum_j Y_j(ky) W_j(kz), W_j = q_j X_j^kx Z_j^kz.  One block of 512 threads per
+// (atom chunk, bead) owns every kx: wave w owns the kz half w & 1, kx = 4 (w >> 1) .. + 3 and both ky
+// halves.  Per SKB atoms the block forms, from FP64 powers of the base phases, the A operands
+// (Y_r, -Y_i) x 2^10 and (Y_i, Y_r) x 2^10 [ky][atom] as split half2, Z^kz [kz][atom] and
+// q X^kx x 2^10 [kx][atom] in FP32, in one of two LDS buffers (one barrier per SKB atoms: the next
+// tables are formed while the MFMAs read the current ones).  Each lane forms its B operand, W for
+// its kz and its four atoms of a 16-atom step, in registers per kx and splits it: no W goes through
+// LDS.  Per 16 atoms a wave issues 4 kx x 2 ky halves x (Re, Im) x 3 MFMAs.  Partial sums per
+// chunk in FP32.
+#define SKB 32
 __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
-                                                int nchunk, KDev kd, float2* __restrict__ skp) {
-    const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
+                                                int chunk, int nchunk, KDev kd, float2* __restrict__ skp) {
+    const int ch = blockIdx.x, b = blockIdx.y;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-    const int qa = (w >> 1) & 1, qc = w & 1, gp = w >> 2;  // ky half, kz half, kx pair
+    const int qc = w & 1, kg = w >> 1;
     const int kmax = kd.kmax;
-    __shared__ uint32_t Ah[KT][HS], Al[KT][HS];
-    __shared__ uint32_t Bh[KXB][KT][HS], Bl[KXB][KT][HS];
-    __shared__ float2 Zs[32][KT + 1];
-    __shared__ float2 Xs[KXB][32];
-    flt4 sr[2], si[2];
+    __shared__ __align__(16) uint32_t Ab[2][4][KT][HS];  // [buffer][(r, i) x (hi, lo)][ky][atom]
+    __shared__ __align__(16) float2 Zb[2][KT][SKB + 2];   // [buffer][kz][atom]
+    __shared__ __align__(16) float2 Xb[2][KH + 1][SKB];   // [buffer][kx][atom]
+    flt4 sr[4][2], si[4][2];
 #pragma unroll
-    for (int g = 0; g < 2; g++) sr[g] = si[g] = (flt4){0, 0, 0, 0};
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) sr[g][h] = si[g][h] = (flt4){0, 0, 0, 0};
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
     const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
-    const int a0 = ch * SKH_CHUNK, a1 = min(na, a0 + SKH_CHUNK);
-    const int kx0 = kxg * KXB;
+    const int a0 = ch * chunk, a1 = min(na, a0 + chunk);
     constexpr double SC = 1024.0;
-    for (int base = a0; base < a1; base += 32) {
+    // tables of the SKB atoms from base into buffer u
+    auto form = [&](int base, int u) {
+        const int j = t & 31, a = base + j;
+        const bool in = a < a1;
         if (t < 256) {
-            // Y: thread -> (atom t / 8, ky = 4 (t % 8) - 16 .. + 3)
-            const int j = t >> 3, k0 = 4 * (t & 7) - KO, a = base + j;
-            const bool in = a < a1;
+            // Y: thread -> (ky = 4 (t / 32) - 16 .. + 3, atom t % 32)
+            const int k0 = 4 * (t >> 5) - KO;
             const double2 y1 = in ? PY[a] : make_double2(1.0, 0.0);
             double2 yk = cpow_sel(y1, k0, 4);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const int k = k0 + m;
                 const bool ok = in && k >= -kmax && k <= kmax;
-                split_pair(ok ? (float)(SC * yk.x) : 0.0f, ok ? -(float)(SC * yk.y) : 0.0f, Ah[k + KO][j],
-                           Al[k + KO][j]);
+                uint32_t h, lo;
+                split_pair(ok ? (float)(SC * yk.x) : 0.0f, ok ? -(float)(SC * yk.y) : 0.0f, h, lo);
+                Ab[u][0][k + KO][j] = h;
+                Ab[u][1][k + KO][j] = lo;
+                Ab[u][2][k + KO][j] = conj_swap1(h);
+                Ab[u][3][k + KO][j] = conj_swap1(lo);
                 yk = cmul(yk, y1);
             }
-        } else if (t < 384) {
-            // Z: thread -> (atom (t - 256) / 4, kz = 8 (t % 4) - 16 .. + 7)
-            const int j = (t - 256) >> 2, k0 = 8 * (t & 3) - KO, a = base + j;
-            const bool in = a < a1;
+        } else {
+            // Z: thread -> (kz = 4 ((t - 256) / 32) - 16 .. + 3, atom t % 32)
+            const int k0 = 4 * ((t - 256) >> 5) - KO;
             const double2 z1 = in ? PZ[a] : make_double2(1.0, 0.0);
-            double2 zk = cpow_sel(z1, k0, 8);
+            double2 zk = cpow_sel(z1, k0, 4);
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
+            for (int m = 0; m < 4; m++) {
                 const int k = k0 + m;
                 const bool ok = in && k >= -kmax && k <= kmax;
-                Zs[j][k + KO] = ok ? make_float2((float)zk.x, (float)zk.y) : make_float2(0.0f, 0.0f);
+                Zb[u][k + KO][j] = ok ? make_float2((float)zk.x, (float)zk.y) : make_float2(0.0f, 0.0f);
                 zk = cmul(zk, z1);
             }
-        } else {
-            // q X^kx x 2^10: thread -> (atom (t - 384) / 4, kx0 + t % 4)
-            const int j = (t - 384) >> 2, g = t & 3, kx = kx0 + g, a = base + j;
-            double2 v = make_double2(0.0, 0.0);
-            if (a < a1 && kx <= kmax) {
-                const double2 x1 = PX[a];
-                double2 p = x1, r = make_double2(1.0, 0.0);
-#pragma unroll
-                for (int s = 1; s <= 8; s <<= 1) {
-                    if (kx & s) r = cmul(r, p);
-                    p = cmul(p, p);
-                }
-                const double qs = SC * q[a];
-                v = make_double2(qs * r.x, qs * r.y);
-            }
-            Xs[g][j] = make_float2((float)v.x, (float)v.y);
         }
-        __syncthreads();
-        {
-            // W: thread -> (kz = t % 32, atoms 4 ((t / 32) % 8) .. + 3, kx pair t / 256)
-            const int kz = t & 31, j0 = 4 * ((t >> 5) & 7), h = t >> 8;
-            float2 z[4];
+        // q X^kx x 2^10: thread -> (kx = t / 32, atom t % 32)
+        const int kx = t >> 5;
+        double2 v = make_double2(0.0, 0.0);
+        if (in && kx <= kmax) {
+            const double2 x1 = PX[a];
+            double2 p = x1, r = make_double2(1.0, 0.0);
 #pragma unroll
-            for (int i = 0; i < 4; i++) z[i] = Zs[j0 + i][kz];
-#pragma unroll
-            for (int g = 2 * h; g < 2 * h + 2; g++) {
-                uint32_t hi[4], lo[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const float2 x = Xs[g][j0 + i];
-                    split_pair(x.x * z[i].x - x.y * z[i].y, x.x * z[i].y + x.y * z[i].x, hi[i], lo[i]);
-                }
-                *(uint4*)&Bh[g][kz][j0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-                *(uint4*)&Bl[g][kz][j0] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+            for (int s = 1; s <= 8; s <<= 1) {
+                if (kx & s) r = cmul(r, p);
+                if (s < 8) p = cmul(p, p);
             }
+            const double qs = SC * q[a];
+            v = make_double2(qs * r.x, qs * r.y);
         }
-        __syncthreads();
+        Xb[u][kx][j] = make_float2((float)v.x, (float)v.y);
+    };
+    const int nst = (a1 - a0 + SKB - 1) / SKB;
+    if (nst > 0) form(a0, 0);
+    __syncthreads();
+    for (int st = 0; st < nst; st++) {
+        const int u = st & 1;
+        if (st + 1 < nst) form(a0 + (st + 1) * SKB, u ^ 1);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            const int c = 16 * ks + 4 * lk;
-            const uint4 ah = *(const uint4*)&Ah[16 * qa + li][c], al = *(const uint4*)&Al[16 * qa + li][c];
-            const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
+            const int c = 16 * ks + 4 * lk;  // this lane's four atoms of the 16-atom step
+            uint4 arh[2], arl[2], aih[2], ail[2];
 #pragma unroll
-            for (int g = 0; g < 2; g++) {
-                const int gg = 2 * gp + g;
-                const uint4 bh = *(const uint4*)&Bh[gg][16 * qc + li][c], bl = *(const uint4*)&Bl[gg][16 * qc + li][c];
-                sr[g] = mfma_h16(ah, bh, sr[g]);
-                sr[g] = mfma_h16(ah, bl, sr[g]);
-                sr[g] = mfma_h16(al, bh, sr[g]);
-                si[g] = mfma_h16(a2h, bh, si[g]);
-                si[g] = mfma_h16(a2h, bl, si[g]);
-                si[g] = mfma_h16(a2l, bh, si[g]);
+            for (int h = 0; h < 2; h++) {
+                arh[h] = *(const uint4*)&Ab[u][0][16 * h + li][c];
+                arl[h] = *(const uint4*)&Ab[u][1][16 * h + li][c];
+                aih[h] = *(const uint4*)&Ab[u][2][16 * h + li][c];
+                ail[h] = *(const uint4*)&Ab[u][3][16 * h + li][c];
+            }
+            const float4 z01 = *(const float4*)&Zb[u][16 * qc + li][c];
+            const float4 z23 = *(const float4*)&Zb[u][16 * qc + li][c + 2];
+            const float2 z[4] = {make_float2(z01.x, z01.y), make_float2(z01.z, z01.w), make_float2(z23.x, z23.y),
+                                 make_float2(z23.z, z23.w)};
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int kx = 4 * kg + g;
+                if (kx > kmax) break;  // uniform
+                const float4 x01 = *(const float4*)&Xb[u][kx][c];
+                const float4 x23 = *(const float4*)&Xb[u][kx][c + 2];
+                const float2 x[4] = {make_float2(x01.x, x01.y), make_float2(x01.z, x01.w),
+                                     make_float2(x23.x, x23.y), make_float2(x23.z, x23.w)};
+                uint32_t hi[4], lo[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    split_pair(x[i].x * z[i].x - x[i].y * z[i].y, x[i].x * z[i].y + x[i].y * z[i].x, hi[i], lo[i]);
+                const uint4 bh = make_uint4(hi[0], hi[1], hi[2], hi[3]), bl = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    sr[g][h] = mfma_h16(arh[h], bh, sr[g][h]);
+                    si[g][h] = mfma_h16(aih[h], bh, si[g][h]);
+                    sr[g][h] = mfma_h16(arh[h], bl, sr[g][h]);
+                    si[g][h] = mfma_h16(aih[h], bl, si[g][h]);
+                    sr[g][h] = mfma_h16(arl[h], bh, sr[g][h]);
+                    si[g][h] = mfma_h16(ail[h], bh, si[g][h]);
+                }
             }
         }
         __syncthreads();
     }
     const float us = (float)(1.0 / (SC * SC));
 #pragma unroll
-    for (int g = 0; g < 2; g++) {
-        const int kx = kx0 + 2 * gp + g;
+    for (int g = 0; g < 4; g++) {
+        const int kx = 4 * kg + g;
         if (kx > kmax) break;
         float2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int iy = 16 * qa + 4 * lk + v, iz = 16 * qc + li;
-            o[iy * KT + iz] = make_float2(sr[g][v] * us, si[g][v] * us);
-        }
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int iy = 16 * h + 4 * lk + v, iz = 16 * qc + li;
+                o[iy * KT + iz] = make_float2(sr[g][h][v] * us, si[g][h][v] * us);
+            }
     }
 }
 
@@ -1517,7 +1540,14 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
 }
 
 int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
-static int kspace_nchunk_h16(int na) { return (na + SKH_CHUNK - 1) / SKH_CHUNK; }
+// mixed structure-factor pass: about one block per CU over all beads (the FP32 partials of at most
+// twice the FP64 pass's chunks fit its buffer); chunks of a multiple of SKB atoms
+static int kspace_nchunk_h16(int na, int nb, int* chunk) {
+    int n = std::min(2 * kspace_nchunk(na), std::max(1, (256 + nb - 1) / nb));
+    const int c = ((na + n - 1) / n + SKB - 1) / SKB * SKB;
+    *chunk = c;
+    return (na + c - 1) / c;
+}
 int kspace_ngroups(int kmax) { return (kmax + KXF) / KXF; }
 
 static KDev make_kdev(const KSet& ks) {
@@ -1541,9 +1571,9 @@ void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks
                hipStream_t s) {
     const KDev kd = make_kdev(ks);
     if (fp32) {
-        const int nchunk = kspace_nchunk_h16(na);
-        k_sk_h16<<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 512, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
-                                                                          (float2*)sk_part);
+        int chunk = 0;
+        const int nchunk = kspace_nchunk_h16(na, nb, &chunk);
+        k_sk_h16<<<dim3(nchunk, nb), 512, 0, s>>>((const double2*)ph, q, na, chunk, nchunk, kd, (float2*)sk_part);
         return;
     }
     const int nchunk = kspace_nchunk(na);
@@ -1554,9 +1584,10 @@ void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask,
                      double* gks, double* parts_k, int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     const dim3 grid(ks.kmax + 1, nb);
+    int chunk = 0;
     if (fp32)
-        k_sk_final<float2, true><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na), kd, dmask,
-                                                       (double2*)gk, gks, parts_k);
+        k_sk_final<float2, true><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na, nb, &chunk), kd,
+                                                       dmask, (double2*)gk, gks, parts_k);
     else
         k_sk_final<double2, false><<<grid, 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
                                                          (double2*)gk, gks, parts_k);
