@@ -1260,6 +1260,16 @@ __device__ __forceinline__ double2 cpow_sel(double2 z, int e, int m) {
 // LDS.  Per 16 atoms a wave issues 4 kx x 2 ky halves x (Re, Im) x 3 MFMAs.  Partial sums per
 // chunk in FP32.
 #define SKB 32
+// scheduling hint for the straight-line block before it: N groups of one MFMA followed by V other
+// vector instructions, so that the vector work issues while the matrix unit runs
+template <int N, int V>
+__device__ __forceinline__ void mfma_interleave() {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, V, 0);
+    }
+}
 __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
                                                 int chunk, int nchunk, KDev kd, float2* __restrict__ skp) {
     const int ch = blockIdx.x, b = blockIdx.y;
@@ -1279,14 +1289,24 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
     const double2* PZ = ph + ((size_t)b * 3 + 2) * na;
     const int a0 = ch * chunk, a1 = min(na, a0 + chunk);
     constexpr double SC = 1024.0;
+    // this thread's base phases (Y or Z, and X) and charge of an atom, loaded a stage ahead of use;
+    // loads past the chunk are clamped into it and their values unused
+    const int j = t & 31;
+    const double2* PV = t < 256 ? PY : PZ;
+    auto load = [&](int base, double2& v, double2& x, double& qv) {
+        const int aa = min(base + j, a1 - 1);
+        v = PV[aa];
+        x = PX[aa];
+        qv = q[aa];
+    };
     // tables of the SKB atoms from base into buffer u
-    auto form = [&](int base, int u) {
-        const int j = t & 31, a = base + j;
-        const bool in = a < a1;
+    auto form = [&](int base, int u, double2 v1, double2 x1, double qa) {
+        const bool in = base + j < a1;
+        const double2 one = make_double2(1.0, 0.0);
         if (t < 256) {
             // Y: thread -> (ky = 4 (t / 32) - 16 .. + 3, atom t % 32)
             const int k0 = 4 * (t >> 5) - KO;
-            const double2 y1 = in ? PY[a] : make_double2(1.0, 0.0);
+            const double2 y1 = in ? v1 : one;
             double2 yk = cpow_sel(y1, k0, 4);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
@@ -1303,7 +1323,7 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
         } else {
             // Z: thread -> (kz = 4 ((t - 256) / 32) - 16 .. + 3, atom t % 32)
             const int k0 = 4 * ((t - 256) >> 5) - KO;
-            const double2 z1 = in ? PZ[a] : make_double2(1.0, 0.0);
+            const double2 z1 = in ? v1 : one;
             double2 zk = cpow_sel(z1, k0, 4);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
@@ -1317,24 +1337,32 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
         const int kx = t >> 5;
         double2 v = make_double2(0.0, 0.0);
         if (in && kx <= kmax) {
-            const double2 x1 = PX[a];
-            double2 p = x1, r = make_double2(1.0, 0.0);
+            double2 p = x1, r = one;
 #pragma unroll
             for (int s = 1; s <= 8; s <<= 1) {
                 if (kx & s) r = cmul(r, p);
                 if (s < 8) p = cmul(p, p);
             }
-            const double qs = SC * q[a];
+            const double qs = SC * qa;
             v = make_double2(qs * r.x, qs * r.y);
         }
         Xb[u][kx][j] = make_float2((float)v.x, (float)v.y);
     };
     const int nst = (a1 - a0 + SKB - 1) / SKB;
-    if (nst > 0) form(a0, 0);
+    double2 pv, px;
+    double pq;
+    load(a0, pv, px, pq);
+    form(a0, 0, pv, px, pq);
+    load(a0 + SKB, pv, px, pq);
     __syncthreads();
     for (int st = 0; st < nst; st++) {
         const int u = st & 1;
-        if (st + 1 < nst) form(a0 + (st + 1) * SKB, u ^ 1);
+        if (st + 1 < nst) {
+            const double2 cv = pv, cx = px;
+            const double cq = pq;
+            load(a0 + (st + 2) * SKB, pv, px, pq);
+            form(a0 + (st + 1) * SKB, u ^ 1, cv, cx, cq);
+        }
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const int c = 16 * ks + 4 * lk;  // this lane's four atoms of the 16-atom step
@@ -1350,27 +1378,37 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
             const float4 z23 = *(const float4*)&Zb[u][16 * qc + li][c + 2];
             const float2 z[4] = {make_float2(z01.x, z01.y), make_float2(z01.z, z01.w), make_float2(z23.x, z23.y),
                                  make_float2(z23.z, z23.w)};
+            float4 xv[4][2];
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                const int kx = 4 * kg + g;
-                if (kx > kmax) break;  // uniform
-                const float4 x01 = *(const float4*)&Xb[u][kx][c];
-                const float4 x23 = *(const float4*)&Xb[u][kx][c + 2];
-                const float2 x[4] = {make_float2(x01.x, x01.y), make_float2(x01.z, x01.w),
-                                     make_float2(x23.x, x23.y), make_float2(x23.z, x23.w)};
+                xv[g][0] = *(const float4*)&Xb[u][4 * kg + g][c];
+                xv[g][1] = *(const float4*)&Xb[u][4 * kg + g][c + 2];
+            }
+            // W of kx 4 kg + g, split: formed one kx ahead of its MFMAs (X is zero past kmax)
+            auto formw = [&](int g, uint4& bh, uint4& bl) {
+                const float2 x[4] = {make_float2(xv[g][0].x, xv[g][0].y), make_float2(xv[g][0].z, xv[g][0].w),
+                                     make_float2(xv[g][1].x, xv[g][1].y), make_float2(xv[g][1].z, xv[g][1].w)};
                 uint32_t hi[4], lo[4];
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     split_pair(x[i].x * z[i].x - x[i].y * z[i].y, x[i].x * z[i].y + x[i].y * z[i].x, hi[i], lo[i]);
-                const uint4 bh = make_uint4(hi[0], hi[1], hi[2], hi[3]), bl = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+                bh = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+                bl = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+            };
+            uint4 bh[2], bl[2];
+            formw(0, bh[0], bl[0]);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                if (g + 1 < 4) formw(g + 1, bh[(g + 1) & 1], bl[(g + 1) & 1]);
+                const uint4 h0 = bh[g & 1], l0 = bl[g & 1];
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    sr[g][h] = mfma_h16(arh[h], bh, sr[g][h]);
-                    si[g][h] = mfma_h16(aih[h], bh, si[g][h]);
-                    sr[g][h] = mfma_h16(arh[h], bl, sr[g][h]);
-                    si[g][h] = mfma_h16(aih[h], bl, si[g][h]);
-                    sr[g][h] = mfma_h16(arl[h], bh, sr[g][h]);
-                    si[g][h] = mfma_h16(ail[h], bh, si[g][h]);
+                    sr[g][h] = mfma_h16(arh[h], h0, sr[g][h]);
+                    si[g][h] = mfma_h16(aih[h], h0, si[g][h]);
+                    sr[g][h] = mfma_h16(arh[h], l0, sr[g][h]);
+                    si[g][h] = mfma_h16(aih[h], l0, si[g][h]);
+                    sr[g][h] = mfma_h16(arl[h], h0, sr[g][h]);
+                    si[g][h] = mfma_h16(ail[h], h0, si[g][h]);
                 }
             }
         }
@@ -1399,8 +1437,9 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // the quantities of the Horner pass.  One block per (64 atoms, bead); wave w owns rows
 // 16 w .. 16 w + 15 of [G; kz G], which it alone reads: its A operand comes from global memory
 // straight into registers, two kx ahead in two register sets, so the kx loop has no barrier.  B
-// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS.  Each lane sums its part of the
-// three components over kx in FP64; lanes and waves are combined once at the end, in fixed order.
+// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS; B and this lane's Y^ky are then
+// held in registers.  Each lane sums its part of the three components over kx in FP64; lanes and
+// waves are combined once at the end, in fixed order.
 __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
                                                     int na, KDev kd, const uint4* __restrict__ ga,
                                                     const double* __restrict__ gks, double* __restrict__ kpart) {
@@ -1448,18 +1487,31 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
     const int r0 = KT * wz + 16 * mb;
     // A operand of lane (li, lk) for kx: rows r0 + li, dwords 16 ks + 4 lk .. + 3, hi and lo
     const uint4* GA = ga + (size_t)b * (kmax + 1) * (KT * KT) + (r0 + li) * 8 + lk;
+    // the kx-invariant operands in registers for the whole kx loop: B (conj Z, split) of the four
+    // 16-atom tiles and Y^ky of this lane's rows iy = 16 mb + 4 lk + v and atoms 16 n + li
+    uint4 bzh[4][2], bzl[4][2];
+    float2 yv[4][4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            bzh[n][ks] = *(const uint4*)&Zh[16 * n + li][16 * ks + 4 * lk];
+            bzl[n][ks] = *(const uint4*)&Zl[16 * n + li][16 * ks + 4 * lk];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) yv[n][v] = Ys[16 * mb + 4 * lk + v][16 * n + li];
+    }
     double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
     auto body = [&](int kx, uint4 ah0, uint4 al0, uint4 ah1, uint4 al1) {
         if (kd.kyb[kx] < 0) return;  // uniform: no member at this kx
         flt4 ur[4], ui[4];
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            const int c = 16 * ks + 4 * lk;
             const uint4 ah = ks ? ah1 : ah0, al = ks ? al1 : al0;
             const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                const uint4 bh = *(const uint4*)&Zh[16 * n + li][c], bl = *(const uint4*)&Zl[16 * n + li][c];
+                const uint4 bh = bzh[n][ks], bl = bzl[n][ks];
                 if (ks == 0) {
                     ur[n] = mfma_h16(ah, bh, (flt4){0, 0, 0, 0});
                     ui[n] = mfma_h16(a2h, bh, (flt4){0, 0, 0, 0});
@@ -1484,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int iy = 16 * mb + 4 * lk + v;
-                const float2 y = Ys[iy][j];
+                const float2 y = yv[n][v];
                 const float pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;  // q X^kx Y^ky
                 const float ipu = pr * ui[n][v] - pi * ur[n][v];                      // Im(conj(p) U)
                 s0 += ipu;
