@@ -201,8 +201,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--prec", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
-                    help="2: real-space Ewald beside the k-space kernels on a second stream; 1: one stream")
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2, 3],
+                    help="2: real-space Ewald on a second stream beside the k-space kernels; 3: also the O-O and "
+                         "intramolecular terms on a third; 1: one stream")
     ap.add_argument("--cpu-beads", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nve-steps", type=int, default=2000, help="NVE window per precision for H_RP drift (0: skip)")

@@ -177,7 +177,8 @@ int rpmd_state_download(rpmd_ctx* ctx, double* p, double* r, double* dvdr, doubl
 /* Capture rpmd_run_steps(ctx, 1) into a HIP graph and replay it for later calls. */
 int rpmd_use_graph(rpmd_ctx* ctx, int enable);
 /* Streams of a device-resident force call (no counterpart in the reference): 2 (default) runs the
- * real-space Ewald branch on a second stream beside the k-space kernels; 1 runs every kernel on one
+ * real-space Ewald branch on a second stream beside the k-space kernels; 3 also runs the O-O list
+ * and pair sums, with the intramolecular forces of a step, on a third; 1 runs every kernel on one
  * stream, as when kernels are timed.  Results are the same either way. */
 int rpmd_ctx_set_streams(rpmd_ctx* ctx, int streams);
 

@@ -246,7 +246,8 @@ class RPMDContext:
         _check(lib().rpmd_use_graph(self._h, C.c_int(1 if enable else 0)))
 
     def set_streams(self, streams: int):
-        """2: the real-space Ewald branch on a second stream (default); 1: one stream"""
+        """2: the real-space Ewald branch on a second stream (default); 3: also the O-O and
+        intramolecular terms on a third; 1: one stream"""
         _check(lib().rpmd_ctx_set_streams(self._h, C.c_int(streams)))
 
     def run_steps(self, nsteps: int):

@@ -130,11 +130,14 @@ struct rpmd_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int use_graph = 0;
-    // second stream for the real-space Ewald branch of device-resident force calls (streams = 2):
-    // forked after the sites, joined before the chain rule; default priority, as the first
-    hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // second stream for the real-space Ewald branch of device-resident force calls (streams >= 2):
+    // forked after the sites, joined before the chain rule; third stream (streams = 3) for the O-O
+    // list and pair sums of those calls and the intramolecular forces of a step, joined before the
+    // chain rule; default priority, as the first
+    hipStream_t st2 = nullptr, st3 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork3 = nullptr, ev_join3 = nullptr;
     int streams = 2;
+    double* goo = nullptr;  // O-O gradient of the oxygens per molecule, [bead][dim][molecule]
     // kernel timing with HIP events on the context stream (eager launches only)
     int prof = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -362,7 +365,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.iperm, na);
         e |= dalloc(&v.cnt, na);
         e |= dalloc(&v.nex, na);
-        e |= dalloc(&v.nl, (size_t)na * (cap + VLS));  // + the padding row
+        e |= dalloc(&v.nl, (size_t)na * (cap + VLG));  // + the padding group
         e |= dalloc(&v.rb, 3 * ns);
         e |= dalloc(&v.srt, ns);
         e |= dalloc(&v.srf, ns);
@@ -393,11 +396,12 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
 }
 
 // defer: append the energy reductions (outputs relative to c->ev_step) instead of launching them
+// main_st: run the call on this stream instead of the context stream (unforked calls only)
 int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
-               bool check_lists, ReduceJobs* defer = nullptr) {
+               bool check_lists, ReduceJobs* defer = nullptr, hipStream_t main_st = nullptr) {
     const rpmd_params& P = c->prm;
     const int na = c->na;
-    hipStream_t s = c->st;
+    hipStream_t s = main_st ? main_st : c->st;
     if (iopt < 0 || iopt > 5) return RPMD_ERR_IOPT;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
@@ -415,6 +419,15 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     ReduceJobs jobs;
     int use_list = 0;
     unsigned long long* oo_dmax = nullptr;
+    // device-resident calls with the Ewald sum: the O-O list and pair sums run on the third stream from
+    // here, their gradient per molecule (goo) added by the chain-rule kernel after the join
+    const bool fork3 = need_oo && iopt <= 2 && c->streams > 2 && !check_lists && !c->prof && !main_st;
+    hipStream_t so = s;
+    if (fork3) {
+        HIPCHK(hipEventRecord(c->ev_fork3, s));
+        HIPCHK(hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
+        so = c->st3;
+    }
     if (need_oo && nb > 3) {
         // centroid O-O list (src/md_forces.f90:34-53)
         const double cut = std::min(P.rcut, 0.5 * boxmin);
@@ -437,11 +450,21 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             const int force = (!c->oo_built || memcmp(key, c->oo_key, sizeof key) != 0) ? 1 : 0;
             memcpy(c->oo_key, key, sizeof key);
             c->oo_built = true;
-            launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, s);
+            launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, so);
             launch_nlist_vl(c->cen, na, bx, cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
-                            c->nnb, c->nbl, s);
+                            c->nnb, c->nbl, so);
             oo_dmax = c->oo_dmax;  // cleared by the O-O kernel
         }
+    }
+    if (fork3) {
+        double* po = c->parts + c->off_oo;
+        if (mixed)
+            launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, so, nullptr, 0, 1.0, 1.0,
+                             0, oo_dmax, c->goo);
+        else
+            launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, so, nullptr, 0, 1.0,
+                              1.0, 0, oo_dmax, c->goo);
+        HIPCHK(hipEventRecord(c->ev_join3, so));
     }
     int chain_npart = 0, chain_acc = 0;
     double chain_scale = 1.0;
@@ -510,13 +533,20 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         chain_npart = ewald_real_slots() + nkslots;
         chain_scale = sc * sc;
         chain_acc = gacc;
-        if (!need_oo) launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
+        if (fork3) {
+            HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
+            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo);
+        } else if (!need_oo) {
+            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
+        }
         gacc = 1;
         // energies: real and k parts into the Ewald slot (two reductions summed on the host)
         jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
         jobs.add(pk, nb * (P.kmax + 1), 10, sc / nb);
     }
-    if (need_oo) {
+    if (need_oo && fork3) {
+        jobs.add(c->parts + c->off_oo, nb * oo_blocks(na), 20, 1.0 / nb);
+    } else if (need_oo) {
         double* po = c->parts + c->off_oo;
         const double* chp = chain_npart ? c->fpart : nullptr;
         if (mixed)
@@ -848,10 +878,20 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
                     c->com_parts, s);
         launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
         ReduceJobs red;
-        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red));
+        // the intramolecular forces on the third stream, beside the other force kernels
+        const bool intra3 = c->streams > 2 && !check && !c->prof;
+        if (intra3) {
+            HIPCHK(hipEventRecord(c->ev_fork3, s));
+            HIPCHK(hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
+        }
+        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red, intra3 ? c->st3 : nullptr));
         std::swap(c->r, c->rst);
         c->lf_merged = true;
         RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check, &red));
+        if (intra3) {
+            HIPCHK(hipEventRecord(c->ev_join3, c->st3));
+            HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
+        }
         launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
         launch_reduce_ev(red, c->ev_step, s);
         if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
@@ -1160,6 +1200,13 @@ int rpmd_setup_ewald(int na, const double boxlxyz[3], rpmd_params* P) {
     return 0;
 }
 
+static void destroy_streams(rpmd_ctx* c) {
+    for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fork3, c->ev_join3})
+        if (e) hipEventDestroy(e);
+    for (hipStream_t t : {c->st3, c->st2, c->st})
+        if (t) hipStreamDestroy(t);
+}
+
 int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, rpmd_ctx** out) {
     if (!prm || !out || na <= 0 || nb_max <= 0) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
@@ -1173,15 +1220,12 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     c->prec = precision;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
-        if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
-    if (c->st2) hipStreamDestroy(c->st2);
-    if (c->st) hipStreamDestroy(c->st);
-        if (c->st2) hipStreamDestroy(c->st2);
-        if (c->ev_fork) hipEventDestroy(c->ev_fork);
-        if (c->ev_join) hipEventDestroy(c->ev_join);
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming) != hipSuccess) {
+        destroy_streams(c);
         delete c;
         return RPMD_ERR_HIP;
     }
@@ -1216,6 +1260,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->gks, (size_t)nbl * (RPMD_KMAX_DEV + 1));
     e |= dalloc(&c->ph, (size_t)6 * na * nbl);
     e |= dalloc(&c->fpart, (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_DEV)) * 3 * na * nbl);
+    e |= dalloc(&c->goo, (size_t)3 * (na / 3) * nbl);
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
@@ -1288,7 +1333,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
-                    c->step_ctr, c->ev_step};
+                    c->step_ctr, c->ev_step, c->goo};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& kv : c->nm) {
@@ -1299,7 +1344,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     }
     if (c->h_stage) hipHostFree(c->h_stage);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    if (c->st) hipStreamDestroy(c->st);
+    destroy_streams(c);
     delete c;
     return 0;
 }
@@ -1603,7 +1648,7 @@ int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double d
 }
 
 int rpmd_ctx_set_streams(rpmd_ctx* c, int streams) {
-    if (!c || streams < 1 || streams > 2) return RPMD_ERR_ARG;
+    if (!c || streams < 1 || streams > 3) return RPMD_ERR_ARG;
     c->streams = streams;
     if (c->gexec) {
         hipGraphExecDestroy(c->gexec);

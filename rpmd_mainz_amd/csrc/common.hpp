@@ -60,8 +60,9 @@ void launch_reduce_ev(const ReduceJobs& jobs, double* out, hipStream_t s);
 void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
 void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
+// lj (optional): the oxygens' O-O gradient per molecule, [bead][dim][molecule], added to the oxygens
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       int acc, hipStream_t s);
+                       int acc, hipStream_t s, const double* lj = nullptr);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
 void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const double* cb, unsigned long long* dmax,
                         hipStream_t s);
@@ -78,13 +79,15 @@ struct VGate {
 };
 // Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
 // each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads
-#define VLS 4   // lanes per site in the Verlet-list walk (list rows of VLS entries)
+#define VLS 4   // lanes per site in the Verlet-list walk
+#define VLG 16  // entries per list group: VLS lanes x 4 consecutive entries of each
 struct VList {
     int* perm;       // [na] sorted index -> site index (of the last build)
     int* iperm;      // [na] site index -> sorted index
     int* cnt;        // [na] list length of sorted site k
     int* nex;        // [na] leading list entries that are the site's own molecule
-    int* nl;         // [cap / VLS + 1][na][VLS] sorted neighbour indices: entry n of k at row n / VLS
+    int* nl;         // [cap / VLG + 1][na][VLS][4] sorted neighbour indices: entry n of k in group n / VLG,
+                     // lane n % VLS, slot (n / VLS) % 4
     double* rb;      // [nb][3][na] site coordinates at the last build
     double4* srt;    // [nb][na] current coordinates and charge in sorted order
     float4* srf;     // [nb][na] wrapped fractional FP32 coordinates and charge, sorted order
@@ -122,12 +125,13 @@ void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int
                   hipStream_t s);
 void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,
                      double* cb, unsigned long long* dmax, int* overflow, int* nnb, int* nbl, hipStream_t s);
-// chain_parts: also apply the Ewald chain rule (launch_chain_rule) in the same launch
+// chain_parts: also apply the Ewald chain rule (launch_chain_rule) in the same launch; lj: store the
+// O-O gradient per molecule there ([bead][dim][molecule]) instead of adding it to grad
 template <typename T>
 void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
                const rpmd_params& p, double* grad, double* parts, hipStream_t s, const double* chain_parts = nullptr,
                int chain_npart = 0, double chain_alpha = 1.0, double chain_scale = 1.0, int chain_acc = 0,
-               unsigned long long* dmax = nullptr);
+               unsigned long long* dmax = nullptr, double* lj = nullptr);
 int oo_blocks(int na);
 void launch_tem(const double* site, const double* q, int na, int nb, Box box, const int* nnb, const int* nbl,
                 int use_list, double sig, double* gsite, double* parts, hipStream_t s);

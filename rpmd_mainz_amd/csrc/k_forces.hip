@@ -114,8 +114,9 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
 
 // grad (+)= scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient,
 // the sum of npart partial site gradients (parts[p*pstride + ...]) taken in fixed order
+// lj (optional): the oxygens' O-O gradient per molecule [bead][dim][molecule], added last
 __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart, size_t pstride, int na,
-                        int nb, double alpha, double scale, int acc) {
+                        int nb, double alpha, double scale, int acc, const double* __restrict__ lj) {
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nm * nb) return;
@@ -132,17 +133,17 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ pa
         }
         double g0 = scale * s0, g1 = scale * s1, g2 = scale * s2;
         const double b0 = acc ? grad[o] : 0.0, b1 = acc ? grad[o + 1] : 0.0, b2 = acc ? grad[o + 2] : 0.0;
-        grad[o] = b0 + alpha * g0;
+        grad[o] = lj ? (b0 + alpha * g0) + lj[(b * 3 + d) * nm + m] : b0 + alpha * g0;
         grad[o + 1] = b1 + (g1 + alpha2 * g0);
         grad[o + 2] = b2 + (g2 + alpha2 * g0);
     }
 }
 // acc = 0: the first writer of grad in a forces call (stores instead of adding)
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       int acc, hipStream_t s) {
+                       int acc, hipStream_t s, const double* lj) {
     size_t n = (size_t)(na / 3) * nb;
     k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale,
-                                                       acc);
+                                                       acc, lj);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
@@ -535,9 +536,11 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // spreads s): every bead pair then lies within rl of each other only if listed, because
 // |x_kb - x_jb| >= |c_k - c_j| - s_k - s_j.  First the other sites of its own molecule (nex[] of them;
 // the real-space term treats those pairs as excluded, src/ewald_cubic.f90:134), then the others in
-// ascending sorted order.  Entry n of site k is nl[((n/VLS)*na + k)*VLS + n%VLS]: the VLS lanes that
-// walk a site's list read one 16-byte row, and consecutive entries are usually neighbouring sites.
-// Row cap/VLS holds k itself four times (the pair kernel reads it past a lane's end).  The test is
+// ascending sorted order.  Entry n of site k is nl[((n/VLG)*na + k)*VLG + (n%VLS)*4 + (n/VLS)%4]: lane
+// n % VLS of the VLS lanes that walk a site's list reads its next four entries as one 16-byte load,
+// the site's lanes read 64 contiguous bytes, and consecutive entries are usually neighbouring sites.
+// The last group of a site is filled up with k, and group cap/VLG holds k sixteen times (the pair
+// kernel reads it past a lane's end).  The test is
 // FP32 on wrapped fractional centroids against the reach enlarged by a margin that covers the FP32
 // error, so the list is a superset (a few pairs beyond do no harm: the pair kernel tests the
 // cut-off exactly).  One wave per BLS sites of cluster I, BPS = 64 / BLS lanes per site: lane s of a
@@ -546,6 +549,7 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // wave also records every bead's reference positions of its sites and writes their current
 // coordinates in the new order, as k_sites_vl does between builds.
 constexpr int BLS = 4, BPS = CLU / BLS;
+static_assert(BPS == VLG, "the build pads a site's last list group with one entry per lane");
 __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
                                                  const int* __restrict__ iperm, const double* __restrict__ site,
                                                  const double* __restrict__ q, int na, int nb, int nclu,
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
     int a = -3, n = 0;
     bool ovf = false;
     auto put = [&](int p, int j) {
-        if (p < cap) nl[((size_t)(p / VLS) * na + k) * VLS + (p & (VLS - 1))] = j;
+        if (p < cap) nl[((size_t)(p / VLG) * na + k) * VLG + (p % VLS) * 4 + (p / VLS) % 4] = j;
         else ovf = true;
     };
     auto reach2 = [&](float sj) {
@@ -658,12 +662,12 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
             n += __popcll(m);
         }
     }
-    if (kv && sl == 0) {
-        nl[((size_t)(cap / VLS) * na + k) * VLS + 0] = k;
-        nl[((size_t)(cap / VLS) * na + k) * VLS + 1] = k;
-        nl[((size_t)(cap / VLS) * na + k) * VLS + 2] = k;
-        nl[((size_t)(cap / VLS) * na + k) * VLS + 3] = k;
-        cnt[k] = min(n, cap);
+    if (kv) {
+        // the rest of the site's last group and the padding group hold k (BPS = VLG lanes per site)
+        const int nf = min(n, cap), pe = min((nf + VLG - 1) / VLG * VLG, cap);
+        if (nf + sl < pe) put(nf + sl, k);
+        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = k;
+        if (sl == 0) cnt[k] = nf;
     }
     if (ovf) ovfl[0] = 1;
 }
@@ -718,10 +722,12 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     if (k < na) {
         const size_t bo = (size_t)b * na;
         const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
-        const int* L = nl + (size_t)k * VLS + sl;
+        // this lane's groups of four entries: uint4 (g na + k) VLS + sl
+        const uint4* L = (const uint4*)nl + (size_t)k * VLS + sl;
         const size_t ls = (size_t)na * VLS;
         const int ntot = cnt[k], ne = nex[k];
         const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
+        const int ng = (nn + 3) / 4;                 // its groups
         const P4 xi = X[k];
         const T qi = (T)xi.w;
         const T Lt0 = (T)L0, Lt1 = (T)L1, Lt2 = (T)L2, it0 = (T)i0, it1 = (T)i1, it2 = (T)i2;
@@ -730,9 +736,10 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         T tg[3] = {0, 0, 0}, tv[7];
 #pragma unroll
         for (int c = 0; c < 7; c++) tv[c] = 0;
-        // past the lane's end: row cap / VLS, which holds the site itself (no branch around the load)
-        const int pad = cap / VLS;
-        auto ent = [&](int r) { return L[(size_t)(r < nn ? r : pad) * ls]; };
+        // past the lane's last group: group cap / VLG, which holds the site itself (no branch around
+        // the load)
+        const int pad = cap / VLG;
+        auto ent = [&](int g) { return L[(size_t)(g < ng ? g : pad) * ls]; };
         auto pair = [&](const P4& cur, int r) {
             T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
             if (UNIT) {
@@ -766,33 +773,28 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 tv[c] = 0;
             }
         };
-        int i0_ = ent(0), i1_ = ent(1), i2_ = ent(2), i3_ = ent(3);
-        P4 a0 = X[i0_], a1 = X[i1_], a2 = X[i2_], a3 = X[i3_];
-        int j0 = ent(4), j1 = ent(5), j2 = ent(6), j3 = ent(7);
-        for (int r = 0; r < nn; r += 8) {
-            const P4 b0 = X[j0], b1 = X[j1], b2 = X[j2], b3 = X[j3];
-            i0_ = ent(r + 8);
-            i1_ = ent(r + 9);
-            i2_ = ent(r + 10);
-            i3_ = ent(r + 11);
+        uint4 ia = ent(0);
+        P4 a0 = X[ia.x], a1 = X[ia.y], a2 = X[ia.z], a3 = X[ia.w];
+        uint4 ib = ent(1);
+        for (int g = 0; g < ng; g += 2) {
+            const int r = 4 * g;
+            const P4 b0 = X[ib.x], b1 = X[ib.y], b2 = X[ib.z], b3 = X[ib.w];
+            ia = ent(g + 2);
             pair(a0, r);
             pair(a1, r + 1);
             pair(a2, r + 2);
             pair(a3, r + 3);
-            if (r + 4 >= nn) break;
-            a0 = X[i0_];
-            a1 = X[i1_];
-            a2 = X[i2_];
-            a3 = X[i3_];
-            j0 = ent(r + 12);
-            j1 = ent(r + 13);
-            j2 = ent(r + 14);
-            j3 = ent(r + 15);
+            if (g + 1 >= ng) break;
+            a0 = X[ia.x];
+            a1 = X[ia.y];
+            a2 = X[ia.z];
+            a3 = X[ia.w];
+            ib = ent(g + 3);
             pair(b0, r + 4);
             pair(b1, r + 5);
             pair(b2, r + 6);
             pair(b3, r + 7);
-            if (!F64 && (r & 15) == 8) flush();
+            if (!F64 && (g & 3) == 2) flush();
         }
         flush();
 #pragma unroll
@@ -835,7 +837,7 @@ int vlist_cap(int na, Box sb, double rl) {
     if (c > na - 1) c = na - 1;
     int cap = (int)ceil(c);
     if (cap < 1) cap = 1;
-    return (cap + VLS - 1) / VLS * VLS;
+    return (cap + VLG - 1) / VLG * VLG;
 }
 
 void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
@@ -1075,7 +1077,19 @@ __global__ __launch_bounds__(1024) void k_sk_final(const P* __restrict__ skp, in
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
     const P* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
     double2 S = make_double2(0.0, 0.0);
-    for (int ch = 0; ch < nchunk; ch++) {
+    // chunks eight at a time: their loads are in flight together, the sum keeps the chunk order
+    int ch = 0;
+    for (; ch + 8 <= nchunk; ch += 8) {
+        P v[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = src[(size_t)(ch + i) * (KT * KT) + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            S.x += (double)v[i].x;
+            S.y += (double)v[i].y;
+        }
+    }
+    for (; ch < nchunk; ch++) {
         const P v = src[(size_t)ch * (KT * KT) + threadIdx.x];
         S.x += (double)v.x;
         S.y += (double)v.y;
@@ -1813,7 +1827,8 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
                                             const int* __restrict__ nbl, int use_list, double rc2, double sigsq,
                                             double ba, double bb, double bc, int buck, double vs, double ds,
                                             ChainArgs ch, unsigned long long* __restrict__ dmax,
-                                            double* __restrict__ grad, double* __restrict__ parts) {
+                                            double* __restrict__ grad, double* __restrict__ parts,
+                                            double* __restrict__ lj) {
     const int b = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int ii = t / OOL, sl = t % OOL;
@@ -1888,11 +1903,18 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
         gz += __shfl_xor(gz, off, 64);
     }
     if (ii < n && !ch.parts && sl == 0) {
-        const int i = 3 * ii;
-        double* G = grad + (size_t)b * 3 * na;
-        G[i] += gx;
-        G[na + i] += gy;
-        G[2 * na + i] += gz;
+        if (lj) {
+            // stored per molecule for the chain-rule kernel to add
+            lj[((size_t)b * 3 + 0) * n + ii] = gx;
+            lj[((size_t)b * 3 + 1) * n + ii] = gy;
+            lj[((size_t)b * 3 + 2) * n + ii] = gz;
+        } else {
+            const int i = 3 * ii;
+            double* G = grad + (size_t)b * 3 * na;
+            G[i] += gx;
+            G[na + i] += gy;
+            G[2 * na + i] += gz;
+        }
     }
     if (ii < n && ch.parts && sl < 3) {
         // k_chain for dimension sl of the molecule, then the oxygen's O-O gradient
@@ -1922,7 +1944,8 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
 template <typename T>
 void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const int* nbl, int use_list, double rcut,
                const rpmd_params& p, double* grad, double* parts, hipStream_t s, const double* chain_parts,
-               int chain_npart, double chain_alpha, double chain_scale, int chain_acc, unsigned long long* dmax) {
+               int chain_npart, double chain_alpha, double chain_scale, int chain_acc, unsigned long long* dmax,
+               double* lj) {
     ChainArgs ch;
     ch.parts = chain_parts;
     ch.npart = chain_npart;
@@ -1941,14 +1964,14 @@ void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const
     const double vs = buck ? 1.0 : 4.0 * p.oo_eps, ds = buck ? 1.0 : 48.0 * p.oo_eps;
     k_oo<T><<<dim3(oo_blocks(na), nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
                                                     p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, ch, dmax, grad,
-                                                    parts);
+                                                    parts, lj);
 }
 template void launch_oo<double>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
                                 double*, double*, hipStream_t, const double*, int, double, double, int,
-                                unsigned long long*);
+                                unsigned long long*, double*);
 template void launch_oo<float>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
                                double*, double*, hipStream_t, const double*, int, double, double, int,
-                               unsigned long long*);
+                               unsigned long long*, double*);
 int oo_blocks(int na) { return (na / 3 * OOL + 255) / 256; }
 
 // TEM switched short-range Coulomb between different molecules (src/correct_tem.f90:73-177);

@@ -325,6 +325,32 @@ def test_device_resident_steps_match_oracle(capi, oracle):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_stream_layouts_give_identical_steps(capi, prec):
+    """Device-resident PILE-L steps (the fused step: O-O Verlet list, both Ewald branches,
+    intramolecular forces) give the same bits on one, two or three streams, launched eagerly or
+    replayed from a graph: the streams change only where kernels run, not what they add."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 8, ecut=1e-6, intra_model=cf.INTRA_SPCFW,
+                                           therm="PRO", ttaufs=100.0)
+    na, nb = r.shape[1], r.shape[0]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    p = np.random.default_rng(11).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    out = []
+    for streams, graph in [(1, False), (2, True), (3, False), (3, True)]:
+        print(f"case streams {streams} graph {graph} {prec}", flush=True)
+        ctx = capi.RPMDContext(prm, na, nb, prec)
+        ctx.set_streams(streams)
+        ctx.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=11)
+        ctx.use_graph(graph)
+        ctx.run_steps(6)
+        out.append(ctx.state_download())
+        ctx.close()
+    for g in out[1:]:
+        assert np.array_equal(g["r"], out[0]["r"])
+        assert np.array_equal(g["p"], out[0]["p"])
+        assert g["v"] == out[0]["v"]
+
+
 def test_contracted_mts_noncubic_matches_oracle(capi, oracle):
     """Config-4 form at small size: q-SPC/Fw in a 1:1:1.2 box (non-cubic Ewald), 64 beads, mts 4,
     Ewald on 7 contracted modes and LJ on 15 (odd, as src/evolve_pi.f90:344-347 requires): two
