@@ -1450,20 +1450,25 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the twin) give the three gradient components,
 // the quantities of the Horner pass.  One block per (64 atoms, bead); wave w owns rows
 // 16 w .. 16 w + 15 of [G; kz G], which it alone reads: its A operand comes from global memory
-// straight into registers, two kx ahead in two register sets, so the kx loop has no barrier.  B
-// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS; B and this lane's Y^ky are then
+// straight into registers, four kx ahead in four register sets, so the kx loop has no barrier.  B
+// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS, with the tile scales; B is then
 // held in registers.  Each lane sums its part of the three components over kx in FP64; lanes and
-// waves are combined once at the end, in fixed order.
+// waves are combined once at the end, in fixed order.  Blocks are dealt to the XCDs by bead.
 __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
-                                                    int na, KDev kd, const uint4* __restrict__ ga,
+                                                    int na, int nbk, KDev kd, const uint4* __restrict__ ga,
                                                     const double* __restrict__ gks, double* __restrict__ kpart) {
-    const int a0 = blockIdx.x * KFH, b = blockIdx.y;
+    // blocks are dealt to the eight XCDs in turn (block id mod 8): XCD x takes beads x, x + 8, ... one
+    // after the other, so the blocks resident on an XCD read the G tiles of one bead from its L2
+    const int nab = (na + KFH - 1) / KFH, m8 = blockIdx.x >> 3;
+    const int b = (blockIdx.x & 7) + 8 * (m8 / nab), a0 = (m8 % nab) * KFH;
+    if (b >= nbk) return;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int kmax = kd.kmax;
     __shared__ uint32_t Zh[KFH][HS], Zl[KFH][HS];   // conj(Z_j(kz)) x 2^10 [atom][kz]
     __shared__ float2 Ys[KT][KFH + 2];              // Y_j(ky) [ky][atom]
     __shared__ float2 Xs[KH + 1][KFH];              // q X_j^kx [kx][atom]
     __shared__ double red[4][KFH][2];               // per wave: its two sums per atom
+    __shared__ double gs[KH + 1];                   // 1 / (tile scale 2^10) per kx
     constexpr double SC = 1024.0;
     const double2* PX = ph + ((size_t)b * 3 + 0) * na;
     const double2* PY = ph + ((size_t)b * 3 + 1) * na;
@@ -1496,28 +1501,25 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
             xk = cmul(xk, x1);
         }
     }
+    if (t <= kmax) gs[t] = gks[(size_t)b * (kmax + 1) + t];
     __syncthreads();
     const int mb = w & 1, wz = w >> 1;  // row block of G, kz-weighted twin
     const int r0 = KT * wz + 16 * mb;
     // A operand of lane (li, lk) for kx: rows r0 + li, dwords 16 ks + 4 lk .. + 3, hi and lo
     const uint4* GA = ga + (size_t)b * (kmax + 1) * (KT * KT) + (r0 + li) * 8 + lk;
-    // the kx-invariant operands in registers for the whole kx loop: B (conj Z, split) of the four
-    // 16-atom tiles and Y^ky of this lane's rows iy = 16 mb + 4 lk + v and atoms 16 n + li
+    // the kx-invariant B operand (conj Z, split) of the four 16-atom tiles in registers for the whole
+    // kx loop
     uint4 bzh[4][2], bzl[4][2];
-    float2 yv[4][4];
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
+    for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             bzh[n][ks] = *(const uint4*)&Zh[16 * n + li][16 * ks + 4 * lk];
             bzl[n][ks] = *(const uint4*)&Zl[16 * n + li][16 * ks + 4 * lk];
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++) yv[n][v] = Ys[16 * mb + 4 * lk + v][16 * n + li];
-    }
     double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
+    // a kx without members has a zero tile: its sums add zero
     auto body = [&](int kx, uint4 ah0, uint4 al0, uint4 ah1, uint4 al1) {
-        if (kd.kyb[kx] < 0) return;  // uniform: no member at this kx
         flt4 ur[4], ui[4];
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
@@ -1539,7 +1541,7 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
                 ui[n] = mfma_h16(a2l, bh, ui[n]);
             }
         }
-        const double us = gks[(size_t)b * (kmax + 1) + kx];
+        const double us = gs[kx];
         const double wa = wz ? us : us * kx;  // x: kx-weighted plain sum; z: the twin's plain sum
         // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
 #pragma unroll
@@ -1550,7 +1552,7 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int iy = 16 * mb + 4 * lk + v;
-                const float2 y = yv[n][v];
+                const float2 y = Ys[iy][j];
                 const float pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;  // q X^kx Y^ky
                 const float ipu = pr * ui[n][v] - pi * ur[n][v];                      // Im(conj(p) U)
                 s0 += ipu;
@@ -1560,26 +1562,31 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
             sb[n] += us * (double)s1;
         }
     };
-    uint4 ah0 = GA[0], ah1 = GA[4], al0 = GA[512], al1 = GA[516];
-    uint4 bh0, bh1, bl0, bl1;
-    for (int kx = 0; kx <= kmax; kx += 2) {
-        if (kx + 1 <= kmax) {
-            const uint4* g = GA + (size_t)(kx + 1) * (KT * KT);
-            bh0 = g[0];
-            bh1 = g[4];
-            bl0 = g[512];
-            bl1 = g[516];
-        }
-        body(kx, ah0, al0, ah1, al1);
+    // A rows four kx ahead in four register sets (loads past kmax re-read kmax's tile, unused)
+    auto load = [&](int kx, uint4 (&A)[4]) {
+        const uint4* g = GA + (size_t)min(kx, kmax) * (KT * KT);
+        A[0] = g[0];
+        A[1] = g[512];
+        A[2] = g[4];
+        A[3] = g[516];
+    };
+    uint4 A0[4], A1[4], A2[4], A3[4];
+    load(0, A0);
+    load(1, A1);
+    load(2, A2);
+    load(3, A3);
+    for (int kx = 0; kx <= kmax; kx += 4) {
+        body(kx, A0[0], A0[1], A0[2], A0[3]);
+        load(kx + 4, A0);
         if (kx + 1 > kmax) break;
-        if (kx + 2 <= kmax) {
-            const uint4* g = GA + (size_t)(kx + 2) * (KT * KT);
-            ah0 = g[0];
-            ah1 = g[4];
-            al0 = g[512];
-            al1 = g[516];
-        }
-        body(kx + 1, bh0, bl0, bh1, bl1);
+        body(kx + 1, A1[0], A1[1], A1[2], A1[3]);
+        load(kx + 5, A1);
+        if (kx + 2 > kmax) break;
+        body(kx + 2, A2[0], A2[1], A2[2], A2[3]);
+        load(kx + 6, A2);
+        if (kx + 3 > kmax) break;
+        body(kx + 3, A3[0], A3[1], A3[2], A3[3]);
+        load(kx + 7, A3);
     }
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -1663,8 +1670,8 @@ int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet&
                   const double* gks, double* kpart, int fp32, hipStream_t s) {
     const KDev kd = make_kdev(ks);
     if (fp32) {
-        k_kforce_h16<<<dim3((na + KFH - 1) / KFH, nb), 256, 0, s>>>((const double2*)ph, q, na, kd, (const uint4*)gk,
-                                                                   gks, kpart);
+        k_kforce_h16<<<(na + KFH - 1) / KFH * ((nb + 7) / 8 * 8), 256, 0, s>>>((const double2*)ph, q, na, nb, kd,
+                                                                              (const uint4*)gk, gks, kpart);
         return 1;
     }
     k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
