@@ -419,44 +419,28 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     ReduceJobs jobs;
     int use_list = 0;
     unsigned long long* oo_dmax = nullptr;
-    // device-resident calls with the Ewald sum: the O-O list and pair sums run on the third stream from
-    // here, their gradient per molecule (goo) added by the chain-rule kernel after the join
-    const bool fork3 = need_oo && iopt <= 2 && c->streams > 2 && !check_lists && !c->prof && !main_st;
-    hipStream_t so = s;
-    if (fork3) {
-        HIPCHK(hipEventRecord(c->ev_fork3, s));
-        HIPCHK(hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
-        so = c->st3;
-    }
-    if (need_oo && nb > 3) {
-        // centroid O-O list (src/md_forces.f90:34-53)
-        const double cut = std::min(P.rcut, 0.5 * boxmin);
+    const double oo_cut = std::min(P.rcut, 0.5 * boxmin);
+    // device-resident calls: a Verlet list of the O-O centroids within cut + 1 A, rebuilt when an
+    // oxygen centroid has moved by about 0.5 A, from which the pairs within cut are taken
+    auto oo_vlist = [&](hipStream_t so) {
+        const double skin = 1.0 / 0.5291772108;
+        const double key[5] = {box[0], box[1], box[2], oo_cut, skin};
+        const int force = (!c->oo_built || memcmp(key, c->oo_key, sizeof key) != 0) ? 1 : 0;
+        memcpy(c->oo_key, key, sizeof key);
+        c->oo_built = true;
+        launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, so);
+        launch_nlist_vl(c->cen, na, bx, oo_cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
+                        c->nnb, c->nbl, so);
         use_list = 1;
-        if (check_lists) {
-            // the list of this call, and lj_driver's fallback when it is empty (src/ljones.f90:14-26)
-            launch_centroid(pos, c->cen, na, nb, s);
-            launch_zero_int(c->ovf, 2, s);
-            launch_nlist(c->cen, na, 3, bx, cut, c->nnb, c->nbl, c->ovf, s);
-            int h[2];
-            HIPCHK(hipMemcpyAsync(h, c->ovf, sizeof h, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            if (h[0] > 0) return RPMD_ERR_NEIGH_OVERFLOW;
-            if (h[1] == 0) use_list = 0;
-        } else {
-            // device-resident calls: a Verlet list of the centroids within cut + 1 A, rebuilt when an
-            // oxygen centroid has moved by about 0.5 A, from which the pairs within cut are taken
-            const double skin = 1.0 / 0.5291772108;
-            const double key[5] = {box[0], box[1], box[2], cut, skin};
-            const int force = (!c->oo_built || memcmp(key, c->oo_key, sizeof key) != 0) ? 1 : 0;
-            memcpy(c->oo_key, key, sizeof key);
-            c->oo_built = true;
-            launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, so);
-            launch_nlist_vl(c->cen, na, bx, cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
-                            c->nnb, c->nbl, so);
-            oo_dmax = c->oo_dmax;  // cleared by the O-O kernel
-        }
-    }
-    if (fork3) {
+        oo_dmax = c->oo_dmax;  // cleared by the O-O kernel
+    };
+    // device-resident calls with the Ewald sum: the O-O list and pair sums run beside the k-space
+    // kernels, after the real-space pairs on the second stream (or from here on the third), their
+    // gradient per molecule (goo) added by the chain-rule kernel after the join
+    const bool oo_branch = need_oo && iopt <= 2 && c->streams > 1 && !check_lists && !c->prof && !main_st;
+    const bool fork3 = oo_branch && c->streams > 2;
+    auto oo_work = [&](hipStream_t so) {
+        if (nb > 3) oo_vlist(so);
         double* po = c->parts + c->off_oo;
         if (mixed)
             launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, so, nullptr, 0, 1.0, 1.0,
@@ -464,7 +448,29 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         else
             launch_oo<double>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, so, nullptr, 0, 1.0,
                               1.0, 0, oo_dmax, c->goo);
-        HIPCHK(hipEventRecord(c->ev_join3, so));
+    };
+    if (fork3) {
+        HIPCHK(hipEventRecord(c->ev_fork3, s));
+        HIPCHK(hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
+        oo_work(c->st3);
+        HIPCHK(hipEventRecord(c->ev_join3, c->st3));
+    }
+    if (need_oo && nb > 3 && !oo_branch) {
+        // centroid O-O list (src/md_forces.f90:34-53)
+        if (check_lists) {
+            // the list of this call, and lj_driver's fallback when it is empty (src/ljones.f90:14-26)
+            use_list = 1;
+            launch_centroid(pos, c->cen, na, nb, s);
+            launch_zero_int(c->ovf, 2, s);
+            launch_nlist(c->cen, na, 3, bx, oo_cut, c->nnb, c->nbl, c->ovf, s);
+            int h[2];
+            HIPCHK(hipMemcpyAsync(h, c->ovf, sizeof h, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (h[0] > 0) return RPMD_ERR_NEIGH_OVERFLOW;
+            if (h[1] == 0) use_list = 0;
+        } else {
+            oo_vlist(s);
+        }
     }
     int chain_npart = 0, chain_acc = 0;
     double chain_scale = 1.0;
@@ -510,6 +516,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             else
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
         }
+        if (oo_branch && !fork3) oo_work(c->st2);
         if (fork) HIPCHK(hipEventRecord(c->ev_join, c->st2));
         double* pk = c->parts + c->off_k;
         {
@@ -533,8 +540,8 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         chain_npart = ewald_real_slots() + nkslots;
         chain_scale = sc * sc;
         chain_acc = gacc;
-        if (fork3) {
-            HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
+        if (oo_branch) {
+            if (fork3) HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
             launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo);
         } else if (!need_oo) {
             launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
@@ -544,7 +551,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
         jobs.add(pk, nb * (P.kmax + 1), 10, sc / nb);
     }
-    if (need_oo && fork3) {
+    if (need_oo && oo_branch) {
         jobs.add(c->parts + c->off_oo, nb * oo_blocks(na), 20, 1.0 / nb);
     } else if (need_oo) {
         double* po = c->parts + c->off_oo;
