@@ -740,7 +740,9 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         // the load)
         const int pad = cap / VLG;
         auto ent = [&](int g) { return L[(size_t)(g < ng ? g : pad) * ls]; };
-        auto pair = [&](const P4& cur, int r) {
+        // a site has at most two own-molecule partners (three sites per molecule), listed first, so
+        // only a lane's first entry can be one: may_same is false for every other call
+        auto pair = [&](const P4& cur, int r, bool may_same) {
             T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
             if (UNIT) {
                 // the unit box (cubic): both forms reduce to this exactly
@@ -760,7 +762,8 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
             }
             const T r2 = ex * ex + ey * ey + ez * ez;
             const bool in = r < nn && r2 < trc2;
-            vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, r * VLS + sl < ne, ta, tsf, tg, tv);
+            vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, may_same && r * VLS + sl < ne, ta, tsf,
+                       tg, tv);
         };
         auto flush = [&]() {
             gx += (double)tg[0];
@@ -780,20 +783,20 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
             const int r = 4 * g;
             const P4 b0 = X[ib.x], b1 = X[ib.y], b2 = X[ib.z], b3 = X[ib.w];
             ia = ent(g + 2);
-            pair(a0, r);
-            pair(a1, r + 1);
-            pair(a2, r + 2);
-            pair(a3, r + 3);
+            pair(a0, r, g == 0);
+            pair(a1, r + 1, false);
+            pair(a2, r + 2, false);
+            pair(a3, r + 3, false);
             if (g + 1 >= ng) break;
             a0 = X[ia.x];
             a1 = X[ia.y];
             a2 = X[ia.z];
             a3 = X[ia.w];
             ib = ent(g + 3);
-            pair(b0, r + 4);
-            pair(b1, r + 5);
-            pair(b2, r + 6);
-            pair(b3, r + 7);
+            pair(b0, r + 4, false);
+            pair(b1, r + 5, false);
+            pair(b2, r + 6, false);
+            pair(b3, r + 7, false);
             if (!F64 && (g & 3) == 2) flush();
         }
         flush();
