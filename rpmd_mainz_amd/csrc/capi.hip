@@ -396,9 +396,13 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
 }
 
 // defer: append the energy reductions (outputs relative to c->ev_step) instead of launching them
-// main_st: run the call on this stream instead of the context stream (unforked calls only)
+// main_st: run the call on this stream instead of the context stream (unforked calls only).
+// intra_grad (with defer, iopt 1): also the intramolecular forces of pos into intra_grad, with their
+// energy in component 3 of the slot at c->ev_step (as a separate iopt 4 call there would); returns
+// 1 through *intra_done when it did so (on the second stream, after the O-O terms), else 0
 int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
-               bool check_lists, ReduceJobs* defer = nullptr, hipStream_t main_st = nullptr) {
+               bool check_lists, ReduceJobs* defer = nullptr, hipStream_t main_st = nullptr,
+               double* intra_grad = nullptr, int* intra_done = nullptr) {
     const rpmd_params& P = c->prm;
     const int na = c->na;
     hipStream_t s = main_st ? main_st : c->st;
@@ -516,7 +520,16 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             else
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
         }
-        if (oo_branch && !fork3) oo_work(c->st2);
+        if (oo_branch && !fork3) {
+            oo_work(c->st2);
+            if (intra_grad && defer && intra_done) {
+                double* pi = c->parts + c->off_intra;
+                launch_intra(pos, na, nb, intra_model(P), P, intra_grad, pi, 0, c->st2);
+                if (defer->n >= RED_JOBS) return RPMD_ERR_ARG;
+                defer->add(pi, nb * intra_blocks(na), 30, 1.0 / nb);
+                *intra_done = 1;
+            }
+        }
         if (fork) HIPCHK(hipEventRecord(c->ev_join, c->st2));
         double* pk = c->parts + c->off_k;
         {
@@ -885,16 +898,22 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
                     c->com_parts, s);
         launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
         ReduceJobs red;
-        // the intramolecular forces on the third stream, beside the other force kernels
+        // the intramolecular forces on the third stream, beside the other force kernels; on two
+        // streams they follow the O-O terms on the second (the same positions: rst becomes r)
         const bool intra3 = c->streams > 2 && !check && !c->prof;
+        const bool intra2 = c->streams == 2 && !check && !c->prof;
         if (intra3) {
             HIPCHK(hipEventRecord(c->ev_fork3, s));
             HIPCHK(hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
         }
-        RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red, intra3 ? c->st3 : nullptr));
+        if (!intra2)
+            RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red, intra3 ? c->st3 : nullptr));
         std::swap(c->r, c->rst);
         c->lf_merged = true;
-        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check, &red));
+        int intra_done = 0;
+        RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check, &red, nullptr,
+                          intra2 ? c->dvdr2 : nullptr, &intra_done));
+        if (intra2 && !intra_done) RETCHK(forces_dev(c, c->r, nb, box, 4, c->dvdr2, c->ev_step, check, &red));
         if (intra3) {
             HIPCHK(hipEventRecord(c->ev_join3, c->st3));
             HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));

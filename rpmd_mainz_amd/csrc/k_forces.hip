@@ -294,22 +294,25 @@ __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
 // bits; NaN reference positions of a list never built count as huge); and, when gather is set, the
 // site written at its sorted index iperm[] (the same for every bead) as (x, y, z, q) in FP64 and as
 // wrapped fractional FP32 coordinates (inverse box side f).  Per site also its bead centroid c and
-// spread s = max_b |x_b - c|, cu = [c_x, c_y, c_z, s][na], for the list build.  A block holds 32 sites
-// x 8 bead groups (thread: site t % 32, beads t / 32, t / 32 + 8, ...); the centroid sums of the
-// groups are added in fixed order.
+// spread s = max_b |x_b - c|, cu = [c_x, c_y, c_z, s][na], for the list build.  A block holds 256 / G
+// sites x G bead lanes (thread: site t % (256 / G), beads t / (256 / G) + G i); the centroid sums of
+// the lanes are added in fixed order.
 __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
                                                   int nb, double alpha, double s0, double s1, double s2, double l0,
                                                   double l1, double l2, double2* __restrict__ ph,
                                                   const double* __restrict__ q, const double* __restrict__ rb,
                                                   const int* __restrict__ iperm, int gather, double f0, double f1,
                                                   double f2, double4* __restrict__ srt, float4* __restrict__ srf,
-                                                  double* __restrict__ cu, unsigned long long* __restrict__ dmax) {
-    const int il = threadIdx.x & 31, g = threadIdx.x >> 5;
-    const int a = blockIdx.x * 32 + il;
+                                                  double* __restrict__ cu, unsigned long long* __restrict__ dmax,
+                                                  int G) {
+    // G bead lanes per site (8, 16 or 32), 256 / G sites per block; lane g takes beads g, g + G, ...
+    const int TA = 256 / G;
+    const int il = threadIdx.x % TA, g = threadIdx.x / TA;
+    const int a = blockIdx.x * TA + il;
     const bool av = a < na;
-    __shared__ double part[8][32][3];
+    __shared__ double part[256][3];
     __shared__ double cen[32][3];
-    __shared__ double red[8][32];
+    __shared__ double red[256];
     double m = 0.0, c[3] = {0.0, 0.0, 0.0};
     if (av) {
         const int m0 = a - a % 3;
@@ -317,7 +320,7 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
         const double qq = q[a];
         const int ka = gather ? iperm[a] : 0;
-        for (int b = g; b < nb; b += 8) {
+        for (int b = g; b < nb; b += G) {
             double x[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -349,20 +352,21 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         }
     }
 #pragma unroll
-    for (int d = 0; d < 3; d++) part[g][il][d] = c[d];
+    for (int d = 0; d < 3; d++) part[threadIdx.x][d] = c[d];
     __syncthreads();
     if (g == 0) {
+        // bead centroid: the lanes' sums in lane order
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double v = part[0][il][d];
-            for (int h = 1; h < 8; h++) v += part[h][il][d];
+            double v = part[il][d];
+            for (int h = 1; h < G; h++) v += part[h * TA + il][d];
             cen[il][d] = v / nb;
         }
     }
     __syncthreads();
     double s2m = 0.0;
     if (av)
-        for (int b = g; b < nb; b += 8) {
+        for (int b = g; b < nb; b += G) {
             double r2 = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -371,22 +375,21 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
             }
             s2m = fmax(s2m, r2);
         }
-    red[g][il] = s2m;
+    red[threadIdx.x] = s2m;
     __syncthreads();
     if (g == 0 && av) {
-        double v = red[0][il];
-        for (int h = 1; h < 8; h++) v = fmax(v, red[h][il]);
+        double v = red[il];
+        for (int h = 1; h < G; h++) v = fmax(v, red[h * TA + il]);
 #pragma unroll
         for (int d = 0; d < 3; d++) cu[(size_t)d * na + a] = cen[il][d];
         cu[(size_t)3 * na + a] = sqrt(v);
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = m;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0][0], red[0][1]),
-                                                                       fmax(red[0][2], red[0][3]))));
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
 // One block, when the gate is on: a counting sort of the site centroids (cu, from k_sites_vl) into
@@ -847,9 +850,12 @@ void launch_sites_vl(const double* pos, double* site, int na, int nb, double alp
                      const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
                      hipStream_t s) {
     const double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
-    k_sites_vl<<<(unsigned)((na + 31) / 32), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
-                                                          lat[2], (double2*)ph, q, v.rb, v.iperm, gather, sb.inv[0],
-                                                          sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu, v.dmax);
+    // bead lanes per site: enough sites per block that the grid fills the CUs, at most 32 per site
+    const int G = nb <= 8 ? 8 : (nb <= 16 ? 16 : 32), TA = 256 / G;
+    k_sites_vl<<<(unsigned)((na + TA - 1) / TA), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
+                                                              lat[2], (double2*)ph, q, v.rb, v.iperm, gather,
+                                                              sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu,
+                                                              v.dmax, G);
 }
 
 // The build runs when a site of any bead has moved by more than skin/2 since the last one (or force
@@ -1065,76 +1071,96 @@ __device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
 }
 
-// one block of 1024 threads per (kx, bead): thread = (iy, iz); P: the partial sums' type.  SPLIT
+// one block of 256 threads per (kx, bead), four (iy, iz) entries per thread: small blocks, so that the
+// kernel is not held back while the real-space pairs occupy the CUs beside it; P: the partial sums'
+// type.  SPLIT
 // (mixed mode): instead of G, the A operand of k_kforce_h16 -- (G_r, -G_i) s as split half2 rows,
 // rows 32..63 weighted by kz, hi then lo ([2][64][32] dwords per (bead, kx)) -- and 1 / (s 2^10) in
 // gks, where the power of two s brings the largest |component| of the tile to [2^10, 2^11)
 template <typename P, bool SPLIT>
-__global__ __launch_bounds__(1024) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
-                                                   const unsigned char* __restrict__ mask, double2* __restrict__ gk,
-                                                   double* __restrict__ gks, double* __restrict__ parts) {
+__global__ __launch_bounds__(256) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
+                                                  const unsigned char* __restrict__ mask, double2* __restrict__ gk,
+                                                  double* __restrict__ gks, double* __restrict__ parts) {
     const int kx = blockIdx.x, b = blockIdx.y, kmax = kd.kmax;
-    const int iy = threadIdx.x >> 5, iz = threadIdx.x & 31;
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
     const P* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
-    double2 S = make_double2(0.0, 0.0);
-    // chunks eight at a time: their loads are in flight together, the sum keeps the chunk order
+    // thread -> entries e = threadIdx.x + 256 i (iy = e / 32, iz = e % 32); each entry sums its chunks in
+    // chunk order, the loads of four entries and up to four chunks in flight together
+    double2 S[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) S[i] = make_double2(0.0, 0.0);
     int ch = 0;
-    for (; ch + 8 <= nchunk; ch += 8) {
-        P v[8];
+    for (; ch + 4 <= nchunk; ch += 4) {
+        P v[4][4];
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = src[(size_t)(ch + i) * (KT * KT) + threadIdx.x];
+        for (int c = 0; c < 4; c++)
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            S.x += (double)v[i].x;
-            S.y += (double)v[i].y;
+            for (int i = 0; i < 4; i++) v[c][i] = src[(size_t)(ch + c) * (KT * KT) + threadIdx.x + 256 * i];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                S[i].x += (double)v[c][i].x;
+                S[i].y += (double)v[c][i].y;
+            }
+    }
+    for (; ch < nchunk; ch++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const P v = src[(size_t)ch * (KT * KT) + threadIdx.x + 256 * i];
+            S[i].x += (double)v.x;
+            S[i].y += (double)v.y;
         }
-    }
-    for (; ch < nchunk; ch++) {
-        const P v = src[(size_t)ch * (KT * KT) + threadIdx.x];
-        S.x += (double)v.x;
-        S.y += (double)v.y;
-    }
-    double2 G = make_double2(0.0, 0.0);
-    if (mask[(kx * KT + iy) * KT + iz]) {
-        const int ky = iy - KO, kz = iz - KO;
-        const double rkx = kd.lat0 * kx, rky = kd.lat1 * ky, rkz = kd.lat2 * kz;
-        const double rk2 = rkx * rkx + rky * rky + rkz * rkz;
-        const double f = kx >= 1 ? 2.0 * kd.f0 : kd.f0;
-        const double w = (f / rk2) * exp(-kd.b * rk2);
-        const double et = w * (S.x * S.x + S.y * S.y);
-        const double term = 2.0 * (1.0 / rk2 + kd.b);
-        // The cubic reference sums the (+-ky, +-kz) members of one loop iteration into one et and
-        // multiplies by the positive rky, rkz (src/ewald_cubic.f90:362-371,415-430,488-503), so its
-        // off-diagonal virial uses |ky|, |kz|; the non-cubic sum keeps the signs
-        // (src/ewald_noncubic.f90:426-441).
-        const double rk[3] = {rkx, kd.cubic ? fabs(rky) : rky, kd.cubic ? fabs(rkz) : rkz};
-        acc[0] = et;
+    double2 G[4];
+    double m = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+    for (int i = 0; i < 4; i++) {
+        const int e = threadIdx.x + 256 * i, iy = e >> 5, iz = e & 31;
+        G[i] = make_double2(0.0, 0.0);
+        if (mask[(kx * KT + iy) * KT + iz]) {
+            const int ky = iy - KO, kz = iz - KO;
+            const double rkx = kd.lat0 * kx, rky = kd.lat1 * ky, rkz = kd.lat2 * kz;
+            const double rk2 = rkx * rkx + rky * rky + rkz * rkz;
+            const double f = kx >= 1 ? 2.0 * kd.f0 : kd.f0;
+            const double w = (f / rk2) * exp(-kd.b * rk2);
+            const double et = w * (S[i].x * S[i].x + S[i].y * S[i].y);
+            const double term = 2.0 * (1.0 / rk2 + kd.b);
+            // The cubic reference sums the (+-ky, +-kz) members of one loop iteration into one et and
+            // multiplies by the positive rky, rkz (src/ewald_cubic.f90:362-371,415-430,488-503), so its
+            // off-diagonal virial uses |ky|, |kz|; the non-cubic sum keeps the signs
+            // (src/ewald_noncubic.f90:426-441).
+            const double rk[3] = {rkx, kd.cubic ? fabs(rky) : rky, kd.cubic ? fabs(rkz) : rkz};
+            acc[0] += et;
 #pragma unroll
-            for (int r = 0; r < 3; r++) acc[1 + c * 3 + r] = et * (term * rk[r] * rk[c] - (r == c ? 1.0 : 0.0));
-        G = make_double2(2.0 * w * S.x, 2.0 * w * S.y);
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) acc[1 + c * 3 + r] += et * (term * rk[r] * rk[c] - (r == c ? 1.0 : 0.0));
+            G[i] = make_double2(2.0 * w * S[i].x, 2.0 * w * S[i].y);
+        }
+        m = fmax(m, fmax(fabs(G[i].x), fabs(G[i].y)));
     }
     if (!SPLIT) {
-        gk[(((size_t)b * (kmax + 1) + kx) * KT + iy) * KT + iz] = G;
+#pragma unroll
+        for (int i = 0; i < 4; i++) gk[((size_t)b * (kmax + 1) + kx) * (KT * KT) + threadIdx.x + 256 * i] = G[i];
     } else {
-        __shared__ double wmax[16];
-        double m = fmax(fabs(G.x), fabs(G.y));
+        __shared__ double wmax[4];
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
         __syncthreads();
-        m = wmax[0];
-        for (int i = 1; i < 16; i++) m = fmax(m, wmax[i]);
+        m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
         int ex = 0;
         frexp(m, &ex);
         const double sc = m > 0.0 ? ldexp(1.0, 11 - ex) : 1.0;
-        const float gr = (float)(sc * G.x), gi = (float)(sc * G.y), kz = (float)(iz - KO);
         uint32_t* o = (uint32_t*)gk + ((size_t)b * (kmax + 1) + kx) * (4 * KT * KT);
-        split_pair(gr, -gi, o[iy * KT + iz], o[2 * KT * KT + iy * KT + iz]);
-        split_pair(kz * gr, -(kz * gi), o[(KT + iy) * KT + iz], o[2 * KT * KT + (KT + iy) * KT + iz]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e = threadIdx.x + 256 * i, iy = e >> 5, iz = e & 31;
+            const float gr = (float)(sc * G[i].x), gi = (float)(sc * G[i].y), kz = (float)(iz - KO);
+            split_pair(gr, -gi, o[iy * KT + iz], o[2 * KT * KT + iy * KT + iz]);
+            split_pair(kz * gr, -(kz * gi), o[(KT + iy) * KT + iz], o[2 * KT * KT + (KT + iy) * KT + iz]);
+        }
         if (threadIdx.x == 0) gks[(size_t)b * (kmax + 1) + kx] = 1.0 / (sc * 1024.0);
     }
     block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
@@ -1662,10 +1688,10 @@ void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask,
     const dim3 grid(ks.kmax + 1, nb);
     int chunk = 0;
     if (fp32)
-        k_sk_final<float2, true><<<grid, 1024, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na, nb, &chunk), kd,
+        k_sk_final<float2, true><<<grid, 256, 0, s>>>((const float2*)sk_part, kspace_nchunk_h16(na, nb, &chunk), kd,
                                                        dmask, (double2*)gk, gks, parts_k);
     else
-        k_sk_final<double2, false><<<grid, 1024, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
+        k_sk_final<double2, false><<<grid, 256, 0, s>>>((const double2*)sk_part, kspace_nchunk(na), kd, dmask,
                                                          (double2*)gk, gks, parts_k);
 }
 // the force pass; returns the number of partial-gradient slots written (kx groups, or 1)
