@@ -491,17 +491,20 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         VList* vl = nullptr;
         int force = 0;
         RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
-        // sites, phases, list displacement and (for a built list) the sorted coordinates
-        launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, c->q, *vl, !force, sb, s);
-        // the real-space branch (list upkeep and pairs) runs on the second stream, beside the k-space
-        // kernels on the first, unless the lists are checked on the host or kernels are being timed
+        // the real-space branch (sites, list upkeep and pairs) runs on the second stream, beside the
+        // k-space kernels on the first, which then form the phases themselves, unless the lists are
+        // checked on the host or kernels are being timed
         fork = c->streams > 1 && !check_lists && !c->prof;
         hipStream_t sr = s;
         if (fork) {
             HIPCHK(hipEventRecord(c->ev_fork, s));
             HIPCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
             sr = c->st2;
+            launch_phases(pos, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, s);
         }
+        // sites, (phases,) list displacement and (for a built list) the sorted coordinates
+        launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, fork ? nullptr : c->ph, c->q, *vl, !force,
+                        sb, sr);
         {
             ProfScope ps(c, 4);
             launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, sr);

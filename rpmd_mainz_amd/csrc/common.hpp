@@ -108,6 +108,9 @@ int vlist_cap(int na, Box sb, double rl);
 void launch_sites_vl(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box,
                      const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
                      hipStream_t s);
+// the phase factors of launch_sites_vl alone (ph = nullptr there skips them)
+void launch_phases(const double* pos, int na, int nb, double alpha, int unit, Box box, const double lat[3], double* ph,
+                   hipStream_t s);
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, hipStream_t s);
 template <typename T>

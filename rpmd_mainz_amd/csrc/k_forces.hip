@@ -333,9 +333,11 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
                     x[d] = sc[d] * p[a];
                 }
                 site[o] = x[d];
-                double sn, cs;
-                sincos(lat[d] * x[d], &sn, &cs);
-                ph[o] = make_double2(cs, sn);
+                if (ph) {
+                    double sn, cs;
+                    sincos(lat[d] * x[d], &sn, &cs);
+                    ph[o] = make_double2(cs, sn);
+                }
                 c[d] += x[d];
             }
             const size_t o = (size_t)b * 3 * na + a;
@@ -856,6 +858,40 @@ void launch_sites_vl(const double* pos, double* site, int na, int nb, double alp
                                                               lat[2], (double2*)ph, q, v.rb, v.iperm, gather,
                                                               sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu,
                                                               v.dmax, G);
+}
+
+// The phase factors alone (the same values as k_sites_vl writes), thread per (site, bead): the k-space
+// branch computes them itself when it runs beside the real-space branch
+__global__ __launch_bounds__(256) void k_phases(const double* __restrict__ pos, int na, int nb, double alpha,
+                                                double s0, double s1, double s2, double l0, double l1, double l2,
+                                                double2* __restrict__ ph) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)na * nb) return;
+    const int b = (int)(i / na), a = (int)(i % na);
+    const int m0 = a - a % 3;
+    const double alpha2 = 0.5 * (1.0 - alpha);
+    const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double* p = pos + ((size_t)b * 3 + d) * na;
+        double x;
+        if (a == m0) {
+            const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
+            x = sc[d] * v;
+        } else {
+            x = sc[d] * p[a];
+        }
+        double sn, cs;
+        sincos(lat[d] * x, &sn, &cs);
+        ph[((size_t)b * 3 + d) * na + a] = make_double2(cs, sn);
+    }
+}
+void launch_phases(const double* pos, int na, int nb, double alpha, int unit, Box box, const double lat[3], double* ph,
+                   hipStream_t s) {
+    const double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
+    const size_t n = (size_t)na * nb;
+    k_phases<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos, na, nb, alpha, s0, s1, s2, lat[0], lat[1], lat[2],
+                                                         (double2*)ph);
 }
 
 // The build runs when a site of any bead has moved by more than skin/2 since the last one (or force
