@@ -402,7 +402,8 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
 // 1 through *intra_done when it did so (on the second stream, after the O-O terms), else 0
 int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int iopt, double* grad, double* ev_slot,
                bool check_lists, ReduceJobs* defer = nullptr, hipStream_t main_st = nullptr,
-               double* intra_grad = nullptr, int* intra_done = nullptr) {
+               double* intra_grad = nullptr, int* intra_done = nullptr, KickArgs* kick = nullptr,
+               int* kick_done = nullptr) {
     const rpmd_params& P = c->prm;
     const int na = c->na;
     hipStream_t s = main_st ? main_st : c->st;
@@ -558,7 +559,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         chain_acc = gacc;
         if (oo_branch) {
             if (fork3) HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
-            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo);
+            // with kick (iopt 1: grad is final here), the step's closing kick in the same launch
+            const bool fuse = kick && kick_done && iopt == 1;
+            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo,
+                              fuse ? kick : nullptr);
+            if (fuse) *kick_done = 1;
         } else if (!need_oo) {
             launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
         }
@@ -913,15 +918,23 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
             RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, check, &red, intra3 ? c->st3 : nullptr));
         std::swap(c->r, c->rst);
         c->lf_merged = true;
-        int intra_done = 0;
+        int intra_done = 0, kick_done = 0;
+        // the closing kicks fused into the chain-rule kernel when the intramolecular forces (dvdr2)
+        // are done before it, i.e. on the second stream
+        KickArgs kick;
+        kick.p = c->p;
+        kick.g1 = c->dvdr2;
+        kick.a1 = halfdtsmall;
+        kick.a2 = halfdt;
+        kick.ctr = c->step_ctr;
         RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step + (size_t)mts * 50, check, &red, nullptr,
-                          intra2 ? c->dvdr2 : nullptr, &intra_done));
+                          intra2 ? c->dvdr2 : nullptr, &intra_done, intra2 ? &kick : nullptr, &kick_done));
         if (intra2 && !intra_done) RETCHK(forces_dev(c, c->r, nb, box, 4, c->dvdr2, c->ev_step, check, &red));
         if (intra3) {
             HIPCHK(hipEventRecord(c->ev_join3, c->st3));
             HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
         }
-        launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
+        if (!(intra_done && kick_done)) launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
         launch_reduce_ev(red, c->ev_step, s);
         if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
         if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;

@@ -60,9 +60,18 @@ void launch_reduce_ev(const ReduceJobs& jobs, double* out, hipStream_t s);
 void launch_aos_to_soa(const double* aos, double* soa, int na, int nb, hipStream_t s);
 void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream_t s);
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
-// lj (optional): the oxygens' O-O gradient per molecule, [bead][dim][molecule], added to the oxygens
+// the closing momentum kick of a step, p = (p - a1 g1) - a2 grad, fused into the chain-rule kernel that
+// writes grad (ctr: the noise counter it advances)
+struct KickArgs {
+    double* p = nullptr;
+    const double* g1 = nullptr;
+    double a1 = 0.0, a2 = 0.0;
+    uint64_t* ctr = nullptr;
+};
+// lj (optional): the oxygens' O-O gradient per molecule, [bead][dim][molecule], added to the oxygens;
+// kick (optional): then the kick of the atoms
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       int acc, hipStream_t s, const double* lj = nullptr);
+                       int acc, hipStream_t s, const double* lj = nullptr, const KickArgs* kick = nullptr);
 void launch_centroid(const double* pos, double* cen, int na, int nb, hipStream_t s);
 void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const double* cb, unsigned long long* dmax,
                         hipStream_t s);

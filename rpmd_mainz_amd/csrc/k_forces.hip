@@ -114,11 +114,16 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
 
 // grad (+)= scale * chain rule of the M-site (src/md_forces.f90:206-215) applied to the site gradient,
 // the sum of npart partial site gradients (parts[p*pstride + ...]) taken in fixed order
-// lj (optional): the oxygens' O-O gradient per molecule [bead][dim][molecule], added last
+// lj (optional): the oxygens' O-O gradient per molecule [bead][dim][molecule], added last.
+// kp (optional): then the momentum kick of the three atoms, p = (p - a1 g1) - a2 grad with each
+// product and difference rounded as k_kick does, and the noise counter's step advance
 __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart, size_t pstride, int na,
-                        int nb, double alpha, double scale, int acc, const double* __restrict__ lj) {
+                        int nb, double alpha, double scale, int acc, const double* __restrict__ lj,
+                        double* __restrict__ kp, const double* __restrict__ kg1, double ka1, double ka2,
+                        uint64_t* __restrict__ ctr) {
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctr && i == 0) *ctr += RPMD_STREAMS_PER_STEP;
     if (i >= (size_t)nm * nb) return;
     size_t b = i / nm, m = i % nm;
     double alpha2 = 0.5 * (1.0 - alpha);
@@ -133,17 +138,31 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ pa
         }
         double g0 = scale * s0, g1 = scale * s1, g2 = scale * s2;
         const double b0 = acc ? grad[o] : 0.0, b1 = acc ? grad[o + 1] : 0.0, b2 = acc ? grad[o + 2] : 0.0;
-        grad[o] = lj ? (b0 + alpha * g0) + lj[(b * 3 + d) * nm + m] : b0 + alpha * g0;
-        grad[o + 1] = b1 + (g1 + alpha2 * g0);
-        grad[o + 2] = b2 + (g2 + alpha2 * g0);
+        const double r0 = lj ? (b0 + alpha * g0) + lj[(b * 3 + d) * nm + m] : b0 + alpha * g0;
+        const double r1 = b1 + (g1 + alpha2 * g0), r2 = b2 + (g2 + alpha2 * g0);
+        grad[o] = r0;
+        grad[o + 1] = r1;
+        grad[o + 2] = r2;
+        if (kp) {
+            const double rr[3] = {r0, r1, r2};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double v = kp[o + k];
+                v = __dadd_rn(v, -__dmul_rn(ka1, kg1[o + k]));
+                v = __dadd_rn(v, -__dmul_rn(ka2, rr[k]));
+                kp[o + k] = v;
+            }
+        }
     }
 }
 // acc = 0: the first writer of grad in a forces call (stores instead of adding)
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
-                       int acc, hipStream_t s, const double* lj) {
+                       int acc, hipStream_t s, const double* lj, const KickArgs* kick) {
     size_t n = (size_t)(na / 3) * nb;
     k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale,
-                                                       acc, lj);
+                                                       acc, lj, kick ? kick->p : nullptr, kick ? kick->g1 : nullptr,
+                                                       kick ? kick->a1 : 0.0, kick ? kick->a2 : 0.0,
+                                                       kick ? kick->ctr : nullptr);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
