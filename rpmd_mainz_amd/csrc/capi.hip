@@ -322,7 +322,7 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
 void free_vlist(VList& v) {
     void* bufs[] = {v.perm, v.iperm, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf, v.cu, v.cs, v.bb, v.ccnt, v.clist,
-                    v.gate, v.dmax};
+                    v.gate, v.dmax, v.nreb};
     for (void* b : bufs)
         if (b) hipFree(b);
     v = VList{};
@@ -376,6 +376,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.clist, (size_t)nclu * nclu);
         e |= dalloc(&v.gate, 2);
         e |= dalloc(&v.dmax, 1);
+        e |= dalloc(&v.nreb, 1);
         if (e) {
             free_vlist(v);
             return RPMD_ERR_NOMEM;
@@ -385,6 +386,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         HIPCHK(hipMemset(v.rb, 0xff, sizeof(double) * 3 * ns));
         HIPCHK(hipMemset(v.gate, 0, sizeof(int) * 2));
         HIPCHK(hipMemset(v.dmax, 0, sizeof(unsigned long long)));
+        HIPCHK(hipMemset(v.nreb, 0, sizeof(int)));
         x.v = v;
         x.alloc = true;
         *force = 1;

@@ -88,6 +88,7 @@ struct VGate {
 };
 // Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
 // each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads
+#define VL_RESORT 8  // list builds per centroid sort (the others keep the last order)
 #define VLS 4   // lanes per site in the Verlet-list walk
 #define VLG 16  // entries per list group: VLS lanes x 4 consecutive entries of each
 struct VList {
@@ -107,6 +108,7 @@ struct VList {
     int* clist;      // [nclu][nclu] cluster-pair lists
     int* gate;       // [2]: [1] list overflow (sticky)
     unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
+    int* nreb;       // [1] builds so far: the sites are sorted again on every VL_RESORT-th
     int cap;
 };
 // site blocks of 256 / VLS sites per bead, padded to a multiple of 8 (the XCD mapping of k_ewald_vl)

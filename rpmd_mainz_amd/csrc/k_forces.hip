@@ -426,13 +426,30 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
                                                   int* __restrict__ start, int* __restrict__ perm,
                                                   int* __restrict__ iperm, const double* __restrict__ cu,
                                                   float4* __restrict__ cs, double* __restrict__ bb,
-                                                  int* __restrict__ ccnt, int* __restrict__ clist, VGate gate) {
+                                                  int* __restrict__ ccnt, int* __restrict__ clist, VGate gate,
+                                                  int* __restrict__ nreb) {
     if (!gate.on()) return;
     const int t = threadIdx.x;
     constexpr int NT = 1024;
     int* cn = count;
     int* st = start;
-    for (int a = t; a < na; a += NT) {
+    // the sites are sorted again on every VL_RESORT-th build (and a forced one); between, the last
+    // order is kept (any order gives the same pair set; the sort only keeps neighbours close)
+    __shared__ int resort;
+    if (t == 0) {
+        const int k = nreb[0];
+        resort = gate.force || k % VL_RESORT == 0;
+        nreb[0] = k + 1;
+    }
+    __syncthreads();
+    if (!resort) {
+        for (int x = t; x < na; x += NT) {
+            const int a = perm[x];
+            cs[x] = make_float4((float)wrap01(cu[a] * i0), (float)wrap01(cu[na + a] * i1),
+                                (float)wrap01(cu[2 * na + a] * i2), (float)cu[3 * na + a]);
+        }
+    }
+    for (int a = t; a < na && resort; a += NT) {
         const int cx = cell_coord(cu[a], i0, g.nc[0]);
         const int cy = cell_coord(cu[na + a], i1, g.nc[1]);
         const int cz = cell_coord(cu[2 * na + a], i2, g.nc[2]);
@@ -446,7 +463,7 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
     __shared__ int carry;
     if (t == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < nbins; base += NT) {
+    for (int base = 0; base < nbins && resort; base += NT) {
         const int i = base + t;
         const int v = i < nbins ? cn[i] : 0;
         buf[t] = v;
@@ -465,15 +482,15 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
         if (t == NT - 1) carry += buf[NT - 1];
         __syncthreads();
     }
-    if (t == 0) st[nbins] = carry;
+    if (t == 0 && resort) st[nbins] = carry;
     __syncthreads();
-    for (int a = t; a < na; a += NT) {
+    for (int a = t; a < na && resort; a += NT) {
         const int cid = cell_of[a];
         perm[st[cid] + atomicAdd(&cn[cid], 1)] = a;
     }
     __syncthreads();
     // order each cell by site index; sorted wrapped centroids and spreads
-    for (int c = t; c < nbins; c += NT) {
+    for (int c = t; c < nbins && resort; c += NT) {
         cn[c] = 0;
         const int s0 = st[c], s1 = st[c + 1];
         for (int x = s0 + 1; x < s1; x++) {
@@ -925,7 +942,8 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
     k_vl_sort<<<1, 1024, 0, s>>>(na, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2], rl, nclu,
-                                 cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist, gate);
+                                 cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist, gate,
+                                 v.nreb);
     const float mrg = (float)(1.0 + pair_margin(sb, rl));
     k_vl_build<<<(na + BLS - 1) / BLS, 64, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist,
                                                   (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0],
