@@ -559,23 +559,35 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         chain_npart = ewald_real_slots() + nkslots;
         chain_scale = sc * sc;
         chain_acc = gacc;
+        // energies: real and k parts into the Ewald slot (two reductions summed on the host)
+        jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
+        jobs.add(pk, nb * (P.kmax + 1), 10, sc / nb);
         if (oo_branch) {
             if (fork3) HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
-            // with kick (iopt 1: grad is final here), the step's closing kick in the same launch
+            jobs.add(c->parts + c->off_oo, nb * oo_blocks(na), 20, 1.0 / nb);
+            // with kick (iopt 1: grad is final here), the step's closing kick in the same launch, and
+            // with defer the step's energy reductions (every part is written by now) as extra blocks
             const bool fuse = kick && kick_done && iopt == 1;
+            if (fuse && defer) {
+                const int base = (int)(ev_slot - c->ev_step);
+                for (int j = 0; j < jobs.n; j++) {
+                    if (defer->n >= RED_JOBS) return RPMD_ERR_ARG;
+                    defer->add(jobs.parts[j], jobs.nparts[j], base + jobs.out[j], jobs.scale[j]);
+                }
+                jobs.n = 0;
+                kick->red = defer;
+                kick->ev_out = c->ev_step;
+            }
             launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo,
                               fuse ? kick : nullptr);
-            if (fuse) *kick_done = 1;
+            if (fuse) *kick_done = defer ? 2 : 1;
         } else if (!need_oo) {
             launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
         }
         gacc = 1;
-        // energies: real and k parts into the Ewald slot (two reductions summed on the host)
-        jobs.add(pr, nb * ewald_real_parts(na), 0, sc / nb);
-        jobs.add(pk, nb * (P.kmax + 1), 10, sc / nb);
     }
     if (need_oo && oo_branch) {
-        jobs.add(c->parts + c->off_oo, nb * oo_blocks(na), 20, 1.0 / nb);
+        // its job was added with the chain rule
     } else if (need_oo) {
         double* po = c->parts + c->off_oo;
         const double* chp = chain_npart ? c->fpart : nullptr;
@@ -937,7 +949,8 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
             HIPCHK(hipStreamWaitEvent(s, c->ev_join3, 0));
         }
         if (!(intra_done && kick_done)) launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
-        launch_reduce_ev(red, c->ev_step, s);
+        // kick_done == 2: the reductions ran with the chain rule (every job was queued by then)
+        if (!(intra_done && kick_done == 2)) launch_reduce_ev(red, c->ev_step, s);
         if (nz.stream_add > RPMD_STREAMS_PER_STEP) return RPMD_ERR_ARG;
         if (hipGetLastError() != hipSuccess) return RPMD_ERR_HIP;
         return 0;

@@ -62,11 +62,15 @@ void launch_soa_to_aos(const double* soa, double* aos, int na, int nb, hipStream
 void launch_sites(const double* pos, double* site, int na, int nb, double alpha, int unit, Box box, hipStream_t s);
 // the closing momentum kick of a step, p = (p - a1 g1) - a2 grad, fused into the chain-rule kernel that
 // writes grad (ctr: the noise counter it advances)
+struct ReduceJobs;
 struct KickArgs {
     double* p = nullptr;
     const double* g1 = nullptr;
     double a1 = 0.0, a2 = 0.0;
     uint64_t* ctr = nullptr;
+    // (optional) energy reductions run by extra blocks of the same launch (launch_reduce_ev)
+    const ReduceJobs* red = nullptr;
+    double* ev_out = nullptr;
 };
 // lj (optional): the oxygens' O-O gradient per molecule, [bead][dim][molecule], added to the oxygens;
 // kick (optional): then the kick of the atoms

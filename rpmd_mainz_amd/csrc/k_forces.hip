@@ -37,10 +37,9 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out)
 }
 
 // out[k] = scale * sum_p parts[p*10 + k], k < 10, for up to RED_JOBS part ranges in one launch; one
-// block per (k, job), fixed order
-__global__ __launch_bounds__(256) void k_reduce_parts(ReduceJobs jobs, double* out) {
+// block of 256 threads per (k, job), fixed order
+__device__ __forceinline__ void reduce_parts_block(const ReduceJobs& jobs, double* out, int k, int jb) {
     __shared__ double red[256];
-    const int k = blockIdx.x, jb = blockIdx.y;
     const double* parts = jobs.parts[jb];
     const int nparts = jobs.nparts[jb];
     double s = 0.0;
@@ -52,6 +51,9 @@ __global__ __launch_bounds__(256) void k_reduce_parts(ReduceJobs jobs, double* o
         __syncthreads();
     }
     if (threadIdx.x == 0) out[jobs.out[jb] + k] = jobs.scale[jb] * red[0];
+}
+__global__ __launch_bounds__(256) void k_reduce_parts(ReduceJobs jobs, double* out) {
+    reduce_parts_block(jobs, out, blockIdx.x, blockIdx.y);
 }
 
 void launch_reduce_ev(const ReduceJobs& jobs, double* out, hipStream_t s) {
@@ -117,10 +119,19 @@ void launch_sites(const double* pos, double* site, int na, int nb, double alpha,
 // lj (optional): the oxygens' O-O gradient per molecule [bead][dim][molecule], added last.
 // kp (optional): then the momentum kick of the three atoms, p = (p - a1 g1) - a2 grad with each
 // product and difference rounded as k_kick does, and the noise counter's step advance
-__global__ void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart, size_t pstride, int na,
-                        int nb, double alpha, double scale, int acc, const double* __restrict__ lj,
-                        double* __restrict__ kp, const double* __restrict__ kg1, double ka1, double ka2,
-                        uint64_t* __restrict__ ctr) {
+// nchain: the blocks of the chain rule; blocks past them run the energy reductions red (component
+// (b - nchain) % 10, job (b - nchain) / 10) into ev_out
+__global__ __launch_bounds__(256) void k_chain(double* __restrict__ grad, const double* __restrict__ parts, int npart,
+                                               size_t pstride, int na, int nb, double alpha, double scale, int acc,
+                                               const double* __restrict__ lj, double* __restrict__ kp,
+                                               const double* __restrict__ kg1, double ka1, double ka2,
+                                               uint64_t* __restrict__ ctr, int nchain, ReduceJobs red,
+                                               double* __restrict__ ev_out) {
+    if ((int)blockIdx.x >= nchain) {
+        const int r = blockIdx.x - nchain;
+        reduce_parts_block(red, ev_out, r % 10, r / 10);
+        return;
+    }
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ctr && i == 0) *ctr += RPMD_STREAMS_PER_STEP;
@@ -159,10 +170,13 @@ __global__ void k_chain(double* __restrict__ grad, const double* __restrict__ pa
 void launch_chain_rule(double* grad, const double* parts, int npart, int na, int nb, double alpha, double scale,
                        int acc, hipStream_t s, const double* lj, const KickArgs* kick) {
     size_t n = (size_t)(na / 3) * nb;
-    k_chain<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale,
-                                                       acc, lj, kick ? kick->p : nullptr, kick ? kick->g1 : nullptr,
-                                                       kick ? kick->a1 : 0.0, kick ? kick->a2 : 0.0,
-                                                       kick ? kick->ctr : nullptr);
+    const int nchain = (int)((n + 255) / 256);
+    ReduceJobs red;
+    if (kick && kick->red) red = *kick->red;
+    k_chain<<<(unsigned)(nchain + 10 * red.n), 256, 0, s>>>(
+        grad, parts, npart, (size_t)3 * na * nb, na, nb, alpha, scale, acc, lj, kick ? kick->p : nullptr,
+        kick ? kick->g1 : nullptr, kick ? kick->a1 : 0.0, kick ? kick->a2 : 0.0, kick ? kick->ctr : nullptr, nchain,
+        red, kick ? kick->ev_out : nullptr);
 }
 
 // bead average (src/md_forces.f90:39-44): sum over beads in order, then divide
