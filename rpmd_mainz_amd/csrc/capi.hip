@@ -94,6 +94,7 @@ struct rpmd_ctx {
            *rst = nullptr, *pnm = nullptr, *rnm = nullptr, *tmp = nullptr, *rb = nullptr, *dvdrb = nullptr,
            *tmp2 = nullptr;
     double *monod = nullptr, *pile_c1 = nullptr, *pile_c3 = nullptr, *com_parts = nullptr, *lang_c3m = nullptr;
+    double* com_parts2 = nullptr;  // a second set of centroid partial sums (read and written in one launch)
     int* maps = nullptr;  // contraction mode maps
     double* noise = nullptr;
     size_t noise_cap = 0;
@@ -910,15 +911,15 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     if (P.therm == RPMD_THERM_PRO && mts == 1 && P.nbdf1 <= 0 && P.nbdf2 <= 0) {
         // the same sequence with fewer launches: the opening kick is applied as p is loaded by the NM
         // transform (p itself is recomputed from pnm below), each COM removal as pnm is loaded by the
-        // next kernel, p = B pnm is taken before the intramolecular forces (which do not read it), the
-        // two closing kicks are one launch, and the noise counter advances in it
+        // next kernel, the free ring and the second thermostat kick are one launch, p = B pnm is taken
+        // before the intramolecular forces (which do not read it), the two closing kicks are one launch,
+        // and the noise counter advances in it
         launch_nm_kick2(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->pnm, c->r, c->rnm, nmm->F, nb, ne, s);
         launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
                     c->com_parts, s);
-        launch_free_ring(c->pnm, c->rnm, c->monod, na, nb, s, c->com_parts, nblk, inv_mtot, c->mass);
-        launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
-                    c->com_parts, s);
-        launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
+        launch_free_ring_pile(c->pnm, c->rnm, c->monod, na, nb, c->com_parts, inv_mtot, c->mass, c->pile_c1,
+                              c->pile_c3, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++, c->com_parts2, s);
+        launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts2, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
         ReduceJobs red;
         // the intramolecular forces on the third stream, beside the other force kernels; on two
         // streams they follow the O-O terms on the second (the same positions: rst becomes r)
@@ -1342,6 +1343,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->pile_c1, nbl);
     e |= dalloc(&c->pile_c3, (size_t)3 * nbl);
     e |= dalloc(&c->com_parts, (size_t)3 * pile_blocks(na));
+    e |= dalloc(&c->com_parts2, (size_t)3 * pile_blocks(na));
     e |= dalloc(&c->lang_c3m, na);
     e |= dalloc(&c->maps, 1024);
     e |= dalloc(&c->step_ctr, 1);
@@ -1390,7 +1392,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
-                    c->step_ctr, c->ev_step, c->goo};
+                    c->step_ctr, c->ev_step, c->goo, c->com_parts2};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& kv : c->nm) {

@@ -168,6 +168,12 @@ void launch_nm_com2(const double* rnm, double* rst, const double* pnm, double* p
                     double inv_mtot, const double* mass, int na, const double* B, int n, size_t ne, hipStream_t s);
 void launch_free_ring(double* p, double* r, const double* monod, int na, int nb, hipStream_t s,
                       const double* com = nullptr, int nblk = 0, double inv_mtot = 0.0, const double* mass = nullptr);
+// launch_free_ring (COM of com_in removed from mode 0) then launch_pile over all modes, one launch;
+// the kick's partial centroid sums go to com_out (not com_in)
+void launch_free_ring_pile(double* p, double* r, const double* monod, int na, int nb, const double* com_in,
+                           double inv_mtot, const double* mass, const double* c1, const double* c3,
+                           const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add,
+                           double* com_out, hipStream_t s);
 int pile_blocks(int na);
 void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
                  uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);

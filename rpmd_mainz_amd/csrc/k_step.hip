@@ -300,6 +300,72 @@ __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const doub
     }
 }
 int pile_blocks(int na) { return (na + 255) / 256; }
+
+// The free ring step (as k_free_ring, the centroid COM in com_in removed from mode 0 first) and then
+// the PILE kick (as k_pile, modes 0..nb-1) in one launch, thread per (atom, mode) as k_pile, so the
+// partial centroid sums of the kick go to com_out in k_pile's layout and order
+__global__ __launch_bounds__(256) void k_free_ring_pile(double* __restrict__ p, double* __restrict__ r,
+                                                        const double* __restrict__ monod, int na,
+                                                        const double* __restrict__ com_in, int nblk, double inv_mtot,
+                                                        const double* __restrict__ mass,
+                                                        const double* __restrict__ c1,
+                                                        const double* __restrict__ c3,
+                                                        const double* __restrict__ noise, uint64_t seed,
+                                                        const uint64_t* __restrict__ streamp, uint64_t stream_add,
+                                                        double* __restrict__ com_out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (a < na) {
+        const int ic = a % 3;
+        const double* mo = monod + (size_t)k * 12;
+        const double cc1 = c1[k], cc3 = c3[k * 3 + ic];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const size_t i = ((size_t)k * 3 + d) * na + a;
+            double pv = p[i];
+            const double rv = r[i];
+            if (k == 0) {
+                double pc = 0.0;
+                for (int q = 0; q < nblk; q++) pc += com_in[q * 3 + d];
+                const double pcd = pc * inv_mtot;
+                pv -= pcd * mass[a];
+            }
+            const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
+            const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
+            r[i] = rr;
+            const size_t ni = ((size_t)k * na + a) * 3 + d;
+            const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
+            const double v = __dadd_rn(__dmul_rn(cc1, pp), __dmul_rn(cc3, xi));
+            p[i] = v;
+            s[d] = v;
+        }
+    }
+    if (k == 0) {
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) s[d] += __shfl_down(s[d], off, 64);
+        __shared__ double red[4][3];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0)
+            for (int d = 0; d < 3; d++) red[w][d] = s[d];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double t = 0.0;
+            for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q][threadIdx.x];
+            com_out[blockIdx.x * 3 + threadIdx.x] = t;
+        }
+    }
+}
+void launch_free_ring_pile(double* p, double* r, const double* monod, int na, int nb, const double* com_in,
+                           double inv_mtot, const double* mass, const double* c1, const double* c3,
+                           const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add,
+                           double* com_out, hipStream_t s) {
+    k_free_ring_pile<<<dim3(pile_blocks(na), nb), 256, 0, s>>>(p, r, monod, na, com_in, pile_blocks(na), inv_mtot,
+                                                               mass, c1, c3, noise, seed, stream, stream_add,
+                                                               com_out);
+}
 void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
                  uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s) {
     if (nb <= k0) return;
