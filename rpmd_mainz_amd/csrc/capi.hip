@@ -368,8 +368,8 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.nex, na);
         e |= dalloc(&v.nl, (size_t)na * (cap + VLG));  // + the padding group
         e |= dalloc(&v.rb, 3 * ns);
-        e |= dalloc(&v.srt, ns);
-        e |= dalloc(&v.srf, ns);
+        e |= dalloc(&v.srt, ns + nbl);  // + the pad site of each bead
+        e |= dalloc(&v.srf, ns + nbl);
         e |= dalloc(&v.cu, (size_t)4 * na);
         e |= dalloc(&v.cs, na);
         e |= dalloc(&v.bb, (size_t)8 * nclu);

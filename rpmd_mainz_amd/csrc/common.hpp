@@ -103,8 +103,11 @@ struct VList {
     int* nl;         // [cap / VLG + 1][na][VLS][4] sorted neighbour indices: entry n of k in group n / VLG,
                      // lane n % VLS, slot (n / VLS) % 4
     double* rb;      // [nb][3][na] site coordinates at the last build
-    double4* srt;    // [nb][na] current coordinates and charge in sorted order
-    float4* srf;     // [nb][na] wrapped fractional FP32 coordinates and charge, sorted order
+    double4* srt;    // [nb][na + 1] current coordinates and charge in sorted order; entry na of a bead is
+                     // the charge-free pad site (zero coordinates)
+    uint4* srf;      // [nb][na + 1] wrapped fractional coordinates in 32-bit fixed point (units of 2^-32,
+                     // low bit clear) and the FP32 bits of the charge, sorted order; entry na of a bead is
+                     // the charge-free pad site (odd coordinates, so never at distance zero)
     double* cu;      // [4][na] centroids and spreads of the build, site order
     float4* cs;      // [na] wrapped fractional centroid and spread, sorted order
     double* bb;      // [nclu][8] cluster boxes (centre, half-width, largest spread)

@@ -320,6 +320,14 @@ __device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float s
 #define CLU 64
 
 __device__ __forceinline__ double wrap01(double f) { return f - floor(f); }
+// a wrapped fractional coordinate in 32-bit fixed point, low bit clear: differences of two in 32-bit
+// integer arithmetic are the minimum image (mod 2^32) exactly
+__device__ __forceinline__ uint32_t frac_fixed(double f) {
+    return (uint32_t)fmin(wrap01(f) * 4294967296.0, 4294967295.0) & ~1u;
+}
+__device__ __forceinline__ uint4 site_fixed(double x, double y, double z, double q, double f0, double f1, double f2) {
+    return make_uint4(frac_fixed(x * f0), frac_fixed(y * f1), frac_fixed(z * f2), __float_as_uint((float)q));
+}
 
 // Per site of every bead: the Ewald site (the M-site combination of the oxygen and its hydrogens at
 // a%3 == 0, as k_sites), scaled by sc; its phase factors e^{i lat_d x_d}; its squared displacement
@@ -335,7 +343,7 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
                                                   double l1, double l2, double2* __restrict__ ph,
                                                   const double* __restrict__ q, const double* __restrict__ rb,
                                                   const int* __restrict__ iperm, int gather, double f0, double f1,
-                                                  double f2, double4* __restrict__ srt, float4* __restrict__ srf,
+                                                  double f2, double4* __restrict__ srt, uint4* __restrict__ srf,
                                                   double* __restrict__ cu, unsigned long long* __restrict__ dmax,
                                                   int G) {
     // G bead lanes per site (8, 16 or 32), 256 / G sites per block; lane g takes beads g, g + G, ...
@@ -379,10 +387,9 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
             if (!(dd <= 1e300)) dd = 1e300;
             m = fmax(m, dd);
             if (gather) {
-                const size_t k = (size_t)b * na + ka;
+                const size_t k = (size_t)b * (na + 1) + ka;
                 srt[k] = make_double4(x[0], x[1], x[2], qq);
-                srf[k] = make_float4((float)wrap01(x[0] * f0), (float)wrap01(x[1] * f1), (float)wrap01(x[2] * f2),
-                                     (float)qq);
+                srf[k] = site_fixed(x[0], x[1], x[2], qq, f0, f1, f2);
             }
         }
     }
@@ -594,8 +601,8 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // ascending sorted order.  Entry n of site k is nl[((n/VLG)*na + k)*VLG + (n%VLS)*4 + (n/VLS)%4]: lane
 // n % VLS of the VLS lanes that walk a site's list reads its next four entries as one 16-byte load,
 // the site's lanes read 64 contiguous bytes, and consecutive entries are usually neighbouring sites.
-// The last group of a site is filled up with k, and group cap/VLG holds k sixteen times (the pair
-// kernel reads it past a lane's end).  The test is
+// The last group of a site is filled up with the pad site na (no charge), and group cap/VLG holds it
+// sixteen times (the pair kernel reads it past a lane's end).  The test is
 // FP32 on wrapped fractional centroids against the reach enlarged by a margin that covers the FP32
 // error, so the list is a superset (a few pairs beyond do no harm: the pair kernel tests the
 // cut-off exactly).  One wave per BLS sites of cluster I, BPS = 64 / BLS lanes per site: lane s of a
@@ -613,7 +620,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
                                                  float rl, float mrg, int cap, int* __restrict__ cnt,
                                                  int* __restrict__ nex, int* __restrict__ nl,
                                                  double* __restrict__ rb, double4* __restrict__ srt,
-                                                 float4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
+                                                 uint4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
     const int t = threadIdx.x, sl = t % BPS;
     const int k0 = blockIdx.x * BLS, k = k0 + t / BPS, I = k0 / CLU;
@@ -628,11 +635,16 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
             rb[o] = x;
             rb[o + na] = y;
             rb[o + 2 * na] = z;
-            srt[(size_t)b * na + kk] = make_double4(x, y, z, qq);
-            srf[(size_t)b * na + kk] =
-                make_float4((float)wrap01(x * f0), (float)wrap01(y * f1), (float)wrap01(z * f2), (float)qq);
+            srt[(size_t)b * (na + 1) + kk] = make_double4(x, y, z, qq);
+            srf[(size_t)b * (na + 1) + kk] = site_fixed(x, y, z, qq, f0, f1, f2);
         }
     }
+    // the charge-free pad site of every bead
+    if (blockIdx.x == 0)
+        for (int b = t; b < nb; b += 64) {
+            srt[(size_t)b * (na + 1) + na] = make_double4(0.0, 0.0, 0.0, 0.0);
+            srf[(size_t)b * (na + 1) + na] = make_uint4(1u, 1u, 1u, 0u);
+        }
     __shared__ float4 cj[CLU];  // candidate centroid (wrapped fractional) and spread
     __shared__ int mj[CLU];     // candidate molecule
     float4 ci = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -718,10 +730,11 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
         }
     }
     if (kv) {
-        // the rest of the site's last group and the padding group hold k (BPS = VLG lanes per site)
+        // the rest of the site's last group and the padding group hold the pad site na (BPS = VLG lanes
+        // per site)
         const int nf = min(n, cap), pe = min((nf + VLG - 1) / VLG * VLG, cap);
-        if (nf + sl < pe) put(nf + sl, k);
-        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = k;
+        if (nf + sl < pe) put(nf + sl, na);
+        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = na;
         if (sl == 0) cnt[k] = nf;
     }
     if (ovf) ovfl[0] = 1;
@@ -748,16 +761,50 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
     tv[6] += dz * fzz;
 }
 
+// FP32 pair body on fixed-point coordinates (the erfc of erfc_pair<float> with p' folded into the
+// polynomial: t' = 1 / (p' + x), erfc = e t' sum_k A_k p'^k t'^(k-1)); na2 = -alpha'^2, so
+// e^{-(alpha r)^2} = 2^{na2 r^2}.  Gradient and the 7 sums as vl_pair.
+constexpr double AS_P2 = AS_P * SQRT_LOG2E;
+__device__ __forceinline__ void vl_pair_fx(float dx, float dy, float dz, float r2, float qij, bool same, float ta,
+                                           float na2, float tsf, float (&tg)[3], float (&tv)[7]) {
+    constexpr float P2 = (float)AS_P2, B1 = (float)(AS_A1 * AS_P2), B2 = (float)(AS_A2 * AS_P2 * AS_P2),
+                    B3 = (float)(AS_A3 * AS_P2 * AS_P2 * AS_P2), B4 = (float)(AS_A4 * AS_P2 * AS_P2 * AS_P2 * AS_P2),
+                    B5 = (float)(AS_A5 * AS_P2 * AS_P2 * AS_P2 * AS_P2 * AS_P2);
+    const float rinv = __frsqrt_rn(r2);
+    const float e = __builtin_amdgcn_exp2f(r2 * na2);
+    const float t = __builtin_amdgcn_rcpf(fmaf(r2 * rinv, ta, P2));  // v_rcp_f32 (1 ulp)
+    float h = e * t * (B1 + t * (B2 + t * (B3 + t * (B4 + t * B5))));
+    if (same) h = h - 1.0f;
+    const float du = h * rinv;
+    const float dur = (tsf * e - du) * (rinv * rinv);
+    const float dv = qij * du, dvr = qij * dur;
+    const float fxx = dvr * dx, fyy = dvr * dy, fzz = dvr * dz;
+    tg[0] += fxx;
+    tg[1] += fyy;
+    tg[2] += fzz;
+    tv[0] += dv;
+    tv[1] += dx * fxx;
+    tv[2] += dx * fyy;
+    tv[3] += dx * fzz;
+    tv[4] += dy * fyy;
+    tv[5] += dy * fzz;
+    tv[6] += dz * fzz;
+}
+
 // VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list (one list for
 // all beads).  The walk takes the lane's entries in groups of four with two register sets: the
 // coordinates of one group and the list entries of the group after it are in flight while a group is
-// computed (no copy of a loaded register, which would wait for its load).  Entries past the lane's
-// end read the site itself and are masked.  The site's gradient is summed over its lanes by a fixed
-// butterfly.  Block id i runs site block (i % 8) + 8 ((i / 8) / nb) of bead (i / 8) % nb, so the nb
-// blocks that read one stretch of the list run on one XCD (blocks go to the 8 XCDs round robin) and
-// share its L2.  Clears dmax for the next call (every gated kernel has run).
+// computed (no copy of a loaded register, which would wait for its load).  Entries past a site's list
+// are the charge-free pad site na (FP64: also masked by position).  The site's gradient is summed over
+// its lanes by a fixed butterfly.  Block id i runs site block (i % 8) + 8 ((i / 8) / nb) of bead
+// (i / 8) % nb, so the nb blocks that read one stretch of the list run on one XCD (blocks go to the 8
+// XCDs round robin) and share its L2.  Clears dmax for the next call (every gated kernel has run).
+// FP32: the differences of the fixed-point coordinates (srf) are the minimum image, converted to FP32
+// exactly up to rounding; for the unit box they stay in units of 2^-32 (the constants and the final
+// sums are scaled by powers of two instead), else they are scaled by L 2^-32 per component.  The pad
+// site has odd coordinates and the others even ones, so no distance is zero, and its pairs add zero.
 template <typename T, bool UNIT>
-__global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const float4* __restrict__ srf,
+__global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const uint4* __restrict__ srf,
                                                   const int* __restrict__ perm, const int* __restrict__ cnt,
                                                   const int* __restrict__ nex, const int* __restrict__ nl, int na,
                                                   int nb, int cap, double L0, double L1, double L2, double i0,
@@ -765,60 +812,86 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                                                   double* __restrict__ grad, double* __restrict__ parts,
                                                   unsigned long long* __restrict__ dmax) {
     constexpr bool F64 = sizeof(T) == 8;
-    typedef typename std::conditional<F64, double4, float4>::type P4;
+    typedef typename std::conditional<F64, double4, uint4>::type P4;
     const int id = blockIdx.x, m8 = id >> 3;
     const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
     const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
     if (id == 0 && threadIdx.x == 0) dmax[0] = 0ull;
-    double acc[7];
+    // the FP64 energy and virial sums of each thread live in LDS (fewer registers, more waves)
+    __shared__ double vacc[7][256];
 #pragma unroll
-    for (int c = 0; c < 7; c++) acc[c] = 0.0;
+    for (int c = 0; c < 7; c++) vacc[c][threadIdx.x] = 0.0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
+    // FP32 unit box: distances in units of 2^-32, gradient x 2^64, energy and virial x 2^32 at the end
+    constexpr double SR = (!F64 && UNIT) ? 0x1p-32 : 1.0;
+    constexpr bool DB = true;  // two register sets of coordinates (a group in flight) or one
     if (k < na) {
-        const size_t bo = (size_t)b * na;
-        const P4* X = F64 ? (const P4*)(srt + bo) : (const P4*)(srf + bo);
-        // this lane's groups of four entries: uint4 (g na + k) VLS + sl
-        const uint4* L = (const uint4*)nl + (size_t)k * VLS + sl;
-        const size_t ls = (size_t)na * VLS;
+        const size_t bo = (size_t)b * (na + 1);
+        const P4* X;
+        if constexpr (F64)
+            X = srt + bo;
+        else
+            X = srf + bo;
+        // this lane's groups of four entries: uint4 (g na + k) VLS + sl, addressed by 32-bit byte
+        // offsets from the uniform list base (the list is far below 4 GB)
+        const uint32_t l0 = (uint32_t)(k * VLS + sl) * 16u, ls = (uint32_t)na * VLS * 16u;
         const int ntot = cnt[k], ne = nex[k];
         const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
         const int ng = (nn + 3) / 4;                 // its groups
         const P4 xi = X[k];
-        const T qi = (T)xi.w;
+        T qi;
+        if constexpr (F64)
+            qi = xi.w;
+        else
+            qi = __uint_as_float(xi.w);
         const T Lt0 = (T)L0, Lt1 = (T)L1, Lt2 = (T)L2, it0 = (T)i0, it1 = (T)i1, it2 = (T)i2;
-        const T trc2 = (T)rc2;
-        const T ta = (T)alpha, tsf = (T)sfac;
+        const float Ls0 = (float)(L0 * 0x1p-32), Ls1 = (float)(L1 * 0x1p-32), Ls2 = (float)(L2 * 0x1p-32);
+        const T trc2 = (T)(rc2 / (SR * SR));
+        const T ta = (T)(alpha * SR), tsf = (T)(sfac * SR);
+        const float na2 = (float)(-(alpha * SR) * (alpha * SR));
         T tg[3] = {0, 0, 0}, tv[7];
 #pragma unroll
         for (int c = 0; c < 7; c++) tv[c] = 0;
-        // past the lane's last group: group cap / VLG, which holds the site itself (no branch around
-        // the load)
+        // past the lane's last group: group cap / VLG, which holds the pad site (no branch around the
+        // load)
         const int pad = cap / VLG;
-        auto ent = [&](int g) { return L[(size_t)(g < ng ? g : pad) * ls]; };
+        auto ent = [&](int g) {
+            return *(const uint4*)((const char*)nl + (l0 + (uint32_t)(g < ng ? g : pad) * ls));
+        };
+        // coordinates of sorted site j of this bead, by a 32-bit byte offset from the bead's base
+        auto at = [&](uint32_t j) { return *(const P4*)((const char*)X + j * (uint32_t)sizeof(P4)); };
         // a site has at most two own-molecule partners (three sites per molecule), listed first, so
         // only a lane's first entry can be one: may_same is false for every other call
         auto pair = [&](const P4& cur, int r, bool may_same) {
-            T ex = (T)(xi.x - cur.x), ey = (T)(xi.y - cur.y), ez = (T)(xi.z - cur.z);
-            if (UNIT) {
-                // the unit box (cubic): both forms reduce to this exactly
-                ex -= rint(ex);
-                ey -= rint(ey);
-                ez -= rint(ez);
-            } else if (F64) {
-                // physical minimum image
-                ex -= Lt0 * rint(ex * it0);
-                ey -= Lt1 * rint(ey * it1);
-                ez -= Lt2 * rint(ez * it2);
+            if constexpr (F64) {
+                T ex = xi.x - cur.x, ey = xi.y - cur.y, ez = xi.z - cur.z;
+                if (UNIT) {
+                    // the unit box (cubic): both forms reduce to this exactly
+                    ex -= rint(ex);
+                    ey -= rint(ey);
+                    ez -= rint(ez);
+                } else {
+                    // physical minimum image
+                    ex -= Lt0 * rint(ex * it0);
+                    ey -= Lt1 * rint(ey * it1);
+                    ez -= Lt2 * rint(ez * it2);
+                }
+                const T r2 = ex * ex + ey * ey + ez * ez;
+                const bool in = r < nn && r2 < trc2;
+                vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, may_same && r * VLS + sl < ne,
+                           ta, tsf, tg, tv);
             } else {
-                // wrapped fractional coordinates
-                ex = (ex - rint(ex)) * Lt0;
-                ey = (ey - rint(ey)) * Lt1;
-                ez = (ez - rint(ez)) * Lt2;
+                float ex = (float)(int)(xi.x - cur.x), ey = (float)(int)(xi.y - cur.y),
+                      ez = (float)(int)(xi.z - cur.z);
+                if (!UNIT) {
+                    ex *= Ls0;
+                    ey *= Ls1;
+                    ez *= Ls2;
+                }
+                const float r2 = ex * ex + ey * ey + ez * ez;
+                const float qij = r2 < trc2 ? qi * __uint_as_float(cur.w) : 0.0f;
+                vl_pair_fx(ex, ey, ez, r2, qij, may_same && sl < ne, ta, na2, tsf, tg, tv);
             }
-            const T r2 = ex * ex + ey * ey + ez * ez;
-            const bool in = r < nn && r2 < trc2;
-            vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, may_same && r * VLS + sl < ne, ta, tsf,
-                       tg, tv);
         };
         auto flush = [&]() {
             gx += (double)tg[0];
@@ -827,37 +900,59 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
             tg[0] = tg[1] = tg[2] = 0;
 #pragma unroll
             for (int c = 0; c < 7; c++) {
-                acc[c] += (double)tv[c];
+                vacc[c][threadIdx.x] += (double)tv[c];
                 tv[c] = 0;
             }
         };
-        uint4 ia = ent(0);
-        P4 a0 = X[ia.x], a1 = X[ia.y], a2 = X[ia.z], a3 = X[ia.w];
-        uint4 ib = ent(1);
-        for (int g = 0; g < ng; g += 2) {
-            const int r = 4 * g;
-            const P4 b0 = X[ib.x], b1 = X[ib.y], b2 = X[ib.z], b3 = X[ib.w];
-            ia = ent(g + 2);
-            pair(a0, r, g == 0);
-            pair(a1, r + 1, false);
-            pair(a2, r + 2, false);
-            pair(a3, r + 3, false);
-            if (g + 1 >= ng) break;
-            a0 = X[ia.x];
-            a1 = X[ia.y];
-            a2 = X[ia.z];
-            a3 = X[ia.w];
-            ib = ent(g + 3);
-            pair(b0, r + 4, false);
-            pair(b1, r + 5, false);
-            pair(b2, r + 6, false);
-            pair(b3, r + 7, false);
-            if (!F64 && (g & 3) == 2) flush();
+        // groups g and g + 1 per trip, also when g + 1 is past the lane's end (the pad group then, whose
+        // pairs add zero): with no exit between a load and its use the loads stay where they are
+        // issued, a group ahead of their use
+        if constexpr (DB) {
+            uint4 ia = ent(0);
+            P4 a0 = at(ia.x), a1 = at(ia.y), a2 = at(ia.z), a3 = at(ia.w);
+            uint4 ib = ent(1);
+            for (int g = 0; g < ng; g += 2) {
+                const int r = 4 * g;
+                const P4 b0 = at(ib.x), b1 = at(ib.y), b2 = at(ib.z), b3 = at(ib.w);
+                ia = ent(g + 2);
+                pair(a0, r, g == 0);
+                pair(a1, r + 1, false);
+                pair(a2, r + 2, false);
+                pair(a3, r + 3, false);
+                a0 = at(ia.x);
+                a1 = at(ia.y);
+                a2 = at(ia.z);
+                a3 = at(ia.w);
+                ib = ent(g + 3);
+                pair(b0, r + 4, false);
+                pair(b1, r + 5, false);
+                pair(b2, r + 6, false);
+                pair(b3, r + 7, false);
+                if (!F64 && (g & 3) == 2) flush();
+            }
+        } else {
+            uint4 ia = ent(0);
+            for (int g = 0; g < ng; g++) {
+                const int r = 4 * g;
+                const P4 a0 = at(ia.x), a1 = at(ia.y), a2 = at(ia.z), a3 = at(ia.w);
+                ia = ent(g + 1);
+                pair(a0, r, g == 0);
+                pair(a1, r + 1, false);
+                pair(a2, r + 2, false);
+                pair(a3, r + 3, false);
+                if (!F64 && (g & 3) == 3) flush();
+            }
         }
         flush();
-#pragma unroll
-        for (int c = 0; c < 7; c++) acc[c] *= 0.5;
+        constexpr double GS = 1.0 / (SR * SR);
+        gx *= GS;
+        gy *= GS;
+        gz *= GS;
     }
+    double acc[7];
+    constexpr double VS = 0.5 / SR;
+#pragma unroll
+    for (int c = 0; c < 7; c++) acc[c] = vacc[c][threadIdx.x] * VS;
 #pragma unroll
     for (int off = 1; off < VLS; off <<= 1) {
         gx += __shfl_xor(gx, off, 64);
