@@ -95,6 +95,7 @@ struct rpmd_ctx {
            *tmp2 = nullptr;
     double *monod = nullptr, *pile_c1 = nullptr, *pile_c3 = nullptr, *com_parts = nullptr, *lang_c3m = nullptr;
     double* com_parts2 = nullptr;  // a second set of centroid partial sums (read and written in one launch)
+    double* pstep_parts = nullptr;  // [pile_step_blocks(na)][6] centroid sums of the fused PILE step
     int* maps = nullptr;  // contraction mode maps
     double* noise = nullptr;
     size_t noise_cap = 0;
@@ -914,12 +915,20 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
         // next kernel, the free ring and the second thermostat kick are one launch, p = B pnm is taken
         // before the intramolecular forces (which do not read it), the two closing kicks are one launch,
         // and the noise counter advances in it
-        launch_nm_kick2(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->pnm, c->r, c->rnm, nmm->F, nb, ne, s);
-        launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++,
-                    c->com_parts, s);
-        launch_free_ring_pile(c->pnm, c->rnm, c->monod, na, nb, c->com_parts, inv_mtot, c->mass, c->pile_c1,
-                              c->pile_c3, nz.take(draws), c->seed, c->step_ctr, nz.stream_add++, c->com_parts2, s);
-        launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts2, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne, s);
+        const double* n1 = nz.take(draws);
+        const double* n2 = nz.take(draws);
+        const uint64_t sa1 = nz.stream_add++, sa2 = nz.stream_add++;
+        // up to 64 beads the whole sequence is one wave per 16 columns plus the COM corrections
+        if (!launch_pile_step(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->r, c->p, c->rst, nmm->F, nmm->B, nb,
+                              na, c->monod, c->mass, inv_mtot, c->pile_c1, c->pile_c3, n1, n2, c->seed, c->step_ctr,
+                              sa1, c->pstep_parts, s)) {
+            launch_nm_kick2(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->pnm, c->r, c->rnm, nmm->F, nb, ne, s);
+            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, n1, c->seed, c->step_ctr, sa1, c->com_parts, s);
+            launch_free_ring_pile(c->pnm, c->rnm, c->monod, na, nb, c->com_parts, inv_mtot, c->mass, c->pile_c1,
+                                  c->pile_c3, n2, c->seed, c->step_ctr, sa2, c->com_parts2, s);
+            launch_nm_com2(c->rnm, c->rst, c->pnm, c->p, c->com_parts2, nblk, inv_mtot, c->mass, na, nmm->B, nb, ne,
+                           s);
+        }
         ReduceJobs red;
         // the intramolecular forces on the third stream, beside the other force kernels; on two
         // streams they follow the O-O terms on the second (the same positions: rst becomes r)
@@ -1344,6 +1353,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->pile_c3, (size_t)3 * nbl);
     e |= dalloc(&c->com_parts, (size_t)3 * pile_blocks(na));
     e |= dalloc(&c->com_parts2, (size_t)3 * pile_blocks(na));
+    e |= dalloc(&c->pstep_parts, (size_t)6 * pile_step_blocks(na));
     e |= dalloc(&c->lang_c3m, na);
     e |= dalloc(&c->maps, 1024);
     e |= dalloc(&c->step_ctr, 1);
@@ -1392,7 +1402,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
                     c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
-                    c->step_ctr, c->ev_step, c->goo, c->com_parts2};
+                    c->step_ctr, c->ev_step, c->goo, c->com_parts2, c->pstep_parts};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& kv : c->nm) {

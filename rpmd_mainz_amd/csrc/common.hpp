@@ -178,6 +178,16 @@ void launch_free_ring_pile(double* p, double* r, const double* monod, int na, in
                            const double* noise, uint64_t seed, const uint64_t* stream, uint64_t stream_add,
                            double* com_out, hipStream_t s);
 int pile_blocks(int na);
+// the fused PILE-L step up to the forces (kick, transform, kick, free ring, kick, inverse transform,
+// centroid COM removals) for nb <= 64 in two launches; returns 0 (nothing launched) for larger nb.
+// Device stream: both kicks from the draws of stream sadd1 (cosine and sine deviates).
+// parts: [pile_step_blocks(na)][6]
+int pile_step_blocks(int na);
+int launch_pile_step(const double* p_in, const double* g1, double a1, const double* g2, double a2, const double* r_in,
+                     double* p_out, double* r_out, const double* F, const double* B, int n, int na,
+                     const double* monod, const double* mass, double inv_mtot, const double* c1, const double* c3,
+                     const double* noise1, const double* noise2, uint64_t seed, const uint64_t* stream,
+                     uint64_t sadd1, double* parts, hipStream_t s);
 void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
                  uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s);
 int bussi_parts_size();

@@ -366,6 +366,231 @@ void launch_free_ring_pile(double* p, double* r, const double* monod, int na, in
                                                                mass, c1, c3, noise, seed, stream, stream_add,
                                                                com_out);
 }
+// ---- the PILE-L step up to the forces in two launches ------------------------------------------
+// both deviates of one Box-Muller draw: z0 = normal_at(seed, stream, idx) and its sine partner
+__device__ __forceinline__ void normal2_at(uint64_t seed, uint64_t stream, uint64_t idx, double& z0, double& z1) {
+    uint32_t o[4];
+    philox4(seed, stream, idx, o);
+    const double u1 = ((double)((((uint64_t)o[0]) << 21) ^ (o[1] >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = ((double)((((uint64_t)o[2]) << 21) ^ (o[3] >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+    const double rad = sqrt(-2.0 * log(u1));
+    z0 = rad * cospi(2.0 * u2);
+    z1 = rad * sinpi(2.0 * u2);
+}
+
+// k_pile_step: one block of 2 NT waves per 16 columns e = d na + a of p and r (NT*16 >= nb modes).
+// Wave (src, q) forms modes 16q.. of p (src 0, with the opening kick on load: p - a1 g1 - a2 g2, each
+// term rounded, as k_kick) or r (src 1) by the forward transform (F, FP64 MFMA as k_nm_mfma) into
+// LDS.  Then every thread takes two (mode, column) entries: the first PILE kick (as k_pile), the free
+// ring step (as k_free_ring) and the second PILE kick; on the device stream the two kicks draw the
+// cosine and sine deviates of one Box-Muller draw of the first kick's stream.  Then wave (src, q)
+// forms beads 16q.. of p or rst by the inverse transform (B).  The centroid COM removals of the
+// separate sequence (after each kick, src/module_gle.f90:55-70) are left out here: they are linear in
+// the removed momentum, the first only shifts the centroid positions by mo_2 dp1 (its change of p is
+// removed again by the second) and the second subtracts m_a P2 / M from the centroid momentum, P1
+// and P2 being the centroid momentum sums after the first and the second kick as computed here.
+// Block partial sums {P1, P2} go to parts[blk][6] and k_pile_com applies both corrections.
+template <int NT>
+__global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict__ p_in, const double* __restrict__ g1,
+                                                        double a1, const double* __restrict__ g2, double a2,
+                                                        const double* __restrict__ r_in, double* __restrict__ p_out,
+                                                        double* __restrict__ r_out, const double* __restrict__ F,
+                                                        const double* __restrict__ B, int n, int na,
+                                                        const double* __restrict__ monod,
+                                                        const double* __restrict__ c1,
+                                                        const double* __restrict__ c3,
+                                                        const double* __restrict__ noise1,
+                                                        const double* __restrict__ noise2, uint64_t seed,
+                                                        const uint64_t* __restrict__ streamp, uint64_t sadd1,
+                                                        double* __restrict__ parts) {
+    constexpr int NW = 2 * NT, KM = 16 * NT;
+    __shared__ double T[2][KM][16];
+    __shared__ double red[NW][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int src = w / NT, q = w % NT;
+    const size_t ne = (size_t)3 * na, e0 = (size_t)blockIdx.x * 16;
+    {
+        const size_t e = e0 + li;
+        const bool ein = e < ne;
+        double bv[NT * 4];
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
+            const int j = st * 4 + lk;
+            const size_t o = (size_t)j * ne + e;
+            double v = 0.0;
+            if (ein && j < n) {
+                if (src == 0) {
+                    v = p_in[o];
+                    v = __dadd_rn(v, -__dmul_rn(a1, g1[o]));
+                    if (g2) v = __dadd_rn(v, -__dmul_rn(a2, g2[o]));
+                } else {
+                    v = r_in[o];
+                }
+            }
+            bv[st] = v;
+        }
+        dbl4 acc = (dbl4){0.0, 0.0, 0.0, 0.0};
+        const int k = q * 16 + li;
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
+            const int j = st * 4 + lk;
+            const double m = (k < n && j < n) ? F[(size_t)k * n + j] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(m, bv[st], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) T[src][q * 16 + lk + 4 * v][li] = acc[v];
+    }
+    __syncthreads();
+    const uint64_t s0 = streamp ? *streamp : 0;
+    double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int idx = threadIdx.x + h * 128 * NT, k = idx >> 4, col = idx & 15;
+        const size_t e = e0 + col;
+        if (e < ne && k < n) {
+            const int d = (int)(e / na), a = (int)(e % na), ic = a % 3;
+            const double cc1 = c1[k], cc3 = c3[k * 3 + ic];
+            const double* mo = monod + (size_t)k * 12;
+            const size_t ni = ((size_t)k * na + a) * 3 + d;
+            double x1, x2;
+            if (noise1) {
+                x1 = noise1[ni];
+                x2 = noise2[ni];
+            } else {
+                normal2_at(seed, s0 + sadd1, ni, x1, x2);
+            }
+            const double pv = __dadd_rn(__dmul_rn(cc1, T[0][k][col]), __dmul_rn(cc3, x1));
+            const double rv = T[1][k][col];
+            const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
+            const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
+            const double pk = __dadd_rn(__dmul_rn(cc1, pp), __dmul_rn(cc3, x2));
+            T[0][k][col] = pk;
+            T[1][k][col] = rr;
+            if (k == 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    sv[c] += c == d ? pv : 0.0;
+                    sv[3 + c] += c == d ? pk : 0.0;
+                }
+            }
+        } else if (k < KM) {
+            T[0][k][col] = 0.0;
+            T[1][k][col] = 0.0;
+        }
+    }
+    __syncthreads();
+    {
+        const size_t e = e0 + li;
+        dbl4 acc = (dbl4){0.0, 0.0, 0.0, 0.0};
+        const int k = q * 16 + li;
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
+            const int j = st * 4 + lk;
+            const double m = (k < n && j < n) ? B[(size_t)k * n + j] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(m, T[src][j][li], acc, 0, 0, 0);
+        }
+        if (e < ne) {
+            double* out = src == 0 ? p_out : r_out;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int kb = q * 16 + lk + 4 * v;
+                if (kb < n) out[(size_t)kb * ne + e] = acc[v];
+            }
+        }
+    }
+    // block partial sums {P1, P2} of the centroid momentum, fixed order
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; c++) sv[c] += __shfl_down(sv[c], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 6; c++) red[w][c] = sv[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double t = 0.0;
+        for (int u = 0; u < NW; u++) t += red[u][threadIdx.x];
+        parts[blockIdx.x * 6 + threadIdx.x] = t;
+    }
+}
+
+// the centroid COM corrections of k_pile_step, thread per element e of PS_B beads: every block sums
+// the partials in the same fixed order into {P1, P2}, then rst += B[b][0] mo_2 (-(P1 / M) m_a) and
+// p += B[b][0] (-(P2 / M) m_a)
+#define PS_B 8
+__global__ __launch_bounds__(256) void k_pile_com(double* __restrict__ p, double* __restrict__ r,
+                                                  const double* __restrict__ parts, int nblk,
+                                                  const double* __restrict__ B, int n, int na,
+                                                  const double* __restrict__ monod, const double* __restrict__ mass,
+                                                  double inv_mtot) {
+    double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 6; c++) sv[c] += parts[q * 6 + c];
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; c++) sv[c] += __shfl_down(sv[c], off, 64);
+    __shared__ double red[4][6];
+    __shared__ double tot[6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 6; c++) red[w][c] = sv[c];
+    __syncthreads();
+    if (threadIdx.x < 6) tot[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    __syncthreads();
+    const size_t ne = (size_t)3 * na;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int d = (int)(e / na), a = (int)(e % na), ic = a % 3;
+    const double ma = mass[a];
+    const double dr = monod[2 * 3 + ic] * (-(tot[d] * inv_mtot) * ma), dp = -(tot[3 + d] * inv_mtot) * ma;
+    const int b0 = blockIdx.y * PS_B;
+    double rv[PS_B], pv[PS_B];
+#pragma unroll
+    for (int i = 0; i < PS_B; i++) {
+        const size_t o = (size_t)(b0 + i) * ne + e;
+        rv[i] = b0 + i < n ? r[o] : 0.0;
+        pv[i] = b0 + i < n ? p[o] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < PS_B; i++)
+        if (b0 + i < n) {
+            const double c0 = B[(size_t)(b0 + i) * n];
+            const size_t o = (size_t)(b0 + i) * ne + e;
+            r[o] = rv[i] + c0 * dr;
+            p[o] = pv[i] + c0 * dp;
+        }
+}
+
+int pile_step_blocks(int na) { return (int)(((size_t)3 * na + 15) / 16); }
+
+int launch_pile_step(const double* p_in, const double* g1, double a1, const double* g2, double a2, const double* r_in,
+                     double* p_out, double* r_out, const double* F, const double* B, int n, int na,
+                     const double* monod, const double* mass, double inv_mtot, const double* c1, const double* c3,
+                     const double* noise1, const double* noise2, uint64_t seed, const uint64_t* stream,
+                     uint64_t sadd1, double* parts, hipStream_t s) {
+    const int nblk = pile_step_blocks(na);
+#define PSTEP(NT)                                                                                               \
+    k_pile_step<NT><<<nblk, 128 * NT, 0, s>>>(p_in, g1, a1, g2, a2, r_in, p_out, r_out, F, B, n, na, monod, c1, c3, \
+                                              noise1, noise2, seed, stream, sadd1, parts)
+    if (n <= 16)
+        PSTEP(1);
+    else if (n <= 32)
+        PSTEP(2);
+    else if (n <= 48)
+        PSTEP(3);
+    else if (n <= 64)
+        PSTEP(4);
+    else
+        return 0;
+#undef PSTEP
+    const unsigned ge = (unsigned)(((size_t)3 * na + 255) / 256);
+    k_pile_com<<<dim3(ge, (n + PS_B - 1) / PS_B), 256, 0, s>>>(p_out, r_out, parts, nblk, B, n, na, monod, mass,
+                                                              inv_mtot);
+    return 1;
+}
+
 void launch_pile(double* p, const double* c1, const double* c3, int na, int nb, int k0, const double* noise,
                  uint64_t seed, const uint64_t* stream, uint64_t stream_add, double* com_parts, hipStream_t s) {
     if (nb <= k0) return;
