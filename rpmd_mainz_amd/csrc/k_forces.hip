@@ -824,7 +824,8 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     double gx = 0.0, gy = 0.0, gz = 0.0;
     // FP32 unit box: distances in units of 2^-32, gradient x 2^64, energy and virial x 2^32 at the end
     constexpr double SR = (!F64 && UNIT) ? 0x1p-32 : 1.0;
-    constexpr bool DB = true;  // two register sets of coordinates (a group in flight) or one
+    // FP32: two register sets of coordinates (a group in flight); FP64 (twice the registers): one
+    constexpr bool DB = !F64;
     if (k < na) {
         const size_t bo = (size_t)b * (na + 1);
         const P4* X;
