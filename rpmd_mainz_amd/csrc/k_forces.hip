@@ -1662,13 +1662,15 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // Im(conj(q X^kx Y^ky) U) (and ky-weighted, and with the twin) give the three gradient components,
 // the quantities of the Horner pass.  One block per (64 atoms, bead); wave w owns rows
 // 16 w .. 16 w + 15 of [G; kz G], which it alone reads: its A operand comes from global memory
-// straight into registers, four kx ahead in four register sets, so the kx loop has no barrier.  B
-// (x 2^10), Y^ky and q X^kx for every kx are formed once in LDS, with the tile scales; B is then
-// held in registers.  Each lane sums its part of the three components over kx in FP64; lanes and
-// waves are combined once at the end, in fixed order.  Blocks are dealt to the XCDs by bead.
-__global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ ph, const double* __restrict__ q,
-                                                    int na, int nbk, KDev kd, const uint4* __restrict__ ga,
-                                                    const double* __restrict__ gks, double* __restrict__ kpart) {
+// straight into registers, two kx ahead, so the kx loop has no barrier.  B (x 2^10), Y^ky and
+// q X^kx for every kx are formed once in LDS, with the tile scales, and read from there.  The MFMAs
+// of kx + 1 are issued before the per-atom sums of kx (two accumulator sets), so a wave keeps the
+// matrix and vector units busy together; at most 256 registers (two waves per SIMD).  Each lane
+// sums its part of the three components over kx in FP64; lanes and waves are combined once at the
+// end, in fixed order.  Blocks are dealt to the XCDs by bead.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_kforce_h16(
+    const double2* __restrict__ ph, const double* __restrict__ q, int na, int nbk, KDev kd,
+    const uint4* __restrict__ ga, const double* __restrict__ gks, double* __restrict__ kpart) {
     // blocks are dealt to the eight XCDs in turn (block id mod 8): XCD x takes beads x, x + 8, ... one
     // after the other, so the blocks resident on an XCD read the G tiles of one bead from its L2
     const int nab = (na + KFH - 1) / KFH, m8 = blockIdx.x >> 3;
@@ -1719,27 +1721,17 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
     const int r0 = KT * wz + 16 * mb;
     // A operand of lane (li, lk) for kx: rows r0 + li, dwords 16 ks + 4 lk .. + 3, hi and lo
     const uint4* GA = ga + (size_t)b * (kmax + 1) * (KT * KT) + (r0 + li) * 8 + lk;
-    // the kx-invariant B operand (conj Z, split) of the four 16-atom tiles in registers for the whole
-    // kx loop
-    uint4 bzh[4][2], bzl[4][2];
-#pragma unroll
-    for (int n = 0; n < 4; n++)
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            bzh[n][ks] = *(const uint4*)&Zh[16 * n + li][16 * ks + 4 * lk];
-            bzl[n][ks] = *(const uint4*)&Zl[16 * n + li][16 * ks + 4 * lk];
-        }
     double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
     // a kx without members has a zero tile: its sums add zero
-    auto body = [&](int kx, uint4 ah0, uint4 al0, uint4 ah1, uint4 al1) {
-        flt4 ur[4], ui[4];
+    auto mma = [&](uint4 ah0, uint4 al0, uint4 ah1, uint4 al1, flt4 (&ur)[4], flt4 (&ui)[4]) {
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const uint4 ah = ks ? ah1 : ah0, al = ks ? al1 : al0;
             const uint4 a2h = conj_swap(ah), a2l = conj_swap(al);
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                const uint4 bh = bzh[n][ks], bl = bzl[n][ks];
+                const uint4 bh = *(const uint4*)&Zh[16 * n + li][16 * ks + 4 * lk];
+                const uint4 bl = *(const uint4*)&Zl[16 * n + li][16 * ks + 4 * lk];
                 if (ks == 0) {
                     ur[n] = mfma_h16(ah, bh, (flt4){0, 0, 0, 0});
                     ui[n] = mfma_h16(a2h, bh, (flt4){0, 0, 0, 0});
@@ -1753,6 +1745,8 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
                 ui[n] = mfma_h16(a2l, bh, ui[n]);
             }
         }
+    };
+    auto sums = [&](int kx, const flt4 (&ur)[4], const flt4 (&ui)[4]) {
         const double us = gs[kx];
         const double wa = wz ? us : us * kx;  // x: kx-weighted plain sum; z: the twin's plain sum
         // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
@@ -1774,7 +1768,8 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
             sb[n] += us * (double)s1;
         }
     };
-    // A rows four kx ahead in four register sets (loads past kmax re-read kmax's tile, unused)
+    // A rows: even kx in set 0, odd kx in set 1, each loaded two kx ahead (loads past kmax re-read
+    // kmax's tile, unused)
     auto load = [&](int kx, uint4 (&A)[4]) {
         const uint4* g = GA + (size_t)min(kx, kmax) * (KT * KT);
         A[0] = g[0];
@@ -1782,23 +1777,20 @@ __global__ __launch_bounds__(256) void k_kforce_h16(const double2* __restrict__ 
         A[2] = g[4];
         A[3] = g[516];
     };
-    uint4 A0[4], A1[4], A2[4], A3[4];
+    uint4 A0[4], A1[4];
+    flt4 urA[4], uiA[4], urB[4], uiB[4];
     load(0, A0);
     load(1, A1);
-    load(2, A2);
-    load(3, A3);
-    for (int kx = 0; kx <= kmax; kx += 4) {
-        body(kx, A0[0], A0[1], A0[2], A0[3]);
-        load(kx + 4, A0);
+    mma(A0[0], A0[1], A0[2], A0[3], urA, uiA);
+    load(2, A0);
+    for (int kx = 0; kx <= kmax; kx += 2) {
+        if (kx + 1 <= kmax) mma(A1[0], A1[1], A1[2], A1[3], urB, uiB);
+        load(kx + 3, A1);
+        sums(kx, urA, uiA);
         if (kx + 1 > kmax) break;
-        body(kx + 1, A1[0], A1[1], A1[2], A1[3]);
-        load(kx + 5, A1);
-        if (kx + 2 > kmax) break;
-        body(kx + 2, A2[0], A2[1], A2[2], A2[3]);
-        load(kx + 6, A2);
-        if (kx + 3 > kmax) break;
-        body(kx + 3, A3[0], A3[1], A3[2], A3[3]);
-        load(kx + 7, A3);
+        if (kx + 2 <= kmax) mma(A0[0], A0[1], A0[2], A0[3], urA, uiA);
+        load(kx + 4, A0);
+        sums(kx + 1, urB, uiB);
     }
 #pragma unroll
     for (int n = 0; n < 4; n++) {
