@@ -524,7 +524,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
+                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr, fork ? EW_BPC : 0);
             else
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
         }

@@ -93,6 +93,7 @@ struct VGate {
 // Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
 // each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads
 #define VL_RESORT 8  // list builds per centroid sort (the others keep the last order)
+#define EW_BPC 6     // FP32 real-space pair blocks per CU when the k-space kernels run beside them
 #define VLS 4   // lanes per site in the Verlet-list walk
 #define VLG 16  // entries per list group: VLS lanes x 4 consecutive entries of each
 struct VList {
@@ -131,9 +132,10 @@ void launch_phases(const double* pos, int na, int nb, double alpha, int unit, Bo
                    hipStream_t s);
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, hipStream_t s);
+// bpc > 0: FP32 kernel capped at bpc blocks per CU (for a launch beside other streams' kernels)
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
-                       hipStream_t s);
+                       hipStream_t s, int bpc = 0);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,

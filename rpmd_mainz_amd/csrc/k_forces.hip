@@ -1061,19 +1061,40 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
                                                   v.rb, v.srt, v.srf, v.gate + 1, gate);
 }
 
+// Dynamic LDS for a kernel so that at most bpc of its blocks fit on a CU (0: none, or bpc <= 0).
+static size_t lds_for_blocks_per_cu(const void* fn, int bpc) {
+    if (bpc <= 0) return 0;
+    int dev = 0, lds_cu = 0;
+    hipFuncAttributes fa{};
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, fn) != hipSuccess || lds_cu <= 0)
+        return 0;
+    size_t total = (size_t)lds_cu / (bpc + 1) + 1024;
+    if (total > (size_t)lds_cu / bpc) total = (size_t)lds_cu / bpc;
+    return total > fa.sharedSizeBytes ? total - fa.sharedSizeBytes : 0;
+}
+
+// bpc > 0: at most bpc blocks per CU (FP32 pair kernel, which otherwise fills every CU slot), so that
+// kernels on another stream keep CU slots beside it
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
-                       hipStream_t s) {
+                       hipStream_t s, int bpc) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
     const double ak = sizeof(T) == 4 ? alpha * SQRT_LOG2E : alpha;  // see erfc_pair<float>
     const bool unit = sb.L[0] == 1.0 && sb.L[1] == 1.0 && sb.L[2] == 1.0;
     auto k = unit ? k_ewald_vl<T, true> : k_ewald_vl<T, false>;
-    k<<<ewald_real_parts(na) * nb, 256, 0, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap, sb.L[0],
+    static size_t dyn[2][2] = {{~(size_t)0, ~(size_t)0}, {~(size_t)0, ~(size_t)0}};  // [unit][capped]
+    size_t& d = dyn[unit][bpc > 0];
+    if (d == ~(size_t)0) d = sizeof(T) == 4 ? lds_for_blocks_per_cu((const void*)k, bpc) : 0;
+    k<<<ewald_real_parts(na) * nb, 256, d, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap, sb.L[0],
                                                 sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, ak,
                                                 sfac, grad, parts, v.dmax);
 }
-template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
-template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t);
+template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t,
+                                        int);
+template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t,
+                                       int);
 
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
