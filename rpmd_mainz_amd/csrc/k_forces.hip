@@ -282,8 +282,8 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
 // within reach; k_vl_build then fills each site's list (sorted indices, cluster order).  The pair
 // kernel gives VLS lanes to a sorted site, which walk its list: every pair is evaluated from both
 // ends (energy and virial with a factor 1/2) and each site's gradient is summed in a fixed order, so
-// results are deterministic.  T is the pair-body type; in FP32 the distances come from wrapped
-// fractional FP32 coordinates and the sums are flushed into FP64 every 16 pairs of a lane.
+// results are deterministic.  T is the pair-body type; in FP32 the distances come from 32-bit
+// fixed-point fractional coordinates and the sums are flushed into FP64 every 64 pairs of a lane.
 // ------------------------------------------------------------------------------------------------
 // FP64: du = erfc_AS(alpha r)/r (erfc - 1 within a molecule), dur = (sfac e^{-x^2} - du)/r^2, the
 // reference's expressions (src/ewald_cubic.f90:126-140) with the two divisions by r and r^2 taken as
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 pair(b1, r + 5, false);
                 pair(b2, r + 6, false);
                 pair(b3, r + 7, false);
-                if (!F64 && (g & 3) == 2) flush();
+                if (!F64 && (g & 15) == 14) flush();
             }
         } else {
             uint4 ia = ent(0);
