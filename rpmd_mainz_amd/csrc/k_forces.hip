@@ -1687,8 +1687,8 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // q X^kx for every kx are formed once in LDS, with the tile scales, and read from there.  The MFMAs
 // of kx + 1 are issued before the per-atom sums of kx (two accumulator sets), so a wave keeps the
 // matrix and vector units busy together; at most 256 registers (two waves per SIMD).  Each lane
-// sums its part of the three components over kx in FP64; lanes and waves are combined once at the
-// end, in fixed order.  Blocks are dealt to the XCDs by bead.
+// sums its part of the three components over kx in FP32 (the tile scales are powers of two); lanes
+// and waves are combined once at the end, in fixed order, the waves in FP64.  Blocks are dealt to the XCDs by bead.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_kforce_h16(
     const double2* __restrict__ ph, const double* __restrict__ q, int na, int nbk, KDev kd,
     const uint4* __restrict__ ga, const double* __restrict__ gks, double* __restrict__ kpart) {
@@ -1742,7 +1742,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int r0 = KT * wz + 16 * mb;
     // A operand of lane (li, lk) for kx: rows r0 + li, dwords 16 ks + 4 lk .. + 3, hi and lo
     const uint4* GA = ga + (size_t)b * (kmax + 1) * (KT * KT) + (r0 + li) * 8 + lk;
-    double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
+    float sa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     // a kx without members has a zero tile: its sums add zero
     auto mma = [&](uint4 ah0, uint4 al0, uint4 ah1, uint4 al1, flt4 (&ur)[4], flt4 (&ui)[4]) {
 #pragma unroll
@@ -1768,8 +1768,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     };
     auto sums = [&](int kx, const flt4 (&ur)[4], const flt4 (&ui)[4]) {
-        const double us = gs[kx];
-        const double wa = wz ? us : us * kx;  // x: kx-weighted plain sum; z: the twin's plain sum
+        const float us = (float)gs[kx];
+        const float wa = wz ? us : us * (float)kx;  // x: kx-weighted plain sum; z: the twin's plain sum
         // rows of this lane: iy = 16 mb + 4 lk + v (ky = iy - 16); column: atom 16 n + li
 #pragma unroll
         for (int n = 0; n < 4; n++) {
@@ -1785,8 +1785,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 s0 += ipu;
                 s1 += (float)(iy - KO) * ipu;
             }
-            sa[n] += wa * (double)s0;
-            sb[n] += us * (double)s1;
+            sa[n] += wa * s0;
+            sb[n] += us * s1;
         }
     };
     // A rows: even kx in set 0, odd kx in set 1, each loaded two kx ahead (loads past kmax re-read
