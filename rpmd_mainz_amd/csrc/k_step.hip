@@ -410,71 +410,99 @@ __global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict
     const int li = lane & 15, lk = lane >> 4;
     const int src = w / NT, q = w % NT;
     const size_t ne = (size_t)3 * na, e0 = (size_t)blockIdx.x * 16;
+    // every load below is unconditional, from a clamped (valid) index, with the value selected
+    // afterwards: a load under a branch would be waited for before the next one is issued
+    const int nm1 = n - 1;
+    const size_t ec = e0 + li < ne ? e0 + li : ne - 1;
+    double fa[NT * 4], ba[NT * 4];
     {
-        const size_t e = e0 + li;
-        const bool ein = e < ne;
+        const int k = q * 16 + li, kc = min(k, nm1);
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) {
+            const int j = st * 4 + lk, jc = min(j, nm1);
+            const bool kj = k < n && j < n;
+            const double f = F[(size_t)kc * n + jc], bm = B[(size_t)kc * n + jc];
+            fa[st] = kj ? f : 0.0;
+            ba[st] = kj ? bm : 0.0;
+        }
+    }
+    {
+        const bool ein = e0 + li < ne;
+        const double* in = src == 0 ? p_in : r_in;
+        double xin[NT * 4], xg1[NT * 4], xg2[NT * 4];
+#pragma unroll
+        for (int st = 0; st < NT * 4; st++) xin[st] = in[(size_t)min(st * 4 + lk, nm1) * ne + ec];
+        if (src == 0) {
+            const double* g2v = g2 ? g2 : g1;
+#pragma unroll
+            for (int st = 0; st < NT * 4; st++) {
+                const size_t o = (size_t)min(st * 4 + lk, nm1) * ne + ec;
+                xg1[st] = g1[o];
+                xg2[st] = g2v[o];
+            }
+        }
         double bv[NT * 4];
 #pragma unroll
         for (int st = 0; st < NT * 4; st++) {
-            const int j = st * 4 + lk;
-            const size_t o = (size_t)j * ne + e;
-            double v = 0.0;
-            if (ein && j < n) {
-                if (src == 0) {
-                    v = p_in[o];
-                    v = __dadd_rn(v, -__dmul_rn(a1, g1[o]));
-                    if (g2) v = __dadd_rn(v, -__dmul_rn(a2, g2[o]));
-                } else {
-                    v = r_in[o];
-                }
+            double v = xin[st];
+            if (src == 0) {
+                v = __dadd_rn(v, -__dmul_rn(a1, xg1[st]));
+                if (g2) v = __dadd_rn(v, -__dmul_rn(a2, xg2[st]));
             }
-            bv[st] = v;
+            bv[st] = ein && st * 4 + lk < n ? v : 0.0;
         }
         dbl4 acc = (dbl4){0.0, 0.0, 0.0, 0.0};
-        const int k = q * 16 + li;
 #pragma unroll
-        for (int st = 0; st < NT * 4; st++) {
-            const int j = st * 4 + lk;
-            const double m = (k < n && j < n) ? F[(size_t)k * n + j] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(m, bv[st], acc, 0, 0, 0);
-        }
+        for (int st = 0; st < NT * 4; st++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[st], bv[st], acc, 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; v++) T[src][q * 16 + lk + 4 * v][li] = acc[v];
     }
-    __syncthreads();
+    // this thread's two (mode, column) entries: coefficients and deviates, loaded and drawn before
+    // the transform results are waited for
     const uint64_t s0 = streamp ? *streamp : 0;
+    double ec1[2], ec3[2], emo[2][4], ex1[2], ex2[2];
+    int ed[2];
+    bool eok[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int idx = threadIdx.x + h * 128 * NT, k = idx >> 4, col = idx & 15;
+        const size_t e = e0 + col < ne ? e0 + col : ne - 1;
+        eok[h] = e0 + col < ne && k < n;
+        const int kc = min(k, nm1), d = (int)(e / na), a = (int)(e % na), ic = a % 3;
+        ed[h] = d;
+        ec1[h] = c1[kc];
+        ec3[h] = c3[kc * 3 + ic];
+#pragma unroll
+        for (int u = 0; u < 4; u++) emo[h][u] = monod[(size_t)kc * 12 + u * 3 + ic];
+        const size_t ni = ((size_t)kc * na + a) * 3 + d;
+        if (noise1) {
+            ex1[h] = noise1[ni];
+            ex2[h] = noise2[ni];
+        } else {
+            normal2_at(seed, s0 + sadd1, ni, ex1[h], ex2[h]);
+        }
+    }
+    __syncthreads();
     double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int idx = threadIdx.x + h * 128 * NT, k = idx >> 4, col = idx & 15;
-        const size_t e = e0 + col;
-        if (e < ne && k < n) {
-            const int d = (int)(e / na), a = (int)(e % na), ic = a % 3;
-            const double cc1 = c1[k], cc3 = c3[k * 3 + ic];
-            const double* mo = monod + (size_t)k * 12;
-            const size_t ni = ((size_t)k * na + a) * 3 + d;
-            double x1, x2;
-            if (noise1) {
-                x1 = noise1[ni];
-                x2 = noise2[ni];
-            } else {
-                normal2_at(seed, s0 + sadd1, ni, x1, x2);
-            }
-            const double pv = __dadd_rn(__dmul_rn(cc1, T[0][k][col]), __dmul_rn(cc3, x1));
+        if (eok[h]) {
+            const double pv = __dadd_rn(__dmul_rn(ec1[h], T[0][k][col]), __dmul_rn(ec3[h], ex1[h]));
             const double rv = T[1][k][col];
-            const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
-            const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
-            const double pk = __dadd_rn(__dmul_rn(cc1, pp), __dmul_rn(cc3, x2));
+            const double pp = __dadd_rn(__dmul_rn(pv, emo[h][0]), __dmul_rn(rv, emo[h][1]));
+            const double rr = __dadd_rn(__dmul_rn(pv, emo[h][2]), __dmul_rn(rv, emo[h][3]));
+            const double pk = __dadd_rn(__dmul_rn(ec1[h], pp), __dmul_rn(ec3[h], ex2[h]));
             T[0][k][col] = pk;
             T[1][k][col] = rr;
             if (k == 0) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    sv[c] += c == d ? pv : 0.0;
-                    sv[3 + c] += c == d ? pk : 0.0;
+                    sv[c] += c == ed[h] ? pv : 0.0;
+                    sv[3 + c] += c == ed[h] ? pk : 0.0;
                 }
             }
-        } else if (k < KM) {
+        } else {
             T[0][k][col] = 0.0;
             T[1][k][col] = 0.0;
         }
@@ -483,13 +511,9 @@ __global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict
     {
         const size_t e = e0 + li;
         dbl4 acc = (dbl4){0.0, 0.0, 0.0, 0.0};
-        const int k = q * 16 + li;
 #pragma unroll
-        for (int st = 0; st < NT * 4; st++) {
-            const int j = st * 4 + lk;
-            const double m = (k < n && j < n) ? B[(size_t)k * n + j] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(m, T[src][j][li], acc, 0, 0, 0);
-        }
+        for (int st = 0; st < NT * 4; st++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ba[st], T[src][st * 4 + lk][li], acc, 0, 0, 0);
         if (e < ne) {
             double* out = src == 0 ? p_out : r_out;
 #pragma unroll
@@ -523,10 +547,39 @@ __global__ __launch_bounds__(256) void k_pile_com(double* __restrict__ p, double
                                                   const double* __restrict__ B, int n, int na,
                                                   const double* __restrict__ monod, const double* __restrict__ mass,
                                                   double inv_mtot) {
-    double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = threadIdx.x; q < nblk; q += blockDim.x)
+    // all loads first, unconditional from clamped indices (a load under a branch is waited for
+    // before the next): this element's values of PS_B beads, then the partials
+    const size_t ne = (size_t)3 * na;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ein = e < ne;
+    const size_t ec = ein ? e : ne - 1;
+    const int d = (int)(ec / na), a = (int)(ec % na), ic = a % 3;
+    const double ma = mass[a], mo2 = monod[2 * 3 + ic];
+    const int b0 = blockIdx.y * PS_B;
+    double rv[PS_B], pv[PS_B], cb[PS_B];
 #pragma unroll
-        for (int c = 0; c < 6; c++) sv[c] += parts[q * 6 + c];
+    for (int i = 0; i < PS_B; i++) {
+        const int bc = min(b0 + i, n - 1);
+        const size_t o = (size_t)bc * ne + ec;
+        rv[i] = r[o];
+        pv[i] = p[o];
+        cb[i] = B[(size_t)bc * n];
+    }
+    double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < nblk; q0 += 4 * (int)blockDim.x) {
+        double pa[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = min(q0 + (int)threadIdx.x + u * (int)blockDim.x, nblk - 1);
+#pragma unroll
+            for (int c = 0; c < 6; c++) pa[u][c] = parts[q * 6 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (q0 + (int)threadIdx.x + u * (int)blockDim.x < nblk)
+#pragma unroll
+                for (int c = 0; c < 6; c++) sv[c] += pa[u][c];
+    }
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
         for (int c = 0; c < 6; c++) sv[c] += __shfl_down(sv[c], off, 64);
@@ -539,27 +592,14 @@ __global__ __launch_bounds__(256) void k_pile_com(double* __restrict__ p, double
     __syncthreads();
     if (threadIdx.x < 6) tot[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
     __syncthreads();
-    const size_t ne = (size_t)3 * na;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    const int d = (int)(e / na), a = (int)(e % na), ic = a % 3;
-    const double ma = mass[a];
-    const double dr = monod[2 * 3 + ic] * (-(tot[d] * inv_mtot) * ma), dp = -(tot[3 + d] * inv_mtot) * ma;
-    const int b0 = blockIdx.y * PS_B;
-    double rv[PS_B], pv[PS_B];
-#pragma unroll
-    for (int i = 0; i < PS_B; i++) {
-        const size_t o = (size_t)(b0 + i) * ne + e;
-        rv[i] = b0 + i < n ? r[o] : 0.0;
-        pv[i] = b0 + i < n ? p[o] : 0.0;
-    }
+    if (!ein) return;
+    const double dr = mo2 * (-(tot[d] * inv_mtot) * ma), dp = -(tot[3 + d] * inv_mtot) * ma;
 #pragma unroll
     for (int i = 0; i < PS_B; i++)
         if (b0 + i < n) {
-            const double c0 = B[(size_t)(b0 + i) * n];
             const size_t o = (size_t)(b0 + i) * ne + e;
-            r[o] = rv[i] + c0 * dr;
-            p[o] = pv[i] + c0 * dp;
+            r[o] = rv[i] + cb[i] * dr;
+            p[o] = pv[i] + cb[i] * dp;
         }
 }
 
