@@ -362,16 +362,24 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         const double qq = q[a];
         const int ka = gather ? iperm[a] : 0;
         for (int b = g; b < nb; b += G) {
+            // the molecule's three atoms in every dimension and the reference position, loaded before
+            // any is used
+            double pm[3][3], rr[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+#pragma unroll
+                for (int u = 0; u < 3; u++) pm[d][u] = pos[((size_t)b * 3 + d) * na + m0 + u];
+                rr[d] = rb[(size_t)b * 3 * na + (size_t)d * na + a];
+            }
             double x[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const double* p = pos + ((size_t)b * 3 + d) * na;
                 const size_t o = ((size_t)b * 3 + d) * na + a;
                 if (a == m0) {
-                    const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
+                    const double v = alpha * pm[d][0] + alpha2 * (pm[d][1] + pm[d][2]);
                     x[d] = sc[d] * v;
                 } else {
-                    x[d] = sc[d] * p[a];
+                    x[d] = sc[d] * (a == m0 + 1 ? pm[d][1] : pm[d][2]);
                 }
                 site[o] = x[d];
                 if (ph) {
@@ -381,8 +389,7 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
                 }
                 c[d] += x[d];
             }
-            const size_t o = (size_t)b * 3 * na + a;
-            const double dx = x[0] - rb[o], dy = x[1] - rb[o + na], dz = x[2] - rb[o + 2 * na];
+            const double dx = x[0] - rr[0], dy = x[1] - rr[1], dz = x[2] - rr[2];
             double dd = dx * dx + dy * dy + dz * dz;
             if (!(dd <= 1e300)) dd = 1e300;
             m = fmax(m, dd);
@@ -1017,15 +1024,21 @@ __global__ __launch_bounds__(256) void k_phases(const double* __restrict__ pos, 
     const int m0 = a - a % 3;
     const double alpha2 = 0.5 * (1.0 - alpha);
     const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
+    // the molecule's three atoms in every dimension, loaded before any is used
+    double pm[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int u = 0; u < 3; u++) pm[d][u] = pos[((size_t)b * 3 + d) * na + m0 + u];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double* p = pos + ((size_t)b * 3 + d) * na;
+        const double pa = a == m0 ? pm[d][0] : (a == m0 + 1 ? pm[d][1] : pm[d][2]);
         double x;
         if (a == m0) {
-            const double v = alpha * p[m0] + alpha2 * (p[m0 + 1] + p[m0 + 2]);
+            const double v = alpha * pm[d][0] + alpha2 * (pm[d][1] + pm[d][2]);
             x = sc[d] * v;
         } else {
-            x = sc[d] * p[a];
+            x = sc[d] * pa;
         }
         double sn, cs;
         sincos(lat[d] * x, &sn, &cs);
