@@ -125,6 +125,15 @@ int rpmd_full_forces(rpmd_ctx* ctx, const double* r, int na, int nb, double* v, 
  * transformed and scaled by 1/sqrt(n) (FFTW half-complex order). */
 int rpmd_realft(rpmd_ctx* ctx, double* data, int m, int n, int mode);
 
+/* The free ring-polymer update inside evolve_pi (src/evolve_pi.f90:109-125) with the monodromy of
+ * ring_rpmd (src/md_pathi.f90:190-231) for time step dt: p and r(3,na,nb) are normal-mode
+ * coordinates (realft mode +1 of the bead arrays) and are updated in place,
+ * p' = p m1 + r m2, r' = p m3 + r m4 per mode, atom type mod(j,3) and component, every product
+ * and sum rounded once in FP64: the same bits as the reference's loop compiled without FMA
+ * contraction.  This is the update every evolve_pi step of this library applies. */
+int rpmd_free_ring(rpmd_ctx* ctx, double* p, double* r, const double* mass, int na, int nb, double dt,
+                   double beta);
+
 /* subroutine evolve_pi(p,r,v,vew,vlj,vint,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,
  *                      irun,nbaro)                               src/evolve_pi.f90:1-322
  * One RPMD step (type 'RPMD', nbaro = 0).  p, r, dvdr, dvdr2 are updated in place.

@@ -146,6 +146,13 @@ class Oracle:
         self.lib.orc_ring_rpmd(_d(m), _d(mon), C.c_int(3), C.c_int(nb), C.c_double(dt), C.c_double(beta))
         return mon
 
+    def free_ring(self, p, r, monod):
+        """the free ring-polymer update (normal-mode p, r of shape (nb, na, 3)); returns copies"""
+        p, r = _f64(p).copy(), _f64(r).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib.orc_free_ring(_d(p), _d(r), _d(_f64(monod)), C.c_int(na), C.c_int(nb))
+        return p, r
+
     # -- RNG / thermostats -------------------------------------------------------------------
     def rng(self, irun: int) -> RngState:
         s = RngState()
@@ -353,6 +360,52 @@ class RefPath:
 
     def gaussian(self, sigma: float = 1.0) -> float:
         return self.lib.gaussian_(C.byref(self._idum), C.byref(C.c_double(sigma)))
+
+    def set_nctot(self, nctot: int):
+        """/constraint/ nctot, the constraint count parinello_therm subtracts from the degrees of
+        freedom (therm_pro_global calls it too); 0 for flexible molecules"""
+        (C.c_int * 2).in_dll(self.lib, "constraint_")[0] = nctot
+
+    def sample_rpmd(self, nb, na, mass, beta):
+        """thermostat::sample_rpmd (src/module_thermostat.f90:199-226) on the current ran2 stream"""
+        p = np.zeros((nb, na, 3))
+        self.lib._QMthermostatPsample_rpmd(_d(p), C.byref(C.c_int(na)), C.byref(C.c_int(nb)), _d(_f64(mass).copy()),
+                                           C.byref(C.c_double(beta)), C.byref(self._idum))
+        return p
+
+    def pshift(self, p, mass):
+        """thermostat::pshift (src/module_thermostat.f90:229-264)"""
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        self.lib._QMthermostatPpshift(_d(p), _d(_f64(mass).copy()), C.byref(C.c_int(na)), C.byref(C.c_int(nb)))
+        return p
+
+    def setup_ewald(self, na: int, box, prm: RpmdParams) -> dict:
+        """setup_ewald (src/setup_options.f90:104-189): reads /ew_param/ and /symmetry/, writes
+        /ew_setup/ and /symmetry/; returns wrcut, walpha, rkmax, kmax and iamcub"""
+        self.set_params(prm)
+        self.lib.setup_ewald_(C.byref(C.c_int(na)), _d(_f64(box).copy()))
+
+        class EwSetup(C.Structure):
+            _fields_ = [("wrcut", C.c_double), ("walpha", C.c_double), ("rkmax", C.c_double), ("kmax", C.c_int)]
+
+        es = EwSetup.in_dll(self.lib, "ew_setup_")
+        return dict(wrcut=es.wrcut, walpha=es.walpha, rkmax=es.rkmax, kmax=es.kmax,
+                    iamcub=1 if C.c_int.in_dll(self.lib, "symmetry_").value else 0)
+
+    def setup_box_size(self, lattice: str, rho: float, ncell, wmass: float):
+        """setup_box_size (src/setup_options.f90:191-270) for the CUB/FCC/ICE lattices: returns
+        (nm, boxlxyz); the cell counts go in through /lattice/"""
+        lat = (C.c_int * 4).in_dll(self.lib, "lattice_")
+        lat[0], lat[1], lat[2] = ncell
+        C.c_int.in_dll(self.lib, "symmetry_").value = 1
+        nm = C.c_int()
+        box = np.zeros(3)
+        name = C.create_string_buffer(lattice.encode()[:3].ljust(3), 3)
+        # character(len=3) dummy: its length travels as a hidden trailing argument
+        self.lib.setup_box_size_(name, C.byref(C.c_double(rho)), C.byref(nm), _d(box), C.byref(C.c_double(wmass)),
+                                 C.c_size_t(3))
+        return nm.value, box
 
     def therm_pro(self, p, mass, beta, ctau, dt):
         p = _f64(p).copy()

@@ -1845,6 +1845,24 @@ static void outer_ostep(const rpmd_params* P, double* p, const double* mass, int
     else if (P->therm == RPMD_THERM_GLE && g) orc_therm_gle(g, p, mass, na, nb, s);
 }
 
+/* the free ring-polymer update of evolve_pi, src/evolve_pi.f90:109-125: for every mode k, atom j
+ * (type ic = mod(j,3): O, H, H) and component, p' = p m1 + r m2, r' = p m3 + r m4 with the
+ * monodromy monod(3,4,nb) of ring_rpmd; p and r in normal-mode coordinates */
+void orc_free_ring(double* p, double* r, const double* monod, int na, int nb) {
+    size_t A = (size_t)3 * na;
+    for (int k = 0; k < nb; k++)
+        for (int j = 0; j < na; j++) {
+            int ic = j % 3;
+            const double* mo = &monod[(size_t)k * 12];
+            for (int c = 0; c < 3; c++) {
+                size_t o = (size_t)k * A + 3 * (size_t)j + c;
+                double pprime = p[o] * mo[0 * 3 + ic] + r[o] * mo[1 * 3 + ic];
+                r[o] = p[o] * mo[2 * 3 + ic] + r[o] * mo[3 * 3 + ic];
+                p[o] = pprime;
+            }
+        }
+}
+
 int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
                   double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
                   const double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
@@ -1868,17 +1886,7 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
     for (int im = 1; im <= mts && !err; im++) {
         if (P->therm == RPMD_THERM_PRO) orc_therm_pro(p, mass, beta, P->ttau, halfdtsmall, na, nb, s);
         else if (P->therm == RPMD_THERM_PRM) orc_therm_pro_global(p, mass, beta, P->ttau, halfdtsmall, na, nb, s);
-        for (int k = 0; k < nb; k++)
-            for (int j = 0; j < na; j++) {
-                int ic = j % 3;
-                const double* mo = &monod[(size_t)k * 12];
-                for (int c = 0; c < 3; c++) {
-                    size_t o = (size_t)k * A + 3 * (size_t)j + c;
-                    double pprime = p[o] * mo[0 * 3 + ic] + r[o] * mo[1 * 3 + ic];
-                    r[o] = p[o] * mo[2 * 3 + ic] + r[o] * mo[3 * 3 + ic];
-                    p[o] = pprime;
-                }
-            }
+        orc_free_ring(p, r, monod, na, nb);
         if (P->therm == RPMD_THERM_PRO) orc_therm_pro(p, mass, beta, P->ttau, halfdtsmall, na, nb, s);
         else if (P->therm == RPMD_THERM_PRM) orc_therm_pro_global(p, mass, beta, P->ttau, halfdtsmall, na, nb, s);
         memcpy(rst, r, sizeof(double) * N);

@@ -44,6 +44,7 @@ int orc_rp_contract_nm(const rpmd_params* P, const double* rnm, double* v, doubl
 
 int  orc_realft(double* data, int m, int n, int mode);
 void orc_ring_rpmd(const double* mass, double* monod, int na, int nb, double dt, double beta);
+void orc_free_ring(double* p, double* r, const double* monod, int na, int nb);
 
 void   orc_rng_init(orc_rng* s);
 void   orc_rng_seed(orc_rng* s, int irun);

@@ -69,7 +69,7 @@ def lib() -> C.CDLL:
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
-                     "rpmd_ctx_set_streams"):
+                     "rpmd_ctx_set_streams", "rpmd_free_ring"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -162,6 +162,17 @@ class RPMDContext:
             raise TypeError("realft needs a C-contiguous float64 array")
         _check(lib().rpmd_realft(self._h, _p(data), C.c_int(m), C.c_int(n), C.c_int(mode)))
         return data
+
+    # -- free ring-polymer update of evolve_pi  src/evolve_pi.f90:109-125 ----------------------
+    def free_ring(self, p, r, mass, na: int, nb: int, dt: float, beta: float):
+        """p, r: normal-mode coordinates (nb, na, 3), updated in place and returned."""
+        for a in (p, r):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.shape == (nb, na, 3)):
+                raise TypeError("free_ring arrays must be C-contiguous float64 (nb, na, 3)")
+        _check(lib().rpmd_free_ring(self._h, _p(p), _p(r), _p(_arr(mass, (na,))), C.c_int(na), C.c_int(nb),
+                                    C.c_double(dt), C.c_double(beta)))
+        return p, r
 
     # -- subroutine evolve_pi(p,r,v,vew,vlj,vint,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,
     #                         vir_lf,irun,nbaro)  src/evolve_pi.f90:1 ------------------------------

@@ -57,6 +57,17 @@ struct NMMat {
     double *F = nullptr, *B = nullptr;    // plain
     double *Fs = nullptr, *Bs = nullptr;  // contraction-scaled: B*sqrt(n/nb), F*sqrt(nb/n)
     int scaled_nb = -1;
+    // contraction mode maps for (n, scaled_nb), src/evolve_pi.f90:384-401,417-425: mi[n] (kept mode
+    // of each contracted mode), then mo[scaled_nb] (contracted mode of each full mode, -1: dropped)
+    int* maps = nullptr;
+};
+
+// the buffers whose roles a step exchanges (r <-> rst every step; dvdr or dvdrl <-> tmp in
+// contracted steps): a captured graph replays correctly only with the roles it was captured with
+struct Roles {
+    double* a[5];
+    bool operator==(const Roles& o) const { return !memcmp(a, o.a, sizeof a); }
+    bool operator!=(const Roles& o) const { return !(*this == o); }
 };
 
 }  // namespace
@@ -69,14 +80,16 @@ struct rpmd_ctx {
     double *aos = nullptr, *pos = nullptr, *grad = nullptr, *site = nullptr, *gsite = nullptr;
     double *q = nullptr, *mass = nullptr, *cen = nullptr, *parts = nullptr, *ev = nullptr, *skp = nullptr, *gk = nullptr, *gks = nullptr;
     double *ph = nullptr, *fpart = nullptr;
-    // real-space Verlet-list caches (two bead sets: e.g. all beads and RP-contracted ones)
+    // real-space Verlet-list caches: two bead sets (e.g. all beads and RP-contracted ones) for the
+    // device-resident state [0, 1] and two for host-synchronous calls [2, 3], so that a forces call
+    // never moves the rebuilds (and with them the summation order) of a state's lists
     struct VCache {
         VList v{};
         bool alloc = false;
         int nb = -1;
         double key[5] = {0, 0, 0, 0, 0};  // box, rc, skin
         long long last_use = 0;
-    } vc[2];
+    } vc[4];
     long long vc_clock = 0;
     int *cell_of = nullptr, *cnt = nullptr, *start = nullptr, *perm = nullptr, *nnb = nullptr, *nbl = nullptr,
         *ovf = nullptr;
@@ -96,7 +109,6 @@ struct rpmd_ctx {
     double *monod = nullptr, *pile_c1 = nullptr, *pile_c3 = nullptr, *com_parts = nullptr, *lang_c3m = nullptr;
     double* com_parts2 = nullptr;  // a second set of centroid partial sums (read and written in one launch)
     double* pstep_parts = nullptr;  // [pile_step_blocks(na)][6] centroid sums of the fused PILE step
-    int* maps = nullptr;  // contraction mode maps
     double* noise = nullptr;
     size_t noise_cap = 0;
     uint64_t* step_ctr = nullptr;
@@ -125,6 +137,7 @@ struct rpmd_ctx {
     Box st_box{};
     double st_beta = 0, st_dt = 0;
     bool st_valid = false;
+    std::vector<double> st_z;  // the state's charges (host calls may upload others)
     std::vector<double> hmass;
     // energies of the last step: slots [HF, EW, LJ, TEM, EWC] x 5 terms x 10
     double* ev_step = nullptr;
@@ -132,6 +145,11 @@ struct rpmd_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int use_graph = 0;
+    // epoch: advanced by every call that may change what a captured step reads (tables, caches,
+    // charges, parameters, the state); a graph (this context's or a replica batch's) records the
+    // epoch and the buffer roles at capture and is replayed only while both still hold
+    uint64_t epoch = 1, g_epoch = 0;
+    Roles g_roles{};
     // second stream for the real-space Ewald branch of device-resident force calls (streams >= 2):
     // forked after the sites, joined before the chain rule; third stream (streams = 3) for the O-O
     // list and pair sums of those calls and the intramolecular forces of a step, joined before the
@@ -147,6 +165,21 @@ struct rpmd_ctx {
 };
 
 namespace {
+
+Roles roles_of(const rpmd_ctx* c) { return Roles{{c->r, c->rst, c->dvdr, c->dvdrl, c->tmp}}; }
+
+void drop_graph(rpmd_ctx* c) {
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+// every captured graph that reads this context (its own, a replica batch's) is stale from here on
+void invalidate(rpmd_ctx* c) {
+    c->epoch++;
+    drop_graph(c);
+}
 
 // timed launch bracket: kernel ids 0 ewald_real, 1 k_sk, 2 k_kforce, 3 k_sk_final
 struct ProfScope {
@@ -293,16 +326,23 @@ CellGrid make_grid(const Box& sb, int na) {
     return g;
 }
 
+// the bead-transform matrices for n beads (allocated on first use, outside stream capture); with
+// nb_full > 0 also the contraction-scaled pair and the mode maps for nb_full physical beads
 NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
     auto it = c->nm.find(n);
     if (it == c->nm.end()) {
         std::vector<double> F, B;
         realft_matrices(n, F, B);
         NMMat m;
-        if (dalloc(&m.F, F.size()) || dalloc(&m.B, B.size()) || dalloc(&m.Fs, F.size()) || dalloc(&m.Bs, B.size()))
+        if (dalloc(&m.F, F.size()) || dalloc(&m.B, B.size()) || dalloc(&m.Fs, F.size()) || dalloc(&m.Bs, B.size()) ||
+            dalloc(&m.maps, (size_t)n + 256))
             return nullptr;
-        hipMemcpy(m.F, F.data(), F.size() * 8, hipMemcpyHostToDevice);
-        hipMemcpy(m.B, B.data(), B.size() * 8, hipMemcpyHostToDevice);
+        // on the context stream (a null-stream copy is not ordered against it), complete before the
+        // host vectors go
+        if (hipMemcpyAsync(m.F, F.data(), F.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+            hipMemcpyAsync(m.B, B.data(), B.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+            hipStreamSynchronize(c->st) != hipSuccess)
+            return nullptr;
         it = c->nm.emplace(n, m).first;
     }
     NMMat& m = it->second;
@@ -312,8 +352,25 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
         const double sB = sqrt((double)n / (double)nb_full), sF = sqrt((double)nb_full / (double)n);
         for (auto& x : B) x *= sB;
         for (auto& x : F) x *= sF;
-        hipMemcpy(m.Fs, F.data(), F.size() * 8, hipMemcpyHostToDevice);
-        hipMemcpy(m.Bs, B.data(), B.size() * 8, hipMemcpyHostToDevice);
+        // ring_contract keeps modes 0..h and their mirrors nb-h..nb-1 (src/evolve_pi.f90:417-425);
+        // force_contract puts contracted mode k back at k and n - k at nb - k (:388-401)
+        std::vector<int> mp((size_t)n + nb_full, -1);
+        int* mi = mp.data();
+        int* mo = mp.data() + n;
+        const int h = (n - 1) / 2;
+        mi[0] = 0;
+        mo[0] = 0;
+        for (int k = 1; k <= h; k++) {
+            mi[k] = k;
+            mi[n - k] = nb_full - k;
+            mo[k] = k;
+            mo[nb_full - k] = n - k;
+        }
+        if (hipMemcpyAsync(m.Fs, F.data(), F.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+            hipMemcpyAsync(m.Bs, B.data(), B.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+            hipMemcpyAsync(m.maps, mp.data(), mp.size() * sizeof(int), hipMemcpyHostToDevice, c->st) != hipSuccess ||
+            hipStreamSynchronize(c->st) != hipSuccess)
+            return nullptr;
         m.scaled_nb = nb_full;
     }
     return &m;
@@ -341,12 +398,12 @@ void free_vcache(rpmd_ctx* c) {
 // the Verlet-list cache for this bead count (least recently used slot otherwise); allocated, or
 // grown, outside of stream capture only; *force = 1 when the cached lists belong to another set-up
 int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, const Box& sb, VList** out,
-              int* force) {
-    const int na = c->na, nbl = c->nb_max;
+              int* force, int host) {
+    const int na = c->na, nbl = c->nb_max, s0 = host ? 2 : 0;
     int slot = -1;
-    for (int i = 0; i < 2; i++)
+    for (int i = s0; i < s0 + 2; i++)
         if (c->vc[i].alloc && c->vc[i].nb == nb) slot = i;
-    if (slot < 0) slot = c->vc[0].last_use <= c->vc[1].last_use ? 0 : 1;
+    if (slot < 0) slot = c->vc[s0].last_use <= c->vc[s0 + 1].last_use ? s0 : s0 + 1;
     auto& x = c->vc[slot];
     x.last_use = ++c->vc_clock;
     const double key[5] = {box[0], box[1], box[2], rc, skin};
@@ -384,11 +441,13 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
             return RPMD_ERR_NOMEM;
         }
         v.cap = cap;
-        // reference positions NaN (all bits set): the first call always builds
-        HIPCHK(hipMemset(v.rb, 0xff, sizeof(double) * 3 * ns));
-        HIPCHK(hipMemset(v.gate, 0, sizeof(int) * 2));
-        HIPCHK(hipMemset(v.dmax, 0, sizeof(unsigned long long)));
-        HIPCHK(hipMemset(v.nreb, 0, sizeof(int)));
+        // reference positions NaN (all bits set): the first call always builds.  Filled on the
+        // context stream, ahead of the kernels that use them (a null-stream memset is not ordered
+        // against this non-blocking stream and could land after the first build)
+        launch_fill32(v.rb, 0xffffffffu, 2 * 3 * ns, c->st);
+        launch_fill32(v.gate, 0u, 2, c->st);
+        launch_fill32(v.dmax, 0u, 2, c->st);
+        launch_fill32(v.nreb, 0u, 1, c->st);
         x.v = v;
         x.alloc = true;
         *force = 1;
@@ -495,7 +554,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         const double skin = (cubic ? 1.0 / box[0] : 1.0) * (1.0 / 0.5291772108);
         VList* vl = nullptr;
         int force = 0;
-        RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force));
+        RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force, check_lists ? 1 : 0));
         // the real-space branch (sites, list upkeep and pairs) runs on the second stream, beside the
         // k-space kernels on the first, which then form the phases themselves, unless the lists are
         // checked on the host or kernels are being timed
@@ -693,7 +752,8 @@ int upload_charges(rpmd_ctx* c, const double* z) {
         c->hq.assign(z, z + na);
         c->sumq2 = 0.0;
         for (int i = 0; i < na; i++) c->sumq2 = c->sumq2 + z[i] * z[i];
-        HIPCHK(hipMemcpy(c->q, z, sizeof(double) * na, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->q, z, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
     }
     return 0;
 }
@@ -734,23 +794,12 @@ int rp_contract_dev(rpmd_ctx* c, const double* rnm, int nb, int nbdf, const doub
     if ((nbdf - 1) % 2 != 0) return RPMD_ERR_NBDF_EVEN;
     if (nbdf > nb) return RPMD_ERR_NBDF_TOO_LARGE;
     const int na = c->na;
+    // matrices and mode maps of (nbdf, nb) are made once, outside stream capture (the first,
+    // eager step of a run does it), and stay on the device
     NMMat* m = get_nm(c, nbdf, nb);
     if (!m) return RPMD_ERR_NOMEM;
-    // maps: in (nbdf entries) and out (nb entries)
-    std::vector<int> mi(nbdf), mo(nb, -1);
-    const int h = (nbdf - 1) / 2;
-    mi[0] = 0;
-    mo[0] = 0;
-    for (int k = 1; k <= h; k++) {
-        mi[k] = k;
-        mi[nbdf - k] = nb - k;
-        mo[k] = k;
-        mo[nb - k] = nbdf - k;
-    }
-    int* dmi = c->maps;
-    int* dmo = c->maps + 512;
-    HIPCHK(hipMemcpyAsync(dmi, mi.data(), sizeof(int) * nbdf, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dmo, mo.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
+    const int* dmi = m->maps;
+    const int* dmo = m->maps + nbdf;
     launch_mode_copy(c->tmp2, rnm, dmi, nbdf, na, 1.0, c->st);                 // ring_contract :417-425
     launch_nm(c->tmp2, c->rb, m->Bs, nbdf, (size_t)3 * na, c->st);           // realft(-1) * sqrt(nbdf/nb)
     RETCHK(forces_dev(c, c->rb, nbdf, box, iopt, c->dvdrb, ev_slot, check));  // :366
@@ -1331,7 +1380,6 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->kmask, (size_t)(RPMD_KMAX_DEV + 1) * RPMD_KTILE * RPMD_KTILE);
     e |= dalloc(&c->cell_of, (size_t)na * nbl);
     e |= dalloc(&c->cnt, (size_t)(c->ncell_cap + 1) * nbl);
-    if (!e && hipMemset(c->cnt, 0, sizeof(int) * (c->ncell_cap + 1) * nbl) != hipSuccess) e = RPMD_ERR_HIP;
     e |= dalloc(&c->start, (size_t)(c->ncell_cap + 1) * nbl);
     e |= dalloc(&c->perm, (size_t)na * nbl);
     e |= dalloc(&c->nnb, na);
@@ -1342,9 +1390,6 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->oo_ovf, 1);
     e |= dalloc(&c->oo_cb, (size_t)3 * (na / 3 + 1));
     e |= dalloc(&c->oo_dmax, 1);
-    if (!e && hipMemset(c->oo_ovf, 0, sizeof(int)) != hipSuccess) e = RPMD_ERR_HIP;
-    if (!e && hipMemset(c->oo_dmax, 0, sizeof(unsigned long long)) != hipSuccess) e = RPMD_ERR_HIP;
-    if (!e && hipMemset(c->oo_cb, 0xff, sizeof(double) * 3 * (na / 3 + 1)) != hipSuccess) e = RPMD_ERR_HIP;
     for (double** b : {&c->p, &c->r, &c->dvdr, &c->dvdr2, &c->dvdrl, &c->dvdre, &c->rst, &c->pnm, &c->rnm, &c->tmp,
                        &c->rb, &c->dvdrb, &c->tmp2})
         e |= dalloc(b, N);
@@ -1355,7 +1400,6 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
     e |= dalloc(&c->com_parts2, (size_t)3 * pile_blocks(na));
     e |= dalloc(&c->pstep_parts, (size_t)6 * pile_step_blocks(na));
     e |= dalloc(&c->lang_c3m, na);
-    e |= dalloc(&c->maps, 1024);
     e |= dalloc(&c->step_ctr, 1);
     e |= dalloc(&c->bussi_parts, (size_t)bussi_parts_size());
     e |= dalloc(&c->bussi_alpha, 1);
@@ -1368,7 +1412,16 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
         rpmd_ctx_destroy(c);
         return RPMD_ERR_NOMEM;
     }
-    hipMemset(c->step_ctr, 0, sizeof(uint64_t));
+    // initial contents, in the context stream's order (see vlist_for)
+    launch_fill32(c->cnt, 0u, (size_t)(c->ncell_cap + 1) * nbl, c->st);
+    launch_fill32(c->oo_ovf, 0u, 1, c->st);
+    launch_fill32(c->oo_dmax, 0u, 2, c->st);
+    launch_fill32(c->oo_cb, 0xffffffffu, (size_t)2 * 3 * (na / 3 + 1), c->st);
+    launch_fill32(c->step_ctr, 0u, 2, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        rpmd_ctx_destroy(c);
+        return RPMD_ERR_HIP;
+    }
     *out = c;
     return 0;
 }
@@ -1378,30 +1431,27 @@ int rpmd_ctx_set_params(rpmd_ctx* c, const rpmd_params* prm) {
     c->prm = *prm;
     c->ks_valid = false;
     c->st_valid = false;
+    invalidate(c);
     return 0;
 }
 
 int rpmd_ctx_set_precision(rpmd_ctx* c, int precision) {
     if (!c) return RPMD_ERR_ARG;
     c->prec = precision;
-    if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    invalidate(c);
     return 0;
 }
 
 int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    drop_graph(c);
     free_vcache(c);
     void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf,
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
                     c->r, c->dvdr, c->dvdr2, c->dvdrl, c->dvdre, c->rst, c->pnm, c->rnm, c->tmp, c->rb, c->dvdrb,
-                    c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->maps, c->noise,
+                    c->tmp2, c->monod, c->pile_c1, c->pile_c3, c->com_parts, c->lang_c3m, c->noise,
                     c->step_ctr, c->ev_step, c->goo, c->com_parts2, c->pstep_parts};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -1410,6 +1460,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
         hipFree(kv.second.B);
         hipFree(kv.second.Fs);
         hipFree(kv.second.Bs);
+        hipFree(kv.second.maps);
     }
     if (c->h_stage) hipHostFree(c->h_stage);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1424,6 +1475,8 @@ int rpmd_forces(rpmd_ctx* c, const double* r, int nb, int na, const double boxlx
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
     if (iopt < 0 || iopt > 5) return RPMD_ERR_IOPT;
+    // the call shares the lists, k set and charges with a device-resident state: graphs re-capture
+    invalidate(c);
     RETCHK(upload_charges(c, z));
     RETCHK(upload_aos(c, r, c->pos, nb));
     RETCHK(forces_dev(c, c->pos, nb, boxlxyz, iopt, c->grad, c->ev, true));
@@ -1440,63 +1493,68 @@ int rpmd_full_forces(rpmd_ctx* c, const double* r, int na, int nb, double* v, do
     const rpmd_params& P = c->prm;
     const size_t N = (size_t)3 * na * nb;
     hipStream_t s = c->st;
+    // a device-resident state keeps p, r, dvdr and dvdr2 between steps; this call works in buffers
+    // the state does not own (the contraction's own scratch, rnm, tmp2, rb and dvdrb, is
+    // rewritten by every step), so it never disturbs a state, but graphs are captured anew
+    invalidate(c);
+    double *gl = c->grad, *gh = c->pnm, *go = c->dvdrl, *ge = c->dvdre, *gt = c->tmp;
     RETCHK(upload_charges(c, z));
     RETCHK(upload_aos(c, r, c->pos, nb));
     double e_int, w_int[9], e_ew = 0, w_ew[9] = {0}, e_oo = 0, w_oo[9] = {0}, tv, tw[9];
     // high-frequency forces (src/md_forces.f90:350)
-    RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 4, c->dvdr2, c->ev, true));
+    RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 4, gh, c->ev, true));
     RETCHK(read_ev(c, c->ev, 4, boxlxyz, &e_int, w_int));
-    launch_zero(c->dvdr, N, s);
-    launch_zero(c->dvdrl, N, s);
+    launch_zero(gl, N, s);
+    launch_zero(go, N, s);
     const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
     if (nb > 1 && (nbdf1 > 0 || nbdf2 > 0)) {
         NMMat* m = get_nm(c, nb, 0);
         if (!m) return RPMD_ERR_NOMEM;
         launch_nm(c->pos, c->rnm, m->F, nb, (size_t)3 * na, s);  // :359
         if (nbdf1 > 0) {
-            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 2, c->dvdr, c->ev, true));
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 2, gl, c->ev, true));
             RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
             if (P.sig > 0.0) {
-                RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 5, c->dvdre, c->ev, true));
+                RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 5, ge, c->ev, true));
                 RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
                 e_ew -= tv;
                 for (int k = 0; k < 9; k++) w_ew[k] -= tw[k];
-                launch_axpy(c->dvdr, c->dvdre, -1.0, N, s);
+                launch_axpy(gl, ge, -1.0, N, s);
             }
         }
         if (nbdf2 > 0) {
-            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, boxlxyz, 3, c->dvdrl, c->ev, true));
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, boxlxyz, 3, go, c->ev, true));
             RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
         }
         if (nbdf1 > 0 && nbdf2 > 0) {
-            launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
-            launch_nm(c->dvdr, c->tmp, m->B, nb, (size_t)3 * na, s);
-            std::swap(c->dvdr, c->tmp);
+            launch_axpy(gl, go, 1.0, N, s);
+            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(gl, gt);
         } else if (nbdf1 > 0) {
-            launch_nm(c->dvdr, c->tmp, m->B, nb, (size_t)3 * na, s);
-            std::swap(c->dvdr, c->tmp);
+            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(gl, gt);
         } else if (nbdf2 > 0) {
-            launch_nm(c->dvdrl, c->tmp, m->B, nb, (size_t)3 * na, s);
-            std::swap(c->dvdrl, c->tmp);
+            launch_nm(go, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(go, gt);
         }
     }
     if (nb == 1 || nbdf1 == 0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 2, c->dvdr, c->ev, true));
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 2, gl, c->ev, true));
         RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
     } else if (P.sig > 0.0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 5, c->dvdre, c->ev, true));
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 5, ge, c->ev, true));
         RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
         e_ew += tv;
         for (int k = 0; k < 9; k++) w_ew[k] += tw[k];
-        launch_axpy(c->dvdr, c->dvdre, 1.0, N, s);
+        launch_axpy(gl, ge, 1.0, N, s);
     }
     if (nb == 1 || nbdf2 == 0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 3, c->dvdrl, c->ev, true));
+        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 3, go, c->ev, true));
         RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
     }
-    if (nb == 1 || nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(c->dvdr, c->dvdrl, 1.0, N, s);
-    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
-    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    if (nb == 1 || nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(gl, go, 1.0, N, s);
+    RETCHK(download_aos(c, gl, dvdr, nb));
+    RETCHK(download_aos(c, gh, dvdr2, nb));
     if (vew) *vew = e_ew;
     if (voo) *voo = e_oo;
     if (vint) *vint = e_int;
@@ -1524,6 +1582,22 @@ int rpmd_realft(rpmd_ctx* c, double* data, int m, int n, int mode) {
     return 0;
 }
 
+int rpmd_free_ring(rpmd_ctx* c, double* p, double* r, const double* mass, int na, int nb, double dt, double beta) {
+    if (!c || !p || !r || !mass || nb < 1 || beta <= 0.0) return RPMD_ERR_ARG;
+    if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
+    if (na != c->na || nb > c->nb_max) return RPMD_ERR_ARG;
+    // scratch buffers only (pnm, rnm, and the monodromy in tmp2): a device-resident state is untouched
+    std::vector<double> monod((size_t)12 * nb);
+    ring_rpmd(mass, monod.data(), nb, dt, beta);
+    HIPCHK(hipMemcpyAsync(c->tmp2, monod.data(), monod.size() * 8, hipMemcpyHostToDevice, c->st));
+    RETCHK(upload_aos(c, p, c->pnm, nb));
+    RETCHK(upload_aos(c, r, c->rnm, nb));
+    launch_free_ring(c->pnm, c->rnm, c->tmp2, na, nb, c->st);
+    RETCHK(download_aos(c, c->pnm, p, nb));
+    RETCHK(download_aos(c, c->rnm, r, nb));
+    return 0;
+}
+
 int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint, double* dvdr,
                    double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
                    const double* z, double beta, double* vir, double* vir_lf, const double* noise, long long n_noise) {
@@ -1533,6 +1607,9 @@ int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, do
     const size_t need = noise_per_step(c->prm, na, nb, c->gle_n1);
     if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
     RETCHK(upload_charges(c, z));
+    // one host round trip: the uploaded arrays replace any device-resident state
+    c->st_valid = false;
+    invalidate(c);
     c->st_nb = nb;
     c->st_box = make_box(boxlxyz);
     c->st_beta = beta;
@@ -1575,6 +1652,9 @@ int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_l
     const size_t need = noise_per_step(c->prm, na, nb, 0);
     if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
     RETCHK(upload_charges(c, z));
+    // one host round trip: the uploaded arrays replace any device-resident state
+    c->st_valid = false;
+    invalidate(c);
     c->st_nb = nb;
     c->st_box = make_box(boxlxyz);
     c->st_beta = beta;
@@ -1620,7 +1700,9 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
     if (!c || !p || !r || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
+    invalidate(c);
     RETCHK(upload_charges(c, z));
+    c->st_z.assign(z, z + na);
     c->st_nb = nb;
     c->st_box = make_box(boxlxyz);
     c->st_beta = beta;
@@ -1629,7 +1711,7 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
     RETCHK(set_state_consts(c, mass, beta, dt, nb));
     RETCHK(upload_aos(c, p, c->p, nb));
     RETCHK(upload_aos(c, r, c->r, nb));
-    HIPCHK(hipMemsetAsync(c->step_ctr, 0, sizeof(uint64_t), c->st));
+    launch_fill32(c->step_ctr, 0u, 2, c->st);
     launch_zero_int(c->rat_err, 1, c->st);
     // initial forces: full_forces semantics for the uncontracted case (src/md_forces.f90:305-472)
     const size_t N = (size_t)3 * na * nb;
@@ -1638,10 +1720,6 @@ int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const doubl
     RETCHK(forces_dev(c, c->r, nb, boxlxyz, 3, c->dvdrl, c->ev, true));
     launch_axpy(c->dvdr, c->dvdrl, 1.0, N, c->st);
     HIPCHK(hipStreamSynchronize(c->st));
-    if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
     c->st_valid = true;
     return 0;
 }
@@ -1666,10 +1744,11 @@ int rpmd_sample_rpmd(rpmd_ctx* c, double* p, int na, int nb, const double* mass,
     double totmass = 0.0;
     for (int j = 0; j < na; j++) totmass = totmass + mass[j];
     totmass = (double)nb * totmass;
-    HIPCHK(hipMemcpyAsync(c->mass, mass, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
-    // scratch: p in tmp2; the per-block partial sums (3 per block) and the shift velocity in tmp
+    // scratch only (a device-resident state keeps its own masses): the masses in rnm, p in tmp2,
+    // the per-block partial sums (3 per block) and the shift velocity in tmp
+    HIPCHK(hipMemcpyAsync(c->rnm, mass, sizeof(double) * na, hipMemcpyHostToDevice, c->st));
     const size_t nparts = (size_t)3 * pile_blocks(na) * nb;
-    launch_sample_rpmd(c->tmp2, c->mass, na, nb, beta, totmass, dn, seed, ~1ull, c->tmp, c->tmp + nparts, c->st);
+    launch_sample_rpmd(c->tmp2, c->rnm, na, nb, beta, totmass, dn, seed, ~1ull, c->tmp, c->tmp + nparts, c->st);
     RETCHK(download_aos(c, c->tmp2, p, nb));
     return 0;
 }
@@ -1709,30 +1788,21 @@ int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double d
     }
     c->gle_n1 = n1;
     c->gle_nb = nb;
-    if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    invalidate(c);
     return 0;
 }
 
 int rpmd_ctx_set_streams(rpmd_ctx* c, int streams) {
     if (!c || streams < 1 || streams > 3) return RPMD_ERR_ARG;
     c->streams = streams;
-    if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    drop_graph(c);  // replica batches run one stream per replica and keep theirs
     return 0;
 }
 
 int rpmd_use_graph(rpmd_ctx* c, int enable) {
     if (!c) return RPMD_ERR_ARG;
     c->use_graph = enable;
-    if (!enable && c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    if (!enable) drop_graph(c);
     return 0;
 }
 
@@ -1746,51 +1816,60 @@ int step_any(rpmd_ctx* c, NoiseCursor& nz) { return c->prm.rigid ? step_rig_dev(
 
 int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
     if (!c || !c->st_valid || nsteps < 0) return RPMD_ERR_ARG;
-    if (c->prm.nbdf1 > 0 || c->prm.nbdf2 > 0) {
-        // contraction maps are uploaded per call: run eagerly
-        for (int i = 0; i < nsteps; i++) {
+    // the state's own charges (a host call may have uploaded others; then the epoch moved too)
+    RETCHK(upload_charges(c, c->st_z.data()));
+    auto eager = [&](int k) -> int {
+        for (int i = 0; i < k; i++) {
             NoiseCursor nz{nullptr, 0, 0, 0};
             RETCHK(step_any(c, nz));
         }
-        HIPCHK(hipStreamSynchronize(c->st));
-        return run_status(c);
-    }
-    if (!c->use_graph || c->prof) {
-        for (int i = 0; i < nsteps; i++) {
-            NoiseCursor nz{nullptr, 0, 0, 0};
-            RETCHK(step_any(c, nz));
-        }
-        HIPCHK(hipStreamSynchronize(c->st));
-        return run_status(c);
-    }
-    // graph path: one step per replay; the step swaps r/rst (and dvdr/tmp), so capture two steps
-    // to return the buffers to their starting roles
-    if (!c->gexec) {
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < 2; i++) {
-            NoiseCursor nz{nullptr, 0, 0, 0};
-            int rc = step_any(c, nz);
-            if (rc) {
-                hipGraph_t g;
-                hipStreamEndCapture(c->st, &g);
-                if (g) hipGraphDestroy(g);
-                return rc;
+        return 0;
+    };
+    int left = nsteps;
+    if (c->use_graph && !c->prof && left >= 2) {
+        if (c->gexec && c->g_epoch != c->epoch) drop_graph(c);
+        if (!c->gexec) {
+            // one eager step first: it brings every list, table and key up to date outside the
+            // capture, so the captured steps hold no forced rebuild; then two steps are captured
+            // (each step exchanges r and rst, so two return the buffers to their roles)
+            RETCHK(eager(1));
+            left--;
+            if (left >= 2) {
+                HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                c->g_roles = roles_of(c);
+                int rc = 0;
+                for (int i = 0; i < 2 && !rc; i++) {
+                    NoiseCursor nz{nullptr, 0, 0, 0};
+                    rc = step_any(c, nz);
+                }
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(c->st, &g);
+                if (rc || e != hipSuccess) {
+                    if (g) hipGraphDestroy(g);
+                    // capture leaves the host-side roles as after two steps: restore them
+                    c->r = c->g_roles.a[0], c->rst = c->g_roles.a[1], c->dvdr = c->g_roles.a[2];
+                    c->dvdrl = c->g_roles.a[3], c->tmp = c->g_roles.a[4];
+                    return rc ? rc : RPMD_ERR_HIP;
+                }
+                c->graph = g;
+                if (hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0) != hipSuccess) {
+                    drop_graph(c);
+                    return RPMD_ERR_HIP;
+                }
+                c->g_epoch = c->epoch;
+                // capturing ran no kernel but exchanged the roles twice: they are as captured
             }
         }
-        HIPCHK(hipStreamEndCapture(c->st, &c->graph));
-        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        if (c->gexec && left >= 2 && roles_of(c) != c->g_roles) {
+            // an odd number of steps ran since the capture: one eager step brings the roles back
+            RETCHK(eager(1));
+            left--;
+            if (roles_of(c) != c->g_roles) drop_graph(c);
+        }
+        if (c->gexec && roles_of(c) == c->g_roles)
+            for (; left >= 2; left -= 2) HIPCHK(hipGraphLaunch(c->gexec, c->st));
     }
-    int i = 0;
-    for (; i + 1 < nsteps; i += 2) HIPCHK(hipGraphLaunch(c->gexec, c->st));
-    if (i < nsteps) {
-        NoiseCursor nz{nullptr, 0, 0, 0};
-        // an odd tail runs eagerly and leaves buffer roles swapped; the graph is rebuilt next time
-        RETCHK(step_any(c, nz));
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-        hipGraphDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    RETCHK(eager(left));
     HIPCHK(hipStreamSynchronize(c->st));
     return run_status(c);
 }
@@ -1806,13 +1885,15 @@ struct rpmd_batch {
     std::vector<hipEvent_t> ev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // per replica at capture: the context's epoch and buffer roles
+    std::vector<uint64_t> epoch;
+    std::vector<Roles> roles;
 };
 
 int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out) {
     if (!ctxs || n < 1 || !out) return RPMD_ERR_ARG;
     for (int i = 0; i < n; i++) {
         if (!ctxs[i] || !ctxs[i]->st_valid) return RPMD_ERR_ARG;
-        if (ctxs[i]->prm.nbdf1 > 0 || ctxs[i]->prm.nbdf2 > 0) return RPMD_ERR_ARG;  // contraction runs eagerly
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return RPMD_ERR_ARG;
     }
@@ -1832,10 +1913,16 @@ int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out) {
     return 0;
 }
 
-int rpmd_batch_destroy(rpmd_batch* b) {
-    if (!b) return 0;
+static void batch_drop_graph(rpmd_batch* b) {
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+int rpmd_batch_destroy(rpmd_batch* b) {
+    if (!b) return 0;
+    batch_drop_graph(b);
     for (hipEvent_t e : b->ev)
         if (e) hipEventDestroy(e);
     if (b->st) hipStreamDestroy(b->st);
@@ -1846,19 +1933,23 @@ int rpmd_batch_destroy(rpmd_batch* b) {
 int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
     if (!b || nsteps < 0) return RPMD_ERR_ARG;
     const int n = (int)b->ctx.size();
-    for (rpmd_ctx* c : b->ctx)
+    for (rpmd_ctx* c : b->ctx) {
         if (!c->st_valid) return RPMD_ERR_ARG;
+        RETCHK(upload_charges(c, c->st_z.data()));
+    }
     // each replica runs on its own context stream only: the replicas already run side by side, and
-    // a second stream per replica made the batch 1.8x slower (48 replicas of config 3)
-    auto fork_steps = [&](int k) -> int {
+    // a second stream per replica made the batch 1.8x slower (48 replicas of config 3).
+    // k[i] steps of replica i, forked from and joined back to the batch stream.
+    auto fork_steps = [&](const std::vector<int>& k) -> int {
         HIPCHK(hipEventRecord(b->ev[2 * n], b->st));
         for (int i = 0; i < n; i++) {
+            if (k[i] <= 0) continue;
             rpmd_ctx* c = b->ctx[i];
             HIPCHK(hipStreamWaitEvent(c->st, b->ev[2 * n], 0));
             const int streams = c->streams;
             c->streams = 1;
             int rc = 0;
-            for (int q = 0; q < k && !rc; q++) {
+            for (int q = 0; q < k[i] && !rc; q++) {
                 NoiseCursor nz{nullptr, 0, 0, 0};
                 rc = step_any(c, nz);
             }
@@ -1869,28 +1960,67 @@ int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
         }
         return 0;
     };
-    if (!b->gexec && nsteps >= 2) {
-        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-        int rc = fork_steps(2);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->st, &g);
-        if (rc || e != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return rc ? rc : RPMD_ERR_HIP;
+    std::vector<int> left(n, nsteps), one(n, 1);
+    if (nsteps >= 2) {
+        bool valid = b->gexec != nullptr;
+        for (int i = 0; i < n && valid; i++) valid = b->epoch[i] == b->ctx[i]->epoch;
+        if (!valid) {
+            batch_drop_graph(b);
+            // one eager step of every replica first (lists and keys up to date outside the
+            // capture), then two steps of every replica captured
+            RETCHK(fork_steps(one));
+            for (int& x : left) x--;
+            if (nsteps - 1 >= 2) {
+                b->epoch.resize(n);
+                b->roles.resize(n);
+                for (int i = 0; i < n; i++) {
+                    b->epoch[i] = b->ctx[i]->epoch;
+                    b->roles[i] = roles_of(b->ctx[i]);
+                }
+                HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                int rc = fork_steps(std::vector<int>(n, 2));
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(b->st, &g);
+                if (rc || e != hipSuccess) {
+                    if (g) hipGraphDestroy(g);
+                    for (int i = 0; i < n; i++) {
+                        rpmd_ctx* c = b->ctx[i];
+                        const Roles& R = b->roles[i];
+                        c->r = R.a[0], c->rst = R.a[1], c->dvdr = R.a[2], c->dvdrl = R.a[3], c->tmp = R.a[4];
+                    }
+                    return rc ? rc : RPMD_ERR_HIP;
+                }
+                b->graph = g;
+                if (hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+                    batch_drop_graph(b);
+                    return RPMD_ERR_HIP;
+                }
+            }
         }
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        if (b->gexec) {
+            // replicas stepped an odd number of times since the capture (solo calls): one eager
+            // step each brings their roles back
+            std::vector<int> flip(n, 0);
+            bool any = false;
+            for (int i = 0; i < n; i++)
+                if (roles_of(b->ctx[i]) != b->roles[i] && left[i] > 0) flip[i] = 1, any = true;
+            if (any) {
+                RETCHK(fork_steps(flip));
+                for (int i = 0; i < n; i++) left[i] -= flip[i];
+            }
+            bool ok = true;
+            for (int i = 0; i < n; i++) ok = ok && roles_of(b->ctx[i]) == b->roles[i];
+            if (!ok) batch_drop_graph(b);
+        }
+        if (b->gexec) {
+            int m = *std::min_element(left.begin(), left.end());
+            for (; m >= 2; m -= 2) {
+                HIPCHK(hipGraphLaunch(b->gexec, b->st));
+                for (int& x : left) x -= 2;
+            }
+        }
     }
-    int i = 0;
-    for (; i + 1 < nsteps; i += 2) HIPCHK(hipGraphLaunch(b->gexec, b->st));
-    if (i < nsteps) {
-        // an odd tail runs eagerly and leaves buffer roles swapped; the graph is rebuilt next time
-        RETCHK(fork_steps(1));
-        if (b->gexec) hipGraphExecDestroy(b->gexec);
-        if (b->graph) hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
-    }
+    RETCHK(fork_steps(left));
     HIPCHK(hipStreamSynchronize(b->st));
     for (rpmd_ctx* c : b->ctx) {
         HIPCHK(hipStreamSynchronize(c->st));
@@ -1903,10 +2033,6 @@ int rpmd_prof_begin(rpmd_ctx* c) {
     if (!c) return RPMD_ERR_ARG;
     c->prof = 1;
     c->ev_marks.clear();
-    if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
     return 0;
 }
 

@@ -19,6 +19,27 @@
 
 namespace rpmd {
 
+// Control words that one kernel of a step writes and a later kernel reads -- the list displacement
+// gates (max-reduced, read, cleared), the rebuild count, the noise-stream counter -- are accessed
+// with agent-scope atomics only: the accesses go to the device-coherent level, never through a
+// per-CU cache or an XCD-local L2 line, whichever CUs and XCDs the kernels of a step land on
+template <typename T>
+__device__ __forceinline__ T ctrl_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void ctrl_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void ctrl_add(T* p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void ctrl_max(T* p, T v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct Box {
     double L[3], inv[3];
 };
@@ -82,13 +103,16 @@ void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const do
 void launch_axpy(double* y, const double* x, double a, size_t n, hipStream_t s);
 void launch_zero(double* y, size_t n, hipStream_t s);
 void launch_zero_int(int* y, size_t n, hipStream_t s);
+// every 32-bit word of y set to v, in stream order (initialisation of buffers that kernels on the same
+// stream use next: a blocking memset on the null stream is not ordered against a non-blocking stream)
+void launch_fill32(void* y, uint32_t v, size_t nwords, hipStream_t s);
 // the rebuild predicate of a Verlet list, evaluated on the device by every build kernel: forced, or
 // the largest squared site displacement since the last build (bits in *dmax) above lim2
 struct VGate {
     const unsigned long long* dmax;
     double lim2;
     int force;
-    __device__ __forceinline__ bool on() const { return force || __longlong_as_double((long long)*dmax) > lim2; }
+    __device__ __forceinline__ bool on() const { return force || __longlong_as_double((long long)ctrl_load(dmax)) > lim2; }
 };
 // Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
 // each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads

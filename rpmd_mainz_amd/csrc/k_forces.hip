@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_chain(double* __restrict__ grad, const 
     }
     int nm = na / 3;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ctr && i == 0) *ctr += RPMD_STREAMS_PER_STEP;
+    if (ctr && i == 0) ctrl_add(ctr, (uint64_t)RPMD_STREAMS_PER_STEP);
     if (i >= (size_t)nm * nb) return;
     size_t b = i / nm, m = i % nm;
     double alpha2 = 0.5 * (1.0 - alpha);
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_centroid_vl(const double* __restrict__ 
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+        ctrl_max(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 void launch_centroid_vl(const double* pos, double* cen, int na, int nb, const double* cb, unsigned long long* dmax,
                         hipStream_t s) {
@@ -247,6 +247,13 @@ __global__ void k_zero_int(int* y, size_t n) {
 }
 void launch_zero_int(int* y, size_t n, hipStream_t s) {
     if (n) k_zero_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, n);
+}
+__global__ void k_fill32(uint32_t* y, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+void launch_fill32(void* y, uint32_t v, size_t nwords, hipStream_t s) {
+    if (nwords) k_fill32<<<(unsigned)((nwords + 255) / 256), 256, 0, s>>>((uint32_t*)y, v, nwords);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicMax(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+        ctrl_max(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
 // One block, when the gate is on: a counting sort of the site centroids (cu, from k_sites_vl) into
@@ -465,9 +472,9 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
     // order is kept (any order gives the same pair set; the sort only keeps neighbours close)
     __shared__ int resort;
     if (t == 0) {
-        const int k = nreb[0];
+        const int k = ctrl_load(nreb);
         resort = gate.force || k % VL_RESORT == 0;
-        nreb[0] = k + 1;
+        ctrl_store(nreb, k + 1);
     }
     __syncthreads();
     if (!resort) {
@@ -823,7 +830,7 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
     const int id = blockIdx.x, m8 = id >> 3;
     const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
     const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
-    if (id == 0 && threadIdx.x == 0) dmax[0] = 0ull;
+    if (id == 0 && threadIdx.x == 0) ctrl_store(dmax, 0ull);
     // the FP64 energy and virial sums of each thread live in LDS (fewer registers, more waves)
     __shared__ double vacc[7][256];
 #pragma unroll
@@ -2078,7 +2085,7 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int ii = t / OOL, sl = t % OOL;
     const int n = na / 3;
-    if (dmax && t == 0 && b == 0) dmax[0] = 0ull;
+    if (dmax && t == 0 && b == 0) ctrl_store(dmax, 0ull);
     double acc[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) acc[k] = 0.0;

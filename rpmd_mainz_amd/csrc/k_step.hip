@@ -183,7 +183,7 @@ void launch_nm_com2(const double* rnm, double* rst, const double* pnm, double* p
 __global__ void k_kick(double* __restrict__ p, const double* __restrict__ g1, double a1, const double* __restrict__ g2,
                        double a2, size_t n, uint64_t* __restrict__ ctr) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ctr && i == 0) *ctr += RPMD_STREAMS_PER_STEP;
+    if (ctr && i == 0) ctrl_add(ctr, (uint64_t)RPMD_STREAMS_PER_STEP);
     if (i >= n) return;
     double v = p[i];
     if (g1) v = __dadd_rn(v, -__dmul_rn(a1, g1[i]));
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const doub
                                               double* __restrict__ com_parts) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y + k0;
-    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
     double s[3] = {0.0, 0.0, 0.0};
     if (a < na) {
         const int ic = a % 3;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_free_ring_pile(double* __restrict__ p, 
                                                         double* __restrict__ com_out) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
     double s[3] = {0.0, 0.0, 0.0};
     if (a < na) {
         const int ic = a % 3;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict
     }
     // this thread's two (mode, column) entries: coefficients and deviates, loaded and drawn before
     // the transform results are waited for
-    const uint64_t s0 = streamp ? *streamp : 0;
+    const uint64_t s0 = streamp ? ctrl_load(streamp) : 0;
     double ec1[2], ec3[2], emo[2][4], ex1[2], ex2[2];
     int ed[2];
     bool eok[2];
@@ -663,7 +663,7 @@ __global__ void k_langevin(double* __restrict__ p, const double* __restrict__ c3
                            const double* __restrict__ noise, uint64_t seed, const uint64_t* __restrict__ streamp,
                            uint64_t stream_add) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
     if (i >= (size_t)na * nb) return;
     const size_t k = i / na, a = i % na;
 #pragma unroll
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(256) void k_bussi_parts(const double* __restrict__ 
                                                      const double* __restrict__ noise, uint64_t seed,
                                                      const uint64_t* __restrict__ streamp, uint64_t stream_add,
                                                      double* __restrict__ parts) {
-    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
     const size_t n = (size_t)3 * na * nbq;
     const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, nth = (size_t)gridDim.x * 256;
     double v[2] = {0.0, 0.0};
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) void k_bussi_alpha(const double* __restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+        const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
         const double kt = red[0][0], snfm1 = red[0][1];
         const double ranno = noise ? noise[ndof] : normal_at(seed, stream, (uint64_t)ndof);
         const double ransq = ranno * ranno;
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(256) void k_gle(const double* __restrict__ p, doubl
                                              double* __restrict__ parts) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    const uint64_t stream = (streamp ? *streamp : 0) + stream_add;
+    const uint64_t stream = (streamp ? ctrl_load(streamp) : 0) + stream_add;
     const size_t N = (size_t)3 * na * nb;
     double pc[N1 * 3];
 #pragma unroll
@@ -998,7 +998,7 @@ void launch_scale(double* y, double a, size_t n, hipStream_t s) {
 }
 
 // per-step noise stream counter kept on the device so that a captured step replays with fresh noise
-__global__ void k_counter_inc(uint64_t* c) { *c += RPMD_STREAMS_PER_STEP; }
+__global__ void k_counter_inc(uint64_t* c) { ctrl_add(c, (uint64_t)RPMD_STREAMS_PER_STEP); }
 void launch_counter_inc(uint64_t* c, hipStream_t s) { k_counter_inc<<<1, 1, 0, s>>>(c); }
 
 }  // namespace rpmd

@@ -1,0 +1,120 @@
+"""Device-resident state and captured graphs: replays stay bit-identical to eager steps whatever the
+order of solo runs, replica-batch runs and host-synchronous force calls on the same context.
+
+Each step exchanges the roles of some buffers (r and rst; in contracted steps dvdr or dvdrl and
+tmp), and host-synchronous calls (forces, full_forces, sample_rpmd) share tables and lists with
+the state.  These tests interleave them and compare the final state with plain eager stepping.
+"""
+import numpy as np
+import pytest
+
+from rpmd_mainz_amd import configs as cf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from rpmd_mainz_amd import capi
+    return capi
+
+
+def _system(nb=8, therm="PRO", **kw):
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), nb, ecut=1e-6, intra_model=cf.INTRA_SPCFW,
+                                           therm=therm, ttaufs=100.0, **kw)
+    na = r.shape[1]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    p = np.random.default_rng(21).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    return r, p, box, z, mass, prm, beta, dt, na, nb
+
+
+def _same(a, b):
+    return (np.array_equal(a["r"], b["r"]) and np.array_equal(a["p"], b["p"]) and np.array_equal(a["dvdr"], b["dvdr"])
+            and a["v"] == b["v"])
+
+
+def _eager(capi, n, sysargs, prec="fp64"):
+    r, p, box, z, mass, prm, beta, dt, na, nb = sysargs
+    c = capi.RPMDContext(prm, na, nb, prec)
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+    c.use_graph(False)
+    c.run_steps(n)
+    st = c.state_download()
+    c.close()
+    return st
+
+
+@pytest.mark.parametrize("order", [(2, 1, 2), (3, 4, 1), (1, 2, 3)])
+def test_solo_and_batch_graphs_interleaved(capi, order):
+    """Solo graph steps, then a replica batch holding the same context, then solo graph steps
+    again: the same bits as the same number of eager steps (a graph captured with the buffer roles
+    of an even step count is never replayed after an odd one)."""
+    sa = _system()
+    r, p, box, z, mass, prm, beta, dt, na, nb = sa
+    ref = _eager(capi, sum(order), sa)
+    c = capi.RPMDContext(prm, na, nb, "fp64")
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+    c.use_graph(True)
+    batch = capi.ReplicaBatch([c])
+    print(f"case order {order}", flush=True)
+    c.run_steps(order[0])
+    batch.run_steps(order[1])
+    c.run_steps(order[2])
+    assert _same(c.state_download(), ref)
+    batch.close()
+    c.close()
+
+
+def test_host_calls_between_steps_leave_the_state(capi):
+    """forces / full_forces / sample_rpmd on other positions (another box as well) between graph
+    steps of a device-resident state: the state continues as if they had not been called."""
+    sa = _system()
+    r, p, box, z, mass, prm, beta, dt, na, nb = sa
+    ref = _eager(capi, 6, sa)
+    c = capi.RPMDContext(prm, na, nb, "fp64")
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+    c.use_graph(True)
+    c.run_steps(3)
+    r2 = r + np.random.default_rng(3).normal(size=r.shape) * 0.05
+    box2 = box * 1.03
+    c.full_forces(r2, na, nb, z, box2)
+    c.forces(r2[:2], 2, na, box2, z, 0)
+    c.sample_rpmd(nb, na, mass * 1.5, beta)
+    c.run_steps(3)
+    assert _same(c.state_download(), ref)
+    c.close()
+
+
+def test_contracted_steps_in_a_graph(capi):
+    """RP-contracted MTS steps (Ewald on 5 modes, LJ on 7 of 16 beads, mts 2) replayed from a graph
+    give the same bits as eager steps; the mode maps live on the device, so nothing is uploaded
+    per step."""
+    sa = _system(nb=16, therm="PRO", nbdf1=5, nbdf2=7, mts=2)
+    r, p, box, z, mass, prm, beta, dt, na, nb = sa
+    ref = _eager(capi, 5, sa)
+    c = capi.RPMDContext(prm, na, nb, "fp64")
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+    c.use_graph(True)
+    c.run_steps(5)
+    assert _same(c.state_download(), ref)
+    c.close()
+
+
+def test_batch_with_a_contracted_replica(capi):
+    """A replica batch may hold contracted replicas beside plain ones; both match solo eager runs."""
+    s1 = _system(nb=16, nbdf1=5, nbdf2=7, mts=2)
+    s2 = _system(nb=8)
+    refs = [_eager(capi, 4, s) for s in (s1, s2)]
+    ctxs = []
+    for s in (s1, s2):
+        r, p, box, z, mass, prm, beta, dt, na, nb = s
+        c = capi.RPMDContext(prm, na, nb, "fp64")
+        c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+        ctxs.append(c)
+    batch = capi.ReplicaBatch(ctxs)
+    batch.run_steps(4)
+    for c, ref in zip(ctxs, refs):
+        assert _same(c.state_download(), ref)
+    batch.close()
+    for c in ctxs:
+        c.close()

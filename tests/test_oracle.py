@@ -442,3 +442,362 @@ def test_evolve_rig_pi_matches_reference(oracle, refpath, therm):
         assert np.array_equal(a["vir"], b["vir"])
     for k in range(nb):
         assert _bond_residuals(a["r"][k], P) <= 1.0001e-6
+
+
+# ---- pins added for sample_rpmd / pshift, setup_ewald, contraction, the free ring and evolve_pi ----
+def test_sample_rpmd_and_pshift_match_reference(oracle, refpath):
+    """sample_rpmd (src/module_thermostat.f90:199-226, which ends in pshift :229-264) against the
+    reference's Fortran on the same ran2 stream: bitwise; pshift alone bitwise on arbitrary
+    momenta; and the property pshift exists for: zero total momentum."""
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    beta = 1.0 / (cf.KBOLTZ * 298.0)
+    for irun in (17, 4242):
+        s = oracle.rng(irun)
+        refpath.seed(irun)
+        a = oracle.sample_rpmd(nb, na, mass, beta, s)
+        b = refpath.sample_rpmd(nb, na, mass, beta)
+        assert np.array_equal(a, b)
+        assert oracle.gaussians(s, 1)[0] == refpath.gaussian()   # both streams consumed 3 na nb draws
+        assert np.abs(a.sum(axis=(0, 1))).max() < 1e-10 * np.abs(a).sum()
+    q = np.random.default_rng(9).normal(size=(nb, na, 3)) * 40.0 + 3.0
+    import ctypes as C
+    qa = np.ascontiguousarray(q).copy()
+    oracle.lib.orc_pshift(qa.ctypes.data_as(C.POINTER(C.c_double)), mass.ctypes.data_as(C.POINTER(C.c_double)),
+                          C.c_int(na), C.c_int(nb))
+    assert np.array_equal(qa, refpath.pshift(q, mass))
+    assert np.abs(qa.sum(axis=(0, 1))).max() < 1e-10 * np.abs(qa).sum()
+
+
+@pytest.mark.parametrize("case", [
+    ("cubic ecut>0 (config 2)", (12, 12, 12), (1, 1, 1), 1e-6),
+    ("cubic ecut>0 (config 1)", (6, 6, 6), (1, 1, 1), 1e-6),
+    ("cubic ecut=0 (gentest rule)", (5, 5, 5), (1, 1, 1), 0.0),
+    ("non-cubic 1:1:1.2 (config 4)", (16, 16, 16), (1, 1, 1.2), 1e-6),
+    ("non-cubic 1:1:1.3", (4, 4, 4), (1, 1, 1.3), 0.0),
+    ("cubic box with iamcub off", (6, 6, 6), (1, 1, 1), 1e-6),
+])
+def test_setup_ewald_matches_reference(oracle, refpath, case):
+    """setup_ewald (src/setup_options.f90:104-189) of the oracle, of the C-ABI and of the
+    reference's own Fortran on the same inputs: the same values bit for bit (iamcub included:
+    the reference turns it off for a non-cubic box and ignores ecut there)."""
+    from rpmd_mainz_amd import capi
+    name, ncell, aspect, ecut = case
+    r, box, z, mass, prm = cf.water_system("qspcfw", ncell, 1, aspect=aspect, ecut=ecut)
+    na = r.shape[1]
+    want_cub = 0 if "off" in name else 1
+    P = RpmdParams.from_dict(dict(prm, iamcub=want_cub))
+    ref = refpath.setup_ewald(na, box, P)
+    P2 = RpmdParams.from_dict(dict(prm, iamcub=want_cub))
+    oracle.setup_ewald(na, box, P2)
+    abi = capi.setup_ewald(na, box, dict(prm, iamcub=want_cub))
+    for k in ("wrcut", "walpha", "rkmax", "kmax", "iamcub"):
+        assert getattr(P2, k) == ref[k], (name, k)
+        assert abi[k] == ref[k], (name, k)
+    if "config 4" in name:
+        assert ref["kmax"] == 15 and ref["iamcub"] == 0
+    if "config 2" in name:
+        assert ref["kmax"] == 15 and ref["iamcub"] == 1
+
+
+def test_box_size_matches_reference(oracle, refpath):
+    """setup_box_size (src/setup_options.f90:191-270), CUB lattice: the oracle, the configuration
+    helper and the reference's Fortran give the same box and molecule count."""
+    for ncell, rho in (((12, 12, 12), 0.997), ((6, 6, 6), 0.997), ((4, 4, 8), 1.03), ((5, 5, 5), 0.95)):
+        nm, b_ref = refpath.setup_box_size("CUB", rho, ncell, 32831.2525)
+        assert nm == int(np.prod(ncell))
+        b_orc = oracle.setup_box_size(nm, rho, 32831.2525, ncell)
+        b_cfg = cf.box_size(nm, rho, 32831.2525, ncell)
+        assert np.array_equal(b_orc, b_ref), ncell
+        assert np.abs(b_cfg - b_ref).max() <= 4e-16 * b_ref.max(), ncell
+
+
+def _rp_contract(oracle, P, rnm, box, z, nbdf, iopt):
+    """the oracle's rp_contract_nm: normal-mode r in, (v, normal-mode dvdr, vir) out"""
+    import ctypes as C
+    rnm = np.ascontiguousarray(rnm, dtype=np.float64)
+    nb, na = rnm.shape[:2]
+    d = np.zeros_like(rnm)
+    vir = np.zeros(9)
+    v = C.c_double()
+    dp = C.POINTER(C.c_double)
+    err = oracle.lib.orc_rp_contract_nm(C.byref(P), rnm.ctypes.data_as(dp), C.byref(v), d.ctypes.data_as(dp),
+                                        C.c_int(nb), C.c_int(na), np.ascontiguousarray(box, dtype=float).ctypes.data_as(dp),
+                                        np.ascontiguousarray(z, dtype=float).ctypes.data_as(dp), vir.ctypes.data_as(dp),
+                                        C.c_int(nbdf), C.c_int(iopt))
+    assert err == 0
+    return v.value, d, vir.reshape(3, 3).T.copy()
+
+
+def _to_nm(oracle, x):
+    nb = x.shape[0]
+    return oracle.realft(x.reshape(nb, -1), +1).reshape(x.shape)
+
+
+def _to_beads(oracle, x):
+    nb = x.shape[0]
+    return oracle.realft(x.reshape(nb, -1), -1).reshape(x.shape)
+
+
+@pytest.mark.parametrize("iopt", [2, 3])
+def test_rp_contract_identities(oracle, iopt):
+    """rp_contract_nm (src/evolve_pi.f90:324-431) where the mathematics fixes the answer:
+    nbdf = nb keeps every mode, so the contracted forces are the plain bead forces; nbdf = 1 keeps
+    the centroid, so every bead receives the gradient evaluated at the centroid and v is the
+    energy of the centroid configuration."""
+    r, box, z, mass, P = _sys(0)
+    na = r.shape[1]
+    for nb in (5, 7):
+        x = cf.thermal_beads(r[0], nb, 0.08, 5)
+        v_u, d_u, w_u = oracle.forces(P, x, box, z, iopt)
+        v_c, dn, w_c = _rp_contract(oracle, P, _to_nm(oracle, x), box, z, nb, iopt)
+        d_c = _to_beads(oracle, dn)
+        assert abs(v_c - v_u) <= 1e-12 * abs(v_u), nb
+        assert np.abs(d_c - d_u).max() <= 1e-12 * np.abs(d_u).max(), nb
+        assert np.abs(w_c - w_u).max() <= 1e-12 * np.abs(w_u).max(), nb
+    for nb in (4, 8):
+        x = cf.thermal_beads(r[0], nb, 0.08, 6)
+        cen = x.mean(axis=0)[None]
+        v0, g0, w0 = oracle.forces(P, cen, box, z, iopt)
+        v_c, dn, w_c = _rp_contract(oracle, P, _to_nm(oracle, x), box, z, 1, iopt)
+        d_c = _to_beads(oracle, dn)
+        assert abs(v_c - v0) <= 1e-12 * abs(v0)
+        for k in range(nb):
+            assert np.abs(d_c[k] - g0[0]).max() <= 1e-12 * np.abs(g0).max(), (nb, k)
+        assert np.abs(w_c - w0).max() <= 1e-12 * np.abs(w0).max()
+
+
+@pytest.mark.parametrize("nb,nbdf", [(8, 3), (8, 5), (8, 7), (6, 3), (9, 5)])
+def test_rp_contract_matches_truncated_fourier_projection(oracle, nb, nbdf):
+    """rp_contract_nm at nbdf > 1 against a brute-force projection in extended precision: the
+    contracted beads are the path's Fourier series truncated to |k| <= (nbdf-1)/2 and sampled at
+    nbdf equally spaced points, r_c(m) = sum_j T_mj x_j with
+    T_mj = (1/nb) [1 + 2 sum_k cos(2 pi k (m/nbdf - j/nb))], and the bead gradients are those of
+    the contracted potential (nb/nbdf) sum_m V(r_c(m)): dV/dx_j = (nb/nbdf) sum_m T_mj g_m.
+    A wrong mode (or mirror) index, scale or sign changes both."""
+    r, box, z, mass, P = _sys(0)
+    x = cf.thermal_beads(r[0], nb, 0.1, 7 + nb)
+    h = (nbdf - 1) // 2
+    ld = np.longdouble
+    m_ = np.arange(nbdf, dtype=ld)[:, None]
+    j_ = np.arange(nb, dtype=ld)[None, :]
+    pi = ld(np.pi)
+    T = np.ones((nbdf, nb), dtype=ld)
+    for k in range(1, h + 1):
+        T += 2 * np.cos(2 * pi * k * (m_ / nbdf - j_ / nb))
+    T /= nb
+    rc = np.einsum("mj,jac->mac", T, x.astype(ld)).astype(np.float64)
+    for iopt in (2, 3):
+        v_ref, g, w_ref = oracle.forces(P, rc, box, z, iopt)
+        d_ref = (ld(nb) / nbdf) * np.einsum("mj,mac->jac", T, g.astype(ld))
+        v_c, dn, w_c = _rp_contract(oracle, P, _to_nm(oracle, x), box, z, nbdf, iopt)
+        d_c = _to_beads(oracle, dn)
+        assert abs(v_c - v_ref) <= 1e-12 * abs(v_ref), iopt
+        assert np.abs(d_c - d_ref.astype(np.float64)).max() <= 1e-11 * np.abs(d_c).max(), iopt
+        assert np.abs(w_c - w_ref).max() <= 1e-11 * np.abs(w_ref).max(), iopt
+
+
+def test_free_ring_is_the_exact_monodromy(oracle, refpath):
+    """The free ring-polymer update (src/evolve_pi.f90:109-125) with ring_rpmd's monodromy (pinned
+    bitwise against the reference): the oracle equals the same expression evaluated with
+    correctly rounded IEEE products and sums (numpy), bit for bit; and physics the update must
+    have: each non-centroid mode is a harmonic oscillator of frequency w_k = (2 nb/beta)
+    sin(k pi/nb), so p^2/(2m) + m w_k^2 r^2/2 is conserved per mode and atom, and the centroid
+    moves freely (r' = r + p dt/m)."""
+    nb, na = 8, 6
+    mass = np.tile([29156.9471, 1837.1527, 1837.1527], na // 3)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=(nb, na, 3)) * 30.0
+    r = rng.normal(size=(nb, na, 3)) * 0.3
+    mon = oracle.ring_rpmd(mass[:3], nb, dt, beta)
+    assert np.array_equal(mon, refpath.ring_rpmd(mass[:3], nb, dt, beta))
+    p1, r1 = oracle.free_ring(p, r, mon)
+    ic = np.arange(na) % 3
+    m = mon[:, :, ic]                                  # (nb, 4, na)
+    pe = p * m[:, 0, :, None] + r * m[:, 1, :, None]
+    re = p * m[:, 2, :, None] + r * m[:, 3, :, None]
+    assert np.array_equal(p1, pe) and np.array_equal(r1, re)
+    wk = (2.0 * nb / beta) * np.sin(np.arange(nb) * np.pi / nb)
+    wk = np.minimum(wk, wk[::-1][np.r_[nb - 1, :nb - 1]])  # mode k and its mirror nb - k share w_k
+    ms = mass[None, :, None]
+    e0 = p ** 2 / (2 * ms) + 0.5 * ms * (wk[:, None, None] * r) ** 2
+    e1 = p1 ** 2 / (2 * ms) + 0.5 * ms * (wk[:, None, None] * r1) ** 2
+    assert np.abs(e1[1:] - e0[1:]).max() <= 1e-12 * np.abs(e0[1:]).max()
+    assert np.allclose(r1[0], r[0] + p[0] * dt / ms[0], rtol=1e-14, atol=0)
+    assert np.array_equal(p1[0], p[0])
+
+
+def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta):
+    """evolve_pi (src/evolve_pi.f90:1-322, type 'RPMD', nbaro = 0) composed from the reference's
+    own compiled routines -- ring_rpmd, therm_pro, therm_pro_global, parinello_therm,
+    parinello_therm_loc, therm_gle, forces -- in the reference's order, with realft from the oracle
+    (pinned above by the closed-form DFT).  ring_contract / force_contract (:377-431) are the
+    reference's mode copies around realft.  The reference routine itself cannot be linked here:
+    it calls realft, whose FFTW library this image lacks."""
+    nb, na = r.shape[:2]
+    mts, nbdf1, nbdf2, therm = P.mts, P.nbdf1, P.nbdf2, P.therm
+    p, r, dvdr, dvdr2 = p.copy(), r.copy(), dvdr.copy(), dvdr2.copy()
+    dtsmall = dt / mts
+    halfdtsmall, halfdt = 0.5 * dtsmall, 0.5 * dt
+    mon = ref.ring_rpmd(mass[:3], nb, dtsmall, beta)
+    m = mon[:, :, np.arange(na) % 3]
+    T = cf.THERM
+    ref.set_nctot(0)  # flexible molecules: no constraints (therm_pro_global's Bussi step reads it)
+
+    def outer(p):
+        if therm == T["PRG"]:
+            return ref.parinello_therm(p, mass, P.ttau, halfdt, beta, 0)
+        if therm == T["PRL"]:
+            return ref.parinello_therm_loc(p, mass, P.ttau, dt, beta)
+        if therm == T["GLE"]:
+            return ref.therm_gle(p, mass)
+        return p
+
+    def inner(p):
+        if therm == T["PRO"]:
+            return ref.therm_pro(p, mass, beta, P.ttau, halfdtsmall)
+        if therm == T["PRM"]:
+            return ref.therm_pro_global(p, mass, beta, P.ttau, halfdtsmall)
+        return p
+
+    def contract(rnm, nbdf, iopt):
+        hh = (nbdf - 1) // 2
+        rc = np.zeros((nbdf, na, 3))
+        rc[0] = rnm[0]
+        for k in range(1, hh + 1):
+            rc[k] = rnm[k]
+            rc[nbdf - k] = rnm[nb - k]
+        rc = _to_beads(orc, rc) * math.sqrt(nbdf / nb)
+        v, dc, w = ref.forces(P, rc, box, z, iopt)
+        dc = _to_nm(orc, dc)
+        sf = math.sqrt(nb / nbdf)
+        out = np.zeros((nb, na, 3))
+        out[0] = sf * dc[0]
+        for k in range(1, hh + 1):
+            out[k] = sf * dc[k]
+            out[nb - k] = sf * dc[nbdf - k]
+        return v, out, w
+
+    p = outer(p)
+    p = p - halfdt * dvdr
+    p = p - halfdtsmall * dvdr2
+    p, r = _to_nm(orc, p), _to_nm(orc, r)
+    vir_hf = np.zeros((3, 3))
+    for im in range(1, mts + 1):
+        p = inner(p)
+        pp = p * m[:, 0, :, None] + r * m[:, 1, :, None]
+        r = p * m[:, 2, :, None] + r * m[:, 3, :, None]
+        p = pp
+        p = inner(p)
+        rst = _to_beads(orc, r)
+        vint, dvdr2, w = ref.forces(P, rst, box, z, 4)
+        vir_hf = vir_hf + w
+        if im != mts:
+            p = p - dtsmall * _to_nm(orc, dvdr2)
+    vir_hf = vir_hf / mts
+    vew = vlj = 0.0
+    vir_ew = vir_lj = np.zeros((3, 3))
+    dvdrl = np.zeros_like(r)
+    if nbdf1 > 0:
+        vew, dvdr, vir_ew = contract(r, nbdf1, 2)
+        if P.sig > 0.0:
+            ve, dvdre, vir_ewc = contract(r, nbdf1, 5)
+            vew, vir_ew, dvdr = vew - ve, vir_ew - vir_ewc, dvdr - dvdre
+    if nbdf2 > 0:
+        vlj, dvdrl, vir_lj = contract(r, nbdf2, 3)
+    p = _to_beads(orc, p)
+    r = rst
+    p = p - halfdtsmall * dvdr2
+    if nbdf1 > 0 and nbdf2 > 0:
+        dvdr = _to_beads(orc, dvdr + dvdrl)
+    elif nbdf1 > 0:
+        dvdr = _to_beads(orc, dvdr)
+    elif nbdf2 > 0:
+        dvdrl = _to_beads(orc, dvdrl)
+    if nbdf1 <= 0:
+        vew, dvdr, vir_ew = ref.forces(P, r, box, z, 2)
+    elif P.sig > 0.0:
+        ve, dvdre, vir_ewc = ref.forces(P, r, box, z, 5)
+        vew, vir_ew, dvdr = vew + ve, vir_ew + vir_ewc, dvdr + dvdre
+    if nbdf2 <= 0:
+        vlj, dvdrl, vir_lj = ref.forces(P, r, box, z, 3)
+    if nbdf1 <= 0 or nbdf2 <= 0:
+        dvdr = dvdr + dvdrl
+    p = p - halfdt * dvdr
+    p = outer(p)
+    vir_lf = vir_lj + vir_ew
+    return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=vew + vlj + vint, vew=vew, vlj=vlj, vint=vint,
+                vir=vir_lf + vir_hf, vir_lf=vir_lf)
+
+
+def _compare_steps(a, b, tag):
+    for k in ("r", "p", "dvdr", "dvdr2"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * np.abs(b[k]).max(), (tag, k)
+    for k in ("v", "vew", "vlj", "vint"):
+        assert abs(a[k] - b[k]) <= 1e-12 * max(abs(b["v"]), abs(b[k])), (tag, k)
+    assert np.abs(a["vir"] - b["vir"]).max() <= 1e-11 * np.abs(b["vir"]).max(), tag
+
+
+@pytest.mark.parametrize("therm,extra", [
+    ("PRO", {}), ("PRM", {}), ("PRG", {}), ("PRL", {}), ("NONE", {}),
+    ("PRO", dict(nbdf1=3, nbdf2=5)), ("PRO", dict(nbdf1=5, nbdf2=0)), ("NONE", dict(nbdf1=0, nbdf2=3)),
+])
+def test_evolve_pi_composed_from_reference(oracle, refpath, therm, extra):
+    """Three evolve_pi steps at nb = 8 and mts = 2 -- the thermostat draw order and kick
+    placement, the MTS loop and the contraction -- of the oracle against the same steps composed
+    from the reference's own routines on one ran2 stream; afterwards both streams are at the same
+    draw."""
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, therm=therm, ttaufs=50.0, mts=2,
+                                           **extra)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(5))
+    ff = oracle.full_forces(P, r, box, z)
+    s = oracle.rng(61)
+    refpath.seed(61)
+    a = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"])
+    b = dict(a)
+    for step in range(3):
+        a = oracle.evolve_pi(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, box, z, beta, s)
+        b = _ref_evolve_pi(refpath, oracle, P, b["p"], b["r"], b["dvdr"], b["dvdr2"], dt, mass, box, z, beta)
+        _compare_steps(a, b, (therm, extra, step))
+    assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
+
+
+def _gle_composed_case(workdir):
+    """GLE steps (the reference's therm_gle_init allocates module state once per process, so this
+    runs in a child process of its own)"""
+    from oracle.oracle import Oracle, RefPath
+    orc, ref = Oracle(), RefPath()
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, therm="GLE", ttaufs=50.0, mts=2)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    A = np.loadtxt(os.path.join(GOLD, "gle_A_ns4.txt"))
+    p = orc.sample_rpmd(nb, na, mass, beta, orc.rng(5))
+    ff = orc.full_forces(P, r, box, z)
+    s = orc.rng(62)
+    ref.seed(62)
+    g = orc.gle_init(A, P.ttau, na, nb, 0.5 * dt, s, beta)
+    ref.gle_init(A, P.ttau, na, nb, 0.5 * dt, beta, workdir)
+    a = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"])
+    b = dict(a)
+    for step in range(3):
+        a = orc.evolve_pi(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, box, z, beta, s, g)
+        b = _ref_evolve_pi(ref, orc, P, b["p"], b["r"], b["dvdr"], b["dvdr2"], dt, mass, box, z, beta)
+        _compare_steps(a, b, ("GLE", step))
+    assert orc.gaussians(s, 1)[0] == ref.gaussian()
+    orc.gle_free(g)
+
+
+def test_evolve_pi_gle_composed_from_reference(refpath, tmp_path):
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r); sys.path.insert(0, %r); import test_oracle as t; "
+            "t._gle_composed_case(%r); print('gle ok')" % (root, os.path.dirname(os.path.abspath(__file__)),
+                                                          str(tmp_path)))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and "gle ok" in out.stdout, out.stdout[-3000:] + out.stderr[-3000:]
