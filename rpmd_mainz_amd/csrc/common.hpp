@@ -40,6 +40,18 @@ __device__ __forceinline__ void ctrl_max(T* p, T v) {
     __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One FP64 product or sum, rounded once: no fusion into an FMA with a neighbouring operation (HIP's
+// __dmul_rn / __dadd_rn are plain * and +, which the compiler contracts).  The propagation kernels
+// use these to reproduce the reference's loops compiled without contraction bit for bit.
+__device__ __forceinline__ double rn_mul(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double rn_add(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 struct Box {
     double L[3], inv[3];
 };

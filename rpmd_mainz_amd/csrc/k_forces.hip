@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256) void k_chain(double* __restrict__ grad, const 
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 double v = kp[o + k];
-                v = __dadd_rn(v, -__dmul_rn(ka1, kg1[o + k]));
-                v = __dadd_rn(v, -__dmul_rn(ka2, rr[k]));
+                v = rn_add(v, -rn_mul(ka1, kg1[o + k]));
+                v = rn_add(v, -rn_mul(ka2, rr[k]));
                 kp[o + k] = v;
             }
         }

@@ -9,6 +9,11 @@
 // 257-259); that molecule's thread raises err[0] here, and the host turns it into RPMD_ERR_RATTLE.
 #include "common.hpp"
 
+// The RATTLE and rigid-step kernels round every product and sum once, as the reference's Fortran
+// loops do without contraction (and as the oracle, built with -ffp-contract=off): HIP's __dmul_rn and
+// __dadd_rn are plain * and + here, which the compiler would otherwise fuse into FMAs.
+#pragma clang fp contract(off)
+
 namespace rpmd {
 
 namespace {

@@ -8,6 +8,11 @@
 
 #include "common.hpp"
 
+// The propagation and thermostat kernels round every product and sum once, as the reference's Fortran
+// loops do without contraction (and as the oracle, built with -ffp-contract=off): HIP's __dmul_rn and
+// __dadd_rn are plain * and + here, which the compiler would otherwise fuse into FMAs.
+#pragma clang fp contract(off)
+
 namespace rpmd {
 
 // ---- NM transform (src/fourier.f90:16-63 as a matrix) ------------------------------------------
@@ -30,8 +35,8 @@ struct NMSrc {
 __device__ __forceinline__ double nm_load(const NMSrc& s, int j, size_t e, size_t ne) {
     const size_t o = (size_t)j * ne + e;
     double v = s.in[o];
-    if (s.g1) v = __dadd_rn(v, -__dmul_rn(s.a1, s.g1[o]));
-    if (s.g2) v = __dadd_rn(v, -__dmul_rn(s.a2, s.g2[o]));
+    if (s.g1) v = rn_add(v, -rn_mul(s.a1, s.g1[o]));
+    if (s.g2) v = rn_add(v, -rn_mul(s.a2, s.g2[o]));
     if (s.com && j == 0) {
         const int d = (int)(e / s.na), a = (int)(e % s.na);
         double pc = 0.0;
@@ -186,8 +191,8 @@ __global__ void k_kick(double* __restrict__ p, const double* __restrict__ g1, do
     if (ctr && i == 0) ctrl_add(ctr, (uint64_t)RPMD_STREAMS_PER_STEP);
     if (i >= n) return;
     double v = p[i];
-    if (g1) v = __dadd_rn(v, -__dmul_rn(a1, g1[i]));
-    if (g2) v = __dadd_rn(v, -__dmul_rn(a2, g2[i]));
+    if (g1) v = rn_add(v, -rn_mul(a1, g1[i]));
+    if (g2) v = rn_add(v, -rn_mul(a2, g2[i]));
     p[i] = v;
 }
 void launch_kick(double* p, const double* g1, double a1, const double* g2, double a2, size_t n, hipStream_t s,
@@ -217,8 +222,8 @@ __global__ void k_free_ring(double* __restrict__ p, double* __restrict__ r, cons
         const double pcd = pc * inv_mtot;
         pv -= pcd * mass[a];
     }
-    const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
-    const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
+    const double pp = rn_add(rn_mul(pv, mo[0 * 3 + ic]), rn_mul(rv, mo[1 * 3 + ic]));
+    const double rr = rn_add(rn_mul(pv, mo[2 * 3 + ic]), rn_mul(rv, mo[3 * 3 + ic]));
     p[i] = pp;
     r[i] = rr;
 }
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void k_pile(double* __restrict__ p, const doub
             const size_t o = ((size_t)k * 3 + d) * na + a;
             const size_t ni = ((size_t)(k - k0) * na + a) * 3 + d;
             const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
-            const double v = __dadd_rn(__dmul_rn(cc1, p[o]), __dmul_rn(cc3, xi));
+            const double v = rn_add(rn_mul(cc1, p[o]), rn_mul(cc3, xi));
             p[o] = v;
             s[d] = v;
         }
@@ -332,12 +337,12 @@ __global__ __launch_bounds__(256) void k_free_ring_pile(double* __restrict__ p, 
                 const double pcd = pc * inv_mtot;
                 pv -= pcd * mass[a];
             }
-            const double pp = __dadd_rn(__dmul_rn(pv, mo[0 * 3 + ic]), __dmul_rn(rv, mo[1 * 3 + ic]));
-            const double rr = __dadd_rn(__dmul_rn(pv, mo[2 * 3 + ic]), __dmul_rn(rv, mo[3 * 3 + ic]));
+            const double pp = rn_add(rn_mul(pv, mo[0 * 3 + ic]), rn_mul(rv, mo[1 * 3 + ic]));
+            const double rr = rn_add(rn_mul(pv, mo[2 * 3 + ic]), rn_mul(rv, mo[3 * 3 + ic]));
             r[i] = rr;
             const size_t ni = ((size_t)k * na + a) * 3 + d;
             const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
-            const double v = __dadd_rn(__dmul_rn(cc1, pp), __dmul_rn(cc3, xi));
+            const double v = rn_add(rn_mul(cc1, pp), rn_mul(cc3, xi));
             p[i] = v;
             s[d] = v;
         }
@@ -446,8 +451,8 @@ __global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict
         for (int st = 0; st < NT * 4; st++) {
             double v = xin[st];
             if (src == 0) {
-                v = __dadd_rn(v, -__dmul_rn(a1, xg1[st]));
-                if (g2) v = __dadd_rn(v, -__dmul_rn(a2, xg2[st]));
+                v = rn_add(v, -rn_mul(a1, xg1[st]));
+                if (g2) v = rn_add(v, -rn_mul(a2, xg2[st]));
             }
             bv[st] = ein && st * 4 + lk < n ? v : 0.0;
         }
@@ -488,11 +493,11 @@ __global__ __launch_bounds__(128 * NT) void k_pile_step(const double* __restrict
     for (int h = 0; h < 2; h++) {
         const int idx = threadIdx.x + h * 128 * NT, k = idx >> 4, col = idx & 15;
         if (eok[h]) {
-            const double pv = __dadd_rn(__dmul_rn(ec1[h], T[0][k][col]), __dmul_rn(ec3[h], ex1[h]));
+            const double pv = rn_add(rn_mul(ec1[h], T[0][k][col]), rn_mul(ec3[h], ex1[h]));
             const double rv = T[1][k][col];
-            const double pp = __dadd_rn(__dmul_rn(pv, emo[h][0]), __dmul_rn(rv, emo[h][1]));
-            const double rr = __dadd_rn(__dmul_rn(pv, emo[h][2]), __dmul_rn(rv, emo[h][3]));
-            const double pk = __dadd_rn(__dmul_rn(ec1[h], pp), __dmul_rn(ec3[h], ex2[h]));
+            const double pp = rn_add(rn_mul(pv, emo[h][0]), rn_mul(rv, emo[h][1]));
+            const double rr = rn_add(rn_mul(pv, emo[h][2]), rn_mul(rv, emo[h][3]));
+            const double pk = rn_add(rn_mul(ec1[h], pp), rn_mul(ec3[h], ex2[h]));
             T[0][k][col] = pk;
             T[1][k][col] = rr;
             if (k == 0) {
@@ -670,7 +675,7 @@ __global__ void k_langevin(double* __restrict__ p, const double* __restrict__ c3
     for (int d = 0; d < 3; d++) {
         const size_t o = (k * 3 + d) * na + a, ni = (k * na + a) * 3 + d;
         const double xi = noise ? noise[ni] : normal_at(seed, stream, ni);
-        p[o] = __dadd_rn(__dmul_rn(c1, p[o]), __dmul_rn(c3m[a], xi));
+        p[o] = rn_add(rn_mul(c1, p[o]), rn_mul(c3m[a], xi));
     }
 }
 void launch_langevin(double* p, const double* c3m, double c1, int na, int nb, const double* noise, uint64_t seed,
