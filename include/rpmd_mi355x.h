@@ -125,6 +125,15 @@ int rpmd_full_forces(rpmd_ctx* ctx, const double* r, int na, int nb, double* v, 
  * transformed and scaled by 1/sqrt(n) (FFTW half-complex order). */
 int rpmd_realft(rpmd_ctx* ctx, double* data, int m, int n, int mode);
 
+/* Allocation-free variant of rpmd_realft.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, transform and errors as rpmd_realft, but no context: the transform matrix and the
+ * staged rows live in a fixed device array of 8.9 MB that the library holds on each GPU it runs on
+ * (rows are processed in chunks of 524288 / n, so m is unbounded; n > 256 returns
+ * RPMD_ERR_REALFT_N).  It calls no hipMalloc or hipFree.  Every call on one GPU uses that GPU's
+ * array: calls that overlap on one GPU (from several host threads) race. */
+int rpmd_realft_noalloc(double* data, int m, int n, int mode);
+
 /* The free ring-polymer update inside evolve_pi (src/evolve_pi.f90:109-125) with the monodromy of
  * ring_rpmd (src/md_pathi.f90:190-231) for time step dt: p and r(3,na,nb) are normal-mode
  * coordinates (realft mode +1 of the bead arrays) and are updated in place,

@@ -69,7 +69,7 @@ def lib() -> C.CDLL:
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
-                     "rpmd_ctx_set_streams", "rpmd_free_ring"):
+                     "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -100,6 +100,15 @@ def setup_ewald(na: int, boxlxyz, prm: dict) -> dict:
     for k in ("wrcut", "walpha", "rkmax", "kmax", "iamcub"):
         out[k] = getattr(P, k)
     return out
+
+
+def realft_noalloc(data: np.ndarray, m: int, n: int, mode: int) -> np.ndarray:
+    """rpmd_realft_noalloc: realft (src/fourier.f90:16) without a context and without any device
+    allocation, in place on a C-contiguous (n, m) array (= Fortran data(m,n)); also returned."""
+    if not (isinstance(data, np.ndarray) and data.dtype == np.float64 and data.flags.c_contiguous):
+        raise TypeError("realft_noalloc needs a C-contiguous float64 array")
+    _check(lib().rpmd_realft_noalloc(_p(data), C.c_int(m), C.c_int(n), C.c_int(mode)))
+    return data
 
 
 class RPMDContext:

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -26,13 +27,16 @@ using namespace rpmd;
 namespace {
 
 // bead transform matrices for one n: F = R2HC/sqrt(n), B = HC2R/sqrt(n) (src/fourier.f90:42-59,
-// FFTW's documented half-complex definitions), row-major [k][j]; built in long double
-void realft_matrices(int n, std::vector<double>& F, std::vector<double>& B) {
-    F.assign((size_t)n * n, 0.0);
-    B.assign((size_t)n * n, 0.0);
+// FFTW's documented half-complex definitions), row-major [k][j]; built in long double.  Either
+// pointer may be null; the outputs hold n*n doubles.
+void realft_matrices_into(int n, double* F, double* B) {
+    if (F)
+        for (size_t i = 0; i < (size_t)n * n; i++) F[i] = 0.0;
+    if (B)
+        for (size_t i = 0; i < (size_t)n * n; i++) B[i] = 0.0;
     const long double pi = acosl(-1.0L);
     const long double sc = sqrtl(1.0L / n);
-    for (int j = 0; j < n; j++) {
+    for (int j = 0; j < n && F; j++) {
         for (int k = 0; k <= n / 2; k++) {
             long double a = 2.0L * pi * (long double)(((long long)j * k) % n) / n;
             F[(size_t)k * n + j] = (double)(sc * cosl(a));
@@ -42,7 +46,7 @@ void realft_matrices(int n, std::vector<double>& F, std::vector<double>& B) {
             F[(size_t)(n - k) * n + j] = (double)(-sc * sinl(a));
         }
     }
-    for (int j = 0; j < n; j++) {
+    for (int j = 0; j < n && B; j++) {
         B[(size_t)j * n + 0] = (double)sc;
         for (int k = 1; k <= (n - 1) / 2; k++) {
             long double a = 2.0L * pi * (long double)(((long long)j * k) % n) / n;
@@ -51,6 +55,11 @@ void realft_matrices(int n, std::vector<double>& F, std::vector<double>& B) {
         }
         if (n % 2 == 0) B[(size_t)j * n + n / 2] = (double)((j % 2 == 0 ? 1.0L : -1.0L) * sc);
     }
+}
+void realft_matrices(int n, std::vector<double>& F, std::vector<double>& B) {
+    F.assign((size_t)n * n, 0.0);
+    B.assign((size_t)n * n, 0.0);
+    realft_matrices_into(n, F.data(), B.data());
 }
 
 struct NMMat {
@@ -1258,6 +1267,34 @@ size_t noise_per_step(const rpmd_params& P, int na, int nb, int ngle) {
 
 }  // namespace
 
+// ---- allocation-free realft ---------------------------------------------------------------------
+// For callers that cannot allocate device memory: the transform matrix and the staged rows live in
+// this fixed device array (each GPU holds its own copy from module load to process end).  Host code
+// never uses the array's name as an address: the device address comes from hipGetSymbolAddress, once
+// per device, and is kept in noalloc_ptr[].
+#define NOALLOC_MAT (256 * 256)     // the n x n matrix, n <= 256
+#define NOALLOC_STAGE (1 << 19)     // doubles per staging half (rows x n of input, of output)
+static __device__ __align__(256) double scratch[NOALLOC_MAT + 2 * NOALLOC_STAGE];
+
+namespace {
+#define NOALLOC_MAX_DEV 64
+std::mutex noalloc_mu;
+double* noalloc_ptr[NOALLOC_MAX_DEV] = {};
+// the device address of scratch on the current device (looked up on its first call there)
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) return RPMD_ERR_HIP;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RPMD_ERR_HIP;
+        noalloc_ptr[dev] = (double*)p;
+    }
+    *out = noalloc_ptr[dev];
+    return 0;
+}
+}  // namespace
+
 // =================================================================================================
 // C-ABI
 // =================================================================================================
@@ -1579,6 +1616,33 @@ int rpmd_realft(rpmd_ctx* c, double* data, int m, int n, int mode) {
     HIPCHK(hipMemcpyAsync(c->h_stage, c->tmp2, N * 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     memcpy(data, c->h_stage, N * 8);
+    return 0;
+}
+
+int rpmd_realft_noalloc(double* data, int m, int n, int mode) {
+    if (!data || m <= 0 || n <= 0) return RPMD_ERR_ARG;
+    if (n > 256) return RPMD_ERR_REALFT_N;
+    if (mode != 1 && mode != -1) return RPMD_ERR_REALFT_MODE;
+    double* sp = nullptr;
+    RETCHK(noalloc_scratch(&sp));
+    double* M = sp;
+    double* in = sp + NOALLOC_MAT;
+    double* out = in + NOALLOC_STAGE;
+    // the matrix, formed in static host storage (no heap), then the rows in chunks that fit the
+    // staging halves: every copy and the kernel on the null stream, in order; the last copy waits
+    static thread_local double hm[NOALLOC_MAT];
+    realft_matrices_into(n, mode == 1 ? hm : nullptr, mode == 1 ? nullptr : hm);
+    HIPCHK(hipMemcpy(M, hm, sizeof(double) * n * n, hipMemcpyHostToDevice));
+    const int rows = NOALLOC_STAGE / n;
+    for (int k0 = 0; k0 < m; k0 += rows) {
+        const int cr = std::min(rows, m - k0);
+        HIPCHK(hipMemcpy2D(in, sizeof(double) * cr, data + k0, sizeof(double) * m, sizeof(double) * cr, n,
+                           hipMemcpyHostToDevice));
+        launch_nm(in, out, M, n, (size_t)cr, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2D(data + k0, sizeof(double) * m, out, sizeof(double) * cr, sizeof(double) * cr, n,
+                           hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 
