@@ -9,8 +9,8 @@
 #include "common.hpp"
 
 // The propagation and thermostat kernels round every product and sum once, as the reference's Fortran
-// loops do without contraction (and as the oracle, built with -ffp-contract=off): HIP's __dmul_rn and
-// __dadd_rn are plain * and + here, which the compiler would otherwise fuse into FMAs.
+// loops do when compiled without contraction: HIP's __dmul_rn and __dadd_rn are plain * and + here,
+// which the compiler would otherwise fuse into FMAs.
 #pragma clang fp contract(off)
 
 namespace rpmd {
