@@ -40,6 +40,7 @@ extern "C" {
 #define RPMD_ERR_THERM            15  /* thermostat not supported by this entry point          */
 #define RPMD_ERR_NOISE            16  /* host noise buffer too short for the draws needed      */
 #define RPMD_ERR_GLE              17  /* therm GLE without rpmd_gle_init for this (na, nb)     */
+#define RPMD_ERR_TTAU             18  /* a thermostat is selected but ttau <= 0               */
 
 /* ---- precision modes ------------------------------------------------------------------ */
 #define RPMD_PREC_FP64   0  /* everything FP64 (the "plain FP64 GPU form")                   */
@@ -89,6 +90,9 @@ typedef struct rpmd_params {
     double ttau;
     /* /structure/ iamrigid: rigid molecules held by RATTLE (rpmd_evolve_rig_pi) */
     int rigid;
+    /* RATTLE tolerance: relative on the squared bond lengths (stage 1), absolute on the bond
+     * velocity multiplier (stage 2); <= 0 selects the reference's 1e-6 (src/md_rattle.f90:60,182) */
+    double rat_tol;
 } rpmd_params;
 
 typedef struct rpmd_ctx rpmd_ctx;

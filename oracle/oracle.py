@@ -37,7 +37,7 @@ class RpmdParams(C.Structure):
         ("sig", C.c_double),
         ("nbdf1", C.c_int), ("nbdf2", C.c_int), ("mts", C.c_int),
         ("intra_model", C.c_int), ("therm", C.c_int), ("ttau", C.c_double),
-        ("rigid", C.c_int),
+        ("rigid", C.c_int), ("rat_tol", C.c_double),
     ]
 
     @classmethod

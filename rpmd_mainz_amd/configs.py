@@ -99,7 +99,7 @@ def masses_charges(model: dict, na: int):
 def make_params(model: dict, box, rcut_A: float = 9.0, ecut: float = 0.0, iamcub: bool = True,
                 intra_model: int = INTRA_AUTO, sig_A: float = 0.0, nbdf1: int = 0, nbdf2: int = 0,
                 mts: int = 1, therm: str = "NONE", ttaufs: float = 0.0, rigid: bool = False,
-                oo_gam: float | None = None) -> dict:
+                oo_gam: float | None = None, rat_tol: float = 0.0) -> dict:
     """The COMMON-block values of src/main.f90 for a model, before setup_ewald."""
     theta = model["thetad"] * (math.pi / 180.0)
     rcut = rcut_A / TOA
@@ -112,7 +112,7 @@ def make_params(model: dict, box, rcut_A: float = 9.0, ecut: float = 0.0, iamcub
                 apot=model["apot"], bpot=model["bpot"], alp=model["alp"], alpb=model["alpb"],
                 theta=theta, reoh=model["reoh"], sig=sig_A / TOA if sig_A > 0 else 0.0,
                 nbdf1=nbdf1, nbdf2=nbdf2, mts=mts, intra_model=intra_model,
-                therm=THERM[therm], ttau=TOFS * ttaufs, rigid=1 if rigid else 0)
+                therm=THERM[therm], ttau=TOFS * ttaufs, rigid=1 if rigid else 0, rat_tol=rat_tol)
 
 
 def setup_ewald(na: int, box, prm: dict) -> dict:

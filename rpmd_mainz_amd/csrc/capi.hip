@@ -1102,9 +1102,17 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     return 0;
 }
 
+// the RATTLE tolerance of the parameters (<= 0: the reference's 1e-6, src/md_rattle.f90:60,182)
+double rattle_tol(const rpmd_params& P) { return P.rat_tol > 0.0 ? P.rat_tol : 1e-6; }
+
 // one rigid RPMD step, evolve_rig_pi (src/evolve_rigid.f90:100-234, nbaro = 0), on the context's
 // device state.  The reference's dvdr_corr, v_corr and vir_corr are never set there; they are zero
 // here.  ev_step[0..49]: the forces(iopt = 1) slot; ev_step[50..58]: vir_hf (column-major).
+// Thermostats PRG and PRL as the reference's; PRO and GLE are additions of this library (the
+// reference's rigid step has no such branch, src/evolve_rigid.f90:123-127): at the same two points
+// of the step, a PILE-L kick of the normal-mode momenta (src/module_gle.f90:8-73) or a GLE kick
+// (:229-283) over dt/2, then the velocity stage of RATTLE on every bead, which puts the momenta
+// back on the constraint surface.
 int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     const rpmd_params& P = c->prm;
     const int na = c->na, nb = c->st_nb, nm = na / 3;
@@ -1117,7 +1125,12 @@ int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     omegan = omegan * omegan;
     // rattle_setup (src/md_rattle.f90:14-28)
     const double rehh = 2.0 * sin(0.5 * P.theta) * P.reoh;
-    const double rcb[3] = {P.reoh * P.reoh, P.reoh * P.reoh, rehh * rehh};
+    const double len2[3] = {P.reoh * P.reoh, P.reoh * P.reoh, rehh * rehh};
+    const double tol = rattle_tol(P);
+    if (P.therm == RPMD_THERM_GLE && (!c->gle_gp || c->gle_nb != nb)) return RPMD_ERR_GLE;
+    NMMat* nmm = P.therm == RPMD_THERM_PRO ? get_nm(c, nb, 0) : nullptr;
+    if (P.therm == RPMD_THERM_PRO && !nmm) return RPMD_ERR_NOMEM;
+    const size_t draws = (size_t)3 * na * nb;
     auto thermo = [&]() {
         if (P.therm == RPMD_THERM_PRG) {
             // parinello_therm with nctot = 3 nm constraints (src/md_rattle.f90:29)
@@ -1127,15 +1140,29 @@ int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
         } else if (P.therm == RPMD_THERM_PRL) {
             const double gamma = 1.0 / (2.0 * P.ttau), c1 = exp(-0.5 * gamma * halfdt);
             launch_langevin(c->p, c->lang_c3m, c1, na, nb, nz.take(N), c->seed, c->step_ctr, nz.stream_add++, s);
+        } else if (P.therm == RPMD_THERM_PRO) {
+            // PILE-L over dt/2 on the normal modes (coefficients of set_state_consts for dt/2), the
+            // centroid COM removal, back to beads, then onto the constraint surface
+            launch_nm(c->p, c->pnm, nmm->F, nb, (size_t)3 * na, s);
+            launch_pile(c->pnm, c->pile_c1, c->pile_c3, na, nb, 0, nz.take(draws), c->seed, c->step_ctr,
+                        nz.stream_add++, c->com_parts, s);
+            launch_com_remove(c->pnm, c->mass, c->com_parts, pile_blocks(na), 1.0 / c->mtot, na, s);
+            launch_nm(c->pnm, c->p, nmm->B, nb, (size_t)3 * na, s);
+            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, c->rat_err, s);
+        } else if (P.therm == RPMD_THERM_GLE) {
+            launch_gle(c->p, c->gle_gp, c->gle_TS, c->mass, na, nb, c->gle_n1, nz.take(draws * c->gle_n1), c->seed,
+                       c->step_ctr, nz.stream_add++, c->gle_parts, c->gle_pcom, 1.0 / sqrt(nb * c->mtot), s);
+            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, c->rat_err, s);
         }
     };
     thermo();
     launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
     launch_bead_forces(c->r, c->dvdr2, c->mass, na, nb, omegan, s);
     for (int is = 0; is < nb; is++)
-        launch_rig_substep(c->p, c->r, c->dvdr2, c->mass, na, nb, halfsmalldt, smalldt, omegan, rcb, c->rat_err, s);
+        launch_rig_substep(c->p, c->r, c->dvdr2, c->mass, na, nb, halfsmalldt, smalldt, omegan, len2, tol, c->rat_err,
+                           s);
     RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step, check));
-    launch_rig_final(c->p, c->r, c->dvdr, c->mass, na, nb, halfdt, dt, rcb, c->rat_err, c->rig_vparts,
+    launch_rig_final(c->p, c->r, c->dvdr, c->mass, na, nb, halfdt, dt, len2, tol, c->rat_err, c->rig_vparts,
                      c->ev_step + 50, s);
     thermo();
     launch_zero(c->dvdr2, N, s);
@@ -1236,7 +1263,9 @@ int set_state_consts(rpmd_ctx* c, const double* mass, double beta, double dt, in
     ring_rpmd(mass, monod.data(), nb, dt / (double)P.mts, beta);
     HIPCHK(hipMemcpyAsync(c->monod, monod.data(), monod.size() * 8, hipMemcpyHostToDevice, c->st));
     std::vector<double> c1(nb), c3((size_t)3 * nb);
-    pile_coeffs(P, mass, beta, nb, 0.5 * dt / (double)P.mts, c1.data(), c3.data());
+    // the kick's time step: dt / (2 mts) inside evolve_pi's MTS loop (src/evolve_pi.f90:103-106), dt / 2
+    // at the two thermostat points of the rigid step
+    pile_coeffs(P, mass, beta, nb, P.rigid ? 0.5 * dt : 0.5 * dt / (double)P.mts, c1.data(), c3.data());
     HIPCHK(hipMemcpyAsync(c->pile_c1, c1.data(), c1.size() * 8, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->pile_c3, c3.data(), c3.size() * 8, hipMemcpyHostToDevice, c->st));
     std::vector<double> c3m(na);
@@ -1254,7 +1283,8 @@ size_t noise_per_step(const rpmd_params& P, int na, int nb, int ngle) {
     const size_t d = (size_t)3 * na * nb;
     if (P.rigid) {
         if (P.therm == RPMD_THERM_PRG) return 2 * (d - (size_t)na - 2);  // ndof + 1 = d - 3 nm - 2
-        if (P.therm == RPMD_THERM_PRL) return 2 * d;
+        if (P.therm == RPMD_THERM_PRL || P.therm == RPMD_THERM_PRO) return 2 * d;
+        if (P.therm == RPMD_THERM_GLE) return 2 * d * (size_t)std::max(ngle, 1);
         return 0;
     }
     if (P.therm == RPMD_THERM_PRO) return 2 * (size_t)P.mts * d;
@@ -1320,6 +1350,7 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_THERM: return "thermostat not supported by this entry point";
         case RPMD_ERR_NOISE: return "noise buffer too short";
         case RPMD_ERR_GLE: return "GLE not initialised for this (na, nb): call rpmd_gle_init";
+        case RPMD_ERR_TTAU: return "thermostat time constant ttau must be > 0";
     }
     return "unknown error";
 }
@@ -1360,8 +1391,13 @@ static void destroy_streams(rpmd_ctx* c) {
         if (t) hipStreamDestroy(t);
 }
 
+// every thermostat divides by ttau (PILE 2 pi / ttau, GLE 2 pi / ttau, Bussi dt / ttau, Langevin
+// 1 / (2 ttau)): a zero or negative value would give infinite or NaN propagators
+static bool ttau_ok(const rpmd_params* prm) { return prm->therm == RPMD_THERM_NONE || prm->ttau > 0.0; }
+
 int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, rpmd_ctx** out) {
     if (!prm || !out || na <= 0 || nb_max <= 0) return RPMD_ERR_ARG;
+    if (!ttau_ok(prm)) return RPMD_ERR_TTAU;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (nb_max > 256) return RPMD_ERR_REALFT_N;
     int ndev = 0;
@@ -1465,6 +1501,7 @@ int rpmd_ctx_create(const rpmd_params* prm, int na, int nb_max, int precision, r
 
 int rpmd_ctx_set_params(rpmd_ctx* c, const rpmd_params* prm) {
     if (!c || !prm) return RPMD_ERR_ARG;
+    if (!ttau_ok(prm)) return RPMD_ERR_TTAU;
     c->prm = *prm;
     c->ks_valid = false;
     c->st_valid = false;
@@ -1713,7 +1750,7 @@ int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_l
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
     if (!c->prm.rigid) return RPMD_ERR_ARG;
-    const size_t need = noise_per_step(c->prm, na, nb, 0);
+    const size_t need = noise_per_step(c->prm, na, nb, c->gle_n1);
     if (noise && (size_t)n_noise < need) return RPMD_ERR_NOISE;
     RETCHK(upload_charges(c, z));
     // one host round trip: the uploaded arrays replace any device-resident state
@@ -1821,6 +1858,7 @@ int rpmd_gle_init(rpmd_ctx* c, int ns, const double* A, int na, int nb, double d
                   unsigned long long seed) {
     if (!c || !A || ns < 0 || ns > 7 || nb < 1 || nb > c->nb_max || beta <= 0.0) return RPMD_ERR_ARG;
     if (na != c->na) return RPMD_ERR_ARG;
+    if (!(c->prm.ttau > 0.0)) return RPMD_ERR_TTAU;  // A is scaled by 2 pi / ttau
     const int n1 = ns + 1;
     const size_t N = (size_t)3 * na * nb;
     double TS[128] = {0};

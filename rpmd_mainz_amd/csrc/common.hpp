@@ -252,9 +252,14 @@ void launch_sample_rpmd(double* p, const double* mass, int na, int nb, double be
 int rigid_blocks(int na);
 void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,
                         hipStream_t s);
+// len2: the squared target lengths of the O-H1, O-H2 and H1-H2 constraints; tol: the RATTLE tolerance
 void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
-                        double omegan2, const double rcb[3], int* err, hipStream_t s);
+                        double omegan2, const double len2[3], double tol, int* err, hipStream_t s);
 void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
-                      double dt, const double rcb[3], int* err, double* vparts, double* vir_out, hipStream_t s);
+                      double dt, const double len2[3], double tol, int* err, double* vparts, double* vir_out,
+                      hipStream_t s);
+// the velocity stage of RATTLE alone (after a thermostat kick of the rigid step)
+void launch_rig_project(double* p, const double* r, const double* mass, int na, int nb, const double len2[3],
+                        double tol, int* err, hipStream_t s);
 
 }  // namespace rpmd

@@ -2,8 +2,8 @@
 // sub-steps of evolve_rig_pi (src/evolve_rigid.f90:100-278).
 //
 // One thread per (molecule, bead).  A molecule's three atoms and three constraints live in
-// registers; the SHAKE/RATTLE iterations run in the reference's order (bonds O-H1, O-H2, H1-H2,
-// repeated until no bond needs a correction), in FP64.  Molecules are independent inside a
+// registers; the sweeps over the constraints run in the reference's order (O-H1, O-H2, H1-H2,
+// repeated until no constraint needs a correction), in FP64, with the tolerance a parameter.  Molecules are independent inside a
 // RATTLE call, so the per-molecule result is the reference's.  The reference stops when the
 // iteration count of the LAST molecule of a call reaches itmax (src/md_rattle.f90:144-146,
 // 257-259); that molecule's thread raises err[0] here, and the host turns it into RPMD_ERR_RATTLE.
@@ -18,10 +18,10 @@ namespace rpmd {
 
 namespace {
 
-constexpr double RAT_TOL = 1e-6;
 constexpr int RAT_ITMAX = 1000;
-constexpr int RAT_BI[3] = {0, 0, 1};
-constexpr int RAT_BJ[3] = {1, 2, 2};
+// the three constraints of a molecule (O, H1, H2): O-H1, O-H2, H1-H2, in the reference's order
+constexpr int CON_A[3] = {0, 0, 1};
+constexpr int CON_B[3] = {1, 2, 2};
 
 struct Mol {
     double x[3][3];  // [atom][xyz]
@@ -39,62 +39,65 @@ __device__ __forceinline__ void store_mol(double* __restrict__ v, int na, int b,
 #pragma unroll
         for (int j = 0; j < 3; j++) v[((size_t)b * 3 + d) * na + 3 * m + j] = o.x[j][d];
 }
+// (x + y) + z of the componentwise products, the reference's summation order
+__device__ __forceinline__ double dot3(const double u[3], const double w[3]) {
+    return u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+}
 
-// rattle_s2 (src/md_rattle.f90:160-271) on one molecule: corrects p (and adds the constraint
-// gradient to g); returns the iteration count.  vir (optional): the molecule's contributions to
-// vir(1,1), (1,2), (2,2), (2,3), (3,3) and, in vlast, gdx dz of its last correction, which the
-// reference's vir(1,3) = vir(1,2) - gdx dz (:232) keeps.
+// Velocity stage of RATTLE (rattle_s2, src/md_rattle.f90:160-271) on one molecule: the relative
+// velocity of every constrained pair along its bond is removed by the multiplier
+// lam = -(u . (p_b/m_b - p_a/m_a)) / ((1/m_a + 1/m_b) |u_0|^2), |u_0| the bond length, until every
+// |lam| <= tol; p is corrected and the constraint force lam u / (dt/2) added to g.  Returns the
+// sweep count.  VIR: the molecule's contributions to vir(1,1), (1,2), (2,2), (2,3), (3,3) and, in
+// vlast, the x-z term of its last correction, which the reference's vir(1,3) = vir(1,2) - ... keeps
+// (:232).
 template <bool VIR>
-__device__ __forceinline__ int rattle2_mol(const Mol& r, Mol& p, Mol& g, const double rm[3], const double rcb[3],
-                                           double halfdt, double* vir, double& vlast, bool& any) {
-    bool done = false;
-    int it = 0;
-    while (!done && it <= RAT_ITMAX) {
-        done = true;
+__device__ __forceinline__ int rattle_vel(const Mol& r, Mol& p, Mol& g, const double inv_m[3], const double len2[3],
+                                          double halfdt, double tol, double* vir, double& vlast, bool& any) {
+    bool settled = false;
+    int sweeps = 0;
+    while (!settled && sweeps <= RAT_ITMAX) {
+        settled = true;
 #pragma unroll
-        for (int b = 0; b < 3; b++) {
-            const int ii = RAT_BI[b], jj = RAT_BJ[b];
-            const double dx1 = r.x[jj][0] - r.x[ii][0];
-            const double dy1 = r.x[jj][1] - r.x[ii][1];
-            const double dz1 = r.x[jj][2] - r.x[ii][2];
-            const double rm1 = rm[jj], rm2 = rm[ii];
-            const double dvx1 = p.x[jj][0] * rm1 - p.x[ii][0] * rm2;
-            const double dvy1 = p.x[jj][1] * rm1 - p.x[ii][1] * rm2;
-            const double dvz1 = p.x[jj][2] * rm1 - p.x[ii][2] * rm2;
-            const double dot = dx1 * dvx1 + dy1 * dvy1 + dz1 * dvz1;
-            const double gamma = -dot / ((rm1 + rm2) * rcb[b]);
-            if (fabs(gamma) > RAT_TOL) {
-                double gdx = gamma * dx1, gdy = gamma * dy1, gdz = gamma * dz1;
-                p.x[jj][0] = p.x[jj][0] + gdx;
-                p.x[jj][1] = p.x[jj][1] + gdy;
-                p.x[jj][2] = p.x[jj][2] + gdz;
-                p.x[ii][0] = p.x[ii][0] - gdx;
-                p.x[ii][1] = p.x[ii][1] - gdy;
-                p.x[ii][2] = p.x[ii][2] - gdz;
-                gdx = gdx / halfdt;
-                gdy = gdy / halfdt;
-                gdz = gdz / halfdt;
+        for (int c = 0; c < 3; c++) {
+            const int ia = CON_A[c], ib = CON_B[c];
+            double u[3], dv[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                u[d] = r.x[ib][d] - r.x[ia][d];
+                dv[d] = p.x[ib][d] * inv_m[ib] - p.x[ia][d] * inv_m[ia];
+            }
+            const double lam = -dot3(u, dv) / ((inv_m[ib] + inv_m[ia]) * len2[c]);
+            if (fabs(lam) > tol) {
+                double f[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    f[d] = lam * u[d];
+                    p.x[ib][d] = p.x[ib][d] + f[d];
+                    p.x[ia][d] = p.x[ia][d] - f[d];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; d++) f[d] = f[d] / halfdt;
                 if (VIR) {
-                    vir[0] = vir[0] - gdx * dx1;  // (1,1)
-                    vir[1] = vir[1] - gdx * dy1;  // (1,2)
-                    vlast = gdx * dz1;            // (1,3) = (1,2) - gdx dz1
-                    vir[3] = vir[3] - gdy * dz1;  // (2,3)
-                    vir[2] = vir[2] - gdy * dy1;  // (2,2)
-                    vir[4] = vir[4] - gdz * dz1;  // (3,3)
+                    vir[0] = vir[0] - f[0] * u[0];  // (1,1)
+                    vir[1] = vir[1] - f[0] * u[1];  // (1,2)
+                    vlast = f[0] * u[2];            // (1,3) = (1,2) - this
+                    vir[3] = vir[3] - f[1] * u[2];  // (2,3)
+                    vir[2] = vir[2] - f[1] * u[1];  // (2,2)
+                    vir[4] = vir[4] - f[2] * u[2];  // (3,3)
                     any = true;
                 }
-                g.x[jj][0] = g.x[jj][0] + gdx;
-                g.x[jj][1] = g.x[jj][1] + gdy;
-                g.x[jj][2] = g.x[jj][2] + gdz;
-                g.x[ii][0] = g.x[ii][0] - gdx;
-                g.x[ii][1] = g.x[ii][1] - gdy;
-                g.x[ii][2] = g.x[ii][2] - gdz;
-                done = false;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    g.x[ib][d] = g.x[ib][d] + f[d];
+                    g.x[ia][d] = g.x[ia][d] - f[d];
+                }
+                settled = false;
             }
         }
-        it = it + 1;
+        sweeps = sweeps + 1;
     }
-    return it;
+    return sweeps;
 }
 
 }  // namespace
@@ -118,11 +121,15 @@ __global__ void k_bead_forces(const double* __restrict__ r, double* __restrict__
     }
 }
 
-// stage 1 of one bead sub-step: p -= hs g; r += p sdt / m; rattle_s1 (src/md_rattle.f90:33-158).
-// The constraint gradient rattle_s1 adds to g is not kept: bead_forces overwrites g next.
+// stage 1 of one bead sub-step: p -= hs g; r += p sdt / m; then the position stage of RATTLE
+// (rattle_s1, src/md_rattle.f90:33-158): per constrained pair with squared length off its target by
+// more than tol (relative), the pair moves along its bond vector u_0 at the start of the sub-step by
+// lam = (|u_0|^2_target - |u|^2) / (2 (1/m_a + 1/m_b) u . u_0), weighted by the inverse masses, and
+// the momenta take lam u_0 / sdt.  The constraint gradient rattle_s1 adds to g is not kept:
+// bead_forces overwrites g next.
 __global__ void k_rig_s1(double* __restrict__ p, double* __restrict__ r, const double* __restrict__ g,
-                         const double* __restrict__ mass, int na, int nb, double hs, double sdt, double rcb0,
-                         double rcb1, double rcb2, int* __restrict__ err) {
+                         const double* __restrict__ mass, int na, int nb, double hs, double sdt, double len2_0,
+                         double len2_1, double len2_2, double tol, int* __restrict__ err) {
     const int nm = na / 3;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nm * nb) return;
@@ -131,11 +138,11 @@ __global__ void k_rig_s1(double* __restrict__ p, double* __restrict__ r, const d
     load_mol(r, na, b, m, R);
     load_mol(p, na, b, m, P);
     load_mol(g, na, b, m, G);
-    double ms[3], rm[3];
+    double ms[3], inv_m[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         ms[j] = mass[3 * m + j];
-        rm[j] = 1.0 / ms[j];
+        inv_m[j] = 1.0 / ms[j];
     }
 #pragma unroll
     for (int j = 0; j < 3; j++)
@@ -145,48 +152,44 @@ __global__ void k_rig_s1(double* __restrict__ p, double* __restrict__ r, const d
             P.x[j][d] = P.x[j][d] - hs * G.x[j][d];
             R.x[j][d] = R.x[j][d] + P.x[j][d] * sdt / ms[j];
         }
-    const double rcb[3] = {rcb0, rcb1, rcb2};
-    bool done = false;
-    int it = 0;
-    while (!done && it < RAT_ITMAX) {
-        done = true;
+    const double len2[3] = {len2_0, len2_1, len2_2};
+    bool settled = false;
+    int sweeps = 0;
+    while (!settled && sweeps < RAT_ITMAX) {
+        settled = true;
 #pragma unroll
-        for (int bd = 0; bd < 3; bd++) {
-            const int ii = RAT_BI[bd], jj = RAT_BJ[bd];
-            const double dx1 = R.x[jj][0] - R.x[ii][0];
-            const double dy1 = R.x[jj][1] - R.x[ii][1];
-            const double dz1 = R.x[jj][2] - R.x[ii][2];
-            const double d2 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-            const double diff = rcb[bd] - d2;
-            if (fabs(diff) > rcb[bd] * RAT_TOL) {
-                const double rm1 = rm[jj], rm2 = rm[ii];
-                const double dx2 = R0.x[jj][0] - R0.x[ii][0];
-                const double dy2 = R0.x[jj][1] - R0.x[ii][1];
-                const double dz2 = R0.x[jj][2] - R0.x[ii][2];
-                const double dot = dx1 * dx2 + dy1 * dy2 + dz1 * dz2;
-                const double gamma = diff / (2.0 * (rm1 + rm2) * dot);
-                double gdx = gamma * dx2, gdy = gamma * dy2, gdz = gamma * dz2;
-                R.x[jj][0] = R.x[jj][0] + rm1 * gdx;
-                R.x[jj][1] = R.x[jj][1] + rm1 * gdy;
-                R.x[jj][2] = R.x[jj][2] + rm1 * gdz;
-                R.x[ii][0] = R.x[ii][0] - rm2 * gdx;
-                R.x[ii][1] = R.x[ii][1] - rm2 * gdy;
-                R.x[ii][2] = R.x[ii][2] - rm2 * gdz;
-                gdx = gdx / sdt;
-                gdy = gdy / sdt;
-                gdz = gdz / sdt;
-                P.x[jj][0] = P.x[jj][0] + gdx;
-                P.x[jj][1] = P.x[jj][1] + gdy;
-                P.x[jj][2] = P.x[jj][2] + gdz;
-                P.x[ii][0] = P.x[ii][0] - gdx;
-                P.x[ii][1] = P.x[ii][1] - gdy;
-                P.x[ii][2] = P.x[ii][2] - gdz;
-                done = false;
+        for (int c = 0; c < 3; c++) {
+            const int ia = CON_A[c], ib = CON_B[c];
+            double u[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) u[d] = R.x[ib][d] - R.x[ia][d];
+            const double miss = len2[c] - dot3(u, u);
+            if (fabs(miss) > len2[c] * tol) {
+                double u0[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) u0[d] = R0.x[ib][d] - R0.x[ia][d];
+                const double lam = miss / (2.0 * (inv_m[ib] + inv_m[ia]) * dot3(u, u0));
+                double f[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) f[d] = lam * u0[d];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    R.x[ib][d] = R.x[ib][d] + inv_m[ib] * f[d];
+                    R.x[ia][d] = R.x[ia][d] - inv_m[ia] * f[d];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; d++) f[d] = f[d] / sdt;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    P.x[ib][d] = P.x[ib][d] + f[d];
+                    P.x[ia][d] = P.x[ia][d] - f[d];
+                }
+                settled = false;
             }
         }
-        it = it + 1;
+        sweeps = sweeps + 1;
     }
-    if (m == nm - 1 && it >= RAT_ITMAX) err[0] = 1;
+    if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
     store_mol(r, na, b, m, R);
     store_mol(p, na, b, m, P);
 }
@@ -195,7 +198,7 @@ __global__ void k_rig_s1(double* __restrict__ p, double* __restrict__ r, const d
 // (which also adds its constraint gradient to g, as the reference's next sub-step then uses it)
 __global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, double* __restrict__ g,
                          const double* __restrict__ mass, int na, int nb, double hs, double sdt, double omegan2,
-                         double rcb0, double rcb1, double rcb2, int* __restrict__ err) {
+                         double len2_0, double len2_1, double len2_2, double tol, int* __restrict__ err) {
     const int nm = na / 3;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nm * nb) return;
@@ -204,12 +207,12 @@ __global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, d
     Mol R, P, G;
     load_mol(r, na, b, m, R);
     load_mol(p, na, b, m, P);
-    double rm[3];
+    double inv_m[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const int a = 3 * m + j;
         const double ms = mass[a];
-        rm[j] = 1.0 / ms;
+        inv_m[j] = 1.0 / ms;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double v = 0.0;
@@ -221,11 +224,11 @@ __global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, d
             P.x[j][d] = P.x[j][d] - hs * v;
         }
     }
-    const double rcb[3] = {rcb0, rcb1, rcb2};
+    const double len2[3] = {len2_0, len2_1, len2_2};
     double vl = 0.0;
     bool any = false;
-    const int it = rattle2_mol<false>(R, P, G, rm, rcb, 0.5 * sdt, nullptr, vl, any);
-    if (m == nm - 1 && it >= RAT_ITMAX) err[0] = 1;
+    const int sweeps = rattle_vel<false>(R, P, G, inv_m, len2, 0.5 * sdt, tol, nullptr, vl, any);
+    if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
     store_mol(p, na, b, m, P);
     store_mol(g, na, b, m, G);
 }
@@ -235,8 +238,9 @@ __global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, d
 // virial components and (highest molecule with a correction, its last gdx dz), vparts[blk][7].
 __global__ __launch_bounds__(256) void k_rig_final(double* __restrict__ p, const double* __restrict__ r,
                                                    double* __restrict__ dvdr, const double* __restrict__ mass,
-                                                   int na, double hdt, double dt, double rcb0, double rcb1,
-                                                   double rcb2, int* __restrict__ err, double* __restrict__ vparts) {
+                                                   int na, double hdt, double dt, double len2_0, double len2_1,
+                                                   double len2_2, double tol, int* __restrict__ err,
+                                                   double* __restrict__ vparts) {
     const int nm = na / 3;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
@@ -248,16 +252,16 @@ __global__ __launch_bounds__(256) void k_rig_final(double* __restrict__ p, const
         load_mol(r, na, b, m, R);
         load_mol(p, na, b, m, P);
         load_mol(dvdr, na, b, m, G);
-        double rm[3];
+        double inv_m[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            rm[j] = 1.0 / mass[3 * m + j];
+            inv_m[j] = 1.0 / mass[3 * m + j];
 #pragma unroll
             for (int d = 0; d < 3; d++) P.x[j][d] = P.x[j][d] - hdt * G.x[j][d];
         }
-        const double rcb[3] = {rcb0, rcb1, rcb2};
-        const int it = rattle2_mol<true>(R, P, G, rm, rcb, 0.5 * dt, vir, vl, any);
-        if (m == nm - 1 && it >= RAT_ITMAX) err[0] = 1;
+        const double len2[3] = {len2_0, len2_1, len2_2};
+        const int sweeps = rattle_vel<true>(R, P, G, inv_m, len2, 0.5 * dt, tol, vir, vl, any);
+        if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
         store_mol(p, na, b, m, P);
         store_mol(dvdr, na, b, m, G);
     }
@@ -326,6 +330,29 @@ __global__ void k_rig_vir(const double* __restrict__ vparts, int nblk, int nb, d
     for (int k = 0; k < 9; k++) out[k] = acc[k] / (double)nb;
 }
 
+// The velocity stage of RATTLE alone on every (molecule, bead): after a thermostat kick, the
+// momenta are put back on the constraint surface's tangent space (no gradient, no virial)
+__global__ void k_rig_project(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ mass,
+                              int na, int nb, double len2_0, double len2_1, double len2_2, double tol,
+                              int* __restrict__ err) {
+    const int nm = na / 3;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nm * nb) return;
+    const int b = (int)(t / nm), m = (int)(t % nm);
+    Mol R, P, G{};
+    load_mol(r, na, b, m, R);
+    load_mol(p, na, b, m, P);
+    double inv_m[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) inv_m[j] = 1.0 / mass[3 * m + j];
+    const double len2[3] = {len2_0, len2_1, len2_2};
+    double vl = 0.0;
+    bool any = false;
+    const int sweeps = rattle_vel<false>(R, P, G, inv_m, len2, 1.0, tol, nullptr, vl, any);
+    if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
+    store_mol(p, na, b, m, P);
+}
+
 int rigid_blocks(int na) { return (na / 3 + 255) / 256; }
 
 void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,
@@ -334,16 +361,23 @@ void launch_bead_forces(const double* r, double* g, const double* mass, int na, 
     k_bead_forces<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(r, g, mass, na, nb, omegan2);
 }
 void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
-                        double omegan2, const double rcb[3], int* err, hipStream_t s) {
+                        double omegan2, const double len2[3], double tol, int* err, hipStream_t s) {
     const size_t n = (size_t)(na / 3) * nb;
     const unsigned grid = (unsigned)((n + 255) / 256);
-    k_rig_s1<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, rcb[0], rcb[1], rcb[2], err);
-    k_rig_s2<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, omegan2, rcb[0], rcb[1], rcb[2], err);
+    k_rig_s1<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, len2[0], len2[1], len2[2], tol, err);
+    k_rig_s2<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, omegan2, len2[0], len2[1], len2[2], tol, err);
+}
+void launch_rig_project(double* p, const double* r, const double* mass, int na, int nb, const double len2[3],
+                        double tol, int* err, hipStream_t s) {
+    const size_t n = (size_t)(na / 3) * nb;
+    k_rig_project<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, r, mass, na, nb, len2[0], len2[1], len2[2], tol,
+                                                            err);
 }
 void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
-                      double dt, const double rcb[3], int* err, double* vparts, double* vir_out, hipStream_t s) {
+                      double dt, const double len2[3], double tol, int* err, double* vparts, double* vir_out,
+                      hipStream_t s) {
     const dim3 grid(rigid_blocks(na), nb);
-    k_rig_final<<<grid, 256, 0, s>>>(p, r, dvdr, mass, na, hdt, dt, rcb[0], rcb[1], rcb[2], err, vparts);
+    k_rig_final<<<grid, 256, 0, s>>>(p, r, dvdr, mass, na, hdt, dt, len2[0], len2[1], len2[2], tol, err, vparts);
     k_rig_vir<<<1, 64, 0, s>>>(vparts, (int)grid.x, nb, vir_out);
 }
 

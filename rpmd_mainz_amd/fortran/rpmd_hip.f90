@@ -1,14 +1,21 @@
 ! rpmd_hip.f90 -- ISO_C_BINDING layer between the reference's Fortran host and librpmd_mi355x.so.
 !
-! Link this file in place of the reference's md_forces.f90 (forces, full_forces) and fourier.f90
-! (realft): the external subroutines below keep the reference's names and dummy-argument lists
+! Link this file in place of the reference's md_forces.f90 (forces, full_forces), fourier.f90
+! (realft), evolve_pi.f90 (evolve_pi) and evolve_rigid.f90's evolve_rig_pi: the external subroutines
+! below keep the reference's names and dummy-argument lists
 !   forces(r,v,dvdr,nb,na,boxlxyz,z,virial,iopt)                src/md_forces.f90:1
 !   full_forces(r,na,nb,v,vew,voo,vint,vir,z,boxlxyz,dvdr,dvdr2) src/md_forces.f90:305
 !   realft(data,m,n,mode)                                       src/fourier.f90:16
+!   evolve_pi(p,r,v,vew,vlj,vint,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,irun,nbaro)
+!                                                               src/evolve_pi.f90:1
+!   evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,irun,nbaro)
+!                                                               src/evolve_rigid.f90:100
 ! so every existing call site compiles and links unchanged.  Model parameters are read from the
-! same COMMON blocks the reference routines read (src/main.f90:53-72) on every call; a context
-! for the largest (na, nb) seen is created on first use.  A non-zero status from the library
-! stops the program with the reference's message, as the reference's STOP statements do.
+! same COMMON blocks the reference routines read (src/main.f90:53-72) on every call; the
+! thermostat (module thermostat's therm and ttau, src/module_thermostat.f90:3-4) is handed over
+! once with rpmd_set_thermostat, since that module belongs to the host.  A context for the largest
+! (na, nb) seen is created on first use.  A non-zero status from the library stops the program with
+! the reference's message, as the reference's STOP statements do.
 module rpmd_hip
   use iso_c_binding
   implicit none
@@ -25,6 +32,7 @@ module rpmd_hip
      integer(c_int) :: intra_model, therm
      real(c_double) :: ttau
      integer(c_int) :: rigid
+     real(c_double) :: rat_tol
   end type rpmd_params_t
 
   interface
@@ -102,12 +110,106 @@ module rpmd_hip
        import :: c_int, c_ptr
        integer(c_int), value :: code
      end function c_strerror
+     ! host-side Ewald set-up, src/setup_options.f90:104-189
+     integer(c_int) function c_setup_ewald(na, box, prm) bind(C, name='rpmd_setup_ewald')
+       import :: c_int, c_double, rpmd_params_t
+       integer(c_int), value :: na
+       real(c_double), intent(in) :: box(3)
+       type(rpmd_params_t), intent(inout) :: prm
+     end function c_setup_ewald
+     integer(c_int) function c_ctx_set_precision(ctx, precision) bind(C, name='rpmd_ctx_set_precision')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: precision
+     end function c_ctx_set_precision
+     ! sample_rpmd, src/module_thermostat.f90:199 (noise: c_null_ptr draws on the device from seed)
+     integer(c_int) function c_sample_rpmd(ctx, p, na, nb, mass, beta, noise, n_noise, seed) &
+          bind(C, name='rpmd_sample_rpmd')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: p(*)
+       integer(c_int), value :: na, nb
+       real(c_double), intent(in) :: mass(*)
+       real(c_double), value :: beta
+       type(c_ptr), value :: noise
+       integer(c_long_long), value :: n_noise, seed
+     end function c_sample_rpmd
+     ! the free ring-polymer update of evolve_pi (normal-mode p, r), src/evolve_pi.f90:109-125
+     integer(c_int) function c_free_ring(ctx, p, r, mass, na, nb, dt, beta) bind(C, name='rpmd_free_ring')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: p(*), r(*)
+       real(c_double), intent(in) :: mass(*)
+       integer(c_int), value :: na, nb
+       real(c_double), value :: dt, beta
+     end function c_free_ring
+     ! realft without a context or device allocation
+     integer(c_int) function c_realft_noalloc(data, m, n, mode) bind(C, name='rpmd_realft_noalloc')
+       import :: c_int, c_double
+       real(c_double), intent(inout) :: data(*)
+       integer(c_int), value :: m, n, mode
+     end function c_realft_noalloc
+     ! device-resident stepping: upload once, run steps in HBM, read back when needed
+     integer(c_int) function c_state_upload(ctx, p, r, mass, z, na, nb, box, beta, dt, seed) &
+          bind(C, name='rpmd_state_upload')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: p(*), r(*), mass(*), z(*), box(3)
+       integer(c_int), value :: na, nb
+       real(c_double), value :: beta, dt
+       integer(c_long_long), value :: seed
+     end function c_state_upload
+     integer(c_int) function c_run_steps(ctx, nsteps) bind(C, name='rpmd_run_steps')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+     end function c_run_steps
+     integer(c_int) function c_state_download(ctx, p, r, dvdr, dvdr2, energies, vir) bind(C, name='rpmd_state_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: p(*), r(*), dvdr(*), dvdr2(*), energies(4), vir(9)
+     end function c_state_download
+     integer(c_int) function c_use_graph(ctx, enable) bind(C, name='rpmd_use_graph')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function c_use_graph
+     integer(c_int) function c_ctx_set_streams(ctx, streams) bind(C, name='rpmd_ctx_set_streams')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: streams
+     end function c_ctx_set_streams
+     ! replica batches: contexts with uploaded states advanced together
+     integer(c_int) function c_batch_create(ctxs, n, batch) bind(C, name='rpmd_batch_create')
+       import :: c_int, c_ptr
+       type(c_ptr), intent(in) :: ctxs(*)
+       integer(c_int), value :: n
+       type(c_ptr), intent(out) :: batch
+     end function c_batch_create
+     integer(c_int) function c_batch_run_steps(batch, nsteps) bind(C, name='rpmd_batch_run_steps')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: nsteps
+     end function c_batch_run_steps
+     integer(c_int) function c_batch_destroy(batch) bind(C, name='rpmd_batch_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+     end function c_batch_destroy
   end interface
 
   type(c_ptr), save :: rpmd_ctx = c_null_ptr
   integer, save :: ctx_na = 0, ctx_nb = 0
   ! RPMD_PREC_FP64 = 0, RPMD_PREC_MIXED = 1; set before the first call to choose the mode
   integer, save :: rpmd_precision = 0
+  ! the thermostat of the step entry points (RPMD_THERM_* and ttau in a.u. time), as the host's
+  ! module thermostat holds it; set with rpmd_set_thermostat
+  integer, save :: rpmd_therm = 0
+  real(8), save :: rpmd_ttau = 0.d0
+  ! RATTLE tolerance of the rigid step (<= 0: the reference's 1e-6)
+  real(8), save :: rpmd_rattle_tol = 0.d0
+  ! replayed Gaussian deviates for the step entry points (rpmd_replay_noise); none: the device's
+  ! own counter-based stream
+  real(c_double), allocatable, target, save :: rpmd_noise(:)
 
 contains
 
@@ -136,10 +238,100 @@ contains
     prm%theta = theta; prm%reoh = reoh; prm%sig = sig
     prm%nbdf1 = nbdf1; prm%nbdf2 = nbdf2; prm%mts = max(mts, 1)
     prm%intra_model = 0          ! RPMD_INTRA_AUTO: the reference's own selection rule
-    prm%therm = 0
-    prm%ttau = 0.d0
+    prm%therm = rpmd_therm
+    prm%ttau = rpmd_ttau
     prm%rigid = merge(1, 0, iamrigid)
+    prm%rat_tol = rpmd_rattle_tol
   end subroutine rpmd_fill_params
+
+  subroutine rpmd_set_thermostat(therm, ttau)
+    ! the host's thermostat choice (src/module_thermostat.f90:3-4; the codes of src/main.f90:218-252):
+    ! 'PRO' PILE-L, 'PRM' PILE-G, 'PRG' Bussi, 'PRL' local Langevin, 'GLE'; any other code (e.g.
+    ! 'AND') leaves the step thermostat-free, as inside the reference's evolve_pi
+    character(len=*), intent(in) :: therm
+    real(8), intent(in) :: ttau
+    select case (therm(1:min(3, len(therm))))
+    case ('PRO'); rpmd_therm = 1
+    case ('PRM'); rpmd_therm = 2
+    case ('PRG'); rpmd_therm = 3
+    case ('PRL'); rpmd_therm = 4
+    case ('GLE'); rpmd_therm = 5
+    case default; rpmd_therm = 0
+    end select
+    rpmd_ttau = ttau
+  end subroutine rpmd_set_thermostat
+
+  subroutine rpmd_replay_noise(noise)
+    ! deviates consumed by the next step entry point in the reference's draw order (the host draws
+    ! them with its gaussian), instead of the device stream
+    real(8), intent(in) :: noise(:)
+    if (allocated(rpmd_noise)) deallocate(rpmd_noise)
+    allocate(rpmd_noise(size(noise)))
+    rpmd_noise = noise
+  end subroutine rpmd_replay_noise
+
+  subroutine rpmd_device_noise()
+    ! back to the device's own stream
+    if (allocated(rpmd_noise)) deallocate(rpmd_noise)
+  end subroutine rpmd_device_noise
+
+  subroutine rpmd_noise_arg(np, nn)
+    type(c_ptr), intent(out) :: np
+    integer(c_long_long), intent(out) :: nn
+    if (allocated(rpmd_noise)) then
+       np = c_loc(rpmd_noise)
+       nn = size(rpmd_noise, kind=c_long_long)
+    else
+       np = c_null_ptr
+       nn = 0
+    endif
+  end subroutine rpmd_noise_arg
+
+  subroutine rpmd_gle_setup(ns, a, na, nb, dt, beta, gp, seed)
+    ! therm_gle_init with the drift matrix given: a(ns+1, ns+1) as the GLE-A file lists it (a(i,:)
+    ! = row i), dt the half step the reference passes (src/main.f90:609); gp(3,na,nb,ns+1) the
+    ! initial auxiliary momenta (optional: drawn on the device from seed otherwise)
+    integer, intent(in) :: ns, na, nb
+    real(8), intent(in) :: a(ns + 1, ns + 1), dt, beta
+    real(8), intent(in), optional, target :: gp(3, na, nb, ns + 1)
+    integer, intent(in), optional :: seed
+    real(c_double), target :: at(ns + 1, ns + 1)
+    integer(c_long_long) :: sd
+    call rpmd_bind(na, nb)
+    at = transpose(a)      ! row order in memory, as the C-ABI takes it
+    sd = 0
+    if (present(seed)) sd = seed
+    if (present(gp)) then
+       call rpmd_check(c_gle_init(rpmd_ctx, int(ns, c_int), at, int(na, c_int), int(nb, c_int), dt, beta, &
+                                  c_loc(gp), sd))
+    else
+       call rpmd_check(c_gle_init(rpmd_ctx, int(ns, c_int), at, int(na, c_int), int(nb, c_int), dt, beta, &
+                                  c_null_ptr, sd))
+    endif
+  end subroutine rpmd_gle_setup
+
+  subroutine rpmd_therm_gle_init(ttau, na, nb, dt, irun, beta)
+    ! therm_gle_init(ttau,na,nb,dt,irun,beta) (src/module_gle.f90:146-227): reads GLE-A from the
+    ! working directory as the reference does (ns, then ns+1 rows); C = (nb/beta) I; the auxiliary
+    ! momenta are drawn on the device (seeded by irun)
+    real(8), intent(in) :: ttau, dt, beta
+    integer, intent(in) :: na, nb, irun
+    real(8), allocatable :: a(:, :)
+    integer :: ns, i, ios
+    rpmd_ttau = ttau
+    open(121, file='GLE-A', status='old', iostat=ios)
+    if (ios .ne. 0) then
+       write(0,*) 'COULD NOT FIND GLE-A!'
+       error stop 'rpmd_hip'
+    endif
+    read(121, *) ns
+    allocate(a(ns + 1, ns + 1))
+    do i = 1, ns + 1
+       read(121, *) a(i, :)
+    enddo
+    close(121)
+    call rpmd_gle_setup(ns, a, na, nb, dt, beta, seed=irun)
+  end subroutine rpmd_therm_gle_init
 
   subroutine rpmd_check(code)
     integer(c_int), intent(in) :: code
@@ -212,3 +404,46 @@ subroutine realft(data, m, n, mode)
        call rpmd_bind(3 * ((m + 8) / 9), n)
   call rpmd_check(c_realft(rpmd_ctx, data, int(m, c_int), int(n, c_int), int(mode, c_int)))
 end subroutine realft
+
+
+subroutine evolve_pi(p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf, irun, nbaro)
+  ! one RPMD step (type 'RPMD'); p, r, dvdr, dvdr2 in and out as in the reference.  The thermostat
+  ! deviates come from rpmd_replay_noise when set, else from the device stream (irun is not
+  ! consumed).  Barostat steps (nbaro = 1) are not on this path.
+  use rpmd_hip
+  implicit none
+  integer na, nb, irun, nbaro
+  real(8) p(3,na,nb), r(3,na,nb), dvdr(3,na,nb), dvdr2(3,na,nb), mass(na), z(na), boxlxyz(3)
+  real(8) v, vew, vlj, vint, dt, beta, vir(3,3), vir_lf(3,3)
+  type(c_ptr) :: np
+  integer(c_long_long) :: nn
+  if (nbaro .ne. 0) then
+     write(6,*) ' ** barostat steps are not on this path **'
+     error stop 'rpmd_hip'
+  endif
+  call rpmd_bind(na, nb)
+  call rpmd_noise_arg(np, nn)
+  call rpmd_check(c_evolve_pi(rpmd_ctx, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, int(na, c_int), &
+                              int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn))
+end subroutine evolve_pi
+
+
+subroutine evolve_rig_pi(p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf, irun, &
+                         nbaro)
+  ! one rigid RPMD step (/structure/ iamrigid); vir in and out as in the reference
+  use rpmd_hip
+  implicit none
+  integer na, nb, irun, nbaro
+  real(8) p(3,na,nb), r(3,na,nb), dvdr(3,na,nb), dvdr2(3,na,nb), mass(na), z(na), boxlxyz(3)
+  real(8) v, v_lf, v_hf, dt, beta, vir(3,3), vir_lf(3,3)
+  type(c_ptr) :: np
+  integer(c_long_long) :: nn
+  if (nbaro .ne. 0) then
+     write(6,*) ' ** barostat steps are not on this path **'
+     error stop 'rpmd_hip'
+  endif
+  call rpmd_bind(na, nb)
+  call rpmd_noise_arg(np, nn)
+  call rpmd_check(c_evolve_rig_pi(rpmd_ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, int(na, c_int), &
+                                  int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn))
+end subroutine evolve_rig_pi

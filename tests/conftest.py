@@ -24,3 +24,11 @@ def refpath():
     if not os.path.exists(LIB_REF):
         pytest.skip("oracle/_ref/libref_path.so not built (needs the reference sources)")
     return RefPath()
+
+
+@pytest.fixture(scope="session")
+def capi():
+    """the ctypes binding of the HIP library (loading fails loudly when it is not built)"""
+    from rpmd_mainz_amd import capi as c
+    c.lib()
+    return c
