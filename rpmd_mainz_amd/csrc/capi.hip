@@ -1148,11 +1148,17 @@ int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
                         nz.stream_add++, c->com_parts, s);
             launch_com_remove(c->pnm, c->mass, c->com_parts, pile_blocks(na), 1.0 / c->mtot, na, s);
             launch_nm(c->pnm, c->p, nmm->B, nb, (size_t)3 * na, s);
-            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, c->rat_err, s);
+            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, 0, c->rat_err, s);
         } else if (P.therm == RPMD_THERM_GLE) {
+            // every channel onto the tangent space: the physical momenta (channel 0 is reloaded from
+            // them by the next kick) and the auxiliary ones, so that the whole GLE process lives
+            // on the constraint surface (projecting p alone leaves the auxiliary momenta feeding
+            // energy into the constrained directions)
             launch_gle(c->p, c->gle_gp, c->gle_TS, c->mass, na, nb, c->gle_n1, nz.take(draws * c->gle_n1), c->seed,
                        c->step_ctr, nz.stream_add++, c->gle_parts, c->gle_pcom, 1.0 / sqrt(nb * c->mtot), s);
-            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, c->rat_err, s);
+            launch_rig_project(c->p, c->r, c->mass, na, nb, len2, tol, 0, c->rat_err, s);
+            for (int ch = 1; ch < c->gle_n1; ch++)
+                launch_rig_project(c->gle_gp + (size_t)ch * draws, c->r, c->mass, na, nb, len2, tol, 1, c->rat_err, s);
         }
     };
     thermo();

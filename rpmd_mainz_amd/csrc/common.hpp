@@ -258,8 +258,9 @@ void launch_rig_substep(double* p, double* r, double* g, const double* mass, int
 void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
                       double dt, const double len2[3], double tol, int* err, double* vparts, double* vir_out,
                       hipStream_t s);
-// the velocity stage of RATTLE alone (after a thermostat kick of the rigid step)
+// the velocity stage of RATTLE alone (after a thermostat kick of the rigid step); scaled: p holds
+// mass-scaled momenta (a GLE auxiliary channel)
 void launch_rig_project(double* p, const double* r, const double* mass, int na, int nb, const double len2[3],
-                        double tol, int* err, hipStream_t s);
+                        double tol, int scaled, int* err, hipStream_t s);
 
 }  // namespace rpmd

@@ -331,9 +331,10 @@ __global__ void k_rig_vir(const double* __restrict__ vparts, int nblk, int nb, d
 }
 
 // The velocity stage of RATTLE alone on every (molecule, bead): after a thermostat kick, the
-// momenta are put back on the constraint surface's tangent space (no gradient, no virial)
+// momenta are put back on the constraint surface's tangent space (no gradient, no virial).
+// scaled: p holds mass-scaled momenta p / sqrt(m) (a GLE auxiliary channel), projected the same way.
 __global__ void k_rig_project(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ mass,
-                              int na, int nb, double len2_0, double len2_1, double len2_2, double tol,
+                              int na, int nb, double len2_0, double len2_1, double len2_2, double tol, int scaled,
                               int* __restrict__ err) {
     const int nm = na / 3;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,14 +343,27 @@ __global__ void k_rig_project(double* __restrict__ p, const double* __restrict__
     Mol R, P, G{};
     load_mol(r, na, b, m, R);
     load_mol(p, na, b, m, P);
-    double inv_m[3];
+    double inv_m[3], sq[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) inv_m[j] = 1.0 / mass[3 * m + j];
+    for (int j = 0; j < 3; j++) {
+        inv_m[j] = 1.0 / mass[3 * m + j];
+        sq[j] = sqrt(mass[3 * m + j]);
+    }
+    if (scaled)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) P.x[j][d] = P.x[j][d] * sq[j];
     const double len2[3] = {len2_0, len2_1, len2_2};
     double vl = 0.0;
     bool any = false;
     const int sweeps = rattle_vel<false>(R, P, G, inv_m, len2, 1.0, tol, nullptr, vl, any);
     if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
+    if (scaled)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) P.x[j][d] = P.x[j][d] / sq[j];
     store_mol(p, na, b, m, P);
 }
 
@@ -368,10 +382,10 @@ void launch_rig_substep(double* p, double* r, double* g, const double* mass, int
     k_rig_s2<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, omegan2, len2[0], len2[1], len2[2], tol, err);
 }
 void launch_rig_project(double* p, const double* r, const double* mass, int na, int nb, const double len2[3],
-                        double tol, int* err, hipStream_t s) {
+                        double tol, int scaled, int* err, hipStream_t s) {
     const size_t n = (size_t)(na / 3) * nb;
     k_rig_project<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, r, mass, na, nb, len2[0], len2[1], len2[2], tol,
-                                                            err);
+                                                            scaled, err);
 }
 void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
                       double dt, const double len2[3], double tol, int* err, double* vparts, double* vir_out,
