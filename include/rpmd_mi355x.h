@@ -196,6 +196,29 @@ int rpmd_state_upload(rpmd_ctx* ctx, const double* p, const double* r, const dou
 int rpmd_run_steps(rpmd_ctx* ctx, int nsteps);
 int rpmd_state_download(rpmd_ctx* ctx, double* p, double* r, double* dvdr, double* dvdr2,
                         double energies[4], double vir[9]);
+/* Estimators of the device-resident state, computed on the device after rpmd_run_steps (no state
+ * download): out[RPMD_EST_N].
+ *   KINETIC   tk = sum over beads and atoms of |p|^2 / (2 m)       kinetic_rpmd, src/kinetic.f90:248-265
+ *   SPRINGS   vrp = sum m wn^2 |r_k - r_{k-1}|^2 / 2, wn = nb/beta  src/md_eq.f90:257-274
+ *   POTENTIAL v, the bead-averaged potential of the last step
+ *   HRP       (nb v + tk + vrp) / na, the conserved quantity of NVE RPMD per atom  src/md_eq.f90:278
+ *   TEMP      toK 2 tk / ((3 na - nctot - 3) nb) / nb, kelvin      src/md_eq.f90:276
+ *   TV, TQ1, TQ2, TVX..TVZ  the centroid-virial kinetic energy of virial_ke with the state's low-
+ *             and high-frequency gradients (src/kinetic.f90:1-54; nc = 3, nctot = 3 nm when rigid) */
+#define RPMD_EST_KINETIC   0
+#define RPMD_EST_SPRINGS   1
+#define RPMD_EST_POTENTIAL 2
+#define RPMD_EST_HRP       3
+#define RPMD_EST_TEMP      4
+#define RPMD_EST_TV        5
+#define RPMD_EST_TQ1       6
+#define RPMD_EST_TQ2       7
+#define RPMD_EST_TVX       8
+#define RPMD_EST_TVY       9
+#define RPMD_EST_TVZ      10
+#define RPMD_EST_N        11
+int rpmd_estimators(rpmd_ctx* ctx, double out[RPMD_EST_N]);
+
 /* Capture rpmd_run_steps(ctx, 1) into a HIP graph and replay it for later calls. */
 int rpmd_use_graph(rpmd_ctx* ctx, int enable);
 /* Streams of a device-resident force call (no counterpart in the reference): 2 (default) runs the

@@ -511,6 +511,25 @@ class RefPath:
         self.lib._QMglePcholesky(A.ctypes.data_as(_dp), out.ctypes.data_as(_dp), C.byref(C.c_int(n)))
         return np.array(out)
 
+    def virial_ke(self, r, dvdr, dvdr2, beta, nctot=0):
+        """virial_ke (src/kinetic.f90:1-54): (tv, tvxyz, tq1, tq2); nctot via /constraint/"""
+        self.set_nctot(nctot)
+        r, g1, g2 = _f64(r).copy(), _f64(dvdr).copy(), _f64(dvdr2).copy()
+        nb, na = r.shape[0], r.shape[1]
+        tv, tq1, tq2 = C.c_double(), C.c_double(), C.c_double()
+        tvxyz = np.zeros(3)
+        self.lib.virial_ke_(_d(r), _d(g1), _d(g2), C.byref(tv), _d(tvxyz), C.byref(tq1), C.byref(tq2),
+                            C.byref(C.c_double(beta)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)))
+        return tv.value, tvxyz, tq1.value, tq2.value
+
+    def kinetic_rpmd(self, p, mass):
+        """kinetic_rpmd (src/kinetic.f90:248-265)"""
+        p = _f64(p).copy()
+        nb, na = p.shape[0], p.shape[1]
+        tk = C.c_double()
+        self.lib.kinetic_rpmd_(_d(p), C.byref(C.c_int(na)), C.byref(tk), _d(_f64(mass).copy()), C.byref(C.c_int(nb)))
+        return tk.value
+
     def ring_rpmd(self, mass3, nb, dt, beta):
         mon = np.zeros((nb, 4, 3))
         self.lib.ring_rpmd_(_d(_f64(mass3)), _d(mon), C.byref(C.c_int(3)), C.byref(C.c_int(nb)),

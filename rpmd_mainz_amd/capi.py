@@ -69,7 +69,7 @@ def lib() -> C.CDLL:
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
-                     "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc"):
+                     "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -283,6 +283,16 @@ class RPMDContext:
         _check(lib().rpmd_prof_end(self._h, ms, cnt))
         names = ("ewald_real", "k_sk", "k_kforce", "k_sk_final", "vlist")
         return {n: (ms[i], cnt[i]) for i, n in enumerate(names)}
+
+    EST_NAMES = ("kinetic", "springs", "potential", "h_rp_per_atom", "temperature_K", "tv", "tq1", "tq2",
+                 "tv_x", "tv_y", "tv_z")
+
+    def estimators(self) -> dict:
+        """rpmd_estimators: kinetic and spring energies, H_RP per atom, temperature and the
+        centroid-virial kinetic energy of the device-resident state, computed on the device."""
+        out = (C.c_double * len(self.EST_NAMES))()
+        _check(lib().rpmd_estimators(self._h, out))
+        return {n: out[i] for i, n in enumerate(self.EST_NAMES)}
 
     def state_download(self):
         nb, na = self._st_nb, self.na

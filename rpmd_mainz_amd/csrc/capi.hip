@@ -2162,6 +2162,48 @@ int rpmd_prof_end(rpmd_ctx* c, double ms[RPMD_PROF_N], int counts[RPMD_PROF_N]) 
     return 0;
 }
 
+int rpmd_estimators(rpmd_ctx* c, double out[RPMD_EST_N]) {
+    if (!c || !c->st_valid || !out) return RPMD_ERR_ARG;
+    const int na = c->na, nb = c->st_nb, nm = na / 3;
+    const double beta = c->st_beta, wn = (double)nb / (beta * 1.0);
+    // parts in tmp (scratch between steps), the ten sums in ev (50 doubles)
+    const int nblk = estimator_blocks(na);
+    launch_estimators(c->p, c->r, c->dvdr, c->dvdr2, c->mass, na, nb, wn * wn, c->tmp, c->st);
+    ReduceJobs jobs;
+    jobs.add(c->tmp, nblk, 0, 1.0);
+    launch_reduce_ev(jobs, c->ev, c->st);
+    double h[10];
+    HIPCHK(hipMemcpyAsync(h, c->ev, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double en[4], vir[9];
+    if (c->prm.rigid) {
+        RETCHK(step_rig_energies(c, &en[0], vir, nullptr));
+    } else {
+        RETCHK(step_energies(c, &en[0], &en[1], &en[2], &en[3], vir, nullptr));
+    }
+    const double v = en[0];
+    // /constraint/ nctot: three constraints per molecule when rigid (src/md_rattle.f90:29)
+    const double nctot = c->prm.rigid ? 3.0 * nm : 0.0;
+    const double tk = h[0], vrp = h[1];
+    // virial_ke (src/kinetic.f90:42-49): nc = 3
+    const double tq1x = 0.5 * h[2] / nb, tq1y = 0.5 * h[3] / nb, tq1z = 0.5 * h[4] / nb;
+    const double tq2x = 0.5 * h[5] / nb, tq2y = 0.5 * h[6] / nb, tq2z = 0.5 * h[7] / nb;
+    const double tq1 = 0.5 * (h[2] + h[3] + h[4]) / nb, tq2 = 0.5 * (h[5] + h[6] + h[7]) / nb;
+    const double tc = 0.5 * (3.0 * na - 3.0 - nctot) / beta;
+    out[RPMD_EST_KINETIC] = tk;
+    out[RPMD_EST_SPRINGS] = vrp;
+    out[RPMD_EST_POTENTIAL] = v;
+    out[RPMD_EST_HRP] = ((double)nb * v + tk + vrp) / na;
+    out[RPMD_EST_TEMP] = 3.157733e+5 * (2.0 * tk / ((3.0 * na - nctot - 3.0) * nb)) / nb;
+    out[RPMD_EST_TV] = tc + tq1 + tq2;
+    out[RPMD_EST_TQ1] = tq1;
+    out[RPMD_EST_TQ2] = tq2;
+    out[RPMD_EST_TVX] = tc / 3 + tq1x + tq2x;
+    out[RPMD_EST_TVY] = tc / 3 + tq1y + tq2y;
+    out[RPMD_EST_TVZ] = tc / 3 + tq1z + tq2z;
+    return 0;
+}
+
 int rpmd_state_download(rpmd_ctx* c, double* p, double* r, double* dvdr, double* dvdr2, double energies[4],
                         double vir[9]) {
     if (!c || !c->st_valid) return RPMD_ERR_ARG;

@@ -248,6 +248,12 @@ void launch_sample_rpmd(double* p, const double* mass, int na, int nb, double be
                         const double* noise, uint64_t seed, uint64_t stream, double* parts, double* v,
                         hipStream_t s);
 
+// estimator sums of the state: parts[estimator_blocks(na)][10] = {kinetic, springs, centroid-virial
+// sums of g1 and g2 per xyz, 0, 0} (reduce with launch_reduce_ev)
+int estimator_blocks(int na);
+void launch_estimators(const double* p, const double* r, const double* g1, const double* g2, const double* mass,
+                       int na, int nb, double wn2, double* parts, hipStream_t s);
+
 // ---- k_rigid.hip ------------------------------------------------------------------------------
 int rigid_blocks(int na);
 void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,

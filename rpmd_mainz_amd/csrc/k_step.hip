@@ -1003,6 +1003,56 @@ void launch_scale(double* y, double a, size_t n, hipStream_t s) {
 }
 
 // per-step noise stream counter kept on the device so that a captured step replays with fresh noise
+// ---- estimators of the state (src/kinetic.f90:1-54, 248-265; src/md_eq.f90:255-278) -------------
+// Thread per atom over every bead and component: kinetic energy sum_k |p|^2 / (2 m); spring energy
+// sum_k m wn^2 |r_k - r_{k-1}|^2 / 2 (k - 1 cyclic); and the centroid-virial sums
+// sum_k (r_k - r_c) . dvdr_k per xyz, for the low- (g1) and high-frequency (g2) gradients.  Block
+// partial sums in fixed order, parts[blk][10] = {tk, vrp, tq1 xyz, tq2 xyz, 0, 0}.
+__global__ __launch_bounds__(256) void k_estimators(const double* __restrict__ p, const double* __restrict__ r,
+                                                    const double* __restrict__ g1, const double* __restrict__ g2,
+                                                    const double* __restrict__ mass, int na, int nb, double wn2,
+                                                    double* __restrict__ parts) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (a < na) {
+        const double m = mass[a];
+        double p2 = 0.0, dist = 0.0;
+        for (int d = 0; d < 3; d++) {
+            double rc = 0.0;
+            for (int k = 0; k < nb; k++) rc += r[((size_t)k * 3 + d) * na + a];
+            rc = rc / (double)nb;
+            for (int k = 0; k < nb; k++) {
+                const size_t o = ((size_t)k * 3 + d) * na + a;
+                const size_t op = ((size_t)(k == 0 ? nb - 1 : k - 1) * 3 + d) * na + a;
+                const double pk = p[o], rk = r[o], dx = rk - r[op], dr = rk - rc;
+                p2 += pk * pk;
+                dist += dx * dx;
+                v[2 + d] += dr * g1[o];
+                v[5 + d] += dr * g2[o];
+            }
+        }
+        v[0] = p2 / (2.0 * m);
+        v[1] = 0.5 * m * wn2 * dist;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 10; c++) v[c] += __shfl_down(v[c], off, 64);
+    __shared__ double red[4][10];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 10; c++) red[w][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 10)
+        parts[(size_t)blockIdx.x * 10 + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+int estimator_blocks(int na) { return (na + 255) / 256; }
+void launch_estimators(const double* p, const double* r, const double* g1, const double* g2, const double* mass,
+                       int na, int nb, double wn2, double* parts, hipStream_t s) {
+    k_estimators<<<estimator_blocks(na), 256, 0, s>>>(p, r, g1, g2, mass, na, nb, wn2, parts);
+}
+
 __global__ void k_counter_inc(uint64_t* c) { ctrl_add(c, (uint64_t)RPMD_STREAMS_PER_STEP); }
 void launch_counter_inc(uint64_t* c, hipStream_t s) { k_counter_inc<<<1, 1, 0, s>>>(c); }
 

@@ -118,3 +118,32 @@ def test_batch_with_a_contracted_replica(capi):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("case", ["flexible", "contracted", "rigid"])
+def test_device_estimators_match_state(capi, case):
+    """rpmd_estimators (kinetic and spring energy, H_RP per atom, temperature, virial_ke) computed on
+    the device after steps, against the same estimators of the downloaded state (pinned against the
+    reference's kinetic_rpmd and virial_ke on the CPU), without touching the state or its graph."""
+    from test_oracle import np_estimators
+    if case == "rigid":
+        r, box, z, mass, prm = cf.water_system("tip4p2005", (4, 4, 4), 8, ecut=1e-6, rigid=True, therm="PRO",
+                                               ttaufs=100.0, amp=0.05)
+        nb, na = r.shape[:2]
+        beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+        p = np.zeros_like(r)
+    else:
+        extra = dict(nbdf1=5, nbdf2=7, mts=2) if case == "contracted" else {}
+        r, p, box, z, mass, prm, beta, dt, na, nb = _system(nb=16 if extra else 8, **extra)
+    c = capi.RPMDContext(prm, na, nb, "fp64")
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=5)
+    c.use_graph(True)
+    c.run_steps(4)
+    e = c.estimators()
+    st = c.state_download()
+    c.run_steps(2)           # the graph is still valid after the estimators
+    c.close()
+    ref = np_estimators(st["p"], st["r"], st["dvdr"], st["dvdr2"], mass, beta, st["v"],
+                        3 * (na // 3) if case == "rigid" else 0)
+    for k, v in ref.items():
+        assert abs(e[k] - v) <= 1e-12 * max(abs(v), 1e-300), (k, e[k], v)

@@ -801,3 +801,52 @@ def test_evolve_pi_gle_composed_from_reference(refpath, tmp_path):
                                                           str(tmp_path)))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0 and "gle ok" in out.stdout, out.stdout[-3000:] + out.stderr[-3000:]
+
+
+# ---- estimators of a ring-polymer state (src/kinetic.f90, src/md_eq.f90:255-278) ---------------------
+def np_estimators(p, r, g1, g2, mass, beta, v, nctot=0):
+    """The estimators the reference's md_eq prints, from a state: kinetic energy (kinetic_rpmd),
+    spring energy, H_RP per atom, temperature, and virial_ke's centroid-virial kinetic energy."""
+    nb, na = r.shape[:2]
+    m = mass[None, :, None]
+    tk = float(np.sum(p ** 2 / (2.0 * m)))
+    wn = nb / beta
+    vrp = float(np.sum(0.5 * m * wn ** 2 * (r - np.roll(r, 1, axis=0)) ** 2))
+    dr = r - r.mean(axis=0)[None]
+    tq1xyz = 0.5 * (dr * g1).sum(axis=(0, 1)) / nb
+    tq2xyz = 0.5 * (dr * g2).sum(axis=(0, 1)) / nb
+    tc = 0.5 * (3 * na - 3 - nctot) / beta
+    return dict(kinetic=tk, springs=vrp, potential=v, h_rp_per_atom=(nb * v + tk + vrp) / na,
+                temperature_K=3.157733e+5 * (2.0 * tk / ((3 * na - nctot - 3) * nb)) / nb,
+                tv=tc + tq1xyz.sum() + tq2xyz.sum(), tq1=tq1xyz.sum(), tq2=tq2xyz.sum(),
+                tv_x=tc / 3 + tq1xyz[0] + tq2xyz[0], tv_y=tc / 3 + tq1xyz[1] + tq2xyz[1],
+                tv_z=tc / 3 + tq1xyz[2] + tq2xyz[2])
+
+
+def test_estimators_match_reference(oracle, refpath):
+    """kinetic_rpmd and virial_ke of the reference's Fortran against the statement above (used as
+    the checker of the device estimators), flexible and with constraints; and the spring energy
+    against a closed form: beads on a circle of radius a (positions a (cos, sin)(2 pi k / nb)) have
+    |r_k - r_{k-1}| = 2 a sin(pi / nb)."""
+    rng = np.random.default_rng(12)
+    nb, na = 6, 27
+    mass = np.tile([29156.9471, 1837.1527, 1837.1527], na // 3)
+    r = rng.normal(size=(nb, na, 3))
+    p = rng.normal(size=(nb, na, 3)) * 20.0
+    g1, g2 = rng.normal(size=(nb, na, 3)), rng.normal(size=(nb, na, 3))
+    beta = 1.0 / (cf.KBOLTZ * 300.0)
+    for nctot in (0, na):
+        e = np_estimators(p, r, g1, g2, mass, beta, 0.0, nctot)
+        tv, tvxyz, tq1, tq2 = refpath.virial_ke(r, g1, g2, beta, nctot)
+        assert abs(e["tv"] - tv) <= 1e-13 * abs(tv) and abs(e["tq1"] - tq1) <= 1e-13 * abs(tq1)
+        assert abs(e["tq2"] - tq2) <= 1e-13 * abs(tq2)
+        assert np.allclose([e["tv_x"], e["tv_y"], e["tv_z"]], tvxyz, rtol=1e-13, atol=0)
+    refpath.set_nctot(0)
+    assert abs(e["kinetic"] - refpath.kinetic_rpmd(p, mass)) <= 1e-13 * e["kinetic"]
+    a, k = 0.7, np.arange(nb)
+    ring = np.zeros((nb, na, 3))
+    ring[:, :, 0] = a * np.cos(2 * np.pi * k / nb)[:, None]
+    ring[:, :, 1] = a * np.sin(2 * np.pi * k / nb)[:, None]
+    e = np_estimators(p, ring, g1, g2, mass, beta, 0.0)
+    closed = 0.5 * (nb / beta) ** 2 * nb * float(mass.sum()) * (2 * a * np.sin(np.pi / nb)) ** 2
+    assert abs(e["springs"] - closed) <= 1e-12 * closed
