@@ -135,6 +135,10 @@ struct rpmd_ctx {
     // host staging (pinned)
     double* h_stage = nullptr;
     size_t h_cap = 0;
+    // FP32 real-space pair-function tables: [0] physical box, [1] unit box (host copies kept for the
+    // upload)
+    PairTab ptab[2];
+    std::vector<float4> ptab_h[2];
     // cached k set
     KSet ks{};
     double ks_key[8] = {0};
@@ -316,6 +320,26 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
     c->ks = ks;
     memcpy(c->ks_key, key, sizeof key);
     c->ks_valid = true;
+    return 0;
+}
+
+// the FP32 pair-function table for (alpha, rc) in the units of box sb, built and uploaded when the
+// parameters change (never under stream capture: a captured step follows an eager one with the same
+// parameters).  The context's streams are drained first, as queued kernels may read the old table.
+static int pair_table_for(rpmd_ctx* c, const Box& sb, double rc, double alpha, hipStream_t s, const PairTab** out) {
+    const int unit = sb.L[0] == 1.0 && sb.L[1] == 1.0 && sb.L[2] == 1.0;
+    const double sr = unit ? 0x1p-32 : 1.0;
+    PairTab& t = c->ptab[unit];
+    *out = &t;
+    if (t.d && t.key[0] == alpha && t.key[1] == rc && t.key[2] == sr) return 0;
+    const size_t bytes = pair_table_bytes();
+    if (!t.d) HIPCHK(hipMalloc((void**)&t.d, bytes));
+    for (hipStream_t q : {s, c->st, c->st2, c->st3}) HIPCHK(hipStreamSynchronize(q));
+    std::vector<float4>& h = c->ptab_h[unit];
+    h.resize(bytes / sizeof(float4));
+    pair_table_fill(alpha, rc, sr, t, h.data());
+    HIPCHK(hipMemcpyAsync(t.d, h.data(), bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 
@@ -557,6 +581,8 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         RETCHK(build_kset(c, box));
         const int cubic = P.iamcub;
         const Box sb = cubic ? unit_box() : bx;
+        const PairTab* ptab = nullptr;
+        if (mixed) RETCHK(pair_table_for(c, sb, P.wrcut, P.walpha, s, &ptab));
         CellGrid g = make_grid(sb, na);
         if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
         // list skin of 1 A (src/main.f90 uses none: the reference rebuilds its cells every call)
@@ -592,7 +618,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr, fork ? EW_BPC : 0);
+                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr, fork ? EW_BPC : 0, ptab);
             else
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
         }
@@ -1535,6 +1561,8 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
                     c->step_ctr, c->ev_step, c->goo, c->com_parts2, c->pstep_parts};
     for (void* b : bufs)
         if (b) hipFree(b);
+    for (auto& t : c->ptab)
+        if (t.d) hipFree(t.d);
     for (auto& kv : c->nm) {
         hipFree(kv.second.F);
         hipFree(kv.second.B);

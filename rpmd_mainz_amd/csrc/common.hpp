@@ -155,7 +155,29 @@ struct VList {
     int* nreb;       // [1] builds so far: the sites are sorted again on every VL_RESORT-th
     int cap;
 };
-// site blocks of 256 / VLS sites per bead, padded to a multiple of 8 (the XCD mapping of k_ewald_vl)
+// FP32 real-space pair functions du(s) = erfc(a r)/r and dur(s) = (sfac e^{-a^2 r^2} - du)/r^2 of
+// s = r^2 in the kernel's distance units, tabulated over the scaled abscissa s' = c s, c = 2^14 / rc^2,
+// so that the cut-off is s' = 2^14 and the table's PT_OCT octaves run from s' = 2 (biased FP32
+// exponents 128 .. 140: their low 4 bits and the top PT_M mantissa bits are the interval, taken by a
+// mask), 2^PT_M cubic intervals per octave in the remaining mantissa bits, then one interval of zeros
+// at s' = 2^14 (every pair at or past the cut-off, after clamping).  The table reaches down to r
+// between rc/91 and rc/64; closer pairs are evaluated without it.
+#define PT_M 5
+#define PT_OCT 13
+#define PT_N (PT_OCT * (1 << PT_M) + 1)
+static_assert(PT_OCT < 15, "the interval index holds 4 exponent bits");
+struct PairTab {
+    float4* d = nullptr;  // [2][PT_N] device: cubic coefficients of du (x 2^ku), then of dur (x 2^kr),
+                          // in w = the low 23 - PT_M mantissa bits of s'
+    float c = 0.0f;       // s' = c s
+    int ku = 0, kr = 0;   // power-of-two scales of the two tables
+    double key[3] = {0.0, 0.0, 0.0};  // (alpha, rc, distance unit) it was built for
+};
+// fill t (host part) and h with the table for Ewald parameter alpha and cut-off rc, distances in
+// units of sr (the unit box: 2^-32)
+void pair_table_fill(double alpha, double rc, double sr, PairTab& t, float4* h);
+size_t pair_table_bytes();
+// site blocks of 256 / VLS sites per bead, padded to a multiple of 8 (the XCD mapping of the real-space pair kernels)
 __host__ __device__ inline int ewald_real_parts(int na) { return ((na * VLS + 255) / 256 + 7) / 8 * 8; }
 int ewald_nclu(int na);
 int ewald_real_slots();
@@ -169,9 +191,10 @@ void launch_phases(const double* pos, int na, int nb, double alpha, int unit, Bo
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, hipStream_t s);
 // bpc > 0: FP32 kernel capped at bpc blocks per CU (for a launch beside other streams' kernels)
+// FP32 (T = float) reads the pair-function table tab, built for (alpha, rcut, sb)
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
-                       hipStream_t s, int bpc = 0);
+                       hipStream_t s, int bpc = 0, const PairTab* tab = nullptr);
 int kspace_nchunk(int na);
 int kspace_ngroups(int kmax);
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,

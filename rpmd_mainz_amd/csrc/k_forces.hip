@@ -6,7 +6,10 @@
 // reduced per block into parts[block][10] and summed in a fixed order by k_reduce_parts.
 #include "common.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
+#include <vector>
 
 namespace rpmd {
 
@@ -290,7 +293,8 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
 // kernel gives VLS lanes to a sorted site, which walk its list: every pair is evaluated from both
 // ends (energy and virial with a factor 1/2) and each site's gradient is summed in a fixed order, so
 // results are deterministic.  T is the pair-body type; in FP32 the distances come from 32-bit
-// fixed-point fractional coordinates and the sums are flushed into FP64 every 64 pairs of a lane.
+// fixed-point fractional coordinates, the pair functions from a table (PairTab), and the sums are
+// flushed into FP64 every 64 pairs of a lane.
 // ------------------------------------------------------------------------------------------------
 // FP64: du = erfc_AS(alpha r)/r (erfc - 1 within a molecule), dur = (sfac e^{-x^2} - du)/r^2, the
 // reference's expressions (src/ewald_cubic.f90:126-140) with the two divisions by r and r^2 taken as
@@ -304,22 +308,6 @@ __device__ __forceinline__ void erfc_pair(T dr2, T alpha, T sfac, bool same, T& 
     T t = (T)AS_P / ((T)AS_P + x);
     T erfc_ = e * t * ((T)AS_A1 + t * ((T)AS_A2 + t * ((T)AS_A3 + t * ((T)AS_A4 + t * (T)AS_A5))));
     if (same) erfc_ = erfc_ - (T)1;
-    du = erfc_ * rinv;
-    dur = (sfac * e - du) * (rinv * rinv);
-}
-// FP32: alpha arrives multiplied by sqrt(log2 e), so e^{-(alpha r)^2} is one v_exp_f32 (2^y) of
-// -x^2, and t = p / (p + alpha r) = p' / (p' + x) with p' = p sqrt(log2 e)
-constexpr double SQRT_LOG2E = 1.2011224087864498;
-template <>
-__device__ __forceinline__ void erfc_pair<float>(float dr2, float alpha, float sfac, bool same, float& du, float& dur) {
-    constexpr float P2 = (float)(AS_P * SQRT_LOG2E);
-    float rinv = __frsqrt_rn(dr2);
-    float dr = dr2 * rinv;
-    float x = alpha * dr;
-    float e = __builtin_amdgcn_exp2f(-x * x);
-    float t = __builtin_amdgcn_rcpf(P2 + x) * P2;  // v_rcp_f32 (1 ulp)
-    float erfc_ = e * t * ((float)AS_A1 + t * ((float)AS_A2 + t * ((float)AS_A3 + t * ((float)AS_A4 + t * (float)AS_A5))));
-    if (same) erfc_ = erfc_ - 1.0f;
     du = erfc_ * rinv;
     dur = (sfac * e - du) * (rinv * rinv);
 }
@@ -775,34 +763,69 @@ __device__ __forceinline__ void vl_pair(T dx, T dy, T dz, T dr2, T qij, bool sam
     tv[6] += dz * fzz;
 }
 
-// FP32 pair body on fixed-point coordinates (the erfc of erfc_pair<float> with p' folded into the
-// polynomial: t' = 1 / (p' + x), erfc = e t' sum_k A_k p'^k t'^(k-1)); na2 = -alpha'^2, so
-// e^{-(alpha r)^2} = 2^{na2 r^2}.  Gradient and the 7 sums as vl_pair.
-constexpr double AS_P2 = AS_P * SQRT_LOG2E;
-__device__ __forceinline__ void vl_pair_fx(float dx, float dy, float dz, float r2, float qij, bool same, float ta,
-                                           float na2, float tsf, float (&tg)[3], float (&tv)[7]) {
-    constexpr float P2 = (float)AS_P2, B1 = (float)(AS_A1 * AS_P2), B2 = (float)(AS_A2 * AS_P2 * AS_P2),
-                    B3 = (float)(AS_A3 * AS_P2 * AS_P2 * AS_P2), B4 = (float)(AS_A4 * AS_P2 * AS_P2 * AS_P2 * AS_P2),
-                    B5 = (float)(AS_A5 * AS_P2 * AS_P2 * AS_P2 * AS_P2 * AS_P2);
-    const float rinv = __frsqrt_rn(r2);
-    const float e = __builtin_amdgcn_exp2f(r2 * na2);
-    const float t = __builtin_amdgcn_rcpf(fmaf(r2 * rinv, ta, P2));  // v_rcp_f32 (1 ulp)
-    float h = e * t * (B1 + t * (B2 + t * (B3 + t * (B4 + t * B5))));
-    if (same) h = h - 1.0f;
-    const float du = h * rinv;
-    const float dur = (tsf * e - du) * (rinv * rinv);
-    const float dv = qij * du, dvr = qij * dur;
+// FP32 pair sums: gradient and the 7 sums as vl_pair, from du and dur, without the site's own charge
+// (a factor of all of a site's sums, applied when they are flushed)
+__device__ __forceinline__ void pair_add(float dx, float dy, float dz, float qj, float du, float dur, float (&tg)[3],
+                                         float (&tv)[7]) {
+    const float dvr = qj * dur;
     const float fxx = dvr * dx, fyy = dvr * dy, fzz = dvr * dz;
     tg[0] += fxx;
     tg[1] += fyy;
     tg[2] += fzz;
-    tv[0] += dv;
+    tv[0] += qj * du;
     tv[1] += dx * fxx;
     tv[2] += dx * fyy;
     tv[3] += dx * fzz;
     tv[4] += dy * fyy;
     tv[5] += dy * fzz;
     tv[6] += dz * fzz;
+}
+// FP32 pair body on fixed-point coordinates: du and dur (erfc, in the tables' scales 2^ku, 2^kr) from
+// the pair-function table in LDS at the scaled abscissa sp = c r^2 (PairTab: clamped to the table, the
+// interval is the exponent and top PT_M mantissa bits, the abscissa w the remaining mantissa bits,
+// exact in FP32; at and past the cut-off the zero interval), no transcendental and no cut-off test.
+// Own-molecule pairs (erfc - 1) and pairs below the table are corrected after the walk
+// (ewald_vl_body).
+__device__ __forceinline__ void vl_pair_tab(float dx, float dy, float dz, float sp, float qj,
+                                            const float4* __restrict__ tl, float (&tg)[3], float (&tv)[7]) {
+    const uint32_t bits = __float_as_uint(__builtin_amdgcn_fmed3f(sp, 2.0f, 0x1p14f));
+    const float w = (float)(bits & ((1u << (23 - PT_M)) - 1u));
+    // byte offset of the interval's entries (16 bytes each, du's then dur's array): the low 4
+    // exponent bits and the top PT_M mantissa bits, by a mask (known non-negative, so the arrays'
+    // bases go into the LDS instructions' offsets); the 16-byte stride spreads a wave's reads over
+    // all LDS banks
+    const uint32_t off = (bits >> (23 - PT_M - 4)) & (((16u << PT_M) - 1u) << 4);
+    const char* tb = (const char*)tl;
+    const float4 A = *(const float4*)(tb + off), B = *(const float4*)(tb + PT_N * 16 + off);
+    const float du = fmaf(fmaf(fmaf(A.w, w, A.z), w, A.y), w, A.x);
+    const float dur = fmaf(fmaf(fmaf(B.w, w, B.z), w, B.y), w, B.x);
+    pair_add(dx, dy, dz, qj, du, dur, tg, tv);
+}
+// the A&S form itself in FP32 (erfc, or erfc - 1 for an own-molecule pair), in the tables' scales;
+// ta = alpha, tsf = sfac in the kernel's distance units
+__device__ __forceinline__ void erfc_pair_scaled(float r2, float ta, float tsf, bool same, float cu, float cr,
+                                                 float& du, float& dur) {
+    const float ri = __frsqrt_rn(r2);
+    const float x = ta * (r2 * ri);
+    const float ex = __expf(-x * x);
+    const float t = __builtin_amdgcn_rcpf((float)AS_P + x) * (float)AS_P;
+    float h = ex * t * ((float)AS_A1 + t * ((float)AS_A2 + t * ((float)AS_A3 + t * ((float)AS_A4 + t * (float)AS_A5))));
+    if (same) h -= 1.0f;
+    const float d = h * ri;
+    du = d * cu;
+    dur = ((tsf * ex - d) * (ri * ri)) * cr;
+}
+
+// sum over the four lanes of a quad (lanes 4i..4i+3), every lane gets the same bits
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // lanes ^ 1
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // lanes ^ 2
+    return v;
+}
+__device__ __forceinline__ double quad_sum(double v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    return v;
 }
 
 // VLS lanes per sorted site, lane s walking entries s, s + VLS, ... of the site's list (one list for
@@ -817,25 +840,40 @@ __device__ __forceinline__ void vl_pair_fx(float dx, float dy, float dz, float r
 // exactly up to rounding; for the unit box they stay in units of 2^-32 (the constants and the final
 // sums are scaled by powers of two instead), else they are scaled by L 2^-32 per component.  The pad
 // site has odd coordinates and the others even ones, so no distance is zero, and its pairs add zero.
+#define EWVL_PARAMS \
+    const double4* __restrict__ srt, const uint4* __restrict__ srf, const int* __restrict__ perm, \
+    const int* __restrict__ cnt, const int* __restrict__ nex, const int* __restrict__ nl, int na, int nb, int cap, \
+    double L0, double L1, double L2, double i0, double i1, double i2, double rc2, double alpha, double sfac, \
+    double* __restrict__ grad, double* __restrict__ parts, unsigned long long* __restrict__ dmax, \
+    const float4* __restrict__ tab, float tsc, int tku, int tkr
+#define EWVL_ARGS                                                                                                  \
+    srt, srf, perm, cnt, nex, nl, na, nb, cap, L0, L1, L2, i0, i1, i2, rc2, alpha, sfac, grad, parts, dmax, tab, tsc, \
+        tku, tkr
 template <typename T, bool UNIT>
-__global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ srt, const uint4* __restrict__ srf,
-                                                  const int* __restrict__ perm, const int* __restrict__ cnt,
-                                                  const int* __restrict__ nex, const int* __restrict__ nl, int na,
-                                                  int nb, int cap, double L0, double L1, double L2, double i0,
-                                                  double i1, double i2, double rc2, double alpha, double sfac,
-                                                  double* __restrict__ grad, double* __restrict__ parts,
-                                                  unsigned long long* __restrict__ dmax) {
+__device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, double4, uint4>::type P4;
     const int id = blockIdx.x, m8 = id >> 3;
     const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
     const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
     if (id == 0 && threadIdx.x == 0) ctrl_store(dmax, 0ull);
-    // the FP64 energy and virial sums of each thread live in LDS (fewer registers, more waves)
-    __shared__ double vacc[7][256];
+    // LDS: the FP64 sums of each site (energy, virial, gradient; fewer registers, more waves), added
+    // to by the site's first lane, then (FP32) the pair-function table, copied in by the block, at
+    // a fixed address that the LDS instructions take as offset.
+    struct Lds {
+        double vacc[10][256 / VLS];
+        float4 tl[F64 ? 1 : 2 * PT_N];
+    };
+    __shared__ Lds lds;
+    double(&vacc)[10][256 / VLS] = lds.vacc;
+    const float4* tl = lds.tl;
+    if (sl == 0)
 #pragma unroll
-    for (int c = 0; c < 7; c++) vacc[c][threadIdx.x] = 0.0;
-    double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int c = 0; c < 10; c++) vacc[c][threadIdx.x / VLS] = 0.0;
+    if constexpr (!F64) {
+        for (int i = threadIdx.x; i < 2 * PT_N; i += blockDim.x) lds.tl[i] = tab[i];
+        __syncthreads();
+    }
     // FP32 unit box: distances in units of 2^-32, gradient x 2^64, energy and virial x 2^32 at the end
     constexpr double SR = (!F64 && UNIT) ? 0x1p-32 : 1.0;
     // FP32: two register sets of coordinates (a group in flight); FP64 (twice the registers): one
@@ -852,7 +890,9 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         const uint32_t l0 = (uint32_t)(k * VLS + sl) * 16u, ls = (uint32_t)na * VLS * 16u;
         const int ntot = cnt[k], ne = nex[k];
         const int nn = (ntot - sl + VLS - 1) / VLS;  // entries of this lane
-        const int ng = (nn + 3) / 4;                 // its groups
+        // groups: those of the site's first lane, for all its lanes (past a lane's entries the groups
+        // hold the pad site), so the site's lanes stay together for the sums over a quad
+        const int ng = ((ntot + VLS - 1) / VLS + 3) / 4;
         const P4 xi = X[k];
         T qi;
         if constexpr (F64)
@@ -863,10 +903,11 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         const float Ls0 = (float)(L0 * 0x1p-32), Ls1 = (float)(L1 * 0x1p-32), Ls2 = (float)(L2 * 0x1p-32);
         const T trc2 = (T)(rc2 / (SR * SR));
         const T ta = (T)(alpha * SR), tsf = (T)(sfac * SR);
-        const float na2 = (float)(-(alpha * SR) * (alpha * SR));
+        const float cu = __builtin_ldexpf(1.0f, tku), cr = __builtin_ldexpf(1.0f, tkr);
         T tg[3] = {0, 0, 0}, tv[7];
 #pragma unroll
         for (int c = 0; c < 7; c++) tv[c] = 0;
+        float lo = 3.0e38f;  // FP32: the lane's smallest scaled r^2
         // past the lane's last group: group cap / VLG, which holds the pad site (no branch around the
         // load)
         const int pad = cap / VLG;
@@ -877,6 +918,21 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
         auto at = [&](uint32_t j) { return *(const P4*)((const char*)X + j * (uint32_t)sizeof(P4)); };
         // a site has at most two own-molecule partners (three sites per molecule), listed first, so
         // only a lane's first entry can be one: may_same is false for every other call
+        // FP32: coordinate difference (as float in the kernel's units), r^2, the table's abscissa
+        // c r^2 and the charge of the site with list entry cur
+        auto diff32 = [&](const P4& cur, float& ex, float& ey, float& ez, float& r2, float& sp, float& qj) {
+            ex = (float)(int)(xi.x - cur.x);
+            ey = (float)(int)(xi.y - cur.y);
+            ez = (float)(int)(xi.z - cur.z);
+            if (!UNIT) {
+                ex *= Ls0;
+                ey *= Ls1;
+                ez *= Ls2;
+            }
+            r2 = ex * ex + ey * ey + ez * ez;
+            sp = tsc * r2;
+            qj = __uint_as_float(cur.w);
+        };
         auto pair = [&](const P4& cur, int r, bool may_same) {
             if constexpr (F64) {
                 T ex = xi.x - cur.x, ey = xi.y - cur.y, ez = xi.z - cur.z;
@@ -896,28 +952,24 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 vl_pair<T>(ex, ey, ez, in ? r2 : (T)1, in ? qi * (T)cur.w : (T)0, may_same && r * VLS + sl < ne,
                            ta, tsf, tg, tv);
             } else {
-                float ex = (float)(int)(xi.x - cur.x), ey = (float)(int)(xi.y - cur.y),
-                      ez = (float)(int)(xi.z - cur.z);
-                if (!UNIT) {
-                    ex *= Ls0;
-                    ey *= Ls1;
-                    ez *= Ls2;
-                }
-                const float r2 = ex * ex + ey * ey + ez * ez;
-                const float qij = r2 < trc2 ? qi * __uint_as_float(cur.w) : 0.0f;
-                vl_pair_fx(ex, ey, ez, r2, qij, may_same && sl < ne, ta, na2, tsf, tg, tv);
+                float ex, ey, ez, r2, sp, qj;
+                diff32(cur, ex, ey, ez, r2, sp, qj);
+                lo = fminf(lo, sp);
+                vl_pair_tab(ex, ey, ez, sp, qj, tl, tg, tv);
             }
         };
+        // the lanes' sums of a site in a fixed order (a quad's butterfly), added to its FP64 sums
+        // (FP32: times the site's charge)
+        const double qf = F64 ? 1.0 : (double)qi;
         auto flush = [&]() {
-            gx += (double)tg[0];
-            gy += (double)tg[1];
-            gz += (double)tg[2];
-            tg[0] = tg[1] = tg[2] = 0;
 #pragma unroll
-            for (int c = 0; c < 7; c++) {
-                vacc[c][threadIdx.x] += (double)tv[c];
-                tv[c] = 0;
+            for (int c = 0; c < 10; c++) {
+                const T t = quad_sum(c < 7 ? tv[c] : tg[c - 7]);
+                if (sl == 0) vacc[c][threadIdx.x / VLS] += (double)t * qf;
             }
+#pragma unroll
+            for (int c = 0; c < 7; c++) tv[c] = 0;
+            tg[0] = tg[1] = tg[2] = 0;
         };
         // groups g and g + 1 per trip, also when g + 1 is past the lane's end (the pad group then, whose
         // pairs add zero): with no exit between a load and its use the loads stay where they are
@@ -958,32 +1010,64 @@ __global__ __launch_bounds__(256) void k_ewald_vl(const double4* __restrict__ sr
                 if (!F64 && (g & 3) == 3) flush();
             }
         }
+        if constexpr (!F64) {
+            // the lane's first entry, when it is of the site's own molecule: the -1 of erfc - 1
+            // (du - 1/r, dur + 1/r^3)
+            if (sl < ne) {
+                float ex, ey, ez, r2, sp, qj;
+                diff32(at(ent(0).x), ex, ey, ez, r2, sp, qj);
+                const float ri = __frsqrt_rn(r2);
+                pair_add(ex, ey, ez, sp < 0x1p14f ? qj : 0.0f, -cu * ri, cr * (ri * ri) * ri, tg, tv);
+            }
+            // pairs closer than the table's first node (no physical configuration has them): the A&S
+            // form in place of the table's value there, which the walk added
+            if (lo < 2.0f) {
+                const float du0 = tl[0].x, dur0 = tl[PT_N].x;
+                for (int g = 0; g < ng; g++) {
+                    const uint4 ie = ent(g);
+                    const uint32_t js[4] = {ie.x, ie.y, ie.z, ie.w};
+                    for (int q = 0; q < 4; q++) {
+                        float ex, ey, ez, r2, sp, qj, du, dur;
+                        diff32(at(js[q]), ex, ey, ez, r2, sp, qj);
+                        if (sp < 2.0f) {
+                            erfc_pair_scaled(r2, ta, tsf, false, cu, cr, du, dur);
+                            pair_add(ex, ey, ez, qj, du - du0, dur - dur0, tg, tv);
+                        }
+                    }
+                }
+            }
+        }
         flush();
-        constexpr double GS = 1.0 / (SR * SR);
-        gx *= GS;
-        gy *= GS;
-        gz *= GS;
     }
+    // FP32: gradient and virial in the dur table's scale, energy in the du table's
+    const double GS = F64 ? 1.0 : ldexp(1.0 / (SR * SR), -tkr);
     double acc[7];
     constexpr double VS = 0.5 / SR;
+    const double VU = F64 ? VS : ldexp(VS, -tku), VR = F64 ? VS : ldexp(VS, -tkr);
+    const bool own = k < na && sl == 0;
 #pragma unroll
-    for (int c = 0; c < 7; c++) acc[c] = vacc[c][threadIdx.x] * VS;
-#pragma unroll
-    for (int off = 1; off < VLS; off <<= 1) {
-        gx += __shfl_xor(gx, off, 64);
-        gy += __shfl_xor(gy, off, 64);
-        gz += __shfl_xor(gz, off, 64);
-    }
-    if (k < na && sl == 0) {
-        const int a = perm[k];
+    for (int c = 0; c < 7; c++) acc[c] = own ? vacc[c][threadIdx.x / VLS] * (c == 0 ? VU : VR) : 0.0;
+    if (own) {
+        const int a = perm[k], sk = threadIdx.x / VLS;
         double* G = grad + (size_t)b * 3 * na;
-        G[a] = gx;
-        G[na + a] = gy;
-        G[2 * na + a] = gz;
+        G[a] = vacc[7][sk] * GS;
+        G[na + a] = vacc[8][sk] * GS;
+        G[2 * na + a] = vacc[9][sk] * GS;
     }
     // {v, vir(3,3) column-major} from the symmetric sums
     double w[10] = {acc[0], acc[1], acc[2], acc[3], acc[2], acc[4], acc[5], acc[3], acc[5], acc[6]};
     block_reduce_store<10>(w, parts + ((size_t)b * ewald_real_parts(na) + sb) * 10);
+}
+
+// FP32: held to 64 VGPRs, so that 8 waves of it fit a SIMD; FP64 (twice the registers per value)
+// as the compiler chooses
+template <bool UNIT>
+__global__ __launch_bounds__(256, 8) void k_ewald_vl32(EWVL_PARAMS) {
+    ewald_vl_body<float, UNIT>(EWVL_ARGS);
+}
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_ewald_vl64(EWVL_PARAMS) {
+    ewald_vl_body<double, UNIT>(EWVL_ARGS);
 }
 
 int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
@@ -1095,26 +1179,97 @@ static size_t lds_for_blocks_per_cu(const void* fn, int bpc) {
     return total > fa.sharedSizeBytes ? total - fa.sharedSizeBytes : 0;
 }
 
+// Pair-function table (PairTab): on interval i of octave o (s' from 2^e (1 + j/2^PT_M), width
+// 2^(e - PT_M), e = 1 + o) each function is the cubic through its values at the four Chebyshev
+// nodes of the interval, in w = (s' - s'_i) 2^(23 - e) (the low mantissa bits of s'), computed in FP64
+// at s = s' / c from the A&S form of the reference (src/ewald_cubic.f90:126-140), then scaled by a
+// power of two that brings the largest coefficient near 1 and rounded to FP32.  c is the FP32 value
+// the kernel multiplies by.
+size_t pair_table_bytes() { return (size_t)PT_N * 2 * sizeof(float4); }
+void pair_table_fill(double alpha, double rc, double sr, PairTab& t, float4* h) {
+    const int per = 1 << PT_M, n = PT_N - 1;
+    const double a = alpha * sr, sfac = -2.0 * (alpha / sqrt(acos(-1.0))) * sr, s_rc = (rc / sr) * (rc / sr);
+    t.c = (float)(ldexp(1.0, 1 + PT_OCT) / s_rc);
+    const double ic = 1.0 / (double)t.c;
+    auto du_dur = [&](double s, double& du, double& dur) {
+        const double r = sqrt(s), x = a * r, e = exp(-x * x), tt = AS_P / (AS_P + x);
+        du = e * tt * (AS_A1 + tt * (AS_A2 + tt * (AS_A3 + tt * (AS_A4 + tt * AS_A5)))) / r;
+        dur = (sfac * e - du) / s;
+    };
+    // inverse of the Vandermonde matrix of the nodes v_k in [0, 1]
+    double v[4], inv[4][4];
+    for (int k = 0; k < 4; k++) v[k] = 0.5 * (1.0 - cos((2 * k + 1) * acos(-1.0) / 8.0));
+    for (int i = 0; i < 4; i++) {
+        // Lagrange basis polynomial i expanded in powers of v
+        double c[4] = {1.0, 0.0, 0.0, 0.0}, den = 1.0;
+        int deg = 0;
+        for (int j = 0; j < 4; j++) {
+            if (j == i) continue;
+            for (int m = deg + 1; m > 0; m--) c[m] = c[m - 1] - v[j] * c[m];
+            c[0] = -v[j] * c[0];
+            deg++;
+            den *= v[i] - v[j];
+        }
+        for (int m = 0; m < 4; m++) inv[m][i] = c[m] / den;
+    }
+    std::vector<double> cf((size_t)n * 8);
+    const double W = ldexp(1.0, -(23 - PT_M));  // v per unit of w
+    for (int i = 0; i < n; i++) {
+        const int o = i / per, j = i % per;
+        const double base = ldexp(1.0, 1 + o), s0 = base * (1.0 + (double)j / per), hw = base / per;
+        double f[2][4];
+        for (int k = 0; k < 4; k++) du_dur((s0 + v[k] * hw) * ic, f[0][k], f[1][k]);
+        for (int q = 0; q < 2; q++)
+            for (int m = 0; m < 4; m++) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; k++) acc += inv[m][k] * f[q][k];
+                cf[(size_t)i * 8 + q * 4 + m] = acc * pow(W, m);
+            }
+    }
+    double mx[2] = {0.0, 0.0};
+    for (int i = 0; i < n; i++)
+        for (int q = 0; q < 2; q++) mx[q] = fmax(mx[q], fabs(cf[(size_t)i * 8 + q * 4]));
+    t.ku = -ilogb(mx[0]);
+    t.kr = -ilogb(mx[1]);
+    for (int i = 0; i < n; i++) {
+        const double* c = &cf[(size_t)i * 8];
+        h[i] = make_float4((float)ldexp(c[0], t.ku), (float)ldexp(c[1], t.ku), (float)ldexp(c[2], t.ku),
+                           (float)ldexp(c[3], t.ku));
+        h[PT_N + i] = make_float4((float)ldexp(c[4], t.kr), (float)ldexp(c[5], t.kr), (float)ldexp(c[6], t.kr),
+                                  (float)ldexp(c[7], t.kr));
+    }
+    h[n] = h[PT_N + n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // at and past the cut-off
+    t.key[0] = alpha;
+    t.key[1] = rc;
+    t.key[2] = sr;
+}
+
 // bpc > 0: at most bpc blocks per CU (FP32 pair kernel, which otherwise fills every CU slot), so that
 // kernels on another stream keep CU slots beside it
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
-                       hipStream_t s, int bpc) {
+                       hipStream_t s, int bpc, const PairTab* tab) {
     const double sfac = -2.0 * (alpha / sqrt(acos(-1.0)));
-    const double ak = sizeof(T) == 4 ? alpha * SQRT_LOG2E : alpha;  // see erfc_pair<float>
     const bool unit = sb.L[0] == 1.0 && sb.L[1] == 1.0 && sb.L[2] == 1.0;
-    auto k = unit ? k_ewald_vl<T, true> : k_ewald_vl<T, false>;
+    void (*k)(EWVL_PARAMS);
+    if constexpr (sizeof(T) == 4)
+        k = unit ? k_ewald_vl32<true> : k_ewald_vl32<false>;
+    else
+        k = unit ? k_ewald_vl64<true> : k_ewald_vl64<false>;
     static size_t dyn[2][2] = {{~(size_t)0, ~(size_t)0}, {~(size_t)0, ~(size_t)0}};  // [unit][capped]
     size_t& d = dyn[unit][bpc > 0];
     if (d == ~(size_t)0) d = sizeof(T) == 4 ? lds_for_blocks_per_cu((const void*)k, bpc) : 0;
-    k<<<ewald_real_parts(na) * nb, 256, d, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap, sb.L[0],
-                                                sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, ak,
-                                                sfac, grad, parts, v.dmax);
+    PairTab none{};
+    const PairTab& t = tab ? *tab : none;
+    const size_t dl = sizeof(T) == 4 ? d : 0;
+    k<<<ewald_real_parts(na) * nb, 256, dl, s>>>(v.srt, v.srf, v.perm, v.cnt, v.nex, v.nl, na, nb, v.cap, sb.L[0],
+                                                 sb.L[1], sb.L[2], sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut,
+                                                 alpha, sfac, grad, parts, v.dmax, t.d, t.c, t.ku, t.kr);
 }
 template void launch_ewald_real<double>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t,
-                                        int);
+                                        int, const PairTab*);
 template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t,
-                                       int);
+                                       int, const PairTab*);
 
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
