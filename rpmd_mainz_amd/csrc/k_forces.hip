@@ -862,6 +862,7 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
     // a fixed address that the LDS instructions take as offset.
     struct Lds {
         double vacc[10][256 / VLS];
+        float qs[256 / VLS];  // FP32: the site's charge, for the end (no register held through the walk)
         float4 tl[F64 ? 1 : 2 * PT_N];
     };
     __shared__ Lds lds;
@@ -895,10 +896,12 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
         const int ng = ((ntot + VLS - 1) / VLS + 3) / 4;
         const P4 xi = X[k];
         T qi;
-        if constexpr (F64)
+        if constexpr (F64) {
             qi = xi.w;
-        else
+        } else {
             qi = __uint_as_float(xi.w);
+            if (sl == 0) lds.qs[threadIdx.x / VLS] = qi;
+        }
         const T Lt0 = (T)L0, Lt1 = (T)L1, Lt2 = (T)L2, it0 = (T)i0, it1 = (T)i1, it2 = (T)i2;
         const float Ls0 = (float)(L0 * 0x1p-32), Ls1 = (float)(L1 * 0x1p-32), Ls2 = (float)(L2 * 0x1p-32);
         const T trc2 = (T)(rc2 / (SR * SR));
@@ -959,13 +962,11 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
             }
         };
         // the lanes' sums of a site in a fixed order (a quad's butterfly), added to its FP64 sums
-        // (FP32: times the site's charge)
-        const double qf = F64 ? 1.0 : (double)qi;
         auto flush = [&]() {
 #pragma unroll
             for (int c = 0; c < 10; c++) {
                 const T t = quad_sum(c < 7 ? tv[c] : tg[c - 7]);
-                if (sl == 0) vacc[c][threadIdx.x / VLS] += (double)t * qf;
+                if (sl == 0) vacc[c][threadIdx.x / VLS] += (double)t;
             }
 #pragma unroll
             for (int c = 0; c < 7; c++) tv[c] = 0;
@@ -1039,12 +1040,14 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
         }
         flush();
     }
-    // FP32: gradient and virial in the dur table's scale, energy in the du table's
-    const double GS = F64 ? 1.0 : ldexp(1.0 / (SR * SR), -tkr);
+    // FP32: gradient and virial in the dur table's scale, energy in the du table's, all without the
+    // site's charge (kept in LDS through the walk)
+    const bool own = k < na && sl == 0;
+    const double qs = F64 || !own ? 1.0 : (double)lds.qs[threadIdx.x / VLS];
+    const double GS = F64 ? 1.0 : ldexp(1.0 / (SR * SR), -tkr) * qs;
     double acc[7];
     constexpr double VS = 0.5 / SR;
-    const double VU = F64 ? VS : ldexp(VS, -tku), VR = F64 ? VS : ldexp(VS, -tkr);
-    const bool own = k < na && sl == 0;
+    const double VU = F64 ? VS : ldexp(VS, -tku) * qs, VR = F64 ? VS : ldexp(VS, -tkr) * qs;
 #pragma unroll
     for (int c = 0; c < 7; c++) acc[c] = own ? vacc[c][threadIdx.x / VLS] * (c == 0 ? VU : VR) : 0.0;
     if (own) {
