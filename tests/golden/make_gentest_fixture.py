@@ -9,6 +9,11 @@ the BOX lines, and the starting lattice vmd_start.xyz.  The namelist values of t
 Frames: the file holds 2 trajectories x 12 frames, one frame every pb = 20 NVE steps
 (src/md_dynamics.f90:499-548).  Frames 0,1 and 12,13 are consecutive pairs inside one
 trajectory (20 steps apart); 11 is the last frame of trajectory 1.
+
+gentest_traj.npz: positions (Angstrom) and forces of all 24 frames of both beads, with the BOX
+lines.  gentest_text/: the reference's own text of the restart file eq.xyz (print_structure,
+src/io.f90:1-17) and of the first frame of each bead's position and force dump (print_vmd_bead,
+print_vmd_bead_forces, src/io.f90:549-700), for the readers' format tests.
 """
 import json
 import os
@@ -73,7 +78,27 @@ def main():
     ns = int(rows[0].split()[0])
     A = np.array([[float(x) for x in rows[1 + i].split()] for i in range(ns + 1)])
     np.savetxt(os.path.join(OUT, "gle_A_ns4.txt"), A, fmt="%.9e")
-    print("wrote", os.path.join(OUT, "gentest_frames.npz"))
+    allf = {}
+    for kind in ("xyz", "frc"):
+        per_bead = []
+        for b in (1, 2):
+            fr, box = read_frames(os.path.join(REF, f"vmd_bead-{b}.{kind}"))
+            per_bead.append(fr)
+            if kind == "xyz" and b == 1:
+                allf["box_A"] = box
+        allf[kind] = np.stack(per_bead, axis=1)  # (24, bead, atom, 3)
+    np.savez_compressed(os.path.join(OUT, "gentest_traj.npz"), **allf)
+    txt = os.path.join(OUT, "gentest_text")
+    os.makedirs(txt, exist_ok=True)
+    with open(os.path.join(REF, "eq.xyz")) as f, open(os.path.join(txt, "eq.xyz"), "w") as g:
+        g.write(f.read())
+    for b in (1, 2):
+        for kind in ("xyz", "frc"):
+            lines = open(os.path.join(REF, f"vmd_bead-{b}.{kind}")).read().split("\n")
+            n = int(lines[0].split()[0])
+            with open(os.path.join(txt, f"vmd_bead-{b}.{kind}.frame0"), "w") as g:
+                g.write("\n".join(lines[:n + 2]) + "\n")
+    print("wrote", os.path.join(OUT, "gentest_frames.npz"), os.path.join(OUT, "gentest_traj.npz"), txt)
 
 
 if __name__ == "__main__":
