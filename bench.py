@@ -179,17 +179,21 @@ def replicas(nrep: int, steps: int, seed: int = 2024):
         ctxs.append(c)
         sum_nb += nb
     batch = capi.ReplicaBatch(ctxs)
-    batch.run_steps(3)   # one eager step, the graph of two captured and replayed: none of it timed
+    # untimed: one eager step, the graphs captured, the autotuner's four steps of each form, one
+    # replay of the kept graph
+    batch.run_steps(11)
     t0 = time.perf_counter()
     batch.run_steps(steps)
     dt = time.perf_counter() - t0
+    tuning = batch.tuning()
     batch.close()
     for c in ctxs:
         c.close()
     return dict(replicas=nrep, beads_total=sum_nb, steps=steps, ms_per_step=dt * 1e3 / steps,
-                bead_force_evals_s=steps * sum_nb / dt, dtype="f32-pair/f64-acc",
+                bead_force_evals_s=steps * sum_nb / dt, dtype="f32-pair/f64-acc", tuning=tuning,
                 workload="config 3: q-SPC/Fw replicas of 128-1024 molecules, 8-64 beads, rho 0.95-1.05, "
-                         "250-350 K, own cubic boxes (kmax 9-13), PILE-L, one batch hipGraph per 2 steps (captured in the warm-up)")
+                         "250-350 K, own cubic boxes (kmax 9-13), PILE-L, one batch hipGraph per 2 steps (captured in the warm-up); "
+                         "batched force launches or the replicas' own steps, whichever the autotuner timed faster")
 
 
 def _ref_forces_worker(args):

@@ -229,12 +229,24 @@ int rpmd_ctx_set_streams(rpmd_ctx* ctx, int streams);
 
 /* Replica batches (no counterpart in the reference, which runs trajectories one after another):
  * contexts with uploaded states (each with its own na, nb, box, Ewald set-up, temperature,
- * thermostat; no RP contraction) advanced together, nsteps steps each, by one hipGraph whose
- * branches are the replicas.  Re-create the batch after a context's state is uploaded again. */
+ * thermostat; no RP contraction) advanced together, nsteps steps each, by one hipGraph.  When
+ * every replica takes the fused PILE-L step in mixed precision (cubic box, mts 1, up to 64 beads),
+ * the step can be batched: the real-space pair walk, k_sk_h16, k_sk_final and k_kforce_h16 of all
+ * replicas as one launch each, the rest on the replicas' streams; otherwise, and when a batch
+ * autotuner found it faster, each replica takes its own step side by side.  The autotuner runs
+ * once, in the first call of nine steps or more: four graph-replayed steps of each form (real
+ * steps: both forms give the same bits), the faster kept.  Re-create the batch after a context's
+ * state is uploaded again. */
 typedef struct rpmd_batch rpmd_batch;
 int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out);
 int rpmd_batch_run_steps(rpmd_batch* batch, int nsteps);
 int rpmd_batch_destroy(rpmd_batch* batch);
+/* The autotuners' choices (no reference counterpart): batched (1) or separate (0) steps in use,
+ * -1 before the first step; the timed pair of steps of each form (ms, 0 if not measured); for the
+ * batched form the pair walk's cap (blocks per CU, 0 none) and whether it runs beside the k-space
+ * kernels (1) or after them (0). */
+int rpmd_batch_tuning(const rpmd_batch* batch, int* batched, double* ms_batched, double* ms_separate, int* pair_cap,
+                      int* pair_beside);
 
 /* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
  * the context stream around the Ewald kernels of eager launches between begin and end.

@@ -69,7 +69,8 @@ def lib() -> C.CDLL:
                      "rpmd_state_upload", "rpmd_run_steps", "rpmd_state_download", "rpmd_use_graph",
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
-                     "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators"):
+                     "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators",
+                     "rpmd_batch_tuning"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -316,6 +317,15 @@ class ReplicaBatch:
 
     def run_steps(self, nsteps: int):
         _check(lib().rpmd_batch_run_steps(self._h, C.c_int(nsteps)))
+
+    def tuning(self) -> dict:
+        """the autotuners' choices: batched steps in use (1), separate (0) or none yet (-1), the timed
+        pair of steps of each form (ms), the pair walk's cap and whether it runs beside k space"""
+        bt, cap, two = C.c_int(), C.c_int(), C.c_int()
+        mf, ms = C.c_double(), C.c_double()
+        _check(lib().rpmd_batch_tuning(self._h, C.byref(bt), C.byref(mf), C.byref(ms), C.byref(cap), C.byref(two)))
+        return dict(batched=bt.value, ms_batched_2steps=mf.value, ms_separate_2steps=ms.value, pair_cap=cap.value,
+                    pair_beside=two.value)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -2015,6 +2015,13 @@ int rpmd_run_steps(rpmd_ctx* c, int nsteps) {
 // together: one hipGraph per batch holds two steps of every replica, each replica on its own
 // stream branch (fork from the batch stream, join back), so one graph launch advances all of them
 // and the replicas' kernels overlap on the GPU.
+// the state of one replica between the two halves of a batched step
+struct FusedRep {
+    ReduceJobs red;
+    int use_list = 0;
+    unsigned long long* oo_dmax = nullptr;
+};
+
 struct rpmd_batch {
     std::vector<rpmd_ctx*> ctx;
     hipStream_t st = nullptr;
@@ -2024,7 +2031,201 @@ struct rpmd_batch {
     // per replica at capture: the context's epoch and buffer roles
     std::vector<uint64_t> epoch;
     std::vector<Roles> roles;
+    // batched steps (every replica takes the fused PILE-L step in mixed precision): the four force
+    // kernels of all replicas as one launch each, on st (and st2 for the pair walk beside the k-space
+    // chain, if the autotuner chose that); ev_a per replica: its first half done
+    bool fused = false;
+    ForceBatch* fb = nullptr;
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_b = nullptr, ev_f2 = nullptr, ev_j2 = nullptr;
+    std::vector<hipEvent_t> ev_a;
+    std::vector<FusedRep> fr;
+    std::vector<FBItem> items;
+    // the autotuner's choice: cap of the pair walk (blocks per CU, 0 none), pair walk on st2
+    bool tuned = false;
+    int bpc = 0, two = 0;
+    // the batch autotuner's choice between the batched step and each replica's own step (the
+    // same bits either way), made once by timing graph replays of both
+    bool mode_tuned = false, mode_fused = true;
+    float ms_fused = 0.0f, ms_separate = 0.0f;
 };
+
+// a replica that the batched step takes: mixed precision, flexible molecules, PILE-L with one time
+// step and no contraction (the fused PILE step, up to 64 beads), cubic Ewald without smeared charges
+static bool fused_ok(const rpmd_ctx* c) {
+    const rpmd_params& P = c->prm;
+    return c->prec == RPMD_PREC_MIXED && !P.rigid && P.therm == RPMD_THERM_PRO && P.mts == 1 && P.nbdf1 <= 0 &&
+           P.nbdf2 <= 0 && P.iamcub && c->st_nb >= 1 && c->st_nb <= 64 && !(P.wm > 0.0 || P.wh > 0.0) && !c->prof &&
+           P.kmax >= 0 && P.kmax <= RPMD_KMAX_DEV;
+}
+
+// The first half of the replica's step on its stream, as step_dev's fused PILE step with one stream:
+// the PILE step, the intramolecular forces, then forces(iopt = 1) up to its four dominant kernels,
+// whose inputs go to it (forces_dev's single-stream sequence: O-O centroid list, k set, pair
+// table, real-space list, sites and phases, list upkeep).
+static int fused_first_half(rpmd_ctx* c, FusedRep& fr, FBItem& it) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nb = c->st_nb;
+    hipStream_t s = c->st;
+    const double dt = c->st_dt, halfdt = 0.5 * dt, halfdtsmall = 0.5 * dt;  // mts = 1
+    const double* box = c->st_box.L;
+    NMMat* nmm = get_nm(c, nb, 0);
+    if (!nmm) return RPMD_ERR_NOMEM;
+    NoiseCursor nz{nullptr, 0, 0, 0};
+    const size_t draws = (size_t)3 * na * nb;
+    const double* n1 = nz.take(draws);
+    const double* n2 = nz.take(draws);
+    const uint64_t sa1 = nz.stream_add++;
+    (void)nz.stream_add++;
+    if (!launch_pile_step(c->p, c->dvdr, halfdt, c->dvdr2, halfdtsmall, c->r, c->p, c->rst, nmm->F, nmm->B, nb, na,
+                          c->monod, c->mass, 1.0 / c->mtot, c->pile_c1, c->pile_c3, n1, n2, c->seed, c->step_ctr, sa1,
+                          c->pstep_parts, s))
+        return RPMD_ERR_ARG;
+    fr = FusedRep{};
+    RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, c->ev_step, false, &fr.red, nullptr));
+    std::swap(c->r, c->rst);
+    c->lf_merged = true;
+    const double* pos = c->r;
+    const Box bx = make_box(box);
+    const double boxmin = std::min(box[0], std::min(box[1], box[2]));
+    const double oo_cut = std::min(P.rcut, 0.5 * boxmin);
+    if (nb > 3) {
+        const double skin = 1.0 / 0.5291772108;
+        const double key[5] = {box[0], box[1], box[2], oo_cut, skin};
+        const int force = (!c->oo_built || memcmp(key, c->oo_key, sizeof key) != 0) ? 1 : 0;
+        memcpy(c->oo_key, key, sizeof key);
+        c->oo_built = true;
+        launch_centroid_vl(pos, c->cen, na, nb, c->oo_cb, c->oo_dmax, s);
+        launch_nlist_vl(c->cen, na, bx, oo_cut, skin, force, c->oo_nnb, c->oo_nbl, c->oo_cb, c->oo_dmax, c->oo_ovf,
+                        c->nnb, c->nbl, s);
+        fr.use_list = 1;
+        fr.oo_dmax = c->oo_dmax;
+    }
+    RETCHK(build_kset(c, box));
+    const Box sb = unit_box();
+    const PairTab* ptab = nullptr;
+    RETCHK(pair_table_for(c, sb, P.wrcut, P.walpha, s, &ptab));
+    CellGrid g = make_grid(sb, na);
+    if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
+    const double skin = (1.0 / box[0]) * (1.0 / 0.5291772108);
+    VList* vl = nullptr;
+    int force = 0;
+    RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force, 0));
+    launch_sites_vl(pos, c->site, na, nb, P.alpha, 1, bx, c->ks.lat, c->ph, c->q, *vl, !force, sb, s);
+    launch_vlist(*vl, c->site, c->q, na, nb, sb, g, P.wrcut, skin, force, c->cell_of, c->cnt, c->start, s);
+    it = FBItem{vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, c->parts + c->off_real, ptab, c->ph, c->q, &c->ks,
+                c->skp, c->kmask, c->gk, c->gks, c->parts + c->off_k,
+                c->fpart + (size_t)ewald_real_slots() * 3 * na * nb};
+    return hipGetLastError() == hipSuccess ? 0 : RPMD_ERR_HIP;
+}
+
+// The second half: the O-O pairs with the chain rule of the Ewald slots, the energy jobs, the closing
+// kicks and the reductions, as the single-stream step after its force kernels.
+static int fused_second_half(rpmd_ctx* c, FusedRep& fr) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nb = c->st_nb;
+    hipStream_t s = c->st;
+    const double dt = c->st_dt, halfdt = 0.5 * dt, halfdtsmall = 0.5 * dt;
+    const double* box = c->st_box.L;
+    const size_t N = (size_t)3 * na * nb;
+    const Box bx = make_box(box);
+    const double sc = 1.0 / box[0];
+    const int chain_npart = ewald_real_slots() + 1;
+    ReduceJobs jobs;
+    jobs.add(c->parts + c->off_real, nb * ewald_real_parts(na), 0, sc / nb);
+    jobs.add(c->parts + c->off_k, nb * (P.kmax + 1), 10, sc / nb);
+    double* po = c->parts + c->off_oo;
+    launch_oo<float>(c->r, na, nb, bx, c->nnb, c->nbl, fr.use_list, P.rcut, P, c->dvdr, po, s, c->fpart, chain_npart,
+                     P.alpha, sc * sc, 0, fr.oo_dmax);
+    jobs.add(po, nb * oo_blocks(na), 20, 1.0 / nb);
+    const int base = 50;  // the forces(iopt = 1) slot follows the one of the intramolecular forces
+    for (int j = 0; j < jobs.n; j++) {
+        if (fr.red.n >= RED_JOBS) return RPMD_ERR_ARG;
+        fr.red.add(jobs.parts[j], jobs.nparts[j], base + jobs.out[j], jobs.scale[j]);
+    }
+    launch_kick(c->p, c->dvdr2, halfdtsmall, c->dvdr, halfdt, N, s, c->step_ctr);
+    launch_reduce_ev(fr.red, c->ev_step, s);
+    return hipGetLastError() == hipSuccess ? 0 : RPMD_ERR_HIP;
+}
+
+// The autotuner of the batched force launches: each choice of the pair walk's cap (none, 6 or 4
+// blocks per CU) and stream layout (after the k-space chain, or beside it on a second stream) is
+// launched three times and timed with events on the batch stream; the fastest is kept.  The
+// kernels only overwrite their outputs, so the extra launches leave the step's results unchanged.
+static int fused_tune(rpmd_batch* b) {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return RPMD_ERR_HIP;
+    float best = 1e30f;
+    int rc = 0;
+    for (int two = 0; two < 2 && !rc; two++)
+        for (int bpc : {0, 6, 4}) {
+            auto once = [&]() {
+                if (two) {
+                    (void)hipEventRecord(b->ev_f2, b->st);
+                    (void)hipStreamWaitEvent(b->st2, b->ev_f2, 0);
+                    force_batch_launch(b->fb, b->st, b->st2, bpc);
+                    (void)hipEventRecord(b->ev_j2, b->st2);
+                    (void)hipStreamWaitEvent(b->st, b->ev_j2, 0);
+                } else {
+                    force_batch_launch(b->fb, b->st, b->st, bpc);
+                }
+            };
+            once();  // warm
+            if (hipEventRecord(t0, b->st) != hipSuccess) rc = RPMD_ERR_HIP;
+            for (int k = 0; k < 3; k++) once();
+            float ms = 0.0f;
+            if (rc || hipEventRecord(t1, b->st) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+                rc = RPMD_ERR_HIP;
+                break;
+            }
+            if (ms < best) best = ms, b->bpc = bpc, b->two = two;
+        }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    b->tuned = !rc;
+    return rc;
+}
+
+// one batched step of every replica: first halves on the replicas' streams, the four force kernels
+// of all replicas as four launches, second halves on the replicas' streams, all joined on b->st
+static int fused_step(rpmd_batch* b) {
+    const int n = (int)b->ctx.size();
+    b->fr.resize(n);
+    b->items.resize(n);
+    HIPCHK(hipEventRecord(b->ev[2 * n], b->st));
+    for (int i = 0; i < n; i++) {
+        rpmd_ctx* c = b->ctx[i];
+        HIPCHK(hipStreamWaitEvent(c->st, b->ev[2 * n], 0));
+        RETCHK(fused_first_half(c, b->fr[i], b->items[i]));
+        HIPCHK(hipEventRecord(b->ev_a[i], c->st));
+        HIPCHK(hipStreamWaitEvent(b->st, b->ev_a[i], 0));
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(b->st, &cs));
+    const int rc = force_batch_set(b->fb, b->items.data(), n, b->st, cs != hipStreamCaptureStatusNone);
+    if (rc < 0) return RPMD_ERR_HIP;
+    if (rc > 0) return RPMD_ERR_ARG;  // the records changed under capture: the caller steps eagerly
+    if (!b->tuned && cs == hipStreamCaptureStatusNone) RETCHK(fused_tune(b));
+    if (b->two) {
+        HIPCHK(hipEventRecord(b->ev_f2, b->st));
+        HIPCHK(hipStreamWaitEvent(b->st2, b->ev_f2, 0));
+        force_batch_launch(b->fb, b->st, b->st2, b->bpc);
+        HIPCHK(hipEventRecord(b->ev_j2, b->st2));
+        HIPCHK(hipStreamWaitEvent(b->st, b->ev_j2, 0));
+    } else {
+        force_batch_launch(b->fb, b->st, b->st, b->bpc);
+    }
+    HIPCHK(hipEventRecord(b->ev_b, b->st));
+    for (int i = 0; i < n; i++) {
+        rpmd_ctx* c = b->ctx[i];
+        HIPCHK(hipStreamWaitEvent(c->st, b->ev_b, 0));
+        RETCHK(fused_second_half(c, b->fr[i]));
+        HIPCHK(hipEventRecord(b->ev[2 * i], c->st));
+        HIPCHK(hipStreamWaitEvent(b->st, b->ev[2 * i], 0));
+    }
+    return 0;
+}
 
 int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out) {
     if (!ctxs || n < 1 || !out) return RPMD_ERR_ARG;
@@ -2035,16 +2236,25 @@ int rpmd_batch_create(rpmd_ctx* const* ctxs, int n, rpmd_batch** out) {
     }
     rpmd_batch* b = new rpmd_batch();
     b->ctx.assign(ctxs, ctxs + n);
-    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess) {
+        rpmd_batch_destroy(b);
         return RPMD_ERR_HIP;
     }
     b->ev.resize((size_t)2 * n + 1, nullptr);
-    for (auto& e : b->ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+    b->ev_a.resize(n, nullptr);
+    for (auto* v : {&b->ev, &b->ev_a})
+        for (auto& e : *v)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                rpmd_batch_destroy(b);
+                return RPMD_ERR_HIP;
+            }
+    for (hipEvent_t* e : {&b->ev_b, &b->ev_f2, &b->ev_j2})
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             rpmd_batch_destroy(b);
             return RPMD_ERR_HIP;
         }
+    b->fb = force_batch_new();
     *out = b;
     return 0;
 }
@@ -2059,9 +2269,14 @@ static void batch_drop_graph(rpmd_batch* b) {
 int rpmd_batch_destroy(rpmd_batch* b) {
     if (!b) return 0;
     batch_drop_graph(b);
-    for (hipEvent_t e : b->ev)
-        if (e) hipEventDestroy(e);
-    if (b->st) hipStreamDestroy(b->st);
+    for (auto* v : {&b->ev, &b->ev_a})
+        for (hipEvent_t e : *v)
+            if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->ev_b, b->ev_f2, b->ev_j2})
+        if (e) (void)hipEventDestroy(e);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    if (b->st2) (void)hipStreamDestroy(b->st2);
+    force_batch_free(b->fb);
     delete b;
     return 0;
 }
@@ -2097,39 +2312,90 @@ int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
         return 0;
     };
     std::vector<int> left(n, nsteps), one(n, 1);
+    // every replica on the batched step (one launch per force kernel for all of them), when each
+    // takes it and the autotuner has not found the replicas' own steps side by side faster
+    bool eligible = true;
+    for (rpmd_ctx* c : b->ctx) eligible = eligible && fused_ok(c);
+    const bool fused = eligible && (!b->mode_tuned || b->mode_fused);
+    auto steps_in = [&](bool fz, int k) -> int {
+        if (!fz) return fork_steps(std::vector<int>(n, k));
+        for (int q = 0; q < k; q++) RETCHK(fused_step(b));
+        return 0;
+    };
+    // two steps of every replica captured into a graph; the buffer roles and epochs at capture
+    auto capture = [&](bool fz, hipGraph_t* gr, hipGraphExec_t* ge) -> int {
+        b->epoch.resize(n);
+        b->roles.resize(n);
+        for (int i = 0; i < n; i++) {
+            b->epoch[i] = b->ctx[i]->epoch;
+            b->roles[i] = roles_of(b->ctx[i]);
+        }
+        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int rc = steps_in(fz, 2);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->st, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            for (int i = 0; i < n; i++) {
+                rpmd_ctx* c = b->ctx[i];
+                const Roles& R = b->roles[i];
+                c->r = R.a[0], c->rst = R.a[1], c->dvdr = R.a[2], c->dvdrl = R.a[3], c->tmp = R.a[4];
+            }
+            return rc ? rc : RPMD_ERR_HIP;
+        }
+        if (hipGraphInstantiate(ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g);
+            return RPMD_ERR_HIP;
+        }
+        *gr = g;
+        return 0;
+    };
+    // one untimed and one timed replay (two steps each) of a graph: real steps of the run
+    auto timed = [&](hipGraphExec_t ge, float* ms) -> int {
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return RPMD_ERR_HIP;
+        int rc = 0;
+        if (hipGraphLaunch(ge, b->st) != hipSuccess || hipEventRecord(t0, b->st) != hipSuccess ||
+            hipGraphLaunch(ge, b->st) != hipSuccess || hipEventRecord(t1, b->st) != hipSuccess ||
+            hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(ms, t0, t1) != hipSuccess)
+            rc = RPMD_ERR_HIP;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        for (int& x : left) x -= 4;
+        return rc;
+    };
     if (nsteps >= 2) {
-        bool valid = b->gexec != nullptr;
+        bool valid = b->gexec != nullptr && b->fused == fused;
         for (int i = 0; i < n && valid; i++) valid = b->epoch[i] == b->ctx[i]->epoch;
         if (!valid) {
             batch_drop_graph(b);
-            // one eager step of every replica first (lists and keys up to date outside the
-            // capture), then two steps of every replica captured
-            RETCHK(fork_steps(one));
+            // one eager step of every replica first (lists, keys and launch records up to date
+            // outside the capture), then two steps of every replica captured
+            RETCHK(steps_in(fused, 1));
             for (int& x : left) x--;
             if (nsteps - 1 >= 2) {
-                b->epoch.resize(n);
-                b->roles.resize(n);
-                for (int i = 0; i < n; i++) {
-                    b->epoch[i] = b->ctx[i]->epoch;
-                    b->roles[i] = roles_of(b->ctx[i]);
-                }
-                HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-                int rc = fork_steps(std::vector<int>(n, 2));
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(b->st, &g);
-                if (rc || e != hipSuccess) {
-                    if (g) hipGraphDestroy(g);
-                    for (int i = 0; i < n; i++) {
-                        rpmd_ctx* c = b->ctx[i];
-                        const Roles& R = b->roles[i];
-                        c->r = R.a[0], c->rst = R.a[1], c->dvdr = R.a[2], c->dvdrl = R.a[3], c->tmp = R.a[4];
+                RETCHK(capture(fused, &b->graph, &b->gexec));
+                b->fused = fused;
+                // the batch autotuner, when a batched step is possible and enough steps are asked
+                // for: four steps of each form, the second two of each timed, the faster kept
+                if (eligible && !b->mode_tuned && nsteps - 1 >= 8) {
+                    RETCHK(timed(b->gexec, &b->ms_fused));
+                    hipGraph_t g2 = nullptr;
+                    hipGraphExec_t ge2 = nullptr;
+                    RETCHK(capture(false, &g2, &ge2));
+                    const int rc = timed(ge2, &b->ms_separate);
+                    b->mode_tuned = true;
+                    b->mode_fused = b->ms_fused <= b->ms_separate;
+                    if (!b->mode_fused) {
+                        batch_drop_graph(b);
+                        b->graph = g2;
+                        b->gexec = ge2;
+                        b->fused = false;
+                    } else {
+                        (void)hipGraphExecDestroy(ge2);
+                        (void)hipGraphDestroy(g2);
                     }
-                    return rc ? rc : RPMD_ERR_HIP;
-                }
-                b->graph = g;
-                if (hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
-                    batch_drop_graph(b);
-                    return RPMD_ERR_HIP;
+                    RETCHK(rc);
                 }
             }
         }
@@ -2162,6 +2428,17 @@ int rpmd_batch_run_steps(rpmd_batch* b, int nsteps) {
         HIPCHK(hipStreamSynchronize(c->st));
         RETCHK(run_status(c));
     }
+    return 0;
+}
+
+int rpmd_batch_tuning(const rpmd_batch* b, int* batched, double* ms_batched, double* ms_separate, int* pair_cap,
+                      int* pair_beside) {
+    if (!b) return RPMD_ERR_ARG;
+    if (batched) *batched = b->gexec ? (b->fused ? 1 : 0) : -1;
+    if (ms_batched) *ms_batched = b->ms_fused;
+    if (ms_separate) *ms_separate = b->ms_separate;
+    if (pair_cap) *pair_cap = b->bpc;
+    if (pair_beside) *pair_beside = b->two;
     return 0;
 }
 

@@ -196,6 +196,32 @@ template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s, int bpc = 0, const PairTab* tab = nullptr);
 int kspace_nchunk(int na);
+
+// Replica batches: one replica's inputs of the four dominant mixed-precision force kernels (the FP32
+// pair walk on the unit box with its table, k_sk_h16, k_sk_final, k_kforce_h16), as the solo
+// launches take them
+struct FBItem {
+    const VList* vl;
+    int na, nb;
+    Box sb;
+    double rcut, alpha;
+    double *grad, *parts;  // the pair walk's gradient slot and energy/virial parts
+    const PairTab* tab;
+    const double *ph, *q;
+    const KSet* ks;
+    double* skp;
+    const unsigned char* kmask;
+    double *gk, *gks, *parts_k, *kpart;
+};
+struct ForceBatch;
+ForceBatch* force_batch_new();
+void force_batch_free(ForceBatch* fb);
+// the launch records of n replicas, uploaded (on s, synchronously) when they differ from the last
+// ones: 0 ready, 1 they differ while s is being captured (nothing uploaded), -1 HIP error
+int force_batch_set(ForceBatch* fb, const FBItem* items, int n, hipStream_t s, bool capturing);
+// every replica's four kernels as four launches: the pair walk on sr, the k-space chain on sk (the
+// same stream: one after the other); bpc > 0 caps the pair walk at bpc blocks per CU
+void force_batch_launch(const ForceBatch* fb, hipStream_t sk, hipStream_t sr, int bpc);
 int kspace_ngroups(int kmax);
 void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks, double* sk_part, int fp32,
                hipStream_t s);

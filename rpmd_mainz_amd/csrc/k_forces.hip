@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -850,10 +851,10 @@ __device__ __forceinline__ double quad_sum(double v) {
     srt, srf, perm, cnt, nex, nl, na, nb, cap, L0, L1, L2, i0, i1, i2, rc2, alpha, sfac, grad, parts, dmax, tab, tsc, \
         tku, tkr
 template <typename T, bool UNIT>
-__device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
+__device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS, const int id) {
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, double4, uint4>::type P4;
-    const int id = blockIdx.x, m8 = id >> 3;
+    const int m8 = id >> 3;
     const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
     const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
     if (id == 0 && threadIdx.x == 0) ctrl_store(dmax, 0ull);
@@ -1066,11 +1067,11 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS) {
 // as the compiler chooses
 template <bool UNIT>
 __global__ __launch_bounds__(256, 8) void k_ewald_vl32(EWVL_PARAMS) {
-    ewald_vl_body<float, UNIT>(EWVL_ARGS);
+    ewald_vl_body<float, UNIT>(EWVL_ARGS, blockIdx.x);
 }
 template <bool UNIT>
 __global__ __launch_bounds__(256) void k_ewald_vl64(EWVL_PARAMS) {
-    ewald_vl_body<double, UNIT>(EWVL_ARGS);
+    ewald_vl_body<double, UNIT>(EWVL_ARGS, blockIdx.x);
 }
 
 int ewald_nclu(int na) { return (na + CLU - 1) / CLU; }
@@ -1460,10 +1461,11 @@ __device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
 // rows 32..63 weighted by kz, hi then lo ([2][64][32] dwords per (bead, kx)) -- and 1 / (s 2^10) in
 // gks, where the power of two s brings the largest |component| of the tile to [2^10, 2^11)
 template <typename P, bool SPLIT>
-__global__ __launch_bounds__(256) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
-                                                  const unsigned char* __restrict__ mask, double2* __restrict__ gk,
-                                                  double* __restrict__ gks, double* __restrict__ parts) {
-    const int kx = blockIdx.x, b = blockIdx.y, kmax = kd.kmax;
+__device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nchunk, const KDev& kd,
+                                              const unsigned char* __restrict__ mask, double2* __restrict__ gk,
+                                              double* __restrict__ gks, double* __restrict__ parts, const int kx,
+                                              const int b) {
+    const int kmax = kd.kmax;
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
@@ -1546,6 +1548,12 @@ __global__ __launch_bounds__(256) void k_sk_final(const P* __restrict__ skp, int
         if (threadIdx.x == 0) gks[(size_t)b * (kmax + 1) + kx] = 1.0 / (sc * 1024.0);
     }
     block_reduce_store<10>(acc, parts + ((size_t)b * (kmax + 1) + kx) * 10);
+}
+template <typename P, bool SPLIT>
+__global__ __launch_bounds__(256) void k_sk_final(const P* __restrict__ skp, int nchunk, KDev kd,
+                                                  const unsigned char* __restrict__ mask, double2* __restrict__ gk,
+                                                  double* __restrict__ gks, double* __restrict__ parts) {
+    sk_final_body<P, SPLIT>(skp, nchunk, kd, mask, gk, gks, parts, blockIdx.x, blockIdx.y);
 }
 
 // Horner over kz of two ky rows: h = sum_m c_m w^m, d = dh/dw, with c_m = G(kz = m - KZ).  After the
@@ -1695,9 +1703,9 @@ __device__ __forceinline__ void mfma_interleave() {
         __builtin_amdgcn_sched_group_barrier(0x002, V, 0);
     }
 }
-__global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
-                                                int chunk, int nchunk, KDev kd, float2* __restrict__ skp) {
-    const int ch = blockIdx.x, b = blockIdx.y;
+__device__ __forceinline__ void sk_h16_body(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                            int chunk, int nchunk, const KDev& kd, float2* __restrict__ skp,
+                                            const int ch, const int b) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int qc = w & 1, kg = w >> 1;
     const int kmax = kd.kmax;
@@ -1854,6 +1862,10 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
             }
     }
 }
+__global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                                int chunk, int nchunk, KDev kd, float2* __restrict__ skp) {
+    sk_h16_body(ph, q, na, chunk, nchunk, kd, skp, blockIdx.x, blockIdx.y);
+}
 
 // The force pass as a GEMM: for each kx, U = [G; kz G] conj(Z), A = the (ky, kz) tile of G = 2 w S
 // (rows 32..63 weighted by kz; split and scaled by k_sk_final) and B = conj(Z_j(kz)) for 64 atoms, so
@@ -1867,13 +1879,14 @@ __global__ __launch_bounds__(512) void k_sk_h16(const double2* __restrict__ ph, 
 // matrix and vector units busy together; at most 256 registers (two waves per SIMD).  Each lane
 // sums its part of the three components over kx in FP32 (the tile scales are powers of two); lanes
 // and waves are combined once at the end, in fixed order, the waves in FP64.  Blocks are dealt to the XCDs by bead.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_kforce_h16(
-    const double2* __restrict__ ph, const double* __restrict__ q, int na, int nbk, KDev kd,
-    const uint4* __restrict__ ga, const double* __restrict__ gks, double* __restrict__ kpart) {
+__device__ __forceinline__ void kforce_h16_body(const double2* __restrict__ ph, const double* __restrict__ q, int na,
+                                                int nbk, const KDev& kd, const uint4* __restrict__ ga,
+                                                const double* __restrict__ gks, double* __restrict__ kpart,
+                                                const int bid) {
     // blocks are dealt to the eight XCDs in turn (block id mod 8): XCD x takes beads x, x + 8, ... one
     // after the other, so the blocks resident on an XCD read the G tiles of one bead from its L2
-    const int nab = (na + KFH - 1) / KFH, m8 = blockIdx.x >> 3;
-    const int b = (blockIdx.x & 7) + 8 * (m8 / nab), a0 = (m8 % nab) * KFH;
+    const int nab = (na + KFH - 1) / KFH, m8 = bid >> 3;
+    const int b = (bid & 7) + 8 * (m8 / nab), a0 = (m8 % nab) * KFH;
     if (b >= nbk) return;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const int kmax = kd.kmax;
@@ -2014,6 +2027,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         o[2 * na + a0 + t] = kd.lat2 * tz;
     }
 }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_kforce_h16(
+    const double2* __restrict__ ph, const double* __restrict__ q, int na, int nbk, KDev kd,
+    const uint4* __restrict__ ga, const double* __restrict__ gks, double* __restrict__ kpart) {
+    kforce_h16_body(ph, q, na, nbk, kd, ga, gks, kpart, blockIdx.x);
+}
 
 int kspace_nchunk(int na) { return (na + SK_CHUNK - 1) / SK_CHUNK; }
 // mixed structure-factor pass: about one block per CU over all beads (the FP32 partials of at most
@@ -2080,6 +2098,262 @@ int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet&
     k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
         (const double2*)ph, q, na, kd, (const double2*)gk, kpart);
     return kspace_ngroups(ks.kmax);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Replica batches: the four dominant force kernels of many contexts (the FP32 pair walk on the unit
+// box, k_sk_h16, k_sk_final, k_kforce_h16), each as one launch over every replica's blocks.  A
+// replica's blocks form a range starting at a multiple of 8 (so the XCD dealing of the pair and
+// force-pass kernels, block id mod 8, is the solo launch's), its arguments are one record in HBM,
+// and every block finds its replica by a binary search of the range starts.  The kernels' bodies
+// are the solo kernels', with the same arguments, so each replica's results are the solo results
+// bit for bit.
+// ------------------------------------------------------------------------------------------------
+struct EwVlArgs {
+    const double4* srt;
+    const uint4* srf;
+    const int *perm, *cnt, *nex, *nl;
+    int na, nb, cap, nblk;
+    double L0, L1, L2, i0, i1, i2, rc2, alpha, sfac;
+    double *grad, *parts;
+    unsigned long long* dmax;
+    const float4* tab;
+    float tsc;
+    int tku, tkr;
+};
+struct SkH16Args {
+    const double2* ph;
+    const double* q;
+    float2* skp;
+    int na, chunk, nchunk, nblk;
+    KDev kd;
+};
+struct SkFinArgs {
+    const float2* skp;
+    const unsigned char* mask;
+    double2* gk;
+    double *gks, *parts;
+    int nchunk, nblk;
+    KDev kd;
+};
+struct KfH16Args {
+    const double2* ph;
+    const double* q;
+    const uint4* ga;
+    const double* gks;
+    double* kpart;
+    int na, nbk, nblk;
+    KDev kd;
+};
+
+// the replica of block bid: the last r with off[r] <= bid (off[0] = 0, n ranges)
+__device__ __forceinline__ int batch_find(const int* __restrict__ off, int n, int bid) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= bid)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256, 8) void k_ewald_vl32_batch(const EwVlArgs* __restrict__ A,
+                                                             const int* __restrict__ off, int n) {
+    const int r = batch_find(off, n, blockIdx.x);
+    const EwVlArgs& a = A[r];
+    const int id = blockIdx.x - off[r];
+    if (id >= a.nblk) return;
+    ewald_vl_body<float, true>(a.srt, a.srf, a.perm, a.cnt, a.nex, a.nl, a.na, a.nb, a.cap, a.L0, a.L1, a.L2, a.i0,
+                               a.i1, a.i2, a.rc2, a.alpha, a.sfac, a.grad, a.parts, a.dmax, a.tab, a.tsc, a.tku, a.tkr,
+                               id);
+}
+__global__ __launch_bounds__(512) void k_sk_h16_batch(const SkH16Args* __restrict__ A, const int* __restrict__ off,
+                                                      int n) {
+    const int r = batch_find(off, n, blockIdx.x);
+    const SkH16Args& a = A[r];
+    const int id = blockIdx.x - off[r];
+    if (id >= a.nblk) return;
+    sk_h16_body(a.ph, a.q, a.na, a.chunk, a.nchunk, a.kd, a.skp, id % a.nchunk, id / a.nchunk);
+}
+__global__ __launch_bounds__(256) void k_sk_final_batch(const SkFinArgs* __restrict__ A, const int* __restrict__ off,
+                                                        int n) {
+    const int r = batch_find(off, n, blockIdx.x);
+    const SkFinArgs& a = A[r];
+    const int id = blockIdx.x - off[r];
+    if (id >= a.nblk) return;
+    const int nkx = a.kd.kmax + 1;
+    sk_final_body<float2, true>(a.skp, a.nchunk, a.kd, a.mask, a.gk, a.gks, a.parts, id % nkx, id / nkx);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_kforce_h16_batch(
+    const KfH16Args* __restrict__ A, const int* __restrict__ off, int n) {
+    const int r = batch_find(off, n, blockIdx.x);
+    const KfH16Args& a = A[r];
+    const int id = blockIdx.x - off[r];
+    if (id >= a.nblk) return;
+    kforce_h16_body(a.ph, a.q, a.na, a.nbk, a.kd, a.ga, a.gks, a.kpart, id);
+}
+
+struct ForceBatch {
+    int n = 0;
+    // what was uploaded (records zero-filled, so that they compare bytewise)
+    std::vector<EwVlArgs> ew;
+    std::vector<SkH16Args> sk;
+    std::vector<SkFinArgs> sf;
+    std::vector<KfH16Args> kf;
+    std::vector<int> off;  // [4][n + 1] range starts: pairs, k_sk_h16, k_sk_final, k_kforce_h16
+    char* d = nullptr;
+    size_t cap = 0, o_ew = 0, o_sk = 0, o_sf = 0, o_kf = 0, o_off = 0;
+};
+
+ForceBatch* force_batch_new() { return new ForceBatch(); }
+void force_batch_free(ForceBatch* fb) {
+    if (!fb) return;
+    if (fb->d) (void)hipFree(fb->d);
+    delete fb;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int force_batch_set(ForceBatch* fb, const FBItem* it, int n, hipStream_t s, bool capturing) {
+    std::vector<EwVlArgs> ew(n);
+    std::vector<SkH16Args> sk(n);
+    std::vector<SkFinArgs> sf(n);
+    std::vector<KfH16Args> kf(n);
+    std::vector<int> off((size_t)4 * (n + 1), 0);
+    memset(ew.data(), 0, sizeof(EwVlArgs) * n);
+    memset(sk.data(), 0, sizeof(SkH16Args) * n);
+    memset(sf.data(), 0, sizeof(SkFinArgs) * n);
+    memset(kf.data(), 0, sizeof(KfH16Args) * n);
+    for (int i = 0; i < n; i++) {
+        const FBItem& x = it[i];
+        const VList& v = *x.vl;
+        const int na = x.na, nb = x.nb;
+        EwVlArgs& e = ew[i];
+        e.srt = v.srt;
+        e.srf = v.srf;
+        e.perm = v.perm;
+        e.cnt = v.cnt;
+        e.nex = v.nex;
+        e.nl = v.nl;
+        e.na = na;
+        e.nb = nb;
+        e.cap = v.cap;
+        e.nblk = ewald_real_parts(na) * nb;
+        e.L0 = x.sb.L[0];
+        e.L1 = x.sb.L[1];
+        e.L2 = x.sb.L[2];
+        e.i0 = x.sb.inv[0];
+        e.i1 = x.sb.inv[1];
+        e.i2 = x.sb.inv[2];
+        e.rc2 = x.rcut * x.rcut;
+        e.alpha = x.alpha;
+        e.sfac = -2.0 * (x.alpha / sqrt(acos(-1.0)));  // as launch_ewald_real
+        e.grad = x.grad;
+        e.parts = x.parts;
+        e.dmax = v.dmax;
+        e.tab = x.tab->d;
+        e.tsc = x.tab->c;
+        e.tku = x.tab->ku;
+        e.tkr = x.tab->kr;
+        const KDev kd = make_kdev(*x.ks);
+        int chunk = 0;
+        const int nchunk = kspace_nchunk_h16(na, nb, &chunk);
+        SkH16Args& a = sk[i];
+        a.ph = (const double2*)x.ph;
+        a.q = x.q;
+        a.skp = (float2*)x.skp;
+        a.na = na;
+        a.chunk = chunk;
+        a.nchunk = nchunk;
+        a.nblk = nchunk * nb;
+        a.kd = kd;
+        SkFinArgs& f = sf[i];
+        f.skp = (const float2*)x.skp;
+        f.mask = x.kmask;
+        f.gk = (double2*)x.gk;
+        f.gks = x.gks;
+        f.parts = x.parts_k;
+        f.nchunk = nchunk;
+        f.nblk = (x.ks->kmax + 1) * nb;
+        f.kd = kd;
+        KfH16Args& k = kf[i];
+        k.ph = (const double2*)x.ph;
+        k.q = x.q;
+        k.ga = (const uint4*)x.gk;
+        k.gks = x.gks;
+        k.kpart = x.kpart;
+        k.na = na;
+        k.nbk = nb;
+        k.nblk = (na + KFH - 1) / KFH * ((nb + 7) / 8 * 8);
+        k.kd = kd;
+        const int nblk[4] = {e.nblk, a.nblk, f.nblk, k.nblk};
+        for (int t = 0; t < 4; t++)
+            off[(size_t)t * (n + 1) + i + 1] = off[(size_t)t * (n + 1) + i] + (nblk[t] + 7) / 8 * 8;
+    }
+    const bool same = fb->n == n && fb->d && !memcmp(ew.data(), fb->ew.data(), sizeof(EwVlArgs) * n) &&
+                      !memcmp(sk.data(), fb->sk.data(), sizeof(SkH16Args) * n) &&
+                      !memcmp(sf.data(), fb->sf.data(), sizeof(SkFinArgs) * n) &&
+                      !memcmp(kf.data(), fb->kf.data(), sizeof(KfH16Args) * n) && off == fb->off;
+    if (same) return 0;
+    if (capturing) return 1;
+    size_t o = 0;
+    const size_t o_ew = o;
+    o = align256(o + sizeof(EwVlArgs) * n);
+    const size_t o_sk = o;
+    o = align256(o + sizeof(SkH16Args) * n);
+    const size_t o_sf = o;
+    o = align256(o + sizeof(SkFinArgs) * n);
+    const size_t o_kf = o;
+    o = align256(o + sizeof(KfH16Args) * n);
+    const size_t o_off = o;
+    o = align256(o + sizeof(int) * off.size());
+    // queued launches may still read the old records
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (o > fb->cap) {
+        if (fb->d) (void)hipFree(fb->d);
+        fb->d = nullptr;
+        fb->cap = 0;
+        if (hipMalloc((void**)&fb->d, o) != hipSuccess) return -1;
+        fb->cap = o;
+    }
+    std::vector<char> h(o, 0);
+    memcpy(h.data() + o_ew, ew.data(), sizeof(EwVlArgs) * n);
+    memcpy(h.data() + o_sk, sk.data(), sizeof(SkH16Args) * n);
+    memcpy(h.data() + o_sf, sf.data(), sizeof(SkFinArgs) * n);
+    memcpy(h.data() + o_kf, kf.data(), sizeof(KfH16Args) * n);
+    memcpy(h.data() + o_off, off.data(), sizeof(int) * off.size());
+    if (hipMemcpyAsync(fb->d, h.data(), o, hipMemcpyHostToDevice, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return -1;
+    fb->n = n;
+    fb->ew.swap(ew);
+    fb->sk.swap(sk);
+    fb->sf.swap(sf);
+    fb->kf.swap(kf);
+    fb->off.swap(off);
+    fb->o_ew = o_ew;
+    fb->o_sk = o_sk;
+    fb->o_sf = o_sf;
+    fb->o_kf = o_kf;
+    fb->o_off = o_off;
+    return 0;
+}
+
+// the four launches: pairs on sr, then (or beside them, sr != sk) the k-space chain on sk.  bpc > 0
+// caps the pair kernel at bpc blocks per CU (dynamic LDS).
+void force_batch_launch(const ForceBatch* fb, hipStream_t sk, hipStream_t sr, int bpc) {
+    const int n = fb->n;
+    const int* off = (const int*)(fb->d + fb->o_off);
+    auto total = [&](int t) { return (unsigned)fb->off[(size_t)t * (n + 1) + n]; };
+    static size_t dyn[9] = {~(size_t)0, ~(size_t)0, ~(size_t)0, ~(size_t)0, ~(size_t)0,
+                            ~(size_t)0, ~(size_t)0, ~(size_t)0, ~(size_t)0};  // [cap], caps 0 .. 8
+    size_t& d = dyn[std::max(0, std::min(bpc, 8))];
+    if (d == ~(size_t)0) d = lds_for_blocks_per_cu((const void*)k_ewald_vl32_batch, std::min(bpc, 8));
+    k_ewald_vl32_batch<<<total(0), 256, d, sr>>>((const EwVlArgs*)(fb->d + fb->o_ew), off, n);
+    k_sk_h16_batch<<<total(1), 512, 0, sk>>>((const SkH16Args*)(fb->d + fb->o_sk), off + (n + 1), n);
+    k_sk_final_batch<<<total(2), 256, 0, sk>>>((const SkFinArgs*)(fb->d + fb->o_sf), off + 2 * (n + 1), n);
+    k_kforce_h16_batch<<<total(3), 256, 0, sk>>>((const KfH16Args*)(fb->d + fb->o_kf), off + 3 * (n + 1), n);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -195,6 +195,14 @@ module rpmd_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: batch
      end function c_batch_destroy
+     ! the autotuners' choices of a batch
+     integer(c_int) function c_batch_tuning(batch, batched, ms_batched, ms_separate, pair_cap, pair_beside) &
+         bind(C, name='rpmd_batch_tuning')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: batch
+       integer(c_int), intent(out) :: batched, pair_cap, pair_beside
+       real(c_double), intent(out) :: ms_batched, ms_separate
+     end function c_batch_tuning
   end interface
 
   type(c_ptr), save :: rpmd_ctx = c_null_ptr

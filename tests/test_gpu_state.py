@@ -147,3 +147,45 @@ def test_device_estimators_match_state(capi, case):
                         3 * (na // 3) if case == "rigid" else 0)
     for k, v in ref.items():
         assert abs(e[k] - v) <= 1e-12 * max(abs(v), 1e-300), (k, e[k], v)
+
+
+def _mixed_replicas(capi, n, seed, beads=(2, 8, 16, 32), fp64_at=None):
+    from test_gpu_parity import _replica_ctx, _replica_specs
+    specs = _replica_specs(n, seed, sizes=(27, 64, 128, 256), beads=beads)
+    return specs, [_replica_ctx(capi, sp, "fp64" if i == fp64_at else "mixed") for i, sp in enumerate(specs)]
+
+
+@pytest.mark.parametrize("case", ["batched", "with_fp64_replica"])
+def test_batched_force_launches_equal_independent_runs(capi, case):
+    """Mixed-precision replicas of uneven size (27-256 molecules, 2-32 beads, own cubic boxes and
+    temperatures) stepped by the batch, whose force kernels then run as one launch per kernel for
+    every replica (and whose stream layout and pair-walk cap the autotuner picks): the same bits as
+    each replica stepped alone.  With one FP64 replica the batch steps each replica on its own
+    stream instead, with the same result.  Includes an odd number of solo steps of one replica
+    between batch calls."""
+    fp64_at = 2 if case == "with_fp64_replica" else None
+    specs, ctxs = _mixed_replicas(capi, 6, 11, fp64_at=fp64_at)
+    solo = []
+    for i, sp in enumerate(specs):
+        from test_gpu_parity import _replica_ctx
+        c = _replica_ctx(capi, sp, "fp64" if i == fp64_at else "mixed")
+        c.use_graph(True)
+        c.run_steps(9)
+        solo.append(c.state_download())
+        c.close()
+    batch = capi.ReplicaBatch(ctxs)
+    print(f"case batched force launches {case}", flush=True)
+    batch.run_steps(4)
+    ctxs[1].run_steps(1)      # one replica one step ahead: the batch brings it back to even roles
+    batch.run_steps(4)
+    # replica 1 has had 9 steps; the others 8, then one more each
+    for i, c in enumerate(ctxs):
+        if i != 1:
+            c.run_steps(1)
+    for c, ref in zip(ctxs, solo):
+        st = c.state_download()
+        assert np.array_equal(st["r"], ref["r"]) and np.array_equal(st["p"], ref["p"])
+        assert st["v"] == ref["v"]
+    batch.close()
+    for c in ctxs:
+        c.close()
