@@ -41,6 +41,8 @@ extern "C" {
 #define RPMD_ERR_NOISE            16  /* host noise buffer too short for the draws needed      */
 #define RPMD_ERR_GLE              17  /* therm GLE without rpmd_gle_init for this (na, nb)     */
 #define RPMD_ERR_TTAU             18  /* a thermostat is selected but ttau <= 0               */
+#define RPMD_ERR_SOCKET           19  /* i-PI socket: no server at the address, or a broken stream */
+#define RPMD_ERR_IPI_NAT          20  /* i-PI POSDATA with another atom count than the context's */
 
 /* ---- precision modes ------------------------------------------------------------------ */
 #define RPMD_PREC_FP64   0  /* everything FP64 (the "plain FP64 GPU form")                   */
@@ -247,6 +249,19 @@ int rpmd_batch_destroy(rpmd_batch* batch);
  * kernels (1) or after them (0). */
 int rpmd_batch_tuning(const rpmd_batch* batch, int* batched, double* ms_batched, double* ms_separate, int* pair_cap,
                       int* pair_beside);
+
+/* subroutine run_driver(p,r,dvdr,dvdr2,na,nb,boxlxyz,z,beta,dt,mass,irun,itst,pt,pb,print)
+ *                                                        src/ipi_driver.f90:1-254, src/sockets.c
+ * The i-PI client: connects to the server at address ("prefix:host:port", as the reference reads
+ * its serverdata file: prefix "unix" gives the UNIX socket /tmp/ipi_<host>, any other an internet
+ * socket to host:port) and serves it until it disconnects or sends anything but STATUS, POSDATA
+ * or GETFORCE: STATUS -> READY or HAVEDATA; POSDATA (cell, its inverse, the atom count, bohr
+ * positions) -> full_forces on one bead of this context with the box the cell's diagonal;
+ * GETFORCE -> FORCEREADY, the potential, the atom count, the forces -(dvdr + dvdr2), minus the
+ * transposed virial, an empty extra field.  z: the charges.  served (optional): GETFORCE replies.
+ * A POSDATA atom count other than na returns RPMD_ERR_IPI_NAT (the reference prints a warning
+ * and reads past its arrays). */
+int rpmd_ipi_run(rpmd_ctx* ctx, const char* address, const double* z, int na, int* served);
 
 /* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
  * the context stream around the Ewald kernels of eager launches between begin and end.

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librpmd_mi355x.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["k_forces.hip", "k_step.hip", "k_rigid.hip", "capi.hip"]
+SOURCES = ["k_forces.hip", "k_step.hip", "k_rigid.hip", "capi.hip", "ipi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -70,7 +70,7 @@ def lib() -> C.CDLL:
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
                      "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators",
-                     "rpmd_batch_tuning"):
+                     "rpmd_batch_tuning", "rpmd_ipi_run"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -90,6 +90,16 @@ def _arr(a, shape=None) -> np.ndarray:
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def run_driver(ctx: "Context", na: int, nb: int, z, serverdata: str = "serverdata") -> int:
+    """subroutine run_driver (src/ipi_driver.f90:1-254): nb must be 1; the server address is the first
+    whitespace-separated word of the file serverdata, as the reference reads it (list-directed)."""
+    if nb != 1:
+        raise ValueError("i-PI mode must have nb = 1!")
+    with open(serverdata) as f:
+        address = f.read().split()[0].strip("'\"")
+    return ctx.ipi_run(address, z, na)
 
 
 def setup_ewald(na: int, boxlxyz, prm: dict) -> dict:
@@ -164,6 +174,15 @@ class RPMDContext:
                                       _p(_arr(boxlxyz, (3,))), _p(dvdr), _p(dvdr2)))
         return dict(v=v.value, vew=vew.value, voo=voo.value, vint=vint.value, vir=vir.reshape(3, 3).T.copy(),
                     dvdr=dvdr, dvdr2=dvdr2)
+
+    # -- subroutine run_driver(...)  src/ipi_driver.f90:1, src/sockets.c ---------------------------
+    def ipi_run(self, address: str, z, na: int) -> int:
+        """Serve the i-PI server at address ("unix:<name>:<port>" for the UNIX socket /tmp/ipi_<name>,
+        "<any>:<host>:<port>" for an internet socket) with full_forces on one bead until it disconnects
+        or sends EXIT.  Returns the number of force replies sent."""
+        served = C.c_int()
+        _check(lib().rpmd_ipi_run(self._h, address.encode(), _p(_arr(z, (na,))), C.c_int(na), C.byref(served)))
+        return served.value
 
     # -- subroutine realft(data,m,n,mode)  src/fourier.f90:16 --------------------------------------
     def realft(self, data: np.ndarray, m: int, n: int, mode: int) -> np.ndarray:

@@ -1383,6 +1383,8 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_NOISE: return "noise buffer too short";
         case RPMD_ERR_GLE: return "GLE not initialised for this (na, nb): call rpmd_gle_init";
         case RPMD_ERR_TTAU: return "thermostat time constant ttau must be > 0";
+        case RPMD_ERR_SOCKET: return "i-PI socket: no server at the address, or the connection broke";
+        case RPMD_ERR_IPI_NAT: return " @DRIVER MODE: Uh-oh! Particle number mismatch between i-pi and rpmd input!";
     }
     return "unknown error";
 }

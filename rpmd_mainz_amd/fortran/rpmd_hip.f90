@@ -10,6 +10,8 @@
 !                                                               src/evolve_pi.f90:1
 !   evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,irun,nbaro)
 !                                                               src/evolve_rigid.f90:100
+!   run_driver(p,r,dvdr,dvdr2,na,nb,boxlxyz,z,beta,dt,mass,irun,itst,pt,pb,print)
+!                                                               src/ipi_driver.f90:1 (with sockets.c)
 ! so every existing call site compiles and links unchanged.  Model parameters are read from the
 ! same COMMON blocks the reference routines read (src/main.f90:53-72) on every call; the
 ! thermostat (module thermostat's therm and ttau, src/module_thermostat.f90:3-4) is handed over
@@ -203,6 +205,15 @@ module rpmd_hip
        integer(c_int), intent(out) :: batched, pair_cap, pair_beside
        real(c_double), intent(out) :: ms_batched, ms_separate
      end function c_batch_tuning
+     ! the i-PI client (run_driver)
+     integer(c_int) function c_ipi_run(ctx, address, z, na, served) bind(C, name='rpmd_ipi_run')
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), dimension(*), intent(in) :: address
+       real(c_double), intent(in) :: z(*)
+       integer(c_int), value :: na
+       integer(c_int), intent(out) :: served
+     end function c_ipi_run
   end interface
 
   type(c_ptr), save :: rpmd_ctx = c_null_ptr
@@ -455,3 +466,27 @@ subroutine evolve_rig_pi(p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box
   call rpmd_check(c_evolve_rig_pi(rpmd_ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, int(na, c_int), &
                                   int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn))
 end subroutine evolve_rig_pi
+
+
+subroutine run_driver(p, r, dvdr, dvdr2, na, nb, boxlxyz, z, beta, dt, mass, irun, itst, pt, pb, print)
+  ! the i-PI client: reads the server address from the file serverdata, as the reference does, and
+  ! serves the server with full_forces on one bead until it disconnects; p, r, dvdr, dvdr2, beta,
+  ! dt, mass, irun, itst, pt, pb and print are accepted for the reference's argument list
+  use rpmd_hip
+  implicit none
+  integer na, nb, irun, itst, pt, pb, print
+  real(8) p(3,na,nb), r(3,na,nb), dvdr(3,na,nb), dvdr2(3,na,nb), z(na), boxlxyz(3), beta, dt, mass(na)
+  character(1024) :: serveraddr
+  integer(c_int) :: served
+  if (nb .ne. 1) then
+     write(*,*) "i-PI mode must have nb = 1!"
+     stop
+  endif
+  write(*,*) "@ i-PI DRIVER BEING LOADED"
+  open(12, file="serverdata")
+  read(12, *) serveraddr
+  close(12)
+  write(*,*) "SERVERDATA ", trim(serveraddr)
+  call rpmd_bind(na, nb)
+  call rpmd_check(c_ipi_run(rpmd_ctx, trim(serveraddr)//c_null_char, z, int(na, c_int), served))
+end subroutine run_driver
