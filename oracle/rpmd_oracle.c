@@ -395,6 +395,8 @@ typedef struct {
     double rcutsq, alpha, sfac;
     int unit;         /* 1: unit box, nint(dx) image; 0: physical box */
     double box[3], onbox[3];
+    int smeared;      /* Gaussian charges: wid = {M-M, H-H, M-H} inverse widths */
+    double wid[3];
 } rw_ctx;
 
 static void rwald_pair(int i1, int j1, void* vctx) {
@@ -419,9 +421,23 @@ static void rwald_pair(int i1, int j1, void* vctx) {
         double e = exp(-x * x);
         double t = AS_P / (AS_P + x);
         double erfc_ = e * t * (AS_A1 + t * (AS_A2 + t * (AS_A3 + t * (AS_A4 + t * AS_A5))));
-        if (c->mol[i] == c->mol[j]) erfc_ = erfc_ - 1.0;
+        double sfacs = 0.0, es = 0.0;
+        if (c->mol[i] == c->mol[j]) {
+            erfc_ = erfc_ - 1.0;
+        } else if (c->smeared) {
+            /* rwald_smeared src/ewald_smeared.f90:71-94: the first site of a molecule is type 1 (M),
+             * the others type 2 (H); like types take wid(type), unlike ones wid(3) */
+            int ti = (i % 3) ? 2 : 1, tj = (j % 3) ? 2 : 1;
+            double wt = ti == tj ? c->wid[ti - 1] : c->wid[2];
+            double xs = dr * wt;
+            sfacs = wt * (-2.0 / sqrt(acos(-1.0)));
+            es = exp(-xs * xs);
+            double ts = AS_P / (AS_P + xs);
+            double erfcs = es * ts * (AS_A1 + ts * (AS_A2 + ts * (AS_A3 + ts * (AS_A4 + ts * AS_A5))));
+            erfc_ = erfc_ - erfcs;
+        }
         double du = erfc_ / dr;
-        double dur = (c->sfac * e - du) / drsq;
+        double dur = (c->sfac * e - sfacs * es - du) / drsq;
         double qij = c->q[i] * c->q[j];
         double dv = qij * du, dvr = qij * dur;
         double dvx = dvr * dx, dvy = dvr * dy, dvz = dvr * dz;
@@ -631,6 +647,14 @@ static void ewald_cubic(const rpmd_params* P, int n, const int* mol, const doubl
     double pi = acos(-1.0);
     c.sfac = -2.0 * (P->walpha / sqrt(pi));
     c.unit = 1;
+    if (P->wm > 0.0 || P->wh > 0.0) {
+        /* smeared charges, widths in unit-box lengths (src/ewald_cubic.f90:40-54) */
+        double sm = sqrt(1.0 / (2.0 * P->wm)) * onboxl, sh = sqrt(1.0 / (2.0 * P->wh)) * onboxl;
+        c.smeared = 1;
+        c.wid[0] = 1.0 / (2.0 * sm);
+        c.wid[1] = 1.0 / (2.0 * sh);
+        c.wid[2] = 1.0 / sqrt(2.0 * (sm * sm + sh * sh));
+    }
     cells_t C;
     memset(&C, 0, sizeof C);
     cell_list_unit(rs, n, ncell, mcell, &C);
@@ -765,7 +789,21 @@ static void ewald_nc(const rpmd_params* P, int n, const int* mol, const double* 
     double bmax = box[0];
     if (box[1] > bmax) bmax = box[1];
     if (box[2] > bmax) bmax = box[2];
-    if (3.0 * P->wrcut > bmax) {
+    if (P->wm > 0.0 || P->wh > 0.0) {
+        /* smeared charges: rwald_smeared_nc, always over cells (src/ewald_noncubic.f90:18-33,
+         * src/ewald_smeared.f90:219-453) */
+        double sm = sqrt(1.0 / (2.0 * P->wm)), sh = sqrt(1.0 / (2.0 * P->wh));
+        c.smeared = 1;
+        c.wid[0] = 1.0 / (2.0 * sm);
+        c.wid[1] = 1.0 / (2.0 * sh);
+        c.wid[2] = 1.0 / sqrt(2.0 * (sm * sm + sh * sh));
+        cells_t C;
+        memset(&C, 0, sizeof C);
+        cell_setup(box, P->wrcut, C.mc, C.nc, &C.nn);
+        cell_list(r, n, box, 1, &C);
+        cells_pairs_global(&C, rwald_pair, &c);
+        cells_free(&C);
+    } else if (3.0 * P->wrcut > bmax) {
         /* rwald_basic :229-312 */
         for (int i = 1; i <= n; i++)
             for (int j = 1; j <= i - 1; j++) rwald_pair(i, j, &c);
@@ -1196,7 +1234,6 @@ static int potenl_opt(const rpmd_params* P, const double* rin, double* v, double
     int nm = na / 3;
     if (3 * nm != na) return RPMD_ERR_NA_NOT_MULT3;
     if (iopt <= 2) {
-        if (P->wm > 0.0 || P->wh > 0.0) return RPMD_ERR_SMEARED;
         double alpha = P->alpha, alpha2 = 0.5 * (1.0 - alpha);
         int* mol = (int*)malloc(sizeof(int) * na);
         for (int j = 0; j < na; j++) mol[j] = j / 3 + 1;

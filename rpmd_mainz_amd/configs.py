@@ -38,6 +38,11 @@ MODELS = {
     "tip4p2005": dict(wmass=32831.2525, omass=29156.9471, hmass=1837.1527, qo=-1.1128, alpha=0.73612,
                       oo_sig=5.96946, oo_eps=2.95147e-4, oo_gam=0.0, thetad=104.52, reoh=1.8088465,
                       apot=0.0, bpot=0.0, alp=0.0, alpb=0.0, wm=0.0, wh=0.0),
+    # params/param_smear: Gaussian (smeared) M-site and hydrogen charges, widths wm, wh; Morse
+    # stretch and bend (alp, alpb > 0)
+    "smear": dict(wmass=32831.2525, omass=29156.9471, hmass=1837.1527, qo=-1.084, alpha=0.797,
+                  oo_sig=5.761, oo_eps=3.5211e-4, oo_gam=0.0, thetad=106.416, reoh=1.77956,
+                  apot=0.17, bpot=0.52, alp=1.22, alpb=0.37, wm=0.482, wh=0.686),
 }
 
 

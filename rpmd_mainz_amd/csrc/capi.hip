@@ -577,7 +577,6 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     double chain_scale = 1.0;
     bool fork = false;
     if (iopt <= 2) {
-        if (P.wm > 0.0 || P.wh > 0.0) return RPMD_ERR_SMEARED;
         RETCHK(build_kset(c, box));
         const int cubic = P.iamcub;
         const Box sb = cubic ? unit_box() : bx;
@@ -621,6 +620,14 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
                 launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr, fork ? EW_BPC : 0, ptab);
             else
                 launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
+        }
+        if (P.wm > 0.0 || P.wh > 0.0) {
+            // Gaussian charges: inverse widths of the M-M, H-H and M-H pairs, in unit-box lengths for
+            // the cubic form (src/ewald_cubic.f90:40-54; src/ewald_noncubic.f90:18-33)
+            const double ob = cubic ? 1.0 / box[0] : 1.0;
+            const double sm = sqrt(1.0 / (2.0 * P.wm)) * ob, sh = sqrt(1.0 / (2.0 * P.wh)) * ob;
+            const double wid[3] = {1.0 / (2.0 * sm), 1.0 / (2.0 * sh), 1.0 / sqrt(2.0 * (sm * sm + sh * sh))};
+            launch_ewald_smear(*vl, na, nb, sb, P.wrcut, wid, c->fpart, pr, sr);
         }
         if (oo_branch && !fork3) {
             oo_work(c->st2);
@@ -1373,7 +1380,7 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_REALFT_N: return "realft 1";
         case RPMD_ERR_REALFT_MODE: return "realft 3";
         case RPMD_ERR_EWC_CUTOFF: return " ewc_basic : rcut too large ";
-        case RPMD_ERR_SMEARED: return "smeared charges are not supported on this path";
+        case RPMD_ERR_SMEARED: return "ewc_correction not yet implemented for smeared charges";
         case RPMD_ERR_KMAX: return "Ewald kmax exceeds the device k tile (15)";
         case RPMD_ERR_ARG: return "invalid argument";
         case RPMD_ERR_HIP: return "HIP runtime error";

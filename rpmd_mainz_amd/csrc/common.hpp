@@ -195,6 +195,10 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
 template <typename T>
 void launch_ewald_real(const VList& v, int na, int nb, Box sb, double rcut, double alpha, double* grad, double* parts,
                        hipStream_t s, int bpc = 0, const PairTab* tab = nullptr);
+// smeared charges: adds the Gaussian-charge part of the real-space pairs to what launch_ewald_real
+// wrote (gradient slot and block sums); wid = inverse widths {M-M, H-H, M-H} in the box's units
+void launch_ewald_smear(const VList& v, int na, int nb, Box sb, double rcut, const double wid[3], double* grad,
+                        double* parts, hipStream_t s);
 int kspace_nchunk(int na);
 
 // Replica batches: one replica's inputs of the four dominant mixed-precision force kernels (the FP32

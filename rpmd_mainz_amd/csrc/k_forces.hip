@@ -1275,6 +1275,108 @@ template void launch_ewald_real<double>(const VList&, int, int, Box, double, dou
 template void launch_ewald_real<float>(const VList&, int, int, Box, double, double, double*, double*, hipStream_t,
                                        int, const PairTab*);
 
+// Smeared charges (src/ewald_smeared.f90:1-453): a pair of sites of different molecules interacts as
+// two Gaussian charges, erfc(alpha r)/r - erfc(w r)/r with the A&S erfc in both terms, w by the
+// sites' types (the first site of a molecule, M, and the two hydrogens, H): w0 for M-M, w1 for H-H,
+// w2 for M-H.  The point-charge part is the real-space kernel's; this kernel, run after it on the
+// same stream over the same list and block layout, adds the -erfc(w r)/r part in FP64 (both
+// precision modes) to each site's gradient and to the block's energy and virial sums:
+// du_s = -erfc(w r)/r, dur_s = (-sfac_w e^{-w^2 r^2} - du_s)/r^2, sfac_w = -2 w/sqrt(pi), the
+// reference's expressions split by linearity.  A site's VLS lanes walk its list entries n = sl,
+// sl + VLS, ... past its own-molecule ones; the lanes' sums are added in a fixed butterfly.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_ewald_smear(const double4* __restrict__ srt, const int* __restrict__ perm,
+                                                     const int* __restrict__ cnt, const int* __restrict__ nex,
+                                                     const int* __restrict__ nl, int na, int nb, double L0, double L1,
+                                                     double L2, double i0, double i1, double i2, double rc2,
+                                                     double w0, double w1, double w2, double* __restrict__ grad,
+                                                     double* __restrict__ parts) {
+    const int id = blockIdx.x;
+    const int m8 = id >> 3;
+    const int b = m8 % nb, sb = (id & 7) + 8 * (m8 / nb);
+    const int k = (sb * blockDim.x + threadIdx.x) / VLS, sl = threadIdx.x % VLS;
+    const double rfac = -2.0 / sqrt(acos(-1.0));
+    double tg[3] = {0.0, 0.0, 0.0}, tv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (k < na) {
+        const double4* X = srt + (size_t)b * (na + 1);
+        const double4 xi = X[k];
+        const bool mi = perm[k] % 3 == 0;
+        const int ntot = cnt[k];
+        // the lane's first entry at or past the own-molecule ones
+        int n = nex[k];
+        n += (sl - n % VLS + VLS) % VLS;
+        for (; n < ntot; n += VLS) {
+            const int j = nl[(((size_t)(n / VLG) * na + k) * VLS + sl) * 4 + (n / VLS) % 4];
+            const double4 xj = X[j];
+            double ex = xi.x - xj.x, ey = xi.y - xj.y, ez = xi.z - xj.z;
+            if (UNIT) {
+                ex -= rint(ex);
+                ey -= rint(ey);
+                ez -= rint(ez);
+            } else {
+                ex -= L0 * rint(ex * i0);
+                ey -= L1 * rint(ey * i1);
+                ez -= L2 * rint(ez * i2);
+            }
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            if (r2 >= rc2) continue;
+            const bool mj = perm[j] % 3 == 0;
+            const double wt = mi == mj ? (mi ? w0 : w1) : w2;
+            const double rinv = 1.0 / sqrt(r2);
+            const double xs = wt * (r2 * rinv);
+            const double es = exp(-xs * xs);
+            const double ts = AS_P / (AS_P + xs);
+            const double erfcs = es * ts * (AS_A1 + ts * (AS_A2 + ts * (AS_A3 + ts * (AS_A4 + ts * AS_A5))));
+            const double du = -erfcs * rinv;
+            const double dur = (-(wt * rfac) * es - du) * (rinv * rinv);
+            const double qij = xi.w * xj.w;
+            const double dvr = qij * dur;
+            const double fx = dvr * ex, fy = dvr * ey, fz = dvr * ez;
+            tg[0] += fx;
+            tg[1] += fy;
+            tg[2] += fz;
+            tv[0] += qij * du;
+            tv[1] += ex * fx;
+            tv[2] += ex * fy;
+            tv[3] += ex * fz;
+            tv[4] += ey * fy;
+            tv[5] += ey * fz;
+            tv[6] += ez * fz;
+        }
+    }
+    // every lane of a quad takes part (the quad shares k)
+#pragma unroll
+    for (int c = 0; c < 3; c++) tg[c] = quad_sum(tg[c]);
+#pragma unroll
+    for (int c = 0; c < 7; c++) tv[c] = quad_sum(tv[c]);
+    const bool own = k < na && sl == 0;
+    if (own) {
+        const int a = perm[k];
+        double* G = grad + (size_t)b * 3 * na;
+        G[a] += tg[0];
+        G[na + a] += tg[1];
+        G[2 * na + a] += tg[2];
+    }
+    // each pair is walked from both ends: half of the sums
+    double acc[7];
+#pragma unroll
+    for (int c = 0; c < 7; c++) acc[c] = own ? 0.5 * tv[c] : 0.0;
+    double w[10] = {acc[0], acc[1], acc[2], acc[3], acc[2], acc[4], acc[5], acc[3], acc[5], acc[6]};
+    __shared__ double blk[10];
+    block_reduce_store<10>(w, blk);
+    // the thread that wrote blk[t] adds it to the real-space kernel's sum of this block
+    if (threadIdx.x < 10) parts[((size_t)b * ewald_real_parts(na) + sb) * 10 + threadIdx.x] += blk[threadIdx.x];
+}
+
+void launch_ewald_smear(const VList& v, int na, int nb, Box sb, double rcut, const double wid[3], double* grad,
+                        double* parts, hipStream_t s) {
+    const bool unit = sb.L[0] == 1.0 && sb.L[1] == 1.0 && sb.L[2] == 1.0;
+    auto k = unit ? k_ewald_smear<true> : k_ewald_smear<false>;
+    k<<<ewald_real_parts(na) * nb, 256, 0, s>>>(v.srt, v.perm, v.cnt, v.nex, v.nl, na, nb, sb.L[0], sb.L[1], sb.L[2],
+                                                sb.inv[0], sb.inv[1], sb.inv[2], rcut * rcut, wid[0], wid[1], wid[2],
+                                                grad, parts);
+}
+
 // ------------------------------------------------------------------------------------------------
 // reciprocal-space Ewald (src/ewald_cubic.f90:235-533; src/ewald_noncubic.f90:314-457)
 //

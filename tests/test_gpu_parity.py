@@ -33,6 +33,9 @@ SYSTEMS = {
     "spcfw_model": ("qspcfw", (5, 5, 5), 2, (1, 1, 1), dict(intra_model=cf.INTRA_SPCFW)),
     "buck_list": ("qspcfw", (4, 4, 4), 4, (1, 1, 1), dict(oo_gam=13.5)),
     "buck_nc": ("qspcfw", (5, 5, 5), 1, (1, 1, 1.2), dict(oo_gam=13.5)),
+    "smear_cubic": ("smear", (4, 4, 4), 2, (1, 1, 1), {}),
+    "smear_nc": ("smear", (6, 6, 6), 3, (1, 1, 1.2), {}),
+    "smear_nc1": ("smear", (3, 3, 3), 2, (1, 1, 1.3), {}),
 }
 
 
@@ -618,4 +621,34 @@ def test_fortran_host_through_iso_c_binding(capi, oracle, tmp_path):
         assert v_f == v_c and np.array_equal(d_f, d_c)
         v_o, d_o, w_o = oracle.forces(P, r, box, z, iopt)
         assert abs(v_f - v_o) < 1e-11 * abs(v_o) and _rel(d_f, d_o) < 1e-11
+    ctx.close()
+
+
+def test_smeared_charges_step_and_tem_error(capi, oracle):
+    """Gaussian (smeared) charges through the RPMD step: three NVE evolve_pi steps of the param_smear
+    model (M-site, Morse stretch and bend) against the oracle's; the TEM correction with smeared
+    charges stops in the reference (src/correct_tem.f90:14-18) and returns RPMD_ERR_SMEARED."""
+    r, box, z, mass, prm = cf.water_system("smear", (4, 4, 4), 4)
+    na, nb = r.shape[1], r.shape[0]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(4711))
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    po, ro, do1, do2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    pg, rg, dg1, dg2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    for k in range(3):
+        print(f"case smeared NVE step {k}", flush=True)
+        o = oracle.evolve_pi(P, po, ro, do1, do2, dt, mass, box, z, beta)
+        po, ro, do1, do2 = o["p"], o["r"], o["dvdr"], o["dvdr2"]
+        g = ctx.evolve_pi(pg, rg, dg1, dg2, dt, mass, na, nb, box, z, beta)
+        assert abs(g["v"] - o["v"]) < 1e-10 * abs(o["v"])
+    assert _rel(rg, ro) < 1e-12
+    assert _rel(pg, po) < 1e-10
+    ctx.close()
+    r, box, z, mass, prm = cf.water_system("smear", (4, 4, 4), 2, sig_A=4.5)
+    ctx = capi.RPMDContext(prm, r.shape[1], 2, "fp64")
+    with pytest.raises(capi.RPMDError) as e:
+        ctx.forces(r, 2, r.shape[1], box, z, 5)
+    assert e.value.code == 9
     ctx.close()

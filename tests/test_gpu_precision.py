@@ -67,6 +67,8 @@ MIXED_SYSTEMS = {
     "spcfw_nc1": ("qspcfw", (6, 6, 6), 1, (1, 1, 1.2), {}),
     "spcfw_ecut": ("qspcfw", (6, 6, 6), 4, (1, 1, 1), dict(ecut=1e-6)),
     "buck_list": ("qspcfw", (4, 4, 4), 4, (1, 1, 1), dict(oo_gam=13.5)),
+    "smear_cubic": ("smear", (4, 4, 4), 2, (1, 1, 1), {}),
+    "smear_nc": ("smear", (6, 6, 6), 3, (1, 1, 1.2), {}),
 }
 
 

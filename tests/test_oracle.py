@@ -30,6 +30,9 @@ SYSTEMS = [
     ("qspcfw", (5, 5, 5), 2, (1, 1, 1), dict(intra_model=cf.INTRA_SPCFW)),
     ("qspcfw", (4, 4, 4), 4, (1, 1, 1), dict(oo_gam=13.5)),              # Buckingham exp-6, list
     ("qspcfw", (5, 5, 5), 1, (1, 1, 1.2), dict(oo_gam=13.5)),            # Buckingham, cell/basic
+    ("smear", (4, 4, 4), 2, (1, 1, 1), {}),                              # smeared charges, cubic
+    ("smear", (6, 6, 6), 1, (1, 1, 1.2), {}),                           # smeared, non-cubic cells
+    ("smear", (3, 3, 3), 2, (1, 1, 1.3), {}),                           # smeared, non-cubic 1 cell
 ]
 
 
