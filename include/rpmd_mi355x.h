@@ -43,6 +43,7 @@ extern "C" {
 #define RPMD_ERR_TTAU             18  /* a thermostat is selected but ttau <= 0               */
 #define RPMD_ERR_SOCKET           19  /* i-PI socket: no server at the address, or a broken stream */
 #define RPMD_ERR_IPI_NAT          20  /* i-PI POSDATA with another atom count than the context's */
+#define RPMD_ERR_BARO             21  /* barostat neither BER nor MCI                          */
 
 /* ---- precision modes ------------------------------------------------------------------ */
 #define RPMD_PREC_FP64   0  /* everything FP64 (the "plain FP64 GPU form")                   */
@@ -158,6 +159,34 @@ int rpmd_evolve_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* vew, 
                    double* vint, double* dvdr, double* dvdr2, double dt, const double* mass,
                    int na, int nb, const double boxlxyz[3], const double* z, double beta,
                    double* vir, double* vir_lf, const double* noise, long long n_noise);
+
+/* module barostat (src/module_barostat.f90:1-5; set up at src/main.f90:257-279): the barostat of
+ * evolve_pi steps with nbaro = 1.  baro: RPMD_BARO_BER (Berendsen: taub = 41.341373 taufs, the
+ * coupling time in a.u.) or RPMD_BARO_MCI (Monte Carlo volume moves: taub = taufs, one attempt per
+ * taub steps on average; dv the largest volume change in bohr^3, the reference's 200); patm the
+ * pressure in a.u. (pres / 2.9421e8 bar).  Resets the MC counters nacc, nmove. */
+#define RPMD_BARO_NONE 0
+#define RPMD_BARO_BER  1
+#define RPMD_BARO_MCI  2
+int rpmd_set_barostat(rpmd_ctx* ctx, int baro, double taub, double patm, double dv);
+int rpmd_barostat_counts(const rpmd_ctx* ctx, long long* nacc, long long* nmove);
+
+/* subroutine evolve_pi(...,irun,nbaro) with nbaro = 1              src/evolve_pi.f90:163-189
+ * As rpmd_evolve_pi, and after the multiple time steps the barostat of rpmd_set_barostat acts on
+ * the bead positions and the box (boxlxyz in/out), which the low-frequency forces of the step then
+ * see.  vir_lf in: the previous step's low-frequency virial (vir = vir_lf + the step's mean
+ * high-frequency virial drives BER); out: this step's.  BER: isotropic pressure in a cubic box
+ * (src/module_barostat.f90:87-122), per axis otherwise (:28-85; the reference's beren_baro_nc reads
+ * a local patm that hides the module's, :34, so its target is undefined there: the module's patm is
+ * used here).  MCI (:124-220): uniforms are the ran2 draws in the reference's order (u in [0,1),
+ * then, when u < 1/taub, v in [-1,1) and w in [0,1)), n_uniforms of them; NULL draws from the
+ * library's own stream.  Every molecule moves by its center_water sum (the sum, not the mean, of
+ * its three atoms' bead-averaged positions, src/md_tools.f90:367-390) times (scale - 1). */
+int rpmd_evolve_pi_npt(rpmd_ctx* ctx, double* p, double* r, double* v, double* vew, double* vlj,
+                       double* vint, double* dvdr, double* dvdr2, double dt, const double* mass,
+                       int na, int nb, double boxlxyz[3], const double* z, double beta,
+                       double* vir, double* vir_lf, const double* noise, long long n_noise,
+                       const double* uniforms, int n_uniforms);
 
 /* subroutine evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,
  *                          irun,nbaro)                          src/evolve_rigid.f90:100-234

@@ -81,6 +81,10 @@ class Oracle:
         L.orc_full_forces.restype = C.c_int
         L.orc_realft.restype = C.c_int
         L.orc_evolve_pi.restype = C.c_int
+        L.orc_evolve_pi_npt.restype = C.c_int
+        L.orc_mc_baro.restype = C.c_int
+        L.orc_pressure.restype = C.c_double
+        L.orc_pres_lj_tail.restype = C.c_double
         L.orc_gaussian.restype = C.c_double
         L.orc_gaussian.argtypes = [C.c_void_p, C.c_double]
         L.orc_ran2.restype = C.c_double
@@ -297,6 +301,77 @@ class Oracle:
             raise RuntimeError(f"orc_evolve_pi error {err}")
         return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, vew=vew.value, vlj=vlj.value,
                     vint=vint.value, vir=vir.reshape(3, 3).T.copy(), vir_lf=vir_lf.reshape(3, 3).T.copy())
+
+    def evolve_pi_npt(self, prm: RpmdParams, p, r, dvdr, dvdr2, dt, mass, box, z, beta, vir_lf, baro: "OrcBaro",
+                      s: RngState | None = None, g: GleState | None = None):
+        """evolve_pi with nbaro = 1: baro is updated (MC counters); vir_lf (3x3) is the previous step's;
+        returns the step's arrays, energies, box and virials"""
+        p, r = _f64(p).copy(), _f64(r).copy()
+        dvdr, dvdr2 = _f64(dvdr).copy(), _f64(dvdr2).copy()
+        nb, na = r.shape[0], r.shape[1]
+        b = _f64(box).copy()
+        vir, vl = np.zeros(9), np.asarray(vir_lf, dtype=np.float64).T.reshape(9).copy()
+        v, vew, vlj, vint = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        if s is None:
+            s = RngState()
+            self.lib.orc_rng_init(C.byref(s))
+        err = self.lib.orc_evolve_pi_npt(C.byref(prm), _d(p), _d(r), C.byref(v), C.byref(vew), C.byref(vlj),
+                                         C.byref(vint), _d(dvdr), _d(dvdr2), C.c_double(dt), _d(_f64(mass)),
+                                         C.c_int(na), C.c_int(nb), _d(b), _d(_f64(z)), C.c_double(beta), _d(vir),
+                                         _d(vl), C.byref(s), C.byref(g) if g is not None else None, C.byref(baro))
+        if err:
+            raise RuntimeError(f"orc_evolve_pi_npt error {err}")
+        return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, vew=vew.value, vlj=vlj.value, box=b,
+                    vint=vint.value, vir=vir.reshape(3, 3).T.copy(), vir_lf=vl.reshape(3, 3).T.copy())
+
+    # -- barostat helpers ----------------------------------------------------------------------
+    def virial_ke(self, r, dvdr, dvdr2, beta, nctot=0):
+        r = _f64(r)
+        nb, na = r.shape[0], r.shape[1]
+        tv, tq1, tq2 = C.c_double(), C.c_double(), C.c_double()
+        tvxyz = np.zeros(3)
+        self.lib.orc_virial_ke(_d(r), _d(_f64(dvdr)), _d(_f64(dvdr2)), C.byref(tv), _d(tvxyz), C.byref(tq1),
+                               C.byref(tq2), C.c_double(beta), C.c_int(na), C.c_int(nb), C.c_int(nctot))
+        return tv.value, tvxyz, tq1.value, tq2.value
+
+    def pressure(self, prm: RpmdParams, vir, tv, na, box):
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        return self.lib.orc_pressure(C.byref(prm), _d(w), C.c_double(tv), C.c_int(na), _d(_f64(box)))
+
+    def center_water(self, r):
+        r = _f64(r)
+        nb, nm = r.shape[0], r.shape[1] // 3
+        pc = np.zeros((nm, 3))
+        self.lib.orc_center_water(_d(r), _d(pc), C.c_int(nm), C.c_int(nb))
+        return pc
+
+    def beren_driver(self, prm: RpmdParams, baro: "OrcBaro", vir, tv, tvxyz, dt, r, box):
+        r, b = _f64(r).copy(), _f64(box).copy()
+        nb, na = r.shape[0], r.shape[1]
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        self.lib.orc_beren_driver(C.byref(prm), C.byref(baro), _d(w), C.c_double(tv), _d(_f64(tvxyz)),
+                                  C.c_double(dt), _d(r), _d(b), C.c_int(na), C.c_int(nb))
+        return r, b
+
+    def mc_baro(self, prm: RpmdParams, baro: "OrcBaro", r, dvdr, dvdr2, vir, z, beta, box, s: RngState):
+        r, g1, g2, b = _f64(r).copy(), _f64(dvdr).copy(), _f64(dvdr2).copy(), _f64(box).copy()
+        nb, na = r.shape[0], r.shape[1]
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        eo = C.c_double(0.0)
+        err = self.lib.orc_mc_baro(C.byref(prm), C.byref(baro), _d(r), _d(g1), _d(g2), _d(w), C.byref(eo),
+                                   _d(_f64(z)), C.c_double(beta), _d(b), C.c_int(na), C.c_int(nb), C.byref(s))
+        if err:
+            raise RuntimeError(f"orc_mc_baro error {err}")
+        return dict(r=r, dvdr=g1, dvdr2=g2, vir=w.reshape(3, 3).T.copy(), eo=eo.value, box=b)
+
+    def ran2(self, s: RngState, lo: float, hi: float) -> float:
+        return self.lib.orc_ran2(C.byref(s), C.c_double(lo), C.c_double(hi))
+
+
+class OrcBaro(C.Structure):
+    """module barostat (src/module_barostat.f90:1-5): baro 1 BER, 2 MCI"""
+    _fields_ = [("baro", C.c_int), ("taub", C.c_double), ("patm", C.c_double), ("dv", C.c_double),
+                ("nacc", C.c_int), ("nmove", C.c_int)]
 
 
 class RefPath:
@@ -521,6 +596,64 @@ class RefPath:
         self.lib.virial_ke_(_d(r), _d(g1), _d(g2), C.byref(tv), _d(tvxyz), C.byref(tq1), C.byref(tq2),
                             C.byref(C.c_double(beta)), C.byref(C.c_int(na)), C.byref(C.c_int(nb)))
         return tv.value, tvxyz, tq1.value, tq2.value
+
+    def set_barostat(self, baro: str, taub: float, patm: float, dv: float, nacc: int = 0, nmove: int = 0):
+        """the variables of module barostat (src/module_barostat.f90:2-4)"""
+        L = self.lib
+        (C.c_char * 3).in_dll(L, "_QMbarostatEbaro").raw = baro.encode()[:3].ljust(3)
+        C.c_double.in_dll(L, "_QMbarostatEtaub").value = taub
+        C.c_double.in_dll(L, "_QMbarostatEpatm").value = patm
+        C.c_double.in_dll(L, "_QMbarostatEdv").value = dv
+        C.c_int.in_dll(L, "_QMbarostatEnacc").value = nacc
+        C.c_int.in_dll(L, "_QMbarostatEnmove").value = nmove
+
+    def barostat_counts(self):
+        return (C.c_int.in_dll(self.lib, "_QMbarostatEnacc").value,
+                C.c_int.in_dll(self.lib, "_QMbarostatEnmove").value)
+
+    def pressure(self, prm: RpmdParams, vir, tv, na, box):
+        """pressure (src/md_pressure.f90:1-35)"""
+        self.set_params(prm)
+        pres = C.c_double()
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        self.lib.pressure_(C.byref(pres), _d(w), C.byref(C.c_double(tv)), C.byref(C.c_int(na)), _d(_f64(box).copy()))
+        return pres.value
+
+    def center_water(self, r):
+        """center_water (src/md_tools.f90:367-390)"""
+        r = _f64(r).copy()
+        nb, nm = r.shape[0], r.shape[1] // 3
+        pc = np.zeros((nm, 3))
+        self.lib.center_water_(_d(r), _d(pc), C.byref(C.c_int(nm)), C.byref(C.c_int(nb)))
+        return pc
+
+    def beren_driver(self, prm: RpmdParams, vir, tv, tvxyz, dt, r, box):
+        """barostat::beren_driver (src/module_barostat.f90:8-26)"""
+        self.set_params(prm)
+        r, b = _f64(r).copy(), _f64(box).copy()
+        nb, na = r.shape[0], r.shape[1]
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        self.lib._QMbarostatPberen_driver(_d(w), C.byref(C.c_double(tv)), _d(_f64(tvxyz).copy()),
+                                          C.byref(C.c_double(dt)), _d(r), _d(b), C.byref(C.c_int(na)),
+                                          C.byref(C.c_int(nb)))
+        return r, b
+
+    def mc_baro(self, prm: RpmdParams, r, dvdr, dvdr2, vir, z, beta, box):
+        """barostat::mc_baro (src/module_barostat.f90:124-220) on the current ran2 stream (no RP
+        contraction here: the reference's full_forces would call realft)"""
+        self.set_params(prm)
+        r, g1, g2, b = _f64(r).copy(), _f64(dvdr).copy(), _f64(dvdr2).copy(), _f64(box).copy()
+        nb, na = r.shape[0], r.shape[1]
+        w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
+        eo = C.c_double(0.0)
+        (C.c_int * 2).in_dll(self.lib, "beaddiabatic_")[:] = [prm.nbdf1, prm.nbdf2]
+        self.lib._QMbarostatPmc_baro(_d(r), _d(g1), _d(g2), _d(w), C.byref(eo), _d(_f64(z).copy()),
+                                     C.byref(C.c_double(beta)), _d(b), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),
+                                     C.byref(self._idum))
+        return dict(r=r, dvdr=g1, dvdr2=g2, vir=w.reshape(3, 3).T.copy(), eo=eo.value, box=b)
+
+    def ran2(self, lo: float, hi: float) -> float:
+        return self.lib.ran2_(C.byref(self._idum), C.byref(C.c_double(lo)), C.byref(C.c_double(hi)))
 
     def kinetic_rpmd(self, p, mass):
         """kinetic_rpmd (src/kinetic.f90:248-265)"""

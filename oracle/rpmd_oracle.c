@@ -1900,10 +1900,211 @@ void orc_free_ring(double* p, double* r, const double* monod, int na, int nb) {
         }
 }
 
+/* ======================================================================================
+ * Barostats (src/module_barostat.f90) and their helpers: virial_ke (src/kinetic.f90:1-54),
+ * pressure and pres_lj_tail (src/md_pressure.f90), center_water (src/md_tools.f90:367-390)
+ * ==================================================================================== */
+
+/* virial_ke src/kinetic.f90:1-54: centroid-virial kinetic energy, nc = 3 */
+void orc_virial_ke(const double* r, const double* dvdr, const double* dvdr2, double* tv, double tvxyz[3],
+                   double* tq1o, double* tq2o, double beta, int na, int nb, int nctot) {
+    size_t A = (size_t)3 * na;
+    double* rc = (double*)calloc(A, sizeof(double));
+    for (int k = 0; k < nb; k++)
+        for (size_t i = 0; i < A; i++) rc[i] = rc[i] + r[(size_t)k * A + i];
+    for (size_t i = 0; i < A; i++) rc[i] = rc[i] / (double)nb;
+    double tq1 = 0.0, tq2 = 0.0, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+    for (int k = 0; k < nb; k++)
+        for (int j = 0; j < na; j++)
+            for (int i = 0; i < 3; i++) {
+                size_t o = (size_t)k * A + 3 * (size_t)j + i;
+                double dr = r[o] - rc[3 * (size_t)j + i];
+                double a = dr * dvdr[o], b = dr * dvdr2[o];
+                tq1 = tq1 + a;
+                tq2 = tq2 + b;
+                t1[i] = t1[i] + a;
+                t2[i] = t2[i] + b;
+            }
+    int nc = 3;
+    tq1 = 0.5 * tq1 / (double)nb;
+    tq2 = 0.5 * tq2 / (double)nb;
+    for (int i = 0; i < 3; i++) {
+        t1[i] = 0.5 * t1[i] / (double)nb;
+        t2[i] = 0.5 * t2[i] / (double)nb;
+    }
+    double tc = 0.5 * (double)(3 * na - nc - nctot) / beta;
+    *tv = tc + tq1 + tq2;
+    for (int i = 0; i < 3; i++) tvxyz[i] = tc / 3.0 + t1[i] + t2[i];
+    if (tq1o) *tq1o = tq1;
+    if (tq2o) *tq2o = tq2;
+    free(rc);
+}
+
+/* pres_lj_tail src/md_pressure.f90:37-56 */
+double orc_pres_lj_tail(const rpmd_params* P, const double box[3], int na) {
+    int nm = na / 3;
+    double pi = acos(-1.0);
+    double vol = box[0] * box[1] * box[2];
+    double rho = (double)nm / vol;
+    double prefac = (16.0 * pi * (rho * rho) * P->oo_eps * (P->oo_sig * P->oo_sig * P->oo_sig)) / 3.0;
+    double x = P->oo_sig / P->rcut, x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x8 = x4 * x4, x9 = x8 * x;
+    return prefac * ((2.0 / 3.0) * x9 - x3);
+}
+
+/* pressure src/md_pressure.f90:1-35 (rpmddft = 0) */
+double orc_pressure(const rpmd_params* P, const double vir[9], double tv, int na, const double box[3]) {
+    double w = VIR(vir, 0, 0) + VIR(vir, 1, 1) + VIR(vir, 2, 2);
+    w = -w / 3.0;
+    double vol = box[0] * box[1] * box[2];
+    double pres = (2.0 * tv / 3.0 + w) / vol;
+    double ptail = P->oo_gam == 0.0 ? orc_pres_lj_tail(P, box, na) : 0.0;
+    return pres + ptail;
+}
+
+/* center_water src/md_tools.f90:367-390: per molecule the bead average of the SUM of its three
+ * atoms' coordinates (not their mean) */
+void orc_center_water(const double* r, double* pc, int nm, int nb) {
+    double wt = 1.0 / (double)nb;
+    size_t A = (size_t)9 * nm;
+    for (int j = 0; j < nm; j++) {
+        double c[3] = {0, 0, 0};
+        for (int k = 0; k < nb; k++) {
+            const double* x = r + (size_t)k * A + 9 * (size_t)j;
+            for (int d = 0; d < 3; d++) c[d] = c[d] + x[d] + x[3 + d] + x[6 + d];
+        }
+        for (int d = 0; d < 3; d++) pc[3 * j + d] = c[d] * wt;
+    }
+}
+
+/* every atom of molecule i in every bead moves by rcm(:, i) (scale(:) - 1) (src/module_barostat.f90:66-79,
+ * 107-120, 176-190) */
+static void shift_molecules(double* r, const double* rcm, const double sc[3], int nm, int nb) {
+    size_t A = (size_t)9 * nm;
+    for (int i = 0; i < nm; i++)
+        for (int k = 0; k < nb; k++)
+            for (int a = 0; a < 3; a++)
+                for (int d = 0; d < 3; d++) {
+                    double* x = r + (size_t)k * A + 9 * (size_t)i + 3 * a + d;
+                    *x = *x + rcm[3 * i + d] * (sc[d] - 1.0);
+                }
+}
+
+/* beren_driver, beren_baro, beren_baro_nc src/module_barostat.f90:8-122 (compi = 13531, the isothermal
+ * compressibility of water in atomic units; rpmddft = 0) */
+void orc_beren_driver(const rpmd_params* P, const orc_baro* B, const double vir[9], double tv, const double tvxyz[3],
+                      double dt, double* r, double box[3], int na, int nb) {
+    int nm = na / 3;
+    double compi = 13531.0, bstat = compi / (3.0 * B->taub), sc[3];
+    if (P->iamcub) {
+        double pres = orc_pressure(P, vir, tv, na, box);
+        double scale = fabs(1.0 - (dt * bstat) * (B->patm - pres));
+        for (int d = 0; d < 3; d++) {
+            box[d] = scale * box[d];
+            sc[d] = scale;
+        }
+    } else {
+        double ptail = P->oo_gam == 0.0 ? orc_pres_lj_tail(P, box, na) : 0.0;
+        double vol = box[0] * box[1] * box[2], pres[3];
+        for (int j = 0; j < 3; j++) {
+            pres[j] = (-VIR(vir, j, j) + 2.0 * tvxyz[j]) / vol;
+            pres[j] = pres[j] + ptail;
+        }
+        for (int d = 0; d < 3; d++) {
+            sc[d] = fabs(1.0 - (dt * bstat) * (B->patm - pres[d]));
+            box[d] = sc[d] * box[d];
+        }
+    }
+    double* rcm = (double*)malloc(sizeof(double) * 3 * (nm > 0 ? nm : 1));
+    orc_center_water(r, rcm, nm, nb);
+    shift_molecules(r, rcm, sc, nm, nb);
+    free(rcm);
+}
+
+/* full_forces as mc_baro sees it: with contraction the reference transforms its position argument
+ * to normal modes and back in place (src/md_forces.f90:357,386), which mc_baro's caller keeps */
+static int full_forces_inplace(const rpmd_params* P, double* r, int na, int nb, double* v, double* vir,
+                               const double* z, const double box[3], double* dvdr, double* dvdr2) {
+    double vew, voo, vint;
+    int err = orc_full_forces(P, r, na, nb, v, &vew, &voo, &vint, vir, z, box, dvdr, dvdr2);
+    if (!err && nb > 1 && (P->nbdf1 > 0 || P->nbdf2 > 0)) {
+        orc_realft(r, 3 * na, nb, +1);
+        orc_realft(r, 3 * na, nb, -1);
+    }
+    return err;
+}
+
+/* mc_baro src/module_barostat.f90:124-220: with probability 1/taub a random volume change
+ * vn = vo + dv u(-1,1), molecules scaled by (vn/vo)^(1/3), accepted when
+ * exp(-beta ((en - eo) + patm (vn - vo) - nm ln(vn/vo) / beta)) > u(0,1); the second definition of
+ * the test (:190) is the one in effect */
+int orc_mc_baro(const rpmd_params* P, orc_baro* B, double* r, double* dvdr, double* dvdr2, double* vir, double* eo,
+                const double* z, double beta, double box[3], int na, int nb, orc_rng* s) {
+    int nm = na / 3, err = 0;
+    double thresh = 1.0 / B->taub;
+    if (!(orc_ran2(s, 0.0, 1.0) < thresh)) return 0;
+    size_t N = (size_t)3 * na * nb;
+    err = full_forces_inplace(P, r, na, nb, eo, vir, z, box, dvdr, dvdr2);
+    if (err) return err;
+    B->nmove = B->nmove + 1;
+    double vo = box[0] * box[1] * box[2];
+    double vn = vo + B->dv * orc_ran2(s, -1.0, 1.0);
+    double scale = pow(vn / vo, 1.0 / 3.0), boxn[3], sc[3] = {scale, scale, scale};
+    for (int d = 0; d < 3; d++) boxn[d] = scale * box[d];
+    double* rcm = (double*)malloc(sizeof(double) * 3 * (nm > 0 ? nm : 1));
+    double* rn = (double*)malloc(sizeof(double) * N);
+    double* dvdrn = (double*)malloc(sizeof(double) * N);
+    double* dvdr2n = (double*)malloc(sizeof(double) * N);
+    orc_center_water(r, rcm, nm, nb);
+    memcpy(rn, r, sizeof(double) * N);
+    shift_molecules(rn, rcm, sc, nm, nb);
+    double en, virn[9];
+    err = full_forces_inplace(P, rn, na, nb, &en, virn, z, boxn, dvdrn, dvdr2n);
+    if (!err) {
+        double test = (en - *eo) + B->patm * (vn - vo) - (double)nm * log(vn / vo) / beta;
+        test = exp(-beta * test);
+        int accept = test > orc_ran2(s, 0.0, 1.0);
+        if (accept) {
+            B->nacc = B->nacc + 1;
+            *eo = en;
+            memcpy(box, boxn, sizeof boxn);
+            memcpy(r, rn, sizeof(double) * N);
+            memcpy(dvdr, dvdrn, sizeof(double) * N);
+            memcpy(dvdr2, dvdr2n, sizeof(double) * N);
+            memcpy(vir, virn, sizeof virn);
+        }
+    }
+    free(rcm); free(rn); free(dvdrn); free(dvdr2n);
+    return err;
+}
+
+static int evolve_pi_core(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
+                          double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                          double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
+                          orc_gle* g, orc_baro* B);
+
 int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
                   double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
                   const double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
                   orc_gle* g) {
+    double b[3] = {box[0], box[1], box[2]};
+    return evolve_pi_core(P, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, b, z, beta, vir, vir_lf, s, g,
+                          NULL);
+}
+
+/* evolve_pi with nbaro = 1: box and vir_lf in/out (vir = vir_lf + the step's high-frequency virial
+ * feeds the barostat, src/evolve_pi.f90:163-189) */
+int orc_evolve_pi_npt(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
+                      double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                      double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
+                      orc_gle* g, orc_baro* B) {
+    return evolve_pi_core(P, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, s, g,
+                          B);
+}
+
+static int evolve_pi_core(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
+                          double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                          double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
+                          orc_gle* g, orc_baro* B) {
     size_t A = (size_t)3 * na, N = A * nb;
     int mts = P->mts, nbdf1 = P->nbdf1, nbdf2 = P->nbdf2, err = 0;
     double* dvdre = (double*)calloc(N, sizeof(double));
@@ -1911,7 +2112,7 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
     double* dvdrl = (double*)calloc(N, sizeof(double));
     double* monod = (double*)calloc((size_t)12 * nb, sizeof(double));
     double vir_itr[9] = {0}, vir_lj[9] = {0}, vir_ew[9] = {0}, vir_hf[9] = {0}, vir_ewc[9] = {0};
-    double ve = 0.0;
+    double ve = 0.0, tv = 0.0, tvxyz[3] = {0, 0, 0};
     *vew = 0.0; *vlj = 0.0; *vint = 0.0;
     double dtsmall = dt / (double)mts, halfdtsmall = 0.5 * dtsmall, halfdt = 0.5 * dt;
     orc_ring_rpmd(mass, monod, na, nb, dtsmall, beta);
@@ -1929,13 +2130,37 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
         memcpy(rst, r, sizeof(double) * N);
         orc_realft(rst, (int)A, nb, -1);
         err = orc_forces(P, rst, nb, na, box, z, 4, vint, dvdr2, vir_itr);
+        if (B) {
+            double tv_itr, tvxyz_itr[3];
+            orc_virial_ke(rst, dvdr, dvdr2, &tv_itr, tvxyz_itr, NULL, NULL, beta, na, nb, 0);
+            tv = tv + tv_itr;
+            for (int d = 0; d < 3; d++) tvxyz[d] = tvxyz[d] + tvxyz_itr[d];
+        }
         for (int m = 0; m < 9; m++) vir_hf[m] = vir_hf[m] + vir_itr[m];
         if (im != mts) {
             orc_realft(dvdr2, (int)A, nb, +1);
             for (size_t i = 0; i < N; i++) p[i] = p[i] - dtsmall * dvdr2[i];
         }
     }
+    tv = tv / (double)mts;
+    for (int d = 0; d < 3; d++) tvxyz[d] = tvxyz[d] / (double)mts;
     for (int m = 0; m < 9; m++) vir_hf[m] = vir_hf[m] / (double)mts;
+    if (!err && B) {
+        /* barostat on the bead positions, then the normal modes of the scaled positions
+         * (src/evolve_pi.f90:163-189) */
+        double vb[9];
+        for (int m = 0; m < 9; m++) vb[m] = vir_lf[m] + vir_hf[m];
+        if (B->baro == ORC_BARO_BER) {
+            orc_beren_driver(P, B, vb, tv, tvxyz, dt, rst, box, na, nb);
+        } else if (B->baro == ORC_BARO_MCI) {
+            double vmc;
+            err = orc_mc_baro(P, B, rst, dvdr, dvdr2, vb, &vmc, z, beta, box, na, nb, s);
+        } else {
+            err = -1;
+        }
+        memcpy(r, rst, sizeof(double) * N);
+        orc_realft(r, (int)A, nb, +1);
+    }
     if (!err && nbdf1 > 0) {
         err = orc_rp_contract_nm(P, r, vew, dvdr, nb, na, box, z, vir_ew, nbdf1, 2);
         if (!err && P->sig > 0.0) {

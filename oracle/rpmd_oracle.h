@@ -70,6 +70,28 @@ void orc_therm_gle(orc_gle* g, double* p, const double* mass, int na, int nb, or
 void orc_matrix_exp(const double* M, int n, int j, int k, double* EM);
 void orc_cholesky(const double* SST, double* S, int n);
 
+/* module barostat (src/module_barostat.f90:1-5): baro, taub, patm, dv and the MC counters */
+#define ORC_BARO_BER 1
+#define ORC_BARO_MCI 2
+typedef struct orc_baro {
+    int baro;
+    double taub, patm, dv;
+    int nacc, nmove;
+} orc_baro;
+void   orc_virial_ke(const double* r, const double* dvdr, const double* dvdr2, double* tv, double tvxyz[3],
+                     double* tq1, double* tq2, double beta, int na, int nb, int nctot);
+double orc_pres_lj_tail(const rpmd_params* P, const double box[3], int na);
+double orc_pressure(const rpmd_params* P, const double vir[9], double tv, int na, const double box[3]);
+void   orc_center_water(const double* r, double* pc, int nm, int nb);
+void   orc_beren_driver(const rpmd_params* P, const orc_baro* B, const double vir[9], double tv,
+                        const double tvxyz[3], double dt, double* r, double box[3], int na, int nb);
+int    orc_mc_baro(const rpmd_params* P, orc_baro* B, double* r, double* dvdr, double* dvdr2, double* vir,
+                   double* eo, const double* z, double beta, double box[3], int na, int nb, orc_rng* s);
+int orc_evolve_pi_npt(const rpmd_params* P, double* p, double* r, double* v, double* vew, double* vlj,
+                      double* vint, double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                      double box[3], const double* z, double beta, double* vir, double* vir_lf, orc_rng* s,
+                      orc_gle* g, orc_baro* B);
+
 int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double* vew,
                   double* vlj, double* vint, double* dvdr, double* dvdr2, double dt,
                   const double* mass, int na, int nb, const double box[3], const double* z,

@@ -70,7 +70,8 @@ def lib() -> C.CDLL:
                      "rpmd_prof_begin", "rpmd_prof_end", "rpmd_gle_init", "rpmd_evolve_rig_pi",
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
                      "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators",
-                     "rpmd_batch_tuning", "rpmd_ipi_run"):
+                     "rpmd_batch_tuning", "rpmd_ipi_run", "rpmd_set_barostat", "rpmd_barostat_counts",
+                     "rpmd_evolve_pi_npt"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -92,7 +93,7 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
-def run_driver(ctx: "Context", na: int, nb: int, z, serverdata: str = "serverdata") -> int:
+def run_driver(ctx: "RPMDContext", na: int, nb: int, z, serverdata: str = "serverdata") -> int:
     """subroutine run_driver (src/ipi_driver.f90:1-254): nb must be 1; the server address is the first
     whitespace-separated word of the file serverdata, as the reference reads it (list-directed)."""
     if nb != 1:
@@ -225,6 +226,52 @@ class RPMDContext:
                                     _p(vir), _p(vir_lf), npt, nn))
         return dict(v=v.value, vew=vew.value, vlj=vlj.value, vint=vint.value, vir=vir.reshape(3, 3).T.copy(),
                     vir_lf=vir_lf.reshape(3, 3).T.copy())
+
+    # -- module barostat (src/module_barostat.f90:1-5) and evolve_pi with nbaro = 1 ----------------
+    BARO = {"NONE": 0, "BER": 1, "MCI": 2}
+
+    def set_barostat(self, baro: str, taub: float, patm: float, dv: float = 200.0):
+        """baro 'BER' (taub = 41.341373 taufs) or 'MCI' (taub = taufs, dv in bohr^3); patm in a.u."""
+        if baro not in self.BARO:
+            raise RPMDError(21, " ** Invalid barostat **")
+        _check(lib().rpmd_set_barostat(self._h, C.c_int(self.BARO[baro]), C.c_double(taub), C.c_double(patm),
+                                       C.c_double(dv)))
+
+    def barostat_counts(self):
+        """(nacc, nmove) of the MC barostat"""
+        a, m = C.c_longlong(), C.c_longlong()
+        _check(lib().rpmd_barostat_counts(self._h, C.byref(a), C.byref(m)))
+        return a.value, m.value
+
+    def evolve_pi_npt(self, p, r, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir_lf, noise=None,
+                      uniforms=None):
+        """One evolve_pi step with nbaro = 1; p, r, dvdr, dvdr2 updated in place; vir_lf (3x3) the previous
+        step's low-frequency virial; uniforms: the MC barostat's ran2 draws in order, or None for the
+        library's stream.  Returns energies, virials and the new box."""
+        for a in (p, r, dvdr, dvdr2):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.shape == (nb, na, 3)):
+                raise TypeError("evolve_pi state arrays must be C-contiguous float64 (nb, na, 3)")
+        vir = np.zeros(9)
+        vl = np.ascontiguousarray(np.asarray(vir_lf, dtype=np.float64).T).reshape(9).copy()
+        box = _arr(boxlxyz, (3,)).copy()
+        v, vew, vlj, vint = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        if noise is not None:
+            noise = _arr(noise)
+            npt, nn = _p(noise), C.c_longlong(noise.size)
+        else:
+            npt, nn = None, C.c_longlong(0)
+        if uniforms is not None:
+            uniforms = _arr(uniforms)
+            up, nu = _p(uniforms), C.c_int(uniforms.size)
+        else:
+            up, nu = None, C.c_int(0)
+        _check(lib().rpmd_evolve_pi_npt(self._h, _p(p), _p(r), C.byref(v), C.byref(vew), C.byref(vlj), C.byref(vint),
+                                        _p(dvdr), _p(dvdr2), C.c_double(dt), _p(_arr(mass, (na,))), C.c_int(na),
+                                        C.c_int(nb), _p(box), _p(_arr(z, (na,))), C.c_double(beta), _p(vir), _p(vl),
+                                        npt, nn, up, nu))
+        return dict(v=v.value, vew=vew.value, vlj=vlj.value, vint=vint.value, vir=vir.reshape(3, 3).T.copy(),
+                    vir_lf=vl.reshape(3, 3).T.copy(), box=box)
 
     # -- subroutine evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,
     #                             vir_lf,irun,nbaro)  src/evolve_rigid.f90:100 ---------------------

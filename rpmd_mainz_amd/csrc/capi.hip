@@ -171,6 +171,14 @@ struct rpmd_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork3 = nullptr, ev_join3 = nullptr;
     int streams = 2;
     double* goo = nullptr;  // O-O gradient of the oxygens per molecule, [bead][dim][molecule]
+    // barostat (module barostat, src/module_barostat.f90:1-5): RPMD_BARO_*, taub, patm, dv and the
+    // MC counters; the library's own stream of the MC draws; device sums of virial_ke per substep
+    int baro = 0;
+    double taub = 0.0, patm = 0.0, dv = 0.0;
+    long long nacc = 0, nmove = 0;
+    uint64_t baro_draws = 0;
+    double *baro_parts = nullptr, *baro_ev = nullptr;
+    int baro_mts = 0;
     // kernel timing with HIP events on the context stream (eager launches only)
     int prof = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -971,7 +979,208 @@ struct NoiseCursor {
 
 // one evolve_pi step (type 'RPMD', nbaro = 0) on the context's device state
 // (src/evolve_pi.f90:1-322).  Energies -> c->ev_step slots; vir_hf averaged on the host.
-int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
+// full_forces (src/md_forces.f90:305-420) on device positions pos (SoA) with the charges uploaded:
+// the high-frequency gradient into gh, the low-frequency one into *gl (a pointer to one of the two
+// buffers gl, gt: the last normal-mode transform may leave it in either), with go, ge as scratch
+// (and rnm for the normal modes); energies and the virial on the host
+int full_forces_core(rpmd_ctx* c, const double* pos, int nb, const double box[3], double** gl_io, double* gh, double* go,
+                     double* ge, double* gt, double* v, double* vew, double* voo, double* vint, double* vir) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na;
+    const size_t N = (size_t)3 * na * nb;
+    hipStream_t s = c->st;
+    double* gl = *gl_io;
+    double e_int, w_int[9], e_ew = 0, w_ew[9] = {0}, e_oo = 0, w_oo[9] = {0}, tv, tw[9];
+    // high-frequency forces (src/md_forces.f90:350)
+    RETCHK(forces_dev(c, pos, nb, box, 4, gh, c->ev, true));
+    RETCHK(read_ev(c, c->ev, 4, box, &e_int, w_int));
+    launch_zero(gl, N, s);
+    launch_zero(go, N, s);
+    const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
+    if (nb > 1 && (nbdf1 > 0 || nbdf2 > 0)) {
+        NMMat* m = get_nm(c, nb, 0);
+        if (!m) return RPMD_ERR_NOMEM;
+        launch_nm(pos, c->rnm, m->F, nb, (size_t)3 * na, s);  // :359
+        if (nbdf1 > 0) {
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, box, 2, gl, c->ev, true));
+            RETCHK(read_ev(c, c->ev, 2, box, &e_ew, w_ew));
+            if (P.sig > 0.0) {
+                RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, box, 5, ge, c->ev, true));
+                RETCHK(read_ev(c, c->ev, 5, box, &tv, tw));
+                e_ew -= tv;
+                for (int k = 0; k < 9; k++) w_ew[k] -= tw[k];
+                launch_axpy(gl, ge, -1.0, N, s);
+            }
+        }
+        if (nbdf2 > 0) {
+            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, box, 3, go, c->ev, true));
+            RETCHK(read_ev(c, c->ev, 3, box, &e_oo, w_oo));
+        }
+        if (nbdf1 > 0 && nbdf2 > 0) {
+            launch_axpy(gl, go, 1.0, N, s);
+            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(gl, gt);
+        } else if (nbdf1 > 0) {
+            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(gl, gt);
+        } else if (nbdf2 > 0) {
+            launch_nm(go, gt, m->B, nb, (size_t)3 * na, s);
+            std::swap(go, gt);
+        }
+    }
+    if (nb == 1 || nbdf1 == 0) {
+        RETCHK(forces_dev(c, pos, nb, box, 2, gl, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 2, box, &e_ew, w_ew));
+    } else if (P.sig > 0.0) {
+        RETCHK(forces_dev(c, pos, nb, box, 5, ge, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 5, box, &tv, tw));
+        e_ew += tv;
+        for (int k = 0; k < 9; k++) w_ew[k] += tw[k];
+        launch_axpy(gl, ge, 1.0, N, s);
+    }
+    if (nb == 1 || nbdf2 == 0) {
+        RETCHK(forces_dev(c, pos, nb, box, 3, go, c->ev, true));
+        RETCHK(read_ev(c, c->ev, 3, box, &e_oo, w_oo));
+    }
+    if (nb == 1 || nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(gl, go, 1.0, N, s);
+    *gl_io = gl;
+    if (vew) *vew = e_ew;
+    if (voo) *voo = e_oo;
+    if (vint) *vint = e_int;
+    if (v) *v = e_ew + e_oo + e_int;
+    if (vir)
+        for (int k = 0; k < 9; k++) vir[k] = w_oo[k] + w_ew[k] + w_int[k];
+    return 0;
+}
+
+// one barostat step's inputs (evolve_pi with nbaro = 1): the previous step's low-frequency virial and
+// the MC draws, replayed in order (u) or, when u is null, from the library's own stream
+struct BaroCall {
+    double vir_lf[9] = {0};
+    const double* u = nullptr;
+    int nu = 0, ui = 0;
+};
+
+// a ran2(irun, lo, hi) draw of the barostat: the next replayed value, or lo + (hi - lo) x with x a
+// 53-bit uniform of splitmix64 over the context seed and a per-context draw counter
+int baro_uniform(rpmd_ctx* c, BaroCall* bc, double lo, double hi, double* out) {
+    if (bc->u) {
+        if (bc->ui >= bc->nu) return RPMD_ERR_NOISE;
+        *out = bc->u[bc->ui++];
+        return 0;
+    }
+    uint64_t z = c->seed * 0x9E3779B97F4A7C15ull + (++c->baro_draws) * 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    *out = lo + (hi - lo) * ((double)(z >> 11) * 0x1p-53);
+    return 0;
+}
+
+// pres_lj_tail (src/md_pressure.f90:37-56): the LJ tail correction to the pressure
+double pres_lj_tail(const rpmd_params& P, const double box[3], int na) {
+    const int nm = na / 3;
+    const double pi = acos(-1.0), vol = box[0] * box[1] * box[2], rho = (double)nm / vol;
+    const double prefac = (16.0 * pi * (rho * rho) * P.oo_eps * (P.oo_sig * P.oo_sig * P.oo_sig)) / 3.0;
+    const double x = P.oo_sig / P.rcut, x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x8 = x4 * x4, x9 = x8 * x;
+    return prefac * ((2.0 / 3.0) * x9 - x3);
+}
+
+// the barostat of evolve_pi with nbaro = 1 (src/evolve_pi.f90:163-189) on the bead positions rst
+// after the multiple time steps; the box of the rest of the step (st_box) follows it, and the
+// normal-mode positions rnm become those of the moved positions (r = rst; realft(r, +1)).
+// BER (src/module_barostat.f90:8-122): the pressure of vir = vir_lf + the substeps' mean
+// high-frequency virial and of the substeps' mean virial_ke, isotropic in a cubic box, per axis
+// otherwise (with the module's patm: the reference's beren_baro_nc hides it behind a local of the
+// same name, :34), compressibility 13531 a.u.; MCI (:124-220): with probability 1/taub a volume
+// move vn = vo + dv u(-1, 1) of the molecules, judged on full_forces energies at both volumes.
+// Scratch while it runs: p (rebuilt from pnm after), pos, grad, dvdrl, dvdre, tmp, rnm.
+int baro_step(rpmd_ctx* c, BaroCall* bc, int nb, int mts) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nm = na / 3;
+    const size_t N = (size_t)3 * na * nb;
+    hipStream_t s = c->st;
+    double box[3] = {c->st_box.L[0], c->st_box.L[1], c->st_box.L[2]};
+    const double beta = c->st_beta, dt = c->st_dt;
+    if (c->baro == RPMD_BARO_BER) {
+        std::vector<double> h((size_t)mts * 50), t((size_t)mts * 10);
+        HIPCHK(hipMemcpyAsync(h.data(), c->ev_step, h.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(t.data(), c->baro_ev, t.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // virial_ke of each substep (src/kinetic.f90:42-49; nc = 3, nctot = 0), its mean, and the mean
+        // high-frequency virial (src/evolve_pi.f90:146-151, 163-166)
+        const double tc = 0.5 * (double)(3 * na - 3) / beta;
+        double tv = 0.0, tvxyz[3] = {0, 0, 0}, whf[9] = {0};
+        for (int im = 0; im < mts; im++) {
+            const double* q = &t[(size_t)im * 10];
+            const double tq1 = 0.5 * (q[2] + q[3] + q[4]) / nb, tq2 = 0.5 * (q[5] + q[6] + q[7]) / nb;
+            tv = tv + (tc + tq1 + tq2);
+            for (int d = 0; d < 3; d++) tvxyz[d] = tvxyz[d] + (tc / 3.0 + 0.5 * q[2 + d] / nb + 0.5 * q[5 + d] / nb);
+            double vv, w[9];
+            combine_ev(c, &h[(size_t)im * 50], 4, box, &vv, w);
+            for (int k = 0; k < 9; k++) whf[k] += w[k];
+        }
+        tv = tv / mts;
+        double vir[9];
+        for (int d = 0; d < 3; d++) tvxyz[d] = tvxyz[d] / mts;
+        for (int k = 0; k < 9; k++) vir[k] = bc->vir_lf[k] + whf[k] / (double)mts;
+        const double bstat = 13531.0 / (3.0 * c->taub), vol = box[0] * box[1] * box[2];
+        const double ptail = P.oo_gam == 0.0 ? pres_lj_tail(P, box, na) : 0.0;
+        double scale[3];
+        if (P.iamcub) {
+            // pressure (src/md_pressure.f90:1-35)
+            const double w = -(vir[0] + vir[4] + vir[8]) / 3.0;
+            const double pres = (2.0 * tv / 3.0 + w) / vol + ptail;
+            scale[0] = scale[1] = scale[2] = fabs(1.0 - (dt * bstat) * (c->patm - pres));
+        } else {
+            for (int d = 0; d < 3; d++) {
+                const double pres = (-vir[4 * d] + 2.0 * tvxyz[d]) / vol + ptail;
+                scale[d] = fabs(1.0 - (dt * bstat) * (c->patm - pres));
+            }
+        }
+        for (int d = 0; d < 3; d++) box[d] = scale[d] * box[d];
+        launch_baro_shift(c->rst, na, nb, scale, s);
+    } else if (c->baro == RPMD_BARO_MCI) {
+        double u;
+        RETCHK(baro_uniform(c, bc, 0.0, 1.0, &u));
+        if (u < 1.0 / c->taub) {
+            double eo, en, *gl = c->grad;
+            RETCHK(full_forces_core(c, c->rst, nb, box, &gl, c->p, c->dvdrl, c->dvdre, c->tmp, &eo, nullptr, nullptr,
+                                    nullptr, nullptr));
+            c->nmove++;
+            const double vo = box[0] * box[1] * box[2];
+            double u2, u3;
+            RETCHK(baro_uniform(c, bc, -1.0, 1.0, &u2));
+            const double vn = vo + c->dv * u2;
+            const double sc = pow(vn / vo, 1.0 / 3.0);
+            const double boxn[3] = {sc * box[0], sc * box[1], sc * box[2]}, scale[3] = {sc, sc, sc};
+            HIPCHK(hipMemcpyAsync(c->pos, c->rst, N * 8, hipMemcpyDeviceToDevice, s));
+            launch_baro_shift(c->pos, na, nb, scale, s);
+            gl = c->grad;
+            RETCHK(full_forces_core(c, c->pos, nb, boxn, &gl, c->p, c->dvdrl, c->dvdre, c->tmp, &en, nullptr, nullptr,
+                                    nullptr, nullptr));
+            double test = (en - eo) + c->patm * (vn - vo) - (double)nm * log(vn / vo) / beta;
+            test = exp(-beta * test);
+            RETCHK(baro_uniform(c, bc, 0.0, 1.0, &u3));
+            if (test > u3) {
+                // accepted: the box, the positions and their high-frequency forces
+                c->nacc++;
+                for (int d = 0; d < 3; d++) box[d] = boxn[d];
+                HIPCHK(hipMemcpyAsync(c->rst, c->pos, N * 8, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(c->dvdr2, c->p, N * 8, hipMemcpyDeviceToDevice, s));
+            }
+        }
+    } else {
+        return RPMD_ERR_BARO;
+    }
+    c->st_box = make_box(box);
+    NMMat* m = get_nm(c, nb, 0);
+    if (!m) return RPMD_ERR_NOMEM;
+    launch_nm(c->rst, c->rnm, m->F, nb, (size_t)3 * na, s);
+    return 0;
+}
+
+int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check, BaroCall* bc) {
     const rpmd_params& P = c->prm;
     const int na = c->na, nb = c->st_nb, mts = P.mts;
     const size_t N = (size_t)3 * na * nb, ne = (size_t)3 * na;
@@ -1000,7 +1209,7 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     };
     const int nblk = pile_blocks(na);
     const double inv_mtot = 1.0 / c->mtot;
-    if (P.therm == RPMD_THERM_PRO && mts == 1 && P.nbdf1 <= 0 && P.nbdf2 <= 0) {
+    if (P.therm == RPMD_THERM_PRO && mts == 1 && P.nbdf1 <= 0 && P.nbdf2 <= 0 && !bc) {
         // the same sequence with fewer launches: the opening kick is applied as p is loaded by the NM
         // transform (p itself is recomputed from pnm below), each COM removal as pnm is loaded by the
         // next kernel, the free ring and the second thermostat kick are one launch, p = B pnm is taken
@@ -1083,11 +1292,20 @@ int step_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
         pile();
         launch_nm(c->rnm, c->rst, nmm->B, nb, ne, s);
         RETCHK(forces_dev(c, c->rst, nb, box, 4, c->dvdr2, ev_hf + (size_t)(im - 1) * 50, check));
+        if (bc && c->baro == RPMD_BARO_BER) {
+            // virial_ke of the substep (src/evolve_pi.f90:146-151): the centroid-virial sums of the
+            // bead positions with the step's low- and the substep's high-frequency gradients
+            launch_estimators(c->pnm, c->rst, c->dvdr, c->dvdr2, c->mass, na, nb, 0.0, c->baro_parts, s);
+            ReduceJobs bj;
+            bj.add(c->baro_parts, estimator_blocks(na), (im - 1) * 10, 1.0);
+            launch_reduce_ev(bj, c->baro_ev, s);
+        }
         if (im != mts) {
             launch_nm(c->dvdr2, c->tmp, nmm->F, nb, ne, s);
             launch_kick(c->pnm, c->tmp, dtsmall, nullptr, 0.0, N, s);
         }
     }
+    if (bc) RETCHK(baro_step(c, bc, nb, mts));
     double* ev_lf = c->ev_step + (size_t)mts * 50;  // [EW, EWC-contracted, LJ, EWC-bead] x 50
     const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
     if (nbdf1 > 0) {
@@ -1391,6 +1609,7 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_GLE: return "GLE not initialised for this (na, nb): call rpmd_gle_init";
         case RPMD_ERR_TTAU: return "thermostat time constant ttau must be > 0";
         case RPMD_ERR_SOCKET: return "i-PI socket: no server at the address, or the connection broke";
+        case RPMD_ERR_BARO: return " ** Invalid barostat **";
         case RPMD_ERR_IPI_NAT: return " @DRIVER MODE: Uh-oh! Particle number mismatch between i-pi and rpmd input!";
     }
     return "unknown error";
@@ -1561,7 +1780,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     if (!c) return 0;
     drop_graph(c);
     free_vcache(c);
-    void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS,
+    void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS, c->baro_parts, c->baro_ev,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf,
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,
@@ -1607,77 +1826,16 @@ int rpmd_full_forces(rpmd_ctx* c, const double* r, int na, int nb, double* v, do
     if (!c || !r || !z || !boxlxyz || !dvdr || !dvdr2) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
-    const rpmd_params& P = c->prm;
-    const size_t N = (size_t)3 * na * nb;
-    hipStream_t s = c->st;
     // a device-resident state keeps p, r, dvdr and dvdr2 between steps; this call works in buffers
     // the state does not own (the contraction's own scratch, rnm, tmp2, rb and dvdrb, is
     // rewritten by every step), so it never disturbs a state, but graphs are captured anew
     invalidate(c);
-    double *gl = c->grad, *gh = c->pnm, *go = c->dvdrl, *ge = c->dvdre, *gt = c->tmp;
+    double* gl = c->grad;
     RETCHK(upload_charges(c, z));
     RETCHK(upload_aos(c, r, c->pos, nb));
-    double e_int, w_int[9], e_ew = 0, w_ew[9] = {0}, e_oo = 0, w_oo[9] = {0}, tv, tw[9];
-    // high-frequency forces (src/md_forces.f90:350)
-    RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 4, gh, c->ev, true));
-    RETCHK(read_ev(c, c->ev, 4, boxlxyz, &e_int, w_int));
-    launch_zero(gl, N, s);
-    launch_zero(go, N, s);
-    const int nbdf1 = P.nbdf1, nbdf2 = P.nbdf2;
-    if (nb > 1 && (nbdf1 > 0 || nbdf2 > 0)) {
-        NMMat* m = get_nm(c, nb, 0);
-        if (!m) return RPMD_ERR_NOMEM;
-        launch_nm(c->pos, c->rnm, m->F, nb, (size_t)3 * na, s);  // :359
-        if (nbdf1 > 0) {
-            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 2, gl, c->ev, true));
-            RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
-            if (P.sig > 0.0) {
-                RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf1, boxlxyz, 5, ge, c->ev, true));
-                RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
-                e_ew -= tv;
-                for (int k = 0; k < 9; k++) w_ew[k] -= tw[k];
-                launch_axpy(gl, ge, -1.0, N, s);
-            }
-        }
-        if (nbdf2 > 0) {
-            RETCHK(rp_contract_dev(c, c->rnm, nb, nbdf2, boxlxyz, 3, go, c->ev, true));
-            RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
-        }
-        if (nbdf1 > 0 && nbdf2 > 0) {
-            launch_axpy(gl, go, 1.0, N, s);
-            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
-            std::swap(gl, gt);
-        } else if (nbdf1 > 0) {
-            launch_nm(gl, gt, m->B, nb, (size_t)3 * na, s);
-            std::swap(gl, gt);
-        } else if (nbdf2 > 0) {
-            launch_nm(go, gt, m->B, nb, (size_t)3 * na, s);
-            std::swap(go, gt);
-        }
-    }
-    if (nb == 1 || nbdf1 == 0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 2, gl, c->ev, true));
-        RETCHK(read_ev(c, c->ev, 2, boxlxyz, &e_ew, w_ew));
-    } else if (P.sig > 0.0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 5, ge, c->ev, true));
-        RETCHK(read_ev(c, c->ev, 5, boxlxyz, &tv, tw));
-        e_ew += tv;
-        for (int k = 0; k < 9; k++) w_ew[k] += tw[k];
-        launch_axpy(gl, ge, 1.0, N, s);
-    }
-    if (nb == 1 || nbdf2 == 0) {
-        RETCHK(forces_dev(c, c->pos, nb, boxlxyz, 3, go, c->ev, true));
-        RETCHK(read_ev(c, c->ev, 3, boxlxyz, &e_oo, w_oo));
-    }
-    if (nb == 1 || nbdf1 <= 0 || nbdf2 <= 0) launch_axpy(gl, go, 1.0, N, s);
+    RETCHK(full_forces_core(c, c->pos, nb, boxlxyz, &gl, c->pnm, c->dvdrl, c->dvdre, c->tmp, v, vew, voo, vint, vir));
     RETCHK(download_aos(c, gl, dvdr, nb));
-    RETCHK(download_aos(c, gh, dvdr2, nb));
-    if (vew) *vew = e_ew;
-    if (voo) *voo = e_oo;
-    if (vint) *vint = e_int;
-    if (v) *v = e_ew + e_oo + e_int;
-    if (vir)
-        for (int k = 0; k < 9; k++) vir[k] = w_oo[k] + w_ew[k] + w_int[k];
+    RETCHK(download_aos(c, c->pnm, dvdr2, nb));
     return 0;
 }
 
@@ -1742,9 +1900,12 @@ int rpmd_free_ring(rpmd_ctx* c, double* p, double* r, const double* mass, int na
     return 0;
 }
 
-int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint, double* dvdr,
+// one evolve_pi step on host arrays; with bc the barostat acts between the multiple time steps and
+// the low-frequency forces and the box comes back in box_out
+int evolve_pi_call(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint, double* dvdr,
                    double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
-                   const double* z, double beta, double* vir, double* vir_lf, const double* noise, long long n_noise) {
+                   const double* z, double beta, double* vir, double* vir_lf, const double* noise, long long n_noise,
+                   BaroCall* bc, double* box_out) {
     if (!c || !p || !r || !dvdr || !dvdr2 || !mass || !z || !boxlxyz) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
@@ -1775,13 +1936,68 @@ int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, do
         nz.base = c->noise;
         nz.cap = need;
     }
-    RETCHK(step_dev(c, nz, true));
+    RETCHK(step_dev(c, nz, true, bc));
     RETCHK(download_aos(c, c->p, p, nb));
     RETCHK(download_aos(c, c->r, r, nb));
     RETCHK(download_aos(c, c->dvdr, dvdr, nb));
     RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
     RETCHK(step_energies(c, v, vew, vlj, vint, vir, vir_lf));
+    if (box_out)
+        for (int d = 0; d < 3; d++) box_out[d] = c->st_box.L[d];
     c->st_valid = false;
+    return 0;
+}
+
+int rpmd_evolve_pi(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint, double* dvdr,
+                   double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
+                   const double* z, double beta, double* vir, double* vir_lf, const double* noise, long long n_noise) {
+    return evolve_pi_call(c, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf,
+                          noise, n_noise, nullptr, nullptr);
+}
+
+int rpmd_set_barostat(rpmd_ctx* c, int baro, double taub, double patm, double dv) {
+    if (!c) return RPMD_ERR_ARG;
+    if (baro != RPMD_BARO_NONE && baro != RPMD_BARO_BER && baro != RPMD_BARO_MCI) return RPMD_ERR_BARO;
+    if (baro != RPMD_BARO_NONE && !(taub > 0.0)) return RPMD_ERR_ARG;
+    c->baro = baro;
+    c->taub = taub;
+    c->patm = patm;
+    c->dv = dv;
+    c->nacc = c->nmove = 0;
+    return 0;
+}
+
+int rpmd_barostat_counts(const rpmd_ctx* c, long long* nacc, long long* nmove) {
+    if (!c) return RPMD_ERR_ARG;
+    if (nacc) *nacc = c->nacc;
+    if (nmove) *nmove = c->nmove;
+    return 0;
+}
+
+int rpmd_evolve_pi_npt(rpmd_ctx* c, double* p, double* r, double* v, double* vew, double* vlj, double* vint,
+                       double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb, double boxlxyz[3],
+                       const double* z, double beta, double* vir, double* vir_lf, const double* noise,
+                       long long n_noise, const double* uniforms, int n_uniforms) {
+    if (!c || !vir_lf || !boxlxyz) return RPMD_ERR_ARG;
+    if (c->baro == RPMD_BARO_NONE) return RPMD_ERR_BARO;
+    if (c->prm.rigid || (uniforms && n_uniforms < 0)) return RPMD_ERR_ARG;
+    BaroCall bc;
+    memcpy(bc.vir_lf, vir_lf, sizeof bc.vir_lf);
+    bc.u = uniforms;
+    bc.nu = uniforms ? n_uniforms : 0;
+    if (c->baro == RPMD_BARO_BER) {
+        const int mts = std::max(c->prm.mts, 1);
+        if (!c->baro_parts) RETCHK(dalloc(&c->baro_parts, (size_t)estimator_blocks(c->na) * 10));
+        if (c->baro_mts < mts) {
+            if (c->baro_ev) hipFree(c->baro_ev);
+            c->baro_ev = nullptr;
+            RETCHK(dalloc(&c->baro_ev, (size_t)mts * 10));
+            c->baro_mts = mts;
+        }
+    }
+    double box[3] = {boxlxyz[0], boxlxyz[1], boxlxyz[2]};
+    RETCHK(evolve_pi_call(c, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, noise,
+                          n_noise, &bc, boxlxyz));
     return 0;
 }
 
@@ -1956,7 +2172,9 @@ int run_status(rpmd_ctx* c) {
     RETCHK(vlist_status(c));
     return c->prm.rigid ? rattle_status(c) : 0;
 }
-int step_any(rpmd_ctx* c, NoiseCursor& nz) { return c->prm.rigid ? step_rig_dev(c, nz, false) : step_dev(c, nz, false); }
+int step_any(rpmd_ctx* c, NoiseCursor& nz) {
+    return c->prm.rigid ? step_rig_dev(c, nz, false) : step_dev(c, nz, false, nullptr);
+}
 }  // namespace
 
 int rpmd_run_steps(rpmd_ctx* c, int nsteps) {

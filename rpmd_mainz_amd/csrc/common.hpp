@@ -304,6 +304,8 @@ void launch_sample_rpmd(double* p, const double* mass, int na, int nb, double be
 // estimator sums of the state: parts[estimator_blocks(na)][10] = {kinetic, springs, centroid-virial
 // sums of g1 and g2 per xyz, 0, 0} (reduce with launch_reduce_ev)
 int estimator_blocks(int na);
+// the barostat's move of every molecule by its centre-of-atoms sum times (scale_d - 1), SoA beads
+void launch_baro_shift(double* r, int na, int nb, const double scale[3], hipStream_t s);
 void launch_estimators(const double* p, const double* r, const double* g1, const double* g2, const double* mass,
                        int na, int nb, double wn2, double* parts, hipStream_t s);
 

@@ -1053,6 +1053,36 @@ void launch_estimators(const double* p, const double* r, const double* g1, const
     k_estimators<<<estimator_blocks(na), 256, 0, s>>>(p, r, g1, g2, mass, na, nb, wn2, parts);
 }
 
+// ---- barostat scaling of the molecules (src/module_barostat.f90:66-79, 107-120, 176-190) ---------
+// Thread per (molecule j, axis d): rcm = the bead average of the SUM of the molecule's three atom
+// coordinates (center_water, src/md_tools.f90:367-390: not their mean), each sum ((c + x_1) + x_2) +
+// x_3 in the reference's order, times 1/nb; every atom of the molecule in every bead then moves by
+// rcm (scale_d - 1).  Plain roundings (no fused multiply-add), as the reference's loops.
+__global__ __launch_bounds__(256) void k_baro_shift(double* __restrict__ r, int na, int nb, double s0, double s1,
+                                                    double s2) {
+    const int nm = na / 3;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nm) return;
+    const int d = t / nm, j = t % nm;
+    const double sc = d == 0 ? s0 : (d == 1 ? s1 : s2);
+    double c = 0.0;
+    for (int k = 0; k < nb; k++) {
+        const double* x = r + ((size_t)k * 3 + d) * na + 3 * j;
+        c = __dadd_rn(__dadd_rn(__dadd_rn(c, x[0]), x[1]), x[2]);
+    }
+    const double sh = __dmul_rn(__dmul_rn(c, 1.0 / (double)nb), sc - 1.0);
+    for (int k = 0; k < nb; k++) {
+        double* x = r + ((size_t)k * 3 + d) * na + 3 * j;
+        x[0] = __dadd_rn(x[0], sh);
+        x[1] = __dadd_rn(x[1], sh);
+        x[2] = __dadd_rn(x[2], sh);
+    }
+}
+void launch_baro_shift(double* r, int na, int nb, const double scale[3], hipStream_t s) {
+    const int n = 3 * (na / 3);
+    if (n) k_baro_shift<<<(n + 255) / 256, 256, 0, s>>>(r, na, nb, scale[0], scale[1], scale[2]);
+}
+
 __global__ void k_counter_inc(uint64_t* c) { ctrl_add(c, (uint64_t)RPMD_STREAMS_PER_STEP); }
 void launch_counter_inc(uint64_t* c, hipStream_t s) { k_counter_inc<<<1, 1, 0, s>>>(c); }
 

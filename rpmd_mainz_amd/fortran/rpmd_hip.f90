@@ -86,6 +86,31 @@ module rpmd_hip
        type(c_ptr), value :: noise
        integer(c_long_long), value :: n_noise
      end function c_evolve_pi
+     ! evolve_pi with nbaro = 1 (box and vir_lf in/out), src/evolve_pi.f90:163-189
+     integer(c_int) function c_evolve_pi_npt(ctx, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, &
+          beta, vir, vir_lf, noise, n_noise, uniforms, n_uniforms) bind(C, name='rpmd_evolve_pi_npt')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double) :: p(*), r(*), v, vew, vlj, vint, dvdr(*), dvdr2(*), vir(9), vir_lf(9), box(3)
+       real(c_double), value :: dt, beta
+       real(c_double), intent(in) :: mass(*), z(*)
+       integer(c_int), value :: na, nb
+       type(c_ptr), value :: noise, uniforms
+       integer(c_long_long), value :: n_noise
+       integer(c_int), value :: n_uniforms
+     end function c_evolve_pi_npt
+     ! module barostat's settings and MC counters
+     integer(c_int) function c_set_barostat(ctx, baro, taub, patm, dv) bind(C, name='rpmd_set_barostat')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: baro
+       real(c_double), value :: taub, patm, dv
+     end function c_set_barostat
+     integer(c_int) function c_barostat_counts(ctx, nacc, nmove) bind(C, name='rpmd_barostat_counts')
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), intent(out) :: nacc, nmove
+     end function c_barostat_counts
      ! the rigid step, src/evolve_rigid.f90:100 (vir in/out as there)
      integer(c_int) function c_evolve_rig_pi(ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box, z, &
           beta, vir, vir_lf, noise, n_noise) bind(C, name='rpmd_evolve_rig_pi')
@@ -229,6 +254,11 @@ module rpmd_hip
   ! replayed Gaussian deviates for the step entry points (rpmd_replay_noise); none: the device's
   ! own counter-based stream
   real(c_double), allocatable, target, save :: rpmd_noise(:)
+  ! the barostat of evolve_pi steps with nbaro = 1 (module barostat: RPMD_BARO_*, taub, patm, dv),
+  ! set with rpmd_set_barostat; replayed MC draws (rpmd_replay_uniforms), else the library's stream
+  integer, save :: rpmd_baro = 0
+  real(8), save :: rpmd_taub = 0.d0, rpmd_patm = 0.d0, rpmd_dv = 0.d0
+  real(c_double), allocatable, target, save :: rpmd_uniforms(:)
 
 contains
 
@@ -279,6 +309,44 @@ contains
     end select
     rpmd_ttau = ttau
   end subroutine rpmd_set_thermostat
+
+  subroutine rpmd_set_barostat(baro, taub, patm, dv)
+    ! the host's barostat (module barostat, src/module_barostat.f90:2-4, as src/main.f90:257-279 sets
+    ! it up): 'BER' or 'MCI', taub, patm and dv in atomic units; resets the MC counters
+    character(len=*), intent(in) :: baro
+    real(8), intent(in) :: taub, patm, dv
+    select case (baro(1:min(3, len(baro))))
+    case ('BER'); rpmd_baro = 1
+    case ('MCI'); rpmd_baro = 2
+    case default
+       write(6,*) ' ** Invalid barostat **'
+       error stop 'rpmd_hip'
+    end select
+    rpmd_taub = taub
+    rpmd_patm = patm
+    rpmd_dv = dv
+    if (c_associated(rpmd_ctx)) call rpmd_check(c_set_barostat(rpmd_ctx, int(rpmd_baro, c_int), taub, patm, dv))
+  end subroutine rpmd_set_barostat
+
+  subroutine rpmd_barostat_counts(nacc, nmove)
+    ! module barostat's nacc and nmove of the MC barostat
+    integer, intent(out) :: nacc, nmove
+    integer(c_long_long) :: a, m
+    a = 0
+    m = 0
+    if (c_associated(rpmd_ctx)) call rpmd_check(c_barostat_counts(rpmd_ctx, a, m))
+    nacc = int(a)
+    nmove = int(m)
+  end subroutine rpmd_barostat_counts
+
+  subroutine rpmd_replay_uniforms(u)
+    ! the MC barostat's ran2 draws of the next steps in the reference's order (the host draws them
+    ! with its ran2), instead of the library's stream
+    real(8), intent(in) :: u(:)
+    if (allocated(rpmd_uniforms)) deallocate(rpmd_uniforms)
+    allocate(rpmd_uniforms(size(u)))
+    rpmd_uniforms = u
+  end subroutine rpmd_replay_uniforms
 
   subroutine rpmd_replay_noise(noise)
     ! deviates consumed by the next step entry point in the reference's draw order (the host draws
@@ -382,6 +450,8 @@ contains
        ctx_na = na
        ctx_nb = max(nb, ctx_nb, 64)
        call rpmd_check(c_ctx_create(prm, int(na, c_int), int(ctx_nb, c_int), int(rpmd_precision, c_int), rpmd_ctx))
+       if (rpmd_baro .ne. 0) call rpmd_check(c_set_barostat(rpmd_ctx, int(rpmd_baro, c_int), rpmd_taub, rpmd_patm, &
+                                                            rpmd_dv))
     else
        call rpmd_check(c_ctx_set_params(rpmd_ctx, prm))
     endif
@@ -428,20 +498,30 @@ end subroutine realft
 subroutine evolve_pi(p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf, irun, nbaro)
   ! one RPMD step (type 'RPMD'); p, r, dvdr, dvdr2 in and out as in the reference.  The thermostat
   ! deviates come from rpmd_replay_noise when set, else from the device stream (irun is not
-  ! consumed).  Barostat steps (nbaro = 1) are not on this path.
+  ! consumed).  nbaro = 1: the barostat of rpmd_set_barostat acts on the box (in/out) after the
+  ! multiple time steps, driven by the incoming vir_lf; its MC draws from rpmd_replay_uniforms
+  ! when set, else from the library's stream.
   use rpmd_hip
   implicit none
   integer na, nb, irun, nbaro
   real(8) p(3,na,nb), r(3,na,nb), dvdr(3,na,nb), dvdr2(3,na,nb), mass(na), z(na), boxlxyz(3)
   real(8) v, vew, vlj, vint, dt, beta, vir(3,3), vir_lf(3,3)
-  type(c_ptr) :: np
+  type(c_ptr) :: np, up
   integer(c_long_long) :: nn
-  if (nbaro .ne. 0) then
-     write(6,*) ' ** barostat steps are not on this path **'
-     error stop 'rpmd_hip'
-  endif
+  integer(c_int) :: nu
   call rpmd_bind(na, nb)
   call rpmd_noise_arg(np, nn)
+  if (nbaro .ne. 0) then
+     up = c_null_ptr
+     nu = 0
+     if (allocated(rpmd_uniforms)) then
+        up = c_loc(rpmd_uniforms)
+        nu = size(rpmd_uniforms, kind=c_int)
+     endif
+     call rpmd_check(c_evolve_pi_npt(rpmd_ctx, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, int(na, c_int), &
+                                     int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn, up, nu))
+     return
+  endif
   call rpmd_check(c_evolve_pi(rpmd_ctx, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, int(na, c_int), &
                               int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn))
 end subroutine evolve_pi

@@ -16,6 +16,9 @@
 !           -> v, vew, vlj, vint, vir(9), vir_lf(9), p, r, dvdr, dvdr2
 !   mode 4: state_upload (seed iopt), nsteps device-resident steps, state_download
 !           -> energies(4), vir(9), p, r, dvdr, dvdr2
+!   mode 5: as mode 3 with nbaro = 1, after the mode-3 data: character(3) baro, f64 taub, patm, dv,
+!           int32 n_u, f64 u(n_u), vir_lf(9); rpmd_set_barostat, rpmd_replay_uniforms when n_u > 0
+!           -> as mode 3, then box(3), nacc, nmove (f64)
 program harness
   use rpmd_hip
   implicit none
@@ -35,8 +38,10 @@ program harness
   real(8), allocatable :: r(:,:,:), dvdr(:,:,:), dvdr2(:,:,:), z(:), p(:,:,:), mass(:), noise(:), a(:,:)
   real(8), allocatable :: gp(:,:,:,:)
   real(8) box(3), v, vew, voo, vint, vir(3,3), vir_lf(3,3), ttau, dt, beta, en(4)
-  character(len=3) therm
-  integer nsteps, ns, has_gp, i
+  character(len=3) therm, baro
+  integer nsteps, ns, has_gp, i, n_u, nacc, nmove
+  real(8) taub, patm, dv
+  real(8), allocatable :: u(:)
   integer(8) n_noise
   character(len=512) fin, fout
   call get_command_argument(1, fin)
@@ -68,6 +73,14 @@ program harness
            read(10) gp
         endif
      endif
+     if (mode .eq. 5) then
+        read(10) baro
+        read(10) taub, patm, dv
+        read(10) n_u
+        allocate(u(max(n_u, 1)))
+        if (n_u .gt. 0) read(10) u(1:n_u)
+        read(10) vir_lf
+     endif
   endif
   close(10)
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -92,6 +105,14 @@ program harness
      if (n_noise .gt. 0) call rpmd_replay_noise(noise(1:n_noise))
      call evolve_pi(p, r, v, vew, voo, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, iopt, 0)
      write(11) v, vew, voo, vint, vir, vir_lf, p, r, dvdr, dvdr2
+  else if (mode .eq. 5) then
+     call rpmd_set_thermostat(therm, ttau)
+     call rpmd_set_barostat(baro, taub, patm, dv)
+     if (n_noise .gt. 0) call rpmd_replay_noise(noise(1:n_noise))
+     if (n_u .gt. 0) call rpmd_replay_uniforms(u(1:n_u))
+     call evolve_pi(p, r, v, vew, voo, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, iopt, 1)
+     call rpmd_barostat_counts(nacc, nmove)
+     write(11) v, vew, voo, vint, vir, vir_lf, p, r, dvdr, dvdr2, box, dble(nacc), dble(nmove)
   else
      ! the device-resident API through its bind(C) interfaces
      call rpmd_set_thermostat(therm, ttau)

@@ -1,0 +1,125 @@
+"""The barostats of evolve_pi with nbaro = 1 (src/evolve_pi.f90:163-189, src/module_barostat.f90) on
+the GPU against the oracle's step, the reference's ran2 stream replayed: the thermostat's Gaussian
+deviates through the noise argument, the Monte Carlo barostat's uniforms (drawn between them, in the
+reference's order) through the uniforms argument.
+
+* Berendsen, isotropic (cubic box) and per axis (non-cubic), with PILE-L or the global Bussi
+  thermostat, and with RP contraction: positions, momenta, forces, energies, virials and the box of
+  four steps, relative 1e-10 or better.
+* Monte Carlo volume moves with PILE-L and with the global Bussi thermostat (uniforms between its two
+  halves): the same, and the move counters equal the oracle's.
+* Mixed precision with the Berendsen barostat: the box after four steps within 1e-7 of the oracle's.
+* An evolve_pi_npt call without a barostat set returns RPMD_ERR_BARO.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OrcBaro, RpmdParams
+from rpmd_mainz_amd import configs as cf
+
+pytestmark = pytest.mark.gpu
+
+BETA = 1.0 / (cf.KBOLTZ * 298.0)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _draws(oracle, s, therm, baro, taub, na, nb, mts):
+    """one step's replayed draws: Gaussians before the barostat, its uniforms, Gaussians after"""
+    d = 3 * na * nb
+    if therm == "PRO":
+        before, after = 2 * mts * d, 0
+    elif therm == "PRG":
+        before, after = d - 2, d - 2
+    else:
+        before, after = 0, 0
+    g1 = oracle.gaussians(s, before)
+    us = []
+    if baro == "MCI":
+        u = oracle.ran2(s, 0.0, 1.0)
+        us.append(u)
+        if u < 1.0 / taub:
+            us.append(oracle.ran2(s, -1.0, 1.0))
+            us.append(oracle.ran2(s, 0.0, 1.0))
+    g2 = oracle.gaussians(s, after)
+    return np.concatenate([g1, g2]), np.array(us)
+
+
+CASES = {
+    "ber_cubic_pro": ("BER", "PRO", (1, 1, 1), {}),
+    "ber_nc_prg": ("BER", "PRG", (1, 1, 1.2), {}),
+    "ber_contracted": ("BER", "PRO", (1, 1, 1), dict(nbdf1=3, nbdf2=5)),
+    "mci_cubic_pro": ("MCI", "PRO", (1, 1, 1), {}),
+    "mci_nc_prg": ("MCI", "PRG", (1, 1, 1.2), {}),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_evolve_pi_npt_matches_oracle(capi, oracle, name):
+    baro, therm, aspect, extra = CASES[name]
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, therm=therm, ttaufs=50.0, mts=2,
+                                           aspect=aspect, **extra)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    dt = 0.5 * cf.TOFS
+    taub, patm, dv = (41.341373 * 20.0, 1e-8, 0.0) if baro == "BER" else (1.5, 1e-8, 40.0)
+    B = OrcBaro(1 if baro == "BER" else 2, taub, patm, dv, 0, 0)
+    p0 = oracle.sample_rpmd(nb, na, mass, BETA, oracle.rng(5))
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    ctx.set_barostat(baro, taub, patm, dv)
+    s_or, s_gp = oracle.rng(63), oracle.rng(63)
+    a = dict(p=p0, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], box=box, vir_lf=np.zeros((3, 3)))
+    g = {k: np.ascontiguousarray(a[k]).copy() for k in ("p", "r", "dvdr", "dvdr2")}
+    gbox, gvl = box.copy(), np.zeros((3, 3))
+    for step in range(4):
+        print(f"case npt {name} step {step}", flush=True)
+        noise, us = _draws(oracle, s_gp, therm, baro, taub, na, nb, prm["mts"])
+        a = oracle.evolve_pi_npt(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, a["box"], z, BETA, a["vir_lf"],
+                                 B, s_or)
+        o = ctx.evolve_pi_npt(g["p"], g["r"], g["dvdr"], g["dvdr2"], dt, mass, na, nb, gbox, z, BETA, gvl,
+                              noise=noise, uniforms=us if baro == "MCI" else None)
+        gbox, gvl = o["box"], o["vir_lf"]
+        assert _rel(gbox, a["box"]) < 1e-13, (step, gbox, a["box"])
+        assert _rel(g["r"], a["r"]) < 1e-12, step
+        assert _rel(g["p"], a["p"]) < 1e-10, step
+        assert _rel(g["dvdr"], a["dvdr"]) < 1e-10 and _rel(g["dvdr2"], a["dvdr2"]) < 1e-10, step
+        assert abs(o["v"] - a["v"]) < 1e-10 * abs(a["v"]), step
+        assert _rel(o["vir"], a["vir"]) < 1e-9, step
+    assert not np.array_equal(gbox, box)
+    if baro == "MCI":
+        assert ctx.barostat_counts() == (B.nacc, B.nmove) and B.nacc > 0
+    ctx.close()
+
+
+def test_evolve_pi_npt_mixed_and_errors(capi, oracle):
+    r, box, z, mass, prm = cf.water_system("qspcfw", (4, 4, 4), 4, ecut=1e-6, therm="PRO", ttaufs=50.0)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    dt = 0.5 * cf.TOFS
+    taub, patm = 41.341373 * 20.0, 1e-8
+    B = OrcBaro(1, taub, patm, 0.0, 0, 0)
+    p0 = oracle.sample_rpmd(nb, na, mass, BETA, oracle.rng(5))
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "mixed")
+    with pytest.raises(capi.RPMDError) as e:
+        ctx.evolve_pi_npt(p0.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy(), dt, mass, na, nb, box, z, BETA,
+                          np.zeros((3, 3)))
+    assert e.value.code == 21
+    ctx.set_barostat("BER", taub, patm)
+    s_or, s_gp = oracle.rng(11), oracle.rng(11)
+    a = dict(p=p0, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], box=box, vir_lf=np.zeros((3, 3)))
+    g = {k: np.ascontiguousarray(a[k]).copy() for k in ("p", "r", "dvdr", "dvdr2")}
+    gbox, gvl = box.copy(), np.zeros((3, 3))
+    for step in range(4):
+        print(f"case npt mixed step {step}", flush=True)
+        noise, _ = _draws(oracle, s_gp, "PRO", "BER", taub, na, nb, 1)
+        a = oracle.evolve_pi_npt(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, a["box"], z, BETA, a["vir_lf"],
+                                 B, s_or)
+        o = ctx.evolve_pi_npt(g["p"], g["r"], g["dvdr"], g["dvdr2"], dt, mass, na, nb, gbox, z, BETA, gvl, noise=noise)
+        gbox, gvl = o["box"], o["vir_lf"]
+    print("  box rel err", _rel(gbox, a["box"]), "box change", _rel(gbox, box), flush=True)
+    assert _rel(gbox, a["box"]) < 1e-7 and _rel(gbox, box) > 1e-5
+    ctx.close()

@@ -128,3 +128,46 @@ def test_fortran_device_resident_api(capi, harness, tmp_path):
     st = ctx.state_download()
     ctx.close()
     assert np.array_equal(pf, st["p"]) and np.array_equal(rf, st["r"]) and en[0] == st["v"]
+
+
+def test_fortran_evolve_pi_npt_drop_in(capi, oracle, harness, tmp_path):
+    """evolve_pi with nbaro = 1 through the Fortran drop-in (rpmd_set_barostat 'MCI', the reference's
+    Gaussian deviates and ran2 uniforms replayed): bit for bit the C-ABI step from Python, the
+    oracle's step to 1e-12; the box and the move counters as the oracle's."""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, mts=2, therm="PRO", ttaufs=100.0)
+    nb, na = r.shape[:2]
+    P = RpmdParams.from_dict(prm)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    taub, patm, dv = 1.0, 1e-8, 40.0
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(4711))
+    ff = oracle.full_forces(P, r, box, z)
+    s_or, s_gp = oracle.rng(98), oracle.rng(98)
+    noise = oracle.gaussians(s_gp, 2 * prm["mts"] * 3 * na * nb)
+    us = np.array([oracle.ran2(s_gp, 0.0, 1.0), oracle.ran2(s_gp, -1.0, 1.0), oracle.ran2(s_gp, 0.0, 1.0)])
+    vl0 = np.diag([1e-3, 2e-3, -1e-3])
+    B = OrcBaro(2, taub, patm, dv, 0, 0)
+    o = oracle.evolve_pi_npt(P, p, r, ff["dvdr"], ff["dvdr2"], dt, mass, box, z, beta, vl0, B, s=s_or)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _step_input(fin, 5, 7, prm, box, z, r, "PRO", 1, dt, beta, mass, p, ff["dvdr"], ff["dvdr2"], noise)
+    with open(fin, "ab") as f:
+        f.write(b"MCI")
+        f.write(struct.pack("3d", taub, patm, dv))
+        f.write(struct.pack("i", us.size))
+        f.write(us.tobytes())
+        f.write(np.ascontiguousarray(vl0.T).tobytes())
+    print("case fortran evolve_pi nbaro=1 MCI", flush=True)
+    subprocess.run([harness, str(fin), str(fout)], check=True, timeout=120)
+    out = np.fromfile(fout, dtype=np.float64)
+    n = 3 * na * nb
+    pf, rf = out[22:22 + n].reshape(nb, na, 3), out[22 + n:22 + 2 * n].reshape(nb, na, 3)
+    boxf, cnt = out[22 + 4 * n:22 + 4 * n + 3], out[22 + 4 * n + 3:]
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    ctx.set_barostat("MCI", taub, patm, dv)
+    pc, rc, dc1, dc2 = p.copy(), r.copy(), ff["dvdr"].copy(), ff["dvdr2"].copy()
+    c = ctx.evolve_pi_npt(pc, rc, dc1, dc2, dt, mass, na, nb, box, z, beta, vl0, noise=noise, uniforms=us)
+    counts = ctx.barostat_counts()
+    ctx.close()
+    assert np.array_equal(pf, pc) and np.array_equal(rf, rc) and out[0] == c["v"] and np.array_equal(boxf, c["box"])
+    assert tuple(int(x) for x in cnt) == counts == (B.nacc, B.nmove) == (1, 1)
+    assert _rel(rf, o["r"]) < 1e-12 and _rel(pf, o["p"]) < 1e-10 and _rel(boxf, o["box"]) < 1e-14

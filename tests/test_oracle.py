@@ -631,7 +631,7 @@ def test_free_ring_is_the_exact_monodromy(oracle, refpath):
     assert np.array_equal(p1[0], p[0])
 
 
-def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta):
+def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta, baro=None, vir_lf_in=None):
     """evolve_pi (src/evolve_pi.f90:1-322, type 'RPMD', nbaro = 0) composed from the reference's
     own compiled routines -- ring_rpmd, therm_pro, therm_pro_global, parinello_therm,
     parinello_therm_loc, therm_gle, forces -- in the reference's order, with realft from the oracle
@@ -687,6 +687,7 @@ def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta):
     p = p - halfdtsmall * dvdr2
     p, r = _to_nm(orc, p), _to_nm(orc, r)
     vir_hf = np.zeros((3, 3))
+    tv, tvxyz = 0.0, np.zeros(3)
     for im in range(1, mts + 1):
         p = inner(p)
         pp = p * m[:, 0, :, None] + r * m[:, 1, :, None]
@@ -695,10 +696,22 @@ def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta):
         p = inner(p)
         rst = _to_beads(orc, r)
         vint, dvdr2, w = ref.forces(P, rst, box, z, 4)
+        if baro:
+            t, txyz, _, _ = ref.virial_ke(rst, dvdr, dvdr2, beta)
+            tv, tvxyz = tv + t, tvxyz + txyz
         vir_hf = vir_hf + w
         if im != mts:
             p = p - dtsmall * _to_nm(orc, dvdr2)
     vir_hf = vir_hf / mts
+    if baro:
+        # the barostat on the bead positions (src/evolve_pi.f90:163-189)
+        tv, tvxyz = tv / mts, tvxyz / mts
+        if baro == "BER":
+            rst, box = ref.beren_driver(P, vir_lf_in + vir_hf, tv, tvxyz, dt, rst, box)
+        else:
+            o = ref.mc_baro(P, rst, dvdr, dvdr2, vir_lf_in + vir_hf, z, beta, box)
+            rst, dvdr, dvdr2, box = o["r"], o["dvdr"], o["dvdr2"], o["box"]
+        r = _to_nm(orc, rst)
     vew = vlj = 0.0
     vir_ew = vir_lj = np.zeros((3, 3))
     dvdrl = np.zeros_like(r)
@@ -731,7 +744,7 @@ def _ref_evolve_pi(ref, orc, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta):
     p = outer(p)
     vir_lf = vir_lj + vir_ew
     return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=vew + vlj + vint, vew=vew, vlj=vlj, vint=vint,
-                vir=vir_lf + vir_hf, vir_lf=vir_lf)
+                vir=vir_lf + vir_hf, vir_lf=vir_lf, box=np.array(box, dtype=np.float64))
 
 
 def _compare_steps(a, b, tag):
@@ -853,3 +866,150 @@ def test_estimators_match_reference(oracle, refpath):
     e = np_estimators(p, ring, g1, g2, mass, beta, 0.0)
     closed = 0.5 * (nb / beta) ** 2 * nb * float(mass.sum()) * (2 * a * np.sin(np.pi / nb)) ** 2
     assert abs(e["springs"] - closed) <= 1e-12 * closed
+
+
+# ---- barostats (src/module_barostat.f90) --------------------------------------------------------
+
+def test_barostat_helpers_match_reference(oracle, refpath):
+    """virial_ke (src/kinetic.f90:1-54), pressure with the LJ tail (src/md_pressure.f90) and
+    center_water (src/md_tools.f90:367-390) against the reference's own Fortran"""
+    for isys in (0, 2):
+        r, box, z, mass, P = _sys(isys)
+        nb, na = r.shape[:2]
+        g = np.random.default_rng(isys)
+        d1, d2 = g.normal(size=r.shape) * 1e-2, g.normal(size=r.shape) * 1e-2
+        beta = 1.0 / (cf.KBOLTZ * 298.0)
+        a, b = oracle.virial_ke(r, d1, d2, beta), refpath.virial_ke(r, d1, d2, beta)
+        assert abs(a[0] - b[0]) <= 1e-14 * abs(b[0]) and np.abs(a[1] - b[1]).max() <= 1e-14 * np.abs(b[1]).max()
+        vir = g.normal(size=(3, 3)) * 0.1
+        for gam in (0.0, 13.5):
+            Pg = RpmdParams.from_dict(dict(P.to_dict(), oo_gam=gam))
+            pa, pb = oracle.pressure(Pg, vir, a[0], na, box), refpath.pressure(Pg, vir, a[0], na, box)
+            assert abs(pa - pb) <= 1e-15 * abs(pb), (gam, pa, pb)
+        assert np.array_equal(oracle.center_water(r), refpath.center_water(r))
+
+
+def test_beren_driver_matches_reference(oracle, refpath):
+    """isotropic Berendsen barostat (cubic box): box and positions bitwise"""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    g = np.random.default_rng(7)
+    vir = g.normal(size=(3, 3)) * 0.05
+    tv, tvxyz = 0.3, np.array([0.1, 0.12, 0.09])
+    taub, patm, dt = 41.341373 * 500.0, 1.0 / 2.9421e8, 0.5 * cf.TOFS
+    B = OrcBaro(1, taub, patm, 0.0, 0, 0)
+    refpath.set_barostat("BER", taub, patm, 0.0)
+    ra, ba = oracle.beren_driver(P, B, vir, tv, tvxyz, dt, r, box)
+    rb, bb = refpath.beren_driver(P, vir, tv, tvxyz, dt, r, box)
+    assert np.array_equal(ba, bb) and np.array_equal(ra, rb)
+    assert not np.array_equal(ba, box)
+    # the molecules move rigidly: every intramolecular vector is unchanged to rounding
+    d0 = r.reshape(nb, -1, 3, 3) - r.reshape(nb, -1, 3, 3)[:, :, :1]
+    d1 = ra.reshape(nb, -1, 3, 3) - ra.reshape(nb, -1, 3, 3)[:, :, :1]
+    assert np.abs(d1 - d0).max() < 1e-12
+
+
+def test_beren_baro_nc_reduces_to_isotropic(oracle):
+    """Per-axis Berendsen barostat (non-cubic box).  The reference's beren_baro_nc declares a local
+    patm (src/module_barostat.f90:34) that hides the module's, so its target pressure is undefined
+    there (NaN boxes in the reference built here); this one uses the module's patm.  Pinned by the
+    isotropic form: with vir = w I and tvxyz = tv/3 every axis has the isotropic pressure, and the
+    boxes and positions equal the cubic form's; changing vir(1,1) alone moves only the x side."""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, P = _sys(2)
+    assert not P.iamcub
+    Pc = RpmdParams.from_dict(dict(P.to_dict(), iamcub=1))
+    tv, w = 0.35, 0.02
+    B = OrcBaro(1, 41.341373 * 300.0, 2e-9, 0.0, 0, 0)
+    dt = 0.5 * cf.TOFS
+    rn, bn = oracle.beren_driver(P, B, w * np.eye(3), tv, np.full(3, tv / 3.0), dt, r, box)
+    rc, bc = oracle.beren_driver(Pc, B, w * np.eye(3), tv, np.full(3, tv / 3.0), dt, r, box)
+    assert np.abs(bn - bc).max() <= 1e-15 * bc.max() and np.abs(rn - rc).max() <= 1e-14 * np.abs(rc).max()
+    assert not np.array_equal(bn, box)
+    v2 = w * np.eye(3)
+    v2[0, 0] = 3 * w
+    rx, bx = oracle.beren_driver(P, B, v2, tv, np.full(3, tv / 3.0), dt, r, box)
+    assert bx[0] != bn[0] and bx[1] == bn[1] and bx[2] == bn[2]
+    assert np.array_equal(rx[..., 1:], rn[..., 1:]) and not np.array_equal(rx[..., 0], rn[..., 0])
+
+
+def test_beren_baro_fixed_point(oracle):
+    """at patm equal to the instantaneous pressure the isotropic Berendsen scale is exactly 1"""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, P = _sys(0)
+    na = r.shape[1]
+    vir, tv = np.diag([0.02, -0.01, 0.03]), 0.4
+    pres = oracle.pressure(P, vir, tv, na, box)
+    B = OrcBaro(1, 1000.0, pres, 0.0, 0, 0)
+    ra, ba = oracle.beren_driver(P, B, vir, tv, np.zeros(3), 10.0, r, box)
+    assert np.array_equal(ba, box) and np.array_equal(ra, r)
+
+
+def test_mc_baro_matches_reference(oracle, refpath):
+    """Monte Carlo volume moves on one ran2 stream: attempted and accepted moves, box, positions,
+    gradients, virial and energy as the reference's mc_baro (no RP contraction)"""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, P = _sys(0)
+    nb, na = r.shape[:2]
+    beta = 1.0 / (cf.KBOLTZ * 298.0)
+    ff = oracle.full_forces(P, r, box, z)
+    s = oracle.rng(17)
+    refpath.seed(17)
+    counts = []
+    for taub, dv, patm in ((1.0, 200.0, 1e-9), (2.0, 2000.0, 3e-4), (1.0, 50.0, 0.0), (3.0, 800.0, 1e-6)):
+        B = OrcBaro(2, taub, patm, dv, 0, 0)
+        refpath.set_barostat("MCI", taub, patm, dv)
+        cur_a = dict(r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], box=box)
+        cur_b = dict(cur_a)
+        for it in range(3):
+            a = oracle.mc_baro(P, B, cur_a["r"], cur_a["dvdr"], cur_a["dvdr2"], np.zeros((3, 3)), z, beta,
+                               cur_a["box"], s)
+            b = refpath.mc_baro(P, cur_b["r"], cur_b["dvdr"], cur_b["dvdr2"], np.zeros((3, 3)), z, beta, cur_b["box"])
+            for k in ("r", "dvdr", "dvdr2", "box"):
+                assert np.abs(a[k] - b[k]).max() <= 1e-13 * max(np.abs(b[k]).max(), 1e-300), (taub, it, k)
+            assert abs(a["eo"] - b["eo"]) <= 1e-13 * max(abs(b["eo"]), 1e-300)
+            cur_a, cur_b = a, b
+        assert (B.nacc, B.nmove) == refpath.barostat_counts()
+        counts.append((B.nacc, B.nmove))
+    # accepted and rejected moves and a step without a move all occur
+    assert any(0 < a < m for a, m in counts) and any(m < 3 for a, m in counts)
+    assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
+
+
+@pytest.mark.parametrize("baro,therm,aspect,extra", [
+    ("BER", "PRO", (1, 1, 1), {}), ("BER", "PRG", (1, 1, 1), {}), ("BER", "PRO", (1, 1, 1), dict(nbdf1=3, nbdf2=5)),
+    ("MCI", "PRO", (1, 1, 1), {}), ("MCI", "PRG", (1, 1, 1.2), {}),
+])
+def test_evolve_pi_npt_composed_from_reference(oracle, refpath, baro, therm, aspect, extra):
+    """Four evolve_pi steps with nbaro = 1 at nb = 8, mts = 2: the barostat between the multiple time
+    steps and the low-frequency forces, with the scaled box for those (src/evolve_pi.f90:163-189),
+    against the same steps composed from the reference's own routines on one ran2 stream (the MC
+    draws between the thermostat's)."""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, prm = cf.water_system("qspcfw", (3, 3, 3), 8, ecut=1e-6, therm=therm, ttaufs=50.0, mts=2,
+                                           aspect=aspect, **extra)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    taub, patm, dv = (41.341373 * 20.0, 1e-8, 0.0) if baro == "BER" else (1.5, 1e-8, 40.0)
+    B = OrcBaro(1 if baro == "BER" else 2, taub, patm, dv, 0, 0)
+    refpath.set_barostat(baro, taub, patm, dv)
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(5))
+    ff = oracle.full_forces(P, r, box, z)
+    s = oracle.rng(63)
+    refpath.seed(63)
+    a = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], box=box, vir_lf=np.zeros((3, 3)))
+    b = dict(a)
+    for step in range(4):
+        a = oracle.evolve_pi_npt(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, a["box"], z, beta, a["vir_lf"],
+                                 B, s)
+        b = _ref_evolve_pi(refpath, oracle, P, b["p"], b["r"], b["dvdr"], b["dvdr2"], dt, mass, b["box"], z, beta,
+                           baro=baro, vir_lf_in=b["vir_lf"])
+        _compare_steps(a, b, (baro, therm, step))
+        assert np.abs(a["box"] - b["box"]).max() <= 1e-14 * b["box"].max()
+    print(baro, therm, "box", box, "->", a["box"], "MC moves", B.nacc, "/", B.nmove)
+    assert not np.array_equal(a["box"], box)
+    if baro == "MCI":
+        assert (B.nacc, B.nmove) == refpath.barostat_counts() and B.nacc > 0
+    assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
