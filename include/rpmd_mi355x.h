@@ -191,10 +191,13 @@ int rpmd_evolve_pi_npt(rpmd_ctx* ctx, double* p, double* r, double* v, double* v
 /* subroutine evolve_rig_pi(p,r,v,v_lf,v_hf,dvdr,dvdr2,dt,mass,na,nb,boxlxyz,z,beta,vir,vir_lf,
  *                          irun,nbaro)                          src/evolve_rigid.f90:100-234
  * One rigid-molecule RPMD step (params.rigid = 1, nbaro = 0): nb bead-spring sub-steps with
- * RATTLE (src/md_rattle.f90, FP64, tolerance 1e-6, at most 1000 iterations), forces(iopt = 1),
- * and RATTLE stage 2 with dt.  vir is in/out: the reference adds the incoming value.  Thermostats
- * PRG and PRL as the reference's; other codes leave the step thermostat-free, as there.  A
- * RATTLE that does not converge returns RPMD_ERR_RATTLE. */
+ * RATTLE (src/md_rattle.f90, FP64, tolerance params.rat_tol, <= 0 the reference's 1e-6, at most
+ * 1000 iterations), forces(iopt = 1), and RATTLE stage 2 with dt.  vir is in/out: the reference
+ * adds the incoming value.  Thermostats PRG and PRL as the reference's; PRO (PILE-L on the normal
+ * modes) and GLE, which the reference's rigid step does not offer, as the same half-step kicks
+ * followed by a projection of the momenta (and the GLE auxiliary momenta) onto the constraint
+ * surface; other codes leave the step thermostat-free, as there.  A RATTLE that does not converge
+ * returns RPMD_ERR_RATTLE. */
 int rpmd_evolve_rig_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* v_lf, double* v_hf,
                        double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
                        const double boxlxyz[3], const double* z, double beta, double* vir,

@@ -230,6 +230,12 @@ module rpmd_hip
        integer(c_int), intent(out) :: batched, pair_cap, pair_beside
        real(c_double), intent(out) :: ms_batched, ms_separate
      end function c_batch_tuning
+     ! estimators of the device-resident state (RPMD_EST_* order, 11 values)
+     integer(c_int) function c_estimators(ctx, est) bind(C, name='rpmd_estimators')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: est(11)
+     end function c_estimators
      ! the i-PI client (run_driver)
      integer(c_int) function c_ipi_run(ctx, address, z, na, served) bind(C, name='rpmd_ipi_run')
        import :: c_int, c_ptr, c_double, c_char
