@@ -203,6 +203,18 @@ int rpmd_evolve_rig_pi(rpmd_ctx* ctx, double* p, double* r, double* v, double* v
                        const double boxlxyz[3], const double* z, double beta, double* vir,
                        double* vir_lf, const double* noise, long long n_noise);
 
+/* subroutine evolve_rig_pi(...,irun,nbaro) with nbaro = 1          src/evolve_rigid.f90:203-220
+ * As rpmd_evolve_rig_pi, then the barostat of rpmd_set_barostat acts on the final positions and the
+ * box (boxlxyz in/out).  BER: the pressure of the step's in/out virial vir and of virial_ke with the
+ * step's intermolecular gradient (nctot = 3 nm); the molecules move rigidly.  MCI: as mc_baro, which
+ * replaces v (v_lf), vir, dvdr and dvdr2 by its full_forces values when it tries a move.  uniforms
+ * as in rpmd_evolve_pi_npt. */
+int rpmd_evolve_rig_pi_npt(rpmd_ctx* ctx, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                           double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb,
+                           double boxlxyz[3], const double* z, double beta, double* vir,
+                           double* vir_lf, const double* noise, long long n_noise,
+                           const double* uniforms, int n_uniforms);
+
 /* subroutine sample_rpmd(p,na,nb,mass,beta,irun)                 src/module_thermostat.f90:199-226
  * Maxwell-Boltzmann bead momenta at beta/nb, then pshift (:229-264) removes the total momentum.
  * noise: 3*na*nb standard normal deviates in the reference's draw order (bead, atom, xyz), or NULL

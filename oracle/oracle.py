@@ -82,6 +82,7 @@ class Oracle:
         L.orc_realft.restype = C.c_int
         L.orc_evolve_pi.restype = C.c_int
         L.orc_evolve_pi_npt.restype = C.c_int
+        L.orc_evolve_rig_pi_npt.restype = C.c_int
         L.orc_mc_baro.restype = C.c_int
         L.orc_pressure.restype = C.c_double
         L.orc_pres_lj_tail.restype = C.c_double
@@ -255,6 +256,28 @@ class Oracle:
         return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, v_lf=vlf.value, v_hf=vhf.value,
                     vir=vir.reshape(3, 3).T.copy(), vir_lf=vir_lf.reshape(3, 3).T.copy())
 
+    def evolve_rig_pi_npt(self, prm: RpmdParams, p, r, dvdr, dvdr2, dt, mass, box, z, beta, vir, baro: "OrcBaro",
+                          s: RngState | None = None):
+        """evolve_rig_pi with nbaro = 1; vir (3x3) in/out as the reference's; returns the box too"""
+        p, r = _f64(p).copy(), _f64(r).copy()
+        dvdr, dvdr2 = _f64(dvdr).copy(), _f64(dvdr2).copy()
+        nb, na = r.shape[0], r.shape[1]
+        b = _f64(box).copy()
+        w = _f64(np.asarray(vir).T).reshape(9).copy()
+        vir_lf = np.zeros(9)
+        v, vlf, vhf = C.c_double(), C.c_double(), C.c_double()
+        if s is None:
+            s = RngState()
+            self.lib.orc_rng_init(C.byref(s))
+        err = self.lib.orc_evolve_rig_pi_npt(C.byref(prm), _d(p), _d(r), C.byref(v), C.byref(vlf), C.byref(vhf),
+                                             _d(dvdr), _d(dvdr2), C.c_double(dt), _d(_f64(mass)), C.c_int(na),
+                                             C.c_int(nb), _d(b), _d(_f64(z)), C.c_double(beta), _d(w), _d(vir_lf),
+                                             C.byref(s), C.byref(baro))
+        if err:
+            raise RuntimeError(f"orc_evolve_rig_pi_npt error {err}")
+        return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v.value, v_lf=vlf.value, v_hf=vhf.value, box=b,
+                    vir=w.reshape(3, 3).T.copy(), vir_lf=vir_lf.reshape(3, 3).T.copy())
+
     def matrix_exp(self, M, j=15, k=15):
         M = _f64(np.asarray(M).T)  # column-major
         n = M.shape[0]
@@ -353,11 +376,12 @@ class Oracle:
                                   C.c_double(dt), _d(r), _d(b), C.c_int(na), C.c_int(nb))
         return r, b
 
-    def mc_baro(self, prm: RpmdParams, baro: "OrcBaro", r, dvdr, dvdr2, vir, z, beta, box, s: RngState):
+    def mc_baro(self, prm: RpmdParams, baro: "OrcBaro", r, dvdr, dvdr2, vir, z, beta, box, s: RngState,
+                eo: float = 0.0):
         r, g1, g2, b = _f64(r).copy(), _f64(dvdr).copy(), _f64(dvdr2).copy(), _f64(box).copy()
         nb, na = r.shape[0], r.shape[1]
         w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
-        eo = C.c_double(0.0)
+        eo = C.c_double(eo)
         err = self.lib.orc_mc_baro(C.byref(prm), C.byref(baro), _d(r), _d(g1), _d(g2), _d(w), C.byref(eo),
                                    _d(_f64(z)), C.c_double(beta), _d(b), C.c_int(na), C.c_int(nb), C.byref(s))
         if err:
@@ -638,14 +662,14 @@ class RefPath:
                                           C.byref(C.c_int(nb)))
         return r, b
 
-    def mc_baro(self, prm: RpmdParams, r, dvdr, dvdr2, vir, z, beta, box):
+    def mc_baro(self, prm: RpmdParams, r, dvdr, dvdr2, vir, z, beta, box, eo: float = 0.0):
         """barostat::mc_baro (src/module_barostat.f90:124-220) on the current ran2 stream (no RP
-        contraction here: the reference's full_forces would call realft)"""
+        contraction here: the reference's full_forces would call realft); eo in: the caller's v"""
         self.set_params(prm)
         r, g1, g2, b = _f64(r).copy(), _f64(dvdr).copy(), _f64(dvdr2).copy(), _f64(box).copy()
         nb, na = r.shape[0], r.shape[1]
         w = np.asarray(vir, dtype=np.float64).T.reshape(9).copy()
-        eo = C.c_double(0.0)
+        eo = C.c_double(eo)
         (C.c_int * 2).in_dll(self.lib, "beaddiabatic_")[:] = [prm.nbdf1, prm.nbdf2]
         self.lib._QMbarostatPmc_baro(_d(r), _d(g1), _d(g2), _d(w), C.byref(eo), _d(_f64(z).copy()),
                                      C.byref(C.c_double(beta)), _d(b), C.byref(C.c_int(na)), C.byref(C.c_int(nb)),

@@ -2430,3 +2430,28 @@ int orc_evolve_rig_pi(const rpmd_params* P, double* p, double* r, double* v, dou
     free(rold);
     return err;
 }
+
+/* evolve_rig_pi with nbaro = 1 (src/evolve_rigid.f90:203-220): after the rigid step (dvdr2 = 0),
+ * BER from virial_ke of the final positions with the step's intermolecular gradient (nctot = 3 nm)
+ * and the step's in/out virial; MCI from mc_baro, which replaces v, vir, dvdr and dvdr2 by its
+ * full_forces values when it tries a move; then v_lf = v, v_hf = 0 */
+int orc_evolve_rig_pi_npt(const rpmd_params* P, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                          double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb, double box[3],
+                          const double* z, double beta, double* vir, double* vir_lf, orc_rng* s, orc_baro* B) {
+    int err = orc_evolve_rig_pi(P, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, s);
+    if (err) return err;
+    int nm = na / 3;
+    if (B->baro == ORC_BARO_BER) {
+        double tv, tvxyz[3];
+        orc_virial_ke(r, dvdr, dvdr2, &tv, tvxyz, NULL, NULL, beta, na, nb, 3 * nm);
+        orc_beren_driver(P, B, vir, tv, tvxyz, dt, r, box, na, nb);
+    } else if (B->baro == ORC_BARO_MCI) {
+        err = orc_mc_baro(P, B, r, dvdr, dvdr2, vir, v, z, beta, box, na, nb, s);
+    } else {
+        err = -1;
+    }
+    *v_lf = *v;
+    *v_hf = 0.0;
+    return err;
+}
+

@@ -97,6 +97,10 @@ int orc_evolve_pi(const rpmd_params* P, double* p, double* r, double* v, double*
                   const double* mass, int na, int nb, const double box[3], const double* z,
                   double beta, double* vir, double* vir_lf, orc_rng* s, orc_gle* g);
 
+int orc_evolve_rig_pi_npt(const rpmd_params* P, double* p, double* r, double* v, double* v_lf, double* v_hf,
+                          double* dvdr, double* dvdr2, double dt, const double* mass, int na, int nb, double box[3],
+                          const double* z, double beta, double* vir, double* vir_lf, orc_rng* s, orc_baro* B);
+
 int orc_rattle_s1(double* r, const double* rold, double* p, int nm, const double* mass, double dt,
                   double* dvdr, double* vir, double reoh, double theta);
 int orc_rattle_s2(const double* r, double* p, int nm, const double* mass, double dt, double* dvdr,

@@ -71,7 +71,7 @@ def lib() -> C.CDLL:
                      "rpmd_batch_create", "rpmd_batch_run_steps", "rpmd_batch_destroy", "rpmd_sample_rpmd",
                      "rpmd_ctx_set_streams", "rpmd_free_ring", "rpmd_realft_noalloc", "rpmd_estimators",
                      "rpmd_batch_tuning", "rpmd_ipi_run", "rpmd_set_barostat", "rpmd_barostat_counts",
-                     "rpmd_evolve_pi_npt"):
+                     "rpmd_evolve_pi_npt", "rpmd_evolve_rig_pi_npt"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -296,6 +296,36 @@ class RPMDContext:
                                         _p(wl), npt, nn))
         return dict(v=v.value, v_lf=vlf.value, v_hf=vhf.value, vir=w.reshape(3, 3).T.copy(),
                     vir_lf=wl.reshape(3, 3).T.copy())
+
+    def evolve_rig_pi_npt(self, p, r, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir=None, noise=None,
+                          uniforms=None):
+        """One rigid step with nbaro = 1 (src/evolve_rigid.f90:203-220); as evolve_rig_pi, and the new box
+        is returned; uniforms: the MC barostat's ran2 draws in order, or None for the library's stream."""
+        for a in (p, r, dvdr, dvdr2):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.shape == (nb, na, 3)):
+                raise TypeError("evolve_rig_pi state arrays must be C-contiguous float64 (nb, na, 3)")
+        w = np.zeros(9) if vir is None else np.ascontiguousarray(np.asarray(vir, dtype=np.float64).T).reshape(9)
+        w = w.copy()
+        wl = np.zeros(9)
+        box = _arr(boxlxyz, (3,)).copy()
+        v, vlf, vhf = C.c_double(), C.c_double(), C.c_double()
+        if noise is not None:
+            noise = _arr(noise)
+            npt, nn = _p(noise), C.c_longlong(noise.size)
+        else:
+            npt, nn = None, C.c_longlong(0)
+        if uniforms is not None:
+            uniforms = _arr(uniforms)
+            up, nu = _p(uniforms), C.c_int(uniforms.size)
+        else:
+            up, nu = None, C.c_int(0)
+        _check(lib().rpmd_evolve_rig_pi_npt(self._h, _p(p), _p(r), C.byref(v), C.byref(vlf), C.byref(vhf), _p(dvdr),
+                                            _p(dvdr2), C.c_double(dt), _p(_arr(mass, (na,))), C.c_int(na), C.c_int(nb),
+                                            _p(box), _p(_arr(z, (na,))), C.c_double(beta), _p(w), _p(wl), npt, nn, up,
+                                            nu))
+        return dict(v=v.value, v_lf=vlf.value, v_hf=vhf.value, vir=w.reshape(3, 3).T.copy(),
+                    vir_lf=wl.reshape(3, 3).T.copy(), box=box)
 
     # -- subroutine sample_rpmd(p,na,nb,mass,beta,irun)  src/module_thermostat.f90:199 ----------
     def sample_rpmd(self, nb: int, na: int, mass, beta: float, noise=None, seed: int = 0) -> np.ndarray:

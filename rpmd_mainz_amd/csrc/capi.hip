@@ -1077,6 +1077,18 @@ int baro_uniform(rpmd_ctx* c, BaroCall* bc, double lo, double hi, double* out) {
     return 0;
 }
 
+// the buffers of the barostat's virial_ke sums (BER)
+int baro_buffers(rpmd_ctx* c, int mts) {
+    if (!c->baro_parts) RETCHK(dalloc(&c->baro_parts, (size_t)estimator_blocks(c->na) * 10));
+    if (c->baro_mts < mts) {
+        if (c->baro_ev) hipFree(c->baro_ev);
+        c->baro_ev = nullptr;
+        RETCHK(dalloc(&c->baro_ev, (size_t)mts * 10));
+        c->baro_mts = mts;
+    }
+    return 0;
+}
+
 // pres_lj_tail (src/md_pressure.f90:37-56): the LJ tail correction to the pressure
 double pres_lj_tail(const rpmd_params& P, const double box[3], int na) {
     const int nm = na / 3;
@@ -1177,6 +1189,89 @@ int baro_step(rpmd_ctx* c, BaroCall* bc, int nb, int mts) {
     NMMat* m = get_nm(c, nb, 0);
     if (!m) return RPMD_ERR_NOMEM;
     launch_nm(c->rst, c->rnm, m->F, nb, (size_t)3 * na, s);
+    return 0;
+}
+
+// the barostat of evolve_rig_pi with nbaro = 1 (src/evolve_rigid.f90:203-220) after the rigid step:
+// dvdr2 is zero; BER from virial_ke of the final positions with the step's intermolecular gradient
+// (nctot = 3 nm) and the step's in/out virial vir; MCI replaces v, vir, dvdr and dvdr2 by its
+// full_forces values when it tries a move, and by those of the moved molecules when it accepts.
+// Scratch: pos, grad, pnm, dvdrl, dvdre, tmp, rnm.
+int rig_baro_tail(rpmd_ctx* c, BaroCall* bc, int nb, double* v, double vir[9]) {
+    const rpmd_params& P = c->prm;
+    const int na = c->na, nm = na / 3;
+    const size_t N = (size_t)3 * na * nb;
+    hipStream_t s = c->st;
+    double box[3] = {c->st_box.L[0], c->st_box.L[1], c->st_box.L[2]};
+    const double beta = c->st_beta, dt = c->st_dt;
+    if (c->baro == RPMD_BARO_BER) {
+        launch_estimators(c->p, c->r, c->dvdr, c->dvdr2, c->mass, na, nb, 0.0, c->baro_parts, s);
+        ReduceJobs bj;
+        bj.add(c->baro_parts, estimator_blocks(na), 0, 1.0);
+        launch_reduce_ev(bj, c->baro_ev, s);
+        double q[10];
+        HIPCHK(hipMemcpyAsync(q, c->baro_ev, sizeof q, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const double tc = 0.5 * (double)(3 * na - 3 - 3 * nm) / beta;
+        const double tv = tc + 0.5 * (q[2] + q[3] + q[4]) / nb + 0.5 * (q[5] + q[6] + q[7]) / nb;
+        double tvxyz[3];
+        for (int d = 0; d < 3; d++) tvxyz[d] = tc / 3.0 + 0.5 * q[2 + d] / nb + 0.5 * q[5 + d] / nb;
+        const double bstat = 13531.0 / (3.0 * c->taub), vol = box[0] * box[1] * box[2];
+        const double ptail = P.oo_gam == 0.0 ? pres_lj_tail(P, box, na) : 0.0;
+        double scale[3];
+        if (P.iamcub) {
+            const double w = -(vir[0] + vir[4] + vir[8]) / 3.0;
+            const double pres = (2.0 * tv / 3.0 + w) / vol + ptail;
+            scale[0] = scale[1] = scale[2] = fabs(1.0 - (dt * bstat) * (c->patm - pres));
+        } else {
+            for (int d = 0; d < 3; d++) {
+                const double pres = (-vir[4 * d] + 2.0 * tvxyz[d]) / vol + ptail;
+                scale[d] = fabs(1.0 - (dt * bstat) * (c->patm - pres));
+            }
+        }
+        for (int d = 0; d < 3; d++) box[d] = scale[d] * box[d];
+        launch_baro_shift(c->r, na, nb, scale, s);
+    } else if (c->baro == RPMD_BARO_MCI) {
+        double u;
+        RETCHK(baro_uniform(c, bc, 0.0, 1.0, &u));
+        if (u < 1.0 / c->taub) {
+            double eo, en, wo[9], wn[9], *gl = c->grad;
+            RETCHK(full_forces_core(c, c->r, nb, box, &gl, c->pnm, c->dvdrl, c->dvdre, c->tmp, &eo, nullptr, nullptr,
+                                    nullptr, wo));
+            // mc_baro keeps the forces of its first full_forces call (src/module_barostat.f90:147-148)
+            HIPCHK(hipMemcpyAsync(c->dvdr, gl, N * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(c->dvdr2, c->pnm, N * 8, hipMemcpyDeviceToDevice, s));
+            *v = eo;
+            for (int k = 0; k < 9; k++) vir[k] = wo[k];
+            c->nmove++;
+            const double vo = box[0] * box[1] * box[2];
+            double u2, u3;
+            RETCHK(baro_uniform(c, bc, -1.0, 1.0, &u2));
+            const double vn = vo + c->dv * u2;
+            const double sc = pow(vn / vo, 1.0 / 3.0);
+            const double boxn[3] = {sc * box[0], sc * box[1], sc * box[2]}, scale[3] = {sc, sc, sc};
+            HIPCHK(hipMemcpyAsync(c->pos, c->r, N * 8, hipMemcpyDeviceToDevice, s));
+            launch_baro_shift(c->pos, na, nb, scale, s);
+            gl = c->grad;
+            RETCHK(full_forces_core(c, c->pos, nb, boxn, &gl, c->pnm, c->dvdrl, c->dvdre, c->tmp, &en, nullptr, nullptr,
+                                    nullptr, wn));
+            double test = (en - eo) + c->patm * (vn - vo) - (double)nm * log(vn / vo) / beta;
+            test = exp(-beta * test);
+            RETCHK(baro_uniform(c, bc, 0.0, 1.0, &u3));
+            if (test > u3) {
+                c->nacc++;
+                for (int d = 0; d < 3; d++) box[d] = boxn[d];
+                HIPCHK(hipMemcpyAsync(c->r, c->pos, N * 8, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(c->dvdr, gl, N * 8, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(c->dvdr2, c->pnm, N * 8, hipMemcpyDeviceToDevice, s));
+                *v = en;
+                for (int k = 0; k < 9; k++) vir[k] = wn[k];
+            }
+        }
+    } else {
+        return RPMD_ERR_BARO;
+    }
+    c->st_box = make_box(box);
     return 0;
 }
 
@@ -1985,26 +2080,19 @@ int rpmd_evolve_pi_npt(rpmd_ctx* c, double* p, double* r, double* v, double* vew
     memcpy(bc.vir_lf, vir_lf, sizeof bc.vir_lf);
     bc.u = uniforms;
     bc.nu = uniforms ? n_uniforms : 0;
-    if (c->baro == RPMD_BARO_BER) {
-        const int mts = std::max(c->prm.mts, 1);
-        if (!c->baro_parts) RETCHK(dalloc(&c->baro_parts, (size_t)estimator_blocks(c->na) * 10));
-        if (c->baro_mts < mts) {
-            if (c->baro_ev) hipFree(c->baro_ev);
-            c->baro_ev = nullptr;
-            RETCHK(dalloc(&c->baro_ev, (size_t)mts * 10));
-            c->baro_mts = mts;
-        }
-    }
+    if (c->baro == RPMD_BARO_BER) RETCHK(baro_buffers(c, std::max(c->prm.mts, 1)));
     double box[3] = {boxlxyz[0], boxlxyz[1], boxlxyz[2]};
     RETCHK(evolve_pi_call(c, p, r, v, vew, vlj, vint, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, noise,
                           n_noise, &bc, boxlxyz));
     return 0;
 }
 
-int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_lf, double* v_hf, double* dvdr,
+// one rigid step on host arrays; with bc the barostat acts after it (src/evolve_rigid.f90:203-220)
+// and the box comes back in box_out
+int evolve_rig_pi_call(rpmd_ctx* c, double* p, double* r, double* v, double* v_lf, double* v_hf, double* dvdr,
                        double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
                        const double* z, double beta, double* vir, double* vir_lf, const double* noise,
-                       long long n_noise) {
+                       long long n_noise, BaroCall* bc, double* box_out) {
     if (!c || !p || !r || !dvdr || !dvdr2 || !mass || !z || !boxlxyz || !vir) return RPMD_ERR_ARG;
     if (na % 3) return RPMD_ERR_NA_NOT_MULT3;
     if (na != c->na || nb < 1 || nb > c->nb_max) return RPMD_ERR_ARG;
@@ -2038,22 +2126,50 @@ int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_l
     }
     RETCHK(step_rig_dev(c, nz, true));
     RETCHK(rattle_status(c));
-    RETCHK(download_aos(c, c->p, p, nb));
-    RETCHK(download_aos(c, c->r, r, nb));
-    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
-    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
     double vv, w[9], wl[9];
     RETCHK(step_rig_energies(c, &vv, w, wl));
-    if (v) *v = vv;
-    if (v_lf) *v_lf = vv;
-    if (v_hf) *v_hf = 0.0;
     for (int k = 0; k < 9; k++) {
         vir[k] = w[k] + vir[k];
         if (vir_lf) vir_lf[k] = wl[k];
     }
+    if (bc) RETCHK(rig_baro_tail(c, bc, nb, &vv, vir));
+    RETCHK(download_aos(c, c->p, p, nb));
+    RETCHK(download_aos(c, c->r, r, nb));
+    RETCHK(download_aos(c, c->dvdr, dvdr, nb));
+    RETCHK(download_aos(c, c->dvdr2, dvdr2, nb));
+    if (v) *v = vv;
+    if (v_lf) *v_lf = vv;
+    if (v_hf) *v_hf = 0.0;
+    if (box_out)
+        for (int d = 0; d < 3; d++) box_out[d] = c->st_box.L[d];
     c->st_valid = false;
     return 0;
 }
+
+int rpmd_evolve_rig_pi(rpmd_ctx* c, double* p, double* r, double* v, double* v_lf, double* v_hf, double* dvdr,
+                       double* dvdr2, double dt, const double* mass, int na, int nb, const double boxlxyz[3],
+                       const double* z, double beta, double* vir, double* vir_lf, const double* noise,
+                       long long n_noise) {
+    return evolve_rig_pi_call(c, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf,
+                              noise, n_noise, nullptr, nullptr);
+}
+
+int rpmd_evolve_rig_pi_npt(rpmd_ctx* c, double* p, double* r, double* v, double* v_lf, double* v_hf, double* dvdr,
+                           double* dvdr2, double dt, const double* mass, int na, int nb, double boxlxyz[3],
+                           const double* z, double beta, double* vir, double* vir_lf, const double* noise,
+                           long long n_noise, const double* uniforms, int n_uniforms) {
+    if (!c || !boxlxyz || !vir) return RPMD_ERR_ARG;
+    if (c->baro == RPMD_BARO_NONE) return RPMD_ERR_BARO;
+    if (uniforms && n_uniforms < 0) return RPMD_ERR_ARG;
+    BaroCall bc;
+    bc.u = uniforms;
+    bc.nu = uniforms ? n_uniforms : 0;
+    if (c->baro == RPMD_BARO_BER) RETCHK(baro_buffers(c, 1));
+    double box[3] = {boxlxyz[0], boxlxyz[1], boxlxyz[2]};
+    return evolve_rig_pi_call(c, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box, z, beta, vir, vir_lf, noise,
+                              n_noise, &bc, boxlxyz);
+}
+
 
 int rpmd_state_upload(rpmd_ctx* c, const double* p, const double* r, const double* mass, const double* z, int na,
                       int nb, const double boxlxyz[3], double beta, double dt, unsigned long long seed) {

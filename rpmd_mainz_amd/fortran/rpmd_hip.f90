@@ -99,6 +99,19 @@ module rpmd_hip
        integer(c_long_long), value :: n_noise
        integer(c_int), value :: n_uniforms
      end function c_evolve_pi_npt
+     ! evolve_rig_pi with nbaro = 1 (box in/out), src/evolve_rigid.f90:203-220
+     integer(c_int) function c_evolve_rig_pi_npt(ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, box, z, &
+          beta, vir, vir_lf, noise, n_noise, uniforms, n_uniforms) bind(C, name='rpmd_evolve_rig_pi_npt')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double) :: p(*), r(*), v, v_lf, v_hf, dvdr(*), dvdr2(*), vir(9), vir_lf(9), box(3)
+       real(c_double), value :: dt, beta
+       real(c_double), intent(in) :: mass(*), z(*)
+       integer(c_int), value :: na, nb
+       type(c_ptr), value :: noise, uniforms
+       integer(c_long_long), value :: n_noise
+       integer(c_int), value :: n_uniforms
+     end function c_evolve_rig_pi_npt
      ! module barostat's settings and MC counters
      integer(c_int) function c_set_barostat(ctx, baro, taub, patm, dv) bind(C, name='rpmd_set_barostat')
        import :: c_int, c_ptr, c_double
@@ -535,20 +548,29 @@ end subroutine evolve_pi
 
 subroutine evolve_rig_pi(p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, na, nb, boxlxyz, z, beta, vir, vir_lf, irun, &
                          nbaro)
-  ! one rigid RPMD step (/structure/ iamrigid); vir in and out as in the reference
+  ! one rigid RPMD step (/structure/ iamrigid); vir in and out as in the reference; nbaro = 1: the
+  ! barostat of rpmd_set_barostat after the step (box in/out)
   use rpmd_hip
   implicit none
   integer na, nb, irun, nbaro
   real(8) p(3,na,nb), r(3,na,nb), dvdr(3,na,nb), dvdr2(3,na,nb), mass(na), z(na), boxlxyz(3)
   real(8) v, v_lf, v_hf, dt, beta, vir(3,3), vir_lf(3,3)
-  type(c_ptr) :: np
+  type(c_ptr) :: np, up
   integer(c_long_long) :: nn
-  if (nbaro .ne. 0) then
-     write(6,*) ' ** barostat steps are not on this path **'
-     error stop 'rpmd_hip'
-  endif
+  integer(c_int) :: nu
   call rpmd_bind(na, nb)
   call rpmd_noise_arg(np, nn)
+  if (nbaro .ne. 0) then
+     up = c_null_ptr
+     nu = 0
+     if (allocated(rpmd_uniforms)) then
+        up = c_loc(rpmd_uniforms)
+        nu = size(rpmd_uniforms, kind=c_int)
+     endif
+     call rpmd_check(c_evolve_rig_pi_npt(rpmd_ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, int(na, c_int), &
+                                         int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn, up, nu))
+     return
+  endif
   call rpmd_check(c_evolve_rig_pi(rpmd_ctx, p, r, v, v_lf, v_hf, dvdr, dvdr2, dt, mass, int(na, c_int), &
                                   int(nb, c_int), boxlxyz, z, beta, vir, vir_lf, np, nn))
 end subroutine evolve_rig_pi

@@ -123,3 +123,58 @@ def test_evolve_pi_npt_mixed_and_errors(capi, oracle):
     print("  box rel err", _rel(gbox, a["box"]), "box change", _rel(gbox, box), flush=True)
     assert _rel(gbox, a["box"]) < 1e-7 and _rel(gbox, box) > 1e-5
     ctx.close()
+
+
+RIG_CASES = {"rig_ber_prg": ("BER", "PRG"), "rig_mci_prl": ("MCI", "PRL"), "rig_mci_none": ("MCI", "NONE")}
+
+
+@pytest.mark.parametrize("name", list(RIG_CASES))
+def test_evolve_rig_pi_npt_matches_oracle(capi, oracle, name):
+    """The rigid step's barostat (src/evolve_rigid.f90:203-220) on the GPU against the oracle: three
+    rigid TIP4P/2005 steps with the thermostat's deviates and the MC uniforms replayed; positions,
+    momenta, gradients, energies, the in/out virial and the box; the molecules stay rigid."""
+    baro, therm = RIG_CASES[name]
+    r, box, z, mass, prm = cf.water_system("tip4p2005", (3, 3, 3), 4, ecut=1e-6, rigid=True, therm=therm,
+                                           ttaufs=50.0, seed=1, amp=0.1)
+    P = RpmdParams.from_dict(prm)
+    nb, na = r.shape[:2]
+    nm = na // 3
+    dt = 0.5 * cf.TOFS
+    taub, patm, dv = (41.341373 * 10.0, 1e-8, 0.0) if baro == "BER" else (1.0, 1e-8, 40.0)
+    B = OrcBaro(1 if baro == "BER" else 2, taub, patm, dv, 0, 0)
+    p0 = oracle.sample_rpmd(nb, na, mass, BETA, oracle.rng(11))
+    for k in range(nb):
+        p0[k] = oracle.rattle_s2(r[k], p0[k], mass, dt, np.zeros((na, 3)), P)[0]
+    ff = oracle.full_forces(P, r, box, z)
+    ctx = capi.RPMDContext(prm, na, nb, "fp64")
+    ctx.set_barostat(baro, taub, patm, dv)
+    s_or, s_gp = oracle.rng(58), oracle.rng(58)
+    per_step = {"NONE": 0, "PRG": 2 * (3 * na * nb - 3 * nm - 2), "PRL": 2 * 3 * na * nb}[therm]
+    a = dict(p=p0, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], vir=np.zeros((3, 3)), box=box)
+    g = {k: np.ascontiguousarray(a[k]).copy() for k in ("p", "r", "dvdr", "dvdr2")}
+    gbox, gvir = box.copy(), np.zeros((3, 3))
+    for step in range(3):
+        print(f"case rigid npt {name} step {step}", flush=True)
+        noise = oracle.gaussians(s_gp, per_step) if per_step else None
+        us = []
+        if baro == "MCI":
+            u = oracle.ran2(s_gp, 0.0, 1.0)
+            us.append(u)
+            if u < 1.0 / taub:
+                us += [oracle.ran2(s_gp, -1.0, 1.0), oracle.ran2(s_gp, 0.0, 1.0)]
+        a = oracle.evolve_rig_pi_npt(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, a["box"], z, BETA, a["vir"],
+                                     B, s_or)
+        o = ctx.evolve_rig_pi_npt(g["p"], g["r"], g["dvdr"], g["dvdr2"], dt, mass, na, nb, gbox, z, BETA, vir=gvir,
+                                  noise=noise, uniforms=np.array(us) if baro == "MCI" else None)
+        gbox, gvir = o["box"], o["vir"]
+        assert _rel(gbox, a["box"]) < 1e-13, step
+        assert _rel(g["r"], a["r"]) < 1e-12, step
+        assert _rel(g["p"], a["p"]) < 1e-9, step
+        assert _rel(g["dvdr"], a["dvdr"]) < 1e-9, step
+        assert np.abs(g["dvdr2"] - a["dvdr2"]).max() <= 1e-9 * max(np.abs(a["dvdr2"]).max(), 1.0), step
+        assert abs(o["v"] - a["v"]) < 1e-10 * abs(a["v"]) and o["v_lf"] == o["v"] and o["v_hf"] == 0.0, step
+        assert _rel(gvir, a["vir"]) < 1e-9, step
+    assert not np.array_equal(gbox, box)
+    if baro == "MCI":
+        assert ctx.barostat_counts() == (B.nacc, B.nmove) and B.nacc > 0
+    ctx.close()

@@ -374,7 +374,7 @@ def test_bead_forces_closed_form(oracle):
     assert np.array_equal(oracle.bead_forces(r[:1], beta, mass), np.zeros((1, na, 3)))
 
 
-def _ref_rig_step(ref, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta, therm, vir):
+def _ref_rig_step(ref, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta, therm, vir, baro=None):
     """evolve_rig_pi (src/evolve_rigid.f90:100-234, nbaro = 0) composed from the reference's own
     compiled routines (parinello_therm[_loc], bead_forces, rattle_s1/s2, forces) in its order.
     The reference routine itself cannot be run as the pin: it adds dvdr_corr, v_corr and vir_corr,
@@ -414,7 +414,15 @@ def _ref_rig_step(ref, P, p, r, dvdr, dvdr2, dt, mass, box, z, beta, therm, vir)
         vir_hf = vir_hf + vt
     p = thermo(p)
     vir = vir_lf + vir_hf / nb + vir
-    return dict(p=p, r=r, dvdr=dvdr, dvdr2=np.zeros_like(r), v=v, vir=vir, vir_lf=vir_lf)
+    dvdr2 = np.zeros_like(r)
+    if baro == "BER":
+        # the rigid step's barostat (src/evolve_rigid.f90:203-216), nctot = 3 nm
+        tv, tvxyz, _, _ = ref.virial_ke(r, dvdr, dvdr2, beta, nctot=3 * nm)
+        r, box = ref.beren_driver(P, vir, tv, tvxyz, dt, r, box)
+    elif baro == "MCI":
+        o = ref.mc_baro(P, r, dvdr, dvdr2, vir, z, beta, box, eo=v)
+        r, dvdr, dvdr2, vir, v, box = o["r"], o["dvdr"], o["dvdr2"], o["vir"], o["eo"], o["box"]
+    return dict(p=p, r=r, dvdr=dvdr, dvdr2=dvdr2, v=v, vir=vir, vir_lf=vir_lf, box=np.array(box, dtype=np.float64))
 
 
 @pytest.mark.parametrize("therm", ["NONE", "PRG", "PRL"])
@@ -1012,4 +1020,42 @@ def test_evolve_pi_npt_composed_from_reference(oracle, refpath, baro, therm, asp
     assert not np.array_equal(a["box"], box)
     if baro == "MCI":
         assert (B.nacc, B.nmove) == refpath.barostat_counts() and B.nacc > 0
+    assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
+
+
+@pytest.mark.parametrize("baro,therm", [("BER", "PRG"), ("MCI", "PRL"), ("MCI", "NONE")])
+def test_evolve_rig_pi_npt_matches_reference(oracle, refpath, baro, therm):
+    """Three rigid steps with nbaro = 1 (src/evolve_rigid.f90:203-220): the oracle against the same
+    steps composed from the reference's routines (its beren_driver, mc_baro, virial_ke with
+    nctot = 3 nm) on one ran2 stream; the molecules stay rigid through the barostat's moves."""
+    from oracle.oracle import OrcBaro
+    r, box, z, mass, P = _rigid(nb=4, therm=therm)
+    nb, na = r.shape[:2]
+    refpath.rattle_setup(na // 3, P)
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.5 * cf.TOFS
+    taub, patm, dv = (41.341373 * 10.0, 1e-8, 0.0) if baro == "BER" else (1.0, 1e-8, 40.0)
+    B = OrcBaro(1 if baro == "BER" else 2, taub, patm, dv, 0, 0)
+    refpath.set_barostat(baro, taub, patm, dv)
+    p = oracle.sample_rpmd(nb, na, mass, beta, oracle.rng(11))
+    for k in range(nb):
+        p[k] = oracle.rattle_s2(r[k], p[k], mass, dt, np.zeros((na, 3)), P)[0]
+    ff = oracle.full_forces(P, r, box, z)
+    s = oracle.rng(58)
+    refpath.seed(58)
+    a = dict(p=p, r=r, dvdr=ff["dvdr"], dvdr2=ff["dvdr2"], vir=np.zeros((3, 3)), box=box)
+    b = dict(a)
+    for step in range(3):
+        a = oracle.evolve_rig_pi_npt(P, a["p"], a["r"], a["dvdr"], a["dvdr2"], dt, mass, a["box"], z, beta, a["vir"],
+                                     B, s)
+        b = _ref_rig_step(refpath, P, b["p"], b["r"], b["dvdr"], b["dvdr2"], dt, mass, b["box"], z, beta, therm,
+                          b["vir"], baro=baro)
+        for key in ("p", "r", "dvdr", "dvdr2", "box"):
+            assert np.abs(a[key] - b[key]).max() <= 1e-13 * max(np.abs(b[key]).max(), 1e-300), (step, key)
+        assert abs(a["v"] - b["v"]) <= 1e-13 * abs(b["v"]) and a["v_lf"] == a["v"] and a["v_hf"] == 0.0
+        assert np.abs(a["vir"] - b["vir"]).max() <= 1e-12 * np.abs(b["vir"]).max()
+    assert not np.array_equal(a["box"], box)
+    if baro == "MCI":
+        assert (B.nacc, B.nmove) == refpath.barostat_counts() and B.nacc > 0
+    for k in range(nb):
+        assert _bond_residuals(a["r"][k], P) <= 1.0001e-6
     assert oracle.gaussians(s, 1)[0] == refpath.gaussian()
