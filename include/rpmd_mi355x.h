@@ -32,7 +32,7 @@ extern "C" {
 #define RPMD_ERR_REALFT_MODE       7  /* 'realft 3'                 src/fourier.f90:56          */
 #define RPMD_ERR_EWC_CUTOFF        8  /* 'ewc_basic : rcut too large' src/correct_tem.f90:104-109 */
 #define RPMD_ERR_SMEARED           9  /* TEM correction (iopt 5) with smeared charges (wm,wh>0) */
-#define RPMD_ERR_KMAX             10  /* Ewald kmax beyond the device k-tile (kmax > 15)      */
+#define RPMD_ERR_KMAX             10  /* Ewald kmax beyond the device k tiles (kmax > 31)      */
 #define RPMD_ERR_ARG              11  /* bad argument (null pointer, nb/na beyond ctx sizes)  */
 #define RPMD_ERR_HIP              12  /* HIP runtime error                                     */
 #define RPMD_ERR_IOPT             13  /* iopt not in {0,1,2,3,4,5}                             */

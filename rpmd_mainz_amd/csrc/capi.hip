@@ -143,6 +143,14 @@ struct rpmd_ctx {
     KSet ks{};
     double ks_key[8] = {0};
     bool ks_valid = false;
+    // kmax 16 .. 31: the k set on 64 x 64 tiles and the FP64 buffers of its kernels (allocated on
+    // first use): mask, chunk partials, G tiles, energy/virial parts, gradient slots
+    KSetBig ksb{};
+    double ksb_key[8] = {0};
+    bool ksb_valid = false;
+    unsigned char* kmask_big = nullptr;
+    double *skp_big = nullptr, *gk_big = nullptr, *parts_big = nullptr, *fpart_big = nullptr;
+    size_t skp_big_n = 0, gk_big_n = 0;
     double sumq2 = 0.0, mtot = 0.0;
     std::vector<double> hq;
     // device-resident state
@@ -252,16 +260,15 @@ int intra_model(const rpmd_params& P) {
 
 // the reference's k set as a mask [kx][ky+15][kz+15] (src/ewald_cubic.f90:292-520 loop exits;
 // src/ewald_noncubic.f90:385-397 strict comparisons)
-int build_kset(rpmd_ctx* c, const double box[3]) {
-    const rpmd_params& P = c->prm;
-    double key[8] = {box[0], box[1], box[2], P.walpha, P.rkmax, (double)P.kmax, (double)P.iamcub, 0};
-    if (c->ks_valid && !memcmp(key, c->ks_key, sizeof key)) return 0;
-    if (P.kmax > RPMD_KMAX_DEV || P.kmax < 0) return RPMD_ERR_KMAX;
-    const int K = P.kmax, T = RPMD_KTILE, H = RPMD_KMAX_DEV, O = RPMD_KTILE / 2;
-    std::vector<unsigned char> m((size_t)(H + 1) * T * T, 0);
+// the reference's k set (src/ewald_cubic.f90:235-533: cubic, the loops with their breaks;
+// src/ewald_noncubic.f90:314-457: every (kx, ky, kz) with 0 < k^2 < rkmax^2, kx >= 0) on tiles of
+// T x T entries per kx (index k + T / 2), kx in [0, H]: the mask m and the per-row bounds of KS
+template <typename KS>
+void kset_fill(const rpmd_params& P, const double box[3], int T, int H, KS& ks, std::vector<unsigned char>& m) {
+    const int K = P.kmax, O = T / 2;
+    m.assign((size_t)(H + 1) * T * T, 0);
     auto M = [&](int kx, int ky, int kz) -> unsigned char& { return m[((size_t)kx * T + ky + O) * T + kz + O]; };
     const double pi = acos(-1.0);
-    KSet ks{};
     ks.kmax = K;
     const double rkmax2 = P.rkmax * P.rkmax;
     if (P.iamcub) {
@@ -309,8 +316,8 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
     ks.cubic = P.iamcub ? 1 : 0;
     for (int kx = 0; kx <= H; kx++) {
         int kyb = -1, kzb = 0;
-        for (int ky = -H; ky <= H; ky++)
-            for (int kz = -H; kz <= H; kz++)
+        for (int ky = -(O - 1); ky <= O - 1; ky++)
+            for (int kz = -(O - 1); kz <= O - 1; kz++)
                 if (kx <= K && M(kx, ky, kz)) {
                     kyb = std::max(kyb, std::abs(ky));
                     kzb = std::max(kzb, std::abs(kz));
@@ -324,10 +331,63 @@ int build_kset(rpmd_ctx* c, const double box[3]) {
             ks.kz_row[kx][iy] = (signed char)r;
         }
     }
+}
+
+int build_kset(rpmd_ctx* c, const double box[3]) {
+    const rpmd_params& P = c->prm;
+    double key[8] = {box[0], box[1], box[2], P.walpha, P.rkmax, (double)P.kmax, (double)P.iamcub, 0};
+    if (c->ks_valid && !memcmp(key, c->ks_key, sizeof key)) return 0;
+    if (P.kmax > RPMD_KMAX_DEV || P.kmax < 0) return RPMD_ERR_KMAX;
+    std::vector<unsigned char> m;
+    KSet ks{};
+    kset_fill(P, box, RPMD_KTILE, RPMD_KMAX_DEV, ks, m);
     HIPCHK(hipMemcpyAsync(c->kmask, m.data(), m.size(), hipMemcpyHostToDevice, c->st));
     c->ks = ks;
     memcpy(c->ks_key, key, sizeof key);
     c->ks_valid = true;
+    return 0;
+}
+
+// kmax 16 .. 31: the k set on 64 x 64 tiles, its mask on the device and the big-tile buffers (first
+// use, or a larger kmax, allocates them: always in an eager call, the step before a capture)
+int build_kset_big(rpmd_ctx* c, const double box[3]) {
+    const rpmd_params& P = c->prm;
+    double key[8] = {box[0], box[1], box[2], P.walpha, P.rkmax, (double)P.kmax, (double)P.iamcub, 0};
+    if (c->ksb_valid && !memcmp(key, c->ksb_key, sizeof key)) return 0;
+    if (P.kmax > RPMD_KMAX_BIG || P.kmax <= RPMD_KMAX_DEV) return RPMD_ERR_KMAX;
+    const int na = c->na, nbl = c->nb_max, T = RPMD_KTILE_BIG;
+    const size_t skp_n = (size_t)nbl * (P.kmax + 1) * kspace_nchunk(na) * T * T * 2;
+    const size_t gk_n = (size_t)nbl * (P.kmax + 1) * T * T * 2;
+    for (hipStream_t q : {c->st, c->st2, c->st3}) HIPCHK(hipStreamSynchronize(q));
+    if (c->skp_big_n < skp_n) {
+        if (c->skp_big) hipFree(c->skp_big);
+        c->skp_big = nullptr;
+        c->skp_big_n = 0;
+        RETCHK(dalloc(&c->skp_big, skp_n) ? RPMD_ERR_NOMEM : 0);
+        c->skp_big_n = skp_n;
+    }
+    if (c->gk_big_n < gk_n) {
+        if (c->gk_big) hipFree(c->gk_big);
+        c->gk_big = nullptr;
+        c->gk_big_n = 0;
+        RETCHK(dalloc(&c->gk_big, gk_n) ? RPMD_ERR_NOMEM : 0);
+        c->gk_big_n = gk_n;
+    }
+    if (!c->kmask_big) HIPCHK(hipMalloc((void**)&c->kmask_big, (size_t)(RPMD_KMAX_BIG + 1) * T * T));
+    if (!c->parts_big) RETCHK(dalloc(&c->parts_big, (size_t)nbl * (RPMD_KMAX_BIG + 1) * 10) ? RPMD_ERR_NOMEM : 0);
+    if (!c->fpart_big)
+        RETCHK(dalloc(&c->fpart_big,
+                      (size_t)(ewald_real_slots() + kspace_ngroups(RPMD_KMAX_BIG)) * 3 * na * nbl) ? RPMD_ERR_NOMEM
+                                                                                                  : 0);
+    std::vector<unsigned char> m;
+    KSetBig ks{};
+    kset_fill(P, box, T, RPMD_KMAX_BIG, ks, m);
+    HIPCHK(hipMemcpyAsync(c->kmask_big, m.data(), m.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->ksb = ks;
+    memcpy(c->ksb_key, key, sizeof key);
+    c->ksb_valid = true;
+    invalidate(c);
     return 0;
 }
 
@@ -583,9 +643,19 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
     }
     int chain_npart = 0, chain_acc = 0;
     double chain_scale = 1.0;
+    double* chain_fp = c->fpart;  // the Ewald gradient slots the chain rule sums
     bool fork = false;
     if (iopt <= 2) {
-        RETCHK(build_kset(c, box));
+        // kmax 16 .. 31: the FP64 big-tile k-space kernels (in either precision mode), with their own
+        // gradient slots
+        const bool bigk = P.kmax > RPMD_KMAX_DEV;
+        if (bigk)
+            RETCHK(build_kset_big(c, box));
+        else
+            RETCHK(build_kset(c, box));
+        const double* lat = bigk ? c->ksb.lat : c->ks.lat;
+        double* fp = bigk ? c->fpart_big : c->fpart;
+        chain_fp = fp;
         const int cubic = P.iamcub;
         const Box sb = cubic ? unit_box() : bx;
         const PairTab* ptab = nullptr;
@@ -606,10 +676,10 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             HIPCHK(hipEventRecord(c->ev_fork, s));
             HIPCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
             sr = c->st2;
-            launch_phases(pos, na, nb, P.alpha, cubic, bx, c->ks.lat, c->ph, s);
+            launch_phases(pos, na, nb, P.alpha, cubic, bx, lat, c->ph, s);
         }
         // sites, (phases,) list displacement and (for a built list) the sorted coordinates
-        launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, c->ks.lat, fork ? nullptr : c->ph, c->q, *vl, !force,
+        launch_sites_vl(pos, c->site, na, nb, P.alpha, cubic, bx, lat, fork ? nullptr : c->ph, c->q, *vl, !force,
                         sb, sr);
         {
             ProfScope ps(c, 4);
@@ -625,9 +695,9 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         {
             ProfScope ps(c, 0);
             if (mixed)
-                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr, fork ? EW_BPC : 0, ptab);
+                launch_ewald_real<float>(*vl, na, nb, sb, P.wrcut, P.walpha, fp, pr, sr, fork ? EW_BPC : 0, ptab);
             else
-                launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, c->fpart, pr, sr);
+                launch_ewald_real<double>(*vl, na, nb, sb, P.wrcut, P.walpha, fp, pr, sr);
         }
         if (P.wm > 0.0 || P.wh > 0.0) {
             // Gaussian charges: inverse widths of the M-M, H-H and M-H pairs, in unit-box lengths for
@@ -635,7 +705,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             const double ob = cubic ? 1.0 / box[0] : 1.0;
             const double sm = sqrt(1.0 / (2.0 * P.wm)) * ob, sh = sqrt(1.0 / (2.0 * P.wh)) * ob;
             const double wid[3] = {1.0 / (2.0 * sm), 1.0 / (2.0 * sh), 1.0 / sqrt(2.0 * (sm * sm + sh * sh))};
-            launch_ewald_smear(*vl, na, nb, sb, P.wrcut, wid, c->fpart, pr, sr);
+            launch_ewald_smear(*vl, na, nb, sb, P.wrcut, wid, fp, pr, sr);
         }
         if (oo_branch && !fork3) {
             oo_work(c->st2);
@@ -648,20 +718,25 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
             }
         }
         if (fork) HIPCHK(hipEventRecord(c->ev_join, c->st2));
-        double* pk = c->parts + c->off_k;
-        {
-            ProfScope ps(c, 1);
-            launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, mixed, s);
-        }
-        {
-            ProfScope ps(c, 3);
-            launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, c->gks, pk, mixed, s);
-        }
+        double* pk = bigk ? c->parts_big : c->parts + c->off_k;
         int nkslots = 0;
-        {
-            ProfScope ps(c, 2);
-            nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->gks,
-                                    c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
+        if (bigk) {
+            nkslots = launch_kspace_big(c->ph, c->q, na, nb, c->ksb, c->kmask_big, c->skp_big, c->gk_big, pk,
+                                        fp + (size_t)ewald_real_slots() * 3 * na * nb, s);
+        } else {
+            {
+                ProfScope ps(c, 1);
+                launch_sk(c->ph, c->q, na, nb, c->ks, c->skp, mixed, s);
+            }
+            {
+                ProfScope ps(c, 3);
+                launch_sk_final(na, nb, c->ks, c->kmask, c->skp, c->gk, c->gks, pk, mixed, s);
+            }
+            {
+                ProfScope ps(c, 2);
+                nkslots = launch_kforce(c->ph, c->q, na, nb, c->ks, c->gk, c->gks,
+                                        c->fpart + (size_t)ewald_real_slots() * 3 * na * nb, mixed, s);
+            }
         }
         if (fork) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
         const double sc = cubic ? 1.0 / box[0] : 1.0;
@@ -689,11 +764,11 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
                 kick->red = defer;
                 kick->ev_out = c->ev_step;
             }
-            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo,
+            launch_chain_rule(grad, chain_fp, chain_npart, na, nb, P.alpha, chain_scale, gacc, s, c->goo,
                               fuse ? kick : nullptr);
             if (fuse) *kick_done = defer ? 2 : 1;
         } else if (!need_oo) {
-            launch_chain_rule(grad, c->fpart, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
+            launch_chain_rule(grad, chain_fp, chain_npart, na, nb, P.alpha, chain_scale, gacc, s);
         }
         gacc = 1;
     }
@@ -701,7 +776,7 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         // its job was added with the chain rule
     } else if (need_oo) {
         double* po = c->parts + c->off_oo;
-        const double* chp = chain_npart ? c->fpart : nullptr;
+        const double* chp = chain_npart ? chain_fp : nullptr;
         if (mixed)
             launch_oo<float>(pos, na, nb, bx, c->nnb, c->nbl, use_list, P.rcut, P, grad, po, s, chp, chain_npart,
                              P.alpha, chain_scale, chain_acc, oo_dmax);
@@ -1694,7 +1769,7 @@ const char* rpmd_strerror(int code) {
         case RPMD_ERR_REALFT_MODE: return "realft 3";
         case RPMD_ERR_EWC_CUTOFF: return " ewc_basic : rcut too large ";
         case RPMD_ERR_SMEARED: return "ewc_correction not yet implemented for smeared charges";
-        case RPMD_ERR_KMAX: return "Ewald kmax exceeds the device k tile (15)";
+        case RPMD_ERR_KMAX: return "Ewald kmax exceeds the device k tiles (31)";
         case RPMD_ERR_ARG: return "invalid argument";
         case RPMD_ERR_HIP: return "HIP runtime error";
         case RPMD_ERR_IOPT: return "invalid iopt";
@@ -1876,6 +1951,7 @@ int rpmd_ctx_destroy(rpmd_ctx* c) {
     drop_graph(c);
     free_vcache(c);
     void* bufs[] = {c->ph, c->fpart, c->bussi_parts, c->bussi_alpha, c->gle_gp, c->gle_TS, c->baro_parts, c->baro_ev,
+                    c->kmask_big, c->skp_big, c->gk_big, c->parts_big, c->fpart_big,
                     c->gle_parts, c->gle_pcom, c->rat_err, c->rig_vparts, c->aos, c->pos, c->grad, c->site, c->gsite, c->q, c->mass, c->cen, c->parts, c->ev,
                     c->skp, c->gk, c->gks, c->kmask, c->cell_of, c->cnt, c->start, c->perm, c->nnb, c->nbl, c->ovf,
                     c->oo_nnb, c->oo_nbl, c->oo_ovf, c->oo_cb, c->oo_dmax, c->p,

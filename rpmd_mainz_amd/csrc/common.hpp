@@ -14,6 +14,8 @@
 
 #define RPMD_KTILE 32   // k-space tile: ky, kz in [-16, 15] (index k + 16) -> kmax <= 15
 #define RPMD_KMAX_DEV 15
+#define RPMD_KTILE_BIG 64  // kmax 16 .. 31: ky, kz in [-32, 31] (index k + 32), FP64 kernels
+#define RPMD_KMAX_BIG 31
 #define RPMD_NCAP 512   // O-O / TEM neighbour capacity per atom (full list)
 #define RPMD_STREAMS_PER_STEP 64  // counter-based noise streams reserved per propagation step
 
@@ -69,6 +71,18 @@ struct KSet {
     double lat[3];                    // 2 pi (cubic, unit box) or 2 pi / L_d
     double f0;                        // 2 pi (cubic) or 2 pi / V
     double b;                         // 1/(4 alpha^2)
+    int cubic;
+};
+
+// the k set of kmax 16 .. 31 on 64 x 64 tiles (same meaning as KSet's fields)
+struct KSetBig {
+    int kmax;
+    int ky_max[RPMD_KMAX_BIG + 1];
+    int kz_bound[RPMD_KMAX_BIG + 1];
+    signed char kz_row[RPMD_KMAX_BIG + 1][RPMD_KTILE_BIG];
+    double lat[3];
+    double f0;
+    double b;
     int cubic;
 };
 
@@ -233,6 +247,10 @@ void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask,
                      double* gks, double* parts_k, int fp32, hipStream_t s);
 int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet& ks, const double* gk,
                   const double* gks, double* kpart, int fp32, hipStream_t s);
+// kmax 16 .. 31: structure factor, weights and force pass in FP64 on 64 x 64 tiles (either precision
+// mode); returns the partial-gradient slots written
+int launch_kspace_big(const double* ph, const double* q, int na, int nb, const KSetBig& ks, const unsigned char* dmask,
+                      double* skp, double* gk, double* parts_k, double* kpart, hipStream_t s);
 void launch_nlist(const double* cen, int na, int njump, Box box, double cut, int* nnb, int* nbl, int* overflow,
                   hipStream_t s);
 void launch_nlist_vl(const double* cen, int na, Box box, double cut, double skin, int force, int* vnnb, int* vnbl,

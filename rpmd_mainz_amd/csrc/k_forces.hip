@@ -1403,15 +1403,22 @@ void launch_ewald_smear(const VList& v, int na, int nb, Box sb, double rcut, con
 #define KXB 4
 #define KXF 4
 
-struct KDev {
+// the k set on the device for tiles of TT entries per ky and kz row (index k + TT / 2) and kx up to HH
+template <int TT, int HH>
+struct KDevT {
+    static constexpr int T = TT, O = TT / 2, H = HH;
     int kmax;
     int cubic;
-    int kyb[KH + 1];   // per kx: max |ky| with a member (-1: none)
-    int kzb[KH + 1];   // per kx: max |kz| with a member
-    signed char kzr[KH + 1][KT];  // per (kx, ky + KO): max |kz| with a member (-1: none)
+    int kyb[HH + 1];   // per kx: max |ky| with a member (-1: none)
+    int kzb[HH + 1];   // per kx: max |kz| with a member
+    signed char kzr[HH + 1][TT];  // per (kx, ky + O): max |kz| with a member (-1: none)
     double lat0, lat1, lat2;
     double f0, b;
 };
+typedef KDevT<KT, KH> KDev;
+// kmax 16 .. 31 (FP64 in both precision modes): 64 x 64 tiles, the structure factor formed by the
+// 32 x 32 kernel over four windows of the tile
+typedef KDevT<2 * KT, 2 * KH + 1> KDevBig;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -1442,10 +1449,12 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 
-// T: MFMA precision (FP64; the mixed mode uses k_sk_h16)
-template <typename T>
+// T: MFMA precision (FP64; the mixed mode uses k_sk_h16).  KD: the k set's tile; the block forms the
+// 32 x 32 window of it with ky from oy and kz from oz (the whole tile, oy = oz = -16, for kmax <= 15)
+template <typename T, typename KD>
 __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph, const double* __restrict__ q, int na,
-                                                 int nchunk, KDev kd, double2* __restrict__ skp) {
+                                                 int nchunk, KD kd, int oy, int oz, double2* __restrict__ skp) {
+    constexpr int TT = KD::T, TO = KD::O;
     typedef typename Mfma<T>::acc acc_t;
     const int kxg = blockIdx.x, ch = blockIdx.y, b = blockIdx.z;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
@@ -1467,23 +1476,23 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
         {
             // Y and Z rows: thread -> (atom j = t/8, entries 4*(t%8) .. +3)
             const int j = t >> 3, part = t & 7, a = base + j;
-            const int k0 = 4 * part - KO;
+            const int ky0 = 4 * part + oy, kz0 = 4 * part + oz;
             double2 y1 = make_double2(1.0, 0.0), z1 = make_double2(1.0, 0.0);
             const bool in = a < a1;
             if (in) {
                 y1 = PY[a];
                 z1 = PZ[a];
             }
-            double2 yk = k0 >= 0 ? cpow(y1, k0) : cpow(make_double2(y1.x, -y1.y), -k0);
-            double2 zk = k0 >= 0 ? cpow(z1, k0) : cpow(make_double2(z1.x, -z1.y), -k0);
+            double2 yk = ky0 >= 0 ? cpow(y1, ky0) : cpow(make_double2(y1.x, -y1.y), -ky0);
+            double2 zk = kz0 >= 0 ? cpow(z1, kz0) : cpow(make_double2(z1.x, -z1.y), -kz0);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const int k = k0 + m;
-                const bool ok = in && k >= -kmax && k <= kmax;
-                Yr[j][4 * part + m] = ok ? (T)yk.x : (T)0;
-                Yi[j][4 * part + m] = ok ? (T)yk.y : (T)0;
-                Zr[j][4 * part + m] = ok ? (T)zk.x : (T)0;
-                Zi[j][4 * part + m] = ok ? (T)zk.y : (T)0;
+                const int ky = ky0 + m, kz = kz0 + m;
+                const bool oky = in && ky >= -kmax && ky <= kmax, okz = in && kz >= -kmax && kz <= kmax;
+                Yr[j][4 * part + m] = oky ? (T)yk.x : (T)0;
+                Yi[j][4 * part + m] = oky ? (T)yk.y : (T)0;
+                Zr[j][4 * part + m] = okz ? (T)zk.x : (T)0;
+                Zi[j][4 * part + m] = okz ? (T)zk.y : (T)0;
                 yk = cmul(yk, y1);
                 zk = cmul(zk, z1);
             }
@@ -1521,11 +1530,11 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2* __restrict__ ph,
     for (int g = 0; g < KXB; g++) {
         const int kx = kxg * KXB + g;
         if (kx > kmax) break;
-        double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (KT * KT);
+        double2* o = skp + (((size_t)b * (kmax + 1) + kx) * nchunk + ch) * (TT * TT);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int iy = 16 * qa + Mfma<T>::row(lk, v), iz = 16 * qc + li;
-            o[iy * KT + iz] = make_double2((double)accR[g][v], (double)accI[g][v]);
+            const int iy = 16 * qa + Mfma<T>::row(lk, v) + oy + TO, iz = 16 * qc + li + oz + TO;
+            o[iy * TT + iz] = make_double2((double)accR[g][v], (double)accI[g][v]);
         }
     }
 }
@@ -1562,17 +1571,20 @@ __device__ __forceinline__ flt4 mfma_h16(uint4 a, uint4 b, flt4 c) {
 // (mixed mode): instead of G, the A operand of k_kforce_h16 -- (G_r, -G_i) s as split half2 rows,
 // rows 32..63 weighted by kz, hi then lo ([2][64][32] dwords per (bead, kx)) -- and 1 / (s 2^10) in
 // gks, where the power of two s brings the largest |component| of the tile to [2^10, 2^11)
-template <typename P, bool SPLIT>
-__device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nchunk, const KDev& kd,
+template <typename P, bool SPLIT, typename KD = KDev>
+__device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nchunk, const KD& kd,
                                               const unsigned char* __restrict__ mask, double2* __restrict__ gk,
                                               double* __restrict__ gks, double* __restrict__ parts, const int kx,
                                               const int b) {
+    // TT x TT entries of the tile, four per thread (blocks of TT * TT / 4 threads)
+    constexpr int TT = KD::T, TO = KD::O, NT = TT * TT / 4;
+    static_assert(!SPLIT || TT == KT, "the split force-pass tiles are 32 x 32");
     const int kmax = kd.kmax;
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    const P* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (KT * KT);
-    // thread -> entries e = threadIdx.x + 256 i (iy = e / 32, iz = e % 32); each entry sums its chunks in
+    const P* src = skp + ((size_t)b * (kmax + 1) + kx) * nchunk * (TT * TT);
+    // thread -> entries e = threadIdx.x + NT i (iy = e / TT, iz = e % TT); each entry sums its chunks in
     // chunk order, the loads of four entries and up to four chunks in flight together
     double2 S[4];
 #pragma unroll
@@ -1583,7 +1595,7 @@ __device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nch
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) v[c][i] = src[(size_t)(ch + c) * (KT * KT) + threadIdx.x + 256 * i];
+            for (int i = 0; i < 4; i++) v[c][i] = src[(size_t)(ch + c) * (TT * TT) + threadIdx.x + NT * i];
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
@@ -1595,7 +1607,7 @@ __device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nch
     for (; ch < nchunk; ch++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const P v = src[(size_t)ch * (KT * KT) + threadIdx.x + 256 * i];
+            const P v = src[(size_t)ch * (TT * TT) + threadIdx.x + NT * i];
             S[i].x += (double)v.x;
             S[i].y += (double)v.y;
         }
@@ -1603,10 +1615,10 @@ __device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nch
     double m = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int e = threadIdx.x + 256 * i, iy = e >> 5, iz = e & 31;
+        const int e = threadIdx.x + NT * i, iy = e / TT, iz = e % TT;
         G[i] = make_double2(0.0, 0.0);
-        if (mask[(kx * KT + iy) * KT + iz]) {
-            const int ky = iy - KO, kz = iz - KO;
+        if (mask[(kx * TT + iy) * TT + iz]) {
+            const int ky = iy - TO, kz = iz - TO;
             const double rkx = kd.lat0 * kx, rky = kd.lat1 * ky, rkz = kd.lat2 * kz;
             const double rk2 = rkx * rkx + rky * rky + rkz * rkz;
             const double f = kx >= 1 ? 2.0 * kd.f0 : kd.f0;
@@ -1627,9 +1639,9 @@ __device__ __forceinline__ void sk_final_body(const P* __restrict__ skp, int nch
         }
         m = fmax(m, fmax(fabs(G[i].x), fabs(G[i].y)));
     }
-    if (!SPLIT) {
+    if constexpr (!SPLIT) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) gk[((size_t)b * (kmax + 1) + kx) * (KT * KT) + threadIdx.x + 256 * i] = G[i];
+        for (int i = 0; i < 4; i++) gk[((size_t)b * (kmax + 1) + kx) * (TT * TT) + threadIdx.x + NT * i] = G[i];
     } else {
         __shared__ double wmax[4];
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
@@ -1657,6 +1669,12 @@ __global__ __launch_bounds__(256) void k_sk_final(const P* __restrict__ skp, int
                                                   double* __restrict__ gks, double* __restrict__ parts) {
     sk_final_body<P, SPLIT>(skp, nchunk, kd, mask, gk, gks, parts, blockIdx.x, blockIdx.y);
 }
+// kmax 16 .. 31: the 64 x 64 tiles in FP64, 1024 threads per (kx, bead)
+__global__ __launch_bounds__(1024) void k_sk_final_big(const double2* __restrict__ skp, int nchunk, KDevBig kd,
+                                                       const unsigned char* __restrict__ mask,
+                                                       double2* __restrict__ gk, double* __restrict__ parts) {
+    sk_final_body<double2, false, KDevBig>(skp, nchunk, kd, mask, gk, nullptr, parts, blockIdx.x, blockIdx.y);
+}
 
 // Horner over kz of two ky rows: h = sum_m c_m w^m, d = dh/dw, with c_m = G(kz = m - KZ).  After the
 // leading coefficient there are 2 KZ steps, taken two at a time with the four coefficients loaded
@@ -1665,15 +1683,16 @@ __device__ __forceinline__ void horner_step(double2& h, double2& d, double2 w, d
     d = make_double2(fma(d.x, w.x, fma(-d.y, w.y, h.x)), fma(d.x, w.y, fma(d.y, w.x, h.y)));
     h = make_double2(fma(h.x, w.x, fma(-h.y, w.y, g.x)), fma(h.x, w.y, fma(h.y, w.x, g.y)));
 }
+template <int TO = KO>
 __device__ __forceinline__ void horner2(const double2* __restrict__ ra, const double2* __restrict__ rb, int KZ,
                                         double2 w, double2& ha, double2& da, double2& hb, double2& db) {
-    ha = ra[KZ + KO];
-    hb = rb[KZ + KO];
+    ha = ra[KZ + TO];
+    hb = rb[KZ + TO];
     da = make_double2(0.0, 0.0);
     db = make_double2(0.0, 0.0);
     for (int kz = KZ - 1; kz >= -KZ; kz -= 2) {
-        const double2 ga0 = ra[kz + KO], gb0 = rb[kz + KO];
-        const double2 ga1 = ra[kz - 1 + KO], gb1 = rb[kz - 1 + KO];
+        const double2 ga0 = ra[kz + TO], gb0 = rb[kz + TO];
+        const double2 ga1 = ra[kz - 1 + TO], gb1 = rb[kz - 1 + TO];
         horner_step(ha, da, w, ga0);
         horner_step(hb, db, w, gb0);
         horner_step(ha, da, w, ga1);
@@ -1682,8 +1701,10 @@ __device__ __forceinline__ void horner2(const double2* __restrict__ ra, const do
 }
 
 // one thread per (bead, atom, kx group); writes the group's partial gradient (unit-box / physical)
+template <typename KD>
 __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, const double* __restrict__ q, int na,
-                                                KDev kd, const double2* __restrict__ gk, double* __restrict__ kpart) {
+                                                KD kd, const double2* __restrict__ gk, double* __restrict__ kpart) {
+    constexpr int TT = KD::T, TO = KD::O;
     const int b = blockIdx.y, g = blockIdx.z, nb = gridDim.y;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= na) return;
@@ -1695,23 +1716,23 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
     const int kx0 = g * KXF, kx1 = min(kmax, kx0 + KXF - 1);
     double2 xk = cpow(X1, kx0);
     xk = make_double2(q[a] * xk.x, q[a] * xk.y);
-    const double2* G0 = gk + (size_t)b * (kmax + 1) * KT * KT;
+    const double2* G0 = gk + (size_t)b * (kmax + 1) * TT * TT;
     for (int kx = kx0; kx <= kx1; kx++) {
         if (kx > kx0) xk = cmul(xk, X1);
         const int KY = kd.kyb[kx];
         if (KY < 0) continue;
         double2 yc = cpow(Yc, KY);   // Y(-KY)
         double tx = 0.0, ty = 0.0, tz = 0.0;
-        const double2* Gx = G0 + (size_t)kx * KT * KT;
+        const double2* Gx = G0 + (size_t)kx * TT * TT;
         int ky = -KY;
         for (; ky + 1 <= KY; ky += 2) {
             // the pair's kz range: the larger of the two rows' bounds (G is zero beyond)
-            const int KZ = max((int)kd.kzr[kx][ky + KO], (int)kd.kzr[kx][ky + 1 + KO]);
+            const int KZ = max((int)kd.kzr[kx][ky + TO], (int)kd.kzr[kx][ky + 1 + TO]);
             const double2 yb = cmul(yc, Y1);
             if (KZ >= 0) {
                 const double2 zK = cpow(Z1, KZ);
                 double2 ha, da, hb, db;
-                horner2(Gx + (ky + KO) * KT, Gx + (ky + 1 + KO) * KT, KZ, w, ha, da, hb, db);
+                horner2<TO>(Gx + (ky + TO) * TT, Gx + (ky + 1 + TO) * TT, KZ, w, ha, da, hb, db);
 #pragma unroll
                 for (int r = 0; r < 2; r++) {
                     const double2 h = r ? hb : ha, d = r ? db : da, yy = r ? yb : yc;
@@ -1728,11 +1749,11 @@ __global__ __launch_bounds__(256) void k_kforce(const double2* __restrict__ ph, 
             }
             yc = cmul(yb, Y1);
         }
-        const int KZ = ky <= KY ? (int)kd.kzr[kx][ky + KO] : -1;
+        const int KZ = ky <= KY ? (int)kd.kzr[kx][ky + TO] : -1;
         if (KZ >= 0) {
             const double2 zK = cpow(Z1, KZ);
             double2 ha, da, hb, db;
-            horner2(Gx + (ky + KO) * KT, Gx + (ky + KO) * KT, KZ, w, ha, da, hb, db);
+            horner2<TO>(Gx + (ky + TO) * TT, Gx + (ky + TO) * TT, KZ, w, ha, da, hb, db);
             const double2 U = cmul(zK, ha);
             const double2 wd = cmul(w, da);
             const double2 Uz = cmul(zK, make_double2(wd.x - KZ * ha.x, wd.y - KZ * ha.y));
@@ -2173,8 +2194,8 @@ void launch_sk(const double* ph, const double* q, int na, int nb, const KSet& ks
         return;
     }
     const int nchunk = kspace_nchunk(na);
-    k_sk_mfma<double><<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>((const double2*)ph, q, na, nchunk, kd,
-                                                                              (double2*)sk_part);
+    k_sk_mfma<double, KDev><<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>(
+        (const double2*)ph, q, na, nchunk, kd, -KO, -KO, (double2*)sk_part);
 }
 void launch_sk_final(int na, int nb, const KSet& ks, const unsigned char* dmask, const double* sk_part, double* gk,
                      double* gks, double* parts_k, int fp32, hipStream_t s) {
@@ -2197,7 +2218,38 @@ int launch_kforce(const double* ph, const double* q, int na, int nb, const KSet&
                                                                               (const uint4*)gk, gks, kpart);
         return 1;
     }
-    k_kforce<<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
+    k_kforce<KDev><<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
+        (const double2*)ph, q, na, kd, (const double2*)gk, kpart);
+    return kspace_ngroups(ks.kmax);
+}
+
+// kmax 16 .. 31 (both precision modes): the FP64 kernels on 64 x 64 tiles -- the structure factor
+// over the tile's four 32 x 32 windows, the weights and sums, the force pass; returns the number of
+// partial-gradient slots written.  skp: (kmax + 1) x chunks x 64 x 64 per bead, gk: (kmax + 1) x 64 x 64
+// per bead (double2), parts_k: (kmax + 1) x 10 per bead.
+int launch_kspace_big(const double* ph, const double* q, int na, int nb, const KSetBig& ks, const unsigned char* dmask,
+                      double* skp, double* gk, double* parts_k, double* kpart, hipStream_t s) {
+    KDevBig kd;
+    kd.kmax = ks.kmax;
+    for (int kx = 0; kx <= KDevBig::H; kx++) {
+        kd.kyb[kx] = ks.ky_max[kx];
+        kd.kzb[kx] = ks.kz_bound[kx];
+        for (int iy = 0; iy < KDevBig::T; iy++) kd.kzr[kx][iy] = ks.kz_row[kx][iy];
+    }
+    kd.lat0 = ks.lat[0];
+    kd.lat1 = ks.lat[1];
+    kd.lat2 = ks.lat[2];
+    kd.f0 = ks.f0;
+    kd.b = ks.b;
+    kd.cubic = ks.cubic;
+    const int nchunk = kspace_nchunk(na);
+    for (int oy = -KDevBig::O; oy < KDevBig::O; oy += KT)
+        for (int oz = -KDevBig::O; oz < KDevBig::O; oz += KT)
+            k_sk_mfma<double, KDevBig><<<dim3((ks.kmax + KXB) / KXB, nchunk, nb), 256, 0, s>>>(
+                (const double2*)ph, q, na, nchunk, kd, oy, oz, (double2*)skp);
+    k_sk_final_big<<<dim3(ks.kmax + 1, nb), 1024, 0, s>>>((const double2*)skp, nchunk, kd, dmask, (double2*)gk,
+                                                          parts_k);
+    k_kforce<KDevBig><<<dim3((na + 255) / 256, nb, kspace_ngroups(ks.kmax)), 256, 0, s>>>(
         (const double2*)ph, q, na, kd, (const double2*)gk, kpart);
     return kspace_ngroups(ks.kmax);
 }
