@@ -334,6 +334,9 @@ void launch_bead_forces(const double* r, double* g, const double* mass, int na, 
 // len2: the squared target lengths of the O-H1, O-H2 and H1-H2 constraints; tol: the RATTLE tolerance
 void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
                         double omegan2, const double len2[3], double tol, int* err, hipStream_t s);
+// all nsub bead sub-steps of a rigid step in one launch (the gradient at entry: bead_forces of r)
+void launch_rig_substeps(double* p, double* r, const double* mass, int na, int nb, int nsub, double hs, double sdt,
+                         double omegan2, const double len2[3], double tol, int* err, hipStream_t s);
 void launch_rig_final(double* p, const double* r, double* dvdr, const double* mass, int na, int nb, double hdt,
                       double dt, const double len2[3], double tol, int* err, double* vparts, double* vir_out,
                       hipStream_t s);

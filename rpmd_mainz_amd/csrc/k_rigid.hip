@@ -7,6 +7,8 @@
 // RATTLE call, so the per-molecule result is the reference's.  The reference stops when the
 // iteration count of the LAST molecule of a call reaches itmax (src/md_rattle.f90:144-146,
 // 257-259); that molecule's thread raises err[0] here, and the host turns it into RPMD_ERR_RATTLE.
+#include <algorithm>
+
 #include "common.hpp"
 
 // The RATTLE and rigid-step kernels round every product and sum once, as the reference's Fortran
@@ -233,6 +235,141 @@ __global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, d
     store_mol(g, na, b, m, G);
 }
 
+// All nsub bead sub-steps of a rigid step in one launch (src/evolve_rigid.f90:131-170): a block
+// owns every bead of MB molecules, thread (molecule m = t % MB, bead b = t / MB).  A molecule's
+// positions, momenta and gradient stay in registers through the sub-steps; the positions go through
+// LDS, where the bead forces read the neighbouring beads after a barrier.  Per sub-step: stage 1 as
+// k_rig_s1 (kick, drift, RATTLE positions), barrier, stage 2 as k_rig_s2 (bead forces, kick, RATTLE
+// velocities, constraint gradient into g), barrier.  The gradient at entry is bead_forces of the
+// entry positions (k_bead_forces).  The same operations, in the same order and rounding, as the
+// separate kernels.
+__global__ __launch_bounds__(256) void k_rig_substeps(double* __restrict__ p, double* __restrict__ r,
+                                                      const double* __restrict__ mass, int na, int nb, int mb,
+                                                      int nsub, double hs, double sdt, double omegan2, double len2_0,
+                                                      double len2_1, double len2_2, double tol,
+                                                      int* __restrict__ err) {
+    extern __shared__ double lds_r[];  // [bead][molecule of the block][atom][xyz]
+    const int nm = na / 3;
+    const int mloc = threadIdx.x % mb, b = threadIdx.x / mb;
+    const int m = blockIdx.x * mb + mloc;
+    const bool live = m < nm && b < nb;
+    const int b2 = b == 0 ? nb - 1 : b - 1, b3 = b == nb - 1 ? 0 : b + 1;
+    double* mine = lds_r + ((size_t)b * mb + mloc) * 9;
+    const double* left = lds_r + ((size_t)b2 * mb + mloc) * 9;
+    const double* right = lds_r + ((size_t)b3 * mb + mloc) * 9;
+    const double len2[3] = {len2_0, len2_1, len2_2};
+    Mol R, P, G;
+    double ms[3], inv_m[3];
+    if (live) {
+        load_mol(r, na, b, m, R);
+        load_mol(p, na, b, m, P);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            ms[j] = mass[3 * m + j];
+            inv_m[j] = 1.0 / ms[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) mine[3 * j + d] = R.x[j][d];
+    }
+    __syncthreads();
+    // bead_forces of the entry positions (k_bead_forces)
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double v = 0.0;
+                if (nb > 1) {
+                    const double dx1 = R.x[j][d], dx2 = left[3 * j + d], dx3 = right[3 * j + d];
+                    v = ms[j] * omegan2 * (2.0 * dx1 - dx2 - dx3);
+                }
+                G.x[j][d] = v;
+            }
+    }
+    for (int is = 0; is < nsub; is++) {
+        __syncthreads();  // every read of the positions in LDS is done
+        if (live) {
+            // stage 1 (k_rig_s1)
+            Mol R0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    R0.x[j][d] = R.x[j][d];
+                    P.x[j][d] = P.x[j][d] - hs * G.x[j][d];
+                    R.x[j][d] = R.x[j][d] + P.x[j][d] * sdt / ms[j];
+                }
+            bool settled = false;
+            int sweeps = 0;
+            while (!settled && sweeps < RAT_ITMAX) {
+                settled = true;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int ia = CON_A[c], ib = CON_B[c];
+                    double u[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) u[d] = R.x[ib][d] - R.x[ia][d];
+                    const double miss = len2[c] - dot3(u, u);
+                    if (fabs(miss) > len2[c] * tol) {
+                        double u0[3];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) u0[d] = R0.x[ib][d] - R0.x[ia][d];
+                        const double lam = miss / (2.0 * (inv_m[ib] + inv_m[ia]) * dot3(u, u0));
+                        double f[3];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) f[d] = lam * u0[d];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            R.x[ib][d] = R.x[ib][d] + inv_m[ib] * f[d];
+                            R.x[ia][d] = R.x[ia][d] - inv_m[ia] * f[d];
+                        }
+#pragma unroll
+                        for (int d = 0; d < 3; d++) f[d] = f[d] / sdt;
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            P.x[ib][d] = P.x[ib][d] + f[d];
+                            P.x[ia][d] = P.x[ia][d] - f[d];
+                        }
+                        settled = false;
+                    }
+                }
+                sweeps = sweeps + 1;
+            }
+            if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) mine[3 * j + d] = R.x[j][d];
+        }
+        __syncthreads();  // every bead's new positions are in LDS
+        if (live) {
+            // stage 2 (k_rig_s2)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double v = 0.0;
+                    if (nb > 1) {
+                        const double dx2 = left[3 * j + d], dx3 = right[3 * j + d];
+                        v = ms[j] * omegan2 * (2.0 * R.x[j][d] - dx2 - dx3);
+                    }
+                    G.x[j][d] = v;
+                    P.x[j][d] = P.x[j][d] - hs * v;
+                }
+            double vl = 0.0;
+            bool any = false;
+            const int sweeps = rattle_vel<false>(R, P, G, inv_m, len2, 0.5 * sdt, tol, nullptr, vl, any);
+            if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
+        }
+    }
+    if (live) {
+        store_mol(r, na, b, m, R);
+        store_mol(p, na, b, m, P);
+    }
+}
+
 // final stage of the step: p -= dt/2 dvdr; rattle_s2 with dt on every bead, adding the constraint
 // gradient to dvdr (src/evolve_rigid.f90:184-200).  Per block: the sums of the five accumulated
 // virial components and (highest molecule with a correction, its last gdx dz), vparts[blk][7].
@@ -380,6 +517,20 @@ void launch_rig_substep(double* p, double* r, double* g, const double* mass, int
     const unsigned grid = (unsigned)((n + 255) / 256);
     k_rig_s1<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, len2[0], len2[1], len2[2], tol, err);
     k_rig_s2<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, omegan2, len2[0], len2[1], len2[2], tol, err);
+}
+// the molecules of a block: every bead of each (nb <= 256), about one block per CU when the molecules
+// allow, at most 256 threads
+int rig_substep_mols(int nm, int nb) {
+    const int cap = std::max(1, 256 / nb);
+    return std::max(1, std::min(cap, nm / 256));
+}
+void launch_rig_substeps(double* p, double* r, const double* mass, int na, int nb, int nsub, double hs, double sdt,
+                         double omegan2, const double len2[3], double tol, int* err, hipStream_t s) {
+    const int nm = na / 3, mb = rig_substep_mols(nm, nb);
+    const unsigned grid = (unsigned)((nm + mb - 1) / mb);
+    const size_t lds = (size_t)mb * nb * 9 * sizeof(double);
+    k_rig_substeps<<<grid, mb * nb, lds, s>>>(p, r, mass, na, nb, mb, nsub, hs, sdt, omegan2, len2[0], len2[1],
+                                              len2[2], tol, err);
 }
 void launch_rig_project(double* p, const double* r, const double* mass, int na, int nb, const double len2[3],
                         double tol, int scaled, int* err, hipStream_t s) {
