@@ -1584,14 +1584,8 @@ int step_rig_dev(rpmd_ctx* c, NoiseCursor& nz, bool check) {
     };
     thermo();
     launch_kick(c->p, c->dvdr, halfdt, nullptr, 0.0, N, s);
-    if (nb <= 256) {
-        launch_rig_substeps(c->p, c->r, c->mass, na, nb, nb, halfsmalldt, smalldt, omegan, len2, tol, c->rat_err, s);
-    } else {
-        launch_bead_forces(c->r, c->dvdr2, c->mass, na, nb, omegan, s);
-        for (int is = 0; is < nb; is++)
-            launch_rig_substep(c->p, c->r, c->dvdr2, c->mass, na, nb, halfsmalldt, smalldt, omegan, len2, tol,
-                               c->rat_err, s);
-    }
+    // nb <= 256 (every context): the sub-steps' blocks hold all beads of their molecules
+    launch_rig_substeps(c->p, c->r, c->mass, na, nb, nb, halfsmalldt, smalldt, omegan, len2, tol, c->rat_err, s);
     RETCHK(forces_dev(c, c->r, nb, box, 1, c->dvdr, c->ev_step, check));
     launch_rig_final(c->p, c->r, c->dvdr, c->mass, na, nb, halfdt, dt, len2, tol, c->rat_err, c->rig_vparts,
                      c->ev_step + 50, s);

@@ -329,11 +329,7 @@ void launch_estimators(const double* p, const double* r, const double* g1, const
 
 // ---- k_rigid.hip ------------------------------------------------------------------------------
 int rigid_blocks(int na);
-void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,
-                        hipStream_t s);
 // len2: the squared target lengths of the O-H1, O-H2 and H1-H2 constraints; tol: the RATTLE tolerance
-void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
-                        double omegan2, const double len2[3], double tol, int* err, hipStream_t s);
 // all nsub bead sub-steps of a rigid step in one launch (the gradient at entry: bead_forces of r)
 void launch_rig_substeps(double* p, double* r, const double* mass, int na, int nb, int nsub, double hs, double sdt,
                          double omegan2, const double len2[3], double tol, int* err, hipStream_t s);

@@ -104,145 +104,22 @@ __device__ __forceinline__ int rattle_vel(const Mol& r, Mol& p, Mol& g, const do
 
 }  // namespace
 
-// bead_forces (src/evolve_rigid.f90:236-278): g = m omega_n^2 (2 x_k - x_{k-1} - x_{k+1})
-__global__ void k_bead_forces(const double* __restrict__ r, double* __restrict__ g, const double* __restrict__ mass,
-                              int na, int nb, double omegan2) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)na * nb) return;
-    const int k = (int)(t / na), a = (int)(t % na);
-    const int k2 = k == 0 ? nb - 1 : k - 1, k3 = k == nb - 1 ? 0 : k + 1;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double v = 0.0;
-        if (nb > 1) {
-            const double dx1 = r[((size_t)k * 3 + d) * na + a], dx2 = r[((size_t)k2 * 3 + d) * na + a],
-                         dx3 = r[((size_t)k3 * 3 + d) * na + a];
-            v = mass[a] * omegan2 * (2.0 * dx1 - dx2 - dx3);
-        }
-        g[((size_t)k * 3 + d) * na + a] = v;
-    }
-}
-
-// stage 1 of one bead sub-step: p -= hs g; r += p sdt / m; then the position stage of RATTLE
-// (rattle_s1, src/md_rattle.f90:33-158): per constrained pair with squared length off its target by
-// more than tol (relative), the pair moves along its bond vector u_0 at the start of the sub-step by
-// lam = (|u_0|^2_target - |u|^2) / (2 (1/m_a + 1/m_b) u . u_0), weighted by the inverse masses, and
-// the momenta take lam u_0 / sdt.  The constraint gradient rattle_s1 adds to g is not kept:
-// bead_forces overwrites g next.
-__global__ void k_rig_s1(double* __restrict__ p, double* __restrict__ r, const double* __restrict__ g,
-                         const double* __restrict__ mass, int na, int nb, double hs, double sdt, double len2_0,
-                         double len2_1, double len2_2, double tol, int* __restrict__ err) {
-    const int nm = na / 3;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nm * nb) return;
-    const int b = (int)(t / nm), m = (int)(t % nm);
-    Mol R, P, G, R0;
-    load_mol(r, na, b, m, R);
-    load_mol(p, na, b, m, P);
-    load_mol(g, na, b, m, G);
-    double ms[3], inv_m[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        ms[j] = mass[3 * m + j];
-        inv_m[j] = 1.0 / ms[j];
-    }
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            R0.x[j][d] = R.x[j][d];
-            P.x[j][d] = P.x[j][d] - hs * G.x[j][d];
-            R.x[j][d] = R.x[j][d] + P.x[j][d] * sdt / ms[j];
-        }
-    const double len2[3] = {len2_0, len2_1, len2_2};
-    bool settled = false;
-    int sweeps = 0;
-    while (!settled && sweeps < RAT_ITMAX) {
-        settled = true;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int ia = CON_A[c], ib = CON_B[c];
-            double u[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) u[d] = R.x[ib][d] - R.x[ia][d];
-            const double miss = len2[c] - dot3(u, u);
-            if (fabs(miss) > len2[c] * tol) {
-                double u0[3];
-#pragma unroll
-                for (int d = 0; d < 3; d++) u0[d] = R0.x[ib][d] - R0.x[ia][d];
-                const double lam = miss / (2.0 * (inv_m[ib] + inv_m[ia]) * dot3(u, u0));
-                double f[3];
-#pragma unroll
-                for (int d = 0; d < 3; d++) f[d] = lam * u0[d];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    R.x[ib][d] = R.x[ib][d] + inv_m[ib] * f[d];
-                    R.x[ia][d] = R.x[ia][d] - inv_m[ia] * f[d];
-                }
-#pragma unroll
-                for (int d = 0; d < 3; d++) f[d] = f[d] / sdt;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    P.x[ib][d] = P.x[ib][d] + f[d];
-                    P.x[ia][d] = P.x[ia][d] - f[d];
-                }
-                settled = false;
-            }
-        }
-        sweeps = sweeps + 1;
-    }
-    if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
-    store_mol(r, na, b, m, R);
-    store_mol(p, na, b, m, P);
-}
-
-// stage 2 of one bead sub-step: g = bead_forces(r); p -= hs g; rattle_s2 with the sub-step
-// (which also adds its constraint gradient to g, as the reference's next sub-step then uses it)
-__global__ void k_rig_s2(double* __restrict__ p, const double* __restrict__ r, double* __restrict__ g,
-                         const double* __restrict__ mass, int na, int nb, double hs, double sdt, double omegan2,
-                         double len2_0, double len2_1, double len2_2, double tol, int* __restrict__ err) {
-    const int nm = na / 3;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nm * nb) return;
-    const int b = (int)(t / nm), m = (int)(t % nm);
-    const int b2 = b == 0 ? nb - 1 : b - 1, b3 = b == nb - 1 ? 0 : b + 1;
-    Mol R, P, G;
-    load_mol(r, na, b, m, R);
-    load_mol(p, na, b, m, P);
-    double inv_m[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        const int a = 3 * m + j;
-        const double ms = mass[a];
-        inv_m[j] = 1.0 / ms;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double v = 0.0;
-            if (nb > 1) {
-                const double dx2 = r[((size_t)b2 * 3 + d) * na + a], dx3 = r[((size_t)b3 * 3 + d) * na + a];
-                v = ms * omegan2 * (2.0 * R.x[j][d] - dx2 - dx3);
-            }
-            G.x[j][d] = v;
-            P.x[j][d] = P.x[j][d] - hs * v;
-        }
-    }
-    const double len2[3] = {len2_0, len2_1, len2_2};
-    double vl = 0.0;
-    bool any = false;
-    const int sweeps = rattle_vel<false>(R, P, G, inv_m, len2, 0.5 * sdt, tol, nullptr, vl, any);
-    if (m == nm - 1 && sweeps >= RAT_ITMAX) err[0] = 1;
-    store_mol(p, na, b, m, P);
-    store_mol(g, na, b, m, G);
-}
-
 // All nsub bead sub-steps of a rigid step in one launch (src/evolve_rigid.f90:131-170): a block
 // owns every bead of MB molecules, thread (molecule m = t % MB, bead b = t / MB).  A molecule's
 // positions, momenta and gradient stay in registers through the sub-steps; the positions go through
-// LDS, where the bead forces read the neighbouring beads after a barrier.  Per sub-step: stage 1 as
-// k_rig_s1 (kick, drift, RATTLE positions), barrier, stage 2 as k_rig_s2 (bead forces, kick, RATTLE
-// velocities, constraint gradient into g), barrier.  The gradient at entry is bead_forces of the
-// entry positions (k_bead_forces).  The same operations, in the same order and rounding, as the
-// separate kernels.
+// LDS, where the bead forces read the neighbouring beads after a barrier.  The gradient at entry is
+// bead_forces (src/evolve_rigid.f90:236-278: g = m omega_n^2 (2 x_k - x_{k-1} - x_{k+1})) of the
+// entry positions.  Per sub-step:
+//   stage 1: p -= hs g; r += p sdt / m; then the position stage of RATTLE (rattle_s1,
+//     src/md_rattle.f90:33-158): per constrained pair with squared length off its target by more
+//     than tol (relative), the pair moves along its bond vector u_0 at the start of the sub-step by
+//     lam = (|u_0|^2_target - |u|^2) / (2 (1/m_a + 1/m_b) u . u_0), weighted by the inverse masses,
+//     and the momenta take lam u_0 / sdt (the constraint gradient rattle_s1 adds to g is not kept:
+//     bead_forces overwrites g next);
+//   barrier;
+//   stage 2: g = bead_forces(r); p -= hs g; rattle_s2 with the sub-step, which adds its constraint
+//     gradient to g (the next sub-step's stage 1 uses it, as the reference's does);
+//   barrier.
 __global__ __launch_bounds__(256) void k_rig_substeps(double* __restrict__ p, double* __restrict__ r,
                                                       const double* __restrict__ mass, int na, int nb, int mb,
                                                       int nsub, double hs, double sdt, double omegan2, double len2_0,
@@ -274,7 +151,7 @@ __global__ __launch_bounds__(256) void k_rig_substeps(double* __restrict__ p, do
             for (int d = 0; d < 3; d++) mine[3 * j + d] = R.x[j][d];
     }
     __syncthreads();
-    // bead_forces of the entry positions (k_bead_forces)
+    // bead_forces of the entry positions
     if (live) {
 #pragma unroll
         for (int j = 0; j < 3; j++)
@@ -291,7 +168,7 @@ __global__ __launch_bounds__(256) void k_rig_substeps(double* __restrict__ p, do
     for (int is = 0; is < nsub; is++) {
         __syncthreads();  // every read of the positions in LDS is done
         if (live) {
-            // stage 1 (k_rig_s1)
+            // stage 1
             Mol R0;
 #pragma unroll
             for (int j = 0; j < 3; j++)
@@ -345,7 +222,7 @@ __global__ __launch_bounds__(256) void k_rig_substeps(double* __restrict__ p, do
         }
         __syncthreads();  // every bead's new positions are in LDS
         if (live) {
-            // stage 2 (k_rig_s2)
+            // stage 2
 #pragma unroll
             for (int j = 0; j < 3; j++)
 #pragma unroll
@@ -445,10 +322,12 @@ __global__ __launch_bounds__(256) void k_rig_final(double* __restrict__ p, const
 
 // vir_hf = (1/nb) sum_k vir_k with vir_k(1,3) = vir_k(1,2) - (last gdx dz of bead k) and the
 // reference's symmetrisation (src/md_rattle.f90:264-266); out[9] column-major
-__global__ void k_rig_vir(const double* __restrict__ vparts, int nblk, int nb, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nb; b++) {
+__global__ __launch_bounds__(256) void k_rig_vir(const double* __restrict__ vparts, int nblk, int nb,
+                                                 double* __restrict__ out) {
+    // thread b: bead b's virial from its blocks' parts in block order; thread 0 then sums the beads in
+    // bead order (the order of one serial loop over beads and blocks)
+    __shared__ double wb[256][9];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double bw = -1.0, bv = 0.0;
         for (int q = 0; q < nblk; q++) {
@@ -462,8 +341,13 @@ __global__ void k_rig_vir(const double* __restrict__ vparts, int nblk, int nb, d
         const double v11 = s[0], v12 = s[1], v22 = s[2], v23 = s[3], v33 = s[4];
         const double v13 = bw >= 0.0 ? v12 - bv : 0.0;
         const double w[9] = {v11, v12, v13, v12, v22, v23, v13, v23, v33};  // column-major, symmetric
-        for (int k = 0; k < 9; k++) acc[k] = acc[k] + w[k];
+        for (int k = 0; k < 9; k++) wb[b][k] = w[k];
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 9; k++) acc[k] = acc[k] + wb[b][k];
     for (int k = 0; k < 9; k++) out[k] = acc[k] / (double)nb;
 }
 
@@ -506,18 +390,6 @@ __global__ void k_rig_project(double* __restrict__ p, const double* __restrict__
 
 int rigid_blocks(int na) { return (na / 3 + 255) / 256; }
 
-void launch_bead_forces(const double* r, double* g, const double* mass, int na, int nb, double omegan2,
-                        hipStream_t s) {
-    const size_t n = (size_t)na * nb;
-    k_bead_forces<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(r, g, mass, na, nb, omegan2);
-}
-void launch_rig_substep(double* p, double* r, double* g, const double* mass, int na, int nb, double hs, double sdt,
-                        double omegan2, const double len2[3], double tol, int* err, hipStream_t s) {
-    const size_t n = (size_t)(na / 3) * nb;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    k_rig_s1<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, len2[0], len2[1], len2[2], tol, err);
-    k_rig_s2<<<grid, 256, 0, s>>>(p, r, g, mass, na, nb, hs, sdt, omegan2, len2[0], len2[1], len2[2], tol, err);
-}
 // the molecules of a block: every bead of each (nb <= 256), about one block per CU when the molecules
 // allow, at most 256 threads
 int rig_substep_mols(int nm, int nb) {
@@ -543,7 +415,7 @@ void launch_rig_final(double* p, const double* r, double* dvdr, const double* ma
                       hipStream_t s) {
     const dim3 grid(rigid_blocks(na), nb);
     k_rig_final<<<grid, 256, 0, s>>>(p, r, dvdr, mass, na, hdt, dt, len2[0], len2[1], len2[2], tol, err, vparts);
-    k_rig_vir<<<1, 64, 0, s>>>(vparts, (int)grid.x, nb, vir_out);
+    k_rig_vir<<<1, 256, 0, s>>>(vparts, (int)grid.x, nb, vir_out);
 }
 
 }  // namespace rpmd
