@@ -151,8 +151,8 @@ struct VList {
     int* iperm;      // [na] site index -> sorted index
     int* cnt;        // [na] list length of sorted site k
     int* nex;        // [na] leading list entries that are the site's own molecule
-    int* nl;         // [cap / VLG + 1][na][VLS][4] sorted neighbour indices: entry n of k in group n / VLG,
-                     // lane n % VLS, slot (n / VLS) % 4
+    int* nl;         // [cap / VLG + 1][na][VLS][4] sorted neighbours j as byte offsets 16 j: entry n of k in
+                     // group n / VLG, lane n % VLS, slot (n / VLS) % 4
     double* rb;      // [nb][3][na] site coordinates at the last build
     double4* srt;    // [nb][na + 1] current coordinates and charge in sorted order; entry na of a bead is
                      // the charge-free pad site (zero coordinates)

@@ -601,7 +601,8 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // spreads s): every bead pair then lies within rl of each other only if listed, because
 // |x_kb - x_jb| >= |c_k - c_j| - s_k - s_j.  First the other sites of its own molecule (nex[] of them;
 // the real-space term treats those pairs as excluded, src/ewald_cubic.f90:134), then the others in
-// ascending sorted order.  Entry n of site k is nl[((n/VLG)*na + k)*VLG + (n%VLS)*4 + (n/VLS)%4]: lane
+// ascending sorted order, each entry stored as 16 j (the byte offset of site j's 16-byte FP32 coordinate
+// record).  Entry n of site k is nl[((n/VLG)*na + k)*VLG + (n%VLS)*4 + (n/VLS)%4]: lane
 // n % VLS of the VLS lanes that walk a site's list reads its next four entries as one 16-byte load,
 // the site's lanes read 64 contiguous bytes, and consecutive entries are usually neighbouring sites.
 // The last group of a site is filled up with the pad site na (no charge), and group cap/VLG holds it
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
     int a = -3, n = 0;
     bool ovf = false;
     auto put = [&](int p, int j) {
-        if (p < cap) nl[((size_t)(p / VLG) * na + k) * VLG + (p % VLS) * 4 + (p / VLS) % 4] = j;
+        if (p < cap) nl[((size_t)(p / VLG) * na + k) * VLG + (p % VLS) * 4 + (p / VLS) % 4] = 16 * j;
         else ovf = true;
     };
     auto reach2 = [&](float sj) {
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
         // per site)
         const int nf = min(n, cap), pe = min((nf + VLG - 1) / VLG * VLG, cap);
         if (nf + sl < pe) put(nf + sl, na);
-        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = na;
+        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = 16 * na;
         if (sl == 0) cnt[k] = nf;
     }
     if (ovf) ovfl[0] = 1;
@@ -918,8 +919,9 @@ __device__ __forceinline__ void ewald_vl_body(EWVL_PARAMS, const int id) {
         auto ent = [&](int g) {
             return *(const uint4*)((const char*)nl + (l0 + (uint32_t)(g < ng ? g : pad) * ls));
         };
-        // coordinates of sorted site j of this bead, by a 32-bit byte offset from the bead's base
-        auto at = [&](uint32_t j) { return *(const P4*)((const char*)X + j * (uint32_t)sizeof(P4)); };
+        // coordinates of the site of list entry e (the byte offset of the site's 16-byte FP32 record,
+        // half that of its FP64 one), by a 32-bit byte offset from the bead's base
+        auto at = [&](uint32_t e) { return *(const P4*)((const char*)X + e * (uint32_t)(sizeof(P4) / 16)); };
         // a site has at most two own-molecule partners (three sites per molecule), listed first, so
         // only a lane's first entry can be one: may_same is false for every other call
         // FP32: coordinate difference (as float in the kernel's units), r^2, the table's abscissa
@@ -1306,7 +1308,7 @@ __global__ __launch_bounds__(256) void k_ewald_smear(const double4* __restrict__
         int n = nex[k];
         n += (sl - n % VLS + VLS) % VLS;
         for (; n < ntot; n += VLS) {
-            const int j = nl[(((size_t)(n / VLG) * na + k) * VLS + sl) * 4 + (n / VLS) % 4];
+            const int j = nl[(((size_t)(n / VLG) * na + k) * VLS + sl) * 4 + (n / VLS) % 4] >> 4;
             const double4 xj = X[j];
             double ex = xi.x - xj.x, ey = xi.y - xj.y, ez = xi.z - xj.z;
             if (UNIT) {
