@@ -481,7 +481,7 @@ NMMat* get_nm(rpmd_ctx* c, int n, int nb_full) {
 // ev terms: 0 Ewald real space, 1 Ewald k space (self term added on the host), 2 O-O, 3 intra,
 // 4 TEM; each is the bead average.  check_lists: read back the neighbour-list status (host sync).
 void free_vlist(VList& v) {
-    void* bufs[] = {v.perm, v.iperm, v.cnt, v.nex, v.nl, v.rb, v.srt, v.srf, v.cu, v.cs, v.bb, v.ccnt, v.clist,
+    void* bufs[] = {v.perm, v.iperm, v.cnt, v.nex, v.nl, v.cb, v.srt, v.srf, v.cu, v.cs, v.bb, v.ccnt, v.clist,
                     v.gate, v.dmax, v.nreb};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -526,7 +526,7 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
         e |= dalloc(&v.cnt, na);
         e |= dalloc(&v.nex, na);
         e |= dalloc(&v.nl, (size_t)na * (cap + VLG));  // + the padding group
-        e |= dalloc(&v.rb, 3 * ns);
+        e |= dalloc(&v.cb, (size_t)4 * na);
         e |= dalloc(&v.srt, ns + nbl);  // + the pad site of each bead
         e |= dalloc(&v.srf, ns + nbl);
         e |= dalloc(&v.cu, (size_t)4 * na);
@@ -542,10 +542,10 @@ int vlist_for(rpmd_ctx* c, int nb, const double box[3], double rc, double skin, 
             return RPMD_ERR_NOMEM;
         }
         v.cap = cap;
-        // reference positions NaN (all bits set): the first call always builds.  Filled on the
+        // build centroids NaN (all bits set): the first call always builds.  Filled on the
         // context stream, ahead of the kernels that use them (a null-stream memset is not ordered
         // against this non-blocking stream and could land after the first build)
-        launch_fill32(v.rb, 0xffffffffu, 2 * 3 * ns, c->st);
+        launch_fill32(v.cb, 0xffffffffu, (size_t)2 * 4 * na, c->st);
         launch_fill32(v.gate, 0u, 2, c->st);
         launch_fill32(v.dmax, 0u, 2, c->st);
         launch_fill32(v.nreb, 0u, 1, c->st);

@@ -133,7 +133,7 @@ void launch_zero_int(int* y, size_t n, hipStream_t s);
 // stream use next: a blocking memset on the null stream is not ordered against a non-blocking stream)
 void launch_fill32(void* y, uint32_t v, size_t nwords, hipStream_t s);
 // the rebuild predicate of a Verlet list, evaluated on the device by every build kernel: forced, or
-// the largest squared site displacement since the last build (bits in *dmax) above lim2
+// the largest squared site movement measured since the last build (bits in *dmax) above lim2
 struct VGate {
     const unsigned long long* dmax;
     double lim2;
@@ -153,7 +153,7 @@ struct VList {
     int* nex;        // [na] leading list entries that are the site's own molecule
     int* nl;         // [cap / VLG + 1][na][VLS][4] sorted neighbours j as byte offsets 16 j: entry n of k in
                      // group n / VLG, lane n % VLS, slot (n / VLS) % 4
-    double* rb;      // [nb][3][na] site coordinates at the last build
+    double* cb;      // [4][na] bead centroid and spread of each site at the last build (site order)
     double4* srt;    // [nb][na + 1] current coordinates and charge in sorted order; entry na of a bead is
                      // the charge-free pad site (zero coordinates)
     uint4* srf;      // [nb][na + 1] wrapped fractional coordinates in 32-bit fixed point (units of 2^-32,
@@ -165,7 +165,7 @@ struct VList {
     int* ccnt;       // [nclu] cluster-pair list lengths
     int* clist;      // [nclu][nclu] cluster-pair lists
     int* gate;       // [2]: [1] list overflow (sticky)
-    unsigned long long* dmax;  // largest squared displacement since the build (bits), this call
+    unsigned long long* dmax;  // largest squared excursion of a bead site beyond its build sphere (bits), this call
     int* nreb;       // [1] builds so far: the sites are sorted again on every VL_RESORT-th
     int cap;
 };

@@ -283,12 +283,14 @@ __device__ __forceinline__ int cell_coord(double x, double inv, int nc) {
 // ------------------------------------------------------------------------------------------------
 // real-space Ewald with the A&S erfc (src/ewald_cubic.f90:78-233; src/ewald_noncubic.f90:54-312)
 //
-// Per-site Verlet lists with a skin.  A list holds, for every site of every bead, the sites within
-// rc + skin at the time it was built; it stays exact -- it contains every pair now within rc --
-// while no site has moved more than skin/2 since (a pair distance changes by at most twice the
-// largest displacement).  Each call measures that displacement on the device (k_sites_vl, an
-// atomic max); the build kernels return at once unless it exceeds skin/2 (VGate), so a captured
-// hipGraph rebuilds exactly when needed.  The build (k_vl_sort) sorts the sites by cell in Morton
+// Per-site Verlet lists with a skin, one list for all beads.  At the build, site k has the bead
+// centroid c_k and the spread s_k (the largest distance of one of its beads from c_k); the list of k
+// holds the sites j with |c_k - c_j| < rc + skin + s_k + s_j.  It stays exact -- it contains every
+// pair of any bead now within rc -- while every bead of every site stays within s_k + skin/2 of its
+// build centroid (then |x_kb - x_jb| >= |c_k - c_j| - s_k - s_j - skin), however the ring polymers
+// move inside those spheres.  Each call measures the largest excursion beyond s_k on the device
+// (k_sites_vl, an atomic max); the build kernels return at once unless it exceeds skin/2 (VGate), so
+// a captured hipGraph rebuilds exactly when needed.  The build (k_vl_sort) sorts the sites by cell in Morton
 // order, groups runs of 64 sorted sites into clusters with bounding boxes and lists the cluster pairs
 // within reach; k_vl_build then fills each site's list (sorted indices, cluster order).  The pair
 // kernel gives VLS lanes to a sorted site, which walk its list: every pair is evaluated from both
@@ -326,9 +328,13 @@ __device__ __forceinline__ uint4 site_fixed(double x, double y, double z, double
 }
 
 // Per site of every bead: the Ewald site (the M-site combination of the oxygen and its hydrogens at
-// a%3 == 0, as k_sites), scaled by sc; its phase factors e^{i lat_d x_d}; its squared displacement
-// since the list build, max-reduced into dmax as bit patterns (non-negative doubles order as their
-// bits; NaN reference positions of a list never built count as huge); and, when gather is set, the
+// a%3 == 0, as k_sites), scaled by sc; its phase factors e^{i lat_d x_d}; how far it has left the
+// sphere of the list build around the site's bead centroid c_k with the site's bead spread s_k as
+// radius, e = max(0, |x_b - c_k| - s_k), squared and max-reduced into dmax as bit patterns
+// (non-negative doubles order as their bits; the NaN centroids of a list never built count as huge):
+// while every bead stays within s_k + skin / 2 of c_k, every pair within rc is on the list (it lists
+// the pairs with |c_k - c_j| < rc + skin + s_k + s_j), however the ring polymers move inside their
+// spheres.  When gather is set, the
 // site written at its sorted index iperm[] (the same for every bead) as (x, y, z, q) in FP64 and as
 // wrapped fractional FP32 coordinates (inverse box side f).  Per site also its bead centroid c and
 // spread s = max_b |x_b - c|, cu = [c_x, c_y, c_z, s][na], for the list build.  A block holds 256 / G
@@ -337,7 +343,7 @@ __device__ __forceinline__ uint4 site_fixed(double x, double y, double z, double
 __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
                                                   int nb, double alpha, double s0, double s1, double s2, double l0,
                                                   double l1, double l2, double2* __restrict__ ph,
-                                                  const double* __restrict__ q, const double* __restrict__ rb,
+                                                  const double* __restrict__ q, const double* __restrict__ cb,
                                                   const int* __restrict__ iperm, int gather, double f0, double f1,
                                                   double f2, double4* __restrict__ srt, uint4* __restrict__ srf,
                                                   double* __restrict__ cu, unsigned long long* __restrict__ dmax,
@@ -357,16 +363,14 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
         const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
         const double qq = q[a];
         const int ka = gather ? iperm[a] : 0;
+        const double rr[3] = {cb[a], cb[(size_t)na + a], cb[(size_t)2 * na + a]}, sr = cb[(size_t)3 * na + a];
         for (int b = g; b < nb; b += G) {
-            // the molecule's three atoms in every dimension and the reference position, loaded before
-            // any is used
-            double pm[3][3], rr[3];
+            // the molecule's three atoms in every dimension, loaded before any is used
+            double pm[3][3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
+            for (int d = 0; d < 3; d++)
 #pragma unroll
                 for (int u = 0; u < 3; u++) pm[d][u] = pos[((size_t)b * 3 + d) * na + m0 + u];
-                rr[d] = rb[(size_t)b * 3 * na + (size_t)d * na + a];
-            }
             double x[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -386,7 +390,8 @@ __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos
                 c[d] += x[d];
             }
             const double dx = x[0] - rr[0], dy = x[1] - rr[1], dz = x[2] - rr[2];
-            double dd = dx * dx + dy * dy + dz * dz;
+            const double ex = fmax(sqrt(dx * dx + dy * dy + dz * dz) - sr, 0.0);
+            double dd = ex * ex;
             if (!(dd <= 1e300)) dd = 1e300;
             m = fmax(m, dd);
             if (gather) {
@@ -612,8 +617,8 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // cut-off exactly).  One wave per BLS sites of cluster I, BPS = 64 / BLS lanes per site: lane s of a
 // site tests candidates s, s + BPS, ... of each listed cluster (staged in LDS), the site's lanes merge
 // their hits into its 64-bit mask, and each stores its own hits at their ascending positions.  The
-// wave also records every bead's reference positions of its sites and writes their current
-// coordinates in the new order, as k_sites_vl does between builds.
+// wave also records the centroid and spread of its sites (the reference of the displacement gate) and
+// writes their current coordinates in the new order, as k_sites_vl does between builds.
 constexpr int BLS = 4, BPS = CLU / BLS;
 static_assert(BPS == VLG, "the build pads a site's last list group with one entry per lane");
 __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
@@ -623,22 +628,25 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
                                                  float L0, float L1, float L2, double f0, double f1, double f2,
                                                  float rl, float mrg, int cap, int* __restrict__ cnt,
                                                  int* __restrict__ nex, int* __restrict__ nl,
-                                                 double* __restrict__ rb, double4* __restrict__ srt,
-                                                 uint4* __restrict__ srf, int* __restrict__ ovfl, VGate gate) {
+                                                 const double* __restrict__ cu, double* __restrict__ cb,
+                                                 double4* __restrict__ srt, uint4* __restrict__ srf,
+                                                 int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
     const int t = threadIdx.x, sl = t % BPS;
     const int k0 = blockIdx.x * BLS, k = k0 + t / BPS, I = k0 / CLU;
     const bool kv = k < na;
-    // reference positions and sorted current coordinates of the wave's sites, every bead
+    // the build's centroid and spread of the wave's sites (the displacement gate's reference)
+    if (t < 4 * BLS && k0 + t % BLS < na) {
+        const int a = perm[k0 + t % BLS];
+        cb[(size_t)(t / BLS) * na + a] = cu[(size_t)(t / BLS) * na + a];
+    }
+    // sorted current coordinates of the wave's sites, every bead
     for (int e = t; e < BLS * nb; e += 64) {
         const int kk = k0 + e % BLS, b = e / BLS;
         if (kk < na) {
             const int a = perm[kk];
             const size_t o = (size_t)b * 3 * na + a;
             const double x = site[o], y = site[o + na], z = site[o + 2 * na], qq = q[a];
-            rb[o] = x;
-            rb[o + na] = y;
-            rb[o + 2 * na] = z;
             srt[(size_t)b * (na + 1) + kk] = make_double4(x, y, z, qq);
             srf[(size_t)b * (na + 1) + kk] = site_fixed(x, y, z, qq, f0, f1, f2);
         }
@@ -1105,7 +1113,7 @@ void launch_sites_vl(const double* pos, double* site, int na, int nb, double alp
     // bead lanes per site: enough sites per block that the grid fills the CUs, at most 32 per site
     const int G = nb <= 8 ? 8 : (nb <= 16 ? 16 : 32), TA = 256 / G;
     k_sites_vl<<<(unsigned)((na + TA - 1) / TA), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
-                                                              lat[2], (double2*)ph, q, v.rb, v.iperm, gather,
+                                                              lat[2], (double2*)ph, q, v.cb, v.iperm, gather,
                                                               sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu,
                                                               v.dmax, G);
 }
@@ -1150,9 +1158,9 @@ void launch_phases(const double* pos, int na, int nb, double alpha, int unit, Bo
                                                          (double2*)ph);
 }
 
-// The build runs when a site of any bead has moved by more than skin/2 since the last one (or force
-// is set); it lists, for all beads at once, the pairs within rl = rc + skin.  count: [nbins + 1] ints,
-// all zero.
+// The build runs when a bead site has left its build sphere (centroid, spread) by more than skin/2
+// (or force is set); it lists, for all beads at once, the pairs whose centroids are within
+// rl + s_k + s_j, rl = rc + skin.  count: [nbins + 1] ints, all zero.
 void launch_vlist(const VList& v, const double* site, const double* q, int na, int nb, Box sb, CellGrid g, double rc,
                   double skin, int force, int* cell_of, int* count, int* start, hipStream_t s) {
     const VGate gate{v.dmax, 0.25 * skin * skin, force};
@@ -1168,7 +1176,7 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
     k_vl_build<<<(na + BLS - 1) / BLS, 64, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist,
                                                   (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0],
                                                   sb.inv[1], sb.inv[2], (float)rl, mrg, v.cap, v.cnt, v.nex, v.nl,
-                                                  v.rb, v.srt, v.srf, v.gate + 1, gate);
+                                                  v.cu, v.cb, v.srt, v.srf, v.gate + 1, gate);
 }
 
 // Dynamic LDS for a kernel so that at most bpc of its blocks fit on a CU (0: none, or bpc <= 0).
