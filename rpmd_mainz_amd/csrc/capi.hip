@@ -662,8 +662,8 @@ int forces_dev(rpmd_ctx* c, const double* pos, int nb, const double box[3], int 
         if (mixed) RETCHK(pair_table_for(c, sb, P.wrcut, P.walpha, s, &ptab));
         CellGrid g = make_grid(sb, na);
         if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
-        // list skin of 1 A (src/main.f90 uses none: the reference rebuilds its cells every call)
-        const double skin = (cubic ? 1.0 / box[0] : 1.0) * (1.0 / 0.5291772108);
+        // list skin EW_SKIN_A (src/main.f90 uses none: the reference rebuilds its cells every call)
+        const double skin = (cubic ? 1.0 / box[0] : 1.0) * (EW_SKIN_A / 0.5291772108);
         VList* vl = nullptr;
         int force = 0;
         RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force, check_lists ? 1 : 0));
@@ -2524,7 +2524,7 @@ static int fused_first_half(rpmd_ctx* c, FusedRep& fr, FBItem& it) {
     RETCHK(pair_table_for(c, sb, P.wrcut, P.walpha, s, &ptab));
     CellGrid g = make_grid(sb, na);
     if (morton_bins(g) > c->ncell_cap) return RPMD_ERR_ARG;
-    const double skin = (1.0 / box[0]) * (1.0 / 0.5291772108);
+    const double skin = (1.0 / box[0]) * (EW_SKIN_A / 0.5291772108);
     VList* vl = nullptr;
     int force = 0;
     RETCHK(vlist_for(c, nb, box, P.wrcut, skin, sb, &vl, &force, 0));

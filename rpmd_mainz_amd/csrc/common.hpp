@@ -143,6 +143,9 @@ struct VGate {
 // Verlet lists of the real-space Ewald sum, one list for all beads (built on the bead centroids with
 // each site's bead spread); one cache per bead set, per-bead arrays sized for nb_max beads
 #define VL_RESORT 8  // list builds per centroid sort (the others keep the last order)
+#ifndef EW_SKIN_A
+#define EW_SKIN_A 0.7  // skin of the real-space Ewald Verlet list (angstrom)
+#endif
 #define EW_BPC 6     // FP32 real-space pair blocks per CU when the k-space kernels run beside them
 #define VLS 4   // lanes per site in the Verlet-list walk
 #define VLG 16  // entries per list group: VLS lanes x 4 consecutive entries of each

@@ -614,34 +614,33 @@ __global__ __launch_bounds__(1024) void k_vl_sort(int na, CellGrid g, int nbins,
 // sixteen times (the pair kernel reads it past a lane's end).  The test is
 // FP32 on wrapped fractional centroids against the reach enlarged by a margin that covers the FP32
 // error, so the list is a superset (a few pairs beyond do no harm: the pair kernel tests the
-// cut-off exactly).  One wave per BLS sites of cluster I, BPS = 64 / BLS lanes per site: lane s of a
-// site tests candidates s, s + BPS, ... of each listed cluster (staged in LDS), the site's lanes merge
-// their hits into its 64-bit mask, and each stores its own hits at their ascending positions.  The
-// wave also records the centroid and spread of its sites (the reference of the displacement gate) and
-// writes their current coordinates in the new order, as k_sites_vl does between builds.
-constexpr int BLS = 4, BPS = CLU / BLS;
-static_assert(BPS == VLG, "the build pads a site's last list group with one entry per lane");
-__global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
-                                                 const int* __restrict__ iperm, const double* __restrict__ site,
-                                                 const double* __restrict__ q, int na, int nb, int nclu,
-                                                 const int* __restrict__ ccnt, const int* __restrict__ clist,
-                                                 float L0, float L1, float L2, double f0, double f1, double f2,
-                                                 float rl, float mrg, int cap, int* __restrict__ cnt,
-                                                 int* __restrict__ nex, int* __restrict__ nl,
-                                                 const double* __restrict__ cu, double* __restrict__ cb,
-                                                 double4* __restrict__ srt, uint4* __restrict__ srf,
-                                                 int* __restrict__ ovfl, VGate gate) {
+// cut-off exactly).  One wave per sorted site k (four per block), a lane per candidate: for each listed
+// cluster J (ascending) lane l tests site 64 J + l, the candidates of the next cluster loaded while the
+// current ones are tested, and the hits are stored at their ascending positions from the wave's mask.
+// The block also records the centroid and spread of its sites (the reference of the displacement
+// gate) and writes their current coordinates in the new order, as k_sites_vl does between builds.
+constexpr int BLS = 4;
+static_assert(CLU % BLS == 0, "a block's sites lie in one cluster");
+__global__ __launch_bounds__(64 * BLS) void k_vl_build(const float4* __restrict__ cs, const int* __restrict__ perm,
+                                                       const int* __restrict__ iperm, const double* __restrict__ site,
+                                                       const double* __restrict__ q, int na, int nb, int nclu,
+                                                       const int* __restrict__ ccnt, const int* __restrict__ clist,
+                                                       float L0, float L1, float L2, double f0, double f1, double f2,
+                                                       float rl, float mrg, int cap, int* __restrict__ cnt,
+                                                       int* __restrict__ nex, int* __restrict__ nl,
+                                                       const double* __restrict__ cu, double* __restrict__ cb,
+                                                       double4* __restrict__ srt, uint4* __restrict__ srf,
+                                                       int* __restrict__ ovfl, VGate gate) {
     if (!gate.on()) return;
-    const int t = threadIdx.x, sl = t % BPS;
-    const int k0 = blockIdx.x * BLS, k = k0 + t / BPS, I = k0 / CLU;
-    const bool kv = k < na;
-    // the build's centroid and spread of the wave's sites (the displacement gate's reference)
+    const int t = threadIdx.x, l = t & 63;
+    const int k0 = blockIdx.x * BLS, k = k0 + (t >> 6);
+    // the build's centroid and spread of the block's sites (the displacement gate's reference)
     if (t < 4 * BLS && k0 + t % BLS < na) {
         const int a = perm[k0 + t % BLS];
         cb[(size_t)(t / BLS) * na + a] = cu[(size_t)(t / BLS) * na + a];
     }
-    // sorted current coordinates of the wave's sites, every bead
-    for (int e = t; e < BLS * nb; e += 64) {
+    // sorted current coordinates of the block's sites, every bead
+    for (int e = t; e < BLS * nb; e += 64 * BLS) {
         const int kk = k0 + e % BLS, b = e / BLS;
         if (kk < na) {
             const int a = perm[kk];
@@ -653,14 +652,14 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
     }
     // the charge-free pad site of every bead
     if (blockIdx.x == 0)
-        for (int b = t; b < nb; b += 64) {
+        for (int b = t; b < nb; b += 64 * BLS) {
             srt[(size_t)b * (na + 1) + na] = make_double4(0.0, 0.0, 0.0, 0.0);
             srf[(size_t)b * (na + 1) + na] = make_uint4(1u, 1u, 1u, 0u);
         }
-    __shared__ float4 cj[CLU];  // candidate centroid (wrapped fractional) and spread
-    __shared__ int mj[CLU];     // candidate molecule
-    float4 ci = make_float4(0.f, 0.f, 0.f, 0.f);
-    int a = -3, n = 0;
+    if (k >= na) return;  // a whole wave (k is the wave's)
+    const float4 ci = cs[k];
+    const int a = perm[k], mi = a / 3, I = k / CLU;
+    int n = 0;
     bool ovf = false;
     auto put = [&](int p, int j) {
         if (p < cap) nl[((size_t)(p / VLG) * na + k) * VLG + (p % VLS) * 4 + (p / VLS) % 4] = 16 * j;
@@ -677,78 +676,46 @@ __global__ __launch_bounds__(64) void k_vl_build(const float4* __restrict__ cs, 
         dz = (dz - rintf(dz)) * L2;
         return dx * dx + dy * dy + dz * dz;
     };
-    if (kv) {
-        ci = cs[k];
-        a = perm[k];
-        if (sl == 0) {
-            const int m0 = 3 * (a / 3);
-            for (int p = m0; p < m0 + 3; p++) {
-                if (p == a || p >= na) continue;
-                const int jk = iperm[p];
-                const float4 c = cs[jk];
-                if (d2(c) < reach2(c.w)) put(n++, jk);
-            }
-            nex[k] = min(n, cap);
+    if (l == 0) {
+        const int m0 = 3 * mi;
+        for (int p = m0; p < m0 + 3; p++) {
+            if (p == a || p >= na) continue;
+            const int jk = iperm[p];
+            const float4 c = cs[jk];
+            if (d2(c) < reach2(c.w)) put(n++, jk);
         }
+        nex[k] = min(n, cap);
     }
-    n = __shfl(n, t - sl, 64);  // the site's count so far, from its lane 0
+    n = __shfl(n, 0, 64);  // the own-molecule entries, from lane 0
     const int nJ = ccnt[I];
     const int* lst = clist + (size_t)I * nclu;
-    const int mi = a / 3;
-    auto stage = [&](int J, float4& c, int& m) {
-        const int j = J * CLU + t;
+    // lane l's candidate of listed cluster e: its centroid and spread, and its molecule (-1: none)
+    auto cand = [&](int e, float4& c, int& m) {
+        const int j = lst[e] * CLU + l;
         c = j < na ? cs[j] : make_float4(0.f, 0.f, 0.f, 0.f);
         m = j < na ? perm[j] / 3 : -1;
     };
-    // bits BPS u of a 64-bit word
-    uint64_t every = 0;
-#pragma unroll
-    for (int u = 0; u < CLU / BPS; u++) every |= 1ull << (BPS * u);
-    // the listed clusters 64 at a time, one per lane, so staging the next cluster waits on one load
-    for (int e0 = 0; e0 < nJ; e0 += 64) {
-        const int ne = min(64, nJ - e0);
-        const int jl = t < ne ? lst[e0 + t] : 0;
-        float4 nc;
-        int nm_;
-        stage(__shfl(jl, 0, 64), nc, nm_);
-        for (int e = 0; e < ne; e++) {
-            const int J = __shfl(jl, e, 64);
-            __syncthreads();
-            cj[t] = nc;
-            mj[t] = nm_;
-            __syncthreads();
-            if (e + 1 < ne) stage(__shfl(jl, e + 1, 64), nc, nm_);
-            uint64_t m = 0;
-            if (kv) {
-#pragma unroll
-                for (int u = 0; u < CLU / BPS; u++) {
-                    const int jj = BPS * u + sl;
-                    const float4 c = cj[jj];
-                    const bool take = mj[jj] >= 0 && mj[jj] != mi && d2(c) < reach2(c.w);
-                    m |= (uint64_t)take << jj;
-                }
-            }
-#pragma unroll
-            for (int off = 1; off < BPS; off <<= 1) m |= __shfl_xor(m, off, 64);
-            if (kv) {
-                uint64_t mine = m & (every << sl);
-                while (mine) {
-                    const int jj = __builtin_ctzll(mine);
-                    mine &= mine - 1;
-                    put(n + __popcll(m & ((1ull << jj) - 1ull)), J * CLU + jj);
-                }
-            }
-            n += __popcll(m);
-        }
+    const uint64_t below = (1ull << l) - 1ull;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    int mj = -1;
+    if (nJ > 0) cand(0, c, mj);
+    for (int e = 0; e < nJ; e++) {
+        const int J = lst[e];
+        float4 cn = c;
+        int mn = -1;
+        if (e + 1 < nJ) cand(e + 1, cn, mn);
+        const bool take = mj >= 0 && mj != mi && d2(c) < reach2(c.w);
+        const uint64_t m = __ballot(take);
+        if (take) put(n + __popcll(m & below), J * CLU + l);
+        n += __popcll(m);
+        c = cn;
+        mj = mn;
     }
-    if (kv) {
-        // the rest of the site's last group and the padding group hold the pad site na (BPS = VLG lanes
-        // per site)
-        const int nf = min(n, cap), pe = min((nf + VLG - 1) / VLG * VLG, cap);
-        if (nf + sl < pe) put(nf + sl, na);
-        nl[((size_t)(cap / VLG) * na + k) * VLG + sl] = 16 * na;
-        if (sl == 0) cnt[k] = nf;
-    }
+    // the rest of the site's last group and the padding group hold the pad site na
+    const int nf = min(n, cap), pe = min((nf + VLG - 1) / VLG * VLG, cap);
+    for (int p = nf + l; p < pe; p += 64) put(p, na);
+    if (l < VLG) nl[((size_t)(cap / VLG) * na + k) * VLG + l] = 16 * na;
+    if (l == 0) cnt[k] = nf;
     if (ovf) ovfl[0] = 1;
 }
 
@@ -1173,7 +1140,7 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
                                  cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist, gate,
                                  v.nreb);
     const float mrg = (float)(1.0 + pair_margin(sb, rl));
-    k_vl_build<<<(na + BLS - 1) / BLS, 64, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist,
+    k_vl_build<<<(na + BLS - 1) / BLS, 64 * BLS, 0, s>>>(v.cs, v.perm, v.iperm, site, q, na, nb, nclu, v.ccnt, v.clist,
                                                   (float)sb.L[0], (float)sb.L[1], (float)sb.L[2], sb.inv[0],
                                                   sb.inv[1], sb.inv[2], (float)rl, mrg, v.cap, v.cnt, v.nex, v.nl,
                                                   v.cu, v.cb, v.srt, v.srf, v.gate + 1, gate);
