@@ -310,7 +310,7 @@ int rpmd_ipi_run(rpmd_ctx* ctx, const char* address, const double* z, int na, in
 /* Kernel timing (no reference counterpart; the reference has no timers): HIP events recorded on
  * the context stream around the Ewald kernels of eager launches between begin and end.
  * ms/counts index: 0 real-space pair kernel, 1 structure-factor GEMM, 2 k-space force kernel,
- * 3 structure-factor weights/energy kernel, 4 real-space list upkeep (displacement check and, when
+ * 3 structure-factor weights/energy kernel, 4 real-space list upkeep (the rebuild test and, when
  * due, the rebuild). */
 #define RPMD_PROF_N 5
 int rpmd_prof_begin(rpmd_ctx* ctx);
