@@ -2643,11 +2643,14 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
                                             double ba, double bb, double bc, int buck, double vs, double ds,
                                             ChainArgs ch, unsigned long long* __restrict__ dmax,
                                             double* __restrict__ grad, double* __restrict__ parts,
-                                            double* __restrict__ lj) {
-    const int b = blockIdx.y;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                            double* __restrict__ lj, int nb) {
+    // blocks are dealt to the eight XCDs in turn (block id mod 8): XCD x takes beads x, x + 8, ... one
+    // after the other, so the blocks resident on an XCD read one bead's positions from its L2
+    const int n = na / 3, nbb = (n * OOL + 255) / 256, m8 = blockIdx.x >> 3;
+    const int b = (blockIdx.x & 7) + 8 * (m8 / nbb), blk = m8 % nbb;
+    if (b >= nb) return;  // the whole block
+    const int t = blk * blockDim.x + threadIdx.x;
     const int ii = t / OOL, sl = t % OOL;
-    const int n = na / 3;
     if (dmax && t == 0 && b == 0) ctrl_store(dmax, 0ull);
     double acc[7];
 #pragma unroll
@@ -2753,7 +2756,7 @@ __global__ __launch_bounds__(256) void k_oo(const double* __restrict__ pos, int 
     // force is symmetric
     double w[10] = {0.5 * acc[0], 0.5 * acc[1], 0.5 * acc[2], 0.5 * acc[3], 0.5 * acc[2],
                     0.5 * acc[4], 0.5 * acc[5], 0.5 * acc[3], 0.5 * acc[5], 0.5 * acc[6]};
-    block_reduce_store<10>(w, parts + ((size_t)b * gridDim.x + blockIdx.x) * 10);
+    block_reduce_store<10>(w, parts + ((size_t)b * nbb + blk) * 10);
 }
 
 template <typename T>
@@ -2777,9 +2780,9 @@ void launch_oo(const double* pos, int na, int nb, Box box, const int* nnb, const
     }
     const int n = na / 3;
     const double vs = buck ? 1.0 : 4.0 * p.oo_eps, ds = buck ? 1.0 : 48.0 * p.oo_eps;
-    k_oo<T><<<dim3(oo_blocks(na), nb), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
-                                                    p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, ch, dmax, grad,
-                                                    parts, lj);
+    k_oo<T><<<oo_blocks(na) * ((nb + 7) / 8 * 8), 256, 0, s>>>(pos, na, box, nnb, nbl, use_list, rcut * rcut,
+                                                                p.oo_sig * p.oo_sig, ba, bb, bc, buck, vs, ds, ch,
+                                                                dmax, grad, parts, lj, nb);
 }
 template void launch_oo<double>(const double*, int, int, Box, const int*, const int*, int, double, const rpmd_params&,
                                 double*, double*, hipStream_t, const double*, int, double, double, int,
