@@ -189,3 +189,34 @@ def test_batched_force_launches_equal_independent_runs(capi, case):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+def test_aged_verlet_lists_stay_exact(capi, oracle):
+    """Device-resident FP64 PILE-L steps over several lifetimes of the real-space list (rebuilt only
+    when a bead site leaves the sphere of its build, bead centroid and spread, by half the skin), from
+    collapsed ring polymers that spread out as the run goes: after each stretch the state's gradient
+    equals the oracle's full_forces at the state's positions, so no pair within the cut-off was
+    missing from an aged list."""
+    from oracle.oracle import RpmdParams
+    r, box, z, mass, prm = cf.water_system("qspcfw", (6, 6, 6), 8, ecut=1e-6, intra_model=cf.INTRA_SPCFW,
+                                           therm="PRO", ttaufs=100.0)
+    na, nb = r.shape[1], r.shape[0]
+    beta, dt = 1.0 / (cf.KBOLTZ * 298.0), 0.25 * cf.TOFS
+    p = np.random.default_rng(31).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+    P = RpmdParams.from_dict(prm)
+    c = capi.RPMDContext(prm, na, nb, "fp64")
+    c.state_upload(p, r, mass, z, na, nb, box, beta, dt, seed=31)
+    c.use_graph(True)
+    moved = 0.0
+    for k in range(5):
+        c.run_steps(24)
+        st = c.state_download()
+        ff = oracle.full_forces(P, st["r"], box, z)
+        g, g0 = st["dvdr"] + st["dvdr2"], ff["dvdr"] + ff["dvdr2"]
+        err = np.abs(g - g0).max() / np.abs(g0).max()
+        moved = max(moved, float(np.abs(st["r"] - r).max()))
+        print(f"stretch {k}: gradient max rel {err:.2e}, largest move {moved:.3f} bohr", flush=True)
+        assert err < 1e-10
+    # some bead moved by more than half the skin (0.7 A = 1.3 bohr): the lists aged and were rebuilt
+    assert moved > 1.0
+    c.close()
