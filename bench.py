@@ -170,15 +170,13 @@ def replicas(nrep: int, steps: int, seed: int = 2024):
     """Config 3: nrep q-SPC/Fw replicas, sizes and conditions drawn from seed, stepped together by
     one replica-batch graph in mixed precision.  Returns bead-force-evals/s over all replicas."""
     rng = np.random.default_rng(seed)
-    lattice = {128: ((4, 4, 8), (2, 2, 1)), 256: ((4, 8, 8), (2, 1, 1)), 512: ((8, 8, 8), (1, 1, 1)),
-               1024: ((8, 8, 16), (2, 2, 1))}
     ctxs, sum_nb, flops = [], 0, 0.0
     for i in range(nrep):
         nm = int(rng.choice([128, 256, 512, 1024]))
         nb = int(rng.choice([8, 16, 32, 64]))
         rho, temp = float(rng.uniform(0.95, 1.05)), float(rng.uniform(250.0, 350.0))
-        ncell, aspect = lattice[nm]
-        r, box, z, mass, prm = cf.water_system("qspcfw", ncell, nb, rho=rho, aspect=aspect, seed=100 + i, ecut=1e-6,
+        ncell, basis = cf.cubic_cells(nm)
+        r, box, z, mass, prm = cf.water_system("qspcfw", ncell, nb, rho=rho, basis=basis, seed=100 + i, ecut=1e-6,
                                                intra_model=cf.INTRA_SPCFW, therm="PRO", ttaufs=100.0)
         na = r.shape[1]
         beta = 1.0 / (cf.KBOLTZ * temp)
@@ -209,7 +207,7 @@ def replicas(nrep: int, steps: int, seed: int = 2024):
     return dict(replicas=nrep, beads_total=sum_nb, steps=steps, ms_per_step=ms, roofline=roof,
                 bead_force_evals_s=steps * sum_nb / dt, dtype="f32-pair/f64-acc", tuning=tuning,
                 workload="config 3: q-SPC/Fw replicas of 128-1024 molecules, 8-64 beads, rho 0.95-1.05, "
-                         "250-350 K, own cubic boxes (kmax 9-13), PILE-L, one batch hipGraph per 2 steps (captured in the warm-up); "
+                         "250-350 K, own cubic boxes (kmax 9-13) filled by bcc/fcc/sc lattices, PILE-L, one batch hipGraph per 2 steps (captured in the warm-up); "
                          "batched force launches or the replicas' own steps, whichever the autotuner timed faster")
 
 
@@ -441,8 +439,8 @@ def main():
                                         intra_model=cf.INTRA_SPCFW, therm="PRO", ttaufs=100.0)
         for th in ("PRO", "GLE"):
             others[f"config5_{th}"] = config_line(
-                f"1024 rigid TIP4P/2005 x 32, {th}, RATTLE 1e-10, dt 0.5 fs, mixed", "tip4p2005", (8, 8, 16), 32, 0.5,
-                "mixed", 50, 3, aspect=(2, 2, 1), ecut=1e-6, rigid=True, therm=th, ttaufs=100.0, rat_tol=1e-10,
+                f"1024 rigid TIP4P/2005 x 32, {th}, RATTLE 1e-10, dt 0.5 fs, mixed", "tip4p2005", (8, 8, 8), 32, 0.5,
+                "mixed", 50, 3, basis="bcc", ecut=1e-6, rigid=True, therm=th, ttaufs=100.0, rat_tol=1e-10,
                 amp=0.02, gle=(th == "GLE"))
     out = {
         "metric": "bead-force-evals/s (1728 q-SPC/Fw H2O x 32 beads, RPMD step)",

@@ -3,8 +3,9 @@
 Units are the reference's atomic units (src/globals.inc:10-21).  Model parameter sets restate
 the reference's namelist files (params/param_qspcfw, params/param_scott,
 params/param_tip05_rigid); the lattice follows src/setup_positions.f90:94-150 (O on a simple
-cubic lattice at i*L/n) with one addition the reference lacks: each molecule is rotated about
-its oxygen by a seeded uniform random rotation.
+cubic lattice at i*L/n) with two additions the reference lacks: each molecule is rotated about
+its oxygen by a seeded uniform random rotation, and body- and face-centred cubic lattices place
+2 m^3 and 4 m^3 molecules in a cubic box.
 """
 from __future__ import annotations
 
@@ -73,13 +74,37 @@ def _random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
     ], 1)
 
 
-def water_lattice(ncell, box, reoh: float, theta: float, seed: int, random_orient: bool = True) -> np.ndarray:
-    """Positions (na, 3) of nm = prod(ncell) molecules, (O,H,H) order, bohr."""
+# oxygen sites per lattice cell, in cell units: simple, body-centred and face-centred cubic
+BASES = {
+    "sc": np.zeros((1, 3)),
+    "bcc": np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]),
+    "fcc": np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]]),
+}
+
+
+def cubic_cells(nm: int):
+    """(ncell, basis) of a cubic lattice of exactly nm molecules in a cubic box: m^3 simple cubic,
+    2 m^3 body-centred or 4 m^3 face-centred, so that neighbouring oxygens are as far apart in
+    every direction (128 = 2 * 4^3 and 1024 = 2 * 8^3 molecules on bcc, 256 = 4 * 4^3 on fcc: nearest
+    oxygens at 3.4 and 3.5 angstrom at 1 g/cm^3, where a 4x4x8 or 8x8x16 simple lattice in a cube
+    would put them 2.0 angstrom apart along its long axis)."""
+    for basis in ("sc", "bcc", "fcc"):
+        m = round((nm / len(BASES[basis])) ** (1.0 / 3.0))
+        if len(BASES[basis]) * m ** 3 == nm:
+            return (m, m, m), basis
+    raise ValueError(f"{nm} molecules fill no cubic lattice (m^3, 2 m^3 or 4 m^3)")
+
+
+def water_lattice(ncell, box, reoh: float, theta: float, seed: int, random_orient: bool = True,
+                  basis: str = "sc") -> np.ndarray:
+    """Positions (na, 3) of nm = len(BASES[basis]) * prod(ncell) molecules, (O,H,H) order, bohr."""
     ncell = tuple(int(c) for c in ncell)
     idx = np.stack(np.meshgrid(*[np.arange(c) for c in ncell], indexing="ij"), -1).reshape(-1, 3)
-    # src/setup_positions.f90 loops x fastest; keep that molecule order
+    # src/setup_positions.f90 loops x fastest; keep that molecule order (the basis sites of a cell
+    # consecutive)
     idx = idx[np.lexsort((idx[:, 0], idx[:, 1], idx[:, 2]))]
-    o = idx * (np.asarray(box) / np.asarray(ncell))
+    cell = np.asarray(box) / np.asarray(ncell)
+    o = ((idx[:, None, :] + BASES[basis][None, :, :]) * cell).reshape(-1, 3)
     h1 = np.array([reoh, 0.0, 0.0])
     h2 = np.array([reoh * math.cos(theta), reoh * math.sin(theta), 0.0])
     nm = o.shape[0]
@@ -138,15 +163,15 @@ def thermal_beads(r0: np.ndarray, nb: int, amp: float, seed: int, rigid: bool = 
 
 
 def water_system(model_name: str, ncell, nb: int, rho: float = 0.997, seed: int = 1, amp: float = 0.05,
-                 aspect=(1.0, 1.0, 1.0), **kw):
+                 aspect=(1.0, 1.0, 1.0), basis: str = "sc", **kw):
     """A seeded bead configuration plus its parameter dict: (r(nb,na,3), box, z, mass, prm)."""
     model = MODELS[model_name]
-    nm = int(np.prod(ncell))
+    nm = len(BASES[basis]) * int(np.prod(ncell))
     box = box_size(nm, rho, model["wmass"], ncell, aspect)
     if np.allclose(box, box[0], rtol=1e-12, atol=0.0):
         box = np.full(3, box[0])          # e.g. an 8x8x16 lattice in a cubic box (aspect 2:2:1)
     theta = model["thetad"] * math.pi / 180.0
-    r0 = water_lattice(ncell, box, model["reoh"], theta, seed)
+    r0 = water_lattice(ncell, box, model["reoh"], theta, seed, basis=basis)
     r = (thermal_beads(r0, nb, amp, seed + 1000, bool(kw.get("rigid", False))) if amp > 0
          else np.repeat(r0[None], nb, 0))
     na = r0.shape[0]

@@ -220,3 +220,39 @@ def test_aged_verlet_lists_stay_exact(capi, oracle):
     # some bead moved by more than half the skin (0.7 A = 1.3 bohr): the lists aged and were rebuilt
     assert moved > 1.0
     c.close()
+
+
+def test_replica_batch_lattices_stay_physical(capi):
+    """The replica batch's system sizes (128 and 1024 molecules on body-centred, 256 on face-centred,
+    512 on simple cubic lattices in cubic boxes) stepped 160 times at 250-350 K from Maxwell-Boltzmann
+    bead momenta: no list overflows (every pair within reach fits each site's list), the kinetic
+    temperatures stay near their targets and no ring polymer tears apart (largest bead distance from
+    its centroid under 2 bohr; a lattice that starts oxygens 2 angstrom apart heats such runs to
+    1.5-2 times the target and stretches hydrogens to 5 bohr within 40 steps)."""
+    import bench
+    specs = [(128, 64, 1.02, 252.0), (256, 8, 0.97, 347.0), (512, 32, 1.0, 298.0), (1024, 16, 1.04, 274.0)]
+    ctxs = []
+    for i, (nm, nb, rho, temp) in enumerate(specs):
+        ncell, basis = cf.cubic_cells(nm)
+        r, box, z, mass, prm = cf.water_system("qspcfw", ncell, nb, rho=rho, basis=basis, seed=100 + i, ecut=1e-6,
+                                               intra_model=cf.INTRA_SPCFW, therm="PRO", ttaufs=100.0)
+        na = r.shape[1]
+        beta = 1.0 / (cf.KBOLTZ * temp)
+        p = np.random.default_rng(100 + i).normal(size=r.shape) * np.sqrt(mass / (beta / nb))[None, :, None]
+        c = capi.RPMDContext(prm, na, nb, "mixed")
+        c.state_upload(p, r, mass, z, na, nb, box, beta, bench.DT_FS * cf.TOFS, seed=100 + i)
+        ctxs.append(c)
+    batch = capi.ReplicaBatch(ctxs)
+    for k in range(4):
+        batch.run_steps(40)
+        print(f"replica lattices: {40 * (k + 1)} steps", flush=True)
+    for c, (nm, nb, rho, temp) in zip(ctxs, specs):
+        t = c.estimators()["temperature_K"]
+        r = c.state_download()["r"]
+        spread = np.sqrt(((r - r.mean(0)[None]) ** 2).sum(-1)).max()
+        print(f"{nm} x {nb} at {temp} K: T {t:.1f} K, largest spread {spread:.3f} bohr", flush=True)
+        assert 0.8 * temp < t < 1.25 * temp
+        assert spread < 2.0
+    batch.close()
+    for c in ctxs:
+        c.close()

@@ -523,6 +523,30 @@ def test_box_size_matches_reference(oracle, refpath):
         assert np.abs(b_cfg - b_ref).max() <= 4e-16 * b_ref.max(), ncell
 
 
+@pytest.mark.parametrize("nm,ncell,basis", [(128, 4, "bcc"), (256, 4, "fcc"), (512, 8, "sc"), (1024, 8, "bcc"),
+                                             (1728, 12, "sc")])
+def test_cubic_lattices_keep_oxygens_apart(nm, ncell, basis):
+    """The replica-batch sizes fill a cubic box on a simple, body- or face-centred cubic lattice:
+    nm molecules, every oxygen exactly once, nearest oxygens at a / 1, a * sqrt(3) / 2 or a / sqrt(2)
+    of the cell side a (a 4x4x8 simple lattice in a cube would put them at a / 2 along z)."""
+    assert cf.cubic_cells(nm) == ((ncell,) * 3, basis)
+    with pytest.raises(ValueError):
+        cf.cubic_cells(nm + 1)
+    box = cf.box_size(nm, 1.0, 32831.2525, (ncell,) * 3)
+    assert np.ptp(box) == 0.0
+    r = cf.water_lattice((ncell,) * 3, box, 1.8, 1.9, seed=5, basis=basis)
+    o = r[0::3]
+    assert o.shape == (nm, 3)
+    d = o[:, None, :] - o[None, :, :]
+    d -= box * np.rint(d / box)
+    dd = np.sqrt((d ** 2).sum(-1)) + np.eye(nm) * 1e9
+    a = box[0] / ncell
+    near = {"sc": a, "bcc": a * math.sqrt(3.0) / 2.0, "fcc": a / math.sqrt(2.0)}[basis]
+    assert abs(dd.min() - near) < 1e-9 * a
+    # oxygen-hydrogen bonds are untouched by the basis
+    assert np.allclose(np.linalg.norm(r[1::3] - o, axis=1), 1.8)
+
+
 def _rp_contract(oracle, P, rnm, box, z, nbdf, iopt):
     """the oracle's rp_contract_nm: normal-mode r in, (v, normal-mode dvdr, vir) out"""
     import ctypes as C
