@@ -409,21 +409,20 @@ def test_config4_shape_device_resident(capi):
 
 def _replica_specs(n, seed, sizes=(128, 256, 512, 1024), beads=(8, 16, 32, 64)):
     """Config-3 style draws: molecule counts, bead counts, densities 0.95-1.05 g/cm^3 and
-    temperatures 250-350 K from a seed; every replica in its own cubic box."""
+    temperatures 250-350 K from a seed; every replica in its own cubic box, filled by a simple,
+    body- or face-centred cubic lattice."""
     rng = np.random.default_rng(seed)
-    lattice = {8: ((2, 2, 2), (1, 1, 1)), 27: ((3, 3, 3), (1, 1, 1)), 64: ((4, 4, 4), (1, 1, 1)),
-               128: ((4, 4, 8), (2, 2, 1)), 256: ((4, 8, 8), (2, 1, 1)), 512: ((8, 8, 8), (1, 1, 1)),
-               1024: ((8, 8, 16), (2, 2, 1))}
     out = []
     for i in range(n):
         nm = int(rng.choice(sizes))
-        out.append(dict(ncell=lattice[nm][0], aspect=lattice[nm][1], nb=int(rng.choice(beads)),
+        ncell, basis = cf.cubic_cells(nm)
+        out.append(dict(nm=nm, ncell=ncell, basis=basis, nb=int(rng.choice(beads)),
                         rho=float(rng.uniform(0.95, 1.05)), temp=float(rng.uniform(250.0, 350.0)), seed=100 + i))
     return out
 
 
 def _replica_ctx(capi, spec, prec, therm="PRO"):
-    r, box, z, mass, prm = cf.water_system("qspcfw", spec["ncell"], spec["nb"], rho=spec["rho"], aspect=spec["aspect"],
+    r, box, z, mass, prm = cf.water_system("qspcfw", spec["ncell"], spec["nb"], rho=spec["rho"], basis=spec["basis"],
                                            seed=spec["seed"], ecut=1e-6, intra_model=cf.INTRA_SPCFW, therm=therm,
                                            ttaufs=100.0)
     assert prm["iamcub"] == 1
@@ -461,15 +460,23 @@ def test_replica_batch_equals_independent_runs(capi):
 
 def test_replica_batch_config3_shape(capi):
     """Config 3: 48 replicas drawn from a seed (128-1024 molecules, 8-64 beads, own cubic boxes,
-    kmax 9-13, 250-350 K), mixed precision, stepped together; every state stays finite."""
+    kmax 9-13, 250-350 K), mixed precision, stepped together for 48 steps: no list overflows, every
+    state stays finite, near its target temperature (0.8-1.25 of it) and with its ring polymers
+    whole (no bead more than 2 bohr from its centroid; thermal spreads reach about 1.2 bohr)."""
     specs = _replica_specs(48, 2024)
     ctxs = [_replica_ctx(capi, sp, "mixed") for sp in specs]
     batch = capi.ReplicaBatch(ctxs)
-    batch.run_steps(4)
-    for c in ctxs:
+    for k in range(4):
+        batch.run_steps(12)
+        print(f"config 3: {12 * (k + 1)} batch steps", flush=True)
+    for c, sp in zip(ctxs, specs):
         st = c.state_download()
         assert np.isfinite(st["v"]) and np.isfinite(st["p"]).all()
         assert 9 <= c.prm["kmax"] <= 13
+        t = c.estimators()["temperature_K"]
+        spread = np.sqrt(((st["r"] - st["r"].mean(0)[None]) ** 2).sum(-1)).max()
+        assert 0.8 * sp["temp"] < t < 1.25 * sp["temp"], (sp, t)
+        assert spread < 2.0, (sp, spread)
     batch.close()
     for c in ctxs:
         c.close()

@@ -168,7 +168,7 @@ def test_mixed_replicas_sampled_against_oracle(capi, oracle):
     ctxs = [_replica_ctx(capi, sp, "mixed") for sp in specs]
     batch = capi.ReplicaBatch(ctxs)
     batch.run_steps(4)
-    cost = [int(np.prod(sp["ncell"])) * sp["nb"] for sp in specs]
+    cost = [sp["nm"] * sp["nb"] for sp in specs]
     order = np.argsort(cost)
     picks = [int(order[0]), int(order[len(order) // 4]), int(order[len(order) // 2])]
     for i, c in enumerate(ctxs):
@@ -177,11 +177,11 @@ def test_mixed_replicas_sampled_against_oracle(capi, oracle):
         if i in picks:
             sp = specs[i]
             r, box, z, mass, prm = cf.water_system("qspcfw", sp["ncell"], sp["nb"], rho=sp["rho"],
-                                                   aspect=sp["aspect"], seed=sp["seed"], ecut=1e-6,
+                                                   basis=sp["basis"], seed=sp["seed"], ecut=1e-6,
                                                    intra_model=cf.INTRA_SPCFW, therm="PRO", ttaufs=100.0)
             v2, d2, w2 = oracle.forces(RpmdParams.from_dict(prm), st["r"], box, z, 0)
             rms = _rms(st["dvdr"] + st["dvdr2"], d2)
-            print(f"replica {i} ({np.prod(sp['ncell'])} x {sp['nb']}) force RMS_rel {rms:.3e}", flush=True)
+            print(f"replica {i} ({sp['nm']} x {sp['nb']}) force RMS_rel {rms:.3e}", flush=True)
             assert rms <= RMS_BAR
             assert abs(st["v"] - v2) <= EBAR * _escale(prm, box, z, v2)
     batch.close()
