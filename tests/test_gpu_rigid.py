@@ -39,7 +39,7 @@ def _residuals(r, v, prm):
 @pytest.mark.parametrize("therm", ["PRO", "GLE"])
 def test_config5_rigid_tip4p2005_thermostatted(capi, oracle, therm):
     tol = 1e-10
-    r, box, z, mass, prm = cf.water_system("tip4p2005", (8, 8, 16), 32, aspect=(2, 2, 1), ecut=1e-6, rigid=True,
+    r, box, z, mass, prm = cf.water_system("tip4p2005", (8, 8, 8), 32, basis="bcc", ecut=1e-6, rigid=True,
                                            therm=therm, ttaufs=100.0, amp=0.1, rat_tol=tol)
     assert prm["iamcub"] == 1 and abs(box[0] * cf.TOA - 31.318) < 0.01 and prm["alpha"] < 1.0
     nb, na = r.shape[:2]
