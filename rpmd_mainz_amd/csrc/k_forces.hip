@@ -334,115 +334,111 @@ __device__ __forceinline__ uint4 site_fixed(double x, double y, double z, double
 // (non-negative doubles order as their bits; the NaN centroids of a list never built count as huge):
 // while every bead stays within s_k + skin / 2 of c_k, every pair within rc is on the list (it lists
 // the pairs with |c_k - c_j| < rc + skin + s_k + s_j), however the ring polymers move inside their
-// spheres.  When gather is set, the
-// site written at its sorted index iperm[] (the same for every bead) as (x, y, z, q) in FP64 and as
-// wrapped fractional FP32 coordinates (inverse box side f).  Per site also its bead centroid c and
-// spread s = max_b |x_b - c|, cu = [c_x, c_y, c_z, s][na], for the list build.  A block holds 256 / G
-// sites x G bead lanes (thread: site t % (256 / G), beads t / (256 / G) + G i); the centroid sums of
-// the lanes are added in fixed order.
+// spheres.  When gather is set, the site written at its sorted index iperm[] (the same for every
+// bead) as (x, y, z, q) in FP64 and as wrapped fractional FP32 coordinates (inverse box side f).
+// Thread per (site, bead), a wave over 64 consecutive sites of one bead (its loads and stores of
+// positions, sites and phases contiguous), a block over four beads; the centroids and spreads that a
+// build needs are formed by k_vl_cu, only when the build runs.
 __global__ __launch_bounds__(256) void k_sites_vl(const double* __restrict__ pos, double* __restrict__ site, int na,
                                                   int nb, double alpha, double s0, double s1, double s2, double l0,
                                                   double l1, double l2, double2* __restrict__ ph,
                                                   const double* __restrict__ q, const double* __restrict__ cb,
                                                   const int* __restrict__ iperm, int gather, double f0, double f1,
                                                   double f2, double4* __restrict__ srt, uint4* __restrict__ srf,
-                                                  double* __restrict__ cu, unsigned long long* __restrict__ dmax,
-                                                  int G) {
-    // G bead lanes per site (8, 16 or 32), 256 / G sites per block; lane g takes beads g, g + G, ...
-    const int TA = 256 / G;
-    const int il = threadIdx.x % TA, g = threadIdx.x / TA;
-    const int a = blockIdx.x * TA + il;
-    const bool av = a < na;
-    __shared__ double part[256][3];
-    __shared__ double cen[32][3];
-    __shared__ double red[256];
-    double m = 0.0, c[3] = {0.0, 0.0, 0.0};
-    if (av) {
+                                                  unsigned long long* __restrict__ dmax) {
+    const int a = blockIdx.x * 64 + (threadIdx.x & 63), b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    __shared__ double red[4];
+    double m = 0.0;
+    if (a < na && b < nb) {
         const int m0 = a - a % 3;
         const double alpha2 = 0.5 * (1.0 - alpha);
         const double sc[3] = {s0, s1, s2}, lat[3] = {l0, l1, l2};
-        const double qq = q[a];
-        const int ka = gather ? iperm[a] : 0;
         const double rr[3] = {cb[a], cb[(size_t)na + a], cb[(size_t)2 * na + a]}, sr = cb[(size_t)3 * na + a];
-        for (int b = g; b < nb; b += G) {
-            // the molecule's three atoms in every dimension, loaded before any is used
-            double pm[3][3];
+        // the molecule's three atoms in every dimension, loaded before any is used
+        double pm[3][3];
 #pragma unroll
-            for (int d = 0; d < 3; d++)
+        for (int d = 0; d < 3; d++)
 #pragma unroll
-                for (int u = 0; u < 3; u++) pm[d][u] = pos[((size_t)b * 3 + d) * na + m0 + u];
-            double x[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const size_t o = ((size_t)b * 3 + d) * na + a;
-                if (a == m0) {
-                    const double v = alpha * pm[d][0] + alpha2 * (pm[d][1] + pm[d][2]);
-                    x[d] = sc[d] * v;
-                } else {
-                    x[d] = sc[d] * (a == m0 + 1 ? pm[d][1] : pm[d][2]);
-                }
-                site[o] = x[d];
-                if (ph) {
-                    double sn, cs;
-                    sincos(lat[d] * x[d], &sn, &cs);
-                    ph[o] = make_double2(cs, sn);
-                }
-                c[d] += x[d];
-            }
-            const double dx = x[0] - rr[0], dy = x[1] - rr[1], dz = x[2] - rr[2];
-            const double ex = fmax(sqrt(dx * dx + dy * dy + dz * dz) - sr, 0.0);
-            double dd = ex * ex;
-            if (!(dd <= 1e300)) dd = 1e300;
-            m = fmax(m, dd);
-            if (gather) {
-                const size_t k = (size_t)b * (na + 1) + ka;
-                srt[k] = make_double4(x[0], x[1], x[2], qq);
-                srf[k] = site_fixed(x[0], x[1], x[2], qq, f0, f1, f2);
-            }
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) part[threadIdx.x][d] = c[d];
-    __syncthreads();
-    if (g == 0) {
-        // bead centroid: the lanes' sums in lane order
+            for (int u = 0; u < 3; u++) pm[d][u] = pos[((size_t)b * 3 + d) * na + m0 + u];
+        double x[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double v = part[il][d];
-            for (int h = 1; h < G; h++) v += part[h * TA + il][d];
-            cen[il][d] = v / nb;
-        }
-    }
-    __syncthreads();
-    double s2m = 0.0;
-    if (av)
-        for (int b = g; b < nb; b += G) {
-            double r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double e = site[((size_t)b * 3 + d) * na + a] - cen[il][d];
-                r2 += e * e;
+            const size_t o = ((size_t)b * 3 + d) * na + a;
+            if (a == m0) {
+                const double v = alpha * pm[d][0] + alpha2 * (pm[d][1] + pm[d][2]);
+                x[d] = sc[d] * v;
+            } else {
+                x[d] = sc[d] * (a == m0 + 1 ? pm[d][1] : pm[d][2]);
             }
-            s2m = fmax(s2m, r2);
+            site[o] = x[d];
+            if (ph) {
+                double sn, cs;
+                sincos(lat[d] * x[d], &sn, &cs);
+                ph[o] = make_double2(cs, sn);
+            }
         }
-    red[threadIdx.x] = s2m;
-    __syncthreads();
-    if (g == 0 && av) {
-        double v = red[il];
-        for (int h = 1; h < G; h++) v = fmax(v, red[h * TA + il]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) cu[(size_t)d * na + a] = cen[il][d];
-        cu[(size_t)3 * na + a] = sqrt(v);
+        const double dx = x[0] - rr[0], dy = x[1] - rr[1], dz = x[2] - rr[2];
+        const double ex = fmax(sqrt(dx * dx + dy * dy + dz * dz) - sr, 0.0);
+        double dd = ex * ex;
+        if (!(dd <= 1e300)) dd = 1e300;
+        m = dd;
+        if (gather) {
+            const size_t k = (size_t)b * (na + 1) + iperm[a];
+            const double qq = q[a];
+            srt[k] = make_double4(x[0], x[1], x[2], qq);
+            srf[k] = site_fixed(x[0], x[1], x[2], qq, f0, f1, f2);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
         ctrl_max(dmax, (unsigned long long)__double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
 }
 
-// One block, when the gate is on: a counting sort of the site centroids (cu, from k_sites_vl) into
+// When the list build runs (gate): per site its bead centroid c and spread s = max_b |x_b - c| from
+// the sites of every bead, cu = [c_x, c_y, c_z, s][na].  Block: 64 sites x 4 bead lanes, lane g
+// summing beads g, g + 4, ...; the lanes' sums are added in lane order.
+__global__ __launch_bounds__(256) void k_vl_cu(const double* __restrict__ site, int na, int nb,
+                                               double* __restrict__ cu, VGate gate) {
+    if (!gate.on()) return;
+    const int il = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int a = blockIdx.x * 64 + il;
+    const bool av = a < na;
+    __shared__ double part[4][64][3];
+    __shared__ double red[4][64];
+    double c[3] = {0.0, 0.0, 0.0};
+    if (av)
+        for (int b = g; b < nb; b += 4)
+#pragma unroll
+            for (int d = 0; d < 3; d++) c[d] += site[((size_t)b * 3 + d) * na + a];
+#pragma unroll
+    for (int d = 0; d < 3; d++) part[g][il][d] = c[d];
+    __syncthreads();
+    double cen[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) cen[d] = (((part[0][il][d] + part[1][il][d]) + part[2][il][d]) + part[3][il][d]) / nb;
+    double s2m = 0.0;
+    if (av)
+        for (int b = g; b < nb; b += 4) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double e = site[((size_t)b * 3 + d) * na + a] - cen[d];
+                r2 += e * e;
+            }
+            s2m = fmax(s2m, r2);
+        }
+    red[g][il] = s2m;
+    __syncthreads();
+    if (g == 0 && av) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) cu[(size_t)d * na + a] = cen[d];
+        cu[(size_t)3 * na + a] = sqrt(fmax(fmax(red[0][il], red[1][il]), fmax(red[2][il], red[3][il])));
+    }
+}
+
+// One block, when the gate is on: a counting sort of the site centroids (cu, from k_vl_cu) into
 // Morton cells, each cell then ordered by site index (perm / iperm, shared by all beads); the sorted
 // centroids as wrapped fractional coordinates with the spreads (cs); the bounding box of each run of
 // CLU sorted centroids (a cluster) with its largest spread, bb[c*8] = {centre, half-width, spread};
@@ -1077,12 +1073,9 @@ void launch_sites_vl(const double* pos, double* site, int na, int nb, double alp
                      const double lat[3], double* ph, const double* q, const VList& v, int gather, Box sb,
                      hipStream_t s) {
     const double s0 = unit ? box.inv[0] : 1.0, s1 = unit ? box.inv[1] : 1.0, s2 = unit ? box.inv[2] : 1.0;
-    // bead lanes per site: enough sites per block that the grid fills the CUs, at most 32 per site
-    const int G = nb <= 8 ? 8 : (nb <= 16 ? 16 : 32), TA = 256 / G;
-    k_sites_vl<<<(unsigned)((na + TA - 1) / TA), 256, 0, s>>>(pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1],
-                                                              lat[2], (double2*)ph, q, v.cb, v.iperm, gather,
-                                                              sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf, v.cu,
-                                                              v.dmax, G);
+    k_sites_vl<<<dim3((unsigned)((na + 63) / 64), (unsigned)((nb + 3) / 4)), 256, 0, s>>>(
+        pos, site, na, nb, alpha, s0, s1, s2, lat[0], lat[1], lat[2], (double2*)ph, q, v.cb, v.iperm, gather,
+        sb.inv[0], sb.inv[1], sb.inv[2], v.srt, v.srf, v.dmax);
 }
 
 // The phase factors alone (the same values as k_sites_vl writes), thread per (site, bead): the k-space
@@ -1136,6 +1129,7 @@ void launch_vlist(const VList& v, const double* site, const double* q, int na, i
     const int nbins = p * p * p;
     const int nclu = ewald_nclu(na);
     const double rl = rc + skin;
+    k_vl_cu<<<(unsigned)((na + 63) / 64), 256, 0, s>>>(site, na, nb, v.cu, gate);
     k_vl_sort<<<1, 1024, 0, s>>>(na, g, nbins, sb.inv[0], sb.inv[1], sb.inv[2], sb.L[0], sb.L[1], sb.L[2], rl, nclu,
                                  cell_of, count, start, v.perm, v.iperm, v.cu, v.cs, v.bb, v.ccnt, v.clist, gate,
                                  v.nreb);
